@@ -1,0 +1,163 @@
+"""Force-evaluation throughput on one MI355X.
+
+Workload: the 24,000-atom periodic system (62 A box, 16 LJ types, 3,000-atom
+chain + 7,000 rigid triplets; r_c 9, r_sw 7.5, list 10.5 A).  One step = one
+force evaluation (nonbonded tiles + bonds/angles/dihedrals/scaled 1-4 +
+fixed-point reduction + f64 output) at positions already resident in HBM.
+The Verlet list is built once before timing (its device time is reported).
+
+Metric: pair interactions / s, where a pair interaction is an unordered,
+non-excluded pair with r <= r_cutoff, counted once.
+
+Prints one JSON line.  cpu_baseline: the C restatement of the reference's
+CpuBackend::compute_forces (Rayon static strips, f64) on the host cores.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "geodesic-m_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+FLOP_PER_PAIR = 40          # canonical count per pair interaction (FMA = 2)
+PEAK_FP32_TFLOPS = 157.3    # MI355X published FP32 peak
+PEAK_HBM_GBS = 8000.0
+
+
+def in_cutoff_pairs(sysobj, g):
+    """Pairs of the exported Verlet list with min-image r2 <= r_c^2 (f64)."""
+    pi, pj = g.pair_list()
+    x, y, z = sysobj
+    L = np.asarray(g.box)
+    d = np.stack([x[pj] - x[pi], y[pj] - y[pi], z[pj] - z[pi]])
+    d -= L[:, None] * np.round(d / L[:, None])
+    r2 = np.einsum("ij,ij->j", d, d)
+    return int(np.count_nonzero(r2 <= g.params.r_cutoff ** 2)), len(pi)
+
+
+def traffic_from_profiles(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(s, n_pairs_cut, evals=20, warm=3):
+    from oracle import oracle as orc
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    x, y, z = s.x.copy(), s.y.copy(), s.z.copy()
+    orc.wrap_positions(x, y, z, s.params.box)
+    t0 = time.perf_counter()
+    pi, pj = orc.neighbor_build(x, y, z, s.params.box, s.params.r_skin, s.topo.excl_i, s.topo.excl_j,
+                                method="brute")
+    list_s = time.perf_counter() - t0
+    times = []
+    for k in range(warm + evals):
+        t0 = time.perf_counter()
+        orc.cpu_backend_compute(x, y, z, s.sigma, s.epsilon, s.topo, pi, pj, s.params.box,
+                                s.params.r_cutoff, s.params.r_switch, n_threads=cores,
+                                scale14=s.params.scale_14, bonded_min_image=s.params.bonded_min_image)
+        if k >= warm:
+            times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    return {"value": n_pairs_cut / med, "unit": "pair-interactions/s", "cores": cores, "kind": "port",
+            "sample": f"{evals} force evaluations (median) of the same 24k system after {warm} warm-ups, "
+                      f"T={cores} static strips; serial O(N^2) list build timed separately",
+            "ms_per_eval": med * 1e3, "list_build_s": list_s}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--chunk", type=int, default=-1, help="entries per work item (tuning)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the force path shards by replica or by spatial domain, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import geodesic_hip as gh
+    from geodesic_hip import systems
+
+    s = systems.system_24k()
+    g = gh.System(s.sigma, s.epsilon, s.topo, s.params)
+    x, y, z = s.x.copy(), s.y.copy(), s.z.copy()
+    g.build_neighbor_list(x, y, z)
+    if args.chunk >= 0:
+        g.set_work_chunk(args.chunk)
+    st = g.stats()
+    n_cut, n_list = in_cutoff_pairs((x, y, z), g)
+
+    for _ in range(args.warmup):
+        g.compute_forces_resident()
+    g.profile_begin()  # create the event pool outside the timed region
+    for _ in range(min(args.steps, 8192)):
+        g.compute_forces_resident()
+    g.profile_end()
+    g.sync()
+
+    g.profile_begin()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.compute_forces_resident()
+    e = g.sync()
+    t1 = time.perf_counter()
+    prof = g.profile_end()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    nb_ms = prof["ms_nonbonded_total"] / max(prof["n_evals"], 1)
+    ev_ms = prof["ms_eval_total"] / max(prof["n_evals"], 1)
+    achieved = n_cut * FLOP_PER_PAIR / (nb_ms * 1e-3) / 1e12
+
+    out = {
+        "metric": "pair-interactions/s (24k-atom force evaluation)",
+        "value": n_cut / (ms_step * 1e-3),
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 pair math, int64 fixed-point accumulation, f64 bonded",
+        "data": "synthetic (seeded jittered lattice, 3000-atom chain + 7000 triplets)",
+        "config": {"workload": "force evaluation, 24,000 atoms, 62 A periodic box, r_c 9 / r_sw 7.5 / list 10.5 A",
+                   "n_atoms": s.n, "n_list_pairs": n_list, "n_pairs_cutoff": n_cut,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix work in this kernel)",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": traffic_from_profiles("nonbonded_24k_traffic.json"),
+                     "kernel": "k_nonbonded", "kernel_ms_avg": nb_ms,
+                     "kernel_ms_min": prof["ms_nonbonded_min"], "flop_per_pair": FLOP_PER_PAIR},
+        "device_ms_per_eval": ev_ms,
+        "list_build_ms": st["ms_build"],
+        "list": {k: st[k] for k in ("n_superclusters", "n_entries", "n_tiles", "n_work_items")},
+        "energy_total": e.total,
+    }
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(s, n_cut)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = out["value"] / cb["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

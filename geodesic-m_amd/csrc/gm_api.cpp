@@ -1,0 +1,676 @@
+// C-ABI of the gfx950 force engine (include/geodesic_hip.h).
+//
+// Host orchestration of the device path that replaces the reference's
+// GpuBackend (geodesic-gpu/src/gpu_backend.rs:11-100; paths relative to the
+// reference repository root).  Everything numeric runs on the GPU; the host
+// only validates descriptors, sizes buffers, scans small per-supercluster
+// counts between the list-build kernels and decodes the list for export.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "gm_internal.hpp"
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+namespace gm {
+
+gm_status set_error(gm_status s, const std::string &msg) {
+    g_last_error = msg;
+    return s;
+}
+
+gm_status from_hip(hipError_t e, const char *what) {
+    gm_status s = GM_ERR_KERNEL;
+    switch (e) {
+        case hipErrorOutOfMemory: s = GM_ERR_OOM; break;
+        case hipErrorNoDevice:
+        case hipErrorInvalidDevice: s = GM_ERR_NO_DEVICE; break;
+        case hipErrorNoBinaryForGpu:
+        case hipErrorInvalidImage:
+        case hipErrorInvalidKernelFile:
+        case hipErrorSharedObjectInitFailed: s = GM_ERR_KERNEL_LOAD; break;
+        case hipErrorLaunchFailure:
+        case hipErrorIllegalAddress:
+        case hipErrorECCNotCorrectable:
+        case hipErrorAssert: s = GM_ERR_DEVICE_LOST; break;
+        default: break;
+    }
+    return set_error(s, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace gm
+
+using namespace gm;
+
+#define GM_REQUIRE(cond, msg)                                   \
+    do {                                                        \
+        if (!(cond)) return set_error(GM_ERR_INVALID_ARG, msg); \
+    } while (0)
+
+namespace {
+
+template <typename T>
+hipError_t upload(DevBuf<T> &b, const T *src, size_t n, hipStream_t st) {
+    hipError_t e = b.ensure(n);
+    if (e != hipSuccess || n == 0) return e;
+    return hipMemcpyAsync(b.p, src, sizeof(T) * n, hipMemcpyHostToDevice, st);
+}
+
+bool indices_ok(const uint32_t *a, int64_t cnt, int64_t n) {
+    for (int64_t k = 0; k < cnt; k++)
+        if ((int64_t)a[k] >= n) return false;
+    return true;
+}
+
+// Work items: each supercluster's entries cut into runs of `chunk` entries;
+// one wavefront per run.
+gm_status make_work_items(gm_sys *s) {
+    const std::vector<int> &start = s->sc_start_host;
+    const int64_t total = s->nent;
+    int64_t chunk = s->chunk;
+    if (chunk <= 0) {
+        const int64_t target = 8LL * (s->ctx->n_cu > 0 ? s->ctx->n_cu : 256);
+        chunk = std::max<int64_t>(4, (total + target - 1) / std::max<int64_t>(target, 1));
+    }
+    std::vector<int> wsc, wst, wen;
+    for (int64_t k = 0; k < s->nsc; k++)
+        for (int64_t e0 = start[(size_t)k]; e0 < start[(size_t)k + 1]; e0 += chunk) {
+            wsc.push_back((int)k);
+            wst.push_back((int)e0);
+            wen.push_back((int)std::min<int64_t>(e0 + chunk, start[(size_t)k + 1]));
+        }
+    s->n_wi = (int64_t)wsc.size();
+    hipStream_t st = s->stream;
+    GM_HIP(upload(s->wi_sc, wsc.data(), wsc.size(), st));
+    GM_HIP(upload(s->wi_start, wst.data(), wst.size(), st));
+    GM_HIP(upload(s->wi_end, wen.data(), wen.size(), st));
+    GM_HIP(s->e_wi.ensure((size_t)std::max<int64_t>(s->n_wi, 1)));
+    GM_HIP(hipStreamSynchronize(st));
+    s->stats.n_work_items = s->n_wi;
+    return GM_OK;
+}
+
+constexpr int64_t kProfMax = 8192;
+
+gm_status build_on_device(gm_sys *s, bool wrap) {
+    hipStream_t st = s->stream;
+    s->built = false;
+    GM_HIP(hipEventRecord(s->ev_b0, st));
+    GM_HIP(launch_build(s, wrap));
+    int meta[4] = {0, 0, 0, 0};
+    GM_HIP(hipMemcpyAsync(meta, s->meta.p, sizeof(meta), hipMemcpyDeviceToHost, st));
+    GM_HIP(hipStreamSynchronize(st));
+    if (meta[1])
+        return set_error(GM_ERR_CAPACITY, "list build: an xy column holds " + std::to_string(meta[2]) +
+                                              " atoms, more than the LDS sort capacity " +
+                                              std::to_string(kColumnCap));
+    s->nsc = meta[0];
+    s->ncl = s->nsc * kClustersPerSc;
+    if (s->nsc > s->max_sc) return set_error(GM_ERR_KERNEL, "list build: supercluster count overflow");
+
+    std::vector<int> cnt((size_t)s->nsc), start((size_t)s->nsc + 1, 0);
+    if (s->nsc > 0) {
+        GM_HIP(launch_pair_count(s));
+        GM_HIP(hipMemcpyAsync(cnt.data(), s->sc_count.p, sizeof(int) * cnt.size(),
+                              hipMemcpyDeviceToHost, st));
+        GM_HIP(hipStreamSynchronize(st));
+    }
+    int64_t total = 0;
+    for (int64_t k = 0; k < s->nsc; k++) {
+        start[(size_t)k] = (int)total;
+        total += cnt[(size_t)k];
+    }
+    start[(size_t)s->nsc] = (int)total;
+    if (total > (int64_t)0x7fffffff) return set_error(GM_ERR_CAPACITY, "list build: too many entries");
+    s->nent = total;
+    GM_HIP(s->ent_sc.ensure((size_t)total));
+    GM_HIP(s->ent_cj.ensure((size_t)total));
+    GM_HIP(s->ent_imask.ensure((size_t)total));
+    GM_HIP(s->ent_mask.ensure((size_t)total * 8));
+    GM_HIP(upload(s->sc_start, start.data(), start.size(), st));
+
+    s->sc_start_host = start;
+    gm_status rc = make_work_items(s);
+    if (rc != GM_OK) return rc;
+
+    GM_HIP(launch_pair_fill_and_masks(s));
+    GM_HIP(hipEventRecord(s->ev_b1, st));
+    unsigned long long counters[2] = {0, 0};
+    GM_HIP(hipMemcpyAsync(counters, s->meta.p + 8, sizeof(counters), hipMemcpyDeviceToHost, st));
+    GM_HIP(hipStreamSynchronize(st));
+    float ms = 0.f;
+    GM_HIP(hipEventElapsedTime(&ms, s->ev_b0, s->ev_b1));
+    s->stats.n_atoms = s->n;
+    s->stats.n_superclusters = s->nsc;
+    s->stats.n_clusters = s->ncl;
+    s->stats.n_entries = s->nent;
+    s->stats.n_list_pairs = (int64_t)counters[0];
+    s->stats.n_tiles = (int64_t)counters[1];
+    s->stats.n_work_items = s->n_wi;
+    s->stats.ms_build = ms;
+    s->built = true;
+    return GM_OK;
+}
+
+gm_status compute_on_device(gm_sys *s, const double *x, const double *y, const double *z,
+                            double *fx, double *fy, double *fz) {
+    if (!s->built) return set_error(GM_ERR_NOT_BUILT, "compute_forces before build_neighbor_list");
+    hipStream_t st = s->stream;
+    hipEvent_t e0 = s->ev_eval0, e1 = s->ev_eval1;
+    s->cur_nb0 = s->ev_nb0;
+    s->cur_nb1 = s->ev_nb1;
+    if (s->profiling) {
+        if (s->prof_used < kProfMax) {
+            const size_t need = 4 * (size_t)(s->prof_used + 1);
+            while (s->prof_pool.size() < need) {
+                hipEvent_t ev;
+                GM_HIP(hipEventCreate(&ev));
+                s->prof_pool.push_back(ev);
+            }
+            const size_t b = 4 * (size_t)s->prof_used;
+            e0 = s->prof_pool[b];
+            s->cur_nb0 = s->prof_pool[b + 1];
+            s->cur_nb1 = s->prof_pool[b + 2];
+            e1 = s->prof_pool[b + 3];
+            s->prof_used++;
+        } else {
+            s->prof_untimed++;
+        }
+    }
+    GM_HIP(hipEventRecord(e0, st));
+    GM_HIP(launch_fill_u64(s->F.p, (size_t)s->nsc * 64 * 3, st));
+    GM_HIP(launch_nonbonded(s, x, y, z));
+    GM_HIP(launch_bonded(s, x, y, z));
+    GM_HIP(launch_finalize(s, fx, fy, fz));
+    GM_HIP(hipEventRecord(e1, st));
+    return GM_OK;
+}
+
+gm_status collect(gm_sys *s, gm_energy_terms *e) {
+    GM_HIP(hipStreamSynchronize(s->stream));
+    double t[5] = {0, 0, 0, 0, 0};
+    GM_HIP(hipMemcpy(t, s->e_out.p, sizeof(t), hipMemcpyDeviceToHost));
+    float a = 0.f, b = 0.f;
+    if (hipEventElapsedTime(&a, s->ev_nb0, s->ev_nb1) == hipSuccess) s->stats.ms_nonbonded = a;
+    if (hipEventElapsedTime(&b, s->ev_eval0, s->ev_eval1) == hipSuccess) s->stats.ms_eval = b;
+    if (e) {
+        e->bond = t[0];
+        e->angle = t[1];
+        e->dihedral = t[2];
+        e->lj14 = t[3];
+        e->lj = t[4];
+        // Reference order (cpu_backend.rs:93-166): bonded terms, then the LJ sum.
+        double tot = 0.0;
+        tot += t[0];
+        tot += t[1];
+        tot += t[2];
+        tot += t[3];
+        tot += t[4];
+        e->total = tot;
+    }
+    return GM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gm_last_error(void) { return g_last_error.c_str(); }
+
+int gm_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+gm_status gm_init(int device_ordinal, gm_ctx **out) {
+    GM_REQUIRE(out != nullptr, "gm_init: out is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return set_error(GM_ERR_NO_DEVICE, "gm_init: no HIP device visible");
+    if (device_ordinal < 0 || device_ordinal >= n)
+        return set_error(GM_ERR_NO_DEVICE, "gm_init: device ordinal out of range");
+    GM_HIP(hipSetDevice(device_ordinal));
+    hipDeviceProp_t prop;
+    GM_HIP(hipGetDeviceProperties(&prop, device_ordinal));
+    std::string arch(prop.gcnArchName);
+    if (arch.rfind("gfx950", 0) != 0)
+        return set_error(GM_ERR_KERNEL_LOAD, "gm_init: code objects are built for gfx950, device is " + arch);
+    gm_ctx *c = new gm_ctx();
+    c->device = device_ordinal;
+    c->n_cu = prop.multiProcessorCount;
+    c->arch = arch;
+    *out = c;
+    return GM_OK;
+}
+
+gm_status gm_ctx_destroy(gm_ctx *ctx) {
+    delete ctx;
+    return GM_OK;
+}
+
+gm_status gm_system_create(gm_ctx *ctx, const gm_system_desc *d, gm_sys **out) {
+    GM_REQUIRE(ctx && d && out, "gm_system_create: NULL argument");
+    *out = nullptr;
+    const int64_t n = d->n_atoms;
+    GM_REQUIRE(n >= 0 && n < (int64_t)1 << 31, "n_atoms out of range");
+    GM_REQUIRE(n == 0 || (d->sigma && d->epsilon), "sigma/epsilon missing");
+    for (int a = 0; a < 3; a++)
+        GM_REQUIRE(std::isfinite(d->box[a]) && d->box[a] > 0.0, "box lengths must be positive");
+    // config.rs:229-236: r_switch < r_cutoff < r_skin
+    GM_REQUIRE(d->r_switch < d->r_cutoff && d->r_cutoff < d->r_skin && d->r_switch >= 0.0,
+               "require 0 <= r_switch < r_cutoff < r_skin");
+    GM_REQUIRE(d->n_bonds >= 0 && d->n_angles >= 0 && d->n_dihedrals >= 0 && d->n_excl >= 0 &&
+                   d->n_pairs14 >= 0, "negative term count");
+    GM_REQUIRE(d->n_bonds == 0 || (d->bond_i && d->bond_j && d->bond_k && d->bond_r0), "bond arrays missing");
+    GM_REQUIRE(d->n_angles == 0 || (d->angle_i && d->angle_j && d->angle_k && d->angle_kth && d->angle_th0),
+               "angle arrays missing");
+    GM_REQUIRE(d->n_dihedrals == 0 || (d->dihed_i && d->dihed_j && d->dihed_k && d->dihed_l &&
+                                       d->dihed_kphi && d->dihed_n && d->dihed_delta),
+               "dihedral arrays missing");
+    GM_REQUIRE(d->n_excl == 0 || (d->excl_i && d->excl_j), "exclusion arrays missing");
+    GM_REQUIRE(d->n_pairs14 == 0 || (d->pair14_i && d->pair14_j), "1-4 arrays missing");
+    GM_REQUIRE(indices_ok(d->bond_i, d->n_bonds, n) && indices_ok(d->bond_j, d->n_bonds, n),
+               "bond index out of range");
+    GM_REQUIRE(indices_ok(d->angle_i, d->n_angles, n) && indices_ok(d->angle_j, d->n_angles, n) &&
+                   indices_ok(d->angle_k, d->n_angles, n), "angle index out of range");
+    GM_REQUIRE(indices_ok(d->dihed_i, d->n_dihedrals, n) && indices_ok(d->dihed_j, d->n_dihedrals, n) &&
+                   indices_ok(d->dihed_k, d->n_dihedrals, n) && indices_ok(d->dihed_l, d->n_dihedrals, n),
+               "dihedral index out of range");
+    GM_REQUIRE(indices_ok(d->excl_i, d->n_excl, n) && indices_ok(d->excl_j, d->n_excl, n),
+               "exclusion index out of range");
+    GM_REQUIRE(indices_ok(d->pair14_i, d->n_pairs14, n) && indices_ok(d->pair14_j, d->n_pairs14, n),
+               "1-4 index out of range");
+    for (int64_t k = 0; k < n; k++)
+        GM_REQUIRE(std::isfinite(d->sigma[k]) && std::isfinite(d->epsilon[k]) && d->epsilon[k] >= 0.0,
+                   "sigma/epsilon must be finite, epsilon >= 0");
+
+    GM_HIP(hipSetDevice(ctx->device));
+    gm_sys *s = new gm_sys();
+    s->ctx = ctx;
+    auto fail = [&](gm_status st) { gm_system_destroy(s); return st; };
+#define TRY(call)                                         \
+    do {                                                  \
+        hipError_t _e = (call);                           \
+        if (_e != hipSuccess) return fail(from_hip(_e, #call)); \
+    } while (0)
+    TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    for (hipEvent_t *ev : {&s->ev_eval0, &s->ev_nb0, &s->ev_nb1, &s->ev_eval1, &s->ev_b0, &s->ev_b1})
+        TRY(hipEventCreate(ev));
+    hipStream_t st = s->stream;
+    s->n = n;
+    for (int a = 0; a < 3; a++) s->box[a] = d->box[a];
+    s->periodic = d->periodic ? 1 : 0;
+    s->rc = d->r_cutoff;
+    s->rsw = d->r_switch;
+    s->rskin = d->r_skin;
+    s->scale14 = d->scale_14;
+    s->bonded_mi = d->bonded_min_image ? 1 : 0;
+    s->nb = d->n_bonds;
+    s->na = d->n_angles;
+    s->nd = d->n_dihedrals;
+    s->n14 = d->n_pairs14;
+    if (const char *ck = std::getenv("GM_NB_CHUNK")) s->chunk = std::atoi(ck);
+
+    const double two32 = 4294967296.0;
+    s->nbp.sx = (float)(d->box[0] / two32);
+    s->nbp.sy = (float)(d->box[1] / two32);
+    s->nbp.sz = (float)(d->box[2] / two32);
+    s->nbp.rc2 = (float)(d->r_cutoff * d->r_cutoff);
+    s->nbp.rsw2 = (float)(d->r_switch * d->r_switch);
+    s->nbp.rsw = (float)d->r_switch;
+    s->nbp.inv_w = (float)(1.0 / (d->r_cutoff - d->r_switch));
+    s->nbp.ds_coef = (float)(-30.0 / (d->r_cutoff - d->r_switch));
+
+    BuildParams &bp = s->bp;
+    for (int a = 0; a < 3; a++) {
+        bp.box[a] = d->box[a];
+        bp.inv_q[a] = two32 / d->box[a];
+        bp.sq[a] = (float)(d->box[a] / two32);
+    }
+    bp.r_skin2 = d->r_skin * d->r_skin;
+    bp.r_search = (float)(d->r_skin * (1.0 + 1e-5) + 1e-3);
+    // xy columns about the width of a cube holding 64 atoms at the mean density
+    const double vol = d->box[0] * d->box[1] * d->box[2];
+    const double rho = n > 0 ? (double)n / vol : 1.0;
+    const double side = std::cbrt(64.0 / rho);
+    bp.ncx = (int)std::max(1.0, std::min(512.0, std::floor(d->box[0] / side + 0.5)));
+    bp.ncy = (int)std::max(1.0, std::min(512.0, std::floor(d->box[1] / side + 0.5)));
+    bp.ncol = bp.ncx * bp.ncy;
+    bp.periodic = s->periodic;
+    bp.n_atoms = (int)n;
+    s->max_sc = (n + 63) / 64 + bp.ncol;
+    bp.max_sc = (int)s->max_sc;
+
+    const size_t N = (size_t)std::max<int64_t>(n, 1);
+    const size_t slots = (size_t)s->max_sc * 64;
+    TRY(upload(s->sigma, d->sigma, (size_t)n, st));
+    TRY(upload(s->eps, d->epsilon, (size_t)n, st));
+    std::vector<float2> par((size_t)n);
+    for (int64_t k = 0; k < n; k++)
+        par[(size_t)k] = make_float2((float)(0.5 * d->sigma[k]), (float)std::sqrt(d->epsilon[k]));
+    TRY(upload(s->atom_par, par.data(), par.size(), st));
+    TRY(upload(s->bi, d->bond_i, (size_t)s->nb, st));
+    TRY(upload(s->bj, d->bond_j, (size_t)s->nb, st));
+    TRY(upload(s->bk, d->bond_k, (size_t)s->nb, st));
+    TRY(upload(s->br0, d->bond_r0, (size_t)s->nb, st));
+    TRY(upload(s->ai, d->angle_i, (size_t)s->na, st));
+    TRY(upload(s->aj, d->angle_j, (size_t)s->na, st));
+    TRY(upload(s->ak, d->angle_k, (size_t)s->na, st));
+    TRY(upload(s->akth, d->angle_kth, (size_t)s->na, st));
+    TRY(upload(s->ath0, d->angle_th0, (size_t)s->na, st));
+    TRY(upload(s->di, d->dihed_i, (size_t)s->nd, st));
+    TRY(upload(s->dj, d->dihed_j, (size_t)s->nd, st));
+    TRY(upload(s->dk, d->dihed_k, (size_t)s->nd, st));
+    TRY(upload(s->dl, d->dihed_l, (size_t)s->nd, st));
+    TRY(upload(s->dkphi, d->dihed_kphi, (size_t)s->nd, st));
+    TRY(upload(s->dn, d->dihed_n, (size_t)s->nd, st));
+    TRY(upload(s->ddelta, d->dihed_delta, (size_t)s->nd, st));
+    TRY(upload(s->p14i, d->pair14_i, (size_t)s->n14, st));
+    TRY(upload(s->p14j, d->pair14_j, (size_t)s->n14, st));
+
+    // Exclusions as a canonical set (neighbor.rs:18-23): partners > i, sorted.
+    std::vector<uint64_t> keys;
+    keys.reserve((size_t)d->n_excl);
+    for (int64_t k = 0; k < d->n_excl; k++) {
+        uint32_t a = d->excl_i[k], b = d->excl_j[k];
+        if (a == b) continue;
+        uint32_t lo = std::min(a, b), hi = std::max(a, b);
+        keys.push_back(((uint64_t)lo << 32) | hi);
+    }
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    std::vector<int64_t> off((size_t)n + 1, 0);
+    std::vector<uint32_t> nbr(keys.size());
+    for (size_t k = 0; k < keys.size(); k++) {
+        off[(size_t)(keys[k] >> 32) + 1]++;
+        nbr[k] = (uint32_t)(keys[k] & 0xffffffffu);
+    }
+    for (int64_t i = 0; i < n; i++) off[(size_t)i + 1] += off[(size_t)i];
+    TRY(upload(s->excl_off, off.data(), off.size(), st));
+    TRY(upload(s->excl_nbr, nbr.data(), nbr.size(), st));
+
+    for (DevBuf<double> *b : {&s->x, &s->y, &s->z, &s->refx, &s->refy, &s->refz, &s->fx, &s->fy, &s->fz,
+                              &s->sx, &s->sy, &s->sz})
+        TRY(b->ensure(N));
+    TRY(s->qatom.ensure(3 * N));
+    for (DevBuf<int> *b : {&s->col_count, &s->col_start, &s->col_sc, &s->col_fill})
+        TRY(b->ensure((size_t)bp.ncol));
+    TRY(s->bucket.ensure(N));
+    TRY(s->sorted_atoms.ensure(N));
+    TRY(s->atom_slot.ensure(N));
+    for (DevBuf<int> *b : {&s->sc_base, &s->sc_m, &s->sc_count}) TRY(b->ensure((size_t)s->max_sc + 1));
+    TRY(s->meta.ensure(16));
+    TRY(s->slot_atom.ensure(slots));
+    TRY(s->sq.ensure(slots));
+    TRY(s->sp.ensure(slots));
+    TRY(s->cl_ref.ensure((size_t)s->max_sc * 8));
+    TRY(s->cl_half.ensure((size_t)s->max_sc * 8));
+    TRY(s->sc_ref.ensure((size_t)s->max_sc));
+    TRY(s->sc_half.ensure((size_t)s->max_sc));
+    TRY(s->F.ensure(3 * slots));
+    TRY(s->e_items.ensure((size_t)(s->nb + s->na + s->nd + s->n14)));
+    TRY(s->e_out.ensure(8));
+    TRY(s->e_wi.ensure(1));
+    TRY(hipStreamSynchronize(st));
+#undef TRY
+    *out = s;
+    return GM_OK;
+}
+
+gm_status gm_system_destroy(gm_sys *s) {
+    if (!s) return GM_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    DevBuf<double> *dbl[] = {&s->sigma, &s->eps, &s->bk, &s->br0, &s->akth, &s->ath0, &s->dkphi,
+                             &s->ddelta, &s->x, &s->y, &s->z, &s->refx, &s->refy, &s->refz,
+                             &s->fx, &s->fy, &s->fz, &s->sx, &s->sy, &s->sz, &s->e_items, &s->e_wi, &s->e_out};
+    for (auto *b : dbl) b->release();
+    DevBuf<uint32_t> *u32[] = {&s->bi, &s->bj, &s->ai, &s->aj, &s->ak, &s->di, &s->dj, &s->dk,
+                               &s->dl, &s->dn, &s->p14i, &s->p14j, &s->excl_nbr, &s->qatom,
+                               &s->ent_imask};
+    for (auto *b : u32) b->release();
+    DevBuf<int> *i32[] = {&s->col_count, &s->col_start, &s->col_sc, &s->col_fill, &s->bucket,
+                          &s->sorted_atoms, &s->sc_base, &s->sc_m, &s->meta, &s->slot_atom,
+                          &s->atom_slot, &s->sc_count, &s->sc_start, &s->ent_sc, &s->ent_cj,
+                          &s->wi_sc, &s->wi_start, &s->wi_end};
+    for (auto *b : i32) b->release();
+    s->excl_off.release();
+    s->atom_par.release();
+    s->sp.release();
+    s->sq.release();
+    s->cl_ref.release();
+    s->sc_ref.release();
+    s->cl_half.release();
+    s->sc_half.release();
+    s->ent_mask.release();
+    s->F.release();
+    for (hipEvent_t ev : {s->ev_eval0, s->ev_nb0, s->ev_nb1, s->ev_eval1, s->ev_b0, s->ev_b1})
+        if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : s->prof_pool) (void)hipEventDestroy(ev);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return GM_OK;
+}
+
+gm_status gm_build_neighbor_list(gm_sys *s, double *x, double *y, double *z) {
+    GM_REQUIRE(s, "NULL system");
+    GM_REQUIRE(s->n == 0 || (x && y && z), "NULL positions");
+    const size_t bytes = sizeof(double) * (size_t)s->n;
+    hipStream_t st = s->stream;
+    if (s->n > 0) {
+        GM_HIP(hipMemcpyAsync(s->x.p, x, bytes, hipMemcpyHostToDevice, st));
+        GM_HIP(hipMemcpyAsync(s->y.p, y, bytes, hipMemcpyHostToDevice, st));
+        GM_HIP(hipMemcpyAsync(s->z.p, z, bytes, hipMemcpyHostToDevice, st));
+    }
+    gm_status rc = build_on_device(s, s->periodic != 0);
+    if (rc != GM_OK) return rc;
+    if (s->periodic && s->n > 0) {
+        GM_HIP(hipMemcpyAsync(x, s->x.p, bytes, hipMemcpyDeviceToHost, st));
+        GM_HIP(hipMemcpyAsync(y, s->y.p, bytes, hipMemcpyDeviceToHost, st));
+        GM_HIP(hipMemcpyAsync(z, s->z.p, bytes, hipMemcpyDeviceToHost, st));
+    }
+    GM_HIP(hipStreamSynchronize(st));
+    return GM_OK;
+}
+
+gm_status gm_build_neighbor_list_device(gm_sys *s, double *dx, double *dy, double *dz) {
+    GM_REQUIRE(s, "NULL system");
+    GM_REQUIRE(s->n == 0 || (dx && dy && dz), "NULL positions");
+    const size_t bytes = sizeof(double) * (size_t)s->n;
+    hipStream_t st = s->stream;
+    if (s->n > 0) {
+        GM_HIP(hipMemcpyAsync(s->x.p, dx, bytes, hipMemcpyDeviceToDevice, st));
+        GM_HIP(hipMemcpyAsync(s->y.p, dy, bytes, hipMemcpyDeviceToDevice, st));
+        GM_HIP(hipMemcpyAsync(s->z.p, dz, bytes, hipMemcpyDeviceToDevice, st));
+    }
+    gm_status rc = build_on_device(s, s->periodic != 0);
+    if (rc != GM_OK) return rc;
+    if (s->periodic && s->n > 0) {
+        GM_HIP(hipMemcpyAsync(dx, s->x.p, bytes, hipMemcpyDeviceToDevice, st));
+        GM_HIP(hipMemcpyAsync(dy, s->y.p, bytes, hipMemcpyDeviceToDevice, st));
+        GM_HIP(hipMemcpyAsync(dz, s->z.p, bytes, hipMemcpyDeviceToDevice, st));
+    }
+    GM_HIP(hipStreamSynchronize(st));
+    return GM_OK;
+}
+
+gm_status gm_needs_rebuild(gm_sys *s, const double *x, const double *y, const double *z, int32_t *out) {
+    GM_REQUIRE(s && out, "NULL argument");
+    GM_REQUIRE(s->n == 0 || (x && y && z), "NULL positions");
+    if (!s->built) return set_error(GM_ERR_NOT_BUILT, "needs_rebuild before build_neighbor_list");
+    const size_t bytes = sizeof(double) * (size_t)s->n;
+    hipStream_t st = s->stream;
+    if (s->n > 0) {
+        GM_HIP(hipMemcpyAsync(s->sx.p, x, bytes, hipMemcpyHostToDevice, st));
+        GM_HIP(hipMemcpyAsync(s->sy.p, y, bytes, hipMemcpyHostToDevice, st));
+        GM_HIP(hipMemcpyAsync(s->sz.p, z, bytes, hipMemcpyHostToDevice, st));
+    }
+    GM_HIP(launch_needs_rebuild(s, s->sx.p, s->sy.p, s->sz.p));
+    int flag = 0;
+    GM_HIP(hipMemcpyAsync(&flag, s->meta.p + 5, sizeof(int), hipMemcpyDeviceToHost, st));
+    GM_HIP(hipStreamSynchronize(st));
+    *out = flag ? 1 : 0;
+    return GM_OK;
+}
+
+gm_status gm_compute_forces(gm_sys *s, const double *x, const double *y, const double *z,
+                            double *fx, double *fy, double *fz, gm_energy_terms *e) {
+    GM_REQUIRE(s, "NULL system");
+    GM_REQUIRE(s->n == 0 || (x && y && z && fx && fy && fz), "NULL array");
+    const size_t bytes = sizeof(double) * (size_t)s->n;
+    hipStream_t st = s->stream;
+    if (s->n > 0) {
+        GM_HIP(hipMemcpyAsync(s->x.p, x, bytes, hipMemcpyHostToDevice, st));
+        GM_HIP(hipMemcpyAsync(s->y.p, y, bytes, hipMemcpyHostToDevice, st));
+        GM_HIP(hipMemcpyAsync(s->z.p, z, bytes, hipMemcpyHostToDevice, st));
+    }
+    gm_status rc = compute_on_device(s, s->x.p, s->y.p, s->z.p, s->fx.p, s->fy.p, s->fz.p);
+    if (rc != GM_OK) return rc;
+    if (s->n > 0) {
+        GM_HIP(hipMemcpyAsync(fx, s->fx.p, bytes, hipMemcpyDeviceToHost, st));
+        GM_HIP(hipMemcpyAsync(fy, s->fy.p, bytes, hipMemcpyDeviceToHost, st));
+        GM_HIP(hipMemcpyAsync(fz, s->fz.p, bytes, hipMemcpyDeviceToHost, st));
+    }
+    return collect(s, e);
+}
+
+gm_status gm_compute_forces_device(gm_sys *s, const double *dx, const double *dy, const double *dz,
+                                   double *dfx, double *dfy, double *dfz) {
+    GM_REQUIRE(s, "NULL system");
+    GM_REQUIRE(s->n == 0 || (dx && dy && dz && dfx && dfy && dfz), "NULL array");
+    return compute_on_device(s, dx, dy, dz, dfx, dfy, dfz);
+}
+
+gm_status gm_set_positions(gm_sys *s, const double *x, const double *y, const double *z) {
+    GM_REQUIRE(s, "NULL system");
+    GM_REQUIRE(s->n == 0 || (x && y && z), "NULL positions");
+    const size_t bytes = sizeof(double) * (size_t)s->n;
+    hipStream_t st = s->stream;
+    if (s->n > 0) {
+        GM_HIP(hipMemcpyAsync(s->x.p, x, bytes, hipMemcpyHostToDevice, st));
+        GM_HIP(hipMemcpyAsync(s->y.p, y, bytes, hipMemcpyHostToDevice, st));
+        GM_HIP(hipMemcpyAsync(s->z.p, z, bytes, hipMemcpyHostToDevice, st));
+    }
+    GM_HIP(hipStreamSynchronize(st));
+    return GM_OK;
+}
+
+gm_status gm_compute_forces_resident(gm_sys *s) {
+    GM_REQUIRE(s, "NULL system");
+    return compute_on_device(s, s->x.p, s->y.p, s->z.p, s->fx.p, s->fy.p, s->fz.p);
+}
+
+gm_status gm_get_forces(gm_sys *s, double *fx, double *fy, double *fz) {
+    GM_REQUIRE(s, "NULL system");
+    GM_REQUIRE(s->n == 0 || (fx && fy && fz), "NULL array");
+    const size_t bytes = sizeof(double) * (size_t)s->n;
+    hipStream_t st = s->stream;
+    if (s->n > 0) {
+        GM_HIP(hipMemcpyAsync(fx, s->fx.p, bytes, hipMemcpyDeviceToHost, st));
+        GM_HIP(hipMemcpyAsync(fy, s->fy.p, bytes, hipMemcpyDeviceToHost, st));
+        GM_HIP(hipMemcpyAsync(fz, s->fz.p, bytes, hipMemcpyDeviceToHost, st));
+    }
+    GM_HIP(hipStreamSynchronize(st));
+    return GM_OK;
+}
+
+gm_status gm_sync(gm_sys *s, gm_energy_terms *e) {
+    GM_REQUIRE(s, "NULL system");
+    return collect(s, e);
+}
+
+gm_status gm_get_pair_list(gm_sys *s, uint32_t *pi, uint32_t *pj, int64_t cap, int64_t *nout) {
+    GM_REQUIRE(s && nout, "NULL argument");
+    GM_REQUIRE(cap == 0 || (pi && pj), "NULL output with nonzero capacity");
+    if (!s->built) return set_error(GM_ERR_NOT_BUILT, "get_pair_list before build_neighbor_list");
+    hipStream_t st = s->stream;
+    const size_t nslots = (size_t)s->nsc * 64, ne = (size_t)s->nent;
+    std::vector<int> slot(nslots), esc(ne), ecj(ne);
+    std::vector<uint32_t> eim(ne);
+    std::vector<unsigned long long> em(ne * 8);
+    if (nslots) GM_HIP(hipMemcpyAsync(slot.data(), s->slot_atom.p, nslots * sizeof(int), hipMemcpyDeviceToHost, st));
+    if (ne) {
+        GM_HIP(hipMemcpyAsync(esc.data(), s->ent_sc.p, ne * sizeof(int), hipMemcpyDeviceToHost, st));
+        GM_HIP(hipMemcpyAsync(ecj.data(), s->ent_cj.p, ne * sizeof(int), hipMemcpyDeviceToHost, st));
+        GM_HIP(hipMemcpyAsync(eim.data(), s->ent_imask.p, ne * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        GM_HIP(hipMemcpyAsync(em.data(), s->ent_mask.p, ne * 8 * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, st));
+    }
+    GM_HIP(hipStreamSynchronize(st));
+    std::vector<uint64_t> pairs;
+    pairs.reserve((size_t)s->stats.n_list_pairs);
+    for (size_t e = 0; e < ne; e++) {
+        for (int ci = 0; ci < 8; ci++) {
+            if (!(eim[e] & (1u << ci))) continue;
+            unsigned long long m = em[e * 8 + ci];
+            while (m) {
+                const int bit = __builtin_ctzll(m);
+                m &= m - 1;
+                const int il = bit & 7, jl = bit >> 3;
+                const int a = slot[(size_t)esc[e] * 64 + ci * 8 + il];
+                const int b = slot[(size_t)ecj[e] * 8 + jl];
+                const uint32_t lo = (uint32_t)std::min(a, b), hi = (uint32_t)std::max(a, b);
+                pairs.push_back(((uint64_t)lo << 32) | hi);
+            }
+        }
+    }
+    std::sort(pairs.begin(), pairs.end());
+    *nout = (int64_t)pairs.size();
+    const int64_t w = std::min<int64_t>(cap, (int64_t)pairs.size());
+    for (int64_t k = 0; k < w; k++) {
+        pi[k] = (uint32_t)(pairs[(size_t)k] >> 32);
+        pj[k] = (uint32_t)(pairs[(size_t)k] & 0xffffffffu);
+    }
+    return GM_OK;
+}
+
+gm_status gm_profile_begin(gm_sys *s) {
+    GM_REQUIRE(s, "NULL system");
+    s->profiling = true;
+    s->prof_used = 0;
+    s->prof_untimed = 0;
+    return GM_OK;
+}
+
+gm_status gm_profile_end(gm_sys *s, gm_profile *out) {
+    GM_REQUIRE(s && out, "NULL argument");
+    GM_HIP(hipStreamSynchronize(s->stream));
+    gm_profile p{};
+    p.n_evals = s->prof_used;
+    p.n_untimed = s->prof_untimed;
+    p.ms_nonbonded_min = 0.0;
+    p.ms_nonbonded_max = 0.0;
+    for (int64_t k = 0; k < s->prof_used; k++) {
+        float nb = 0.f, ev = 0.f;
+        GM_HIP(hipEventElapsedTime(&nb, s->prof_pool[4 * (size_t)k + 1], s->prof_pool[4 * (size_t)k + 2]));
+        GM_HIP(hipEventElapsedTime(&ev, s->prof_pool[4 * (size_t)k], s->prof_pool[4 * (size_t)k + 3]));
+        p.ms_nonbonded_total += nb;
+        p.ms_eval_total += ev;
+        if (k == 0 || nb < p.ms_nonbonded_min) p.ms_nonbonded_min = nb;
+        if (k == 0 || nb > p.ms_nonbonded_max) p.ms_nonbonded_max = nb;
+    }
+    s->profiling = false;
+    *out = p;
+    return GM_OK;
+}
+
+gm_status gm_set_work_chunk(gm_sys *s, int32_t entries_per_item) {
+    GM_REQUIRE(s && entries_per_item >= 0, "bad argument");
+    s->chunk = entries_per_item;
+    if (!s->built) return GM_OK;
+    return make_work_items(s);
+}
+
+gm_status gm_get_stats(gm_sys *s, gm_stats *out) {
+    GM_REQUIRE(s && out, "NULL argument");
+    *out = s->stats;
+    return GM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,171 @@
+// Internal definitions shared by the C-ABI implementation and the gfx950
+// kernels.  Not part of the public interface (include/geodesic_hip.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "geodesic_hip.h"
+
+namespace gm {
+
+// Slot layout of the pair-tile list.  Atoms are reordered spatially into
+// "slots": a supercluster is 64 consecutive slots (one wavefront of i-atoms),
+// split into 8 clusters of 8 slots.  Padding slots hold atom -1.
+constexpr int kClusterSize = 8;
+constexpr int kClustersPerSc = 8;
+constexpr int kScSlots = 64;
+// Column sort capacity (atoms per xy column, sorted in LDS).
+constexpr int kColumnCap = 8192;
+// Fixed-point force unit: 2^-32 kcal/mol/A per integer step.
+constexpr double kFixedScale = 4294967296.0;
+constexpr double kFixedInv = 1.0 / 4294967296.0;
+
+// Per-system constants the nonbonded kernel reads.
+struct NbParams {
+    float sx, sy, sz;   // box_len / 2^32: A per fixed-point step, per axis
+    float rc2;          // r_cutoff^2
+    float rsw2;         // r_switch^2
+    float rsw;          // r_switch
+    float inv_w;        // 1 / (r_cutoff - r_switch)
+    float ds_coef;      // -30 / (r_cutoff - r_switch)
+};
+
+// Per-system constants for the list build.
+struct BuildParams {
+    double box[3];
+    double inv_q[3];    // 2^32 / box_len (quantisation)
+    float sq[3];        // box_len / 2^32
+    double r_skin2;     // r_skin^2 (f64, decision of neighbor.rs:38)
+    float r_search;     // r_skin + margin, bounding-box search radius
+    int ncx, ncy, ncol;
+    int periodic;
+    int n_atoms;
+    int max_sc;
+};
+
+}  // namespace gm
+
+// Device buffer with grow-only reallocation.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        if (p) { (void)hipFree(p); p = nullptr; n = 0; }
+        hipError_t e = hipMalloc(&p, sizeof(T) * (count > 0 ? count : 1));
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+    void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+};
+
+struct gm_ctx {
+    int device = 0;
+    int n_cu = 0;
+    std::string arch;
+};
+
+struct gm_sys {
+    gm_ctx *ctx = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_eval0 = nullptr, ev_nb0 = nullptr, ev_nb1 = nullptr, ev_eval1 = nullptr;
+    hipEvent_t ev_b0 = nullptr, ev_b1 = nullptr;
+
+    // parameters
+    int64_t n = 0;
+    double box[3] = {0, 0, 0};
+    int periodic = 0;
+    double rc = 0, rsw = 0, rskin = 0, scale14 = 0;
+    int bonded_mi = 0;
+    gm::NbParams nbp{};
+    gm::BuildParams bp{};
+
+    // topology (device, original atom order)
+    int64_t nb = 0, na = 0, nd = 0, n14 = 0;
+    DevBuf<double> sigma, eps;
+    DevBuf<float2> atom_par;  // (sigma/2, sqrt(eps)) in f32
+    DevBuf<uint32_t> bi, bj, ai, aj, ak, di, dj, dk, dl, dn, p14i, p14j;
+    DevBuf<double> bk, br0, akth, ath0, dkphi, ddelta;
+    DevBuf<int64_t> excl_off;
+    DevBuf<uint32_t> excl_nbr;
+
+    // working positions (device, original order) and rebuild reference
+    DevBuf<double> x, y, z, refx, refy, refz;
+    DevBuf<double> fx, fy, fz;  // resident f64 forces (atom order)
+    DevBuf<double> sx, sy, sz;  // staging for needs_rebuild positions
+
+    // spatial sort
+    DevBuf<int> col_count, col_start, col_sc, col_fill, bucket, sorted_atoms;
+    DevBuf<int> sc_base, sc_m;
+    DevBuf<uint32_t> qatom;  // 3*n fixed-point coordinates, atom order
+    DevBuf<int> meta;        // [0] nsc, [1] overflow, [2] max column, [5] rebuild flag;
+                             // ints 8..11 hold two u64 counters (list pairs, tiles)
+    int64_t max_sc = 0;
+
+    // slots
+    DevBuf<int> slot_atom, atom_slot;
+    DevBuf<int4> sq;         // fixed-point coords per slot (x, y, z, 0)
+    DevBuf<float2> sp;       // (sigma/2, sqrt eps) per slot
+    DevBuf<int4> cl_ref;     // cluster bbox centre (fixed point) + valid flag
+    DevBuf<float4> cl_half;  // cluster bbox half extents, A
+    DevBuf<int4> sc_ref;
+    DevBuf<float4> sc_half;
+
+    // pair-tile list
+    DevBuf<int> sc_count, sc_start;
+    DevBuf<int> ent_sc, ent_cj;
+    DevBuf<uint32_t> ent_imask;
+    DevBuf<unsigned long long> ent_mask;
+    DevBuf<int> wi_sc, wi_start, wi_end;
+    int chunk = 0;  // entries per work item (0 = automatic)
+
+    // accumulators
+    DevBuf<unsigned long long> F;  // 3 * n_pad, int64 fixed point, slot order
+    DevBuf<double> e_items;        // per bonded term energies
+    DevBuf<double> e_wi;           // per work item LJ energies
+    DevBuf<double> e_out;          // 5 reduced terms
+
+    // profiling: event pairs per evaluation (nonbonded start/end, eval start/end)
+    bool profiling = false;
+    std::vector<hipEvent_t> prof_pool;
+    int64_t prof_used = 0, prof_untimed = 0;
+    hipEvent_t cur_nb0 = nullptr, cur_nb1 = nullptr;
+    std::vector<int> sc_start_host;
+
+    bool built = false;
+    int64_t nsc = 0, ncl = 0, nent = 0, n_wi = 0;
+    gm_stats stats{};
+};
+
+// ---- error plumbing ----------------------------------------------------
+namespace gm {
+gm_status set_error(gm_status s, const std::string &msg);
+gm_status from_hip(hipError_t e, const char *what);
+}  // namespace gm
+
+#define GM_HIP(call)                                                   \
+    do {                                                               \
+        hipError_t _e = (call);                                        \
+        if (_e != hipSuccess) return gm::from_hip(_e, #call);          \
+    } while (0)
+
+// ---- kernel launchers (defined in the .hip files) --------------------------
+namespace gm {
+// gm_build.hip
+hipError_t launch_build(gm_sys *s, bool wrap);
+hipError_t launch_pair_count(gm_sys *s);
+hipError_t launch_pair_fill_and_masks(gm_sys *s);
+hipError_t launch_needs_rebuild(gm_sys *s, const double *x, const double *y, const double *z);
+// gm_nonbonded.hip
+hipError_t launch_nonbonded(gm_sys *s, const double *x, const double *y, const double *z);
+// gm_bonded.hip
+hipError_t launch_bonded(gm_sys *s, const double *x, const double *y, const double *z);
+hipError_t launch_finalize(gm_sys *s, double *fx, double *fy, double *fz);
+hipError_t launch_fill_u64(unsigned long long *p, size_t n, hipStream_t st);
+hipError_t launch_fill_i32(int *p, size_t n, int v, hipStream_t st);
+}  // namespace gm

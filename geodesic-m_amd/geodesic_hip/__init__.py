@@ -1,0 +1,378 @@
+"""Python front-end of libgeodesic_hip.so (the gfx950 force engine).
+
+Thin ctypes layer over the C-ABI in include/geodesic_hip.h, plus `GpuBackend`,
+a mirror of the reference's `ComputeBackend` implementation
+(geodesic-core/src/backend.rs:11-23, geodesic-gpu/src/gpu_backend.rs:11-100;
+paths relative to the reference repository root) so host code reads like the
+reference's own backend calls.
+
+There is no CPU fallback: if the library or a gfx950 device is missing, every
+entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libgeodesic_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "geodesic_hip.h")
+
+GM_OK = 0
+STATUS_NAMES = {
+    0: "GM_OK", 1: "GM_ERR_DEVICE_LOST", 2: "GM_ERR_KERNEL_LOAD", 3: "GM_ERR_OOM",
+    4: "GM_ERR_NO_DEVICE", 5: "GM_ERR_INVALID_ARG", 6: "GM_ERR_KERNEL", 7: "GM_ERR_NOT_BUILT",
+    8: "GM_ERR_CAPACITY",
+}
+
+_P = C.c_void_p
+_I64 = C.c_int64
+_D = C.c_double
+
+
+class GmError(RuntimeError):
+    """A non-zero gm_status.  `.status` holds the code (see STATUS_NAMES);
+    codes 1-4 correspond to geodesic-core BackendError (error.rs:82-91)."""
+
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [
+        ("n_atoms", _I64), ("sigma", _P), ("epsilon", _P), ("box", _D * 3), ("periodic", C.c_int32),
+        ("r_cutoff", _D), ("r_switch", _D), ("r_skin", _D), ("scale_14", _D),
+        ("bonded_min_image", C.c_int32),
+        ("n_bonds", _I64), ("bond_i", _P), ("bond_j", _P), ("bond_k", _P), ("bond_r0", _P),
+        ("n_angles", _I64), ("angle_i", _P), ("angle_j", _P), ("angle_k", _P), ("angle_kth", _P),
+        ("angle_th0", _P),
+        ("n_dihedrals", _I64), ("dihed_i", _P), ("dihed_j", _P), ("dihed_k", _P), ("dihed_l", _P),
+        ("dihed_kphi", _P), ("dihed_n", _P), ("dihed_delta", _P),
+        ("n_excl", _I64), ("excl_i", _P), ("excl_j", _P),
+        ("n_pairs14", _I64), ("pair14_i", _P), ("pair14_j", _P),
+    ]
+
+
+class EnergyTerms(C.Structure):
+    _fields_ = [("bond", _D), ("angle", _D), ("dihedral", _D), ("lj", _D), ("lj14", _D), ("total", _D)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Profile(C.Structure):
+    _fields_ = [("n_evals", _I64), ("n_untimed", _I64), ("ms_nonbonded_total", _D),
+                ("ms_nonbonded_min", _D), ("ms_nonbonded_max", _D), ("ms_eval_total", _D)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Stats(C.Structure):
+    _fields_ = [("n_atoms", _I64), ("n_superclusters", _I64), ("n_clusters", _I64),
+                ("n_entries", _I64), ("n_tiles", _I64), ("n_list_pairs", _I64),
+                ("n_work_items", _I64), ("ms_nonbonded", C.c_float), ("ms_eval", C.c_float),
+                ("ms_build", C.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load libgeodesic_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libgeodesic_hip.so not built at {LIB_PATH}; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "gm_init": ([C.c_int, C.POINTER(_P)], C.c_int),
+            "gm_ctx_destroy": ([_P], C.c_int),
+            "gm_last_error": ([], C.c_char_p),
+            "gm_device_count": ([], C.c_int),
+            "gm_system_create": ([_P, C.POINTER(SystemDesc), C.POINTER(_P)], C.c_int),
+            "gm_system_destroy": ([_P], C.c_int),
+            "gm_build_neighbor_list": ([_P, _P, _P, _P], C.c_int),
+            "gm_needs_rebuild": ([_P, _P, _P, _P, C.POINTER(C.c_int32)], C.c_int),
+            "gm_compute_forces": ([_P, _P, _P, _P, _P, _P, _P, C.POINTER(EnergyTerms)], C.c_int),
+            "gm_get_pair_list": ([_P, _P, _P, _I64, C.POINTER(_I64)], C.c_int),
+            "gm_build_neighbor_list_device": ([_P, _P, _P, _P], C.c_int),
+            "gm_compute_forces_device": ([_P, _P, _P, _P, _P, _P, _P], C.c_int),
+            "gm_sync": ([_P, C.POINTER(EnergyTerms)], C.c_int),
+            "gm_get_stats": ([_P, C.POINTER(Stats)], C.c_int),
+            "gm_profile_begin": ([_P], C.c_int),
+            "gm_profile_end": ([_P, C.POINTER(Profile)], C.c_int),
+            "gm_set_work_chunk": ([_P, C.c_int32], C.c_int),
+            "gm_set_positions": ([_P, _P, _P, _P], C.c_int),
+            "gm_compute_forces_resident": ([_P], C.c_int),
+            "gm_get_forces": ([_P, _P, _P, _P], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    """Function names declared in include/geodesic_hip.h."""
+    import re
+    with open(HEADER_PATH) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"^\s*(?:gm_status|const char \*|int)\s+(gm_\w+)\s*\(", text, re.M)))
+
+
+def _check(rc):
+    if rc != GM_OK:
+        raise GmError(rc, lib().gm_last_error().decode())
+
+
+def device_count():
+    return int(lib().gm_device_count())
+
+
+def _p(a):
+    return a.ctypes.data_as(_P) if a is not None and a.size else None
+
+
+class Context:
+    """gm_ctx: one per device (device.rs:8-36 try_new)."""
+
+    def __init__(self, device=0):
+        h = _P()
+        _check(lib().gm_init(int(device), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().gm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+@dataclass
+class Params:
+    """The SimParams fields the force path reads (params.rs:4-23) + extensions."""
+    box: tuple
+    periodic: bool
+    r_cutoff: float
+    r_switch: float
+    r_skin: float
+    scale_14: float = 0.0
+    bonded_min_image: bool = False
+
+
+class System:
+    """gm_sys: one simulated system on the device."""
+
+    def __init__(self, sigma, epsilon, topo, params: Params, ctx: Context | None = None):
+        ctx = ctx or default_context()
+        self.ctx = ctx
+        keep = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a
+
+        d = SystemDesc()
+        sig = arr(sigma, np.float64)
+        eps = arr(epsilon, np.float64)
+        self.n = len(sig)
+        d.n_atoms = self.n
+        d.sigma, d.epsilon = _p(sig), _p(eps)
+        box = np.broadcast_to(np.asarray(params.box, np.float64), (3,))
+        for k in range(3):
+            d.box[k] = float(box[k])
+        self.box = tuple(float(b) for b in box)
+        d.periodic = int(bool(params.periodic))
+        d.r_cutoff, d.r_switch, d.r_skin = params.r_cutoff, params.r_switch, params.r_skin
+        d.scale_14 = params.scale_14
+        d.bonded_min_image = int(bool(params.bonded_min_image))
+        u32, f64 = np.uint32, np.float64
+        g = lambda name, dt: arr(getattr(topo, name), dt)  # noqa: E731
+        bi, bj, bk, br0 = g("bond_i", u32), g("bond_j", u32), g("bond_k", f64), g("bond_r0", f64)
+        d.n_bonds = len(bi)
+        d.bond_i, d.bond_j, d.bond_k, d.bond_r0 = _p(bi), _p(bj), _p(bk), _p(br0)
+        ai, aj, ak = g("angle_i", u32), g("angle_j", u32), g("angle_k", u32)
+        akth, ath0 = g("angle_kth", f64), g("angle_th0", f64)
+        d.n_angles = len(ai)
+        d.angle_i, d.angle_j, d.angle_k, d.angle_kth, d.angle_th0 = _p(ai), _p(aj), _p(ak), _p(akth), _p(ath0)
+        di, dj, dk, dl = g("dihed_i", u32), g("dihed_j", u32), g("dihed_k", u32), g("dihed_l", u32)
+        dkp, dn, dd = g("dihed_kphi", f64), g("dihed_n", u32), g("dihed_delta", f64)
+        d.n_dihedrals = len(di)
+        d.dihed_i, d.dihed_j, d.dihed_k, d.dihed_l = _p(di), _p(dj), _p(dk), _p(dl)
+        d.dihed_kphi, d.dihed_n, d.dihed_delta = _p(dkp), _p(dn), _p(dd)
+        ei, ej = g("excl_i", u32), g("excl_j", u32)
+        d.n_excl = len(ei)
+        d.excl_i, d.excl_j = _p(ei), _p(ej)
+        p14i = arr(getattr(topo, "pair14_i", np.zeros(0)), u32)
+        p14j = arr(getattr(topo, "pair14_j", np.zeros(0)), u32)
+        d.n_pairs14 = len(p14i)
+        d.pair14_i, d.pair14_j = _p(p14i), _p(p14j)
+        h = _P()
+        _check(lib().gm_system_create(ctx.h, C.byref(d), C.byref(h)))
+        self.h = h
+        self.params = params
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gm_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def build_neighbor_list(self, x, y, z):
+        """Wraps x, y, z (float64, C-contiguous) in place when periodic."""
+        for a in (x, y, z):
+            assert a.dtype == np.float64 and a.flags.c_contiguous and len(a) == self.n
+        _check(lib().gm_build_neighbor_list(self.h, _p(x), _p(y), _p(z)))
+
+    def needs_rebuild(self, x, y, z):
+        out = C.c_int32(0)
+        x, y, z = (np.ascontiguousarray(a, np.float64) for a in (x, y, z))
+        _check(lib().gm_needs_rebuild(self.h, _p(x), _p(y), _p(z), C.byref(out)))
+        return bool(out.value)
+
+    def compute_forces(self, x, y, z):
+        """Returns (EnergyTerms, fx, fy, fz)."""
+        x, y, z = (np.ascontiguousarray(a, np.float64) for a in (x, y, z))
+        fx, fy, fz = np.empty(self.n), np.empty(self.n), np.empty(self.n)
+        e = EnergyTerms()
+        _check(lib().gm_compute_forces(self.h, _p(x), _p(y), _p(z), _p(fx), _p(fy), _p(fz), C.byref(e)))
+        return e, fx, fy, fz
+
+    def pair_list(self):
+        n = _I64(0)
+        _check(lib().gm_get_pair_list(self.h, None, None, 0, C.byref(n)))
+        pi = np.zeros(max(n.value, 1), np.uint32)
+        pj = np.zeros(max(n.value, 1), np.uint32)
+        _check(lib().gm_get_pair_list(self.h, _p(pi), _p(pj), n.value, C.byref(n)))
+        return pi[:n.value], pj[:n.value]
+
+    # device-resident variants: integer device pointers (e.g. tensor.data_ptr())
+    def build_neighbor_list_device(self, dx, dy, dz):
+        _check(lib().gm_build_neighbor_list_device(self.h, _P(dx), _P(dy), _P(dz)))
+
+    def compute_forces_device(self, dx, dy, dz, dfx, dfy, dfz):
+        _check(lib().gm_compute_forces_device(self.h, _P(dx), _P(dy), _P(dz), _P(dfx), _P(dfy), _P(dfz)))
+
+    def set_positions(self, x, y, z):
+        x, y, z = (np.ascontiguousarray(a, np.float64) for a in (x, y, z))
+        _check(lib().gm_set_positions(self.h, _p(x), _p(y), _p(z)))
+
+    def compute_forces_resident(self):
+        """Asynchronous evaluation at the resident positions; see sync()."""
+        _check(lib().gm_compute_forces_resident(self.h))
+
+    def get_forces(self):
+        fx, fy, fz = np.empty(self.n), np.empty(self.n), np.empty(self.n)
+        _check(lib().gm_get_forces(self.h, _p(fx), _p(fy), _p(fz)))
+        return fx, fy, fz
+
+    def sync(self):
+        e = EnergyTerms()
+        _check(lib().gm_sync(self.h, C.byref(e)))
+        return e
+
+    def profile_begin(self):
+        _check(lib().gm_profile_begin(self.h))
+
+    def profile_end(self):
+        p = Profile()
+        _check(lib().gm_profile_end(self.h, C.byref(p)))
+        return p.as_dict()
+
+    def set_work_chunk(self, entries_per_item):
+        _check(lib().gm_set_work_chunk(self.h, int(entries_per_item)))
+
+    def stats(self):
+        s = Stats()
+        _check(lib().gm_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+
+# ---- mirror of the reference's backend API ------------------------------------
+
+@dataclass
+class ForceBuffer:
+    """geodesic-core ForceBuffer (buffers.rs:5-9)."""
+    fx: np.ndarray
+    fy: np.ndarray
+    fz: np.ndarray
+
+
+class GpuBackend:
+    """ComputeBackend over the gfx950 engine (gpu_backend.rs:11-100).
+
+    `try_new(atoms, topology, params)` raises GmError with the BackendError
+    code on failure, like GpuBackend::try_new.  `state` objects need float64
+    `pos_x/pos_y/pos_z` arrays (SimState, state.rs:3-23).  geodesic_drift is
+    the integrator (outside the force path) and is not provided here."""
+
+    def __init__(self, system: System):
+        self._sys = system
+        self._reduced = None
+        self._potential = 0.0
+        self._terms = None
+
+    @classmethod
+    def try_new(cls, atoms, topology, params: Params, ctx: Context | None = None):
+        return cls(System(atoms.sigma, atoms.epsilon, topology, params, ctx))
+
+    def build_neighbor_list(self, state, params=None):
+        self._sys.build_neighbor_list(state.pos_x, state.pos_y, state.pos_z)
+
+    def compute_forces(self, state):
+        e, fx, fy, fz = self._sys.compute_forces(state.pos_x, state.pos_y, state.pos_z)
+        self._reduced = ForceBuffer(fx, fy, fz)
+        self._potential = e.total
+        self._terms = e
+        return self._reduced
+
+    def reduce_forces(self):
+        r = self._reduced
+        return ForceBuffer(r.fx.copy(), r.fy.copy(), r.fz.copy())
+
+    def potential_energy(self):
+        return self._potential
+
+    def energy_terms(self):
+        """Per-term breakdown (EnergyReport, lib.rs:37-45), an inherent addition."""
+        return self._terms.as_dict()
+
+    def needs_rebuild(self, state):
+        return self._sys.needs_rebuild(state.pos_x, state.pos_y, state.pos_z)
+
+    def n_threads(self):
+        return 1
+
+    def geodesic_drift(self, state, dt):
+        raise NotImplementedError("geodesic_drift (integrator + constraints) is outside the GPU force path")

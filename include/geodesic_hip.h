@@ -1,0 +1,181 @@
+/*
+ * geodesic_hip.h -- C-ABI of the MI355X (gfx950) force-evaluation engine.
+ *
+ * This is the drop-in boundary for GEODESIC-M's `ComputeBackend` trait
+ * (geodesic-core/src/backend.rs:11-23; paths are relative to the reference
+ * repository root).  A Rust FFI crate binds these symbols and implements the
+ * trait over them; the reference's wgpu `GpuBackend`
+ * (geodesic-gpu/src/gpu_backend.rs:11-100) is what it replaces.
+ *
+ * Conventions
+ *   - Every function returns gm_status (0 = GM_OK).  gm_last_error() gives a
+ *     message for the most recent failure on the calling thread.
+ *   - Arrays are plain pointers + sizes, SoA, f64 for positions/forces/
+ *     parameters and uint32 for indices, exactly the layouts of SimState
+ *     (state.rs:3-23), AtomData (atoms.rs:3-12), BondedTopology
+ *     (topology.rs:4-37) and ForceBuffer (buffers.rs:5-9).
+ *   - gm_system_create copies every array; the caller keeps ownership.
+ *   - One host thread per gm_sys at a time; distinct gm_sys objects may be
+ *     used concurrently.  Each gm_sys owns one HIP stream (default priority).
+ *   - Functions taking host pointers are synchronous.  The *_device variants
+ *     take device pointers (HBM resident) and are ordered on the system's
+ *     stream; gm_sync waits for them.
+ *   - Results are bit-for-bit reproducible run to run: pair forces are
+ *     computed in FP32 and accumulated in int64 fixed point (2^-32 kcal/mol/A
+ *     per unit); energies are reduced in f64 in a fixed order.
+ */
+#ifndef GEODESIC_HIP_H
+#define GEODESIC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  1..4 map 1:1 onto geodesic-core BackendError
+ * (error.rs:82-91): DeviceLost, ShaderCompilation, OutOfGpuMemory, NoAdapter. */
+typedef int gm_status;
+enum {
+    GM_OK = 0,
+    GM_ERR_DEVICE_LOST = 1,   /* BackendError::DeviceLost */
+    GM_ERR_KERNEL_LOAD = 2,   /* BackendError::ShaderCompilation (code object missing for this GPU) */
+    GM_ERR_OOM = 3,           /* BackendError::OutOfGpuMemory */
+    GM_ERR_NO_DEVICE = 4,     /* BackendError::NoAdapter */
+    GM_ERR_INVALID_ARG = 5,   /* bad descriptor / index out of range / r_switch<r_cutoff<r_skin violated */
+    GM_ERR_KERNEL = 6,        /* a kernel launch or HIP call failed */
+    GM_ERR_NOT_BUILT = 7,     /* compute before build_neighbor_list */
+    GM_ERR_CAPACITY = 8,      /* output buffer too small / fixed-point or LDS capacity exceeded */
+};
+
+typedef struct gm_ctx gm_ctx; /* one per device */
+typedef struct gm_sys gm_sys; /* one per simulated system */
+
+/* Everything GpuBackend::try_new receives (gpu_backend.rs:24-53):
+ * AtomData (sigma, epsilon), BondedTopology (bonds, angles, dihedrals,
+ * exclusions) and the SimParams fields the force path reads (params.rs:4-23).
+ * Extensions with no reference counterpart are marked [ext]. */
+typedef struct {
+    int64_t n_atoms;
+    const double *sigma;   /* per atom, A */
+    const double *epsilon; /* per atom, kcal/mol */
+    double box[3];         /* SimParams::box_size */
+    int32_t periodic;      /* SimParams::periodic: wrap positions at list build */
+    double r_cutoff, r_switch, r_skin; /* r_skin is the list radius (topology.rs:53) */
+    double scale_14;           /* [ext] LJ scale for pair14_*; 0 = reference behaviour */
+    int32_t bonded_min_image;  /* [ext] min-image bonded displacements; 0 = reference raw */
+    int64_t n_bonds;
+    const uint32_t *bond_i, *bond_j;
+    const double *bond_k, *bond_r0;
+    int64_t n_angles;
+    const uint32_t *angle_i, *angle_j, *angle_k; /* j is the central atom */
+    const double *angle_kth, *angle_th0;
+    int64_t n_dihedrals;
+    const uint32_t *dihed_i, *dihed_j, *dihed_k, *dihed_l;
+    const double *dihed_kphi;
+    const uint32_t *dihed_n;
+    const double *dihed_delta;
+    int64_t n_excl;
+    const uint32_t *excl_i, *excl_j;
+    int64_t n_pairs14;                 /* [ext] scaled 1-4 pairs */
+    const uint32_t *pair14_i, *pair14_j;
+} gm_system_desc;
+
+/* Per-term potential energies in kcal/mol (geodesic/src/lib.rs:37-45
+ * EnergyReport: bond, angle, dihedral, nonbonded, total), plus the [ext]
+ * scaled 1-4 term.  total = bond + angle + dihedral + lj14 + lj. */
+typedef struct {
+    double bond, angle, dihedral, lj, lj14, total;
+} gm_energy_terms;
+
+/* Device / context.  Replaces geodesic-gpu/src/device.rs:8-36 (try_new):
+ * GM_ERR_NO_DEVICE when no gfx950 device exists at that ordinal. */
+gm_status gm_init(int device_ordinal, gm_ctx **out);
+gm_status gm_ctx_destroy(gm_ctx *ctx);
+/* Message for the most recent failure on the calling thread ("" if none). */
+const char *gm_last_error(void);
+/* Number of visible HIP devices (0 without a GPU); never fails. */
+int gm_device_count(void);
+
+/* GpuBackend::try_new (gpu_backend.rs:24-53): validates and copies the
+ * descriptor to the device. */
+gm_status gm_system_create(gm_ctx *ctx, const gm_system_desc *desc, gm_sys **out);
+gm_status gm_system_destroy(gm_sys *sys);
+
+/* ComputeBackend::build_neighbor_list (backend.rs:15; neighbor.rs:8-55).
+ * Host positions; wraps them in place into [0, box) when periodic and
+ * records them as the reference positions for needs_rebuild. */
+gm_status gm_build_neighbor_list(gm_sys *sys, double *x, double *y, double *z);
+
+/* ComputeBackend::needs_rebuild (neighbor.rs:60-72). */
+gm_status gm_needs_rebuild(gm_sys *sys, const double *x, const double *y, const double *z,
+                           int32_t *out);
+
+/* ComputeBackend::compute_forces + potential_energy (cpu_backend.rs:92-182,
+ * gpu_backend.rs:64-82): forces into caller memory (f64 SoA, length n_atoms),
+ * per-term energies into *e (may be NULL). */
+gm_status gm_compute_forces(gm_sys *sys, const double *x, const double *y, const double *z,
+                            double *fx, double *fy, double *fz, gm_energy_terms *e);
+
+/* Canonical export of the current Verlet list for bit-exact checks: unordered
+ * pairs (i<j) sorted i-major, exactly the set neighbor.rs:25-43 produces.
+ * Writes min(n, cap) pairs; *n receives the full count. */
+gm_status gm_get_pair_list(gm_sys *sys, uint32_t *pair_i, uint32_t *pair_j, int64_t cap,
+                           int64_t *n);
+
+/* Device-resident variants: x/y/z/f* are device pointers (length n_atoms).
+ * Ordered on the system's stream; energies are delivered by gm_sync. */
+gm_status gm_build_neighbor_list_device(gm_sys *sys, double *d_x, double *d_y, double *d_z);
+gm_status gm_compute_forces_device(gm_sys *sys, const double *d_x, const double *d_y,
+                                   const double *d_z, double *d_fx, double *d_fy, double *d_fz);
+/* Waits for the system's stream; writes the last evaluation's energies (e may be NULL). */
+gm_status gm_sync(gm_sys *sys, gm_energy_terms *e);
+
+/* Resident variants: the system keeps one set of f64 positions and forces in
+ * HBM (atom order).  gm_build_neighbor_list / gm_compute_forces leave their
+ * (wrapped) positions there; gm_set_positions replaces them from the host.
+ * gm_compute_forces_resident evaluates at the resident positions into the
+ * resident forces, ordered on the system's stream; gm_get_forces copies the
+ * forces out (synchronous). */
+gm_status gm_set_positions(gm_sys *sys, const double *x, const double *y, const double *z);
+gm_status gm_compute_forces_resident(gm_sys *sys);
+gm_status gm_get_forces(gm_sys *sys, double *fx, double *fy, double *fz);
+
+/* Statistics of the current list and the last evaluation. */
+typedef struct {
+    int64_t n_atoms;
+    int64_t n_superclusters;   /* 64-slot i-groups */
+    int64_t n_clusters;        /* 8-slot groups */
+    int64_t n_entries;         /* (supercluster, j-cluster) list entries */
+    int64_t n_tiles;           /* (i-cluster, j-cluster) 8x8 tiles with a nonzero mask */
+    int64_t n_list_pairs;      /* pairs in the Verlet list */
+    int64_t n_work_items;      /* waves launched by the nonbonded kernel */
+    float ms_nonbonded;        /* last nonbonded kernel, hipEvent timing */
+    float ms_eval;             /* last whole evaluation (device time) */
+    float ms_build;            /* last list build (device time) */
+} gm_stats;
+gm_status gm_get_stats(gm_sys *sys, gm_stats *out);
+
+/* Per-evaluation device timings, recorded with HIP events on the system's
+ * stream between gm_profile_begin and gm_profile_end (at most 8192
+ * evaluations are timed; later ones are counted in n_untimed). */
+typedef struct {
+    int64_t n_evals;            /* evaluations timed */
+    int64_t n_untimed;          /* evaluations past the event pool */
+    double ms_nonbonded_total;  /* sum over timed evaluations, nonbonded kernel */
+    double ms_nonbonded_min, ms_nonbonded_max;
+    double ms_eval_total;       /* sum over timed evaluations, whole evaluation */
+} gm_profile;
+gm_status gm_profile_begin(gm_sys *sys);
+gm_status gm_profile_end(gm_sys *sys, gm_profile *out);
+
+/* Tuning: list entries per nonbonded work item (one wavefront each); 0 picks
+ * a size that gives about 8 work items per compute unit.  Takes effect
+ * immediately on a built list (no rebuild). */
+gm_status gm_set_work_chunk(gm_sys *sys, int32_t entries_per_item);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GEODESIC_HIP_H */

@@ -1,0 +1,274 @@
+"""GPU parity: the gfx950 engine, called through the C-ABI, against the f64 oracle.
+
+Mirrors the reference's GPU acceptance tests (geodesic-gpu/tests/
+forces_gpu_vs_cpu.rs:7-24, gpu_determinism.rs:7-20; paths relative to the
+reference repository root) with tighter bars:
+  - Verlet list: bit-exact pair set vs neighbor.rs:25-43;
+  - per-term energies: relative error <= 1e-6 (terms that are exactly zero
+    in the oracle must be exactly zero);
+  - forces: RMS error relative to the RMS force <= 1e-5, and every component
+    within 1e-4 * max(|F|, 1) (the reference's own bar);
+  - two evaluations, and two independent builds, byte-identical.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import geodesic_hip as gh
+from geodesic_hip import systems
+from oracle import oracle as orc
+
+from _common import force_rms_rel, load
+
+pytestmark = pytest.mark.gpu
+
+E_REL = 1e-6
+F_RMS = 1e-5
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_reference(x, y, z, sigma, epsilon, topo, p: gh.Params, method="auto", threads=8):
+    """Wrap (when periodic) + list + CpuBackend::compute_forces in f64."""
+    x, y, z = x.copy(), y.copy(), z.copy()
+    if p.periodic:
+        orc.wrap_positions(x, y, z, p.box)
+    pi, pj = orc.neighbor_build(x, y, z, p.box, p.r_skin, topo.excl_i, topo.excl_j, method=method)
+    t, fx, fy, fz = orc.cpu_backend_compute(x, y, z, sigma, epsilon, topo, pi, pj, p.box, p.r_cutoff,
+                                            p.r_switch, n_threads=threads, scale14=p.scale_14,
+                                            bonded_min_image=p.bonded_min_image, periodic=p.periodic)
+    return (x, y, z), (pi, pj), t, (fx, fy, fz)
+
+
+def check_terms(e, t, tag):
+    for name in ("bond", "angle", "dihedral", "lj", "lj14", "total"):
+        g, r = getattr(e, name), getattr(t, name)
+        if r == 0.0:
+            assert g == 0.0, f"{tag}: {name} gpu={g} oracle=0"
+        else:
+            assert abs(g - r) <= E_REL * abs(r), f"{tag}: {name} gpu={g!r} oracle={r!r} rel={abs(g - r) / abs(r):.3e}"
+
+
+def check_forces(f, g, tag):
+    fx, fy, fz = f
+    gx, gy, gz = g
+    rms = force_rms_rel(fx, fy, fz, gx, gy, gz)
+    assert rms <= F_RMS, f"{tag}: force RMS rel {rms:.3e}"
+    for a, b in ((fx, gx), (fy, gy), (fz, gz)):
+        bound = 1e-4 * np.maximum(np.abs(b), 1.0)
+        assert np.all(np.abs(a - b) <= bound), f"{tag}: component bound, max {np.max(np.abs(a - b) / bound):.3e}"
+    return rms
+
+
+def run_gpu(x, y, z, sigma, epsilon, topo, p):
+    s = gh.System(sigma, epsilon, topo, p)
+    x, y, z = x.copy(), y.copy(), z.copy()
+    s.build_neighbor_list(x, y, z)
+    e, fx, fy, fz = s.compute_forces(x, y, z)
+    return s, (x, y, z), e, (fx, fy, fz)
+
+
+def fixture_params(n):
+    """geodesic-gpu/tests/common/mod.rs:16-32."""
+    return gh.Params(box=(100.0,) * 3, periodic=True, r_cutoff=12.0, r_switch=10.0, r_skin=14.0)
+
+
+@pytest.mark.parametrize("name", ["lj_pair", "harmonic_dimer", "water_box_4", "ala_dipeptide"])
+def test_fixture_full_backend_matches_cpu(name):
+    say(f"[gpu] fixture {name}")
+    (x, y, z), atoms, topo = load(name)
+    p = fixture_params(len(x))
+    (wx, wy, wz), (pi, pj), t, f = oracle_reference(x, y, z, atoms.sigma, atoms.epsilon, topo, p)
+    s, (gx, gy, gz), e, g = run_gpu(x, y, z, atoms.sigma, atoms.epsilon, topo, p)
+    assert gx.tobytes() == wx.tobytes() and gy.tobytes() == wy.tobytes() and gz.tobytes() == wz.tobytes()
+    qi, qj = s.pair_list()
+    assert np.array_equal(qi, pi) and np.array_equal(qj, pj)
+    check_terms(e, t, name)
+    check_forces(g, f, name)
+
+
+def ala_dipeptide_cfg0():
+    """22-atom alanine dipeptide, vacuum, coordinates + U[-0.05, 0.05] (seed 0),
+    1-4 LJ scaled by 1/2, r_c/r_skin/r_sw 12/14/10 (lib.rs:31-34), box 1000."""
+    (x, y, z), atoms, topo = load("ala_dipeptide")
+    r = systems.Rng(0)
+    x, y, z = (v + r.uniform(len(v), -0.05, 0.05) for v in (x, y, z))
+    p = gh.Params(box=(1000.0,) * 3, periodic=False, r_cutoff=12.0, r_switch=10.0, r_skin=14.0,
+                  scale_14=0.5)
+    return (x, y, z), atoms, topo, p
+
+
+def test_ala_dipeptide_vacuum_with_scaled_14_matches_cpu():
+    say("[gpu] ala_dipeptide vacuum + 1-4")
+    (x, y, z), atoms, topo, p = ala_dipeptide_cfg0()
+    (wx, wy, wz), (pi, pj), t, f = oracle_reference(x, y, z, atoms.sigma, atoms.epsilon, topo, p)
+    assert len(pi) == 133 and t.lj14 != 0.0
+    s, (gx, gy, gz), e, g = run_gpu(x, y, z, atoms.sigma, atoms.epsilon, topo, p)
+    assert gx.tobytes() == x.tobytes()  # non-periodic: not wrapped
+    qi, qj = s.pair_list()
+    assert np.array_equal(qi, pi) and np.array_equal(qj, pj)
+    check_terms(e, t, "ala_dipeptide")
+    check_forces(g, f, "ala_dipeptide")
+
+
+@pytest.fixture(scope="module")
+def sys24k():
+    s = systems.system_24k()
+    say("[gpu] 24k oracle reference")
+    ref = oracle_reference(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params, method="cells")
+    return s, ref
+
+
+def test_24k_list_energies_forces_match_oracle(sys24k):
+    s, ((wx, wy, wz), (pi, pj), t, f) = sys24k
+    say("[gpu] 24k build + compute")
+    g, (gx, gy, gz), e, fg = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params)
+    assert gx.tobytes() == wx.tobytes() and gy.tobytes() == wy.tobytes() and gz.tobytes() == wz.tobytes()
+    qi, qj = g.pair_list()
+    assert len(qi) == len(pi)
+    assert np.array_equal(qi, pi) and np.array_equal(qj, pj)
+    st = g.stats()
+    assert st["n_list_pairs"] == len(pi)
+    check_terms(e, t, "24k")
+    rms = check_forces(fg, f, "24k")
+    say(f"[gpu] 24k force rms rel {rms:.3e} stats {st}")
+
+
+def test_24k_reruns_are_byte_identical(sys24k):
+    """gpu_determinism.rs:7-20, at 24k atoms, across evaluations and builds."""
+    s, _ = sys24k
+    say("[gpu] 24k determinism")
+    g1, (x, y, z), e1, f1 = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params)
+    e2, *f2 = g1.compute_forces(x, y, z)
+    g2, _, e3, f3 = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params)
+    for a, b in ((f1, f2), (f1, f3)):
+        for u, v in zip(a, b):
+            assert u.tobytes() == v.tobytes()
+    for name in ("bond", "angle", "dihedral", "lj", "lj14", "total"):
+        assert getattr(e1, name) == getattr(e2, name) == getattr(e3, name)
+
+
+def test_24k_work_chunking_does_not_change_results(sys24k):
+    s, _ = sys24k
+    say("[gpu] 24k chunk sizes")
+    g, (x, y, z), e0, f0 = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params)
+    for chunk in (1, 7, 64, 100000):
+        g.set_work_chunk(chunk)
+        e, *f = g.compute_forces(x, y, z)
+        # fixed-point force accumulation: order- and partition-independent to
+        # within the rounding of each partial, far below the parity bars
+        for u, v in zip(f, f0):
+            assert np.max(np.abs(u - v)) < 1e-6 * max(1.0, np.max(np.abs(v)))
+        assert abs(e.lj - e0.lj) <= 1e-9 * abs(e0.lj)
+
+
+def test_needs_rebuild_matches_oracle(sys24k):
+    """neighbor.rs:60-72 at half-skin 0.75 A: 0.74 A -> no rebuild, 0.76 A -> rebuild."""
+    s, ((wx, wy, wz), _, _, _) = sys24k
+    say("[gpu] needs_rebuild")
+    g = gh.System(s.sigma, s.epsilon, s.topo, s.params)
+    x, y, z = s.x.copy(), s.y.copy(), s.z.copy()
+    g.build_neighbor_list(x, y, z)
+    assert not g.needs_rebuild(x, y, z)
+    for d, expect in ((0.74, False), (0.76, True)):
+        x2 = x.copy()
+        x2[12345] += d
+        assert g.needs_rebuild(x2, y, z) == expect
+        assert orc.needs_rebuild(x2, y, z, wx, wy, wz, s.params.r_skin, s.params.r_cutoff) == expect
+
+
+@pytest.mark.parametrize("n", [500, 926, 2000])
+def test_small_periodic_boxes_use_the_single_nearest_image(n):
+    """Replica-sized boxes down to L = 17.1 A < 2 r_list: still one image per pair."""
+    say(f"[gpu] small box n={n}")
+    s = systems.protein_solvent(n, (n / 0.1) ** (1 / 3), n // 8, seed=11 + n)
+    (wx, wy, wz), (pi, pj), t, f = oracle_reference(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params,
+                                                    method="brute")
+    g, _, e, fg = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params)
+    qi, qj = g.pair_list()
+    assert np.array_equal(qi, pi) and np.array_equal(qj, pj)
+    check_terms(e, t, f"n={n}")
+    check_forces(fg, f, f"n={n}")
+
+
+def test_empty_and_single_atom_systems():
+    say("[gpu] empty / single")
+    z32, zf = np.zeros(0, np.uint32), np.zeros(0)
+    topo = systems.Topology(z32, z32, zf, zf, z32, z32, z32, zf, zf, z32, z32, z32, z32, zf, z32, zf, z32, z32)
+    p = gh.Params(box=(20.0,) * 3, periodic=True, r_cutoff=9.0, r_switch=7.5, r_skin=10.5)
+    g = gh.System(zf, zf, topo, p)
+    g.build_neighbor_list(zf.copy(), zf.copy(), zf.copy())
+    e, fx, fy, fz = g.compute_forces(zf, zf, zf)
+    assert e.total == 0.0 and len(fx) == 0
+    g1 = gh.System(np.array([3.0]), np.array([0.1]), topo, p)
+    x, y, z = np.array([25.0]), np.array([-1.0]), np.array([3.0])
+    g1.build_neighbor_list(x, y, z)
+    assert x[0] == 5.0 and y[0] == 19.0
+    e, fx, fy, fz = g1.compute_forces(x, y, z)
+    assert e.total == 0.0 and fx[0] == 0.0
+
+
+def test_all_pairs_excluded_gives_empty_list():
+    say("[gpu] all excluded")
+    (x, y, z), atoms, topo = load("water_box_4")
+    ii, jj = np.triu_indices(12, 1)
+    topo.excl_i, topo.excl_j = ii.astype(np.uint32), jj.astype(np.uint32)
+    g, _, e, f = run_gpu(x, y, z, atoms.sigma, atoms.epsilon, topo, fixture_params(12))
+    qi, _ = g.pair_list()
+    assert len(qi) == 0 and e.lj == 0.0
+
+
+def test_errors_are_reported_not_crashed():
+    say("[gpu] error paths")
+    (x, y, z), atoms, topo = load("water_box_4")
+    g = gh.System(atoms.sigma, atoms.epsilon, topo, fixture_params(12))
+    with pytest.raises(gh.GmError) as ei:
+        g.compute_forces(x, y, z)
+    assert ei.value.status == 7  # GM_ERR_NOT_BUILT
+    bad = gh.Params(box=(100.0,) * 3, periodic=True, r_cutoff=12.0, r_switch=13.0, r_skin=14.0)
+    with pytest.raises(gh.GmError) as ei:
+        gh.System(atoms.sigma, atoms.epsilon, topo, bad)
+    assert ei.value.status == 5  # GM_ERR_INVALID_ARG: r_switch < r_cutoff violated
+    topo.bond_j = topo.bond_j.copy()
+    topo.bond_j[0] = 99
+    with pytest.raises(gh.GmError) as ei:
+        gh.System(atoms.sigma, atoms.epsilon, topo, fixture_params(12))
+    assert ei.value.status == 5
+
+
+def test_device_resident_path_equals_host_path(sys24k):
+    torch = pytest.importorskip("torch")
+    s, _ = sys24k
+    say("[gpu] device-resident 24k")
+    g, (x, y, z), e, f = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params)
+    dev = torch.device("cuda:0")
+    dx, dy, dz = (torch.from_numpy(v.copy()).to(dev) for v in (x, y, z))
+    out = [torch.empty(len(x), dtype=torch.float64, device=dev) for _ in range(3)]
+    torch.cuda.synchronize()
+    g.compute_forces_device(dx.data_ptr(), dy.data_ptr(), dz.data_ptr(), *[o.data_ptr() for o in out])
+    e2 = g.sync()
+    for a, b in zip(out, f):
+        assert a.cpu().numpy().tobytes() == b.tobytes()
+    assert e2.total == e.total
+
+
+def test_250k_full_size_properties():
+    """At the full 250k size: the list equals the oracle's, energies and forces
+    within the bars, and reruns byte-identical."""
+    s = systems.system_250k()
+    say("[gpu] 250k oracle reference (cells)")
+    (wx, wy, wz), (pi, pj), t, f = oracle_reference(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params,
+                                                    method="cells", threads=16)
+    say("[gpu] 250k build + compute")
+    g, (x, y, z), e, fg = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params)
+    qi, qj = g.pair_list()
+    assert np.array_equal(qi, pi) and np.array_equal(qj, pj)
+    check_terms(e, t, "250k")
+    check_forces(fg, f, "250k")
+    e2, *f2 = g.compute_forces(x, y, z)
+    for u, v in zip(fg, f2):
+        assert u.tobytes() == v.tobytes()
