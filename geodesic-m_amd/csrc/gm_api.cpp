@@ -198,6 +198,7 @@ gm_status collect(gm_sys *s, gm_energy_terms *e) {
     float a = 0.f, b = 0.f;
     if (hipEventElapsedTime(&a, s->ev_nb0, s->ev_nb1) == hipSuccess) s->stats.ms_nonbonded = a;
     if (hipEventElapsedTime(&b, s->ev_eval0, s->ev_eval1) == hipSuccess) s->stats.ms_eval = b;
+    (void)hipGetLastError();  // the events are unrecorded while profiling; do not leave that error sticky
     if (e) {
         e->bond = t[0];
         e->angle = t[1];
@@ -247,6 +248,32 @@ gm_status gm_init(int device_ordinal, gm_ctx **out) {
     c->n_cu = prop.multiProcessorCount;
     c->arch = arch;
     *out = c;
+    return GM_OK;
+}
+
+gm_status gm_device_alloc(gm_ctx *ctx, size_t bytes, void **out) {
+    GM_REQUIRE(ctx && out, "NULL argument");
+    *out = nullptr;
+    GM_HIP(hipSetDevice(ctx->device));
+    GM_HIP(hipMalloc(out, bytes > 0 ? bytes : 1));
+    return GM_OK;
+}
+
+gm_status gm_device_free(gm_ctx *ctx, void *p) {
+    GM_REQUIRE(ctx, "NULL context");
+    if (p) GM_HIP(hipFree(p));
+    return GM_OK;
+}
+
+gm_status gm_copy_to_device(gm_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    GM_REQUIRE(ctx && (bytes == 0 || (dst && src)), "NULL argument");
+    if (bytes) GM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return GM_OK;
+}
+
+gm_status gm_copy_to_host(gm_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    GM_REQUIRE(ctx && (bytes == 0 || (dst && src)), "NULL argument");
+    if (bytes) GM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return GM_OK;
 }
 

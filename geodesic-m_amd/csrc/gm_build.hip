@@ -197,7 +197,9 @@ __global__ void k_slot_setup(int nslots, int n, const int *slot_atom, const uint
 __device__ inline int wrapdiff(int a, int b) { return (int)((unsigned)a - (unsigned)b); }
 
 // Bounding boxes (fixed-point centre, half extents in A) of every cluster and
-// supercluster.  A cluster is valid iff its first slot holds an atom.
+// supercluster.  A cluster is valid iff its first slot holds an atom.  Offsets
+// are taken relative to the group's first atom (minimum image), so min <= 0 <=
+// max; spans can exceed half the box in small boxes, hence computed in float.
 __global__ __launch_bounds__(64) void k_bbox(const int *meta, const int *slot_atom, const int4 *sq,
                                              BuildParams bp, int4 *cl_ref, float4 *cl_half,
                                              int4 *sc_ref, float4 *sc_half) {
@@ -223,9 +225,9 @@ __global__ __launch_bounds__(64) void k_bbox(const int *meta, const int *slot_at
         cl_ref[c] = make_int4((int)((unsigned)r.x + (unsigned)((mnx + mxx) / 2)),
                               (int)((unsigned)r.y + (unsigned)((mny + mxy) / 2)),
                               (int)((unsigned)r.z + (unsigned)((mnz + mxz) / 2)), cvalid ? 1 : 0);
-        cl_half[c] = make_float4(((float)(mxx - mnx) * 0.5f + 1.0f) * bp.sq[0],
-                                 ((float)(mxy - mny) * 0.5f + 1.0f) * bp.sq[1],
-                                 ((float)(mxz - mnz) * 0.5f + 1.0f) * bp.sq[2], 0.f);
+        cl_half[c] = make_float4((((float)mxx - (float)mnx) * 0.5f + 1.0f) * bp.sq[0],
+                                 (((float)mxy - (float)mny) * 0.5f + 1.0f) * bp.sq[1],
+                                 (((float)mxz - (float)mnz) * 0.5f + 1.0f) * bp.sq[2], 0.f);
     }
     const int4 r0 = sq[(size_t)sc * 64];
     dx = valid ? wrapdiff(v.x, r0.x) : 0;
@@ -241,9 +243,9 @@ __global__ __launch_bounds__(64) void k_bbox(const int *meta, const int *slot_at
         sc_ref[sc] = make_int4((int)((unsigned)r0.x + (unsigned)((mnx + mxx) / 2)),
                                (int)((unsigned)r0.y + (unsigned)((mny + mxy) / 2)),
                                (int)((unsigned)r0.z + (unsigned)((mnz + mxz) / 2)), 1);
-        sc_half[sc] = make_float4(((float)(mxx - mnx) * 0.5f + 1.0f) * bp.sq[0],
-                                  ((float)(mxy - mny) * 0.5f + 1.0f) * bp.sq[1],
-                                  ((float)(mxz - mnz) * 0.5f + 1.0f) * bp.sq[2], 0.f);
+        sc_half[sc] = make_float4((((float)mxx - (float)mnx) * 0.5f + 1.0f) * bp.sq[0],
+                                  (((float)mxy - (float)mny) * 0.5f + 1.0f) * bp.sq[1],
+                                  (((float)mxz - (float)mnz) * 0.5f + 1.0f) * bp.sq[2], 0.f);
     }
 }
 

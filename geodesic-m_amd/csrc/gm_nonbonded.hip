@@ -164,9 +164,9 @@ namespace gm {
 
 hipError_t launch_nonbonded(gm_sys *s, const double *x, const double *y, const double *z) {
     const size_t nslots = (size_t)s->nsc * 64;
-    if (nslots == 0) return hipSuccess;
-    k_quantize_slots<<<grid_for(nslots, 256), 256, 0, s->stream>>>(
-        (int)nslots, s->slot_atom.p, x, y, z, s->bp.inv_q[0], s->bp.inv_q[1], s->bp.inv_q[2], s->sq.p);
+    if (nslots > 0)
+        k_quantize_slots<<<grid_for(nslots, 256), 256, 0, s->stream>>>(
+            (int)nslots, s->slot_atom.p, x, y, z, s->bp.inv_q[0], s->bp.inv_q[1], s->bp.inv_q[2], s->sq.p);
     hipError_t e = hipEventRecord(s->cur_nb0, s->stream);
     if (e != hipSuccess) return e;
     if (s->n_wi > 0)

@@ -113,6 +113,10 @@ def lib():
             "gm_set_positions": ([_P, _P, _P, _P], C.c_int),
             "gm_compute_forces_resident": ([_P], C.c_int),
             "gm_get_forces": ([_P, _P, _P, _P], C.c_int),
+            "gm_device_alloc": ([_P, C.c_size_t, C.POINTER(_P)], C.c_int),
+            "gm_device_free": ([_P, _P], C.c_int),
+            "gm_copy_to_device": ([_P, _P, _P, C.c_size_t], C.c_int),
+            "gm_copy_to_host": ([_P, _P, _P, C.c_size_t], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -159,6 +163,40 @@ class Context:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A float64 array in device memory owned through the C-ABI helpers."""
+
+    def __init__(self, n, ctx: "Context | None" = None):
+        self.ctx = ctx or default_context()
+        self.n = int(n)
+        h = _P()
+        _check(lib().gm_device_alloc(self.ctx.h, 8 * self.n, C.byref(h)))
+        self.ptr = h.value
+
+    @classmethod
+    def from_host(cls, a, ctx=None):
+        a = np.ascontiguousarray(a, np.float64)
+        d = cls(len(a), ctx)
+        _check(lib().gm_copy_to_device(d.ctx.h, _P(d.ptr), _p(a), 8 * d.n))
+        return d
+
+    def to_host(self):
+        out = np.empty(self.n)
+        _check(lib().gm_copy_to_host(self.ctx.h, _p(out), _P(self.ptr), 8 * self.n))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().gm_device_free(self.ctx.h, _P(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
         except Exception:
             pass
 

@@ -142,6 +142,14 @@ gm_status gm_set_positions(gm_sys *sys, const double *x, const double *y, const 
 gm_status gm_compute_forces_resident(gm_sys *sys);
 gm_status gm_get_forces(gm_sys *sys, double *fx, double *fy, double *fz);
 
+/* Device memory helpers for callers without their own HIP binding (e.g. an
+ * FFI crate): allocate / free / copy on the context's device.  Copies are
+ * synchronous. */
+gm_status gm_device_alloc(gm_ctx *ctx, size_t bytes, void **out);
+gm_status gm_device_free(gm_ctx *ctx, void *p);
+gm_status gm_copy_to_device(gm_ctx *ctx, void *dst, const void *src, size_t bytes);
+gm_status gm_copy_to_host(gm_ctx *ctx, void *dst, const void *src, size_t bytes);
+
 /* Statistics of the current list and the last evaluation. */
 typedef struct {
     int64_t n_atoms;
