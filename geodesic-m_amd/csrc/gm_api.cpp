@@ -74,9 +74,10 @@ gm_status make_work_items(gm_sys *s) {
     const int64_t total = s->nent;
     int64_t chunk = s->chunk;
     if (chunk <= 0) {
-        const int64_t target = 8LL * (s->ctx->n_cu > 0 ? s->ctx->n_cu : 256);
+        const int64_t target = 24LL * (s->ctx->n_cu > 0 ? s->ctx->n_cu : 256);
         chunk = std::max<int64_t>(4, (total + target - 1) / std::max<int64_t>(target, 1));
     }
+    chunk = std::min<int64_t>(chunk, kMaxRun);
     std::vector<int> wsc, wst, wen;
     for (int64_t k = 0; k < s->nsc; k++)
         for (int64_t e0 = start[(size_t)k]; e0 < start[(size_t)k + 1]; e0 += chunk) {
@@ -132,6 +133,9 @@ gm_status build_on_device(gm_sys *s, bool wrap) {
     GM_HIP(s->ent_cj.ensure((size_t)total));
     GM_HIP(s->ent_imask.ensure((size_t)total));
     GM_HIP(s->ent_mask.ensure((size_t)total * 8));
+    GM_HIP(s->ent_meta.ensure((size_t)total));
+    GM_HIP(s->ent_ja.ensure((size_t)total * 8));
+    GM_HIP(s->ent_lb.ensure((size_t)total * 64));
     GM_HIP(upload(s->sc_start, start.data(), start.size(), st));
 
     s->sc_start_host = start;
@@ -139,8 +143,11 @@ gm_status build_on_device(gm_sys *s, bool wrap) {
     if (rc != GM_OK) return rc;
 
     GM_HIP(launch_pair_fill_and_masks(s));
+    // the slot count changed: start the next evaluation from a zero accumulator
+    GM_HIP(launch_fill_u64(s->F.p, (size_t)s->max_sc * 64 * 3, st));
+    s->F_dirty = false;
     GM_HIP(hipEventRecord(s->ev_b1, st));
-    unsigned long long counters[2] = {0, 0};
+    unsigned long long counters[3] = {0, 0, 0};
     GM_HIP(hipMemcpyAsync(counters, s->meta.p + 8, sizeof(counters), hipMemcpyDeviceToHost, st));
     GM_HIP(hipStreamSynchronize(st));
     float ms = 0.f;
@@ -151,6 +158,7 @@ gm_status build_on_device(gm_sys *s, bool wrap) {
     s->stats.n_entries = s->nent;
     s->stats.n_list_pairs = (int64_t)counters[0];
     s->stats.n_tiles = (int64_t)counters[1];
+    s->stats.n_tiles_cutoff = (int64_t)counters[2];
     s->stats.n_work_items = s->n_wi;
     s->stats.ms_build = ms;
     s->built = true;
@@ -183,10 +191,11 @@ gm_status compute_on_device(gm_sys *s, const double *x, const double *y, const d
         }
     }
     GM_HIP(hipEventRecord(e0, st));
-    GM_HIP(launch_fill_u64(s->F.p, (size_t)s->nsc * 64 * 3, st));
-    GM_HIP(launch_nonbonded(s, x, y, z));
-    GM_HIP(launch_bonded(s, x, y, z));
-    GM_HIP(launch_finalize(s, fx, fy, fz));
+    if (s->F_dirty) GM_HIP(launch_fill_u64(s->F.p, (size_t)s->nsc * 64 * 3, st));
+    s->F_dirty = true;
+    GM_HIP(launch_forces(s, x, y, z));
+    GM_HIP(launch_finish(s, fx, fy, fz));  // also clears the accumulator
+    s->F_dirty = false;
     GM_HIP(hipEventRecord(e1, st));
     return GM_OK;
 }
@@ -354,6 +363,16 @@ gm_status gm_system_create(gm_ctx *ctx, const gm_system_desc *d, gm_sys **out) {
     s->nbp.rsw = (float)d->r_switch;
     s->nbp.inv_w = (float)(1.0 / (d->r_cutoff - d->r_switch));
     s->nbp.ds_coef = (float)(-30.0 / (d->r_cutoff - d->r_switch));
+    {
+        const double step = d->box[0] / two32;  // fixed-point step, A (cubic boxes)
+        s->nbp.inv_s = (float)(1.0 / step);
+        s->nbp.rc2_u = (float)((d->r_cutoff / step) * (d->r_cutoff / step));
+        s->nbp.rsw2_u = (float)((d->r_switch / step) * (d->r_switch / step));
+        s->nbp.rsw_u = (float)(d->r_switch / step);
+        s->nbp.inv_w_u = (float)(step / (d->r_cutoff - d->r_switch));
+        s->nbp.ds_coef_u = (float)(-30.0 * step / (d->r_cutoff - d->r_switch));
+        s->nbp.fscale_u = (float)(two32 / step);
+    }
 
     BuildParams &bp = s->bp;
     for (int a = 0; a < 3; a++) {
@@ -362,6 +381,7 @@ gm_status gm_system_create(gm_ctx *ctx, const gm_system_desc *d, gm_sys **out) {
         bp.sq[a] = (float)(d->box[a] / two32);
     }
     bp.r_skin2 = d->r_skin * d->r_skin;
+    bp.r_cut2 = d->r_cutoff * d->r_cutoff;
     bp.r_search = (float)(d->r_skin * (1.0 + 1e-5) + 1e-3);
     // xy columns about the width of a cube holding 64 atoms at the mean density
     const double vol = d->box[0] * d->box[1] * d->box[2];
@@ -476,6 +496,9 @@ gm_status gm_system_destroy(gm_sys *s) {
     s->cl_half.release();
     s->sc_half.release();
     s->ent_mask.release();
+    s->ent_meta.release();
+    s->ent_ja.release();
+    s->ent_lb.release();
     s->F.release();
     for (hipEvent_t ev : {s->ev_eval0, s->ev_nb0, s->ev_nb1, s->ev_eval1, s->ev_b0, s->ev_b1})
         if (ev) (void)hipEventDestroy(ev);

@@ -23,10 +23,7 @@ namespace {
 __device__ inline double wrap_coord(double x, double L) { return x - L * floor(x / L); }
 __device__ inline double min_image(double d, double L) { return d - L * round(d / L); }
 
-// floor(x * 2^32 / L) modulo 2^32: position as an unsigned fraction of the box.
-__device__ inline uint32_t quantize(double x, double inv_q) {
-    return (uint32_t)(int64_t)floor(x * inv_q);
-}
+__device__ inline uint32_t quantize(double x, double inv_q) { return (uint32_t)quantize_fixed(x, inv_q); }
 
 __global__ void k_fill_i32(int *p, size_t n, int v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -322,6 +319,7 @@ __device__ inline bool excluded(const int64_t *off, const uint32_t *nbr, uint32_
 // positions, with i = min(atom), j = max(atom).
 __global__ __launch_bounds__(64) void k_masks(int nent, const int *ent_sc, const int *ent_cj,
                                               uint32_t *ent_imask, unsigned long long *ent_mask,
+                                              int2 *ent_meta, int *ent_ja, uint8_t *ent_lb,
                                               const int *slot_atom, const double *x, const double *y,
                                               const double *z, const int64_t *excl_off,
                                               const uint32_t *excl_nbr, BuildParams bp,
@@ -333,14 +331,15 @@ __global__ __launch_bounds__(64) void k_masks(int nent, const int *ent_sc, const
     const int sc = ent_sc[e], cj = ent_cj[e];
     const uint32_t im = ent_imask[e];
     const int b = slot_atom[(size_t)cj * 8 + jl];
-    uint32_t newmask = 0;
-    unsigned long long npairs = 0;
+    uint32_t newmask = 0, lanebits = 0;
+    unsigned long long npairs = 0, ncut_tiles = 0;
+    if (il == 0) ent_ja[(size_t)e * 8 + jl] = b;
     for (int ci = 0; ci < 8; ci++) {
         unsigned long long m = 0ull;
         if (im & (1u << ci)) {
             const int a = slot_atom[(size_t)sc * 64 + ci * 8 + il];
             const int cg = sc * 8 + ci;
-            bool in = a >= 0 && b >= 0 && (cg < cj || il < jl);
+            bool in = a >= 0 && b >= 0 && (cg < cj || il < jl), cut = false;
             if (in) {
                 const uint32_t lo = (uint32_t)min(a, b), hi = (uint32_t)max(a, b);
                 in = !excluded(excl_off, excl_nbr, lo, hi);
@@ -350,17 +349,23 @@ __global__ __launch_bounds__(64) void k_masks(int nent, const int *ent_sc, const
                     double dz = min_image(z[hi] - z[lo], bp.box[2]);
                     double r2 = dx * dx + dy * dy + dz * dz;
                     in = r2 <= bp.r_skin2;
+                    cut = r2 <= bp.r_cut2;
                 }
             }
             m = __ballot(in);
+            if (__ballot(cut)) ncut_tiles++;
+            lanebits |= in ? (1u << ci) : 0u;
         }
         if (t == 0) ent_mask[(size_t)e * 8 + ci] = m;
         if (m) { newmask |= 1u << ci; npairs += __popcll(m); }
     }
+    ent_lb[(size_t)e * 64 + t] = (uint8_t)lanebits;
     if (t == 0) {
         ent_imask[e] = newmask;
+        ent_meta[e] = make_int2(cj, (int)newmask);
         atomicAdd(&counters[0], npairs);
         atomicAdd(&counters[1], (unsigned long long)__popc(newmask));
+        atomicAdd(&counters[2], ncut_tiles);
     }
 }
 
@@ -441,11 +446,12 @@ hipError_t launch_pair_fill_and_masks(gm_sys *s) {
         k_pair_search<1><<<(unsigned)s->nsc, 64, 0, st>>>(
         s->meta.p, (int)s->ncl, s->bp, s->cl_ref.p, s->cl_half.p, s->sc_ref.p, s->sc_half.p,
         nullptr, s->sc_start.p, s->ent_sc.p, s->ent_cj.p, s->ent_imask.p);
-    hipError_t e = launch_fill_u64((unsigned long long *)(s->meta.p + 8), 2, st);
+    hipError_t e = launch_fill_u64((unsigned long long *)(s->meta.p + 8), 3, st);
     if (e != hipSuccess) return e;
     if (s->nent > 0)
         k_masks<<<(unsigned)s->nent, 64, 0, st>>>((int)s->nent, s->ent_sc.p, s->ent_cj.p,
-                                                  s->ent_imask.p, s->ent_mask.p, s->slot_atom.p,
+                                                  s->ent_imask.p, s->ent_mask.p, s->ent_meta.p,
+                                                  s->ent_ja.p, s->ent_lb.p, s->slot_atom.p,
                                                   s->x.p, s->y.p, s->z.p, s->excl_off.p,
                                                   s->excl_nbr.p, s->bp,
                                                   (unsigned long long *)(s->meta.p + 8));

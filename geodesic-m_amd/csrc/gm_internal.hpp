@@ -20,6 +20,8 @@ constexpr int kClustersPerSc = 8;
 constexpr int kScSlots = 64;
 // Column sort capacity (atoms per xy column, sorted in LDS).
 constexpr int kColumnCap = 8192;
+// Longest run of list entries one nonbonded wavefront processes.
+constexpr int kMaxRun = 32;
 // Fixed-point force unit: 2^-32 kcal/mol/A per integer step.
 constexpr double kFixedScale = 4294967296.0;
 constexpr double kFixedInv = 1.0 / 4294967296.0;
@@ -32,6 +34,10 @@ struct NbParams {
     float rsw;          // r_switch
     float inv_w;        // 1 / (r_cutoff - r_switch)
     float ds_coef;      // -30 / (r_cutoff - r_switch)
+    // the same constants with lengths in fixed-point steps (cubic boxes only)
+    float inv_s;        // 2^32 / box_len
+    float rc2_u, rsw2_u, rsw_u, inv_w_u, ds_coef_u;
+    float fscale_u;     // force (kcal/mol per step) -> 2^-32 kcal/mol/A units
 };
 
 // Per-system constants for the list build.
@@ -40,12 +46,22 @@ struct BuildParams {
     double inv_q[3];    // 2^32 / box_len (quantisation)
     float sq[3];        // box_len / 2^32
     double r_skin2;     // r_skin^2 (f64, decision of neighbor.rs:38)
+    double r_cut2;      // r_cutoff^2 (statistics only)
     float r_search;     // r_skin + margin, bounding-box search radius
     int ncx, ncy, ncol;
     int periodic;
     int n_atoms;
     int max_sc;
 };
+
+// floor(x * 2^32 / L) mod 2^32, as a 32-bit pattern: the position as an
+// unsigned fraction of the box.  t is an integer-valued double, and
+// t - 2^32 floor(t / 2^32) is exact and lies in [0, 2^32) for |t| < 2^53.
+__device__ inline int quantize_fixed(double x, double inv_q) {
+    const double t = floor(x * inv_q);
+    const double r = fma(-4294967296.0, floor(t * 2.3283064365386963e-10), t);
+    return (int)(unsigned)r;
+}
 
 }  // namespace gm
 
@@ -104,7 +120,7 @@ struct gm_sys {
     DevBuf<int> sc_base, sc_m;
     DevBuf<uint32_t> qatom;  // 3*n fixed-point coordinates, atom order
     DevBuf<int> meta;        // [0] nsc, [1] overflow, [2] max column, [5] rebuild flag;
-                             // ints 8..11 hold two u64 counters (list pairs, tiles)
+                             // ints 8..13 hold u64 counters (list pairs, tiles, tiles with a pair within r_cutoff)
     int64_t max_sc = 0;
 
     // slots
@@ -120,7 +136,11 @@ struct gm_sys {
     DevBuf<int> sc_count, sc_start;
     DevBuf<int> ent_sc, ent_cj;
     DevBuf<uint32_t> ent_imask;
-    DevBuf<unsigned long long> ent_mask;
+    DevBuf<unsigned long long> ent_mask;  // per entry 8 x 64-bit pair masks (export)
+    DevBuf<int2> ent_meta;                // per entry (j cluster, i-cluster mask)
+    DevBuf<int> ent_ja;                   // per entry the j cluster's 8 atom indices (-1 padding)
+    DevBuf<uint8_t> ent_lb;               // per entry, per lane: mask bits over the 8 i clusters
+    bool F_dirty = true;                  // accumulator not known to be zero
     DevBuf<int> wi_sc, wi_start, wi_end;
     int chunk = 0;  // entries per work item (0 = automatic)
 
@@ -161,11 +181,10 @@ hipError_t launch_build(gm_sys *s, bool wrap);
 hipError_t launch_pair_count(gm_sys *s);
 hipError_t launch_pair_fill_and_masks(gm_sys *s);
 hipError_t launch_needs_rebuild(gm_sys *s, const double *x, const double *y, const double *z);
-// gm_nonbonded.hip
-hipError_t launch_nonbonded(gm_sys *s, const double *x, const double *y, const double *z);
-// gm_bonded.hip
-hipError_t launch_bonded(gm_sys *s, const double *x, const double *y, const double *z);
-hipError_t launch_finalize(gm_sys *s, double *fx, double *fy, double *fz);
+// gm_nonbonded.hip: tiles + bonded terms in one launch
+hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z);
+// gm_bonded.hip: energy reduction, f64 output, accumulator clear
+hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz);
 hipError_t launch_fill_u64(unsigned long long *p, size_t n, hipStream_t st);
 hipError_t launch_fill_i32(int *p, size_t n, int v, hipStream_t st);
 }  // namespace gm

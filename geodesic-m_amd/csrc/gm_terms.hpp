@@ -1,0 +1,205 @@
+// Device functions for one bonded term each: bond, angle, dihedral, scaled
+// 1-4 LJ.  f64 restatement of force/bonded.rs (paths relative to the reference
+// repository root) in the reference's operation order; every function body
+// disables FMA contraction so the arithmetic is the CPU path's whatever flags
+// the including file uses.  Forces go to the int64 fixed-point accumulator
+// (2^-32 kcal/mol/A) in slot order with integer atomics.
+#pragma once
+
+#include "gm_internal.hpp"
+
+namespace gm {
+
+__device__ inline double term_min_image(double d, double L) {
+#pragma clang fp contract(off)
+    return d - L * round(d / L);
+}
+
+__device__ inline double term_diff(double a, double b, int mi, double L) {
+#pragma clang fp contract(off)
+    double d = a - b;
+    return mi ? term_min_image(d, L) : d;
+}
+
+__device__ inline void fixed_add_d(unsigned long long *p, double v) {
+    const long long q = __double2ll_rn(v * 4294967296.0);
+    __hip_atomic_fetch_add(p, (unsigned long long)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Everything a bonded term reads: positions (atom order, f64), topology,
+// the slot map and the accumulator.
+struct TermArgs {
+    const double *x, *y, *z;
+    double box[3];
+    int mi;  // min-image bonded displacements
+    unsigned long long *F;
+    const int *atom_slot;
+    size_t n_pad;
+    const uint32_t *bi, *bj;
+    const double *bk, *br0;
+    const uint32_t *ai, *aj, *ak;
+    const double *akth, *ath0;
+    const uint32_t *di, *dj, *dk, *dl, *dn;
+    const double *dkphi, *ddelta;
+    const uint32_t *p14i, *p14j;
+    const double *sigma, *epsilon;
+    double scale14;
+    int64_t nb, na, nd, n14;  // n14 = 0 when scale14 == 0
+    double *e_items;           // nb + na + nd + n14 energies
+};
+
+__device__ inline void term_add(const TermArgs &t, uint32_t atom, double fx, double fy, double fz) {
+    const size_t s = (size_t)t.atom_slot[atom];
+    fixed_add_d(&t.F[s], fx);
+    fixed_add_d(&t.F[t.n_pad + s], fy);
+    fixed_add_d(&t.F[2 * t.n_pad + s], fz);
+}
+
+// force/bonded.rs:5-40
+__device__ inline double term_bond(const TermArgs &t, int64_t n) {
+#pragma clang fp contract(off)
+    const uint32_t i = t.bi[n], j = t.bj[n];
+    const double k = t.bk[n], r0 = t.br0[n];
+    const double dx = term_diff(t.x[j], t.x[i], t.mi, t.box[0]);
+    const double dy = term_diff(t.y[j], t.y[i], t.mi, t.box[1]);
+    const double dz = term_diff(t.z[j], t.z[i], t.mi, t.box[2]);
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    if (r == 0.0) return 0.0;
+    const double dv_dr = 2.0 * k * (r - r0);
+    const double e = k * (r - r0) * (r - r0);
+    const double ux = dx / r, uy = dy / r, uz = dz / r;
+    term_add(t, i, dv_dr * ux, dv_dr * uy, dv_dr * uz);
+    term_add(t, j, -(dv_dr * ux), -(dv_dr * uy), -(dv_dr * uz));
+    return e;
+}
+
+// force/bonded.rs:44-102
+__device__ inline double term_angle(const TermArgs &t, int64_t n) {
+#pragma clang fp contract(off)
+    const uint32_t i = t.ai[n], j = t.aj[n], kk = t.ak[n];
+    const double k_const = t.akth[n], theta0 = t.ath0[n];
+    const double b1[3] = {term_diff(t.x[i], t.x[j], t.mi, t.box[0]), term_diff(t.y[i], t.y[j], t.mi, t.box[1]),
+                          term_diff(t.z[i], t.z[j], t.mi, t.box[2])};
+    const double b2[3] = {term_diff(t.x[kk], t.x[j], t.mi, t.box[0]), term_diff(t.y[kk], t.y[j], t.mi, t.box[1]),
+                          term_diff(t.z[kk], t.z[j], t.mi, t.box[2])};
+    const double r1 = sqrt(b1[0] * b1[0] + b1[1] * b1[1] + b1[2] * b1[2]);
+    const double r2 = sqrt(b2[0] * b2[0] + b2[1] * b2[1] + b2[2] * b2[2]);
+    if (r1 == 0.0 || r2 == 0.0) return 0.0;
+    const double dot = b1[0] * b2[0] + b1[1] * b2[1] + b1[2] * b2[2];
+    double cos_theta = dot / (r1 * r2);
+    cos_theta = cos_theta < -1.0 ? -1.0 : (cos_theta > 1.0 ? 1.0 : cos_theta);
+    double sin_theta = sqrt(1.0 - cos_theta * cos_theta);
+    sin_theta = sin_theta < 1e-8 ? 1e-8 : sin_theta;
+    const double theta = acos(cos_theta);
+    const double dv_dtheta = 2.0 * k_const * (theta - theta0);
+    const double e = k_const * (theta - theta0) * (theta - theta0);
+    const double a = dv_dtheta / sin_theta;
+    double f_i[3], f_k[3];
+    for (int c = 0; c < 3; c++) {
+        f_i[c] = a * (b2[c] / (r1 * r2) - cos_theta * b1[c] / (r1 * r1));
+        f_k[c] = a * (b1[c] / (r1 * r2) - cos_theta * b2[c] / (r2 * r2));
+    }
+    term_add(t, i, f_i[0], f_i[1], f_i[2]);
+    term_add(t, kk, f_k[0], f_k[1], f_k[2]);
+    term_add(t, j, -(f_i[0] + f_k[0]), -(f_i[1] + f_k[1]), -(f_i[2] + f_k[2]));
+    return e;
+}
+
+__device__ inline void term_cross(const double *a, const double *b, double *o) {
+#pragma clang fp contract(off)
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ inline double term_dot(const double *a, const double *b) {
+#pragma clang fp contract(off)
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// force/bonded.rs:112-164 (Blondel-Karplus)
+__device__ inline double term_dihedral(const TermArgs &t, int64_t n) {
+#pragma clang fp contract(off)
+    const uint32_t i = t.di[n], j = t.dj[n], k = t.dk[n], l = t.dl[n];
+    const double k_phi = t.dkphi[n], mult = (double)t.dn[n], delta = t.ddelta[n];
+    const double b1[3] = {term_diff(t.x[j], t.x[i], t.mi, t.box[0]), term_diff(t.y[j], t.y[i], t.mi, t.box[1]),
+                          term_diff(t.z[j], t.z[i], t.mi, t.box[2])};
+    const double b2[3] = {term_diff(t.x[k], t.x[j], t.mi, t.box[0]), term_diff(t.y[k], t.y[j], t.mi, t.box[1]),
+                          term_diff(t.z[k], t.z[j], t.mi, t.box[2])};
+    const double b3[3] = {term_diff(t.x[l], t.x[k], t.mi, t.box[0]), term_diff(t.y[l], t.y[k], t.mi, t.box[1]),
+                          term_diff(t.z[l], t.z[k], t.mi, t.box[2])};
+    double m[3], nv[3], mn[3];
+    term_cross(b1, b2, m);
+    term_cross(b2, b3, nv);
+    const double m_len2 = term_dot(m, m);
+    const double n_len2 = term_dot(nv, nv);
+    const double b2_len = sqrt(term_dot(b2, b2));
+    if (m_len2 == 0.0 || n_len2 == 0.0 || b2_len == 0.0) return 0.0;
+    const double xx = term_dot(m, nv);
+    term_cross(m, nv, mn);
+    const double yy = term_dot(mn, b2) / b2_len;
+    const double phi = atan2(yy, xx);
+    const double e = k_phi * (1.0 + cos(mult * phi - delta));
+    const double dv_dphi = k_phi * mult * sin(mult * phi - delta);
+    const double si = -dv_dphi * b2_len / m_len2;
+    const double sl = dv_dphi * b2_len / n_len2;
+    const double f_i[3] = {m[0] * si, m[1] * si, m[2] * si};
+    const double f_l[3] = {nv[0] * sl, nv[1] * sl, nv[2] * sl};
+    const double p = term_dot(b1, b2) / (b2_len * b2_len);
+    const double q = term_dot(b3, b2) / (b2_len * b2_len);
+    const double cj = -(p + 1.0), ck = q + 1.0;
+    double f_j[3], f_k[3];
+    for (int c = 0; c < 3; c++) {
+        f_j[c] = f_i[c] * cj + f_l[c] * q;
+        f_k[c] = f_i[c] * p - f_l[c] * ck;
+    }
+    term_add(t, i, f_i[0], f_i[1], f_i[2]);
+    term_add(t, j, f_j[0], f_j[1], f_j[2]);
+    term_add(t, k, f_k[0], f_k[1], f_k[2]);
+    term_add(t, l, f_l[0], f_l[1], f_l[2]);
+    return e;
+}
+
+// Scaled 1-4 LJ: scale * unswitched LJ, no cutoff, min-image displacement,
+// sign convention of force/nonbonded.rs:68-78.  No reference counterpart
+// (prmtop.rs:132-137 excludes 1-4 pairs entirely).
+__device__ inline double term_lj14(const TermArgs &t, int64_t n) {
+#pragma clang fp contract(off)
+    const uint32_t i = t.p14i[n], j = t.p14j[n];
+    const double dx = term_min_image(t.x[j] - t.x[i], t.box[0]);
+    const double dy = term_min_image(t.y[j] - t.y[i], t.box[1]);
+    const double dz = term_min_image(t.z[j] - t.z[i], t.box[2]);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double sig = 0.5 * (t.sigma[i] + t.sigma[j]);
+    const double eps = sqrt(t.epsilon[i] * t.epsilon[j]);
+    if (r2 == 0.0 || eps == 0.0) return 0.0;
+    const double r = sqrt(r2);
+    const double s = sig / r;
+    const double s2 = s * s;
+    const double sr6 = s2 * s2 * s2;
+    const double sr12 = sr6 * sr6;
+    const double e = t.scale14 * (4.0 * eps * (sr12 - sr6));
+    const double f_radial = t.scale14 * (24.0 * eps / r * (2.0 * sr12 - sr6));
+    const double fx = f_radial * dx / r, fy = f_radial * dy / r, fz = f_radial * dz / r;
+    term_add(t, i, -fx, -fy, -fz);
+    term_add(t, j, fx, fy, fz);
+    return e;
+}
+
+// Term k of the flat range [bonds | angles | dihedrals | 1-4]; writes its energy.
+__device__ inline void term_dispatch(const TermArgs &t, int64_t k) {
+    double e;
+    int64_t n = k;
+    if (n < t.nb) {
+        e = term_bond(t, n);
+    } else if ((n -= t.nb) < t.na) {
+        e = term_angle(t, n);
+    } else if ((n -= t.na) < t.nd) {
+        e = term_dihedral(t, n);
+    } else {
+        n -= t.nd;
+        e = term_lj14(t, n);
+    }
+    t.e_items[k] = e;
+}
+
+}  // namespace gm

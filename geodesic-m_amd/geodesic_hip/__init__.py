@@ -18,7 +18,8 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libgeodesic_hip.so")
+LIB_PATH = os.environ.get("GEODESIC_HIP_LIB") or os.path.join(os.path.dirname(_HERE), "lib",
+                                                                "libgeodesic_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "geodesic_hip.h")
 
 GM_OK = 0
@@ -74,7 +75,7 @@ class Profile(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("n_atoms", _I64), ("n_superclusters", _I64), ("n_clusters", _I64),
-                ("n_entries", _I64), ("n_tiles", _I64), ("n_list_pairs", _I64),
+                ("n_entries", _I64), ("n_tiles", _I64), ("n_tiles_cutoff", _I64), ("n_list_pairs", _I64),
                 ("n_work_items", _I64), ("ms_nonbonded", C.c_float), ("ms_eval", C.c_float),
                 ("ms_build", C.c_float)]
 

@@ -157,6 +157,7 @@ typedef struct {
     int64_t n_clusters;        /* 8-slot groups */
     int64_t n_entries;         /* (supercluster, j-cluster) list entries */
     int64_t n_tiles;           /* (i-cluster, j-cluster) 8x8 tiles with a nonzero mask */
+    int64_t n_tiles_cutoff;    /* of those, tiles with a pair within r_cutoff at build time */
     int64_t n_list_pairs;      /* pairs in the Verlet list */
     int64_t n_work_items;      /* waves launched by the nonbonded kernel */
     float ms_nonbonded;        /* last nonbonded kernel, hipEvent timing */
@@ -178,8 +179,8 @@ typedef struct {
 gm_status gm_profile_begin(gm_sys *sys);
 gm_status gm_profile_end(gm_sys *sys, gm_profile *out);
 
-/* Tuning: list entries per nonbonded work item (one wavefront each); 0 picks
- * a size that gives about 8 work items per compute unit.  Takes effect
+/* Tuning: list entries per nonbonded work item (one wavefront each, at most
+ * 32); 0 picks a size that gives about 24 work items per compute unit.  Takes effect
  * immediately on a built list (no rebuild). */
 gm_status gm_set_work_chunk(gm_sys *sys, int32_t entries_per_item);
 
