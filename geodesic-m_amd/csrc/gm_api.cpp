@@ -52,7 +52,7 @@ using namespace gm;
         if (!(cond)) return set_error(GM_ERR_INVALID_ARG, msg); \
     } while (0)
 
-namespace {
+namespace gm {
 
 template <typename T>
 hipError_t upload(DevBuf<T> &b, const T *src, size_t n, hipStream_t st) {
@@ -226,7 +226,7 @@ gm_status collect(gm_sys *s, gm_energy_terms *e) {
     return GM_OK;
 }
 
-}  // namespace
+}  // namespace gm
 
 extern "C" {
 
@@ -292,6 +292,13 @@ gm_status gm_ctx_destroy(gm_ctx *ctx) {
 }
 
 gm_status gm_system_create(gm_ctx *ctx, const gm_system_desc *d, gm_sys **out) {
+    return gm::system_create_impl(ctx, d, nullptr, out);
+}
+
+}  // extern "C"
+
+gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream_t ext_stream,
+                                 gm_sys **out) {
     GM_REQUIRE(ctx && d && out, "gm_system_create: NULL argument");
     *out = nullptr;
     const int64_t n = d->n_atoms;
@@ -336,7 +343,12 @@ gm_status gm_system_create(gm_ctx *ctx, const gm_system_desc *d, gm_sys **out) {
         hipError_t _e = (call);                           \
         if (_e != hipSuccess) return fail(from_hip(_e, #call)); \
     } while (0)
-    TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    if (ext_stream) {
+        s->stream = ext_stream;
+        s->owns_stream = false;
+    } else {
+        TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    }
     for (hipEvent_t *ev : {&s->ev_eval0, &s->ev_nb0, &s->ev_nb1, &s->ev_eval1, &s->ev_b0, &s->ev_b1})
         TRY(hipEventCreate(ev));
     hipStream_t st = s->stream;
@@ -471,6 +483,8 @@ gm_status gm_system_create(gm_ctx *ctx, const gm_system_desc *d, gm_sys **out) {
     return GM_OK;
 }
 
+extern "C" {
+
 gm_status gm_system_destroy(gm_sys *s) {
     if (!s) return GM_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
@@ -503,7 +517,7 @@ gm_status gm_system_destroy(gm_sys *s) {
     for (hipEvent_t ev : {s->ev_eval0, s->ev_nb0, s->ev_nb1, s->ev_eval1, s->ev_b0, s->ev_b1})
         if (ev) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : s->prof_pool) (void)hipEventDestroy(ev);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->stream && s->owns_stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return GM_OK;
 }

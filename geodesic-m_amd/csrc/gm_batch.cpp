@@ -1,0 +1,345 @@
+// Batched replicas: many independent systems evaluated by one persistent
+// launch, replayed from a captured hipGraph.  No reference counterpart (the
+// reference evaluates one system per CpuBackend / GpuBackend).
+//
+// Each replica is an ordinary gm_sys (own box, topology, Verlet list) sharing
+// the batch's stream.  After the lists are built, a device table of replica
+// descriptors and two item lists (replica, bonded group) and (replica, tile
+// work item) are written.  k_batch_bonded runs the bonded terms on a side
+// branch while the persistent k_batch_tiles drains the tile queue with a
+// fixed grid of wavefronts; k_batch_finish then reduces per-replica f64
+// energies, writes the concatenated f64 forces and re-arms the queue.  The
+// three kernels are captured once per list build into a hipGraph, so a
+// batched evaluation is one graph launch.  Every item runs the single-system arithmetic, so each replica's
+// forces and energies are bit-identical to evaluating it alone with the same
+// work shape.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "gm_internal.hpp"
+#include "gm_terms.hpp"
+
+using namespace gm;
+
+struct gm_batch {
+    gm_ctx *ctx = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<gm_sys *> reps;
+    std::vector<int64_t> off;  // replica atom offsets, size R+1
+    DevBuf<ReplicaDesc> d_reps;
+    DevBuf<int2> d_items;  // bonded items, then tile items
+    int n_bonded_items = 0;
+    hipStream_t side = nullptr;  // second branch of the captured graph
+    hipEvent_t fork = nullptr, join = nullptr;
+    DevBuf<unsigned int> ticket;
+    DevBuf<int> atom_rep;
+    DevBuf<int64_t> atom_off;
+    DevBuf<double> fx, fy, fz, E;
+    int n_items = 0;
+    int chunk = 0;         // entries per tile work item (0 = automatic)
+    int waves_per_cu = 16; // persistent wavefronts per compute unit
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool built = false;
+};
+
+#define GM_REQUIRE(cond, msg)                                   \
+    do {                                                        \
+        if (!(cond)) return set_error(GM_ERR_INVALID_ARG, msg); \
+    } while (0)
+
+namespace {
+
+void drop_graph(gm_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// Descriptor table, item queue and graph for the current lists and shape.
+gm_status make_plan(gm_batch *b) {
+    hipStream_t st = b->stream;
+    const int R = (int)b->reps.size();
+    std::vector<ReplicaDesc> desc((size_t)R);
+    std::vector<int2> bonded_items, tile_items;
+    for (int r = 0; r < R; r++) {
+        gm_sys *s = b->reps[(size_t)r];
+        ReplicaDesc &d = desc[(size_t)r];
+        d.tile = tile_args(s, s->x.p, s->y.p, s->z.p);
+        d.terms = term_args(s, s->x.p, s->y.p, s->z.p);
+        d.n_bonded_groups = (int)((d.terms.nb + d.terms.na + d.terms.nd + d.terms.n14 + 63) / 64);
+        d.n_wi = (int)s->n_wi;
+        for (int g = 0; g < d.n_bonded_groups; g++) bonded_items.push_back(make_int2(r, g));
+        for (int w = 0; w < d.n_wi; w++) tile_items.push_back(make_int2(r, w));
+    }
+    std::vector<int2> items(bonded_items);
+    items.insert(items.end(), tile_items.begin(), tile_items.end());
+    b->n_bonded_items = (int)bonded_items.size();
+    b->n_items = (int)tile_items.size();
+    GM_HIP(b->d_reps.ensure((size_t)R));
+    GM_HIP(hipMemcpyAsync(b->d_reps.p, desc.data(), sizeof(ReplicaDesc) * (size_t)R, hipMemcpyHostToDevice, st));
+    GM_HIP(b->d_items.ensure(items.size()));
+    if (!items.empty())
+        GM_HIP(hipMemcpyAsync(b->d_items.p, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice, st));
+    GM_HIP(hipMemsetAsync(b->ticket.p, 0, sizeof(unsigned int), st));
+    GM_HIP(hipStreamSynchronize(st));
+
+    drop_graph(b);
+    const int grid = std::max(1, b->waves_per_cu * (b->ctx->n_cu > 0 ? b->ctx->n_cu : 256));
+    // graph: fork -> {bonded terms on the side branch | persistent tiles} -> join -> finish
+    GM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t e1 = hipEventRecord(b->fork, st);
+    if (e1 == hipSuccess) e1 = hipStreamWaitEvent(b->side, b->fork, 0);
+    if (e1 == hipSuccess) e1 = launch_batch_bonded(b->d_reps.p, b->d_items.p, b->n_bonded_items, b->side);
+    if (e1 == hipSuccess) e1 = hipEventRecord(b->join, b->side);
+    if (e1 == hipSuccess)
+        e1 = launch_batch_tiles(b->d_reps.p, b->d_items.p + b->n_bonded_items, b->n_items, b->ticket.p, grid, st);
+    if (e1 == hipSuccess) e1 = hipStreamWaitEvent(st, b->join, 0);
+    if (e1 == hipSuccess)
+        e1 = launch_batch_finish(b->d_reps.p, R, b->atom_rep.p, b->atom_off.p, b->off.back(), b->E.p,
+                                 b->fx.p, b->fy.p, b->fz.p, b->ticket.p, st);
+    hipGraph_t g = nullptr;
+    hipError_t e3 = hipStreamEndCapture(st, &g);
+    if (e1 != hipSuccess) { if (g) (void)hipGraphDestroy(g); return from_hip(e1, "batched graph capture"); }
+    GM_HIP(e3);
+    b->graph = g;
+    GM_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return GM_OK;
+}
+
+gm_status apply_chunk(gm_batch *b) {
+    for (gm_sys *s : b->reps) {
+        s->chunk = b->chunk;
+        if (s->built) {
+            gm_status rc = make_work_items(s);
+            if (rc != GM_OK) return rc;
+        }
+    }
+    return GM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+gm_status gm_batch_create(gm_ctx *ctx, const gm_system_desc *descs, int32_t n_replicas, gm_batch **out) {
+    GM_REQUIRE(ctx && descs && out && n_replicas > 0, "gm_batch_create: bad argument");
+    for (int r = 0; r < n_replicas; r++)
+        GM_REQUIRE(descs[r].box[0] == descs[r].box[1] && descs[r].box[1] == descs[r].box[2],
+                   "gm_batch_create: batched replicas need cubic boxes");
+    *out = nullptr;
+    GM_HIP(hipSetDevice(ctx->device));
+    gm_batch *b = new gm_batch();
+    b->ctx = ctx;
+    auto fail = [&](gm_status st) { gm_batch_destroy(b); return st; };
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(from_hip(e, "hipStreamCreate"));
+    if ((e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking)) != hipSuccess)
+        return fail(from_hip(e, "hipStreamCreate"));
+    if ((e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming)) != hipSuccess)
+        return fail(from_hip(e, "hipEventCreate"));
+    b->off.push_back(0);
+    for (int r = 0; r < n_replicas; r++) {
+        gm_sys *s = nullptr;
+        gm_status rc = system_create_impl(ctx, &descs[r], b->stream, &s);
+        if (rc != GM_OK) return fail(rc);
+        b->reps.push_back(s);
+        b->off.push_back(b->off.back() + descs[r].n_atoms);
+    }
+    const int64_t n = b->off.back();
+    std::vector<int> rep((size_t)n);
+    for (int r = 0; r < n_replicas; r++)
+        for (int64_t a = b->off[(size_t)r]; a < b->off[(size_t)r + 1]; a++) rep[(size_t)a] = r;
+    if ((e = b->atom_rep.ensure((size_t)std::max<int64_t>(n, 1))) != hipSuccess ||
+        (e = b->atom_off.ensure(b->off.size())) != hipSuccess || (e = b->ticket.ensure(1)) != hipSuccess ||
+        (e = b->E.ensure((size_t)n_replicas * 5)) != hipSuccess)
+        return fail(from_hip(e, "hipMalloc"));
+    for (DevBuf<double> *d : {&b->fx, &b->fy, &b->fz})
+        if ((e = d->ensure((size_t)std::max<int64_t>(n, 1))) != hipSuccess) return fail(from_hip(e, "hipMalloc"));
+    if (n > 0 && (e = hipMemcpy(b->atom_rep.p, rep.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(from_hip(e, "hipMemcpy"));
+    if ((e = hipMemcpy(b->atom_off.p, b->off.data(), sizeof(int64_t) * b->off.size(), hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(from_hip(e, "hipMemcpy"));
+    if ((e = hipMemset(b->ticket.p, 0, sizeof(unsigned int))) != hipSuccess) return fail(from_hip(e, "hipMemset"));
+    *out = b;
+    return GM_OK;
+}
+
+gm_status gm_batch_destroy(gm_batch *b) {
+    if (!b) return GM_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (gm_sys *s : b->reps) gm_system_destroy(s);
+    b->d_reps.release();
+    b->d_items.release();
+    b->ticket.release();
+    b->atom_rep.release();
+    b->atom_off.release();
+    b->fx.release();
+    b->fy.release();
+    b->fz.release();
+    b->E.release();
+    for (hipEvent_t ev : {b->ev0, b->ev1, b->fork, b->join})
+        if (ev) (void)hipEventDestroy(ev);
+    if (b->side) (void)hipStreamDestroy(b->side);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return GM_OK;
+}
+
+gm_status gm_batch_replica(gm_batch *b, int32_t r, gm_sys **out) {
+    GM_REQUIRE(b && out && r >= 0 && r < (int32_t)b->reps.size(), "gm_batch_replica: bad argument");
+    *out = b->reps[(size_t)r];
+    return GM_OK;
+}
+
+gm_status gm_batch_build_neighbor_lists(gm_batch *b, double *x, double *y, double *z) {
+    GM_REQUIRE(b, "NULL batch");
+    const int64_t n = b->off.back();
+    GM_REQUIRE(n == 0 || (x && y && z), "NULL positions");
+    hipStream_t st = b->stream;
+    b->built = false;
+    for (size_t r = 0; r < b->reps.size(); r++) {
+        gm_sys *s = b->reps[r];
+        s->chunk = b->chunk;
+        const int64_t o = b->off[r];
+        const size_t bytes = sizeof(double) * (size_t)s->n;
+        if (s->n > 0) {
+            GM_HIP(hipMemcpyAsync(s->x.p, x + o, bytes, hipMemcpyHostToDevice, st));
+            GM_HIP(hipMemcpyAsync(s->y.p, y + o, bytes, hipMemcpyHostToDevice, st));
+            GM_HIP(hipMemcpyAsync(s->z.p, z + o, bytes, hipMemcpyHostToDevice, st));
+        }
+        gm_status rc = build_on_device(s, s->periodic != 0);
+        if (rc != GM_OK) return rc;
+        if (s->periodic && s->n > 0) {
+            GM_HIP(hipMemcpyAsync(x + o, s->x.p, bytes, hipMemcpyDeviceToHost, st));
+            GM_HIP(hipMemcpyAsync(y + o, s->y.p, bytes, hipMemcpyDeviceToHost, st));
+            GM_HIP(hipMemcpyAsync(z + o, s->z.p, bytes, hipMemcpyDeviceToHost, st));
+        }
+    }
+    GM_HIP(hipStreamSynchronize(st));
+    gm_status rc = make_plan(b);
+    if (rc != GM_OK) return rc;
+    b->built = true;
+    return GM_OK;
+}
+
+gm_status gm_batch_set_positions(gm_batch *b, const double *x, const double *y, const double *z) {
+    GM_REQUIRE(b, "NULL batch");
+    GM_REQUIRE(b->off.back() == 0 || (x && y && z), "NULL positions");
+    for (size_t r = 0; r < b->reps.size(); r++) {
+        gm_sys *s = b->reps[r];
+        const int64_t o = b->off[r];
+        const size_t bytes = sizeof(double) * (size_t)s->n;
+        if (s->n == 0) continue;
+        GM_HIP(hipMemcpyAsync(s->x.p, x + o, bytes, hipMemcpyHostToDevice, b->stream));
+        GM_HIP(hipMemcpyAsync(s->y.p, y + o, bytes, hipMemcpyHostToDevice, b->stream));
+        GM_HIP(hipMemcpyAsync(s->z.p, z + o, bytes, hipMemcpyHostToDevice, b->stream));
+    }
+    GM_HIP(hipStreamSynchronize(b->stream));
+    return GM_OK;
+}
+
+gm_status gm_batch_compute(gm_batch *b) {
+    GM_REQUIRE(b, "NULL batch");
+    if (!b->built) return set_error(GM_ERR_NOT_BUILT, "gm_batch_compute before gm_batch_build_neighbor_lists");
+    GM_HIP(hipEventRecord(b->ev0, b->stream));
+    GM_HIP(hipGraphLaunch(b->exec, b->stream));
+    GM_HIP(hipEventRecord(b->ev1, b->stream));
+    return GM_OK;
+}
+
+gm_status gm_batch_sync(gm_batch *b, gm_energy_terms *e, float *ms) {
+    GM_REQUIRE(b, "NULL batch");
+    GM_HIP(hipStreamSynchronize(b->stream));
+    if (ms) {
+        float t = 0.f;
+        GM_HIP(hipEventElapsedTime(&t, b->ev0, b->ev1));
+        *ms = t;
+    }
+    if (e) {
+        const size_t R = b->reps.size();
+        std::vector<double> E(R * 5);
+        GM_HIP(hipMemcpy(E.data(), b->E.p, sizeof(double) * R * 5, hipMemcpyDeviceToHost));
+        for (size_t r = 0; r < R; r++) {
+            const double *t = &E[r * 5];
+            e[r].bond = t[0];
+            e[r].angle = t[1];
+            e[r].dihedral = t[2];
+            e[r].lj14 = t[3];
+            e[r].lj = t[4];
+            double tot = 0.0;
+            tot += t[0];
+            tot += t[1];
+            tot += t[2];
+            tot += t[3];
+            tot += t[4];
+            e[r].total = tot;
+        }
+    }
+    return GM_OK;
+}
+
+gm_status gm_batch_get_forces(gm_batch *b, double *fx, double *fy, double *fz) {
+    GM_REQUIRE(b, "NULL batch");
+    const int64_t n = b->off.back();
+    GM_REQUIRE(n == 0 || (fx && fy && fz), "NULL array");
+    GM_HIP(hipStreamSynchronize(b->stream));
+    if (n > 0) {
+        GM_HIP(hipMemcpy(fx, b->fx.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+        GM_HIP(hipMemcpy(fy, b->fy.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+        GM_HIP(hipMemcpy(fz, b->fz.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+    }
+    return GM_OK;
+}
+
+gm_status gm_batch_set_shape(gm_batch *b, int32_t entries_per_item, int32_t waves_per_cu) {
+    GM_REQUIRE(b && entries_per_item >= 0 && entries_per_item <= kMaxRun && waves_per_cu >= 1 &&
+                   waves_per_cu <= 64, "gm_batch_set_shape: bad argument");
+    b->chunk = entries_per_item;
+    b->waves_per_cu = waves_per_cu;
+    gm_status rc = apply_chunk(b);
+    if (rc != GM_OK) return rc;
+    if (b->built) return make_plan(b);
+    return GM_OK;
+}
+
+gm_status gm_batch_autotune(gm_batch *b, int32_t *entries_per_item, int32_t *waves_per_cu, float *best_ms) {
+    GM_REQUIRE(b, "NULL batch");
+    if (!b->built) return set_error(GM_ERR_NOT_BUILT, "gm_batch_autotune before gm_batch_build_neighbor_lists");
+    static const int chunks[] = {4, 6, 8, 12, 16};
+    static const int waves[] = {8, 16, 24};
+    float best = 1e30f;
+    int bc = b->chunk, bw = b->waves_per_cu;
+    for (int c : chunks) {
+        for (int w : waves) {
+            gm_status rc = gm_batch_set_shape(b, c, w);
+            if (rc != GM_OK) return rc;
+            float total = 0.f;
+            for (int it = 0; it < 8; it++) {
+                GM_HIP(hipEventRecord(b->ev0, b->stream));
+                GM_HIP(hipGraphLaunch(b->exec, b->stream));
+                GM_HIP(hipEventRecord(b->ev1, b->stream));
+                GM_HIP(hipEventSynchronize(b->ev1));
+                float t = 0.f;
+                GM_HIP(hipEventElapsedTime(&t, b->ev0, b->ev1));
+                if (it >= 2) total += t;  // two warm-up replays
+            }
+            const float avg = total / 6.0f;
+            if (avg < best) { best = avg; bc = c; bw = w; }
+        }
+    }
+    gm_status rc = gm_batch_set_shape(b, bc, bw);
+    if (rc != GM_OK) return rc;
+    if (entries_per_item) *entries_per_item = bc;
+    if (waves_per_cu) *waves_per_cu = bw;
+    if (best_ms) *best_ms = best;
+    return GM_OK;
+}
+
+}  // extern "C"

@@ -7,6 +7,7 @@
 // paths relative to the reference repository root).  Forces leave the
 // fixed-point accumulator as value * 2^-32 kcal/mol/A.
 #include "gm_internal.hpp"
+#include "gm_terms.hpp"
 
 using namespace gm;
 
@@ -18,23 +19,30 @@ struct Segs {
     int nseg;
 };
 
-// Blocks [0, nseg): block b sums segment b (thread t adds items t, t+1024, ...
-// in order, then a fixed LDS tree).  Blocks >= nseg: one thread per atom.
+// Fixed-order f64 sum of one segment by a 1024-thread block: thread t adds
+// items t, t+1024, ... in order, then a fixed LDS tree.
+__device__ inline double block_sum(const double *ptr, long long len, double *sh) {
+    const int t = threadIdx.x;
+    double a = 0.0;
+    for (long long i = t; i < len; i += 1024) a += ptr[i];
+    sh[t] = a;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (t < w) sh[t] = sh[t] + sh[t + w];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+// Blocks [0, nseg): block b sums segment b.  Blocks >= nseg: one thread per atom.
 __global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, int n, const int *atom_slot,
                                                  unsigned long long *F, size_t n_pad, double *fx,
                                                  double *fy, double *fz) {
     __shared__ double sh[1024];
     const int b = blockIdx.x, t = threadIdx.x;
     if (b < sg.nseg) {
-        double a = 0.0;
-        for (long long i = t; i < sg.len[b]; i += 1024) a += sg.ptr[b][i];
-        sh[t] = a;
-        __syncthreads();
-        for (int w = 512; w > 0; w >>= 1) {
-            if (t < w) sh[t] = sh[t] + sh[t + w];
-            __syncthreads();
-        }
-        if (t == 0) eout[b] = sh[0];
+        const double v = block_sum(sg.ptr[b], sg.len[b], sh);
+        if (t == 0) eout[b] = v;
         return;
     }
     const int a = (b - sg.nseg) * 1024 + t;
@@ -48,9 +56,63 @@ __global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, int n, c
     F[2 * n_pad + s] = 0ull;
 }
 
+// Batched end of evaluation: blocks [0, 5 R) sum replica b/5's segment b%5
+// (bond, angle, dihedral, 1-4, LJ) exactly like k_finish; the remaining
+// blocks convert every atom's force of every replica and clear the
+// accumulators; block 0 re-arms the work ticket for the next launch.
+__global__ __launch_bounds__(1024) void k_batch_finish(const ReplicaDesc *__restrict__ reps, int n_reps,
+                                                       const int *__restrict__ atom_rep,
+                                                       const int64_t *__restrict__ atom_off,
+                                                       int64_t n_atoms, double *eout, double *fx,
+                                                       double *fy, double *fz, unsigned int *ticket) {
+    __shared__ double sh[1024];
+    const int b = blockIdx.x, t = threadIdx.x;
+    if (b == 0 && t == 0) *ticket = 0u;
+    if (b < 5 * n_reps) {
+        const ReplicaDesc &R = reps[b / 5];
+        const int seg = b % 5;
+        const double *eb = R.terms.e_items;
+        const double *ptr;
+        long long len;
+        switch (seg) {
+            case 0: ptr = eb; len = R.terms.nb; break;
+            case 1: ptr = eb + R.terms.nb; len = R.terms.na; break;
+            case 2: ptr = eb + R.terms.nb + R.terms.na; len = R.terms.nd; break;
+            case 3: ptr = eb + R.terms.nb + R.terms.na + R.terms.nd; len = R.terms.n14; break;
+            default: ptr = R.tile.e_wi; len = R.n_wi; break;
+        }
+        const double v = block_sum(ptr, len, sh);
+        if (t == 0) eout[b] = v;
+        return;
+    }
+    const int64_t a = (int64_t)(b - 5 * n_reps) * 1024 + t;
+    if (a >= n_atoms) return;
+    const int r = atom_rep[a];
+    const ReplicaDesc &R = reps[r];
+    unsigned long long *F = R.terms.F;
+    const size_t n_pad = R.terms.n_pad;
+    const size_t s = (size_t)R.terms.atom_slot[a - atom_off[r]];
+    fx[a] = (double)(long long)F[s] * kFixedInv;
+    fy[a] = (double)(long long)F[n_pad + s] * kFixedInv;
+    fz[a] = (double)(long long)F[2 * n_pad + s] * kFixedInv;
+    F[s] = 0ull;
+    F[n_pad + s] = 0ull;
+    F[2 * n_pad + s] = 0ull;
+}
+
 }  // namespace
 
 namespace gm {
+
+hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int *atom_rep,
+                               const int64_t *atom_off, int64_t n_atoms, double *eout,
+                               double *fx, double *fy, double *fz, unsigned int *ticket,
+                               hipStream_t st) {
+    const unsigned grid = 5u * (unsigned)n_reps + (unsigned)((n_atoms + 1023) / 1024);
+    k_batch_finish<<<grid, 1024, 0, st>>>(reps, n_reps, atom_rep, atom_off, n_atoms, eout, fx, fy,
+                                          fz, ticket);
+    return hipGetLastError();
+}
 
 hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz) {
     Segs sg{};

@@ -89,6 +89,7 @@ struct gm_ctx {
 struct gm_sys {
     gm_ctx *ctx = nullptr;
     hipStream_t stream = nullptr;
+    bool owns_stream = true;
     hipEvent_t ev_eval0 = nullptr, ev_nb0 = nullptr, ev_nb1 = nullptr, ev_eval1 = nullptr;
     hipEvent_t ev_b0 = nullptr, ev_b1 = nullptr;
 
@@ -174,6 +175,13 @@ gm_status from_hip(hipError_t e, const char *what);
         if (_e != hipSuccess) return gm::from_hip(_e, #call);          \
     } while (0)
 
+// ---- host helpers shared by the single-system and batch APIs (gm_api.cpp) --
+namespace gm {
+gm_status system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream_t ext_stream, gm_sys **out);
+gm_status build_on_device(gm_sys *s, bool wrap);
+gm_status make_work_items(gm_sys *s);
+}  // namespace gm
+
 // ---- kernel launchers (defined in the .hip files) --------------------------
 namespace gm {
 // gm_build.hip
@@ -185,6 +193,20 @@ hipError_t launch_needs_rebuild(gm_sys *s, const double *x, const double *y, con
 hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z);
 // gm_bonded.hip: energy reduction, f64 output, accumulator clear
 hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz);
+}  // namespace gm
+namespace gm {
+struct ReplicaDesc;
+struct TermArgs;
+struct TileArgs;
+TermArgs term_args(gm_sys *s, const double *x, const double *y, const double *z);
+TileArgs tile_args(gm_sys *s, const double *x, const double *y, const double *z);
+hipError_t launch_batch_tiles(const ReplicaDesc *reps, const int2 *items, int n_items,
+                              unsigned int *ticket, int grid, hipStream_t st);
+hipError_t launch_batch_bonded(const ReplicaDesc *reps, const int2 *items, int n_items, hipStream_t st);
+hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int *atom_rep,
+                               const int64_t *atom_off, int64_t n_atoms, double *eout,
+                               double *fx, double *fy, double *fz, unsigned int *ticket,
+                               hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long *p, size_t n, hipStream_t st);
 hipError_t launch_fill_i32(int *p, size_t n, int v, hipStream_t st);
 }  // namespace gm

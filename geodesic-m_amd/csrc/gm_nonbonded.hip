@@ -74,15 +74,19 @@ __device__ inline JAtom load_j(const double *__restrict__ x, const double *__res
 // needed; the unit change is folded into the constants and into the scale at
 // which forces enter the fixed-point accumulator.
 template <bool CUBIC>
-__device__ void tile_work(const int w, const int lane, const double *__restrict__ x,
-                          const double *__restrict__ y, const double *__restrict__ z,
-                          const float2 *__restrict__ atom_par, const int *__restrict__ slot_atom,
-                          const int *__restrict__ wi_sc, const int *__restrict__ wi_start,
-                          const int *__restrict__ wi_end, const int2 *__restrict__ ent_meta,
-                          const int *__restrict__ ent_ja, const uint8_t *__restrict__ ent_lb,
-                          unsigned long long *__restrict__ F, const int n_pad,
-                          double *__restrict__ e_wi, const NbParams &p, const double iqx,
-                          const double iqy, const double iqz) {
+__device__ __attribute__((always_inline)) inline void tile_work(const int w, const int lane, const TileArgs &A) {
+    const double *__restrict__ x = A.x;
+    const double *__restrict__ y = A.y;
+    const double *__restrict__ z = A.z;
+    const float2 *__restrict__ atom_par = A.atom_par;
+    const int *__restrict__ slot_atom = A.slot_atom;
+    const int2 *__restrict__ ent_meta = A.ent_meta;
+    const int *__restrict__ ent_ja = A.ent_ja;
+    const uint8_t *__restrict__ ent_lb = A.ent_lb;
+    unsigned long long *__restrict__ F = A.F;
+    const int n_pad = A.n_pad;
+    const NbParams &p = A.p;
+    const double iqx = A.iq[0], iqy = A.iq[1], iqz = A.iq[2];
     // The run's list data (mask bytes, j atoms, metadata) arrive in LDS in one
     // burst of independent loads; the loop then only gathers positions, one
     // entry ahead of the arithmetic.  Reduced j forces wait in LDS and leave
@@ -94,8 +98,8 @@ __device__ void tile_work(const int w, const int lane, const double *__restrict_
     __shared__ int2 s_meta[kMaxRun];
     __shared__ float jf[3][kMaxRun * 8];
     const int il = lane & 7, jl = lane >> 3;
-    const int sc = wi_sc[w];
-    const int e0 = wi_start[w], ne = wi_end[w] - e0;
+    const int sc = A.wi_sc[w];
+    const int e0 = A.wi_start[w], ne = A.wi_end[w] - e0;
     {
         const uint32_t *lb = reinterpret_cast<const uint32_t *>(ent_lb + (size_t)e0 * 64);
         uint32_t lbv[kMaxRun / 4];
@@ -229,17 +233,13 @@ __device__ void tile_work(const int w, const int lane, const double *__restrict_
         }
     }
     for (int o = 1; o < 64; o <<= 1) e_acc += __shfl_xor(e_acc, o);
-    if (lane == 0) e_wi[w] = e_acc;
+    if (lane == 0) A.e_wi[w] = e_acc;
+    __syncthreads();  // LDS is reused by the next work item of a persistent wave
 }
 
 template <bool CUBIC>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_forces(
-    const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
-    const float2 *__restrict__ atom_par, const int *__restrict__ slot_atom,
-    const int *__restrict__ wi_sc, const int *__restrict__ wi_start, const int *__restrict__ wi_end,
-    const int2 *__restrict__ ent_meta, const int *__restrict__ ent_ja,
-    const uint8_t *__restrict__ ent_lb, unsigned long long *__restrict__ F, int n_pad,
-    double *__restrict__ e_wi, NbParams p, double iqx, double iqy, double iqz, TermArgs ta) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_forces(TileArgs A,
+                                                                                       TermArgs ta) {
     // bonded workgroups come first so they start with the tiles instead of
     // forming a tail after them
     const int lane = threadIdx.x;
@@ -250,8 +250,36 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
         if (k < n_terms) term_dispatch(ta, k);
         return;
     }
-    tile_work<CUBIC>((int)blockIdx.x - n_bonded_groups, lane, x, y, z, atom_par, slot_atom, wi_sc,
-                     wi_start, wi_end, ent_meta, ent_ja, ent_lb, F, n_pad, e_wi, p, iqx, iqy, iqz);
+    tile_work<CUBIC>((int)blockIdx.x - n_bonded_groups, lane, A);
+}
+
+// Persistent batched tile kernel: a fixed grid of wavefronts takes tile work
+// items (replica, work item) from an atomic ticket until the queue is empty.
+// Each item's arithmetic is the single-system kernel's, so results do not
+// depend on which wavefront takes which item.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_batch_tiles(
+    const ReplicaDesc *__restrict__ reps, const int2 *__restrict__ items, int n_items,
+    unsigned int *__restrict__ ticket) {
+    const int lane = threadIdx.x;
+    __shared__ int s_item;
+    for (;;) {
+        if (lane == 0) s_item = (int)atomicAdd(ticket, 1u);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= n_items) break;
+        const int2 rw = items[it];
+        tile_work<true>(rw.y, lane, reps[rw.x].tile);  // replicas are cubic
+    }
+}
+
+// Bonded terms of every replica: one workgroup per (replica, group of 64 terms).
+__global__ __launch_bounds__(64) void k_batch_bonded(const ReplicaDesc *__restrict__ reps,
+                                                     const int2 *__restrict__ items) {
+    const int2 rg = items[blockIdx.x];
+    const TermArgs &t = reps[rg.x].terms;
+    const int64_t k = (int64_t)rg.y * 64 + threadIdx.x;
+    if (k < t.nb + t.na + t.nd + t.n14) term_dispatch(t, k);
 }
 
 }  // namespace
@@ -279,24 +307,48 @@ TermArgs term_args(gm_sys *s, const double *x, const double *y, const double *z)
     return t;
 }
 
+TileArgs tile_args(gm_sys *s, const double *x, const double *y, const double *z) {
+    TileArgs A{};
+    A.x = x; A.y = y; A.z = z;
+    A.atom_par = s->atom_par.p;
+    A.slot_atom = s->slot_atom.p;
+    A.wi_sc = s->wi_sc.p; A.wi_start = s->wi_start.p; A.wi_end = s->wi_end.p;
+    A.ent_meta = s->ent_meta.p; A.ent_ja = s->ent_ja.p; A.ent_lb = s->ent_lb.p;
+    A.F = s->F.p;
+    A.n_pad = (int)(s->nsc * 64);
+    A.e_wi = s->e_wi.p;
+    A.p = s->nbp;
+    for (int a = 0; a < 3; a++) A.iq[a] = s->bp.inv_q[a];
+    A.cubic = (s->box[0] == s->box[1] && s->box[1] == s->box[2]) ? 1 : 0;
+    return A;
+}
+
 // One launch: ceil(n_terms / 64) bonded workgroups, then the tile work items.
 hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z) {
     const TermArgs ta = term_args(s, x, y, z);
+    const TileArgs A = tile_args(s, x, y, z);
     const int64_t nterms = ta.nb + ta.na + ta.nd + ta.n14;
     const int64_t grid = s->n_wi + (nterms + 63) / 64;
     hipError_t e = hipEventRecord(s->cur_nb0, s->stream);
     if (e != hipSuccess) return e;
-    const bool cubic = s->box[0] == s->box[1] && s->box[1] == s->box[2];
     if (grid > 0) {
-        auto kern = cubic ? k_forces<true> : k_forces<false>;
-        kern<<<(unsigned)grid, 64, 0, s->stream>>>(
-            x, y, z, s->atom_par.p, s->slot_atom.p, s->wi_sc.p, s->wi_start.p, s->wi_end.p,
-            s->ent_meta.p, s->ent_ja.p, s->ent_lb.p, s->F.p, (int)(s->nsc * 64), s->e_wi.p, s->nbp,
-            s->bp.inv_q[0], s->bp.inv_q[1], s->bp.inv_q[2], ta);
+        auto kern = A.cubic ? k_forces<true> : k_forces<false>;
+        kern<<<(unsigned)grid, 64, 0, s->stream>>>(A, ta);
     }
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     return hipEventRecord(s->cur_nb1, s->stream);
+}
+
+hipError_t launch_batch_tiles(const ReplicaDesc *reps, const int2 *items, int n_items,
+                              unsigned int *ticket, int grid, hipStream_t st) {
+    if (n_items > 0 && grid > 0) k_batch_tiles<<<(unsigned)grid, 64, 0, st>>>(reps, items, n_items, ticket);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_bonded(const ReplicaDesc *reps, const int2 *items, int n_items, hipStream_t st) {
+    if (n_items > 0) k_batch_bonded<<<(unsigned)n_items, 64, 0, st>>>(reps, items);
+    return hipGetLastError();
 }
 
 }  // namespace gm

@@ -185,8 +185,33 @@ __device__ inline double term_lj14(const TermArgs &t, int64_t n) {
     return e;
 }
 
+// Everything one nonbonded work item reads (one system or one replica).
+struct TileArgs {
+    const double *x, *y, *z;           // positions, atom order
+    const float2 *atom_par;            // (sigma/2, sqrt(eps)) per atom
+    const int *slot_atom;              // slot -> atom (-1 padding)
+    const int *wi_sc, *wi_start, *wi_end;
+    const int2 *ent_meta;              // (j cluster, i-cluster mask)
+    const int *ent_ja;                 // j-cluster atoms per entry
+    const uint8_t *ent_lb;             // per-lane mask bytes per entry
+    unsigned long long *F;             // int64 fixed-point accumulator, slot order
+    int n_pad;                         // slots
+    double *e_wi;                      // per work item LJ energy
+    NbParams p;
+    double iq[3];                      // 2^32 / box
+    int cubic;
+};
+
+// One replica of a batch: its tile and bonded arguments and item split.
+struct ReplicaDesc {
+    TileArgs tile;
+    TermArgs terms;
+    int n_bonded_groups;               // items [0, n) are bonded groups, then work items
+    int n_wi;
+};
+
 // Term k of the flat range [bonds | angles | dihedrals | 1-4]; writes its energy.
-__device__ inline void term_dispatch(const TermArgs &t, int64_t k) {
+__device__ __attribute__((always_inline)) inline void term_dispatch(const TermArgs &t, int64_t k) {
     double e;
     int64_t n = k;
     if (n < t.nb) {

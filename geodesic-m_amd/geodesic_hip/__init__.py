@@ -118,6 +118,17 @@ def lib():
             "gm_device_free": ([_P, _P], C.c_int),
             "gm_copy_to_device": ([_P, _P, _P, C.c_size_t], C.c_int),
             "gm_copy_to_host": ([_P, _P, _P, C.c_size_t], C.c_int),
+            "gm_batch_create": ([_P, _P, C.c_int32, C.POINTER(_P)], C.c_int),
+            "gm_batch_destroy": ([_P], C.c_int),
+            "gm_batch_replica": ([_P, C.c_int32, C.POINTER(_P)], C.c_int),
+            "gm_batch_build_neighbor_lists": ([_P, _P, _P, _P], C.c_int),
+            "gm_batch_set_positions": ([_P, _P, _P, _P], C.c_int),
+            "gm_batch_compute": ([_P], C.c_int),
+            "gm_batch_sync": ([_P, _P, C.POINTER(C.c_float)], C.c_int),
+            "gm_batch_get_forces": ([_P, _P, _P, _P], C.c_int),
+            "gm_batch_set_shape": ([_P, C.c_int32, C.c_int32], C.c_int),
+            "gm_batch_autotune": ([_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_float)],
+                                  C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -224,63 +235,83 @@ class Params:
     bonded_min_image: bool = False
 
 
+def make_desc(sigma, epsilon, topo, params: Params, keep: list) -> SystemDesc:
+    """gm_system_desc for the arrays given; `keep` holds the converted arrays
+    alive while the descriptor is in use."""
+
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a
+
+    d = SystemDesc()
+    sig = arr(sigma, np.float64)
+    eps = arr(epsilon, np.float64)
+    d.n_atoms = len(sig)
+    d.sigma, d.epsilon = _p(sig), _p(eps)
+    box = np.broadcast_to(np.asarray(params.box, np.float64), (3,))
+    for k in range(3):
+        d.box[k] = float(box[k])
+    d.periodic = int(bool(params.periodic))
+    d.r_cutoff, d.r_switch, d.r_skin = params.r_cutoff, params.r_switch, params.r_skin
+    d.scale_14 = params.scale_14
+    d.bonded_min_image = int(bool(params.bonded_min_image))
+    u32, f64 = np.uint32, np.float64
+    g = lambda name, dt: arr(getattr(topo, name), dt)  # noqa: E731
+    bi, bj, bk, br0 = g("bond_i", u32), g("bond_j", u32), g("bond_k", f64), g("bond_r0", f64)
+    d.n_bonds = len(bi)
+    d.bond_i, d.bond_j, d.bond_k, d.bond_r0 = _p(bi), _p(bj), _p(bk), _p(br0)
+    ai, aj, ak = g("angle_i", u32), g("angle_j", u32), g("angle_k", u32)
+    akth, ath0 = g("angle_kth", f64), g("angle_th0", f64)
+    d.n_angles = len(ai)
+    d.angle_i, d.angle_j, d.angle_k, d.angle_kth, d.angle_th0 = _p(ai), _p(aj), _p(ak), _p(akth), _p(ath0)
+    di, dj, dk, dl = g("dihed_i", u32), g("dihed_j", u32), g("dihed_k", u32), g("dihed_l", u32)
+    dkp, dn, dd = g("dihed_kphi", f64), g("dihed_n", u32), g("dihed_delta", f64)
+    d.n_dihedrals = len(di)
+    d.dihed_i, d.dihed_j, d.dihed_k, d.dihed_l = _p(di), _p(dj), _p(dk), _p(dl)
+    d.dihed_kphi, d.dihed_n, d.dihed_delta = _p(dkp), _p(dn), _p(dd)
+    ei, ej = g("excl_i", u32), g("excl_j", u32)
+    d.n_excl = len(ei)
+    d.excl_i, d.excl_j = _p(ei), _p(ej)
+    p14i = arr(getattr(topo, "pair14_i", np.zeros(0)), u32)
+    p14j = arr(getattr(topo, "pair14_j", np.zeros(0)), u32)
+    d.n_pairs14 = len(p14i)
+    d.pair14_i, d.pair14_j = _p(p14i), _p(p14j)
+    return d
+
+
 class System:
     """gm_sys: one simulated system on the device."""
 
     def __init__(self, sigma, epsilon, topo, params: Params, ctx: Context | None = None):
         ctx = ctx or default_context()
         self.ctx = ctx
+        self._owned = True
         keep = []
-
-        def arr(a, dt):
-            a = np.ascontiguousarray(a, dtype=dt)
-            keep.append(a)
-            return a
-
-        d = SystemDesc()
-        sig = arr(sigma, np.float64)
-        eps = arr(epsilon, np.float64)
-        self.n = len(sig)
-        d.n_atoms = self.n
-        d.sigma, d.epsilon = _p(sig), _p(eps)
-        box = np.broadcast_to(np.asarray(params.box, np.float64), (3,))
-        for k in range(3):
-            d.box[k] = float(box[k])
-        self.box = tuple(float(b) for b in box)
-        d.periodic = int(bool(params.periodic))
-        d.r_cutoff, d.r_switch, d.r_skin = params.r_cutoff, params.r_switch, params.r_skin
-        d.scale_14 = params.scale_14
-        d.bonded_min_image = int(bool(params.bonded_min_image))
-        u32, f64 = np.uint32, np.float64
-        g = lambda name, dt: arr(getattr(topo, name), dt)  # noqa: E731
-        bi, bj, bk, br0 = g("bond_i", u32), g("bond_j", u32), g("bond_k", f64), g("bond_r0", f64)
-        d.n_bonds = len(bi)
-        d.bond_i, d.bond_j, d.bond_k, d.bond_r0 = _p(bi), _p(bj), _p(bk), _p(br0)
-        ai, aj, ak = g("angle_i", u32), g("angle_j", u32), g("angle_k", u32)
-        akth, ath0 = g("angle_kth", f64), g("angle_th0", f64)
-        d.n_angles = len(ai)
-        d.angle_i, d.angle_j, d.angle_k, d.angle_kth, d.angle_th0 = _p(ai), _p(aj), _p(ak), _p(akth), _p(ath0)
-        di, dj, dk, dl = g("dihed_i", u32), g("dihed_j", u32), g("dihed_k", u32), g("dihed_l", u32)
-        dkp, dn, dd = g("dihed_kphi", f64), g("dihed_n", u32), g("dihed_delta", f64)
-        d.n_dihedrals = len(di)
-        d.dihed_i, d.dihed_j, d.dihed_k, d.dihed_l = _p(di), _p(dj), _p(dk), _p(dl)
-        d.dihed_kphi, d.dihed_n, d.dihed_delta = _p(dkp), _p(dn), _p(dd)
-        ei, ej = g("excl_i", u32), g("excl_j", u32)
-        d.n_excl = len(ei)
-        d.excl_i, d.excl_j = _p(ei), _p(ej)
-        p14i = arr(getattr(topo, "pair14_i", np.zeros(0)), u32)
-        p14j = arr(getattr(topo, "pair14_j", np.zeros(0)), u32)
-        d.n_pairs14 = len(p14i)
-        d.pair14_i, d.pair14_j = _p(p14i), _p(p14j)
+        d = make_desc(sigma, epsilon, topo, params, keep)
+        self.n = int(d.n_atoms)
+        self.box = tuple(float(d.box[k]) for k in range(3))
         h = _P()
         _check(lib().gm_system_create(ctx.h, C.byref(d), C.byref(h)))
         self.h = h
         self.params = params
 
+    @classmethod
+    def borrowed(cls, handle, n, params):
+        """A view of a system owned elsewhere (a batch replica)."""
+        self = cls.__new__(cls)
+        self.ctx = None
+        self._owned = False
+        self.h = handle
+        self.n = int(n)
+        self.params = params
+        self.box = tuple(float(b) for b in np.broadcast_to(np.asarray(params.box, np.float64), (3,)))
+        return self
+
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self, "_owned", False):
             lib().gm_system_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -356,6 +387,78 @@ class System:
         s = Stats()
         _check(lib().gm_get_stats(self.h, C.byref(s)))
         return s.as_dict()
+
+
+class Batch:
+    """gm_batch: independent replicas evaluated by one persistent launch from a
+    captured hipGraph.  `members` is a list of (sigma, epsilon, topo, params);
+    positions and forces are concatenated in member order."""
+
+    def __init__(self, members, ctx: Context | None = None):
+        ctx = ctx or default_context()
+        self.ctx = ctx
+        keep = []
+        descs = (SystemDesc * len(members))()
+        for k, (sig, eps, topo, params) in enumerate(members):
+            descs[k] = make_desc(sig, eps, topo, params, keep)
+        self.sizes = [int(descs[k].n_atoms) for k in range(len(members))]
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        self.n = int(self.offsets[-1])
+        self.params = [m[3] for m in members]
+        h = _P()
+        _check(lib().gm_batch_create(ctx.h, C.cast(descs, _P), len(members), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.sizes)
+
+    def replica(self, r):
+        h = _P()
+        _check(lib().gm_batch_replica(self.h, int(r), C.byref(h)))
+        return System.borrowed(h, self.sizes[r], self.params[r])
+
+    def build_neighbor_lists(self, x, y, z):
+        for a in (x, y, z):
+            assert a.dtype == np.float64 and a.flags.c_contiguous and len(a) == self.n
+        _check(lib().gm_batch_build_neighbor_lists(self.h, _p(x), _p(y), _p(z)))
+
+    def set_positions(self, x, y, z):
+        x, y, z = (np.ascontiguousarray(a, np.float64) for a in (x, y, z))
+        _check(lib().gm_batch_set_positions(self.h, _p(x), _p(y), _p(z)))
+
+    def compute(self):
+        _check(lib().gm_batch_compute(self.h))
+
+    def sync(self):
+        """Returns (list of EnergyTerms per replica, device ms of the last compute)."""
+        e = (EnergyTerms * len(self.sizes))()
+        ms = C.c_float(0.0)
+        _check(lib().gm_batch_sync(self.h, C.cast(e, _P), C.byref(ms)))
+        return [e[k] for k in range(len(self.sizes))], ms.value
+
+    def get_forces(self):
+        fx, fy, fz = np.empty(self.n), np.empty(self.n), np.empty(self.n)
+        _check(lib().gm_batch_get_forces(self.h, _p(fx), _p(fy), _p(fz)))
+        return fx, fy, fz
+
+    def set_shape(self, entries_per_item, waves_per_cu):
+        _check(lib().gm_batch_set_shape(self.h, int(entries_per_item), int(waves_per_cu)))
+
+    def autotune(self):
+        c, w, ms = C.c_int32(0), C.c_int32(0), C.c_float(0.0)
+        _check(lib().gm_batch_autotune(self.h, C.byref(c), C.byref(w), C.byref(ms)))
+        return c.value, w.value, ms.value
 
 
 # ---- mirror of the reference's backend API ------------------------------------

@@ -184,6 +184,38 @@ gm_status gm_profile_end(gm_sys *sys, gm_profile *out);
  * immediately on a built list (no rebuild). */
 gm_status gm_set_work_chunk(gm_sys *sys, int32_t entries_per_item);
 
+/* ---- Batched replicas [ext: no reference counterpart] ---------------------
+ * Many independent systems (own atom counts, boxes, topologies, parameters)
+ * evaluated by one persistent launch replayed from a captured hipGraph, for
+ * landscape mapping.  Replica boxes must be cubic.  Positions/forces are
+ * concatenated in replica order.
+ * Each replica's forces and energies are bit-identical to evaluating it alone
+ * with the same work shape. */
+typedef struct gm_batch gm_batch;
+gm_status gm_batch_create(gm_ctx *ctx, const gm_system_desc *descs, int32_t n_replicas,
+                          gm_batch **out);
+gm_status gm_batch_destroy(gm_batch *b);
+/* Borrowed handle of replica r (owned by the batch): pair lists, stats. */
+gm_status gm_batch_replica(gm_batch *b, int32_t r, gm_sys **out);
+/* Builds every replica's list (wrapping positions in place when periodic),
+ * then the descriptor table, work queue and graph. */
+gm_status gm_batch_build_neighbor_lists(gm_batch *b, double *x, double *y, double *z);
+gm_status gm_batch_set_positions(gm_batch *b, const double *x, const double *y, const double *z);
+/* One graph launch: every replica's forces and energies (asynchronous). */
+gm_status gm_batch_compute(gm_batch *b);
+/* Waits; e receives n_replicas per-replica f64 energy breakdowns (may be
+ * NULL); ms the device time of the last gm_batch_compute (may be NULL). */
+gm_status gm_batch_sync(gm_batch *b, gm_energy_terms *e, float *ms);
+gm_status gm_batch_get_forces(gm_batch *b, double *fx, double *fy, double *fz);
+/* Work shape: list entries per tile work item (0 = automatic, at most 32)
+ * and persistent wavefronts per compute unit.  The shape fixes how pair
+ * forces are grouped before fixed-point conversion, so pin it (or the
+ * autotuner's choice) for byte-identical reruns. */
+gm_status gm_batch_set_shape(gm_batch *b, int32_t entries_per_item, int32_t waves_per_cu);
+/* Times a grid of shapes on the current lists and keeps the fastest. */
+gm_status gm_batch_autotune(gm_batch *b, int32_t *entries_per_item, int32_t *waves_per_cu,
+                            float *best_ms);
+
 #ifdef __cplusplus
 }
 #endif

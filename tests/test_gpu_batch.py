@@ -1,0 +1,131 @@
+"""Batched replicas on the GPU (no reference counterpart): every replica of a
+batch against its own f64 oracle, against the same replica evaluated alone
+(byte-identical), across graph replays, and across work shapes."""
+import sys
+
+import numpy as np
+import pytest
+
+import geodesic_hip as gh
+from geodesic_hip import systems
+from oracle import oracle as orc
+
+from _common import force_rms_rel
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [500, 926, 1500, 2400, 3100, 5200]
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def make_replicas(sizes, seed=21):
+    reps = []
+    for k, n in enumerate(sizes):
+        reps.append(systems.protein_solvent(int(n), (n / 0.1) ** (1 / 3), int(n) // 8, seed=seed + k))
+    return reps
+
+
+def concat(reps):
+    return (np.concatenate([r.x for r in reps]), np.concatenate([r.y for r in reps]),
+            np.concatenate([r.z for r in reps]))
+
+
+@pytest.fixture(scope="module")
+def batch_case():
+    reps = make_replicas(SIZES)
+    say("[gpu] batch build")
+    b = gh.Batch([(r.sigma, r.epsilon, r.topo, r.params) for r in reps])
+    x, y, z = concat(reps)
+    b.build_neighbor_lists(x, y, z)
+    b.compute()
+    e, ms = b.sync()
+    f = b.get_forces()
+    return reps, b, (x, y, z), e, f
+
+
+def test_each_replica_matches_its_oracle(batch_case):
+    reps, b, (x, y, z), e, (fx, fy, fz) = batch_case
+    for k, r in enumerate(reps):
+        say(f"[gpu] batch replica {k} n={r.n}")
+        o0, o1 = b.offsets[k], b.offsets[k + 1]
+        wx, wy, wz = r.x.copy(), r.y.copy(), r.z.copy()
+        orc.wrap_positions(wx, wy, wz, r.params.box)
+        assert x[o0:o1].tobytes() == wx.tobytes()
+        pi, pj = orc.neighbor_build(wx, wy, wz, r.params.box, r.params.r_skin, r.topo.excl_i, r.topo.excl_j,
+                                    method="brute")
+        qi, qj = b.replica(k).pair_list()
+        assert np.array_equal(qi, pi) and np.array_equal(qj, pj)
+        t, gx, gy, gz = orc.cpu_backend_compute(wx, wy, wz, r.sigma, r.epsilon, r.topo, pi, pj, r.params.box,
+                                                r.params.r_cutoff, r.params.r_switch, n_threads=8,
+                                                scale14=r.params.scale_14, bonded_min_image=True)
+        for name in ("bond", "angle", "dihedral", "lj", "lj14", "total"):
+            g, ref = getattr(e[k], name), getattr(t, name)
+            assert abs(g - ref) <= 1e-6 * abs(ref), (k, name, g, ref)
+        assert force_rms_rel(fx[o0:o1], fy[o0:o1], fz[o0:o1], gx, gy, gz) <= 1e-5
+
+
+def test_batch_equals_each_replica_alone(batch_case):
+    """Same work items, same arithmetic: forces and energies byte-identical."""
+    reps, b, _, e, (fx, fy, fz) = batch_case
+    for k, r in enumerate(reps):
+        say(f"[gpu] plain replica {k}")
+        g = gh.System(r.sigma, r.epsilon, r.topo, r.params)
+        x, y, z = r.x.copy(), r.y.copy(), r.z.copy()
+        g.build_neighbor_list(x, y, z)
+        ep, px, py, pz = g.compute_forces(x, y, z)
+        o0, o1 = b.offsets[k], b.offsets[k + 1]
+        assert px.tobytes() == fx[o0:o1].tobytes()
+        assert py.tobytes() == fy[o0:o1].tobytes() and pz.tobytes() == fz[o0:o1].tobytes()
+        assert ep.total == e[k].total and ep.lj == e[k].lj
+
+
+def test_graph_replays_are_byte_identical(batch_case):
+    """Replays of the captured graph, with eager copies and syncs between them,
+    reproduce the first evaluation exactly."""
+    reps, b, (x, y, z), e0, f0 = batch_case
+    for it in range(3):
+        say(f"[gpu] graph replay {it}")
+        b.set_positions(x, y, z)
+        b.compute()
+        e, ms = b.sync()
+        f = b.get_forces()
+        assert ms > 0.0
+        for u, v in zip(f, f0):
+            assert u.tobytes() == v.tobytes()
+        assert [t.total for t in e] == [t.total for t in e0]
+
+
+def test_autotuned_shape_is_pinned_and_reproducible(batch_case):
+    reps, b, (x, y, z), e0, f0 = batch_case
+    say("[gpu] autotune")
+    c, w, ms = b.autotune()
+    assert 4 <= c <= 32 and 1 <= w <= 64 and ms > 0.0
+    b.compute()
+    e1, _ = b.sync()
+    f1 = b.get_forces()
+    b.compute()
+    e2, _ = b.sync()
+    f2 = b.get_forces()
+    for u, v, r in zip(f1, f2, f0):
+        assert u.tobytes() == v.tobytes()
+        # a different work shape groups f32 partial sums differently
+        assert np.max(np.abs(u - r)) <= 1e-6 * max(1.0, np.max(np.abs(r)))
+    for a, c0 in zip(e1, e0):
+        assert abs(a.total - c0.total) <= 1e-9 * abs(c0.total)
+
+
+def test_batch_errors():
+    reps = make_replicas([500, 700])
+    b = gh.Batch([(r.sigma, r.epsilon, r.topo, r.params) for r in reps])
+    with pytest.raises(gh.GmError) as ei:
+        b.compute()
+    assert ei.value.status == 7
+    r = reps[0]
+    bad = gh.Params(box=(17.0, 18.0, 17.0), periodic=True, r_cutoff=9.0, r_switch=7.5, r_skin=10.5)
+    with pytest.raises(gh.GmError) as ei:
+        gh.Batch([(r.sigma, r.epsilon, r.topo, bad)])
+    assert ei.value.status == 5
