@@ -31,17 +31,6 @@ PEAK_FP32_TFLOPS = 157.3    # MI355X published FP32 peak
 PEAK_HBM_GBS = 8000.0
 
 
-def in_cutoff_pairs(sysobj, g):
-    """Pairs of the exported Verlet list with min-image r2 <= r_c^2 (f64)."""
-    pi, pj = g.pair_list()
-    x, y, z = sysobj
-    L = np.asarray(g.box)
-    d = np.stack([x[pj] - x[pi], y[pj] - y[pi], z[pj] - z[pi]])
-    d -= L[:, None] * np.round(d / L[:, None])
-    r2 = np.einsum("ij,ij->j", d, d)
-    return int(np.count_nonzero(r2 <= g.params.r_cutoff ** 2)), len(pi)
-
-
 def traffic_from_profiles(name):
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
@@ -80,6 +69,99 @@ def cpu_baseline(s, n_pairs_cut, evals=20, warm=3):
             "ms_per_eval": med * 1e3, "list_build_s": list_s}
 
 
+def time_resident(g, steps, warmup):
+    """Average wall ms per evaluation at resident positions, and the event
+    profile of the timed region."""
+    for _ in range(warmup):
+        g.compute_forces_resident()
+    g.profile_begin()  # create the event pool outside the timed region
+    for _ in range(min(steps, 8192)):
+        g.compute_forces_resident()
+    g.profile_end()
+    g.sync()
+    g.profile_begin()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.compute_forces_resident()
+    e = g.sync()
+    t1 = time.perf_counter()
+    prof = g.profile_end()
+    return (t1 - t0) * 1e3 / steps, prof, e
+
+
+def bench_250k(steps=100, warmup=10):
+    """250,000 atoms, 136 A box: force evaluation at resident positions and the
+    list build, amortised over the 20-step rebuild interval."""
+    import geodesic_hip as gh
+    from geodesic_hip import systems
+    s = systems.system_250k()
+    g = gh.System(s.sigma, s.epsilon, s.topo, s.params)
+    x, y, z = s.x.copy(), s.y.copy(), s.z.copy()
+    g.build_neighbor_list(x, y, z)
+    builds = []
+    for _ in range(3):
+        g.build_neighbor_list(x, y, z)
+        builds.append(g.stats()["ms_build"])
+    st = g.stats()
+    ms, prof, e = time_resident(g, steps, warmup)
+    build_ms = statistics.median(builds)
+    n_cut = st["n_pairs_cutoff"]
+    return {"workload": "force evaluation, 250,000 atoms, 136 A periodic box, rebuild every 20 steps",
+            "n_pairs_cutoff": n_cut, "ms_per_eval": ms,
+            "kernel_ms_avg": prof["ms_nonbonded_total"] / max(prof["n_evals"], 1),
+            "list_build_ms": build_ms, "ms_per_step_amortised": ms + build_ms / 20.0,
+            "pair_interactions_per_s": n_cut / (ms * 1e-3),
+            "pair_interactions_per_s_amortised": n_cut / ((ms + build_ms / 20.0) * 1e-3)}
+
+
+def bench_batched(n_replicas=512, steps=20):
+    """512 independent replicas (log-uniform 500..20000 atoms): one graph launch
+    per batched step vs the plain path evaluating the replicas one by one."""
+    import geodesic_hip as gh
+    from geodesic_hip import systems
+    t0 = time.perf_counter()
+    reps = systems.replica_set(n_replicas)
+    t_gen = time.perf_counter() - t0
+    b = gh.Batch([(r.sigma, r.epsilon, r.topo, r.params) for r in reps])
+    x = np.concatenate([r.x for r in reps])
+    y = np.concatenate([r.y for r in reps])
+    z = np.concatenate([r.z for r in reps])
+    t0 = time.perf_counter()
+    b.build_neighbor_lists(x, y, z)
+    t_build = time.perf_counter() - t0
+    n_cut = sum(b.replica(k).stats()["n_pairs_cutoff"] for k in range(len(reps)))
+    chunk, waves, tune_ms = b.autotune()
+    for _ in range(3):
+        b.compute()
+    b.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.compute()
+    _, last_ms = b.sync()
+    batched_ms = (time.perf_counter() - t0) * 1e3 / steps
+    # plain path: each replica's own two launches, back to back on one stream
+    plain = [b.replica(k) for k in range(len(reps))]
+    for g in plain:
+        g.compute_forces_resident()
+    plain[-1].sync()
+    sweeps = 3
+    t0 = time.perf_counter()
+    for _ in range(sweeps):
+        for g in plain:
+            g.compute_forces_resident()
+    plain[-1].sync()
+    plain_ms = (time.perf_counter() - t0) * 1e3 / sweeps
+    return {"workload": f"{len(reps)} replicas, {b.n} atoms, log-uniform 500..20000 atoms, "
+                        "one persistent launch per step from a captured hipGraph",
+            "n_atoms": b.n, "n_pairs_cutoff": n_cut, "ms_per_batched_step": batched_ms,
+            "device_ms_last_step": last_ms, "ms_plain_path_all_replicas": plain_ms,
+            "pair_interactions_per_s": n_cut / (batched_ms * 1e-3),
+            "plain_pair_interactions_per_s": n_cut / (plain_ms * 1e-3),
+            "speedup_vs_plain": plain_ms / batched_ms,
+            "autotuned_shape": {"entries_per_item": chunk, "waves_per_cu": waves, "ms": tune_ms},
+            "setup_s": {"generate": t_gen, "build_lists": t_build}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +169,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--chunk", type=int, default=-1, help="entries per work item (tuning)")
+    ap.add_argument("--no-250k", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -103,24 +187,9 @@ def main():
     if args.chunk >= 0:
         g.set_work_chunk(args.chunk)
     st = g.stats()
-    n_cut, n_list = in_cutoff_pairs((x, y, z), g)
+    n_cut, n_list = st["n_pairs_cutoff"], st["n_list_pairs"]
 
-    for _ in range(args.warmup):
-        g.compute_forces_resident()
-    g.profile_begin()  # create the event pool outside the timed region
-    for _ in range(min(args.steps, 8192)):
-        g.compute_forces_resident()
-    g.profile_end()
-    g.sync()
-
-    g.profile_begin()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        g.compute_forces_resident()
-    e = g.sync()
-    t1 = time.perf_counter()
-    prof = g.profile_end()
-    ms_step = (t1 - t0) * 1e3 / args.steps
+    ms_step, prof, e = time_resident(g, args.steps, args.warmup)
     nb_ms = prof["ms_nonbonded_total"] / max(prof["n_evals"], 1)
     ev_ms = prof["ms_eval_total"] / max(prof["n_evals"], 1)
     achieved = n_cut * FLOP_PER_PAIR / (nb_ms * 1e-3) / 1e12
@@ -153,6 +222,11 @@ def main():
                                     "n_work_items")},
         "energy_total": e.total,
     }
+    if not args.no_250k:
+        out["system_250k"] = bench_250k()
+    if not args.no_batched:
+        bb = bench_batched()
+        out["batched"] = bb
     if not args.no_cpu_baseline:
         cb = cpu_baseline(s, n_cut)
         out["cpu_baseline"] = cb

@@ -147,7 +147,7 @@ gm_status build_on_device(gm_sys *s, bool wrap) {
     GM_HIP(launch_fill_u64(s->F.p, (size_t)s->max_sc * 64 * 3, st));
     s->F_dirty = false;
     GM_HIP(hipEventRecord(s->ev_b1, st));
-    unsigned long long counters[3] = {0, 0, 0};
+    unsigned long long counters[4] = {0, 0, 0, 0};
     GM_HIP(hipMemcpyAsync(counters, s->meta.p + 8, sizeof(counters), hipMemcpyDeviceToHost, st));
     GM_HIP(hipStreamSynchronize(st));
     float ms = 0.f;
@@ -159,6 +159,7 @@ gm_status build_on_device(gm_sys *s, bool wrap) {
     s->stats.n_list_pairs = (int64_t)counters[0];
     s->stats.n_tiles = (int64_t)counters[1];
     s->stats.n_tiles_cutoff = (int64_t)counters[2];
+    s->stats.n_pairs_cutoff = (int64_t)counters[3];
     s->stats.n_work_items = s->n_wi;
     s->stats.ms_build = ms;
     s->built = true;
@@ -465,7 +466,7 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     TRY(s->sorted_atoms.ensure(N));
     TRY(s->atom_slot.ensure(N));
     for (DevBuf<int> *b : {&s->sc_base, &s->sc_m, &s->sc_count}) TRY(b->ensure((size_t)s->max_sc + 1));
-    TRY(s->meta.ensure(16));
+    TRY(s->meta.ensure(24));
     TRY(s->slot_atom.ensure(slots));
     TRY(s->sq.ensure(slots));
     TRY(s->sp.ensure(slots));

@@ -332,7 +332,7 @@ __global__ __launch_bounds__(64) void k_masks(int nent, const int *ent_sc, const
     const uint32_t im = ent_imask[e];
     const int b = slot_atom[(size_t)cj * 8 + jl];
     uint32_t newmask = 0, lanebits = 0;
-    unsigned long long npairs = 0, ncut_tiles = 0;
+    unsigned long long npairs = 0, ncut_tiles = 0, ncut_pairs = 0;
     if (il == 0) ent_ja[(size_t)e * 8 + jl] = b;
     for (int ci = 0; ci < 8; ci++) {
         unsigned long long m = 0ull;
@@ -353,7 +353,9 @@ __global__ __launch_bounds__(64) void k_masks(int nent, const int *ent_sc, const
                 }
             }
             m = __ballot(in);
-            if (__ballot(cut)) ncut_tiles++;
+            const unsigned long long mc = __ballot(in && cut);
+            if (mc) ncut_tiles++;
+            ncut_pairs += __popcll(mc);
             lanebits |= in ? (1u << ci) : 0u;
         }
         if (t == 0) ent_mask[(size_t)e * 8 + ci] = m;
@@ -366,6 +368,7 @@ __global__ __launch_bounds__(64) void k_masks(int nent, const int *ent_sc, const
         atomicAdd(&counters[0], npairs);
         atomicAdd(&counters[1], (unsigned long long)__popc(newmask));
         atomicAdd(&counters[2], ncut_tiles);
+        atomicAdd(&counters[3], ncut_pairs);
     }
 }
 
@@ -446,7 +449,7 @@ hipError_t launch_pair_fill_and_masks(gm_sys *s) {
         k_pair_search<1><<<(unsigned)s->nsc, 64, 0, st>>>(
         s->meta.p, (int)s->ncl, s->bp, s->cl_ref.p, s->cl_half.p, s->sc_ref.p, s->sc_half.p,
         nullptr, s->sc_start.p, s->ent_sc.p, s->ent_cj.p, s->ent_imask.p);
-    hipError_t e = launch_fill_u64((unsigned long long *)(s->meta.p + 8), 3, st);
+    hipError_t e = launch_fill_u64((unsigned long long *)(s->meta.p + 8), 4, st);
     if (e != hipSuccess) return e;
     if (s->nent > 0)
         k_masks<<<(unsigned)s->nent, 64, 0, st>>>((int)s->nent, s->ent_sc.p, s->ent_cj.p,

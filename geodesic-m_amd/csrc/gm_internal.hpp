@@ -121,7 +121,8 @@ struct gm_sys {
     DevBuf<int> sc_base, sc_m;
     DevBuf<uint32_t> qatom;  // 3*n fixed-point coordinates, atom order
     DevBuf<int> meta;        // [0] nsc, [1] overflow, [2] max column, [5] rebuild flag;
-                             // ints 8..13 hold u64 counters (list pairs, tiles, tiles with a pair within r_cutoff)
+                             // ints 8..15 hold u64 counters: list pairs, tiles, tiles with a pair
+                             // within r_cutoff, pairs within r_cutoff (f64, build positions)
     int64_t max_sc = 0;
 
     // slots

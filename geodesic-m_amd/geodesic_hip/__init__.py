@@ -76,6 +76,7 @@ class Profile(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("n_atoms", _I64), ("n_superclusters", _I64), ("n_clusters", _I64),
                 ("n_entries", _I64), ("n_tiles", _I64), ("n_tiles_cutoff", _I64), ("n_list_pairs", _I64),
+                ("n_pairs_cutoff", _I64),
                 ("n_work_items", _I64), ("ms_nonbonded", C.c_float), ("ms_eval", C.c_float),
                 ("ms_build", C.c_float)]
 

@@ -159,6 +159,7 @@ typedef struct {
     int64_t n_tiles;           /* (i-cluster, j-cluster) 8x8 tiles with a nonzero mask */
     int64_t n_tiles_cutoff;    /* of those, tiles with a pair within r_cutoff at build time */
     int64_t n_list_pairs;      /* pairs in the Verlet list */
+    int64_t n_pairs_cutoff;    /* of those, pairs with r <= r_cutoff (f64) at the build positions */
     int64_t n_work_items;      /* waves launched by the nonbonded kernel */
     float ms_nonbonded;        /* last nonbonded kernel, hipEvent timing */
     float ms_eval;             /* last whole evaluation (device time) */

@@ -47,7 +47,7 @@ int main(void) {
   F(gm_system_desc, scale_14) F(gm_system_desc, bonded_min_image) F(gm_system_desc, n_bonds)
   F(gm_system_desc, n_angles) F(gm_system_desc, n_dihedrals) F(gm_system_desc, dihed_delta)
   F(gm_system_desc, n_excl) F(gm_system_desc, n_pairs14) F(gm_system_desc, pair14_j)
-  F(gm_stats, n_tiles_cutoff) F(gm_stats, ms_nonbonded) F(gm_stats, ms_build)
+  F(gm_stats, n_tiles_cutoff) F(gm_stats, n_pairs_cutoff) F(gm_stats, ms_nonbonded) F(gm_stats, ms_build)
   return 0;
 }
 """

@@ -140,6 +140,10 @@ def test_24k_list_energies_forces_match_oracle(sys24k):
     assert np.array_equal(qi, pi) and np.array_equal(qj, pj)
     st = g.stats()
     assert st["n_list_pairs"] == len(pi)
+    # pair interactions counted by the build: min-image r2 <= r_c^2 in f64
+    d = np.stack([wx[pj] - wx[pi], wy[pj] - wy[pi], wz[pj] - wz[pi]])
+    d -= 62.0 * np.round(d / 62.0)
+    assert st["n_pairs_cutoff"] == int(np.count_nonzero(np.einsum("ij,ij->j", d, d) <= 81.0))
     check_terms(e, t, "24k")
     rms = check_forces(fg, f, "24k", synthetic=True)
     say(f"[gpu] 24k force rms rel {rms:.3e} stats {st}")
