@@ -29,6 +29,7 @@ import numpy as np  # noqa: E402
 FLOP_PER_PAIR = 40          # canonical count per pair interaction (FMA = 2)
 PEAK_FP32_TFLOPS = 157.3    # MI355X published FP32 peak
 PEAK_HBM_GBS = 8000.0
+PROFILE_EVERY = 10          # timed region: HIP events on every 10th evaluation
 
 
 def traffic_from_profiles(name):
@@ -69,24 +70,33 @@ def cpu_baseline(s, n_pairs_cut, evals=20, warm=3):
             "ms_per_eval": med * 1e3, "list_build_s": list_s}
 
 
-def time_resident(g, steps, warmup):
-    """Average wall ms per evaluation at resident positions, and the event
-    profile of the timed region."""
+def time_resident(g, steps, warmup, every=PROFILE_EVERY):
+    """Average wall ms per evaluation at resident positions over the timed
+    region, the HIP-event profile of that region (every `every`-th evaluation
+    carries events around its kernels), and the wall ms per evaluation of the
+    same loop with events on every evaluation."""
     for _ in range(warmup):
         g.compute_forces_resident()
-    g.profile_begin()  # create the event pool outside the timed region
+    g.profile_begin(1)  # create the event pool outside the timed region
     for _ in range(min(steps, 8192)):
         g.compute_forces_resident()
     g.profile_end()
     g.sync()
-    g.profile_begin()
+    g.profile_begin(every)
     t0 = time.perf_counter()
     for _ in range(steps):
         g.compute_forces_resident()
     e = g.sync()
     t1 = time.perf_counter()
     prof = g.profile_end()
-    return (t1 - t0) * 1e3 / steps, prof, e
+    g.profile_begin(1)
+    t2 = time.perf_counter()
+    for _ in range(steps):
+        g.compute_forces_resident()
+    g.sync()
+    t3 = time.perf_counter()
+    g.profile_end()
+    return (t1 - t0) * 1e3 / steps, prof, e, (t3 - t2) * 1e3 / steps
 
 
 def bench_250k(steps=100, warmup=10):
@@ -103,11 +113,11 @@ def bench_250k(steps=100, warmup=10):
         g.build_neighbor_list(x, y, z)
         builds.append(g.stats()["ms_build"])
     st = g.stats()
-    ms, prof, e = time_resident(g, steps, warmup)
+    ms, prof, e, ms_all_events = time_resident(g, steps, warmup)
     build_ms = statistics.median(builds)
     n_cut = st["n_pairs_cutoff"]
     return {"workload": "force evaluation, 250,000 atoms, 136 A periodic box, rebuild every 20 steps",
-            "n_pairs_cutoff": n_cut, "ms_per_eval": ms,
+            "n_pairs_cutoff": n_cut, "ms_per_eval": ms, "ms_per_eval_events_every_eval": ms_all_events,
             "kernel_ms_avg": prof["ms_nonbonded_total"] / max(prof["n_evals"], 1),
             "list_build_ms": build_ms, "ms_per_step_amortised": ms + build_ms / 20.0,
             "pair_interactions_per_s": n_cut / (ms * 1e-3),
@@ -171,11 +181,20 @@ def main():
     ap.add_argument("--chunk", type=int, default=-1, help="entries per work item (tuning)")
     ap.add_argument("--no-250k", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--only", choices=("250k", "batched"), default=None,
+                    help="run one secondary section alone and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the force path shards by replica or by spatial domain, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.only == "250k":
+        print(json.dumps(bench_250k()))
+        return
+    if args.only == "batched":
+        print(json.dumps(bench_batched()))
+        return
 
     import geodesic_hip as gh
     from geodesic_hip import systems
@@ -189,7 +208,7 @@ def main():
     st = g.stats()
     n_cut, n_list = st["n_pairs_cutoff"], st["n_list_pairs"]
 
-    ms_step, prof, e = time_resident(g, args.steps, args.warmup)
+    ms_step, prof, e, ms_all_events = time_resident(g, args.steps, args.warmup)
     nb_ms = prof["ms_nonbonded_total"] / max(prof["n_evals"], 1)
     ev_ms = prof["ms_eval_total"] / max(prof["n_evals"], 1)
     achieved = n_cut * FLOP_PER_PAIR / (nb_ms * 1e-3) / 1e12
@@ -215,8 +234,10 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic_from_profiles("nonbonded_24k_traffic.json"),
                      "kernel": "k_forces (tiles + bonded workgroups)", "kernel_ms_avg": nb_ms,
-                     "kernel_ms_min": prof["ms_nonbonded_min"], "flop_per_pair": FLOP_PER_PAIR},
+                     "kernel_ms_min": prof["ms_nonbonded_min"], "flop_per_pair": FLOP_PER_PAIR,
+                     "timed_launches": prof["n_evals"], "events_every": PROFILE_EVERY},
         "device_ms_per_eval": ev_ms,
+        "ms_per_step_events_every_step": ms_all_events,
         "list_build_ms": st["ms_build"],
         "list": {k: st[k] for k in ("n_superclusters", "n_entries", "n_tiles", "n_tiles_cutoff",
                                     "n_work_items")},

@@ -170,34 +170,34 @@ gm_status compute_on_device(gm_sys *s, const double *x, const double *y, const d
                             double *fx, double *fy, double *fz) {
     if (!s->built) return set_error(GM_ERR_NOT_BUILT, "compute_forces before build_neighbor_list");
     hipStream_t st = s->stream;
-    hipEvent_t e0 = s->ev_eval0, e1 = s->ev_eval1;
-    s->cur_nb0 = s->ev_nb0;
-    s->cur_nb1 = s->ev_nb1;
-    if (s->profiling) {
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    if (s->profiling && (s->prof_seen++ % s->prof_every) == 0) {
         if (s->prof_used < kProfMax) {
-            const size_t need = 4 * (size_t)(s->prof_used + 1);
+            const size_t need = 3 * (size_t)(s->prof_used + 1);
             while (s->prof_pool.size() < need) {
                 hipEvent_t ev;
                 GM_HIP(hipEventCreate(&ev));
                 s->prof_pool.push_back(ev);
             }
-            const size_t b = 4 * (size_t)s->prof_used;
+            const size_t b = 3 * (size_t)s->prof_used;
             e0 = s->prof_pool[b];
-            s->cur_nb0 = s->prof_pool[b + 1];
-            s->cur_nb1 = s->prof_pool[b + 2];
-            e1 = s->prof_pool[b + 3];
+            e1 = s->prof_pool[b + 1];
+            e2 = s->prof_pool[b + 2];
             s->prof_used++;
         } else {
             s->prof_untimed++;
         }
     }
-    GM_HIP(hipEventRecord(e0, st));
+    // Unprofiled evaluations record no events: the stream holds only kernels.
+    if (e0) GM_HIP(hipEventRecord(e0, st));
     if (s->F_dirty) GM_HIP(launch_fill_u64(s->F.p, (size_t)s->nsc * 64 * 3, st));
     s->F_dirty = true;
     GM_HIP(launch_forces(s, x, y, z));
+    if (e1) GM_HIP(hipEventRecord(e1, st));
     GM_HIP(launch_finish(s, fx, fy, fz));  // also clears the accumulator
     s->F_dirty = false;
-    GM_HIP(hipEventRecord(e1, st));
+    if (e2) GM_HIP(hipEventRecord(e2, st));
+    s->last_timed = e0 ? s->prof_used - 1 : -1;
     return GM_OK;
 }
 
@@ -205,10 +205,14 @@ gm_status collect(gm_sys *s, gm_energy_terms *e) {
     GM_HIP(hipStreamSynchronize(s->stream));
     double t[5] = {0, 0, 0, 0, 0};
     GM_HIP(hipMemcpy(t, s->e_out.p, sizeof(t), hipMemcpyDeviceToHost));
-    float a = 0.f, b = 0.f;
-    if (hipEventElapsedTime(&a, s->ev_nb0, s->ev_nb1) == hipSuccess) s->stats.ms_nonbonded = a;
-    if (hipEventElapsedTime(&b, s->ev_eval0, s->ev_eval1) == hipSuccess) s->stats.ms_eval = b;
-    (void)hipGetLastError();  // the events are unrecorded while profiling; do not leave that error sticky
+    if (s->last_timed >= 0) {
+        const size_t k = 3 * (size_t)s->last_timed;
+        float a = 0.f, b = 0.f;
+        GM_HIP(hipEventElapsedTime(&a, s->prof_pool[k], s->prof_pool[k + 1]));
+        GM_HIP(hipEventElapsedTime(&b, s->prof_pool[k], s->prof_pool[k + 2]));
+        s->stats.ms_nonbonded = a;
+        s->stats.ms_eval = b;
+    }
     if (e) {
         e->bond = t[0];
         e->angle = t[1];
@@ -350,7 +354,7 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     } else {
         TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     }
-    for (hipEvent_t *ev : {&s->ev_eval0, &s->ev_nb0, &s->ev_nb1, &s->ev_eval1, &s->ev_b0, &s->ev_b1})
+    for (hipEvent_t *ev : {&s->ev_b0, &s->ev_b1})
         TRY(hipEventCreate(ev));
     hipStream_t st = s->stream;
     s->n = n;
@@ -403,6 +407,19 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     bp.ncx = (int)std::max(1.0, std::min(512.0, std::floor(d->box[0] / side + 0.5)));
     bp.ncy = (int)std::max(1.0, std::min(512.0, std::floor(d->box[1] / side + 0.5)));
     bp.ncol = bp.ncx * bp.ncy;
+    bp.r_skin2f = (float)bp.r_skin2;
+    bp.r_cut2f = (float)bp.r_cut2;
+    {
+        // The f32 estimate of r2 from fixed-point displacements errs by a few
+        // ulp plus 2 r (quantisation step) sqrt(3); the band is far wider.
+        const double qmax = std::max(d->box[0], std::max(d->box[1], d->box[2])) / two32;
+        bp.band_skin = (float)(1e-4 * bp.r_skin2 + 16.0 * d->r_skin * qmax);
+        bp.band_cut = (float)(1e-4 * bp.r_cut2 + 16.0 * d->r_cutoff * qmax);
+    }
+    bp.wcol[0] = (float)(d->box[0] / bp.ncx);
+    bp.wcol[1] = (float)(d->box[1] / bp.ncy);
+    bp.kx = (int)std::min<double>(bp.ncx, std::floor(bp.r_search / bp.wcol[0]) + 1.0);
+    bp.ky = (int)std::min<double>(bp.ncy, std::floor(bp.r_search / bp.wcol[1]) + 1.0);
     bp.periodic = s->periodic;
     bp.n_atoms = (int)n;
     s->max_sc = (n + 63) / 64 + bp.ncol;
@@ -453,8 +470,13 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
         nbr[k] = (uint32_t)(keys[k] & 0xffffffffu);
     }
     for (int64_t i = 0; i < n; i++) off[(size_t)i + 1] += off[(size_t)i];
+    std::vector<uint2> rng((size_t)N, make_uint2(1u, 0u));
+    for (int64_t i = 0; i < n; i++)
+        if (off[(size_t)i + 1] > off[(size_t)i])
+            rng[(size_t)i] = make_uint2(nbr[(size_t)off[(size_t)i]], nbr[(size_t)off[(size_t)i + 1] - 1]);
     TRY(upload(s->excl_off, off.data(), off.size(), st));
     TRY(upload(s->excl_nbr, nbr.data(), nbr.size(), st));
+    TRY(upload(s->excl_rng, rng.data(), rng.size(), st));
 
     for (DevBuf<double> *b : {&s->x, &s->y, &s->z, &s->refx, &s->refy, &s->refz, &s->fx, &s->fy, &s->fz,
                               &s->sx, &s->sy, &s->sz})
@@ -465,7 +487,7 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     TRY(s->bucket.ensure(N));
     TRY(s->sorted_atoms.ensure(N));
     TRY(s->atom_slot.ensure(N));
-    for (DevBuf<int> *b : {&s->sc_base, &s->sc_m, &s->sc_count}) TRY(b->ensure((size_t)s->max_sc + 1));
+    for (DevBuf<int> *b : {&s->sc_base, &s->sc_m, &s->sc_col, &s->sc_count}) TRY(b->ensure((size_t)s->max_sc + 1));
     TRY(s->meta.ensure(24));
     TRY(s->slot_atom.ensure(slots));
     TRY(s->sq.ensure(slots));
@@ -498,11 +520,12 @@ gm_status gm_system_destroy(gm_sys *s) {
                                &s->ent_imask};
     for (auto *b : u32) b->release();
     DevBuf<int> *i32[] = {&s->col_count, &s->col_start, &s->col_sc, &s->col_fill, &s->bucket,
-                          &s->sorted_atoms, &s->sc_base, &s->sc_m, &s->meta, &s->slot_atom,
+                          &s->sorted_atoms, &s->sc_base, &s->sc_m, &s->sc_col, &s->meta, &s->slot_atom,
                           &s->atom_slot, &s->sc_count, &s->sc_start, &s->ent_sc, &s->ent_cj,
                           &s->wi_sc, &s->wi_start, &s->wi_end};
     for (auto *b : i32) b->release();
     s->excl_off.release();
+    s->excl_rng.release();
     s->atom_par.release();
     s->sp.release();
     s->sq.release();
@@ -515,7 +538,7 @@ gm_status gm_system_destroy(gm_sys *s) {
     s->ent_ja.release();
     s->ent_lb.release();
     s->F.release();
-    for (hipEvent_t ev : {s->ev_eval0, s->ev_nb0, s->ev_nb1, s->ev_eval1, s->ev_b0, s->ev_b1})
+    for (hipEvent_t ev : {s->ev_b0, s->ev_b1})
         if (ev) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : s->prof_pool) (void)hipEventDestroy(ev);
     if (s->stream && s->owns_stream) (void)hipStreamDestroy(s->stream);
@@ -695,11 +718,15 @@ gm_status gm_get_pair_list(gm_sys *s, uint32_t *pi, uint32_t *pj, int64_t cap, i
     return GM_OK;
 }
 
-gm_status gm_profile_begin(gm_sys *s) {
+gm_status gm_profile_begin(gm_sys *s, int32_t every) {
     GM_REQUIRE(s, "NULL system");
+    GM_REQUIRE(every >= 1, "gm_profile_begin: every must be >= 1");
     s->profiling = true;
+    s->prof_every = every;
+    s->prof_seen = 0;
     s->prof_used = 0;
     s->prof_untimed = 0;
+    s->last_timed = -1;
     return GM_OK;
 }
 
@@ -713,8 +740,8 @@ gm_status gm_profile_end(gm_sys *s, gm_profile *out) {
     p.ms_nonbonded_max = 0.0;
     for (int64_t k = 0; k < s->prof_used; k++) {
         float nb = 0.f, ev = 0.f;
-        GM_HIP(hipEventElapsedTime(&nb, s->prof_pool[4 * (size_t)k + 1], s->prof_pool[4 * (size_t)k + 2]));
-        GM_HIP(hipEventElapsedTime(&ev, s->prof_pool[4 * (size_t)k], s->prof_pool[4 * (size_t)k + 3]));
+        GM_HIP(hipEventElapsedTime(&nb, s->prof_pool[3 * (size_t)k], s->prof_pool[3 * (size_t)k + 1]));
+        GM_HIP(hipEventElapsedTime(&ev, s->prof_pool[3 * (size_t)k], s->prof_pool[3 * (size_t)k + 2]));
         p.ms_nonbonded_total += nb;
         p.ms_eval_total += ev;
         if (k == 0 || nb < p.ms_nonbonded_min) p.ms_nonbonded_min = nb;

@@ -32,7 +32,6 @@ struct gm_batch {
     int n_bonded_items = 0;
     hipStream_t side = nullptr;  // second branch of the captured graph
     hipEvent_t fork = nullptr, join = nullptr;
-    DevBuf<unsigned int> ticket;
     DevBuf<int> atom_rep;
     DevBuf<int64_t> atom_off;
     DevBuf<double> fx, fy, fz, E;
@@ -84,7 +83,6 @@ gm_status make_plan(gm_batch *b) {
     GM_HIP(b->d_items.ensure(items.size()));
     if (!items.empty())
         GM_HIP(hipMemcpyAsync(b->d_items.p, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice, st));
-    GM_HIP(hipMemsetAsync(b->ticket.p, 0, sizeof(unsigned int), st));
     GM_HIP(hipStreamSynchronize(st));
 
     drop_graph(b);
@@ -96,11 +94,11 @@ gm_status make_plan(gm_batch *b) {
     if (e1 == hipSuccess) e1 = launch_batch_bonded(b->d_reps.p, b->d_items.p, b->n_bonded_items, b->side);
     if (e1 == hipSuccess) e1 = hipEventRecord(b->join, b->side);
     if (e1 == hipSuccess)
-        e1 = launch_batch_tiles(b->d_reps.p, b->d_items.p + b->n_bonded_items, b->n_items, b->ticket.p, grid, st);
+        e1 = launch_batch_tiles(b->d_reps.p, b->d_items.p + b->n_bonded_items, b->n_items, grid, st);
     if (e1 == hipSuccess) e1 = hipStreamWaitEvent(st, b->join, 0);
     if (e1 == hipSuccess)
         e1 = launch_batch_finish(b->d_reps.p, R, b->atom_rep.p, b->atom_off.p, b->off.back(), b->E.p,
-                                 b->fx.p, b->fy.p, b->fz.p, b->ticket.p, st);
+                                 b->fx.p, b->fy.p, b->fz.p, st);
     hipGraph_t g = nullptr;
     hipError_t e3 = hipStreamEndCapture(st, &g);
     if (e1 != hipSuccess) { if (g) (void)hipGraphDestroy(g); return from_hip(e1, "batched graph capture"); }
@@ -156,7 +154,7 @@ gm_status gm_batch_create(gm_ctx *ctx, const gm_system_desc *descs, int32_t n_re
     for (int r = 0; r < n_replicas; r++)
         for (int64_t a = b->off[(size_t)r]; a < b->off[(size_t)r + 1]; a++) rep[(size_t)a] = r;
     if ((e = b->atom_rep.ensure((size_t)std::max<int64_t>(n, 1))) != hipSuccess ||
-        (e = b->atom_off.ensure(b->off.size())) != hipSuccess || (e = b->ticket.ensure(1)) != hipSuccess ||
+        (e = b->atom_off.ensure(b->off.size())) != hipSuccess ||
         (e = b->E.ensure((size_t)n_replicas * 5)) != hipSuccess)
         return fail(from_hip(e, "hipMalloc"));
     for (DevBuf<double> *d : {&b->fx, &b->fy, &b->fz})
@@ -165,7 +163,6 @@ gm_status gm_batch_create(gm_ctx *ctx, const gm_system_desc *descs, int32_t n_re
         return fail(from_hip(e, "hipMemcpy"));
     if ((e = hipMemcpy(b->atom_off.p, b->off.data(), sizeof(int64_t) * b->off.size(), hipMemcpyHostToDevice)) != hipSuccess)
         return fail(from_hip(e, "hipMemcpy"));
-    if ((e = hipMemset(b->ticket.p, 0, sizeof(unsigned int))) != hipSuccess) return fail(from_hip(e, "hipMemset"));
     *out = b;
     return GM_OK;
 }
@@ -177,7 +174,6 @@ gm_status gm_batch_destroy(gm_batch *b) {
     for (gm_sys *s : b->reps) gm_system_destroy(s);
     b->d_reps.release();
     b->d_items.release();
-    b->ticket.release();
     b->atom_rep.release();
     b->atom_off.release();
     b->fx.release();

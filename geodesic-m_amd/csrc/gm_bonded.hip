@@ -59,15 +59,14 @@ __global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, int n, c
 // Batched end of evaluation: blocks [0, 5 R) sum replica b/5's segment b%5
 // (bond, angle, dihedral, 1-4, LJ) exactly like k_finish; the remaining
 // blocks convert every atom's force of every replica and clear the
-// accumulators; block 0 re-arms the work ticket for the next launch.
+// accumulators.
 __global__ __launch_bounds__(1024) void k_batch_finish(const ReplicaDesc *__restrict__ reps, int n_reps,
                                                        const int *__restrict__ atom_rep,
                                                        const int64_t *__restrict__ atom_off,
                                                        int64_t n_atoms, double *eout, double *fx,
-                                                       double *fy, double *fz, unsigned int *ticket) {
+                                                       double *fy, double *fz) {
     __shared__ double sh[1024];
     const int b = blockIdx.x, t = threadIdx.x;
-    if (b == 0 && t == 0) *ticket = 0u;
     if (b < 5 * n_reps) {
         const ReplicaDesc &R = reps[b / 5];
         const int seg = b % 5;
@@ -106,11 +105,9 @@ namespace gm {
 
 hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int *atom_rep,
                                const int64_t *atom_off, int64_t n_atoms, double *eout,
-                               double *fx, double *fy, double *fz, unsigned int *ticket,
-                               hipStream_t st) {
+                               double *fx, double *fy, double *fz, hipStream_t st) {
     const unsigned grid = 5u * (unsigned)n_reps + (unsigned)((n_atoms + 1023) / 1024);
-    k_batch_finish<<<grid, 1024, 0, st>>>(reps, n_reps, atom_rep, atom_off, n_atoms, eout, fx, fy,
-                                          fz, ticket);
+    k_batch_finish<<<grid, 1024, 0, st>>>(reps, n_reps, atom_rep, atom_off, n_atoms, eout, fx, fy, fz);
     return hipGetLastError();
 }
 

@@ -109,7 +109,7 @@ __global__ void k_scatter(int n, const uint32_t *q, const int *col_start, int *c
 __global__ __launch_bounds__(1024) void k_column_sort(int n, const uint32_t *q, const int *bucket,
                                                      const int *col_start, const int *col_count,
                                                      const int *col_sc, int *sorted_atoms,
-                                                     int *sc_base, int *sc_m) {
+                                                     int *sc_base, int *sc_m, int *sc_col) {
     extern __shared__ unsigned long long keys[];
     const int c = blockIdx.x;
     const int m = col_count[c];
@@ -144,6 +144,7 @@ __global__ __launch_bounds__(1024) void k_column_sort(int n, const uint32_t *q, 
     for (int k = threadIdx.x; k < nsc_c; k += blockDim.x) {
         sc_base[s0 + k] = base + 64 * k;
         sc_m[s0 + k] = min(64, m - 64 * k);
+        sc_col[s0 + k] = c;
     }
 }
 
@@ -257,12 +258,35 @@ __device__ inline bool boxes_close(int4 a, float4 ha, int4 b, float4 hb, const B
     return gx * gx + gy * gy + gz * gz <= bp.r_search * bp.r_search;
 }
 
-// One wave per supercluster: scan j-clusters cj >= first cluster of the
-// supercluster; an entry is (sc, cj) with a nonzero 8-bit i-cluster mask
-// (ci <= cj, both valid, boxes within r_skin).  Pass 0 counts, pass 1 writes
-// entries in cj order at sc_start[sc].
+// Columns of a periodic ring within `k` of `c`, as at most two ascending
+// [first, last] pieces, so that visiting them in order visits ascending indices.
+__device__ inline int ring_pieces(int c, int k, int n, int2 *pc) {
+    if (2 * k + 1 >= n) { pc[0] = make_int2(0, n - 1); return 1; }
+    const int lo = c - k, hi = c + k;
+    if (lo < 0) { pc[0] = make_int2(0, hi); pc[1] = make_int2(lo + n, n - 1); return 2; }
+    if (hi >= n) { pc[0] = make_int2(0, hi - n); pc[1] = make_int2(lo, n - 1); return 2; }
+    pc[0] = make_int2(lo, hi);
+    return 1;
+}
+
+// Smallest possible distance along one axis between atoms of two columns
+// `d` apart on a ring of `n` (quantisation error is far inside r_search's margin).
+__device__ inline float column_gap(int d, int n, float w) {
+    d = d < 0 ? -d : d;
+    d = min(d, n - d);
+    return (float)max(d - 1, 0) * w;
+}
+
+// One wave per supercluster: scan the j-clusters cj >= its first cluster that
+// lie in xy columns within reach of its own column (columns are periodic in
+// the fixed-point frame, so every pair within r_skin is in reach).  An entry
+// is (sc, cj) with a nonzero 8-bit i-cluster mask (ci <= cj, both valid,
+// boxes within r_skin).  Columns are visited in ascending index, and clusters
+// are numbered column by column, so entries come in ascending cj.  Pass 0
+// counts, pass 1 writes entries at sc_start[sc].
 template <int FILL>
-__global__ __launch_bounds__(64) void k_pair_search(const int *meta, int ncl, BuildParams bp,
+__global__ __launch_bounds__(64) void k_pair_search(const int *meta, BuildParams bp, const int *sc_col,
+                                                   const int *col_sc, const int *col_count,
                                                    const int4 *cl_ref, const float4 *cl_half,
                                                    const int4 *sc_ref, const float4 *sc_half,
                                                    int *sc_count, const int *sc_start, int *ent_sc,
@@ -270,7 +294,7 @@ __global__ __launch_bounds__(64) void k_pair_search(const int *meta, int ncl, Bu
     __shared__ int4 cir[8];
     __shared__ float4 cih[8];
     const int sc = blockIdx.x;
-    if (sc >= meta[0]) return;
+    if (meta[1] || sc >= meta[0]) return;
     const int t = threadIdx.x;
     if (t < 8) { cir[t] = cl_ref[sc * 8 + t]; cih[t] = cl_half[sc * 8 + t]; }
     __syncthreads();
@@ -278,29 +302,51 @@ __global__ __launch_bounds__(64) void k_pair_search(const int *meta, int ncl, Bu
     const float4 sh = sc_half[sc];
     const int first = sc * 8;
     const int base = FILL ? sc_start[sc] : 0;
+    const int c0 = sc_col[sc];
+    const int cx0 = c0 / bp.ncy, cy0 = c0 - (c0 / bp.ncy) * bp.ncy;
+    int2 px[2], py[2];
+    const int npx = ring_pieces(cx0, bp.kx, bp.ncx, px);
+    const int npy = ring_pieces(cy0, bp.ky, bp.ncy, py);
+    const float rs2 = bp.r_search * bp.r_search;
     int count = 0;
     const unsigned long long lt_mask = (t == 0) ? 0ull : (~0ull >> (64 - t));
-    for (int c0 = first; c0 < ncl; c0 += 64) {
-        const int cj = c0 + t;
-        uint32_t imask = 0;
-        if (cj < ncl) {
-            const int4 jr = cl_ref[cj];
-            const float4 jh = cl_half[cj];
-            if (jr.w && boxes_close(sr, sh, jr, jh, bp)) {
-                for (int ci = 0; ci < 8; ci++) {
-                    if (first + ci <= cj && cir[ci].w && boxes_close(cir[ci], cih[ci], jr, jh, bp))
-                        imask |= 1u << ci;
+    for (int a = 0; a < npx; a++) {
+        for (int cx = px[a].x; cx <= px[a].y; cx++) {
+            const float gx = column_gap(cx - cx0, bp.ncx, bp.wcol[0]);
+            for (int b = 0; b < npy; b++) {
+                for (int cy = py[b].x; cy <= py[b].y; cy++) {
+                    const float gy = column_gap(cy - cy0, bp.ncy, bp.wcol[1]);
+                    const int c = cx * bp.ncy + cy;
+                    if (c < c0 || gx * gx + gy * gy > rs2) continue;
+                    const int m = col_count[c];
+                    const int lo = (c == c0) ? first : col_sc[c] * 8;
+                    const int hi = col_sc[c] * 8 + ((m + 63) / 64) * 8;
+                    for (int j0 = lo; j0 < hi; j0 += 64) {
+                        const int cj = j0 + t;
+                        uint32_t imask = 0;
+                        if (cj < hi) {
+                            const int4 jr = cl_ref[cj];
+                            const float4 jh = cl_half[cj];
+                            if (jr.w && boxes_close(sr, sh, jr, jh, bp)) {
+                                for (int ci = 0; ci < 8; ci++) {
+                                    if (first + ci <= cj && cir[ci].w &&
+                                        boxes_close(cir[ci], cih[ci], jr, jh, bp))
+                                        imask |= 1u << ci;
+                                }
+                            }
+                        }
+                        const unsigned long long bl = __ballot(imask != 0);
+                        if (FILL && imask) {
+                            const int off = base + count + __popcll(bl & lt_mask);
+                            ent_sc[off] = sc;
+                            ent_cj[off] = cj;
+                            ent_imask[off] = imask;
+                        }
+                        count += __popcll(bl);
+                    }
                 }
             }
         }
-        const unsigned long long b = __ballot(imask != 0);
-        if (FILL && imask) {
-            const int off = base + count + __popcll(b & lt_mask);
-            ent_sc[off] = sc;
-            ent_cj[off] = cj;
-            ent_imask[off] = imask;
-        }
-        count += __popcll(b);
     }
     if (!FILL && t == 0) sc_count[sc] = count;
 }
@@ -314,61 +360,107 @@ __device__ inline bool excluded(const int64_t *off, const uint32_t *nbr, uint32_
     return a < off[lo + 1] && nbr[a] == hi;
 }
 
-// One wave per entry: exact pair masks.  Lane = (jl = lane>>3, il = lane&7).
-// The inclusion test is neighbor.rs:31-38 verbatim in f64 on the wrapped
-// positions, with i = min(atom), j = max(atom).
-__global__ __launch_bounds__(64) void k_masks(int nent, const int *ent_sc, const int *ent_cj,
-                                              uint32_t *ent_imask, unsigned long long *ent_mask,
-                                              int2 *ent_meta, int *ent_ja, uint8_t *ent_lb,
-                                              const int *slot_atom, const double *x, const double *y,
-                                              const double *z, const int64_t *excl_off,
-                                              const uint32_t *excl_nbr, BuildParams bp,
-                                              unsigned long long *counters) {
-    const int e = blockIdx.x;
-    if (e >= nent) return;
-    const int t = threadIdx.x;
+// One wavefront per entry (grid-stride over entries, four wavefronts per
+// workgroup): exact pair masks.  Lane = (jl = lane>>3, il = lane&7).
+// The inclusion test is neighbor.rs:31-38: min-image r2 <= r_skin^2 in f64 on
+// the wrapped positions, with i = min(atom), j = max(atom), then the exclusion
+// set (neighbor.rs:33-35).  r2 is first estimated in f32 from the exact
+// fixed-point min-image displacement; a pair whose estimate lies within the
+// band of a threshold is decided by the f64 expression itself, and the band
+// is many times the estimate's worst error, so every decision is the f64 one.
+// The exclusion lookup runs only for pairs inside r_skin, and only when the
+// partner lies in the atom's (smallest, largest) exclusion-partner range.
+__global__ __launch_bounds__(256) void k_masks(int nent, const int *ent_sc, const int *ent_cj,
+                                               uint32_t *ent_imask, unsigned long long *ent_mask,
+                                               int2 *ent_meta, int *ent_ja, uint8_t *ent_lb,
+                                               const int *slot_atom, const int4 *sq, const double *x,
+                                               const double *y, const double *z,
+                                               const int64_t *excl_off, const uint32_t *excl_nbr,
+                                               const uint2 *excl_rng, BuildParams bp,
+                                               unsigned long long *counters) {
+    const int t = threadIdx.x & 63;
     const int il = t & 7, jl = t >> 3;
-    const int sc = ent_sc[e], cj = ent_cj[e];
-    const uint32_t im = ent_imask[e];
-    const int b = slot_atom[(size_t)cj * 8 + jl];
-    uint32_t newmask = 0, lanebits = 0;
-    unsigned long long npairs = 0, ncut_tiles = 0, ncut_pairs = 0;
-    if (il == 0) ent_ja[(size_t)e * 8 + jl] = b;
-    for (int ci = 0; ci < 8; ci++) {
-        unsigned long long m = 0ull;
-        if (im & (1u << ci)) {
-            const int a = slot_atom[(size_t)sc * 64 + ci * 8 + il];
+    const int nwaves = (int)gridDim.x * 4;
+    unsigned long long tot_pairs = 0, tot_tiles = 0, tot_ctiles = 0, tot_cpairs = 0;
+    for (int e = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); e < nent; e += nwaves) {
+        const int sc = ent_sc[e], cj = ent_cj[e];
+        const uint32_t im = ent_imask[e];
+        const size_t js = (size_t)cj * 8 + jl;
+        const int b = slot_atom[js];
+        const int4 qb = sq[js];
+        if (il == 0) ent_ja[(size_t)e * 8 + jl] = b;
+        int a[8];
+        int4 qa[8];
+#pragma unroll
+        for (int ci = 0; ci < 8; ci++) {
+            a[ci] = -1;
+            qa[ci] = make_int4(0, 0, 0, 0);
+            if (im & (1u << ci)) {
+                const size_t is = (size_t)sc * 64 + ci * 8 + il;
+                a[ci] = slot_atom[is];
+                qa[ci] = sq[is];
+            }
+        }
+        uint32_t inb = 0, cutb = 0;
+#pragma unroll
+        for (int ci = 0; ci < 8; ci++) {
             const int cg = sc * 8 + ci;
-            bool in = a >= 0 && b >= 0 && (cg < cj || il < jl), cut = false;
-            if (in) {
-                const uint32_t lo = (uint32_t)min(a, b), hi = (uint32_t)max(a, b);
-                in = !excluded(excl_off, excl_nbr, lo, hi);
-                if (in) {
-                    double dx = min_image(x[hi] - x[lo], bp.box[0]);
-                    double dy = min_image(y[hi] - y[lo], bp.box[1]);
-                    double dz = min_image(z[hi] - z[lo], bp.box[2]);
-                    double r2 = dx * dx + dy * dy + dz * dz;
+            if (a[ci] >= 0 && b >= 0 && (cg < cj || il < jl)) {
+                const float dx = (float)wrapdiff(qb.x, qa[ci].x) * bp.sq[0];
+                const float dy = (float)wrapdiff(qb.y, qa[ci].y) * bp.sq[1];
+                const float dz = (float)wrapdiff(qb.z, qa[ci].z) * bp.sq[2];
+                const float r2f = dx * dx + dy * dy + dz * dz;
+                bool in = r2f <= bp.r_skin2f, cut = r2f <= bp.r_cut2f;
+                if (fabsf(r2f - bp.r_skin2f) <= bp.band_skin || fabsf(r2f - bp.r_cut2f) <= bp.band_cut) {
+                    const uint32_t lo = (uint32_t)min(a[ci], b), hi = (uint32_t)max(a[ci], b);
+                    const double ddx = min_image(x[hi] - x[lo], bp.box[0]);
+                    const double ddy = min_image(y[hi] - y[lo], bp.box[1]);
+                    const double ddz = min_image(z[hi] - z[lo], bp.box[2]);
+                    const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
                     in = r2 <= bp.r_skin2;
                     cut = r2 <= bp.r_cut2;
                 }
+                inb |= in ? (1u << ci) : 0u;
+                cutb |= (in && cut) ? (1u << ci) : 0u;
             }
-            m = __ballot(in);
-            const unsigned long long mc = __ballot(in && cut);
+        }
+        for (int ci = 0; ci < 8; ci++) {
+            if (inb & (1u << ci)) {
+                const uint32_t lo = (uint32_t)min(a[ci], b), hi = (uint32_t)max(a[ci], b);
+                const uint2 rg = excl_rng[lo];
+                if (hi >= rg.x && hi <= rg.y && excluded(excl_off, excl_nbr, lo, hi)) {
+                    inb &= ~(1u << ci);
+                    cutb &= ~(1u << ci);
+                }
+            }
+        }
+        uint32_t newmask = 0;
+        unsigned long long npairs = 0, ncut_tiles = 0, ncut_pairs = 0;
+#pragma unroll
+        for (int ci = 0; ci < 8; ci++) {
+            const unsigned long long m = __ballot((inb >> ci) & 1u);
+            const unsigned long long mc = __ballot((cutb >> ci) & 1u);
+            if (t == 0) ent_mask[(size_t)e * 8 + ci] = m;
+            if (m) { newmask |= 1u << ci; npairs += __popcll(m); }
             if (mc) ncut_tiles++;
             ncut_pairs += __popcll(mc);
-            lanebits |= in ? (1u << ci) : 0u;
         }
-        if (t == 0) ent_mask[(size_t)e * 8 + ci] = m;
-        if (m) { newmask |= 1u << ci; npairs += __popcll(m); }
+        ent_lb[(size_t)e * 64 + t] = (uint8_t)inb;
+        if (t == 0) {
+            ent_imask[e] = newmask;
+            ent_meta[e] = make_int2(cj, (int)newmask);
+        }
+        tot_pairs += npairs;
+        tot_tiles += (unsigned long long)__popc(newmask);
+        tot_ctiles += ncut_tiles;
+        tot_cpairs += ncut_pairs;
     }
-    ent_lb[(size_t)e * 64 + t] = (uint8_t)lanebits;
-    if (t == 0) {
-        ent_imask[e] = newmask;
-        ent_meta[e] = make_int2(cj, (int)newmask);
-        atomicAdd(&counters[0], npairs);
-        atomicAdd(&counters[1], (unsigned long long)__popc(newmask));
-        atomicAdd(&counters[2], ncut_tiles);
-        atomicAdd(&counters[3], ncut_pairs);
+    // one set of counter atomics per wavefront, not per entry
+    if (t == 0 && tot_tiles) {
+        atomicAdd(&counters[0], tot_pairs);
+        atomicAdd(&counters[1], tot_tiles);
+        atomicAdd(&counters[2], tot_ctiles);
+        atomicAdd(&counters[3], tot_cpairs);
     }
 }
 
@@ -423,7 +515,7 @@ hipError_t launch_build(gm_sys *s, bool wrap) {
                                                       s->bucket.p, bp);
     k_column_sort<<<bp.ncol, 1024, kColumnCap * sizeof(unsigned long long), st>>>(
         n, s->qatom.p, s->bucket.p, s->col_start.p, s->col_count.p, s->col_sc.p,
-        s->sorted_atoms.p, s->sc_base.p, s->sc_m.p);
+        s->sorted_atoms.p, s->sc_base.p, s->sc_m.p, s->sc_col.p);
     if ((e = launch_fill_i32(s->slot_atom.p, (size_t)s->max_sc * 64, -1, st)) != hipSuccess) return e;
     k_sc_split<<<(unsigned)s->max_sc, 64, 0, st>>>(s->meta.p, n, s->qatom.p, s->sorted_atoms.p,
                                                    s->sc_base.p, s->sc_m.p, s->slot_atom.p);
@@ -438,8 +530,8 @@ hipError_t launch_build(gm_sys *s, bool wrap) {
 hipError_t launch_pair_count(gm_sys *s) {
     if (s->nsc == 0) return hipSuccess;
     k_pair_search<0><<<(unsigned)s->nsc, 64, 0, s->stream>>>(
-        s->meta.p, (int)s->ncl, s->bp, s->cl_ref.p, s->cl_half.p, s->sc_ref.p, s->sc_half.p,
-        s->sc_count.p, nullptr, nullptr, nullptr, nullptr);
+        s->meta.p, s->bp, s->sc_col.p, s->col_sc.p, s->col_count.p, s->cl_ref.p, s->cl_half.p,
+        s->sc_ref.p, s->sc_half.p, s->sc_count.p, nullptr, nullptr, nullptr, nullptr);
     return hipGetLastError();
 }
 
@@ -447,16 +539,17 @@ hipError_t launch_pair_fill_and_masks(gm_sys *s) {
     hipStream_t st = s->stream;
     if (s->nsc > 0)
         k_pair_search<1><<<(unsigned)s->nsc, 64, 0, st>>>(
-        s->meta.p, (int)s->ncl, s->bp, s->cl_ref.p, s->cl_half.p, s->sc_ref.p, s->sc_half.p,
-        nullptr, s->sc_start.p, s->ent_sc.p, s->ent_cj.p, s->ent_imask.p);
+            s->meta.p, s->bp, s->sc_col.p, s->col_sc.p, s->col_count.p, s->cl_ref.p, s->cl_half.p,
+            s->sc_ref.p, s->sc_half.p, nullptr, s->sc_start.p, s->ent_sc.p, s->ent_cj.p,
+            s->ent_imask.p);
     hipError_t e = launch_fill_u64((unsigned long long *)(s->meta.p + 8), 4, st);
     if (e != hipSuccess) return e;
     if (s->nent > 0)
-        k_masks<<<(unsigned)s->nent, 64, 0, st>>>((int)s->nent, s->ent_sc.p, s->ent_cj.p,
+        k_masks<<<(unsigned)std::min<int64_t>((s->nent + 3) / 4, 2048), 256, 0, st>>>((int)s->nent, s->ent_sc.p, s->ent_cj.p,
                                                   s->ent_imask.p, s->ent_mask.p, s->ent_meta.p,
                                                   s->ent_ja.p, s->ent_lb.p, s->slot_atom.p,
-                                                  s->x.p, s->y.p, s->z.p, s->excl_off.p,
-                                                  s->excl_nbr.p, s->bp,
+                                                  s->sq.p, s->x.p, s->y.p, s->z.p, s->excl_off.p,
+                                                  s->excl_nbr.p, s->excl_rng.p, s->bp,
                                                   (unsigned long long *)(s->meta.p + 8));
     return hipGetLastError();
 }

@@ -48,6 +48,10 @@ struct BuildParams {
     double r_skin2;     // r_skin^2 (f64, decision of neighbor.rs:38)
     double r_cut2;      // r_cutoff^2 (statistics only)
     float r_search;     // r_skin + margin, bounding-box search radius
+    float r_skin2f, r_cut2f;   // f32 thresholds of the pair-mask pre-test
+    float band_skin, band_cut; // |r2_f32 - threshold| at or below which f64 decides
+    float wcol[2];      // column widths in x and y, A
+    int kx, ky;         // column reach: columns more than k apart hold no pair within r_search
     int ncx, ncy, ncol;
     int periodic;
     int n_atoms;
@@ -90,7 +94,6 @@ struct gm_sys {
     gm_ctx *ctx = nullptr;
     hipStream_t stream = nullptr;
     bool owns_stream = true;
-    hipEvent_t ev_eval0 = nullptr, ev_nb0 = nullptr, ev_nb1 = nullptr, ev_eval1 = nullptr;
     hipEvent_t ev_b0 = nullptr, ev_b1 = nullptr;
 
     // parameters
@@ -110,6 +113,7 @@ struct gm_sys {
     DevBuf<double> bk, br0, akth, ath0, dkphi, ddelta;
     DevBuf<int64_t> excl_off;
     DevBuf<uint32_t> excl_nbr;
+    DevBuf<uint2> excl_rng;  // per atom (smallest, largest) exclusion partner above it; (1, 0) if none
 
     // working positions (device, original order) and rebuild reference
     DevBuf<double> x, y, z, refx, refy, refz;
@@ -118,7 +122,7 @@ struct gm_sys {
 
     // spatial sort
     DevBuf<int> col_count, col_start, col_sc, col_fill, bucket, sorted_atoms;
-    DevBuf<int> sc_base, sc_m;
+    DevBuf<int> sc_base, sc_m, sc_col;  // per supercluster: first sorted atom, atom count, column
     DevBuf<uint32_t> qatom;  // 3*n fixed-point coordinates, atom order
     DevBuf<int> meta;        // [0] nsc, [1] overflow, [2] max column, [5] rebuild flag;
                              // ints 8..15 hold u64 counters: list pairs, tiles, tiles with a pair
@@ -152,11 +156,13 @@ struct gm_sys {
     DevBuf<double> e_wi;           // per work item LJ energies
     DevBuf<double> e_out;          // 5 reduced terms
 
-    // profiling: event pairs per evaluation (nonbonded start/end, eval start/end)
+    // profiling: three events per timed evaluation (start, nonbonded end, end);
+    // one evaluation in every prof_every is timed
     bool profiling = false;
     std::vector<hipEvent_t> prof_pool;
-    int64_t prof_used = 0, prof_untimed = 0;
-    hipEvent_t cur_nb0 = nullptr, cur_nb1 = nullptr;
+    int64_t prof_used = 0, prof_untimed = 0, prof_seen = 0;
+    int prof_every = 1;
+    int64_t last_timed = -1;  // pool index of the last profiled evaluation, -1 if unprofiled
     std::vector<int> sc_start_host;
 
     bool built = false;
@@ -201,12 +207,12 @@ struct TermArgs;
 struct TileArgs;
 TermArgs term_args(gm_sys *s, const double *x, const double *y, const double *z);
 TileArgs tile_args(gm_sys *s, const double *x, const double *y, const double *z);
-hipError_t launch_batch_tiles(const ReplicaDesc *reps, const int2 *items, int n_items,
-                              unsigned int *ticket, int grid, hipStream_t st);
+hipError_t launch_batch_tiles(const ReplicaDesc *reps, const int2 *items, int n_items, int grid,
+                              hipStream_t st);
 hipError_t launch_batch_bonded(const ReplicaDesc *reps, const int2 *items, int n_items, hipStream_t st);
 hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int *atom_rep,
                                const int64_t *atom_off, int64_t n_atoms, double *eout,
-                               double *fx, double *fy, double *fz, unsigned int *ticket,
+                               double *fx, double *fy, double *fz,
                                hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long *p, size_t n, hipStream_t st);
 hipError_t launch_fill_i32(int *p, size_t n, int v, hipStream_t st);

@@ -253,21 +253,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
     tile_work<CUBIC>((int)blockIdx.x - n_bonded_groups, lane, A);
 }
 
-// Persistent batched tile kernel: a fixed grid of wavefronts takes tile work
-// items (replica, work item) from an atomic ticket until the queue is empty.
+// Persistent batched tile kernel: a fixed grid of wavefronts walks the tile
+// work items (replica, work item) in a static order, with no shared counter.
+// Workgroups are dealt to the 8 XCDs in turn, so wavefront w runs on XCD
+// w % 8; XCD x owns the contiguous item range [x n / 8, (x + 1) n / 8), which
+// keeps neighbouring items (same replica, nearby superclusters) in one L2.
 // Each item's arithmetic is the single-system kernel's, so results do not
 // depend on which wavefront takes which item.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_batch_tiles(
-    const ReplicaDesc *__restrict__ reps, const int2 *__restrict__ items, int n_items,
-    unsigned int *__restrict__ ticket) {
+    const ReplicaDesc *__restrict__ reps, const int2 *__restrict__ items, int n_items) {
     const int lane = threadIdx.x;
-    __shared__ int s_item;
-    for (;;) {
-        if (lane == 0) s_item = (int)atomicAdd(ticket, 1u);
-        __syncthreads();
-        const int it = s_item;
-        __syncthreads();
-        if (it >= n_items) break;
+    const int w = blockIdx.x, nw = gridDim.x;
+    const int x = w & 7, l = w >> 3;
+    const int nwx = (nw - x + 7) >> 3;  // wavefronts on XCD x
+    const int lo = (int)(((int64_t)n_items * x) >> 3), hi = (int)(((int64_t)n_items * (x + 1)) >> 3);
+    for (int it = lo + l; it < hi; it += nwx) {
         const int2 rw = items[it];
         tile_work<true>(rw.y, lane, reps[rw.x].tile);  // replicas are cubic
     }
@@ -329,20 +329,16 @@ hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const doub
     const TileArgs A = tile_args(s, x, y, z);
     const int64_t nterms = ta.nb + ta.na + ta.nd + ta.n14;
     const int64_t grid = s->n_wi + (nterms + 63) / 64;
-    hipError_t e = hipEventRecord(s->cur_nb0, s->stream);
-    if (e != hipSuccess) return e;
     if (grid > 0) {
         auto kern = A.cubic ? k_forces<true> : k_forces<false>;
         kern<<<(unsigned)grid, 64, 0, s->stream>>>(A, ta);
     }
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return hipEventRecord(s->cur_nb1, s->stream);
+    return hipGetLastError();
 }
 
-hipError_t launch_batch_tiles(const ReplicaDesc *reps, const int2 *items, int n_items,
-                              unsigned int *ticket, int grid, hipStream_t st) {
-    if (n_items > 0 && grid > 0) k_batch_tiles<<<(unsigned)grid, 64, 0, st>>>(reps, items, n_items, ticket);
+hipError_t launch_batch_tiles(const ReplicaDesc *reps, const int2 *items, int n_items, int grid,
+                              hipStream_t st) {
+    if (n_items > 0 && grid > 0) k_batch_tiles<<<(unsigned)grid, 64, 0, st>>>(reps, items, n_items);
     return hipGetLastError();
 }
 

@@ -109,7 +109,7 @@ def lib():
             "gm_compute_forces_device": ([_P, _P, _P, _P, _P, _P, _P], C.c_int),
             "gm_sync": ([_P, C.POINTER(EnergyTerms)], C.c_int),
             "gm_get_stats": ([_P, C.POINTER(Stats)], C.c_int),
-            "gm_profile_begin": ([_P], C.c_int),
+            "gm_profile_begin": ([_P, C.c_int32], C.c_int),
             "gm_profile_end": ([_P, C.POINTER(Profile)], C.c_int),
             "gm_set_work_chunk": ([_P, C.c_int32], C.c_int),
             "gm_set_positions": ([_P, _P, _P, _P], C.c_int),
@@ -373,8 +373,9 @@ class System:
         _check(lib().gm_sync(self.h, C.byref(e)))
         return e
 
-    def profile_begin(self):
-        _check(lib().gm_profile_begin(self.h))
+    def profile_begin(self, every=1):
+        """Time the next evaluation and every `every`-th one after it with HIP events."""
+        _check(lib().gm_profile_begin(self.h, int(every)))
 
     def profile_end(self):
         p = Profile()

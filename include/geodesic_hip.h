@@ -161,15 +161,17 @@ typedef struct {
     int64_t n_list_pairs;      /* pairs in the Verlet list */
     int64_t n_pairs_cutoff;    /* of those, pairs with r <= r_cutoff (f64) at the build positions */
     int64_t n_work_items;      /* waves launched by the nonbonded kernel */
-    float ms_nonbonded;        /* last nonbonded kernel, hipEvent timing */
-    float ms_eval;             /* last whole evaluation (device time) */
+    float ms_nonbonded;        /* nonbonded kernel of the last profiled evaluation (hipEvent) */
+    float ms_eval;             /* last profiled evaluation, whole (device time) */
     float ms_build;            /* last list build (device time) */
 } gm_stats;
 gm_status gm_get_stats(gm_sys *sys, gm_stats *out);
 
 /* Per-evaluation device timings, recorded with HIP events on the system's
- * stream between gm_profile_begin and gm_profile_end (at most 8192
- * evaluations are timed; later ones are counted in n_untimed). */
+ * stream between gm_profile_begin and gm_profile_end: the first evaluation
+ * and every `every`-th one after it are timed (at most 8192; later ones are
+ * counted in n_untimed), the others record nothing.  Outside a profiling
+ * session an evaluation records no events at all. */
 typedef struct {
     int64_t n_evals;            /* evaluations timed */
     int64_t n_untimed;          /* evaluations past the event pool */
@@ -177,7 +179,7 @@ typedef struct {
     double ms_nonbonded_min, ms_nonbonded_max;
     double ms_eval_total;       /* sum over timed evaluations, whole evaluation */
 } gm_profile;
-gm_status gm_profile_begin(gm_sys *sys);
+gm_status gm_profile_begin(gm_sys *sys, int32_t every);
 gm_status gm_profile_end(gm_sys *sys, gm_profile *out);
 
 /* Tuning: list entries per nonbonded work item (one wavefront each, at most
