@@ -209,6 +209,15 @@ def main():
     n_cut, n_list = st["n_pairs_cutoff"], st["n_list_pairs"]
 
     ms_step, prof, e, ms_all_events = time_resident(g, args.steps, args.warmup)
+    # PCIe-inclusive rate of the host-buffer entry point (positions in, forces out)
+    hx, hy, hz = s.x.copy(), s.y.copy(), s.z.copy()
+    for _ in range(5):
+        g.compute_forces(hx, hy, hz)
+    n_host = 100
+    t0 = time.perf_counter()
+    for _ in range(n_host):
+        g.compute_forces(hx, hy, hz)
+    ms_host = (time.perf_counter() - t0) * 1e3 / n_host
     nb_ms = prof["ms_nonbonded_total"] / max(prof["n_evals"], 1)
     ev_ms = prof["ms_eval_total"] / max(prof["n_evals"], 1)
     achieved = n_cut * FLOP_PER_PAIR / (nb_ms * 1e-3) / 1e12
@@ -238,6 +247,7 @@ def main():
                      "timed_launches": prof["n_evals"], "events_every": PROFILE_EVERY},
         "device_ms_per_eval": ev_ms,
         "ms_per_step_events_every_step": ms_all_events,
+        "ms_per_eval_host_buffers": ms_host,
         "list_build_ms": st["ms_build"],
         "list": {k: st[k] for k in ("n_superclusters", "n_entries", "n_tiles", "n_tiles_cutoff",
                                     "n_work_items")},
