@@ -172,6 +172,90 @@ def bench_batched(n_replicas=512, steps=20):
             "setup_s": {"generate": t_gen, "build_lists": t_build}}
 
 
+def bench_surrogate(frames=20000, steps=400, traj_cycles=25, min_steps=300):
+    """Shell-force surrogate on the 24k system with LJ switched over 7.5-12 A
+    (r_c 12, list 13.5): steepest descent, then training on the GPU along a
+    seeded Brownian trajectory; held-out loss against the plain multiple-time-
+    step method (shell force held from the last exact step); force error along
+    a Brownian trajectory driven by the multiple-time-step forces; throughput
+    of the 4:1 driver against the exact model evaluated every step."""
+    import geodesic_hip as gh
+    from geodesic_hip import systems
+    s = systems.system_24k()
+    p = gh.Params(box=s.params.box, periodic=True, r_cutoff=12.0, r_switch=7.5, r_skin=13.5,
+                  scale_14=s.params.scale_14, bonded_min_image=s.params.bonded_min_image)
+    sur = gh.Surrogate(s.sigma, s.epsilon, s.topo, p, s.types % 4, seed=5)
+    d = tuple(gh.DeviceArray.from_host(a) for a in (s.x, s.y, s.z))
+    t0 = time.perf_counter()
+    fmax = sur.minimize(*d, steps=min_steps)
+    min_s = time.perf_counter() - t0
+    relaxed = [a.to_host() for a in d]
+    dev = lambda: tuple(gh.DeviceArray.from_host(a) for a in relaxed)  # noqa: E731
+    hist, rep = sur.train(*d, n_frames=frames, seed=101, lr=1e-3, batch=8192)
+    _, held = sur.train(*dev(), n_frames=300, seed=202, lr=0.0, batch=8192, update=False)
+    n = s.n
+    # force error at inner steps along a Brownian trajectory driven by the
+    # multiple-time-step forces, against the exact model on the GPU
+    ex = sur.system(0)
+    f = [gh.DeviceArray(n) for _ in range(3)]
+    g = [gh.DeviceArray(n) for _ in range(3)]
+    errs = {}
+    modes = {"network": gh.MTS_NET, "held_shell_force": gh.MTS_HELD, "taylor": gh.MTS_TAYLOR}
+    for name, mode in modes.items():
+        q = dev()
+        sur.build(*q)
+        sur.mts_reset()
+        num = den = 0.0
+        for k in range(4 * traj_cycles):
+            if k % 4 == 0:
+                sur.maybe_rebuild(*q)
+            if sur.mts_step(*q, *f, mode=mode) is False:
+                ex.compute_forces_device(*(a.ptr for a in q), *(a.ptr for a in g))
+                ex.sync()
+                fm = np.stack([a.to_host() for a in f])
+                fe = np.stack([a.to_host() for a in g])
+                num += float(np.sum((fm - fe) ** 2))
+                den += float(np.sum(fe ** 2))
+            sur.bd_step(*q, *f, seed=303, step=k)
+        errs[name] = (num / den) ** 0.5
+    # throughput at fixed positions: exact model every step vs the 4:1 driver
+    q = dev()
+    sur.build(*q)
+    st = ex.stats()
+    for _ in range(8):
+        ex.compute_forces_device(*(a.ptr for a in q), *(a.ptr for a in f))
+    ex.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ex.compute_forces_device(*(a.ptr for a in q), *(a.ptr for a in f))
+    ex.sync()
+    plain_ms = (time.perf_counter() - t0) * 1e3 / steps
+    sur.mts_reset()
+    for _ in range(8):
+        sur.mts_step(*q, *f)
+    sur.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sur.mts_step(*q, *f)
+    sur.sync()
+    mts_ms = (time.perf_counter() - t0) * 1e3 / steps
+    n_cut = st["n_pairs_cutoff"]
+    return {"workload": "24,000-atom system, LJ switched 7.5-12 A (r_c 12, list 13.5); inner model r_c 7.5 "
+                        "switched from 6.5 (list 9.0); 96 radial features, 96-128-128-3 SiLU, BF16 MFMA; 4:1 MTS",
+            "relax": {"steps": min_steps, "seconds": min_s, "max_force_after": fmax},
+            "train": {"frames": rep["frames"], "steps": rep["steps"], "seconds": rep["ms_total"] / 1e3,
+                      "trajectory": "Brownian, 300 K, 0.02 A noise per coordinate per step",
+                      "loss_first": rep["loss_first"], "loss_last": rep["loss_last"],
+                      "loss_mean_network": rep["loss_net"], "loss_mean_held_shell": rep["loss_stale"],
+                      "loss_mean_taylor": rep["loss_taylor"]},
+            "held_out": {"frames": held["frames"], "loss_network": held["loss_net"],
+                         "loss_held_shell": held["loss_stale"], "loss_taylor": held["loss_taylor"]},
+            "force_rms_rel_err_inner_steps": dict(errs, against="exact model on the GPU (itself within 1e-5 of f64)"),
+            "n_pairs_cutoff_exact": n_cut, "ms_per_step_exact_every_step": plain_ms, "ms_per_step_mts": mts_ms,
+            "speedup_vs_plain": plain_ms / mts_ms,
+            "pair_interactions_per_s_equivalent": n_cut / (mts_ms * 1e-3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,7 +265,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=-1, help="entries per work item (tuning)")
     ap.add_argument("--no-250k", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--only", choices=("250k", "batched"), default=None,
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--only", choices=("250k", "batched", "surrogate"), default=None,
                     help="run one secondary section alone and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -194,6 +279,9 @@ def main():
         return
     if args.only == "batched":
         print(json.dumps(bench_batched()))
+        return
+    if args.only == "surrogate":
+        print(json.dumps(bench_surrogate()))
         return
 
     import geodesic_hip as gh
@@ -258,6 +346,8 @@ def main():
     if not args.no_batched:
         bb = bench_batched()
         out["batched"] = bb
+    if not args.no_surrogate:
+        out["surrogate"] = bench_surrogate()
     if not args.no_cpu_baseline:
         cb = cpu_baseline(s, n_cut)
         out["cpu_baseline"] = cb

@@ -167,7 +167,7 @@ gm_status build_on_device(gm_sys *s, bool wrap) {
 }
 
 gm_status compute_on_device(gm_sys *s, const double *x, const double *y, const double *z,
-                            double *fx, double *fy, double *fz) {
+                            double *fx, double *fy, double *fz, bool shell) {
     if (!s->built) return set_error(GM_ERR_NOT_BUILT, "compute_forces before build_neighbor_list");
     hipStream_t st = s->stream;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
@@ -192,9 +192,9 @@ gm_status compute_on_device(gm_sys *s, const double *x, const double *y, const d
     if (e0) GM_HIP(hipEventRecord(e0, st));
     if (s->F_dirty) GM_HIP(launch_fill_u64(s->F.p, (size_t)s->nsc * 64 * 3, st));
     s->F_dirty = true;
-    GM_HIP(launch_forces(s, x, y, z));
+    GM_HIP(launch_forces(s, x, y, z, shell));
     if (e1) GM_HIP(hipEventRecord(e1, st));
-    GM_HIP(launch_finish(s, fx, fy, fz));  // also clears the accumulator
+    GM_HIP(launch_finish(s, fx, fy, fz, shell));  // also clears the accumulator
     s->F_dirty = false;
     if (e2) GM_HIP(hipEventRecord(e2, st));
     s->last_timed = e0 ? s->prof_used - 1 : -1;
@@ -369,7 +369,6 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     s->na = d->n_angles;
     s->nd = d->n_dihedrals;
     s->n14 = d->n_pairs14;
-    if (const char *ck = std::getenv("GM_NB_CHUNK")) s->chunk = std::atoi(ck);
 
     const double two32 = 4294967296.0;
     s->nbp.sx = (float)(d->box[0] / two32);
@@ -477,6 +476,15 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     TRY(upload(s->excl_off, off.data(), off.size(), st));
     TRY(upload(s->excl_nbr, nbr.data(), nbr.size(), st));
     TRY(upload(s->excl_rng, rng.data(), rng.size(), st));
+    std::vector<uint2> rng_any((size_t)N, make_uint2(0xffffffffu, 0u));
+    for (uint64_t k : keys) {
+        const uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)(k & 0xffffffffu);
+        rng_any[a] = make_uint2(std::min(rng_any[a].x, b), std::max(rng_any[a].y, b));
+        rng_any[b] = make_uint2(std::min(rng_any[b].x, a), std::max(rng_any[b].y, a));
+    }
+    for (uint2 &r : rng_any)
+        if (r.x > r.y) r = make_uint2(1u, 0u);
+    TRY(upload(s->excl_rng_any, rng_any.data(), rng_any.size(), st));
 
     for (DevBuf<double> *b : {&s->x, &s->y, &s->z, &s->refx, &s->refy, &s->refz, &s->fx, &s->fy, &s->fz,
                               &s->sx, &s->sy, &s->sz})
@@ -526,6 +534,7 @@ gm_status gm_system_destroy(gm_sys *s) {
     for (auto *b : i32) b->release();
     s->excl_off.release();
     s->excl_rng.release();
+    s->excl_rng_any.release();
     s->atom_par.release();
     s->sp.release();
     s->sq.release();
@@ -600,6 +609,20 @@ gm_status gm_needs_rebuild(gm_sys *s, const double *x, const double *y, const do
         GM_HIP(hipMemcpyAsync(s->sz.p, z, bytes, hipMemcpyHostToDevice, st));
     }
     GM_HIP(launch_needs_rebuild(s, s->sx.p, s->sy.p, s->sz.p));
+    int flag = 0;
+    GM_HIP(hipMemcpyAsync(&flag, s->meta.p + 5, sizeof(int), hipMemcpyDeviceToHost, st));
+    GM_HIP(hipStreamSynchronize(st));
+    *out = flag ? 1 : 0;
+    return GM_OK;
+}
+
+gm_status gm_needs_rebuild_device(gm_sys *s, const double *dx, const double *dy, const double *dz,
+                                  int32_t *out) {
+    GM_REQUIRE(s && out, "NULL argument");
+    GM_REQUIRE(s->n == 0 || (dx && dy && dz), "NULL positions");
+    if (!s->built) return set_error(GM_ERR_NOT_BUILT, "needs_rebuild before build_neighbor_list");
+    hipStream_t st = s->stream;
+    GM_HIP(launch_needs_rebuild(s, dx, dy, dz));
     int flag = 0;
     GM_HIP(hipMemcpyAsync(&flag, s->meta.p + 5, sizeof(int), hipMemcpyDeviceToHost, st));
     GM_HIP(hipStreamSynchronize(st));

@@ -6,13 +6,13 @@
 // the batch's stream.  After the lists are built, a device table of replica
 // descriptors and two item lists (replica, bonded group) and (replica, tile
 // work item) are written.  k_batch_bonded runs the bonded terms on a side
-// branch while the persistent k_batch_tiles drains the tile queue with a
-// fixed grid of wavefronts; k_batch_finish then reduces per-replica f64
-// energies, writes the concatenated f64 forces and re-arms the queue.  The
-// three kernels are captured once per list build into a hipGraph, so a
-// batched evaluation is one graph launch.  Every item runs the single-system arithmetic, so each replica's
-// forces and energies are bit-identical to evaluating it alone with the same
-// work shape.
+// branch while the persistent k_batch_tiles walks the tile items with a
+// fixed grid of wavefronts (static, XCD-contiguous ranges); k_batch_finish
+// then reduces per-replica f64 energies and writes the concatenated f64
+// forces.  The three kernels are captured once per list build into a
+// hipGraph, so a batched evaluation is one graph launch.  Every item runs the
+// single-system arithmetic, so each replica's forces and energies are
+// bit-identical to evaluating it alone with the same work shape.
 #include <algorithm>
 #include <cstring>
 #include <vector>

@@ -111,14 +111,15 @@ hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int *a
     return hipGetLastError();
 }
 
-hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz) {
+hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz, bool lj_only) {
     Segs sg{};
     const double *eb = s->e_items.p;
     const int64_t n14 = (s->scale14 != 0.0) ? s->n14 : 0;
-    sg.ptr[0] = eb;                         sg.len[0] = s->nb;
-    sg.ptr[1] = eb + s->nb;                 sg.len[1] = s->na;
-    sg.ptr[2] = eb + s->nb + s->na;         sg.len[2] = s->nd;
-    sg.ptr[3] = eb + s->nb + s->na + s->nd; sg.len[3] = n14;
+    const int64_t nb = lj_only ? 0 : s->nb, na = lj_only ? 0 : s->na, nd = lj_only ? 0 : s->nd;
+    sg.ptr[0] = eb;                         sg.len[0] = nb;
+    sg.ptr[1] = eb + s->nb;                 sg.len[1] = na;
+    sg.ptr[2] = eb + s->nb + s->na;         sg.len[2] = nd;
+    sg.ptr[3] = eb + s->nb + s->na + s->nd; sg.len[3] = lj_only ? 0 : n14;
     sg.ptr[4] = s->e_wi.p;                  sg.len[4] = s->n_wi;
     sg.nseg = 5;
     const unsigned grid = 5u + (unsigned)((s->n + 1023) / 1024);

@@ -38,6 +38,10 @@ struct NbParams {
     float inv_s;        // 2^32 / box_len
     float rc2_u, rsw2_u, rsw_u, inv_w_u, ds_coef_u;
     float fscale_u;     // force (kcal/mol per step) -> 2^-32 kcal/mol/A units
+    // shell mode: the pair term is weighted by S(r) - S_in(r), S_in the same
+    // quintic switch over [rin_sw, rin_sw + 1 / inv_w_in] (plain and step units)
+    float rin_sw, inv_w_in, ds_coef_in;
+    float rin_sw_u, inv_w_in_u, ds_coef_in_u;
 };
 
 // Per-system constants for the list build.
@@ -114,6 +118,7 @@ struct gm_sys {
     DevBuf<int64_t> excl_off;
     DevBuf<uint32_t> excl_nbr;
     DevBuf<uint2> excl_rng;  // per atom (smallest, largest) exclusion partner above it; (1, 0) if none
+    DevBuf<uint2> excl_rng_any;  // per atom (smallest, largest) exclusion partner, either side; (1, 0) if none
 
     // working positions (device, original order) and rebuild reference
     DevBuf<double> x, y, z, refx, refy, refz;
@@ -163,6 +168,7 @@ struct gm_sys {
     int64_t prof_used = 0, prof_untimed = 0, prof_seen = 0;
     int prof_every = 1;
     int64_t last_timed = -1;  // pool index of the last profiled evaluation, -1 if unprofiled
+    gm::NbParams nbp_shell{}; // shell-mode constants (set by a surrogate that owns this system)
     std::vector<int> sc_start_host;
 
     bool built = false;
@@ -186,6 +192,11 @@ gm_status from_hip(hipError_t e, const char *what);
 namespace gm {
 gm_status system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream_t ext_stream, gm_sys **out);
 gm_status build_on_device(gm_sys *s, bool wrap);
+// Forces at device positions into device arrays (f64, atom order), on the
+// system's stream.  shell: LJ only, each pair weighted by S(r) - S_in(r)
+// (nbp_shell), no bonded terms.
+gm_status compute_on_device(gm_sys *s, const double *x, const double *y, const double *z, double *fx,
+                            double *fy, double *fz, bool shell = false);
 gm_status make_work_items(gm_sys *s);
 }  // namespace gm
 
@@ -197,9 +208,11 @@ hipError_t launch_pair_count(gm_sys *s);
 hipError_t launch_pair_fill_and_masks(gm_sys *s);
 hipError_t launch_needs_rebuild(gm_sys *s, const double *x, const double *y, const double *z);
 // gm_nonbonded.hip: tiles + bonded terms in one launch
-hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z);
-// gm_bonded.hip: energy reduction, f64 output, accumulator clear
-hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz);
+// (shell: tiles only, with nbp_shell)
+hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z, bool shell = false);
+// gm_bonded.hip: energy reduction, f64 output, accumulator clear (lj_only:
+// the bonded energy segments are empty, as after a shell-mode launch)
+hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz, bool lj_only = false);
 }  // namespace gm
 namespace gm {
 struct ReplicaDesc;

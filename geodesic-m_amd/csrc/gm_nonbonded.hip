@@ -73,7 +73,10 @@ __device__ inline JAtom load_j(const double *__restrict__ x, const double *__res
 // units of the fixed-point step (box / 2^32) and no per-pair rescaling is
 // needed; the unit change is folded into the constants and into the scale at
 // which forces enter the fixed-point accumulator.
-template <bool CUBIC>
+// SHELL: each pair's switched LJ is weighted by S(r) - S_in(r) instead of
+// S(r) (the shell part of a split of the switched LJ into inner + shell);
+// below the inner switch both are 1, so short-range pairs contribute exactly 0.
+template <bool CUBIC, bool SHELL>
 __device__ __attribute__((always_inline)) inline void tile_work(const int w, const int lane, const TileArgs &A) {
     const double *__restrict__ x = A.x;
     const double *__restrict__ y = A.y;
@@ -144,6 +147,8 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     const unsigned rc2_bits = (unsigned)__float_as_int(rc2);
     const float rsw = CUBIC ? p.rsw_u : p.rsw, inv_w = CUBIC ? p.inv_w_u : p.inv_w;
     const float ds_coef = CUBIC ? p.ds_coef_u : p.ds_coef;
+    const float rin = CUBIC ? p.rin_sw_u : p.rin_sw, inv_w_in = CUBIC ? p.inv_w_in_u : p.inv_w_in;
+    const float ds_coef_in = CUBIC ? p.ds_coef_in_u : p.ds_coef_in;
     double e_acc = 0.0;
     JAtom cur = load_j(x, y, z, atom_par, s_ja[jl]);
     for (int k = 0; k < ne; k++) {
@@ -185,9 +190,15 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
                 const float r = r2 * inv_r;
                 const float u = fmaxf((r - rsw) * inv_w, 0.0f);
                 const float u2 = u * u;
-                const float sw = 1.0f + u2 * u * (-10.0f + u * (15.0f - 6.0f * u));
+                float sw = 1.0f + u2 * u * (-10.0f + u * (15.0f - 6.0f * u));
                 const float om = 1.0f - u;
-                const float dsdr = ds_coef * u2 * om * om;
+                float dsdr = ds_coef * u2 * om * om;
+                if (SHELL) {
+                    const float ui = fminf(fmaxf((r - rin) * inv_w_in, 0.0f), 1.0f);
+                    const float ui2 = ui * ui, omi = 1.0f - ui;
+                    sw -= 1.0f + ui2 * ui * (-10.0f + ui * (15.0f - 6.0f * ui));
+                    dsdr -= ds_coef_in * ui2 * omi * omi;
+                }
                 const float fr = ok ? f_lj * sw - v_lj * dsdr * inv_r : 0.0f;
                 const float v = ok ? v_lj * sw : 0.0f;
                 const float tx = fr * dx, ty = fr * dy, tz = fr * dz;
@@ -237,7 +248,7 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     __syncthreads();  // LDS is reused by the next work item of a persistent wave
 }
 
-template <bool CUBIC>
+template <bool CUBIC, bool SHELL>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_forces(TileArgs A,
                                                                                        TermArgs ta) {
     // bonded workgroups come first so they start with the tiles instead of
@@ -250,7 +261,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
         if (k < n_terms) term_dispatch(ta, k);
         return;
     }
-    tile_work<CUBIC>((int)blockIdx.x - n_bonded_groups, lane, A);
+    tile_work<CUBIC, SHELL>((int)blockIdx.x - n_bonded_groups, lane, A);
 }
 
 // Persistent batched tile kernel: a fixed grid of wavefronts walks the tile
@@ -269,7 +280,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
     const int lo = (int)(((int64_t)n_items * x) >> 3), hi = (int)(((int64_t)n_items * (x + 1)) >> 3);
     for (int it = lo + l; it < hi; it += nwx) {
         const int2 rw = items[it];
-        tile_work<true>(rw.y, lane, reps[rw.x].tile);  // replicas are cubic
+        tile_work<true, false>(rw.y, lane, reps[rw.x].tile);  // replicas are cubic
     }
 }
 
@@ -324,13 +335,19 @@ TileArgs tile_args(gm_sys *s, const double *x, const double *y, const double *z)
 }
 
 // One launch: ceil(n_terms / 64) bonded workgroups, then the tile work items.
-hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z) {
-    const TermArgs ta = term_args(s, x, y, z);
-    const TileArgs A = tile_args(s, x, y, z);
+// Shell mode: tile work items only.
+hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z, bool shell) {
+    TermArgs ta = term_args(s, x, y, z);
+    TileArgs A = tile_args(s, x, y, z);
+    if (shell) {
+        ta.nb = ta.na = ta.nd = ta.n14 = 0;
+        A.p = s->nbp_shell;
+    }
     const int64_t nterms = ta.nb + ta.na + ta.nd + ta.n14;
     const int64_t grid = s->n_wi + (nterms + 63) / 64;
     if (grid > 0) {
-        auto kern = A.cubic ? k_forces<true> : k_forces<false>;
+        auto kern = shell ? (A.cubic ? k_forces<true, true> : k_forces<false, true>)
+                          : (A.cubic ? k_forces<true, false> : k_forces<false, false>);
         kern<<<(unsigned)grid, 64, 0, s->stream>>>(A, ta);
     }
     return hipGetLastError();

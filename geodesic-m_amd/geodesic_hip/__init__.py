@@ -87,6 +87,34 @@ class Stats(C.Structure):
 _lib = None
 
 
+class SurrogateDesc(C.Structure):
+    _fields_ = [("r_inner_cutoff", _D), ("r_inner_switch", _D), ("r_inner_skin", _D),
+                ("r_lo", _D), ("r_hi", _D), ("fade", _D), ("atom_class", _P), ("seed", C.c_uint64)]
+
+
+class TrainDesc(C.Structure):
+    _fields_ = [("n_frames", _I64), ("mu", _D), ("sigma", _D), ("cap", _D), ("seed", C.c_uint64),
+                ("lr", C.c_float), ("batch", _I64), ("update", C.c_int32)]
+
+
+class TrainReport(C.Structure):
+    _fields_ = [("loss_first", _D), ("loss_last", _D), ("loss_net", _D), ("loss_stale", _D),
+                ("loss_taylor", _D), ("ms_total", _D), ("frames", _I64), ("steps", _I64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SUR_NPARAM = 29315
+SUR_NFEAT = 96
+MTS_HELD, MTS_NET, MTS_TAYLOR = 0, 1, 2
+# Brownian trajectories at 300 K: kT = 0.5962 kcal/mol, 0.02 A of noise per
+# coordinate per step, mobility mu = sigma^2 / (2 kT), steps capped at 0.1 A.
+BD_SIGMA = 0.02
+BD_MU = BD_SIGMA ** 2 / (2.0 * 0.5962)
+BD_CAP = 0.1
+
+
 def lib():
     """Load libgeodesic_hip.so; raises if it has not been built."""
     global _lib
@@ -130,6 +158,28 @@ def lib():
             "gm_batch_set_shape": ([_P, C.c_int32, C.c_int32], C.c_int),
             "gm_batch_autotune": ([_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_float)],
                                   C.c_int),
+            "gm_surrogate_create": ([_P, C.POINTER(SystemDesc), C.POINTER(SurrogateDesc), C.POINTER(_P)],
+                                    C.c_int),
+            "gm_surrogate_destroy": ([_P], C.c_int),
+            "gm_surrogate_system": ([_P, C.c_int32, C.POINTER(_P)], C.c_int),
+            "gm_surrogate_build": ([_P, _P, _P, _P], C.c_int),
+            "gm_surrogate_features": ([_P, _P, _P, _P, _P], C.c_int),
+            "gm_surrogate_set_params": ([_P, _P, _P, _P, _P], C.c_int),
+            "gm_surrogate_get_params": ([_P, _P, _P, _P, _P], C.c_int),
+            "gm_surrogate_infer": ([_P, _P, _I64, _P], C.c_int),
+            "gm_surrogate_loss_grad": ([_P, _P, _P, _P, _I64, C.POINTER(C.c_double), _P], C.c_int),
+            "gm_surrogate_train": ([_P, _P, _P, _P, C.POINTER(TrainDesc), _P, C.POINTER(TrainReport)], C.c_int),
+            "gm_mts_reset": ([_P], C.c_int),
+            "gm_mts_step": ([_P, _P, _P, _P, _P, _P, _P, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
+            "gm_surrogate_random_walk": ([_P, _P, _P, _P, C.c_double, C.c_uint64, C.c_uint64], C.c_int),
+            "gm_surrogate_bd_step": ([_P, _P, _P, _P, _P, _P, _P, C.c_double, C.c_double, C.c_double,
+                                      C.c_uint64, C.c_uint64], C.c_int),
+            "gm_surrogate_minimize": ([_P, _P, _P, _P, C.c_int32, C.c_double, C.c_double,
+                                       C.POINTER(C.c_double)], C.c_int),
+            "gm_surrogate_maybe_rebuild": ([_P, _P, _P, _P, C.POINTER(C.c_int32)], C.c_int),
+            "gm_surrogate_shell_forces": ([_P, _P, _P, _P, _P, _P, _P], C.c_int),
+            "gm_needs_rebuild_device": ([_P, _P, _P, _P, C.POINTER(C.c_int32)], C.c_int),
+            "gm_surrogate_get_hessian": ([_P, _P], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -461,6 +511,144 @@ class Batch:
         c, w, ms = C.c_int32(0), C.c_int32(0), C.c_float(0.0)
         _check(lib().gm_batch_autotune(self.h, C.byref(c), C.byref(w), C.byref(ms)))
         return c.value, w.value, ms.value
+
+
+class Surrogate:
+    """gm_surrogate: the shell-force network and the 4:1 multiple-time-step
+    driver over an exact model (`params`) and an inner model (same atoms,
+    LJ cut at r_inner_cutoff).  Positions and forces passed to the device
+    methods are DeviceArray objects."""
+
+    def __init__(self, sigma, epsilon, topo, params: Params, atom_class, r_inner=(7.5, 6.5, 9.0),
+                 window=(7.5, 12.0, 0.5), seed=7, ctx: Context | None = None):
+        ctx = ctx or default_context()
+        self.ctx = ctx
+        keep = []
+        d = make_desc(sigma, epsilon, topo, params, keep)
+        self.n = int(d.n_atoms)
+        cls = np.ascontiguousarray(atom_class, dtype=np.uint8)
+        keep.append(cls)
+        sd = SurrogateDesc()
+        sd.r_inner_cutoff, sd.r_inner_switch, sd.r_inner_skin = (float(v) for v in r_inner)
+        sd.r_lo, sd.r_hi, sd.fade = (float(v) for v in window)
+        sd.atom_class = _p(cls)
+        sd.seed = int(seed)
+        self.params = params
+        self.inner_params = Params(box=params.box, periodic=params.periodic, r_cutoff=float(r_inner[0]),
+                                   r_switch=float(r_inner[1]), r_skin=float(r_inner[2]),
+                                   scale_14=params.scale_14, bonded_min_image=params.bonded_min_image)
+        h = _P()
+        _check(lib().gm_surrogate_create(ctx.h, C.byref(d), C.byref(sd), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gm_surrogate_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def system(self, which):
+        """Borrowed System: 0 = exact model, 1 = inner model."""
+        h = _P()
+        _check(lib().gm_surrogate_system(self.h, int(which), C.byref(h)))
+        return System.borrowed(h, self.n, self.params if which == 0 else self.inner_params)
+
+    def build(self, dx, dy, dz):
+        _check(lib().gm_surrogate_build(self.h, _P(dx.ptr), _P(dy.ptr), _P(dz.ptr)))
+
+    def features(self, dx, dy, dz):
+        out = np.empty((self.n, SUR_NFEAT), np.float32)
+        _check(lib().gm_surrogate_features(self.h, _P(dx.ptr), _P(dy.ptr), _P(dz.ptr), _p(out)))
+        return out
+
+    def hessian(self):
+        """Shell Hessian K (n x 6) of the last features() call."""
+        out = np.empty((self.n, 6), np.float32)
+        _check(lib().gm_surrogate_get_hessian(self.h, _p(out)))
+        return out
+
+    def set_params(self, params=None, fmean=None, finv=None, scales=None):
+        conv = lambda a, dt: None if a is None else np.ascontiguousarray(a, dtype=dt)  # noqa: E731
+        p, m, v, sc = conv(params, np.float32), conv(fmean, np.float32), conv(finv, np.float32), conv(scales, np.float64)
+        _check(lib().gm_surrogate_set_params(self.h, _p(p), _p(m), _p(v), _p(sc)))
+
+    def get_params(self):
+        p = np.empty(SUR_NPARAM, np.float32)
+        m, v = np.empty(SUR_NFEAT, np.float32), np.empty(SUR_NFEAT, np.float32)
+        sc = np.empty(3, np.float64)
+        _check(lib().gm_surrogate_get_params(self.h, _p(p), _p(m), _p(v), _p(sc)))
+        return p, m, v, sc
+
+    def infer(self, feat):
+        feat = np.ascontiguousarray(feat, np.float32)
+        out = np.empty((len(feat), 3), np.float32)
+        _check(lib().gm_surrogate_infer(self.h, _p(feat), len(feat), _p(out)))
+        return out
+
+    def loss_grad(self, feat, basis, target):
+        feat = np.ascontiguousarray(feat, np.float32)
+        basis = np.ascontiguousarray(basis, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        loss = C.c_double(0.0)
+        g = np.empty(SUR_NPARAM, np.float32)
+        _check(lib().gm_surrogate_loss_grad(self.h, _p(feat), _p(basis), _p(target), len(feat), C.byref(loss),
+                                            _p(g)))
+        return loss.value, g
+
+    def train(self, dx, dy, dz, n_frames, mu=BD_MU, sigma=BD_SIGMA, cap=BD_CAP, seed=11, lr=1e-3, batch=8192,
+              update=True):
+        """Brownian trajectory from (dx, dy, dz), one training frame per inner
+        step.  Returns (loss history n_frames x 3 [network, o = 0, Taylor], report)."""
+        td = TrainDesc(n_frames=int(n_frames), mu=float(mu), sigma=float(sigma), cap=float(cap), seed=int(seed),
+                       lr=float(lr), batch=int(batch), update=int(bool(update)))
+        hist = np.empty((max(int(n_frames), 1), 3), np.float32)
+        rep = TrainReport()
+        _check(lib().gm_surrogate_train(self.h, _P(dx.ptr), _P(dy.ptr), _P(dz.ptr), C.byref(td), _p(hist),
+                                        C.byref(rep)))
+        return hist[:int(n_frames)], rep.as_dict()
+
+    def mts_reset(self):
+        _check(lib().gm_mts_reset(self.h))
+
+    def mts_step(self, dx, dy, dz, dfx, dfy, dfz, mode=MTS_NET):
+        """mode: MTS_HELD (shell force held), MTS_NET (network), MTS_TAYLOR."""
+        was = C.c_int32(0)
+        _check(lib().gm_mts_step(self.h, _P(dx.ptr), _P(dy.ptr), _P(dz.ptr), _P(dfx.ptr), _P(dfy.ptr),
+                                 _P(dfz.ptr), int(mode), C.byref(was)))
+        return bool(was.value)
+
+    def bd_step(self, dx, dy, dz, dfx, dfy, dfz, mu=BD_MU, sigma=BD_SIGMA, cap=BD_CAP, seed=0, step=0):
+        _check(lib().gm_surrogate_bd_step(self.h, _P(dx.ptr), _P(dy.ptr), _P(dz.ptr), _P(dfx.ptr), _P(dfy.ptr),
+                                          _P(dfz.ptr), float(mu), float(sigma), float(cap), int(seed), int(step)))
+
+    def minimize(self, dx, dy, dz, steps=300, mu=1e-3, cap=0.05):
+        """Steepest descent on the exact model; returns the largest force component after."""
+        m = C.c_double(0.0)
+        _check(lib().gm_surrogate_minimize(self.h, _P(dx.ptr), _P(dy.ptr), _P(dz.ptr), int(steps), float(mu),
+                                           float(cap), C.byref(m)))
+        return m.value
+
+    def maybe_rebuild(self, dx, dy, dz):
+        r = C.c_int32(0)
+        _check(lib().gm_surrogate_maybe_rebuild(self.h, _P(dx.ptr), _P(dy.ptr), _P(dz.ptr), C.byref(r)))
+        return bool(r.value)
+
+    def shell_forces(self, dx, dy, dz, dfx, dfy, dfz):
+        _check(lib().gm_surrogate_shell_forces(self.h, _P(dx.ptr), _P(dy.ptr), _P(dz.ptr), _P(dfx.ptr),
+                                               _P(dfy.ptr), _P(dfz.ptr)))
+
+    def random_walk(self, dx, dy, dz, sigma, seed, step):
+        _check(lib().gm_surrogate_random_walk(self.h, _P(dx.ptr), _P(dy.ptr), _P(dz.ptr), float(sigma),
+                                              int(seed), int(step)))
+
+    def sync(self):
+        """Waits for the surrogate's stream (shared by both systems)."""
+        self.system(0).sync()
 
 
 # ---- mirror of the reference's backend API ------------------------------------

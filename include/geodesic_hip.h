@@ -129,6 +129,9 @@ gm_status gm_get_pair_list(gm_sys *sys, uint32_t *pair_i, uint32_t *pair_j, int6
 gm_status gm_build_neighbor_list_device(gm_sys *sys, double *d_x, double *d_y, double *d_z);
 gm_status gm_compute_forces_device(gm_sys *sys, const double *d_x, const double *d_y,
                                    const double *d_z, double *d_fx, double *d_fy, double *d_fz);
+/* ComputeBackend::needs_rebuild at device positions (synchronous). */
+gm_status gm_needs_rebuild_device(gm_sys *sys, const double *d_x, const double *d_y, const double *d_z,
+                                  int32_t *out);
 /* Waits for the system's stream; writes the last evaluation's energies (e may be NULL). */
 gm_status gm_sync(gm_sys *sys, gm_energy_terms *e);
 
@@ -218,6 +221,122 @@ gm_status gm_batch_set_shape(gm_batch *b, int32_t entries_per_item, int32_t wave
 /* Times a grid of shapes on the current lists and keeps the fastest. */
 gm_status gm_batch_autotune(gm_batch *b, int32_t *entries_per_item, int32_t *waves_per_cu,
                             float *best_ms);
+
+/* ---- Shell-force surrogate and 4:1 multiple-time-step driver -------------
+ * [ext: no reference counterpart]
+ *
+ * The exact model is the system described by `exact` (its r_cutoff bounds the
+ * feature window).  The surrogate also creates an inner model: the same
+ * atoms and topology with LJ cut at r_inner_cutoff, switched from
+ * r_inner_switch, on its own list of radius r_inner_skin.  Shell force =
+ * exact - inner.
+ *
+ * Per atom, 96 radial features: 24 Gaussians (centres evenly spaced over
+ * [r_lo, r_hi], width = spacing) times 4 atom classes, faded to zero over
+ * `fade` A at both window edges.  They pass through a 96 -> 128 -> 128 -> 3
+ * SiLU network (BF16 weights and activations on MFMA, FP32 accumulate).  The
+ * three outputs o weight three equivariant vectors, so that at an inner step
+ *   F = F_inner(t) + (1 + o1) F_shell(t0) - o2 s2 K dx + o3 s3 (F_inner(t) - F_inner(t0)),
+ * with t0 the last exact step, dx the displacement since t0, and K the shell
+ * potential's Hessian with respect to the atom's own position at t0 (summed
+ * over its shell pairs, computed with the features).  o = 0 is the plain
+ * multiple-time-step method (shell force held from t0); o = (0, 1/s2, 0) is
+ * the first-order Taylor correction in the atom's own displacement.
+ *
+ * Parameters: 29315 fp32 values, W1[128][96], b1[128], W2[128][128], b2[128],
+ * W3[3][128], b3[3]; feature standardisation mean[96], inv_std[96]; scales
+ * {s2, s3, sigma} (sigma normalises forces in the training loss). */
+typedef struct gm_surrogate gm_surrogate;
+enum { GM_SUR_NPARAM = 29315, GM_SUR_NFEAT = 96 };
+typedef struct {
+    double r_inner_cutoff, r_inner_switch, r_inner_skin; /* inner model */
+    double r_lo, r_hi, fade;  /* feature window and edge fade, A */
+    const uint8_t *atom_class; /* per atom, 0..3 */
+    uint64_t seed;             /* weight initialisation */
+} gm_surrogate_desc;
+gm_status gm_surrogate_create(gm_ctx *ctx, const gm_system_desc *exact, const gm_surrogate_desc *d,
+                              gm_surrogate **out);
+gm_status gm_surrogate_destroy(gm_surrogate *s);
+/* Borrowed handle of the exact (which = 0) or inner (which = 1) system. */
+gm_status gm_surrogate_system(gm_surrogate *s, int32_t which, gm_sys **out);
+/* Builds both lists at device positions (wrapped in place when periodic). */
+gm_status gm_surrogate_build(gm_surrogate *s, double *d_x, double *d_y, double *d_z);
+/* Features at device positions (lists built within their skin); h_out
+ * (n x 96, may be NULL) receives a copy. */
+gm_status gm_surrogate_features(gm_surrogate *s, const double *d_x, const double *d_y,
+                                const double *d_z, float *h_out);
+/* Shell Hessian K (n x 6: xx, yy, zz, xy, xz, yz) of the last
+ * gm_surrogate_features call: for each atom, the sum over its non-excluded
+ * pairs of the Hessian of the shell pair potential. */
+gm_status gm_surrogate_get_hessian(gm_surrogate *s, float *h_out);
+/* Any NULL argument leaves that part unchanged. */
+gm_status gm_surrogate_set_params(gm_surrogate *s, const float *params, const float *fmean,
+                                  const float *finv, const double *scales);
+gm_status gm_surrogate_get_params(gm_surrogate *s, float *params, float *fmean, float *finv,
+                                  double *scales);
+/* Network outputs (n x 3) for host features (n x 96). */
+gm_status gm_surrogate_infer(gm_surrogate *s, const float *h_feat, int64_t n, float *h_out);
+/* Training loss and its gradient (GM_SUR_NPARAM) for host features (n x 96),
+ * normalised basis vectors (n x 9: B1, B2, B3) and targets (n x 3):
+ * loss = (1/n) sum |(1 + o1) B1 + o2 B2 + o3 B3 - T|^2.  No update. */
+gm_status gm_surrogate_loss_grad(gm_surrogate *s, const float *h_feat, const float *h_basis,
+                                 const float *h_target, int64_t n, double *loss, float *h_grad);
+typedef struct {
+    int64_t n_frames;       /* inner-step frames to train on (3 per exact step) */
+    double mu, sigma, cap;  /* Brownian step dx = mu F + sigma xi per coordinate, clamped to +-cap (A) */
+    uint64_t seed;          /* trajectory seed */
+    float lr;               /* Adam learning rate */
+    int64_t batch;          /* atoms per frame (contiguous, rotating) */
+    int32_t update;         /* 0: evaluate only (no parameter change) */
+} gm_train_desc;
+typedef struct {
+    double loss_first, loss_last;  /* network loss of the first / last frame */
+    double loss_net, loss_stale;   /* mean over frames: network, and o = 0 */
+    double loss_taylor;            /* mean over frames: o = (0, 1/s2, 0) */
+    double ms_total;               /* wall time */
+    int64_t frames, steps;
+} gm_train_report;
+/* Runs a seeded Brownian trajectory from the device positions d_x/d_y/d_z
+ * (advanced in place, driven by the exact force): inner and shell forces at
+ * every step, features at every 4th; each inner step is one training frame
+ * (one Adam step over `batch` atoms, target = the shell force).  Lists are
+ * rebuilt when either model's needs_rebuild criterion fires (checked at every
+ * exact step).  The first call also fixes the feature standardisation and
+ * basis scales from the first frames.  h_loss (3 per frame: network, o = 0,
+ * Taylor; may be NULL) receives the loss history. */
+gm_status gm_surrogate_train(gm_surrogate *s, double *d_x, double *d_y, double *d_z,
+                             const gm_train_desc *td, float *h_loss, gm_train_report *rep);
+/* 4:1 multiple-time-step force provider.  Step k % 4 == 0: exact forces
+ * (inner + shell, was_exact = 1) and, except in GM_MTS_HELD mode, features
+ * and shell Hessian (and network coefficients in GM_MTS_NET mode) are
+ * refreshed.  Otherwise: inner forces plus the shell force predicted by the
+ * mode: held from the last exact step, network, or Taylor correction.
+ * Forces go to device arrays; ordered on the surrogate's stream (gm_sync on
+ * the exact system waits). */
+enum { GM_MTS_HELD = 0, GM_MTS_NET = 1, GM_MTS_TAYLOR = 2 };
+gm_status gm_mts_reset(gm_surrogate *s);
+gm_status gm_mts_step(gm_surrogate *s, const double *d_x, const double *d_y, const double *d_z,
+                      double *d_fx, double *d_fy, double *d_fz, int32_t mode, int32_t *was_exact);
+/* Seeded Gaussian displacement of every coordinate of device positions. */
+gm_status gm_surrogate_random_walk(gm_surrogate *s, double *d_x, double *d_y, double *d_z,
+                                   double sigma, uint64_t seed, uint64_t step);
+/* One Brownian step of device positions under device forces d_f*:
+ * dx = mu F + sigma xi per coordinate, clamped to +-cap (A). */
+gm_status gm_surrogate_bd_step(gm_surrogate *s, double *d_x, double *d_y, double *d_z, const double *d_fx,
+                               const double *d_fy, const double *d_fz, double mu, double sigma, double cap,
+                               uint64_t seed, uint64_t step);
+/* Steepest descent on the exact model (dx = mu F, clamped to +-cap), with
+ * list rebuilds as needed; leaves both lists built.  *max_force (may be
+ * NULL) receives the largest force component afterwards. */
+gm_status gm_surrogate_minimize(gm_surrogate *s, double *d_x, double *d_y, double *d_z, int32_t steps,
+                                double mu, double cap, double *max_force);
+/* Shell force alone (exact system in shell mode) into device arrays; ordered
+ * on the surrogate's stream. */
+gm_status gm_surrogate_shell_forces(gm_surrogate *s, const double *d_x, const double *d_y, const double *d_z,
+                                    double *d_fx, double *d_fy, double *d_fz);
+/* Rebuilds both lists if either model's needs_rebuild criterion fires. */
+gm_status gm_surrogate_maybe_rebuild(gm_surrogate *s, double *d_x, double *d_y, double *d_z,
+                                     int32_t *rebuilt);
 
 #ifdef __cplusplus
 }

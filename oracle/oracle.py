@@ -34,6 +34,16 @@ def lib():
             f = getattr(L, name)
             f.argtypes = [_I64, _P, _P, _P, _P, _D, _I64, _P, _P, _P, _P, _I64]
             f.restype = _I64
+        L.or_sur_features.argtypes = [_I64, _P, _P, _P, _P, _P, _D, _D, _D, _P]
+        L.or_sur_features.restype = None
+        L.or_sur_forward.argtypes = [_I64, _P, _P, _P, _P, C.c_int, _P]
+        L.or_sur_forward.restype = None
+        L.or_sur_loss_grad.argtypes = [_I64, _P, _P, _P, _P, _P, _P, C.c_int, _P]
+        L.or_sur_loss_grad.restype = _D
+        L.or_sur_shell.argtypes = [_I64, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _D, _D, _D, _D, _D, _P, _P, _P, _P]
+        L.or_sur_shell.restype = None
+        L.or_sur_nparam.argtypes = []
+        L.or_sur_nparam.restype = _I64
         L.or_needs_rebuild.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _D, _D]
         L.or_needs_rebuild.restype = C.c_int
         L.or_pair_forces.argtypes = [_P, _P, _P, _P, _P, _I64, _P, _P, _D, _D, _P, _P, _P, _P]
@@ -246,3 +256,48 @@ def constrain_velocities(x, y, z, vx, vy, vz, topo, mass, max_iter, tol):
     return lib().or_constrain_velocities(len(x), _p(_f64(x)), _p(_f64(y)), _p(_f64(z)), _p(vx),
                                          _p(vy), _p(vz), len(ci), _p(ci), _p(cj), _p(_f64(mass)),
                                          int(max_iter), float(tol))
+
+
+# ---- shell-force surrogate (no reference counterpart) -------------------------
+
+def sur_features(x, y, z, box, cls, r_lo=7.5, r_hi=12.0, fade=0.5):
+    """Brute-force f64 features, n x 96."""
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    cls = np.ascontiguousarray(cls, dtype=np.uint8)
+    out = np.zeros((len(x), 96))
+    lib().or_sur_features(len(x), _p(x), _p(y), _p(z), _p(_box(box)), _p(cls), float(r_lo), float(r_hi),
+                          float(fade), _p(out))
+    return out
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def sur_forward(feat, fmean, finv, params, round_bf16=True):
+    feat, fmean, finv, params = _f32(feat), _f32(fmean), _f32(finv), _f32(params)
+    out = np.zeros((len(feat), 3))
+    lib().or_sur_forward(len(feat), _p(feat), _p(fmean), _p(finv), _p(params), int(bool(round_bf16)), _p(out))
+    return out
+
+
+def sur_loss_grad(feat, fmean, finv, params, basis, target, round_bf16=True):
+    feat, fmean, finv, params = _f32(feat), _f32(fmean), _f32(finv), _f32(params)
+    basis, target = _f32(basis), _f32(target)
+    g = np.zeros(int(lib().or_sur_nparam()))
+    loss = lib().or_sur_loss_grad(len(feat), _p(feat), _p(fmean), _p(finv), _p(params), _p(basis), _p(target),
+                                  int(bool(round_bf16)), _p(g))
+    return float(loss), g
+
+
+def sur_shell(x, y, z, sigma, epsilon, pi, pj, box, r_switch=7.5, r_cutoff=12.0, r_in_switch=6.5,
+              r_in_cutoff=7.5):
+    """Shell force (3 x n) and shell Hessian K (n x 6) over a pair list."""
+    x, y, z, sigma, epsilon = _f64(x), _f64(y), _f64(z), _f64(sigma), _f64(epsilon)
+    pi, pj = _u32(pi), _u32(pj)
+    n = len(x)
+    fx, fy, fz, K = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros((n, 6))
+    lib().or_sur_shell(n, _p(x), _p(y), _p(z), _p(sigma), _p(epsilon), len(pi), _p(pi), _p(pj), _p(_box(box)),
+                       float(r_switch), float(r_cutoff - r_switch), float(r_in_switch),
+                       float(r_in_cutoff - r_in_switch), float(r_cutoff), _p(fx), _p(fy), _p(fz), _p(K))
+    return np.stack([fx, fy, fz]), K

@@ -48,6 +48,14 @@ int main(void) {
   F(gm_system_desc, n_angles) F(gm_system_desc, n_dihedrals) F(gm_system_desc, dihed_delta)
   F(gm_system_desc, n_excl) F(gm_system_desc, n_pairs14) F(gm_system_desc, pair14_j)
   F(gm_stats, n_tiles_cutoff) F(gm_stats, n_pairs_cutoff) F(gm_stats, ms_nonbonded) F(gm_stats, ms_build)
+  printf("gm_surrogate_desc %zu\n", sizeof(gm_surrogate_desc));
+  printf("gm_train_desc %zu\n", sizeof(gm_train_desc));
+  printf("gm_train_report %zu\n", sizeof(gm_train_report));
+  F(gm_surrogate_desc, fade) F(gm_surrogate_desc, atom_class) F(gm_surrogate_desc, seed)
+  F(gm_train_desc, mu) F(gm_train_desc, cap) F(gm_train_desc, lr) F(gm_train_desc, batch) F(gm_train_desc, update)
+  F(gm_train_report, loss_taylor) F(gm_train_report, ms_total) F(gm_train_report, steps)
+  printf("GM_SUR_NPARAM %d\n", (int)GM_SUR_NPARAM);
+  printf("GM_SUR_NFEAT %d\n", (int)GM_SUR_NFEAT);
   return 0;
 }
 """
@@ -63,7 +71,12 @@ def test_ctypes_structs_match_c_layout(tmp_path):
     assert int(out["gm_energy_terms"]) == C.sizeof(gh.EnergyTerms)
     assert int(out["gm_stats"]) == C.sizeof(gh.Stats)
     assert int(out["gm_profile"]) == C.sizeof(gh.Profile)
-    classes = {"gm_system_desc": gh.SystemDesc, "gm_stats": gh.Stats}
+    assert int(out["gm_surrogate_desc"]) == C.sizeof(gh.SurrogateDesc)
+    assert int(out["gm_train_desc"]) == C.sizeof(gh.TrainDesc)
+    assert int(out["gm_train_report"]) == C.sizeof(gh.TrainReport)
+    assert int(out["GM_SUR_NPARAM"]) == gh.SUR_NPARAM and int(out["GM_SUR_NFEAT"]) == gh.SUR_NFEAT
+    classes = {"gm_system_desc": gh.SystemDesc, "gm_stats": gh.Stats, "gm_surrogate_desc": gh.SurrogateDesc,
+               "gm_train_desc": gh.TrainDesc, "gm_train_report": gh.TrainReport}
     for key, val in out.items():
         if "." in key:
             t, m = key.split(".")

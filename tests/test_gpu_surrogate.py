@@ -1,0 +1,249 @@
+"""Shell-force surrogate on the GPU (no reference counterpart): features,
+BF16 MFMA inference and the hand-written gradient against the f64 oracle,
+training, the gain over the plain multiple-time-step method, and the
+multiple-time-step forces against their parts and against the f64 oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+import geodesic_hip as gh
+from geodesic_hip import systems
+from oracle import oracle as orc
+
+from test_oracle_surrogate import rand_params, OFF_W1, OFF_B1, OFF_W2, OFF_B2, OFF_W3, OFF_B3
+
+pytestmark = pytest.mark.gpu
+
+N = 3000
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def exact_params(s):
+    return gh.Params(box=s.params.box, periodic=True, r_cutoff=12.0, r_switch=7.5, r_skin=13.5,
+                     scale_14=s.params.scale_14, bonded_min_image=s.params.bonded_min_image)
+
+
+def make_system(seed=31):
+    s = systems.protein_solvent(N, (N / 0.1) ** (1 / 3), N // 8, seed=seed)
+    return s, exact_params(s)
+
+
+def device_positions(s):
+    return tuple(gh.DeviceArray.from_host(a) for a in (s.x, s.y, s.z))
+
+
+@pytest.fixture(scope="module")
+def small():
+    say("[gpu] surrogate: small system")
+    s, p = make_system()
+    sur = gh.Surrogate(s.sigma, s.epsilon, s.topo, p, s.types % 4, seed=5)
+    d = device_positions(s)
+    sur.build(*d)
+    return s, p, sur, d
+
+
+def test_features_match_oracle(small):
+    s, p, sur, d = small
+    say("[gpu] surrogate features")
+    f = sur.features(*d)
+    x, y, z = (a.to_host() for a in d)
+    ref = orc.sur_features(x, y, z, p.box, s.types % 4)
+    assert ref.max() > 1.0
+    err = np.abs(f - ref)
+    assert np.all(err <= 1e-4 * np.maximum(1.0, np.abs(ref))), float(err.max())
+    assert np.sqrt(np.mean(err ** 2)) <= 1e-5 * np.sqrt(np.mean(ref ** 2))
+    K = sur.hessian()
+    pi, pj = orc.neighbor_build(x, y, z, p.box, p.r_skin, s.topo.excl_i, s.topo.excl_j, method="cells")
+    _, Kref = orc.sur_shell(x, y, z, s.sigma, s.epsilon, pi, pj, p.box)
+    kerr = np.abs(K - Kref)
+    assert np.sqrt(np.mean(kerr ** 2)) <= 1e-5 * np.sqrt(np.mean(Kref ** 2))
+    assert kerr.max() <= 1e-4 * np.abs(Kref).max()
+    f2 = sur.features(*d)
+    assert f2.tobytes() == f.tobytes() and sur.hessian().tobytes() == K.tobytes()
+
+
+def oracle_forces(s, x, y, z, params):
+    pi, pj = orc.neighbor_build(x, y, z, params.box, params.r_skin, s.topo.excl_i, s.topo.excl_j, method="cells")
+    t, gx, gy, gz = orc.cpu_backend_compute(x, y, z, s.sigma, s.epsilon, s.topo, pi, pj, params.box,
+                                            params.r_cutoff, params.r_switch, n_threads=8, scale14=params.scale_14,
+                                            bonded_min_image=params.bonded_min_image)
+    return np.stack([gx, gy, gz])
+
+
+def test_shell_force_matches_oracle_split(small):
+    """Shell mode = exact model minus inner model, both in f64 by the oracle
+    (no cancellation at f64 precision); and inner + shell reproduces the exact
+    model's force."""
+    s, p, sur, d = small
+    say("[gpu] shell force vs oracle")
+    f = [gh.DeviceArray(N) for _ in range(3)]
+    sur.shell_forces(*d, *f)
+    sur.sync()
+    fsh = np.stack([a.to_host() for a in f])
+    x, y, z = (a.to_host() for a in d)
+    ex = oracle_forces(s, x, y, z, p)
+    inn = oracle_forces(s, x, y, z, sur.inner_params)
+    ref = ex - inn
+    rms = np.sqrt(np.mean(ref ** 2))
+    assert rms > 0.0
+    assert np.sqrt(np.mean((fsh - ref) ** 2)) <= 1e-5 * rms, np.sqrt(np.mean((fsh - ref) ** 2)) / rms
+
+
+def norm_of(f):
+    return f.mean(axis=0).astype(np.float32), (1.0 / np.sqrt(f.var(axis=0) + 1e-4)).astype(np.float32)
+
+
+def test_inference_matches_oracle(small):
+    s, p, sur, d = small
+    say("[gpu] surrogate inference")
+    f = sur.features(*d)
+    fmean, finv = norm_of(f)
+    prm = rand_params(np.random.default_rng(8))
+    sur.set_params(prm, fmean, finv, np.ones(3))
+    o = sur.infer(f)
+    ref = orc.sur_forward(f, fmean, finv, prm, round_bf16=True)
+    scale = np.abs(ref).max()
+    err = np.abs(o - ref)
+    # fp32 vs f64 accumulation can move a BF16 activation across a rounding
+    # boundary now and then; those rare flips bound the largest difference
+    assert err.max() <= 2e-2 * scale, (err.max(), scale)
+    assert np.sqrt(np.mean(err ** 2)) <= 1e-3 * scale
+    assert np.median(err) <= 1e-4 * scale
+
+
+def test_gradient_matches_oracle(small):
+    s, p, sur, d = small
+    say("[gpu] surrogate gradient")
+    f = sur.features(*d)
+    fmean, finv = norm_of(f)
+    rng = np.random.default_rng(9)
+    prm = rand_params(rng)
+    sur.set_params(prm, fmean, finv, np.ones(3))
+    basis = rng.normal(size=(len(f), 9)).astype(np.float32)
+    target = (1.2 * basis[:, 0:3] - 0.3 * basis[:, 6:9] + 0.1 * rng.normal(size=(len(f), 3))).astype(np.float32)
+    loss, g = sur.loss_grad(f, basis, target)
+    lref, gref = orc.sur_loss_grad(f, fmean, finv, prm, basis, target, round_bf16=True)
+    assert abs(loss - lref) <= 1e-5 * abs(lref)
+    for name, a, b in (("W1", OFF_W1, OFF_B1), ("b1", OFF_B1, OFF_W2), ("W2", OFF_W2, OFF_B2),
+                       ("b2", OFF_B2, OFF_W3), ("W3", OFF_W3, OFF_B3), ("b3", OFF_B3, len(g))):
+        num = np.linalg.norm(g[a:b] - gref[a:b])
+        den = np.linalg.norm(gref[a:b])
+        assert num <= 2e-3 * den, (name, num / den)
+
+
+def relaxed(sur, seed):
+    """Device positions of a fresh small system after steepest descent."""
+    d = device_positions(systems.protein_solvent(N, (N / 0.1) ** (1 / 3), N // 8, seed=seed))
+    fmax = sur.minimize(*d, steps=300)
+    say(f"[gpu] relaxed seed {seed}: max force component {fmax:.4g}")
+    return d
+
+
+@pytest.fixture(scope="module")
+def trained():
+    say("[gpu] surrogate training (small system)")
+    s, p = make_system(seed=41)
+    sur = gh.Surrogate(s.sigma, s.epsilon, s.topo, p, s.types % 4, seed=3)
+    d = relaxed(sur, 41)
+    hist, rep = sur.train(*d, n_frames=1500, seed=101, lr=1e-3, batch=N)
+    return s, p, sur, d, hist, rep
+
+
+def test_training_reduces_loss(trained):
+    s, p, sur, d, hist, rep = trained
+    say(f"[gpu] training report {rep}")
+    assert np.all(np.isfinite(hist))
+    first, last = hist[:100, 0].mean(), hist[-100:, 0].mean()
+    assert last < 0.95 * first, (first, last)
+    # the network's loss falls below the held-shell loss of the same frames
+    assert hist[-300:, 0].mean() < hist[-300:, 1].mean()
+    assert rep["frames"] == 1500 and rep["steps"] >= 2000
+
+
+def test_network_beats_plain_mts_on_held_out_trajectory(trained):
+    """Evaluation only (no update) on a trajectory the network never saw: the
+    shell force it predicts between exact steps is closer to the exact shell
+    force than holding the last exact one (o = 0).  Under Brownian dynamics
+    most of the change comes from the independent motion of shell neighbours,
+    which no per-atom input sees, so the gain is a modest fraction."""
+    s, p, sur, d, hist, rep = trained
+    params_before = sur.get_params()[0]
+    e = relaxed(sur, 43)
+    h2, r2 = sur.train(*e, n_frames=300, seed=202, lr=0.0, batch=N, update=False)
+    say(f"[gpu] held-out: network {r2['loss_net']:.4g}  held shell force {r2['loss_stale']:.4g}  "
+        f"Taylor {r2['loss_taylor']:.4g}")
+    assert sur.get_params()[0].tobytes() == params_before.tobytes()
+    assert r2["loss_net"] < 0.9 * r2["loss_stale"], r2
+    assert r2["loss_net"] < 1.05 * r2["loss_taylor"], r2
+
+
+def test_mts_step_composition(trained):
+    s, p, sur, d, hist, rep = trained
+    say("[gpu] mts composition")
+    q = device_positions(systems.protein_solvent(N, (N / 0.1) ** (1 / 3), N // 8, seed=44))
+    sur.build(*q)
+    f = [gh.DeviceArray(N) for _ in range(3)]
+    g = [gh.DeviceArray(N) for _ in range(3)]
+    sur.mts_reset()
+    assert sur.mts_step(*q, *f) is True
+    sur.sync()
+    ftot = np.stack([a.to_host() for a in f])
+    inn = sur.system(1)
+    inn.compute_forces_device(*(a.ptr for a in q), *(a.ptr for a in g))
+    inn.sync()
+    fin0 = np.stack([a.to_host() for a in g])
+    sur.shell_forces(*q, *g)
+    sur.sync()
+    fsh0 = np.stack([a.to_host() for a in g])
+    assert ftot.tobytes() == (fin0 + fsh0).tobytes()
+    p0 = np.stack([a.to_host() for a in q])
+    coef = sur.infer(sur.features(*q)).astype(np.float64)
+    K = sur.hessian().astype(np.float64)
+    s2, s3, _ = sur.get_params()[3]
+    sur.random_walk(*q, 0.02, 77, 0)
+    assert sur.mts_step(*q, *f) is False
+    sur.sync()
+    fm = np.stack([a.to_host() for a in f])
+    inn.compute_forces_device(*(a.ptr for a in q), *(a.ptr for a in g))
+    inn.sync()
+    fin1 = np.stack([a.to_host() for a in g])
+    p1 = np.stack([a.to_host() for a in q])
+    box = np.asarray(p.box)[:, None]
+    dxm = (p1 - p0) - box * np.round((p1 - p0) / box)
+    o = coef.T
+    Kd = np.stack([K[:, 0] * dxm[0] + K[:, 3] * dxm[1] + K[:, 4] * dxm[2],
+                   K[:, 3] * dxm[0] + K[:, 1] * dxm[1] + K[:, 5] * dxm[2],
+                   K[:, 4] * dxm[0] + K[:, 5] * dxm[1] + K[:, 2] * dxm[2]])
+    want = fin1 + (1.0 + o[0]) * fsh0 - o[1] * s2 * Kd + o[2] * s3 * (fin1 - fin0)
+    assert np.allclose(fm, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+
+
+def test_mts_forces_against_f64_oracle(trained):
+    """Brownian steps driven by the multiple-time-step forces; at the third
+    inner step after an exact step, that force (inner exact + predicted shell)
+    against the f64 oracle's exact force at the same positions."""
+    s, p, sur, d, hist, rep = trained
+    say("[gpu] mts vs f64 oracle")
+    errs = {}
+    for mode in (gh.MTS_NET, gh.MTS_HELD, gh.MTS_TAYLOR):
+        q = relaxed(sur, 45)
+        f = [gh.DeviceArray(N) for _ in range(3)]
+        sur.mts_reset()
+        for k in range(4):
+            if k > 0:
+                sur.bd_step(*q, *f, seed=303, step=k - 1)
+            sur.mts_step(*q, *f, mode=mode)
+        sur.sync()
+        fm = np.stack([a.to_host() for a in f])
+        x, y, z = (a.to_host() for a in q)
+        ref = oracle_forces(s, x, y, z, p)
+        errs[mode] = np.sqrt(np.mean((fm - ref) ** 2)) / np.sqrt(np.mean(ref ** 2))
+    say(f"[gpu] force RMS error vs f64: network {errs[gh.MTS_NET]:.3g}, held shell force "
+        f"{errs[gh.MTS_HELD]:.3g}, Taylor {errs[gh.MTS_TAYLOR]:.3g}")
+    assert errs[gh.MTS_NET] <= 1e-3
+    assert errs[gh.MTS_NET] < errs[gh.MTS_HELD]
