@@ -519,6 +519,8 @@ extern "C" {
 gm_status gm_system_destroy(gm_sys *s) {
     if (!s) return GM_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    gm_dyn_free(s->dyn);
+    s->dyn = nullptr;
     DevBuf<double> *dbl[] = {&s->sigma, &s->eps, &s->bk, &s->br0, &s->akth, &s->ath0, &s->dkphi,
                              &s->ddelta, &s->x, &s->y, &s->z, &s->refx, &s->refy, &s->refz,
                              &s->fx, &s->fy, &s->fz, &s->sx, &s->sy, &s->sz, &s->e_items, &s->e_wi, &s->e_out};

@@ -1,0 +1,337 @@
+// Device-resident dynamics, host side: masses and constraints attached to a
+// system, the integrator steps on device arrays, and the reference's run loop
+// (geodesic/src/lib.rs:255-330; paths relative to the reference repository
+// root) with forces from the GPU force path and frames streamed to the host.
+//
+// The SHAKE / RATTLE iterations run without a host round trip per iteration:
+// after every iteration a one-thread kernel compares the largest multiplier
+// with the tolerance and raises a device flag, which makes the remaining
+// launched iterations return at once; the host reads the flag every
+// kCheckEvery iterations.  The result is the reference's: same iterations,
+// same stopping point.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gm_dynamics.hpp"
+
+using namespace gm;
+using namespace gm::dyn;
+
+struct gm_dyn {
+    int64_t n = 0, nc = 0;
+    DevBuf<double> mass, dsq;
+    DevBuf<uint32_t> ci, cj;
+    DevBuf<int64_t> inc_off, inc_con;
+    DevBuf<double> ref, scratch, lambda, force, ke;  // ref/scratch/force: 3n blocks
+    DevBuf<unsigned long long> maxslot;
+    DevBuf<int> flag;
+    int max_iter_alloc = 0;
+};
+
+void gm_dyn_free(gm_dyn *d) {
+    if (!d) return;
+    for (DevBuf<double> *b : {&d->mass, &d->dsq, &d->ref, &d->scratch, &d->lambda, &d->force, &d->ke})
+        b->release();
+    d->ci.release();
+    d->cj.release();
+    d->inc_off.release();
+    d->inc_con.release();
+    d->maxslot.release();
+    d->flag.release();
+    delete d;
+}
+
+#define GM_REQUIRE(cond, msg)                                   \
+    do {                                                        \
+        if (!(cond)) return set_error(GM_ERR_INVALID_ARG, msg); \
+    } while (0)
+
+namespace {
+
+constexpr int kCheckEvery = 4;
+
+DynArgs dyn_args(const gm_dyn *d) {
+    DynArgs D{};
+    D.n = d->n;
+    D.mass = d->mass.p;
+    D.ci = d->ci.p;
+    D.cj = d->cj.p;
+    D.dsq = d->dsq.p;
+    D.inc_off = d->inc_off.p;
+    D.inc_con = d->inc_con.p;
+    return D;
+}
+
+gm_status ensure_iter_buffers(gm_dyn *d, int max_iter) {
+    if (d->max_iter_alloc < max_iter) {
+        GM_HIP(d->maxslot.ensure((size_t)max_iter));
+        d->max_iter_alloc = max_iter;
+    }
+    return GM_OK;
+}
+
+// The largest |multiplier| of the last iteration and its first constraint
+// (constraint.rs:140-146, ties to the lowest index).
+void worst_of(const std::vector<double> &lam, int64_t &idx, double &val) {
+    idx = 0;
+    val = 0.0;
+    for (size_t c = 0; c < lam.size(); c++)
+        if (std::fabs(lam[c]) > val) {
+            val = std::fabs(lam[c]);
+            idx = (int64_t)c;
+        }
+}
+
+// Jacobi iterations of SHAKE (velocity = false) or RATTLE (velocity = true)
+// on the 3n-block `buf0` (the caller's arrays), with the scratch buffer as the
+// second half of the double buffer.  On return the result is in buf0.
+gm_status project(gm_sys *s, bool velocity, Vec3 buf0, Vec3C other, int max_iter, double tol, int64_t step,
+                  int32_t *iters) {
+    gm_dyn *d = s->dyn;
+    hipStream_t st = s->stream;
+    if (d->nc == 0) {
+        if (iters) *iters = 0;
+        return GM_OK;
+    }
+    gm_status rc = ensure_iter_buffers(d, max_iter);
+    if (rc != GM_OK) return rc;
+    const int64_t n = d->n;
+    Vec3 buf1{d->scratch.p, d->scratch.p + n, d->scratch.p + 2 * n};
+    GM_HIP(launch_fill_i32(d->flag.p, 3, 0, st));
+    GM_HIP(launch_fill_u64(d->maxslot.p, (size_t)max_iter, st));
+    const DynArgs D = dyn_args(d);
+    int flag[3] = {0, 0, 0};
+    for (int it = 0; it < max_iter; it++) {
+        const Vec3 in = (it & 1) ? buf1 : buf0, out = (it & 1) ? buf0 : buf1;
+        if (velocity)
+            GM_HIP(launch_rattle_iter(D, other, in, out, d->maxslot.p + it, d->lambda.p, d->flag.p, st));
+        else
+            GM_HIP(launch_shake_iter(D, in, out, other, d->maxslot.p + it, d->lambda.p, d->flag.p, st));
+        GM_HIP(launch_check(d->maxslot.p, tol, it, d->flag.p, st));
+        if ((it + 1) % kCheckEvery == 0 || it + 1 == max_iter) {
+            GM_HIP(hipMemcpyAsync(flag, d->flag.p, sizeof(flag), hipMemcpyDeviceToHost, st));
+            GM_HIP(hipStreamSynchronize(st));
+            if (flag[0]) break;
+        }
+    }
+    if (flag[2] == 1) {
+        const size_t b = sizeof(double) * (size_t)n;
+        GM_HIP(hipMemcpyAsync(buf0.x, buf1.x, b, hipMemcpyDeviceToDevice, st));
+        GM_HIP(hipMemcpyAsync(buf0.y, buf1.y, b, hipMemcpyDeviceToDevice, st));
+        GM_HIP(hipMemcpyAsync(buf0.z, buf1.z, b, hipMemcpyDeviceToDevice, st));
+    }
+    if (iters) *iters = flag[1];
+    if (!flag[0]) {
+        std::vector<double> lam((size_t)d->nc);
+        std::vector<uint32_t> hi(1), hj(1);
+        GM_HIP(hipMemcpyAsync(lam.data(), d->lambda.p, sizeof(double) * lam.size(), hipMemcpyDeviceToHost, st));
+        GM_HIP(hipStreamSynchronize(st));
+        int64_t w = 0;
+        double v = 0.0;
+        worst_of(lam, w, v);
+        GM_HIP(hipMemcpy(hi.data(), d->ci.p + w, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        GM_HIP(hipMemcpy(hj.data(), d->cj.p + w, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        return set_error(GM_ERR_CONVERGENCE,
+                         std::string(velocity ? "velocity constraints" : "constraint solver") +
+                             " did not converge at step " + std::to_string(step) + ": constraint " +
+                             std::to_string(w) + " (atoms " + std::to_string(hi[0]) + ", " +
+                             std::to_string(hj[0]) + "), residual " + std::to_string(v) + " after " +
+                             std::to_string(max_iter) + " iterations");
+    }
+    return GM_OK;
+}
+
+gm_status drift_and_constrain(gm_sys *s, double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                              double dt, int max_iter, double tol, int64_t step, int32_t *iters) {
+    gm_dyn *d = s->dyn;
+    const int64_t n = d->n;
+    double *rx = d->ref.p, *ry = d->ref.p + n, *rz = d->ref.p + 2 * n;
+    GM_HIP(launch_drift(n, x, y, z, vx, vy, vz, rx, ry, rz, dt, s->stream));
+    gm_status rc = project(s, false, Vec3{x, y, z}, Vec3C{rx, ry, rz}, max_iter, tol, step, iters);
+    if (rc != GM_OK) return rc;
+    GM_HIP(launch_resync(n, x, y, z, rx, ry, rz, vx, vy, vz, dt, s->stream));
+    return GM_OK;
+}
+
+// 0.5 * sum_i m v^2 / 20.455^2 in atom order (geodesic/src/lib.rs:68-77).
+gm_status kinetic_energy(gm_sys *s, const double *vx, const double *vy, const double *vz, double *out) {
+    gm_dyn *d = s->dyn;
+    GM_HIP(launch_kinetic_terms(d->n, vx, vy, vz, d->mass.p, d->ke.p, s->stream));
+    std::vector<double> t((size_t)d->n);
+    if (d->n > 0)
+        GM_HIP(hipMemcpyAsync(t.data(), d->ke.p, sizeof(double) * t.size(), hipMemcpyDeviceToHost, s->stream));
+    GM_HIP(hipStreamSynchronize(s->stream));
+    double sum = 0.0;
+    for (double v : t) sum += v;
+    *out = 0.5 * sum * (1.0 / (20.455 * 20.455));
+    return GM_OK;
+}
+
+gm_status frame_to_host(gm_sys *s, const double *x, const double *y, const double *z, float *dst) {
+    const size_t n = (size_t)s->n;
+    std::vector<double> h(3 * n);
+    if (n > 0) {
+        GM_HIP(hipMemcpyAsync(h.data(), x, 8 * n, hipMemcpyDeviceToHost, s->stream));
+        GM_HIP(hipMemcpyAsync(h.data() + n, y, 8 * n, hipMemcpyDeviceToHost, s->stream));
+        GM_HIP(hipMemcpyAsync(h.data() + 2 * n, z, 8 * n, hipMemcpyDeviceToHost, s->stream));
+    }
+    GM_HIP(hipStreamSynchronize(s->stream));
+    for (size_t k = 0; k < 3 * n; k++) dst[k] = (float)h[k];
+    return GM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+gm_status gm_dynamics_set(gm_sys *s, const double *mass, int64_t n_constr, const uint32_t *ci, const uint32_t *cj,
+                          const double *dsq) {
+    GM_REQUIRE(s && (s->n == 0 || mass) && n_constr >= 0 && (n_constr == 0 || (ci && cj && dsq)),
+               "gm_dynamics_set: bad argument");
+    const int64_t n = s->n;
+    for (int64_t i = 0; i < n; i++)
+        GM_REQUIRE(std::isfinite(mass[i]) && mass[i] > 0.0, "gm_dynamics_set: masses must be positive");
+    for (int64_t c = 0; c < n_constr; c++)
+        GM_REQUIRE(ci[c] < (uint64_t)n && cj[c] < (uint64_t)n && ci[c] != cj[c],
+                   "gm_dynamics_set: constraint atom index out of range");
+    gm_dyn_free(s->dyn);
+    s->dyn = nullptr;
+    gm_dyn *d = new gm_dyn();
+    d->n = n;
+    d->nc = n_constr;
+    // per-atom incident constraints, ascending constraint index
+    std::vector<int64_t> off((size_t)n + 1, 0), con((size_t)(2 * n_constr));
+    for (int64_t c = 0; c < n_constr; c++) {
+        off[(size_t)ci[c] + 1]++;
+        off[(size_t)cj[c] + 1]++;
+    }
+    for (int64_t i = 0; i < n; i++) off[(size_t)i + 1] += off[(size_t)i];
+    std::vector<int64_t> fill(off.begin(), off.end() - 1);
+    for (int64_t c = 0; c < n_constr; c++) {
+        con[(size_t)fill[ci[c]]++] = c;
+        con[(size_t)fill[cj[c]]++] = c;
+    }
+    const size_t N = (size_t)std::max<int64_t>(n, 1), C = (size_t)std::max<int64_t>(n_constr, 1);
+    auto fail = [&](hipError_t e) {
+        gm_dyn_free(d);
+        return from_hip(e, "gm_dynamics_set");
+    };
+    hipError_t e;
+    if ((e = d->mass.ensure(N)) != hipSuccess || (e = d->dsq.ensure(C)) != hipSuccess ||
+        (e = d->ci.ensure(C)) != hipSuccess || (e = d->cj.ensure(C)) != hipSuccess ||
+        (e = d->inc_off.ensure(N + 1)) != hipSuccess || (e = d->inc_con.ensure(2 * C)) != hipSuccess ||
+        (e = d->ref.ensure(3 * N)) != hipSuccess || (e = d->scratch.ensure(3 * N)) != hipSuccess ||
+        (e = d->lambda.ensure(C)) != hipSuccess || (e = d->force.ensure(3 * N)) != hipSuccess ||
+        (e = d->ke.ensure(N)) != hipSuccess || (e = d->flag.ensure(3)) != hipSuccess)
+        return fail(e);
+    if (n > 0 && (e = hipMemcpy(d->mass.p, mass, 8 * (size_t)n, hipMemcpyHostToDevice)) != hipSuccess) return fail(e);
+    if ((e = hipMemcpy(d->inc_off.p, off.data(), 8 * off.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e);
+    if (n_constr > 0) {
+        if ((e = hipMemcpy(d->ci.p, ci, 4 * (size_t)n_constr, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(d->cj.p, cj, 4 * (size_t)n_constr, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(d->dsq.p, dsq, 8 * (size_t)n_constr, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(d->inc_con.p, con.data(), 8 * con.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return fail(e);
+    }
+    s->dyn = d;
+    return GM_OK;
+}
+
+gm_status gm_half_kick_device(gm_sys *s, double *vx, double *vy, double *vz, const double *fx, const double *fy,
+                              const double *fz, double dt_half) {
+    GM_REQUIRE(s && s->dyn, "gm_half_kick_device: no dynamics attached (gm_dynamics_set)");
+    GM_REQUIRE(s->n == 0 || (vx && vy && vz && fx && fy && fz), "gm_half_kick_device: NULL array");
+    GM_HIP(launch_half_kick(s->n, vx, vy, vz, fx, fy, fz, s->dyn->mass.p, dt_half, s->stream));
+    return GM_OK;
+}
+
+gm_status gm_drift_and_constrain_device(gm_sys *s, double *x, double *y, double *z, double *vx, double *vy,
+                                        double *vz, double dt, int32_t max_iter, double tol, int64_t step,
+                                        int32_t *iters) {
+    GM_REQUIRE(s && s->dyn, "gm_drift_and_constrain_device: no dynamics attached (gm_dynamics_set)");
+    GM_REQUIRE(s->n == 0 || (x && y && z && vx && vy && vz), "gm_drift_and_constrain_device: NULL array");
+    GM_REQUIRE(max_iter > 0 && dt != 0.0, "gm_drift_and_constrain_device: need max_iter > 0, dt != 0");
+    gm_status rc = drift_and_constrain(s, x, y, z, vx, vy, vz, dt, max_iter, tol, step, iters);
+    if (rc != GM_OK) return rc;
+    GM_HIP(hipStreamSynchronize(s->stream));
+    return GM_OK;
+}
+
+gm_status gm_constrain_velocities_device(gm_sys *s, const double *x, const double *y, const double *z, double *vx,
+                                         double *vy, double *vz, int32_t max_iter, double tol, int64_t step,
+                                         int32_t *iters) {
+    GM_REQUIRE(s && s->dyn, "gm_constrain_velocities_device: no dynamics attached (gm_dynamics_set)");
+    GM_REQUIRE(s->n == 0 || (x && y && z && vx && vy && vz), "gm_constrain_velocities_device: NULL array");
+    GM_REQUIRE(max_iter > 0, "gm_constrain_velocities_device: need max_iter > 0");
+    gm_status rc = project(s, true, Vec3{vx, vy, vz}, Vec3C{x, y, z}, max_iter, tol, step, iters);
+    if (rc != GM_OK) return rc;
+    GM_HIP(hipStreamSynchronize(s->stream));
+    return GM_OK;
+}
+
+gm_status gm_md_run(gm_sys *s, double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                    const gm_md_desc *md, float *h_frames, double *h_epot, double *h_ekin, int64_t *n_frames) {
+    GM_REQUIRE(s && s->dyn && md, "gm_md_run: no dynamics attached (gm_dynamics_set) or NULL description");
+    GM_REQUIRE(s->n == 0 || (x && y && z && vx && vy && vz), "gm_md_run: NULL array");
+    GM_REQUIRE(md->n_steps >= 0 && md->frame_interval > 0 && md->max_iter > 0 && md->dt != 0.0,
+               "gm_md_run: need n_steps >= 0, frame_interval > 0, max_iter > 0, dt != 0");
+    const int64_t n = s->n;
+    double *fx = s->dyn->force.p, *fy = fx + n, *fz = fx + 2 * n;
+    const double dt = md->dt;
+    int64_t frames = 0;
+    auto record = [&](double epot) -> gm_status {
+        double ke = 0.0;
+        gm_status r = kinetic_energy(s, vx, vy, vz, &ke);
+        if (r != GM_OK) return r;
+        if (h_frames && (r = frame_to_host(s, x, y, z, h_frames + (size_t)frames * 3 * (size_t)n)) != GM_OK)
+            return r;
+        if (h_epot) h_epot[frames] = epot;
+        if (h_ekin) h_ekin[frames] = ke;
+        frames++;
+        return GM_OK;
+    };
+    // lib.rs:255-276: list, forces, velocity projection, frame 0
+    gm_status rc = gm_build_neighbor_list_device(s, x, y, z);
+    if (rc != GM_OK) return rc;
+    if ((rc = compute_on_device(s, x, y, z, fx, fy, fz)) != GM_OK) return rc;
+    gm_energy_terms e{};
+    if ((rc = collect(s, &e)) != GM_OK) return rc;
+    if ((rc = project(s, true, Vec3{vx, vy, vz}, Vec3C{x, y, z}, md->max_iter, md->tol, 0, nullptr)) != GM_OK)
+        return rc;
+    if ((rc = record(e.total)) != GM_OK) return rc;
+    // lib.rs:283-322: B, A (drift + SHAKE), rebuild if needed, forces, B, RATTLE
+    for (int64_t step = 0; step < md->n_steps; step++) {
+        GM_HIP(launch_half_kick(n, vx, vy, vz, fx, fy, fz, s->dyn->mass.p, dt / 2.0, s->stream));
+        if ((rc = drift_and_constrain(s, x, y, z, vx, vy, vz, dt, md->max_iter, md->tol, step, nullptr)) != GM_OK)
+            return rc;
+        int32_t rebuild = 0;
+        if ((rc = gm_needs_rebuild_device(s, x, y, z, &rebuild)) != GM_OK) return rc;
+        if (rebuild && (rc = gm_build_neighbor_list_device(s, x, y, z)) != GM_OK) return rc;
+        if ((rc = compute_on_device(s, x, y, z, fx, fy, fz)) != GM_OK) return rc;
+        GM_HIP(launch_half_kick(n, vx, vy, vz, fx, fy, fz, s->dyn->mass.p, dt / 2.0, s->stream));
+        if ((rc = project(s, true, Vec3{vx, vy, vz}, Vec3C{x, y, z}, md->max_iter, md->tol, step, nullptr)) != GM_OK)
+            return rc;
+        if ((step + 1) % md->frame_interval == 0) {
+            if ((rc = collect(s, &e)) != GM_OK) return rc;
+            if ((rc = record(e.total)) != GM_OK) return rc;
+        }
+    }
+    GM_HIP(hipStreamSynchronize(s->stream));
+    if (n_frames) *n_frames = frames;
+    return GM_OK;
+}
+
+gm_status gm_jacobi_factor(gm_sys *s, double total_energy, double *out) {
+    GM_REQUIRE(s && out, "gm_jacobi_factor: NULL argument");
+    if (!s->built) return set_error(GM_ERR_NOT_BUILT, "gm_jacobi_factor before any force evaluation");
+    gm_energy_terms e{};
+    gm_status rc = collect(s, &e);
+    if (rc != GM_OK) return rc;
+    *out = 2.0 * (total_energy - e.total);
+    return GM_OK;
+}
+
+}  // extern "C"

@@ -169,6 +169,7 @@ struct gm_sys {
     int prof_every = 1;
     int64_t last_timed = -1;  // pool index of the last profiled evaluation, -1 if unprofiled
     gm::NbParams nbp_shell{}; // shell-mode constants (set by a surrogate that owns this system)
+    struct gm_dyn *dyn = nullptr;  // masses, constraints and dynamics buffers (gm_dynamics_set)
     std::vector<int> sc_start_host;
 
     bool built = false;
@@ -188,6 +189,9 @@ gm_status from_hip(hipError_t e, const char *what);
         if (_e != hipSuccess) return gm::from_hip(_e, #call);          \
     } while (0)
 
+struct gm_dyn;
+void gm_dyn_free(gm_dyn *d);  // gm_dynamics.cpp
+
 // ---- host helpers shared by the single-system and batch APIs (gm_api.cpp) --
 namespace gm {
 gm_status system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream_t ext_stream, gm_sys **out);
@@ -197,6 +201,8 @@ gm_status build_on_device(gm_sys *s, bool wrap);
 // (nbp_shell), no bonded terms.
 gm_status compute_on_device(gm_sys *s, const double *x, const double *y, const double *z, double *fx,
                             double *fy, double *fz, bool shell = false);
+// Waits for the system's stream and reads the last evaluation's energies.
+gm_status collect(gm_sys *s, gm_energy_terms *e);
 gm_status make_work_items(gm_sys *s);
 }  // namespace gm
 

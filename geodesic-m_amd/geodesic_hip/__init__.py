@@ -105,6 +105,11 @@ class TrainReport(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class MdDesc(C.Structure):
+    _fields_ = [("n_steps", _I64), ("dt", _D), ("frame_interval", C.c_int32), ("max_iter", C.c_int32),
+                ("tol", _D)]
+
+
 SUR_NPARAM = 29315
 SUR_NFEAT = 96
 MTS_HELD, MTS_NET, MTS_TAYLOR = 0, 1, 2
@@ -180,6 +185,14 @@ def lib():
             "gm_surrogate_shell_forces": ([_P, _P, _P, _P, _P, _P, _P], C.c_int),
             "gm_needs_rebuild_device": ([_P, _P, _P, _P, C.POINTER(C.c_int32)], C.c_int),
             "gm_surrogate_get_hessian": ([_P, _P], C.c_int),
+            "gm_dynamics_set": ([_P, _P, _I64, _P, _P, _P], C.c_int),
+            "gm_half_kick_device": ([_P, _P, _P, _P, _P, _P, _P, C.c_double], C.c_int),
+            "gm_drift_and_constrain_device": ([_P, _P, _P, _P, _P, _P, _P, C.c_double, C.c_int32, C.c_double,
+                                               _I64, C.POINTER(C.c_int32)], C.c_int),
+            "gm_constrain_velocities_device": ([_P, _P, _P, _P, _P, _P, _P, C.c_int32, C.c_double, _I64,
+                                                C.POINTER(C.c_int32)], C.c_int),
+            "gm_md_run": ([_P, _P, _P, _P, _P, _P, _P, C.POINTER(MdDesc), _P, _P, _P, C.POINTER(_I64)], C.c_int),
+            "gm_jacobi_factor": ([_P, C.c_double, C.POINTER(C.c_double)], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -422,6 +435,49 @@ class System:
         e = EnergyTerms()
         _check(lib().gm_sync(self.h, C.byref(e)))
         return e
+
+    # ---- device-resident dynamics (integrator.rs, constraint.rs, lib.rs run loop)
+
+    def dynamics_set(self, mass, constr_i=(), constr_j=(), constr_dsq=()):
+        m = np.ascontiguousarray(mass, np.float64)
+        ci = np.ascontiguousarray(constr_i, np.uint32)
+        cj = np.ascontiguousarray(constr_j, np.uint32)
+        dsq = np.ascontiguousarray(constr_dsq, np.float64)
+        _check(lib().gm_dynamics_set(self.h, _p(m), len(ci), _p(ci), _p(cj), _p(dsq)))
+
+    def half_kick_device(self, dvx, dvy, dvz, dfx, dfy, dfz, dt_half):
+        _check(lib().gm_half_kick_device(self.h, *(_P(a.ptr) for a in (dvx, dvy, dvz, dfx, dfy, dfz)),
+                                         float(dt_half)))
+
+    def drift_and_constrain_device(self, dx, dy, dz, dvx, dvy, dvz, dt, max_iter, tol, step=0):
+        it = C.c_int32(0)
+        _check(lib().gm_drift_and_constrain_device(self.h, *(_P(a.ptr) for a in (dx, dy, dz, dvx, dvy, dvz)),
+                                                   float(dt), int(max_iter), float(tol), int(step), C.byref(it)))
+        return it.value
+
+    def constrain_velocities_device(self, dx, dy, dz, dvx, dvy, dvz, max_iter, tol, step=0):
+        it = C.c_int32(0)
+        _check(lib().gm_constrain_velocities_device(self.h, *(_P(a.ptr) for a in (dx, dy, dz, dvx, dvy, dvz)),
+                                                    int(max_iter), float(tol), int(step), C.byref(it)))
+        return it.value
+
+    def md_run(self, dx, dy, dz, dvx, dvy, dvz, n_steps, dt, frame_interval=1, max_iter=100, tol=1e-8):
+        """Returns (frames [n_frames, 3, n] f32, potential, kinetic)."""
+        md = MdDesc(n_steps=int(n_steps), dt=float(dt), frame_interval=int(frame_interval),
+                    max_iter=int(max_iter), tol=float(tol))
+        cap = int(n_steps) // int(frame_interval) + 1
+        frames = np.empty((cap, 3, self.n), np.float32)
+        epot, ekin = np.empty(cap), np.empty(cap)
+        nf = _I64(0)
+        _check(lib().gm_md_run(self.h, *(_P(a.ptr) for a in (dx, dy, dz, dvx, dvy, dvz)), C.byref(md),
+                               _p(frames), _p(epot), _p(ekin), C.byref(nf)))
+        k = nf.value
+        return frames[:k], epot[:k], ekin[:k]
+
+    def jacobi_factor(self, total_energy):
+        out = C.c_double(0.0)
+        _check(lib().gm_jacobi_factor(self.h, float(total_energy), C.byref(out)))
+        return out.value
 
     def profile_begin(self, every=1):
         """Time the next evaluation and every `every`-th one after it with HIP events."""

@@ -47,6 +47,7 @@ enum {
     GM_ERR_KERNEL = 6,        /* a kernel launch or HIP call failed */
     GM_ERR_NOT_BUILT = 7,     /* compute before build_neighbor_list */
     GM_ERR_CAPACITY = 8,      /* output buffer too small / fixed-point or LDS capacity exceeded */
+    GM_ERR_CONVERGENCE = 9,   /* ConvergenceError::ConstraintSolver (error.rs): SHAKE/RATTLE did not converge */
 };
 
 typedef struct gm_ctx gm_ctx; /* one per device */
@@ -189,6 +190,47 @@ gm_status gm_profile_end(gm_sys *sys, gm_profile *out);
  * 32); 0 picks a size that gives about 24 work items per compute unit.  Takes effect
  * immediately on a built list (no rebuild). */
 gm_status gm_set_work_chunk(gm_sys *sys, int32_t entries_per_item);
+
+/* ---- Device-resident dynamics ----------------------------------------------
+ * integrator.rs:14-71 (half_kick, drift_and_constrain), constraint.rs:62-259
+ * (solve, constrain_velocities) and the run loop of geodesic/src/lib.rs:255-330,
+ * on device arrays, in f64 with the reference's operation order: given the
+ * same forces every step is bit-for-bit the CPU path's.  Masses and
+ * holonomic constraints (BondedTopology constr_i/constr_j/constr_dsq,
+ * topology.rs:26-32) are attached to a system once. */
+gm_status gm_dynamics_set(gm_sys *sys, const double *mass, int64_t n_constr, const uint32_t *constr_i,
+                          const uint32_t *constr_j, const double *constr_dsq);
+/* v += dt_half F / m * 20.455^2 (ordered on the system's stream). */
+gm_status gm_half_kick_device(gm_sys *sys, double *d_vx, double *d_vy, double *d_vz, const double *d_fx,
+                              const double *d_fy, const double *d_fz, double dt_half);
+/* r += dt v, SHAKE projection, v = (r - r_old) / dt.  *iters (may be NULL)
+ * receives the solver's iteration count; GM_ERR_CONVERGENCE after max_iter. */
+gm_status gm_drift_and_constrain_device(gm_sys *sys, double *d_x, double *d_y, double *d_z, double *d_vx,
+                                        double *d_vy, double *d_vz, double dt, int32_t max_iter, double tol,
+                                        int64_t step, int32_t *iters);
+/* RATTLE velocity projection (constraint.rs:159-259). */
+gm_status gm_constrain_velocities_device(gm_sys *sys, const double *d_x, const double *d_y, const double *d_z,
+                                         double *d_vx, double *d_vy, double *d_vz, int32_t max_iter, double tol,
+                                         int64_t step, int32_t *iters);
+typedef struct {
+    int64_t n_steps;         /* SimParams::n_steps */
+    double dt;               /* ps */
+    int32_t frame_interval;  /* steps between frames */
+    int32_t max_iter;        /* constraint iterations */
+    double tol;              /* constraint tolerance */
+} gm_md_desc;
+/* The reference run loop on the device: list, forces and velocity projection,
+ * then per step B, A (drift + SHAKE), list rebuild when needs_rebuild fires,
+ * forces, B, RATTLE.  Every frame_interval steps (and at step 0) the f32
+ * positions (3 x n: x block, y block, z block) go to h_frames, the potential
+ * and kinetic energy (kcal/mol) to h_epot / h_ekin (each may be NULL);
+ * *n_frames receives the count.  Positions and velocities advance in place. */
+gm_status gm_md_run(gm_sys *sys, double *d_x, double *d_y, double *d_z, double *d_vx, double *d_vy,
+                    double *d_vz, const gm_md_desc *md, float *h_frames, double *h_epot, double *h_ekin,
+                    int64_t *n_frames);
+/* Jacobi-metric factor 2 (E - V(x)) in f64 for total energy E (SimParams
+ * total_energy, params.rs:21-22) and the potential of the last evaluation. */
+gm_status gm_jacobi_factor(gm_sys *sys, double total_energy, double *out);
 
 /* ---- Batched replicas [ext: no reference counterpart] ---------------------
  * Many independent systems (own atom counts, boxes, topologies, parameters)

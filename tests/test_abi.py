@@ -54,6 +54,9 @@ int main(void) {
   F(gm_surrogate_desc, fade) F(gm_surrogate_desc, atom_class) F(gm_surrogate_desc, seed)
   F(gm_train_desc, mu) F(gm_train_desc, cap) F(gm_train_desc, lr) F(gm_train_desc, batch) F(gm_train_desc, update)
   F(gm_train_report, loss_taylor) F(gm_train_report, ms_total) F(gm_train_report, steps)
+  printf("gm_md_desc %zu\n", sizeof(gm_md_desc));
+  F(gm_md_desc, dt) F(gm_md_desc, frame_interval) F(gm_md_desc, max_iter) F(gm_md_desc, tol)
+  printf("GM_ERR_CONVERGENCE %d\n", (int)GM_ERR_CONVERGENCE);
   printf("GM_SUR_NPARAM %d\n", (int)GM_SUR_NPARAM);
   printf("GM_SUR_NFEAT %d\n", (int)GM_SUR_NFEAT);
   return 0;
@@ -75,8 +78,9 @@ def test_ctypes_structs_match_c_layout(tmp_path):
     assert int(out["gm_train_desc"]) == C.sizeof(gh.TrainDesc)
     assert int(out["gm_train_report"]) == C.sizeof(gh.TrainReport)
     assert int(out["GM_SUR_NPARAM"]) == gh.SUR_NPARAM and int(out["GM_SUR_NFEAT"]) == gh.SUR_NFEAT
+    assert int(out["gm_md_desc"]) == C.sizeof(gh.MdDesc) and int(out["GM_ERR_CONVERGENCE"]) == 9
     classes = {"gm_system_desc": gh.SystemDesc, "gm_stats": gh.Stats, "gm_surrogate_desc": gh.SurrogateDesc,
-               "gm_train_desc": gh.TrainDesc, "gm_train_report": gh.TrainReport}
+               "gm_train_desc": gh.TrainDesc, "gm_train_report": gh.TrainReport, "gm_md_desc": gh.MdDesc}
     for key, val in out.items():
         if "." in key:
             t, m = key.split(".")
