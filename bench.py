@@ -149,7 +149,7 @@ def bench_batched(n_replicas=512, steps=20):
         b.compute()
     _, last_ms = b.sync()
     batched_ms = (time.perf_counter() - t0) * 1e3 / steps
-    # plain path: each replica's own two launches, back to back on one stream
+    # plain path: each replica's own launches (prep, forces, finish), back to back on one stream
     plain = [b.replica(k) for k in range(len(reps))]
     for g in plain:
         g.compute_forces_resident()
@@ -271,7 +271,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the force path shards by replica or by spatial domain, see DESIGN.md)", file=sys.stderr)
+              "(the force path would shard by replica or by spatial domain across processes)", file=sys.stderr)
         sys.exit(2)
 
     if args.only == "250k":
@@ -308,6 +308,30 @@ def main():
     ms_host = (time.perf_counter() - t0) * 1e3 / n_host
     nb_ms = prof["ms_nonbonded_total"] / max(prof["n_evals"], 1)
     ev_ms = prof["ms_eval_total"] / max(prof["n_evals"], 1)
+    prep_ms = prof["ms_prep_total"] / max(prof["n_evals"], 1)
+    # Tile pruning skips tiles that lie beyond r_c at the current positions,
+    # bounded by the largest displacement since the build: the same evaluation
+    # with pruning off, and at positions drifted as after a whole 20-step
+    # rebuild interval of 0.02 A Gaussian moves per coordinate (the 250k
+    # workload's protocol), where the bound has grown the most.
+    g.set_tile_pruning(False)
+    ms_off, prof_off, e_off, _ = time_resident(g, args.steps, args.warmup)
+    g.set_tile_pruning(True)
+    assert e_off.total == e.total
+    rng = np.random.default_rng(20)
+    d = rng.normal(0.0, 0.02 * 20 ** 0.5, (3, s.n))
+    g.set_positions(x + d[0], y + d[1], z + d[2])
+    ms_drift, prof_drift, _, _ = time_resident(g, args.steps, args.warmup)
+    g.set_tile_pruning(False)
+    ms_drift_off, _, _, _ = time_resident(g, args.steps, args.warmup)
+    g.set_tile_pruning(True)
+    g.set_positions(x, y, z)
+    pruning = {"ms_per_step_pruning_off": ms_off,
+               "kernel_ms_pruning_off": prof_off["ms_nonbonded_total"] / max(prof_off["n_evals"], 1),
+               "drift_max_displacement_A": float(np.sqrt((d ** 2).sum(axis=0)).max()),
+               "ms_per_step_drifted": ms_drift,
+               "kernel_ms_drifted": prof_drift["ms_nonbonded_total"] / max(prof_drift["n_evals"], 1),
+               "ms_per_step_drifted_pruning_off": ms_drift_off}
     achieved = n_cut * FLOP_PER_PAIR / (nb_ms * 1e-3) / 1e12
 
     out = {
@@ -330,10 +354,12 @@ def main():
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic_from_profiles("nonbonded_24k_traffic.json"),
-                     "kernel": "k_forces (tiles + bonded workgroups)", "kernel_ms_avg": nb_ms,
+                     "kernel": "k_forces<cubic> (tiles + bonded workgroups)", "kernel_ms_avg": nb_ms,
                      "kernel_ms_min": prof["ms_nonbonded_min"], "flop_per_pair": FLOP_PER_PAIR,
                      "timed_launches": prof["n_evals"], "events_every": PROFILE_EVERY},
         "device_ms_per_eval": ev_ms,
+        "prep_kernel_ms": prep_ms,
+        "tile_pruning": pruning,
         "ms_per_step_events_every_step": ms_all_events,
         "ms_per_eval_host_buffers": ms_host,
         "list_build_ms": st["ms_build"],

@@ -97,6 +97,7 @@ gm_status make_work_items(gm_sys *s) {
 }
 
 constexpr int64_t kProfMax = 8192;
+constexpr int kProfStride = 4;  // events per profiled evaluation
 
 gm_status build_on_device(gm_sys *s, bool wrap) {
     hipStream_t st = s->stream;
@@ -134,7 +135,7 @@ gm_status build_on_device(gm_sys *s, bool wrap) {
     GM_HIP(s->ent_imask.ensure((size_t)total));
     GM_HIP(s->ent_mask.ensure((size_t)total * 8));
     GM_HIP(s->ent_meta.ensure((size_t)total));
-    GM_HIP(s->ent_ja.ensure((size_t)total * 8));
+    GM_HIP(s->ent_dmin.ensure((size_t)total));
     GM_HIP(s->ent_lb.ensure((size_t)total * 64));
     GM_HIP(upload(s->sc_start, start.data(), start.size(), st));
 
@@ -170,34 +171,35 @@ gm_status compute_on_device(gm_sys *s, const double *x, const double *y, const d
                             double *fx, double *fy, double *fz, bool shell) {
     if (!s->built) return set_error(GM_ERR_NOT_BUILT, "compute_forces before build_neighbor_list");
     hipStream_t st = s->stream;
-    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    hipEvent_t ev[kProfStride] = {nullptr, nullptr, nullptr, nullptr};
     if (s->profiling && (s->prof_seen++ % s->prof_every) == 0) {
         if (s->prof_used < kProfMax) {
-            const size_t need = 3 * (size_t)(s->prof_used + 1);
+            const size_t need = kProfStride * (size_t)(s->prof_used + 1);
             while (s->prof_pool.size() < need) {
-                hipEvent_t ev;
-                GM_HIP(hipEventCreate(&ev));
-                s->prof_pool.push_back(ev);
+                hipEvent_t e;
+                GM_HIP(hipEventCreate(&e));
+                s->prof_pool.push_back(e);
             }
-            const size_t b = 3 * (size_t)s->prof_used;
-            e0 = s->prof_pool[b];
-            e1 = s->prof_pool[b + 1];
-            e2 = s->prof_pool[b + 2];
+            const size_t b = kProfStride * (size_t)s->prof_used;
+            for (int k = 0; k < kProfStride; k++) ev[k] = s->prof_pool[b + (size_t)k];
             s->prof_used++;
         } else {
             s->prof_untimed++;
         }
     }
     // Unprofiled evaluations record no events: the stream holds only kernels.
-    if (e0) GM_HIP(hipEventRecord(e0, st));
+    // Profiled ones: start | prep | force kernel | finish, an event after each.
+    if (ev[0]) GM_HIP(hipEventRecord(ev[0], st));
     if (s->F_dirty) GM_HIP(launch_fill_u64(s->F.p, (size_t)s->nsc * 64 * 3, st));
     s->F_dirty = true;
+    GM_HIP(launch_prep(s, x, y, z));
+    if (ev[1]) GM_HIP(hipEventRecord(ev[1], st));
     GM_HIP(launch_forces(s, x, y, z, shell));
-    if (e1) GM_HIP(hipEventRecord(e1, st));
+    if (ev[2]) GM_HIP(hipEventRecord(ev[2], st));
     GM_HIP(launch_finish(s, fx, fy, fz, shell));  // also clears the accumulator
     s->F_dirty = false;
-    if (e2) GM_HIP(hipEventRecord(e2, st));
-    s->last_timed = e0 ? s->prof_used - 1 : -1;
+    if (ev[3]) GM_HIP(hipEventRecord(ev[3], st));
+    s->last_timed = ev[0] ? s->prof_used - 1 : -1;
     return GM_OK;
 }
 
@@ -206,10 +208,10 @@ gm_status collect(gm_sys *s, gm_energy_terms *e) {
     double t[5] = {0, 0, 0, 0, 0};
     GM_HIP(hipMemcpy(t, s->e_out.p, sizeof(t), hipMemcpyDeviceToHost));
     if (s->last_timed >= 0) {
-        const size_t k = 3 * (size_t)s->last_timed;
+        const size_t k = kProfStride * (size_t)s->last_timed;
         float a = 0.f, b = 0.f;
-        GM_HIP(hipEventElapsedTime(&a, s->prof_pool[k], s->prof_pool[k + 1]));
-        GM_HIP(hipEventElapsedTime(&b, s->prof_pool[k], s->prof_pool[k + 2]));
+        GM_HIP(hipEventElapsedTime(&a, s->prof_pool[k + 1], s->prof_pool[k + 2]));
+        GM_HIP(hipEventElapsedTime(&b, s->prof_pool[k], s->prof_pool[k + 3]));
         s->stats.ms_nonbonded = a;
         s->stats.ms_eval = b;
     }
@@ -379,6 +381,8 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     s->nbp.rsw = (float)d->r_switch;
     s->nbp.inv_w = (float)(1.0 / (d->r_cutoff - d->r_switch));
     s->nbp.ds_coef = (float)(-30.0 / (d->r_cutoff - d->r_switch));
+    s->nbp.nrw = (float)(-d->r_switch / (d->r_cutoff - d->r_switch));
+    s->nbp.nrw_in = 0.f;
     {
         const double step = d->box[0] / two32;  // fixed-point step, A (cubic boxes)
         s->nbp.inv_s = (float)(1.0 / step);
@@ -419,6 +423,11 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     bp.wcol[1] = (float)(d->box[1] / bp.ncy);
     bp.kx = (int)std::min<double>(bp.ncx, std::floor(bp.r_search / bp.wcol[0]) + 1.0);
     bp.ky = (int)std::min<double>(bp.ncy, std::floor(bp.r_search / bp.wcol[1]) + 1.0);
+    bp.prune_rc = (float)d->r_cutoff;
+    bp.prune_inv_step = (float)(kPruneSteps / (d->r_skin - d->r_cutoff));
+    // the pruning bound needs every pair's minimum image to stay the nearest
+    // one under displacements below (r_skin - r_cutoff) / 2: box edges >= 2 r_skin
+    s->prune_geom = std::min(d->box[0], std::min(d->box[1], d->box[2])) >= 2.0 * d->r_skin * (1.0 + 1e-6);
     bp.periodic = s->periodic;
     bp.n_atoms = (int)n;
     s->max_sc = (n + 63) / 64 + bp.ncol;
@@ -500,6 +509,10 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     TRY(s->slot_atom.ensure(slots));
     TRY(s->sq.ensure(slots));
     TRY(s->sp.ensure(slots));
+    TRY(s->qpos.ensure(slots));
+    TRY(s->qeps.ensure(slots));
+    TRY(s->disp.ensure(1));
+    TRY(launch_fill_i32(reinterpret_cast<int *>(s->disp.p), 1, 0, st));
     TRY(s->cl_ref.ensure((size_t)s->max_sc * 8));
     TRY(s->cl_half.ensure((size_t)s->max_sc * 8));
     TRY(s->sc_ref.ensure((size_t)s->max_sc));
@@ -546,7 +559,10 @@ gm_status gm_system_destroy(gm_sys *s) {
     s->sc_half.release();
     s->ent_mask.release();
     s->ent_meta.release();
-    s->ent_ja.release();
+    s->ent_dmin.release();
+    s->qpos.release();
+    s->qeps.release();
+    s->disp.release();
     s->ent_lb.release();
     s->F.release();
     for (hipEvent_t ev : {s->ev_b0, s->ev_b1})
@@ -764,9 +780,12 @@ gm_status gm_profile_end(gm_sys *s, gm_profile *out) {
     p.ms_nonbonded_min = 0.0;
     p.ms_nonbonded_max = 0.0;
     for (int64_t k = 0; k < s->prof_used; k++) {
-        float nb = 0.f, ev = 0.f;
-        GM_HIP(hipEventElapsedTime(&nb, s->prof_pool[3 * (size_t)k], s->prof_pool[3 * (size_t)k + 1]));
-        GM_HIP(hipEventElapsedTime(&ev, s->prof_pool[3 * (size_t)k], s->prof_pool[3 * (size_t)k + 2]));
+        float nb = 0.f, ev = 0.f, pr = 0.f;
+        const size_t b = kProfStride * (size_t)k;
+        GM_HIP(hipEventElapsedTime(&pr, s->prof_pool[b], s->prof_pool[b + 1]));
+        GM_HIP(hipEventElapsedTime(&nb, s->prof_pool[b + 1], s->prof_pool[b + 2]));
+        GM_HIP(hipEventElapsedTime(&ev, s->prof_pool[b], s->prof_pool[b + 3]));
+        p.ms_prep_total += pr;
         p.ms_nonbonded_total += nb;
         p.ms_eval_total += ev;
         if (k == 0 || nb < p.ms_nonbonded_min) p.ms_nonbonded_min = nb;
@@ -782,6 +801,12 @@ gm_status gm_set_work_chunk(gm_sys *s, int32_t entries_per_item) {
     s->chunk = entries_per_item;
     if (!s->built) return GM_OK;
     return make_work_items(s);
+}
+
+gm_status gm_set_tile_pruning(gm_sys *s, int32_t enable) {
+    GM_REQUIRE(s && (enable == 0 || enable == 1), "bad argument");
+    s->prune = enable != 0;
+    return GM_OK;
 }
 
 gm_status gm_get_stats(gm_sys *s, gm_stats *out) {

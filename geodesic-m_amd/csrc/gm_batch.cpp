@@ -6,11 +6,12 @@
 // the batch's stream.  After the lists are built, a device table of replica
 // descriptors and two item lists (replica, bonded group) and (replica, tile
 // work item) are written.  k_batch_bonded runs the bonded terms on a side
-// branch while the persistent k_batch_tiles walks the tile items with a
-// fixed grid of wavefronts (static, XCD-contiguous ranges); k_batch_finish
+// branch while k_batch_prep writes every replica's slot-order fixed-point
+// positions and displacement bound and the persistent k_batch_tiles walks the
+// tile items with a fixed grid of wavefronts (static, XCD-contiguous ranges); k_batch_finish
 // then reduces per-replica f64 energies and writes the concatenated f64
 // forces.  The three kernels are captured once per list build into a
-// hipGraph, so a batched evaluation is one graph launch.  Every item runs the
+// hipGraph, so a batched evaluation is one graph launch (the four kernels).  Every item runs the
 // single-system arithmetic, so each replica's forces and energies are
 // bit-identical to evaluating it alone with the same work shape.
 #include <algorithm>
@@ -67,7 +68,8 @@ gm_status make_plan(gm_batch *b) {
     for (int r = 0; r < R; r++) {
         gm_sys *s = b->reps[(size_t)r];
         ReplicaDesc &d = desc[(size_t)r];
-        d.tile = tile_args(s, s->x.p, s->y.p, s->z.p);
+        d.tile = tile_args(s);
+        d.prep = prep_args(s, s->x.p, s->y.p, s->z.p);
         d.terms = term_args(s, s->x.p, s->y.p, s->z.p);
         d.n_bonded_groups = (int)((d.terms.nb + d.terms.na + d.terms.nd + d.terms.n14 + 63) / 64);
         d.n_wi = (int)s->n_wi;
@@ -93,6 +95,7 @@ gm_status make_plan(gm_batch *b) {
     if (e1 == hipSuccess) e1 = hipStreamWaitEvent(b->side, b->fork, 0);
     if (e1 == hipSuccess) e1 = launch_batch_bonded(b->d_reps.p, b->d_items.p, b->n_bonded_items, b->side);
     if (e1 == hipSuccess) e1 = hipEventRecord(b->join, b->side);
+    if (e1 == hipSuccess) e1 = launch_batch_prep(b->d_reps.p, b->atom_rep.p, b->atom_off.p, b->off.back(), st);
     if (e1 == hipSuccess)
         e1 = launch_batch_tiles(b->d_reps.p, b->d_items.p + b->n_bonded_items, b->n_items, grid, st);
     if (e1 == hipSuccess) e1 = hipStreamWaitEvent(st, b->join, 0);

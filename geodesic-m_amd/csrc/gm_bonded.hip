@@ -35,11 +35,13 @@ __device__ inline double block_sum(const double *ptr, long long len, double *sh)
 }
 
 // Blocks [0, nseg): block b sums segment b.  Blocks >= nseg: one thread per atom.
+// Block 0 also clears the displacement bound the prep kernel raised.
 __global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, int n, const int *atom_slot,
                                                  unsigned long long *F, size_t n_pad, double *fx,
-                                                 double *fy, double *fz) {
+                                                 double *fy, double *fz, unsigned *disp) {
     __shared__ double sh[1024];
     const int b = blockIdx.x, t = threadIdx.x;
+    if (b == 0 && t == 0) *disp = 0u;
     if (b < sg.nseg) {
         const double v = block_sum(sg.ptr[b], sg.len[b], sh);
         if (t == 0) eout[b] = v;
@@ -70,6 +72,7 @@ __global__ __launch_bounds__(1024) void k_batch_finish(const ReplicaDesc *__rest
     if (b < 5 * n_reps) {
         const ReplicaDesc &R = reps[b / 5];
         const int seg = b % 5;
+        if (seg == 0 && t == 0) *R.prep.disp = 0u;  // displacement bound of the next evaluation
         const double *eb = R.terms.e_items;
         const double *ptr;
         long long len;
@@ -124,7 +127,7 @@ hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz, bool lj_
     sg.nseg = 5;
     const unsigned grid = 5u + (unsigned)((s->n + 1023) / 1024);
     k_finish<<<grid, 1024, 0, s->stream>>>(sg, s->e_out.p, (int)s->n, s->atom_slot.p, s->F.p,
-                                            (size_t)s->nsc * 64, fx, fy, fz);
+                                            (size_t)s->nsc * 64, fx, fy, fz, s->disp.p);
     return hipGetLastError();
 }
 

@@ -178,18 +178,28 @@ __global__ __launch_bounds__(64) void k_sc_split(const int *meta, int n, const u
     slot_atom[(size_t)sc * 64 + pos] = a;
 }
 
+// Slot tables.  qpos/qeps (the force kernel's per-slot inputs) are filled for
+// every slot, padding included, so the per-evaluation prep kernel only needs
+// to overwrite the atoms' slots.
 __global__ void k_slot_setup(int nslots, int n, const int *slot_atom, const uint32_t *q,
-                             const float2 *atom_par, int *atom_slot, int4 *sq, float2 *sp) {
+                             const float2 *atom_par, int *atom_slot, int4 *sq, float2 *sp,
+                             int4 *qpos, float *qeps, float inv_s) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
     int a = slot_atom[s];
     if (a >= 0) {
         atom_slot[a] = s;
-        sq[s] = make_int4((int)q[a], (int)q[n + a], (int)q[2 * n + a], 0);
-        sp[s] = atom_par[a];
+        const int4 v = make_int4((int)q[a], (int)q[n + a], (int)q[2 * n + a], 0);
+        const float2 p = atom_par[a];
+        sq[s] = v;
+        sp[s] = p;
+        qpos[s] = make_int4(v.x, v.y, v.z, __float_as_int(p.x * inv_s));
+        qeps[s] = 2.0f * p.y;
     } else {
         sq[s] = make_int4(0, 0, 0, 0);
         sp[s] = make_float2(0.f, 0.f);
+        qpos[s] = make_int4(0, 0, 0, 0);
+        qeps[s] = 0.f;
     }
 }
 
@@ -339,9 +349,13 @@ __device__ inline bool excluded(const int64_t *off, const uint32_t *nbr, uint32_
 // is many times the estimate's worst error, so every decision is the f64 one.
 // The exclusion lookup runs only for pairs inside r_skin, and only when the
 // partner lies in the atom's (smallest, largest) exclusion-partner range.
+// Per tile, the byte floor((d_lo - r_cutoff) / step) (0 if negative, 255 for
+// a tile without listed pairs) goes to ent_dmin, d_lo a lower bound on the
+// distance of the tile's closest listed pair (the f32 estimate less a
+// relative 2e-6 and 1e-5 A, far above its error).
 __global__ __launch_bounds__(256) void k_masks(int nent, const int *ent_sc, const int *ent_cj,
                                                uint32_t *ent_imask, unsigned long long *ent_mask,
-                                               int2 *ent_meta, int *ent_ja, uint8_t *ent_lb,
+                                               int2 *ent_meta, uint2 *ent_dmin, uint8_t *ent_lb,
                                                const int *slot_atom, const int4 *sq, const double *x,
                                                const double *y, const double *z,
                                                const int64_t *excl_off, const uint32_t *excl_nbr,
@@ -357,7 +371,6 @@ __global__ __launch_bounds__(256) void k_masks(int nent, const int *ent_sc, cons
         const size_t js = (size_t)cj * 8 + jl;
         const int b = slot_atom[js];
         const int4 qb = sq[js];
-        if (il == 0) ent_ja[(size_t)e * 8 + jl] = b;
         int a[8];
         int4 qa[8];
 #pragma unroll
@@ -371,14 +384,17 @@ __global__ __launch_bounds__(256) void k_masks(int nent, const int *ent_sc, cons
             }
         }
         uint32_t inb = 0, cutb = 0;
+        float r2v[8];
 #pragma unroll
         for (int ci = 0; ci < 8; ci++) {
             const int cg = sc * 8 + ci;
+            r2v[ci] = 0.f;
             if (a[ci] >= 0 && b >= 0 && (cg < cj || il < jl)) {
                 const float dx = (float)wrapdiff(qb.x, qa[ci].x) * bp.sq[0];
                 const float dy = (float)wrapdiff(qb.y, qa[ci].y) * bp.sq[1];
                 const float dz = (float)wrapdiff(qb.z, qa[ci].z) * bp.sq[2];
                 const float r2f = dx * dx + dy * dy + dz * dz;
+                r2v[ci] = r2f;
                 bool in = r2f <= bp.r_skin2f, cut = r2f <= bp.r_cut2f;
                 if (fabsf(r2f - bp.r_skin2f) <= bp.band_skin || fabsf(r2f - bp.r_cut2f) <= bp.band_cut) {
                     const uint32_t lo = (uint32_t)min(a[ci], b), hi = (uint32_t)max(a[ci], b);
@@ -405,6 +421,7 @@ __global__ __launch_bounds__(256) void k_masks(int nent, const int *ent_sc, cons
         }
         uint32_t newmask = 0;
         unsigned long long npairs = 0, ncut_tiles = 0, ncut_pairs = 0;
+        uint32_t dlo = 0, dhi = 0;
 #pragma unroll
         for (int ci = 0; ci < 8; ci++) {
             const unsigned long long m = __ballot((inb >> ci) & 1u);
@@ -413,11 +430,21 @@ __global__ __launch_bounds__(256) void k_masks(int nent, const int *ent_sc, cons
             if (m) { newmask |= 1u << ci; npairs += __popcll(m); }
             if (mc) ncut_tiles++;
             ncut_pairs += __popcll(mc);
+            int db = 255;
+            if ((inb >> ci) & 1u) {
+                const float dl = sqrtf(r2v[ci]) * (1.0f - 2e-6f) - 1e-5f;
+                const float st = floorf((dl - bp.prune_rc) * bp.prune_inv_step);
+                db = st <= 0.f ? 0 : (st >= 255.f ? 255 : (int)st);
+            }
+            for (int o = 1; o < 64; o <<= 1) db = min(db, __shfl_xor(db, o));
+            if (ci < 4) dlo |= (uint32_t)db << (8 * ci);
+            else dhi |= (uint32_t)db << (8 * (ci - 4));
         }
         ent_lb[(size_t)e * 64 + t] = (uint8_t)inb;
         if (t == 0) {
             ent_imask[e] = newmask;
             ent_meta[e] = make_int2(cj, (int)newmask);
+            ent_dmin[e] = make_uint2(dlo, dhi);
         }
         tot_pairs += npairs;
         tot_tiles += (unsigned long long)__popc(newmask);
@@ -446,6 +473,11 @@ __global__ void k_needs_rebuild(int n, const double *x, const double *y, const d
 }
 
 inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
+
+// sigma/2 in the force kernel's length unit: fixed-point steps for cubic boxes, else A
+inline float kernel_sigma_scale(const gm_sys *s) {
+    return (s->box[0] == s->box[1] && s->box[1] == s->box[2]) ? s->nbp.inv_s : 1.0f;
+}
 
 }  // namespace
 
@@ -490,7 +522,7 @@ hipError_t launch_build(gm_sys *s, bool wrap) {
                                                    s->sc_base.p, s->sc_m.p, s->slot_atom.p);
     k_slot_setup<<<grid_for((size_t)s->max_sc * 64, 256), 256, 0, st>>>(
         (int)(s->max_sc * 64), n, s->slot_atom.p, s->qatom.p, s->atom_par.p, s->atom_slot.p,
-        s->sq.p, s->sp.p);
+        s->sq.p, s->sp.p, s->qpos.p, s->qeps.p, kernel_sigma_scale(s));
     k_bbox<<<(unsigned)s->max_sc, 64, 0, st>>>(s->meta.p, s->slot_atom.p, s->sq.p, bp, s->cl_ref.p,
                                                s->cl_half.p, s->sc_ref.p, s->sc_half.p);
     return hipGetLastError();
@@ -516,7 +548,7 @@ hipError_t launch_pair_fill_and_masks(gm_sys *s) {
     if (s->nent > 0)
         k_masks<<<(unsigned)std::min<int64_t>((s->nent + 3) / 4, 2048), 256, 0, st>>>((int)s->nent, s->ent_sc.p, s->ent_cj.p,
                                                   s->ent_imask.p, s->ent_mask.p, s->ent_meta.p,
-                                                  s->ent_ja.p, s->ent_lb.p, s->slot_atom.p,
+                                                  s->ent_dmin.p, s->ent_lb.p, s->slot_atom.p,
                                                   s->sq.p, s->x.p, s->y.p, s->z.p, s->excl_off.p,
                                                   s->excl_nbr.p, s->excl_rng.p, s->bp,
                                                   (unsigned long long *)(s->meta.p + 8));

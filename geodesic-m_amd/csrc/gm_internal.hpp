@@ -42,6 +42,9 @@ struct NbParams {
     // quintic switch over [rin_sw, rin_sw + 1 / inv_w_in] (plain and step units)
     float rin_sw, inv_w_in, ds_coef_in;
     float rin_sw_u, inv_w_in_u, ds_coef_in_u;
+    // -r_switch / (r_cutoff - r_switch) and the same for the inner switch:
+    // u = r inv_w + nrw, dimensionless, so one value serves both length units
+    float nrw, nrw_in;
 };
 
 // Per-system constants for the list build.
@@ -56,6 +59,8 @@ struct BuildParams {
     float band_skin, band_cut; // |r2_f32 - threshold| at or below which f64 decides
     float wcol[2];      // column widths in x and y, A
     int kx, ky;         // column reach: columns more than k apart hold no pair within r_search
+    float prune_rc;     // r_cutoff, A
+    float prune_inv_step;  // 1 / width of one tile-distance byte step, 1/A
     int ncx, ncy, ncol;
     int periodic;
     int n_atoms;
@@ -70,6 +75,19 @@ __device__ inline int quantize_fixed(double x, double inv_q) {
     const double r = fma(-4294967296.0, floor(t * 2.3283064365386963e-10), t);
     return (int)(unsigned)r;
 }
+
+// Wrapping difference of two fixed-point coordinates as f32: the exact
+// minimum-image displacement in fixed-point steps, rounded once.
+__device__ inline float fixdiff(int a, int b) { return (float)(int)((unsigned)a - (unsigned)b); }
+
+// Tile pruning.  The list build stores, per tile, a byte b with
+//   (closest listed pair of the tile at build) >= r_cutoff + b * step.
+// If no atom has moved farther than d since the build, every pair of the tile
+// is now at least r_cutoff + b * step - 2 d apart, so with b * step > 2 d plus
+// a margin the whole tile lies beyond r_cutoff and contributes exactly zero.
+// The margin (two steps and 1e-4 r_cutoff) covers the f32 rounding of the
+// stored distance, of the displacement and of r2 in the force kernel.
+constexpr int kPruneSteps = 254;  // bytes 0..254 span [r_cutoff, r_skin]
 
 }  // namespace gm
 
@@ -136,8 +154,14 @@ struct gm_sys {
 
     // slots
     DevBuf<int> slot_atom, atom_slot;
-    DevBuf<int4> sq;         // fixed-point coords per slot (x, y, z, 0)
+    DevBuf<int4> sq;         // fixed-point coords per slot (x, y, z, 0) at the list build
     DevBuf<float2> sp;       // (sigma/2, sqrt eps) per slot
+    DevBuf<int4> qpos;       // per evaluation: fixed-point coords per slot + bits of sigma/2
+    DevBuf<float> qeps;      // 2 sqrt(eps) per slot
+    DevBuf<unsigned> disp;   // [0]: largest displacement since the build (f32 bits, A);
+                             // raised by the prep kernel, cleared by the finish kernel
+    bool prune = true;       // tile pruning requested (gm_set_tile_pruning)
+    bool prune_geom = false; // box wide enough for the pruning bound (every edge >= 2 r_skin)
     DevBuf<int4> cl_ref;     // cluster bbox centre (fixed point) + valid flag
     DevBuf<float4> cl_half;  // cluster bbox half extents, A
     DevBuf<int4> sc_ref;
@@ -149,7 +173,7 @@ struct gm_sys {
     DevBuf<uint32_t> ent_imask;
     DevBuf<unsigned long long> ent_mask;  // per entry 8 x 64-bit pair masks (export)
     DevBuf<int2> ent_meta;                // per entry (j cluster, i-cluster mask)
-    DevBuf<int> ent_ja;                   // per entry the j cluster's 8 atom indices (-1 padding)
+    DevBuf<uint2> ent_dmin;               // per entry, per tile: closest listed pair byte
     DevBuf<uint8_t> ent_lb;               // per entry, per lane: mask bits over the 8 i clusters
     bool F_dirty = true;                  // accumulator not known to be zero
     DevBuf<int> wi_sc, wi_start, wi_end;
@@ -215,6 +239,8 @@ hipError_t launch_pair_fill_and_masks(gm_sys *s);
 hipError_t launch_needs_rebuild(gm_sys *s, const double *x, const double *y, const double *z);
 // gm_nonbonded.hip: tiles + bonded terms in one launch
 // (shell: tiles only, with nbp_shell)
+// (after launch_prep: slot-order fixed-point positions and the displacement bound)
+hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double *z);
 hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z, bool shell = false);
 // gm_bonded.hip: energy reduction, f64 output, accumulator clear (lj_only:
 // the bonded energy segments are empty, as after a shell-mode launch)
@@ -225,7 +251,11 @@ struct ReplicaDesc;
 struct TermArgs;
 struct TileArgs;
 TermArgs term_args(gm_sys *s, const double *x, const double *y, const double *z);
-TileArgs tile_args(gm_sys *s, const double *x, const double *y, const double *z);
+TileArgs tile_args(gm_sys *s);
+struct PrepArgs;
+PrepArgs prep_args(gm_sys *s, const double *x, const double *y, const double *z);
+hipError_t launch_batch_prep(const ReplicaDesc *reps, const int *atom_rep, const int64_t *atom_off,
+                             int64_t n_atoms, hipStream_t st);
 hipError_t launch_batch_tiles(const ReplicaDesc *reps, const int2 *items, int n_items, int grid,
                               hipStream_t st);
 hipError_t launch_batch_bonded(const ReplicaDesc *reps, const int2 *items, int n_items, hipStream_t st);

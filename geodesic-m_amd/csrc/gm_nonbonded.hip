@@ -8,6 +8,12 @@
 // Berthelot combining, quintic switch over [r_switch, r_cutoff], pairs with
 // r2 > r_cutoff^2 or r2 == 0 skipped, F_i -= f d/r, F_j += f d/r.
 //
+// Prep kernel (one thread per atom, before the force launch): each atom's
+// position quantised to 32-bit fixed-point fractions of the box and written to
+// its slot together with sigma/2, so the tile loop reads one contiguous int4
+// per j atom; and the largest displacement of any atom since the list build,
+// which bounds how far every listed pair can have come closer (tile pruning).
+//
 // Tile path (workgroups after the bonded ones): one wavefront per work item =
 // (supercluster, run of list entries).  Lane = (jl = lane>>3, il = lane&7):
 // each lane owns one i atom of each of the supercluster's 8 clusters (il) and
@@ -16,16 +22,16 @@
 //  - i forces stay in registers across the run and are reduced over the 8 jl
 //    lanes once at the end; j forces are reduced over the 8 il lanes with
 //    three DPP adds once per entry (up to 8 tiles).
-//  - Each entry carries its j-cluster's atom indices and one mask byte per
-//    lane (bit ci = "this lane's pair in tile ci is in the list"), so the next
-//    entries' loads depend only on the entry index; they are issued one and
-//    two entries ahead of the arithmetic.
-//  - Positions are quantised on the fly to 32-bit fixed-point fractions of the
-//    box; the wrapping int32 difference is the exact minimum-image
-//    displacement and its f32 conversion keeps 24 significant bits anywhere in
-//    the box.  Pair math is FP32; partial sums are converted once per reduction
-//    to int64 fixed point (2^-32 kcal/mol/A) and added with integer atomics, so
-//    the result does not depend on scheduling order.
+//  - Each entry carries one mask byte per lane (bit ci = "this lane's pair in
+//    tile ci is in the list") and one distance byte per tile; tiles whose
+//    closest listed pair at build time is provably beyond r_cutoff now are
+//    dropped from the entry's tile mask before the loop.  Such a tile would
+//    only add zeros, so results are bit-identical with and without pruning.
+//  - The wrapping int32 difference of fixed-point positions is the exact
+//    minimum-image displacement and its f32 conversion keeps 24 significant
+//    bits anywhere in the box.  Pair math is FP32; partial sums are converted
+//    once per reduction to int64 fixed point (2^-32 kcal/mol/A) and added with
+//    integer atomics, so the result does not depend on scheduling order.
 // Bonded path (the first ceil(n_terms/64) workgroups): one thread per bonded term (f64,
 // gm_terms.hpp), running beside the tiles instead of in launches of its own.
 #include "gm_internal.hpp"
@@ -50,23 +56,115 @@ __device__ inline void fixed_add(unsigned long long *p, float v, float scale) {
     __hip_atomic_fetch_add(p, (unsigned long long)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ inline float fixdiff(int a, int b) { return (float)(int)((unsigned)a - (unsigned)b); }
+// Exchange-and-add across lane bit 5: with a, b one value each per lane,
+// returns a[l] + a[l + 32] on lanes l < 32 and b[l - 32] + b[l] on the others
+// (v_permlane32_swap: the upper half of a trades places with the lower half of b).
+__device__ inline float swap_add32(float a, float b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
 
-struct JAtom {
-    double x, y, z;
-    float2 par;
-};
+// The same across lane bit 4 (rows of 16 lanes): a summed over row pairs on
+// even rows, b on odd rows (v_permlane16_swap: odd rows of a trade with even rows of b).
+__device__ inline float swap_add16(float a, float b) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
 
-__device__ inline JAtom load_j(const double *__restrict__ x, const double *__restrict__ y,
-                               const double *__restrict__ z, const float2 *__restrict__ atom_par,
-                               int ja) {
-    JAtom r;
-    const int a = ja < 0 ? 0 : ja;  // padding slot: any valid atom, its mask bits are 0
-    r.x = x[a];
-    r.y = y[a];
-    r.z = z[a];
-    r.par = atom_par[a];
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ inline double swap_self_f64(double v, bool rows) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)b, hi = (unsigned)(b >> 32);
+    const auto l = rows ? __builtin_amdgcn_permlane16_swap(lo, lo, false, false)
+                        : __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h = rows ? __builtin_amdgcn_permlane16_swap(hi, hi, false, false)
+                        : __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    return __longlong_as_double(((long long)h[0] << 32) | l[0]) +
+           __longlong_as_double(((long long)h[1] << 32) | l[1]);
+}
+
+// Sum over the wavefront in a fixed order (lane bits 0, 1, 2, 3, 4, 5); every
+// lane receives the total.
+__device__ inline double wave_sum_f64(double v) {
+    v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);  // row_half_mirror: the other quad of the 8
+    v += dpp_f64<0x128>(v);  // row_ror:8: the other 8 of the row
+    v = swap_self_f64(v, true);
+    return swap_self_f64(v, false);
+}
+
+// A wave-uniform value copied into a vector register (a plain move, no load).
+__device__ inline float in_vgpr(float v) {
+    float r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "s"(v));
     return r;
+}
+
+__device__ inline float wave_max(float v) {
+    for (int o = 1; o < 64; o <<= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// One atom of the prep pass: quantise, store x, y, z to the slot (sigma/2,
+// the slot's fourth word, is written at the list build), return the
+// displacement from the build position (A).  Every load is by atom index.
+__device__ inline float prep_atom(const PrepArgs &P, int a) {
+    const int s = P.atom_slot[a];
+    const int qx = quantize_fixed(P.x[a], P.iq[0]);
+    const int qy = quantize_fixed(P.y[a], P.iq[1]);
+    const int qz = quantize_fixed(P.z[a], P.iq[2]);
+    *reinterpret_cast<int3 *>(&P.qpos[s]) = make_int3(qx, qy, qz);
+    const float dx = fixdiff(qx, (int)P.qref[a]) * P.sq_len[0];
+    const float dy = fixdiff(qy, (int)P.qref[P.n + a]) * P.sq_len[1];
+    const float dz = fixdiff(qz, (int)P.qref[2 * (size_t)P.n + a]) * P.sq_len[2];
+    return sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
+__device__ inline void atomic_max_nonneg(unsigned *slot, float v) {
+    // non-negative floats order like their bit patterns
+    __hip_atomic_fetch_max(slot, __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(256) void k_prep(PrepArgs P) {
+    const int a = blockIdx.x * 256 + threadIdx.x;
+    float d = 0.f;
+    if (a < P.n) d = prep_atom(P, a);
+    d = wave_max(d);
+    __shared__ float sm[4];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) atomic_max_nonneg(P.disp, fmaxf(fmaxf(sm[0], sm[1]), fmaxf(sm[2], sm[3])));
+}
+
+// Batched prep: atom a of the concatenated replicas.  A wavefront whose atoms
+// all belong to one replica raises that replica's bound once; one that
+// straddles a boundary raises it per lane.
+__global__ __launch_bounds__(256) void k_batch_prep(const ReplicaDesc *__restrict__ reps,
+                                                    const int *__restrict__ atom_rep,
+                                                    const int64_t *__restrict__ atom_off, int64_t n_atoms) {
+    const int64_t a = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int r = -1;
+    float d = 0.f;
+    if (a < n_atoms) {
+        r = atom_rep[a];
+        d = prep_atom(reps[r].prep, (int)(a - atom_off[r]));
+    }
+    const int r0 = __shfl(r, 0);
+    const bool uniform = __all(r == r0);
+    if (uniform) {
+        d = wave_max(d);
+        if ((threadIdx.x & 63) == 0 && r0 >= 0) atomic_max_nonneg(reps[r0].prep.disp, d);
+    } else if (r >= 0) {
+        atomic_max_nonneg(reps[r].prep.disp, d);
+    }
 }
 
 // One work item.  CUBIC: all three box edges equal, so lengths are carried in
@@ -76,68 +174,66 @@ __device__ inline JAtom load_j(const double *__restrict__ x, const double *__res
 // SHELL: each pair's switched LJ is weighted by S(r) - S_in(r) instead of
 // S(r) (the shell part of a split of the switched LJ into inner + shell);
 // below the inner switch both are 1, so short-range pairs contribute exactly 0.
+//
+// Pair arithmetic, with E4 = 4 eps = (2 sqrt eps_i)(2 sqrt eps_j), s6 = (sig/r)^6,
+// u = clamp((r - r_switch) / w, 0, 1), S = 1 + u^3 (-10 + u (15 - 6 u)),
+// dS/dr = (-30 / w) (u - u^2)^2:
+//   energy  v = E4 (s6^2 - s6) S
+//   F / r   = E4 / r^2 [6 (2 s6^2 - s6) S - (s6^2 - s6) r dS/dr]
+// (the reference's 24 eps (2 s12 - s6) / r^2 S - v_lj dS/dr / r).
 template <bool CUBIC, bool SHELL>
 __device__ __attribute__((always_inline)) inline void tile_work(const int w, const int lane, const TileArgs &A) {
-    const double *__restrict__ x = A.x;
-    const double *__restrict__ y = A.y;
-    const double *__restrict__ z = A.z;
-    const float2 *__restrict__ atom_par = A.atom_par;
-    const int *__restrict__ slot_atom = A.slot_atom;
-    const int2 *__restrict__ ent_meta = A.ent_meta;
-    const int *__restrict__ ent_ja = A.ent_ja;
+    const int4 *__restrict__ qpos = A.qpos;
+    const float *__restrict__ qeps = A.qeps;
     const uint8_t *__restrict__ ent_lb = A.ent_lb;
     unsigned long long *__restrict__ F = A.F;
     const int n_pad = A.n_pad;
     const NbParams &p = A.p;
-    const double iqx = A.iq[0], iqy = A.iq[1], iqz = A.iq[2];
-    // The run's list data (mask bytes, j atoms, metadata) arrive in LDS in one
-    // burst of independent loads; the loop then only gathers positions, one
-    // entry ahead of the arithmetic.  Reduced j forces wait in LDS and leave
-    // with atomics after the loop: gfx950 counts atomics in vmcnt, so issuing
-    // them inside the loop would make every iteration wait for the previous
-    // one's.
+    // The run's list data (mask bytes, metadata) arrive in LDS in one burst of
+    // independent loads; the loop then only gathers j positions, one entry
+    // ahead of the arithmetic.  Reduced j forces wait in LDS and leave with
+    // atomics after the loop: gfx950 counts atomics in vmcnt, so issuing them
+    // inside the loop would make every iteration wait for the previous one's.
     __shared__ uint32_t s_lb[kMaxRun * 16];
-    __shared__ int s_ja[kMaxRun * 8];
     __shared__ int2 s_meta[kMaxRun];
     __shared__ float jf[3][kMaxRun * 8];
     const int il = lane & 7, jl = lane >> 3;
     const int sc = A.wi_sc[w];
     const int e0 = A.wi_start[w], ne = A.wi_end[w] - e0;
     {
+        // keep tile ci of an entry unless byte_ci > bound (see kPruneSteps)
+        float bound = 3.0e38f;
+        if (A.disp) bound = fmaf(__uint_as_float(*A.disp), A.prune_scale, A.prune_bias);
         const uint32_t *lb = reinterpret_cast<const uint32_t *>(ent_lb + (size_t)e0 * 64);
         uint32_t lbv[kMaxRun / 4];
-        int jav[kMaxRun / 8];
 #pragma unroll
         for (int r = 0; r < kMaxRun / 4; r++) {
             const int k = lane + 64 * r;
             lbv[r] = k < ne * 16 ? lb[k] : 0u;
         }
+        int2 mv = make_int2(0, 0);
+        if (lane < ne) {
+            mv = A.ent_meta[e0 + lane];
+            const uint2 dm = A.ent_dmin[e0 + lane];
+            uint32_t keep = 0;
 #pragma unroll
-        for (int r = 0; r < kMaxRun / 8; r++) {
-            const int k = lane + 64 * r;
-            jav[r] = k < ne * 8 ? ent_ja[(size_t)e0 * 8 + k] : -1;
+            for (int ci = 0; ci < 8; ci++) {
+                const uint32_t b = ((ci < 4 ? dm.x : dm.y) >> (8 * (ci & 3))) & 255u;
+                keep |= ((float)b > bound) ? 0u : (1u << ci);  // a NaN bound keeps every tile
+            }
+            mv.y &= (int)keep;
         }
-        const int2 mv = lane < ne ? ent_meta[e0 + lane] : make_int2(-1, 0);
 #pragma unroll
         for (int r = 0; r < kMaxRun / 4; r++) s_lb[lane + 64 * r] = lbv[r];
-#pragma unroll
-        for (int r = 0; r < kMaxRun / 8; r++) s_ja[lane + 64 * r] = jav[r];
         if (lane < kMaxRun) s_meta[lane] = mv;
     }
     // The supercluster's 64 i atoms live in LDS (fixed-point position + sigma/2
-    // in one int4, sqrt(eps) beside it): every tile reads 8 distinct entries,
+    // in one int4, 2 sqrt(eps) beside it): every tile reads 8 distinct entries,
     // broadcast to 8 lanes each, which keeps 40 registers free for occupancy.
     __shared__ int4 s_iq[64];
     __shared__ float s_ie[64];
-    const float inv_s = CUBIC ? p.inv_s : 1.0f;
-    {
-        const int a = slot_atom[sc * 64 + lane];
-        const int aa = a < 0 ? 0 : a;
-        const float2 pp = atom_par[aa];
-        s_iq[lane] = make_int4(quantize_fixed(x[aa], iqx), quantize_fixed(y[aa], iqy),
-                               quantize_fixed(z[aa], iqz), __float_as_int(pp.x * inv_s));
-        s_ie[lane] = pp.y;
-    }
+    s_iq[lane] = qpos[(size_t)sc * 64 + lane];
+    s_ie[lane] = qeps[(size_t)sc * 64 + lane];
     float fix[8], fiy[8], fiz[8];
 #pragma unroll
     for (int ci = 0; ci < 8; ci++) { fix[ci] = 0.f; fiy[ci] = 0.f; fiz[ci] = 0.f; }
@@ -145,105 +241,134 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     const uint8_t *s_lb8 = reinterpret_cast<const uint8_t *>(s_lb);
     const float rc2 = CUBIC ? p.rc2_u : p.rc2;
     const unsigned rc2_bits = (unsigned)__float_as_int(rc2);
-    const float rsw = CUBIC ? p.rsw_u : p.rsw, inv_w = CUBIC ? p.inv_w_u : p.inv_w;
+    // the switch offsets live in VGPRs: an FMA reads at most one scalar operand
+    const float inv_w = CUBIC ? p.inv_w_u : p.inv_w, nrw = in_vgpr(p.nrw);
     const float ds_coef = CUBIC ? p.ds_coef_u : p.ds_coef;
-    const float rin = CUBIC ? p.rin_sw_u : p.rin_sw, inv_w_in = CUBIC ? p.inv_w_in_u : p.inv_w_in;
+    const float inv_w_in = CUBIC ? p.inv_w_in_u : p.inv_w_in, nrw_in = in_vgpr(p.nrw_in);
     const float ds_coef_in = CUBIC ? p.ds_coef_in_u : p.ds_coef_in;
     double e_acc = 0.0;
-    JAtom cur = load_j(x, y, z, atom_par, s_ja[jl]);
+    int4 cq = qpos[(size_t)s_meta[0].x * 8 + jl];
+    float ce = qeps[(size_t)s_meta[0].x * 8 + jl];
     for (int k = 0; k < ne; k++) {
-        const JAtom nxt = load_j(x, y, z, atom_par, s_ja[min(k + 1, ne - 1) * 8 + jl]);
+        const int cjn = __builtin_amdgcn_readfirstlane(s_meta[min(k + 1, ne - 1)].x);
+        const int4 nq = qpos[(size_t)cjn * 8 + jl];
+        const float nce = qeps[(size_t)cjn * 8 + jl];
         const uint32_t im = (uint32_t)__builtin_amdgcn_readfirstlane(s_meta[k].y);
-        const uint32_t bits = s_lb8[k * 64 + lane];
-
-        const int qjx = quantize_fixed(cur.x, iqx);
-        const int qjy = quantize_fixed(cur.y, iqy);
-        const int qjz = quantize_fixed(cur.z, iqz);
-        const float shj = cur.par.x * inv_s, sej = cur.par.y;
-        float fjx = 0.f, fjy = 0.f, fjz = 0.f, ee = 0.f;
+        float fjx = 0.f, fjy = 0.f, fjz = 0.f;
+        if (im) {
+            const uint32_t bits = s_lb8[k * 64 + lane];
+            const float shj = __int_as_float(cq.w), ej = ce;
+            float ee = 0.f;
 #pragma unroll
-        for (int ci = 0; ci < 8; ci++) {
-            if (im & (1u << ci)) {
-                // Branch-free tile: every lane evaluates its pair; pairs not in
-                // the list, beyond r_cutoff or coincident get zero force and
-                // energy through one select (which also discards the inf/NaN
-                // an r2 of 0 produces).  Below r_switch, u clamps to 0, where
-                // S = 1 and dS/dr = 0 exactly.
-                const int4 qi = s_iq[ci * 8 + il];
-                float dx = fixdiff(qjx, qi.x);
-                float dy = fixdiff(qjy, qi.y);
-                float dz = fixdiff(qjz, qi.z);
-                if (!CUBIC) { dx *= p.sx; dy *= p.sy; dz *= p.sz; }
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                // 0 < r2 <= rc2 as one unsigned compare of the bit patterns
-                const bool ok = ((bits >> ci) & 1u) &&
-                                ((unsigned)__float_as_int(r2) - 1u) < rc2_bits;
-                const float inv_r2 = __builtin_amdgcn_rcpf(r2);
-                const float inv_r = __builtin_amdgcn_rsqf(r2);
-                const float sig = __int_as_float(qi.w) + shj;
-                const float s2 = sig * sig * inv_r2;
-                const float s6 = s2 * s2 * s2;
-                const float s12 = s6 * s6;
-                const float eps = s_ie[ci * 8 + il] * sej;
-                const float v_lj = 4.0f * eps * (s12 - s6);
-                const float f_lj = 24.0f * eps * (2.0f * s12 - s6) * inv_r2;  // F_radial / r
-                const float r = r2 * inv_r;
-                const float u = fmaxf((r - rsw) * inv_w, 0.0f);
-                const float u2 = u * u;
-                float sw = 1.0f + u2 * u * (-10.0f + u * (15.0f - 6.0f * u));
-                const float om = 1.0f - u;
-                float dsdr = ds_coef * u2 * om * om;
-                if (SHELL) {
-                    const float ui = fminf(fmaxf((r - rin) * inv_w_in, 0.0f), 1.0f);
-                    const float ui2 = ui * ui, omi = 1.0f - ui;
-                    sw -= 1.0f + ui2 * ui * (-10.0f + ui * (15.0f - 6.0f * ui));
-                    dsdr -= ds_coef_in * ui2 * omi * omi;
+            for (int ci = 0; ci < 8; ci++) {
+                if (im & (1u << ci)) {
+                    // Branch-free tile: every lane evaluates its pair; pairs not
+                    // in the list, beyond r_cutoff or coincident get zero force
+                    // and energy through one select each (which also discards
+                    // the inf/NaN an r2 of 0 produces).  Below r_switch u
+                    // clamps to 0, where S = 1 and dS/dr = 0 exactly.
+                    const int4 qi = s_iq[ci * 8 + il];
+                    float dx = fixdiff(cq.x, qi.x);
+                    float dy = fixdiff(cq.y, qi.y);
+                    float dz = fixdiff(cq.z, qi.z);
+                    if (!CUBIC) { dx *= p.sx; dy *= p.sy; dz *= p.sz; }
+                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    // 0 < r2 <= rc2 as one unsigned compare of the bit patterns
+                    const bool ok = ((bits >> ci) & 1u) &&
+                                    ((unsigned)__float_as_int(r2) - 1u) < rc2_bits;
+                    const float inv_r = __builtin_amdgcn_rsqf(r2);
+                    // one Newton step brings rsq^2 (~2.5 ulp) to ~1 ulp, the
+                    // accuracy (sigma/r)^12 needs, for two FMAs instead of a rcp
+                    float inv_r2 = inv_r * inv_r;
+                    inv_r2 = fmaf(inv_r2, fmaf(-r2, inv_r2, 1.0f), inv_r2);
+                    const float sig = __int_as_float(qi.w) + shj;
+                    const float s2 = sig * sig * inv_r2;
+                    const float s6 = s2 * s2 * s2;
+                    const float lj_a = fmaf(s6, s6, -s6);                 // s12 - s6
+                    const float lj_b = s6 * fmaf(s6, 12.0f, -6.0f);       // 6 (2 s12 - s6)
+                    const float e4 = s_ie[ci * 8 + il] * ej;
+                    const float r = r2 * inv_r;
+                    const float u = __builtin_amdgcn_fmed3f(fmaf(r, inv_w, nrw), 0.0f, 1.0f);
+                    const float u2 = u * u;
+                    const float pu = fmaf(u, fmaf(u, -6.0f, 15.0f), -10.0f);
+                    const float qu = u - u2;
+                    float sw, dsdr;
+                    if (SHELL) {
+                        const float ui = __builtin_amdgcn_fmed3f(fmaf(r, inv_w_in, nrw_in), 0.0f, 1.0f);
+                        const float ui2 = ui * ui;
+                        const float pi = fmaf(ui, fmaf(ui, -6.0f, 15.0f), -10.0f);
+                        const float qi2 = ui - ui2;
+                        sw = fmaf(u2 * u, pu, -(ui2 * ui * pi));           // S - S_in
+                        dsdr = fmaf(ds_coef * qu, qu, -(ds_coef_in * qi2 * qi2));
+                    } else {
+                        sw = fmaf(u2 * u, pu, 1.0f);
+                        dsdr = ds_coef * qu * qu;
+                    }
+                    const float inner = fmaf(-(lj_a * dsdr), r, lj_b * sw);
+                    const float fr = ok ? e4 * inv_r2 * inner : 0.0f;
+                    const float v = ok ? e4 * lj_a * sw : 0.0f;
+                    fix[ci] = fmaf(-fr, dx, fix[ci]);
+                    fiy[ci] = fmaf(-fr, dy, fiy[ci]);
+                    fiz[ci] = fmaf(-fr, dz, fiz[ci]);
+                    fjx = fmaf(fr, dx, fjx);
+                    fjy = fmaf(fr, dy, fjy);
+                    fjz = fmaf(fr, dz, fjz);
+                    ee += v;
                 }
-                const float fr = ok ? f_lj * sw - v_lj * dsdr * inv_r : 0.0f;
-                const float v = ok ? v_lj * sw : 0.0f;
-                const float tx = fr * dx, ty = fr * dy, tz = fr * dz;
-                fix[ci] -= tx; fiy[ci] -= ty; fiz[ci] -= tz;
-                fjx += tx; fjy += ty; fjz += tz;
-                ee += v;
             }
+            fjx = dpp_sum8(fjx);
+            fjy = dpp_sum8(fjy);
+            fjz = dpp_sum8(fjz);
+            e_acc += (double)ee;
         }
-        fjx = dpp_sum8(fjx);
-        fjy = dpp_sum8(fjy);
-        fjz = dpp_sum8(fjz);
         if (il == 0) {
             jf[0][k * 8 + jl] = fjx;
             jf[1][k * 8 + jl] = fjy;
             jf[2][k * 8 + jl] = fjz;
         }
-        e_acc += (double)ee;
-        cur = nxt;
+        cq = nq;
+        ce = nce;
     }
     __syncthreads();
     const float fscale = CUBIC ? p.fscale_u : 4294967296.0f;
-    for (int c = 0; c < 3; c++) {
-        for (int idx = lane; idx < ne * 8; idx += 64) {
-            const int2 m = s_meta[idx >> 3];
+    for (int idx = lane; idx < ne * 8; idx += 64) {
+        const int2 m = s_meta[idx >> 3];
+        if (m.y == 0) continue;
+        const size_t js = (size_t)m.x * 8 + (idx & 7);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
             const float v = jf[c][idx];
-            if (m.y != 0 && v != 0.0f)
-                fixed_add(&F[(size_t)c * n_pad + (size_t)m.x * 8 + (idx & 7)], v, fscale);
+            if (v != 0.0f) fixed_add(&F[(size_t)c * n_pad + js], v, fscale);
         }
+    }
+    // i forces: lane (jl, il) holds partial sums for atom il of all 8 clusters;
+    // the 8 jl lanes of each atom are reduced by a transpose: exchange halves
+    // across lane bit 5, then bit 4 (gfx950 permlane swaps), then bit 3 (DPP),
+    // keeping the half that belongs to the lane's own bit each time.  Lane
+    // (jl, il) ends with the total of cluster jl, atom il: slot sc * 64 + lane.
+    float h4[3][4], h2[3][2], fo[3];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        h4[0][c] = swap_add32(fix[c], fix[c + 4]);
+        h4[1][c] = swap_add32(fiy[c], fiy[c + 4]);
+        h4[2][c] = swap_add32(fiz[c], fiz[c + 4]);
     }
 #pragma unroll
-    for (int ci = 0; ci < 8; ci++) {
-        float vx = fix[ci], vy = fiy[ci], vz = fiz[ci];
-        for (int o = 8; o < 64; o <<= 1) {
-            vx += __shfl_xor(vx, o);
-            vy += __shfl_xor(vy, o);
-            vz += __shfl_xor(vz, o);
-        }
-        if (jl == 0) {
-            const size_t s = (size_t)sc * 64 + ci * 8 + il;
-            fixed_add(&F[s], vx, fscale);
-            fixed_add(&F[(size_t)n_pad + s], vy, fscale);
-            fixed_add(&F[2 * (size_t)n_pad + s], vz, fscale);
-        }
+    for (int k = 0; k < 3; k++) {
+        h2[k][0] = swap_add16(h4[k][0], h4[k][2]);
+        h2[k][1] = swap_add16(h4[k][1], h4[k][3]);
+        const bool odd = (lane & 8) != 0;
+        const float keep = odd ? h2[k][1] : h2[k][0];
+        const float send = odd ? h2[k][0] : h2[k][1];
+        fo[k] = keep + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(send), 0x128, 0xF, 0xF, false));
     }
-    for (int o = 1; o < 64; o <<= 1) e_acc += __shfl_xor(e_acc, o);
+    {
+        const size_t s = (size_t)sc * 64 + lane;
+        fixed_add(&F[s], fo[0], fscale);
+        fixed_add(&F[(size_t)n_pad + s], fo[1], fscale);
+        fixed_add(&F[2 * (size_t)n_pad + s], fo[2], fscale);
+    }
+    e_acc = wave_sum_f64(e_acc);
     if (lane == 0) A.e_wi[w] = e_acc;
     __syncthreads();  // LDS is reused by the next work item of a persistent wave
 }
@@ -318,27 +443,51 @@ TermArgs term_args(gm_sys *s, const double *x, const double *y, const double *z)
     return t;
 }
 
-TileArgs tile_args(gm_sys *s, const double *x, const double *y, const double *z) {
+TileArgs tile_args(gm_sys *s) {
     TileArgs A{};
-    A.x = x; A.y = y; A.z = z;
-    A.atom_par = s->atom_par.p;
-    A.slot_atom = s->slot_atom.p;
+    A.qpos = s->qpos.p;
+    A.qeps = s->qeps.p;
     A.wi_sc = s->wi_sc.p; A.wi_start = s->wi_start.p; A.wi_end = s->wi_end.p;
-    A.ent_meta = s->ent_meta.p; A.ent_ja = s->ent_ja.p; A.ent_lb = s->ent_lb.p;
+    A.ent_meta = s->ent_meta.p; A.ent_dmin = s->ent_dmin.p; A.ent_lb = s->ent_lb.p;
+    A.disp = (s->prune && s->prune_geom) ? s->disp.p : nullptr;
+    // keep iff byte <= (2 disp + 1e-4 r_c) / step + 1 (see kPruneSteps)
+    A.prune_scale = 2.0f * s->bp.prune_inv_step;
+    A.prune_bias = (float)(1e-4 * s->rc) * s->bp.prune_inv_step + 1.0f;
     A.F = s->F.p;
     A.n_pad = (int)(s->nsc * 64);
     A.e_wi = s->e_wi.p;
     A.p = s->nbp;
-    for (int a = 0; a < 3; a++) A.iq[a] = s->bp.inv_q[a];
     A.cubic = (s->box[0] == s->box[1] && s->box[1] == s->box[2]) ? 1 : 0;
     return A;
+}
+
+PrepArgs prep_args(gm_sys *s, const double *x, const double *y, const double *z) {
+    PrepArgs P{};
+    P.x = x; P.y = y; P.z = z;
+    P.atom_slot = s->atom_slot.p;
+    P.qref = s->qatom.p;
+    P.qpos = s->qpos.p;
+    P.disp = s->disp.p;
+    for (int a = 0; a < 3; a++) {
+        P.iq[a] = s->bp.inv_q[a];
+        P.sq_len[a] = s->bp.sq[a];
+    }
+    P.n = (int)s->n;
+    return P;
+}
+
+// Slot-order fixed-point positions and the displacement bound; precedes every
+// launch_forces on the same positions.
+hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double *z) {
+    if (s->n > 0) k_prep<<<(unsigned)((s->n + 255) / 256), 256, 0, s->stream>>>(prep_args(s, x, y, z));
+    return hipGetLastError();
 }
 
 // One launch: ceil(n_terms / 64) bonded workgroups, then the tile work items.
 // Shell mode: tile work items only.
 hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z, bool shell) {
     TermArgs ta = term_args(s, x, y, z);
-    TileArgs A = tile_args(s, x, y, z);
+    TileArgs A = tile_args(s);
     if (shell) {
         ta.nb = ta.na = ta.nd = ta.n14 = 0;
         A.p = s->nbp_shell;
@@ -350,6 +499,13 @@ hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const doub
                           : (A.cubic ? k_forces<true, false> : k_forces<false, false>);
         kern<<<(unsigned)grid, 64, 0, s->stream>>>(A, ta);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_prep(const ReplicaDesc *reps, const int *atom_rep, const int64_t *atom_off,
+                             int64_t n_atoms, hipStream_t st) {
+    if (n_atoms > 0)
+        k_batch_prep<<<(unsigned)((n_atoms + 255) / 256), 256, 0, st>>>(reps, atom_rep, atom_off, n_atoms);
     return hipGetLastError();
 }
 

@@ -364,6 +364,7 @@ gm_status gm_surrogate_create(gm_ctx *ctx, const gm_system_desc *exact, const gm
         q.rin_sw_u = (float)(d->r_inner_switch / step);
         q.inv_w_in_u = (float)(step / w_in);
         q.ds_coef_in_u = (float)(-30.0 * step / w_in);
+        q.nrw_in = (float)(-d->r_inner_switch / w_in);
         s->ex->nbp_shell = q;
     }
     rc = alloc_state(s);

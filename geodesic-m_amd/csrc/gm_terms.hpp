@@ -187,25 +187,40 @@ __device__ inline double term_lj14(const TermArgs &t, int64_t n) {
 
 // Everything one nonbonded work item reads (one system or one replica).
 struct TileArgs {
-    const double *x, *y, *z;           // positions, atom order
-    const float2 *atom_par;            // (sigma/2, sqrt(eps)) per atom
-    const int *slot_atom;              // slot -> atom (-1 padding)
+    const int4 *qpos;                  // per slot: fixed-point x, y, z and the bits of sigma/2
+                                       // (kernel length unit), written by the prep kernel
+    const float *qeps;                 // per slot: 2 sqrt(eps)
     const int *wi_sc, *wi_start, *wi_end;
     const int2 *ent_meta;              // (j cluster, i-cluster mask)
-    const int *ent_ja;                 // j-cluster atoms per entry
+    const uint2 *ent_dmin;             // per entry, one byte per tile: closest listed pair at build
     const uint8_t *ent_lb;             // per-lane mask bytes per entry
+    const unsigned *disp;              // largest displacement since the build (f32 bits, A);
+                                       // null: no tile pruning
+    float prune_scale, prune_bias;     // keep a tile iff byte <= disp * scale + bias
     unsigned long long *F;             // int64 fixed-point accumulator, slot order
     int n_pad;                         // slots
     double *e_wi;                      // per work item LJ energy
     NbParams p;
-    double iq[3];                      // 2^32 / box
     int cubic;
+};
+
+// Everything the per-evaluation prep kernel reads and writes (one system or replica).
+struct PrepArgs {
+    const double *x, *y, *z;           // positions, atom order
+    const int *atom_slot;
+    const uint32_t *qref;              // fixed-point positions at the list build, 3 x n, atom order
+    int4 *qpos;                        // x, y, z written here; w (sigma/2) set by the build
+    unsigned *disp;
+    double iq[3];                      // 2^32 / box
+    float sq_len[3];                   // box / 2^32, A
+    int n;
 };
 
 // One replica of a batch: its tile and bonded arguments and item split.
 struct ReplicaDesc {
     TileArgs tile;
     TermArgs terms;
+    PrepArgs prep;
     int n_bonded_groups;               // items [0, n) are bonded groups, then work items
     int n_wi;
 };

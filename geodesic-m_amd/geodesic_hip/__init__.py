@@ -67,7 +67,8 @@ class EnergyTerms(C.Structure):
 
 class Profile(C.Structure):
     _fields_ = [("n_evals", _I64), ("n_untimed", _I64), ("ms_nonbonded_total", _D),
-                ("ms_nonbonded_min", _D), ("ms_nonbonded_max", _D), ("ms_eval_total", _D)]
+                ("ms_nonbonded_min", _D), ("ms_nonbonded_max", _D), ("ms_eval_total", _D),
+                ("ms_prep_total", _D)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -145,6 +146,7 @@ def lib():
             "gm_profile_begin": ([_P, C.c_int32], C.c_int),
             "gm_profile_end": ([_P, C.POINTER(Profile)], C.c_int),
             "gm_set_work_chunk": ([_P, C.c_int32], C.c_int),
+            "gm_set_tile_pruning": ([_P, C.c_int32], C.c_int),
             "gm_set_positions": ([_P, _P, _P, _P], C.c_int),
             "gm_compute_forces_resident": ([_P], C.c_int),
             "gm_get_forces": ([_P, _P, _P, _P], C.c_int),
@@ -490,6 +492,11 @@ class System:
 
     def set_work_chunk(self, entries_per_item):
         _check(lib().gm_set_work_chunk(self.h, int(entries_per_item)))
+
+    def set_tile_pruning(self, enable):
+        """Skip tiles provably beyond r_cutoff (default on); results are
+        bit-identical either way."""
+        _check(lib().gm_set_tile_pruning(self.h, 1 if enable else 0))
 
     def stats(self):
         s = Stats()

@@ -182,6 +182,7 @@ typedef struct {
     double ms_nonbonded_total;  /* sum over timed evaluations, nonbonded kernel */
     double ms_nonbonded_min, ms_nonbonded_max;
     double ms_eval_total;       /* sum over timed evaluations, whole evaluation */
+    double ms_prep_total;       /* sum over timed evaluations, position prep kernel */
 } gm_profile;
 gm_status gm_profile_begin(gm_sys *sys, int32_t every);
 gm_status gm_profile_end(gm_sys *sys, gm_profile *out);
@@ -190,6 +191,16 @@ gm_status gm_profile_end(gm_sys *sys, gm_profile *out);
  * 32); 0 picks a size that gives about 24 work items per compute unit.  Takes effect
  * immediately on a built list (no rebuild). */
 gm_status gm_set_work_chunk(gm_sys *sys, int32_t entries_per_item);
+
+/* Tile pruning (default on): each evaluation bounds how far any atom has moved
+ * since the list build, and skips the tiles whose closest listed pair at
+ * build time must now lie beyond r_cutoff.  Those tiles would add exactly
+ * zero, so forces and energies are bit-identical either way; 0 turns the
+ * test off (every listed tile is evaluated).  Pruning applies only when every
+ * box edge is at least 2 r_skin.  No reference counterpart (the reference
+ * tests every listed pair, neighbor.rs / nonbonded.rs:38-41).  A batch reads
+ * its replicas' setting when its lists are built. */
+gm_status gm_set_tile_pruning(gm_sys *sys, int32_t enable);
 
 /* ---- Device-resident dynamics ----------------------------------------------
  * integrator.rs:14-71 (half_kick, drift_and_constrain), constraint.rs:62-259

@@ -65,9 +65,12 @@ def check_forces(f, g, tag, synthetic=False):
     assert rms <= F_RMS, f"{tag}: force RMS rel {rms:.3e}"
     f_rms = float(np.sqrt(np.mean(gx ** 2 + gy ** 2 + gz ** 2)))
     extra = 1e-6 * f_rms if synthetic else 0.0
+    worst = 0.0
     for a, b in ((fx, gx), (fy, gy), (fz, gz)):
         bound = 1e-4 * np.maximum(np.abs(b), 1.0) + extra
+        worst = max(worst, float(np.max(np.abs(a - b) / bound)))
         assert np.all(np.abs(a - b) <= bound), f"{tag}: component bound, max {np.max(np.abs(a - b) / bound):.3e}"
+    say(f"[gpu] {tag}: force rms rel {rms:.3e}, worst component / bound {worst:.3f}")
     return rms
 
 
@@ -175,6 +178,44 @@ def test_24k_work_chunking_does_not_change_results(sys24k):
         for u, v in zip(f, f0):
             assert np.max(np.abs(u - v)) < 1e-6 * max(1.0, np.max(np.abs(v)))
         assert abs(e.lj - e0.lj) <= 1e-9 * abs(e0.lj)
+
+
+def displaced(x, y, z, amp, seed):
+    """Every atom moved by a random direction times U[0, amp] (A)."""
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(3, len(x)))
+    d *= amp * rng.uniform(0.0, 1.0, len(x)) / np.linalg.norm(d, axis=0)
+    return x + d[0], y + d[1], z + d[2]
+
+
+def test_tile_pruning_is_exact(sys24k):
+    """Tiles whose closest listed pair at build time is provably beyond r_c
+    after the atoms' largest displacement are skipped: results are byte-
+    identical with pruning off, at the build positions, after displacements
+    below half the skin (0.75 A; the bound shrinks what is pruned) and past it
+    (nothing can be pruned).  At 0.7 A the forces from the stale list also match
+    the oracle evaluating the same list (CpuBackend between rebuilds)."""
+    s, ((wx, wy, wz), (pi, pj), _, _) = sys24k
+    say("[gpu] 24k tile pruning")
+    g, (x, y, z), _, _ = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params)
+    p = s.params
+    for amp in (0.0, 0.2, 0.7, 1.2):
+        xs, ys, zs = (x, y, z) if amp == 0.0 else displaced(x, y, z, amp, seed=int(amp * 10))
+        g.set_tile_pruning(True)
+        e1, *f1 = g.compute_forces(xs, ys, zs)
+        g.set_tile_pruning(False)
+        e2, *f2 = g.compute_forces(xs, ys, zs)
+        g.set_tile_pruning(True)
+        for u, v in zip(f1, f2):
+            assert u.tobytes() == v.tobytes(), f"amp {amp}"
+        for name in ("bond", "angle", "dihedral", "lj", "lj14", "total"):
+            assert getattr(e1, name) == getattr(e2, name), f"amp {amp} {name}"
+        if amp == 0.7:
+            t, *fo = orc.cpu_backend_compute(xs, ys, zs, s.sigma, s.epsilon, s.topo, pi, pj, p.box,
+                                             p.r_cutoff, p.r_switch, n_threads=8, scale14=p.scale_14,
+                                             bonded_min_image=p.bonded_min_image, periodic=p.periodic)
+            check_terms(e1, t, "24k displaced 0.7 A")
+            check_forces(f1, fo, "24k displaced 0.7 A", synthetic=True)
 
 
 def test_needs_rebuild_matches_oracle(sys24k):
