@@ -67,24 +67,43 @@ bool indices_ok(const uint32_t *a, int64_t cnt, int64_t n) {
     return true;
 }
 
-// Work items: each supercluster's entries cut into runs of `chunk` entries;
-// one wavefront per run.
+// Work items: runs of at most `chunk` consecutive list entries of one
+// supercluster, one wavefront each.  Automatic chunk: about 1.5 work items per
+// resident wavefront slot (5 per SIMD), 4 to 32 entries.  Items are issued
+// heaviest first (estimated cost: tiles plus per-entry and per-item overhead,
+// in tile units): workgroups are dispatched in index order as slots free up,
+// so this is longest-processing-time-first list scheduling and the light
+// items fill the tail.
 gm_status make_work_items(gm_sys *s) {
     const std::vector<int> &start = s->sc_start_host;
-    const int64_t total = s->nent;
+    const std::vector<uint8_t> &nt = s->ent_tiles_host;
+    constexpr double kEntryCost = 0.5, kItemCost = 4.0;
+    struct Item {
+        int sc, e0, e1;
+        double cost;
+    };
     int64_t chunk = s->chunk;
     if (chunk <= 0) {
-        const int64_t target = 24LL * (s->ctx->n_cu > 0 ? s->ctx->n_cu : 256);
-        chunk = std::max<int64_t>(4, (total + target - 1) / std::max<int64_t>(target, 1));
+        const int64_t slots = 20LL * (s->ctx->n_cu > 0 ? s->ctx->n_cu : 256);
+        chunk = (2 * s->nent + 3 * slots - 1) / (3 * slots);
+        chunk = std::max<int64_t>(4, chunk);
     }
     chunk = std::min<int64_t>(chunk, kMaxRun);
-    std::vector<int> wsc, wst, wen;
+    std::vector<Item> items;
     for (int64_t k = 0; k < s->nsc; k++)
         for (int64_t e0 = start[(size_t)k]; e0 < start[(size_t)k + 1]; e0 += chunk) {
-            wsc.push_back((int)k);
-            wst.push_back((int)e0);
-            wen.push_back((int)std::min<int64_t>(e0 + chunk, start[(size_t)k + 1]));
+            const int64_t e1 = std::min<int64_t>(e0 + chunk, start[(size_t)k + 1]);
+            double c = kItemCost;
+            for (int64_t e = e0; e < e1; e++) c += (double)nt[(size_t)e] + kEntryCost;
+            items.push_back({(int)k, (int)e0, (int)e1, c});
         }
+    std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
+    std::vector<int> wsc, wst, wen;
+    for (const Item &it : items) {
+        wsc.push_back(it.sc);
+        wst.push_back(it.e0);
+        wen.push_back(it.e1);
+    }
     s->n_wi = (int64_t)wsc.size();
     hipStream_t st = s->stream;
     GM_HIP(upload(s->wi_sc, wsc.data(), wsc.size(), st));
@@ -140,10 +159,18 @@ gm_status build_on_device(gm_sys *s, bool wrap) {
     GM_HIP(upload(s->sc_start, start.data(), start.size(), st));
 
     s->sc_start_host = start;
+    GM_HIP(launch_pair_fill_and_masks(s));
+    {
+        // tiles per entry, for the cost of each work item
+        std::vector<uint32_t> im((size_t)total);
+        if (total > 0)
+            GM_HIP(hipMemcpyAsync(im.data(), s->ent_imask.p, sizeof(uint32_t) * im.size(), hipMemcpyDeviceToHost, st));
+        GM_HIP(hipStreamSynchronize(st));
+        s->ent_tiles_host.resize((size_t)total);
+        for (size_t e = 0; e < im.size(); e++) s->ent_tiles_host[e] = (uint8_t)__builtin_popcount(im[e] & 0xffu);
+    }
     gm_status rc = make_work_items(s);
     if (rc != GM_OK) return rc;
-
-    GM_HIP(launch_pair_fill_and_masks(s));
     // the slot count changed: start the next evaluation from a zero accumulator
     GM_HIP(launch_fill_u64(s->F.p, (size_t)s->max_sc * 64 * 3, st));
     s->F_dirty = false;
@@ -511,8 +538,8 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     TRY(s->sp.ensure(slots));
     TRY(s->qpos.ensure(slots));
     TRY(s->qeps.ensure(slots));
-    TRY(s->disp.ensure(1));
-    TRY(launch_fill_i32(reinterpret_cast<int *>(s->disp.p), 1, 0, st));
+    TRY(s->disp.ensure((size_t)kDispSlots * kDispStride));
+    TRY(launch_fill_i32(reinterpret_cast<int *>(s->disp.p), (size_t)kDispSlots * kDispStride, 0, st));
     TRY(s->cl_ref.ensure((size_t)s->max_sc * 8));
     TRY(s->cl_half.ensure((size_t)s->max_sc * 8));
     TRY(s->sc_ref.ensure((size_t)s->max_sc));
@@ -802,6 +829,7 @@ gm_status gm_set_work_chunk(gm_sys *s, int32_t entries_per_item) {
     if (!s->built) return GM_OK;
     return make_work_items(s);
 }
+
 
 gm_status gm_set_tile_pruning(gm_sys *s, int32_t enable) {
     GM_REQUIRE(s && (enable == 0 || enable == 1), "bad argument");

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, int n, c
                                                  double *fy, double *fz, unsigned *disp) {
     __shared__ double sh[1024];
     const int b = blockIdx.x, t = threadIdx.x;
-    if (b == 0 && t == 0) *disp = 0u;
+    if (b == 0 && t < kDispSlots) disp[t * kDispStride] = 0u;
     if (b < sg.nseg) {
         const double v = block_sum(sg.ptr[b], sg.len[b], sh);
         if (t == 0) eout[b] = v;
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(1024) void k_batch_finish(const ReplicaDesc *__rest
     if (b < 5 * n_reps) {
         const ReplicaDesc &R = reps[b / 5];
         const int seg = b % 5;
-        if (seg == 0 && t == 0) *R.prep.disp = 0u;  // displacement bound of the next evaluation
+        if (seg == 0 && t < kDispSlots) R.prep.disp[t * kDispStride] = 0u;  // bound of the next evaluation
         const double *eb = R.terms.e_items;
         const double *ptr;
         long long len;

@@ -88,6 +88,10 @@ __device__ inline float fixdiff(int a, int b) { return (float)(int)((unsigned)a 
 // The margin (two steps and 1e-4 r_cutoff) covers the f32 rounding of the
 // stored distance, of the displacement and of r2 in the force kernel.
 constexpr int kPruneSteps = 254;  // bytes 0..254 span [r_cutoff, r_skin]
+// The displacement bound is raised in kDispSlots words kDispStride apart (a
+// block raises word blockIdx % kDispSlots) so that same-address atomics do not
+// serialise; readers take the largest.
+constexpr int kDispSlots = 16, kDispStride = 16;
 
 }  // namespace gm
 
@@ -158,8 +162,8 @@ struct gm_sys {
     DevBuf<float2> sp;       // (sigma/2, sqrt eps) per slot
     DevBuf<int4> qpos;       // per evaluation: fixed-point coords per slot + bits of sigma/2
     DevBuf<float> qeps;      // 2 sqrt(eps) per slot
-    DevBuf<unsigned> disp;   // [0]: largest displacement since the build (f32 bits, A);
-                             // raised by the prep kernel, cleared by the finish kernel
+    DevBuf<unsigned> disp;   // largest displacement since the build (f32 bits, A), in
+                             // kDispSlots words; raised by the prep kernel, cleared by the finish kernel
     bool prune = true;       // tile pruning requested (gm_set_tile_pruning)
     bool prune_geom = false; // box wide enough for the pruning bound (every edge >= 2 r_skin)
     DevBuf<int4> cl_ref;     // cluster bbox centre (fixed point) + valid flag
@@ -195,6 +199,7 @@ struct gm_sys {
     gm::NbParams nbp_shell{}; // shell-mode constants (set by a surrogate that owns this system)
     struct gm_dyn *dyn = nullptr;  // masses, constraints and dynamics buffers (gm_dynamics_set)
     std::vector<int> sc_start_host;
+    std::vector<uint8_t> ent_tiles_host;  // tiles per list entry (work item costs)
 
     bool built = false;
     int64_t nsc = 0, ncl = 0, nent = 0, n_wi = 0;

@@ -133,15 +133,19 @@ __device__ inline void atomic_max_nonneg(unsigned *slot, float v) {
     __hip_atomic_fetch_max(slot, __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ __launch_bounds__(256) void k_prep(PrepArgs P) {
-    const int a = blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(1024) void k_prep(PrepArgs P) {
+    const int a = blockIdx.x * 1024 + threadIdx.x;
     float d = 0.f;
     if (a < P.n) d = prep_atom(P, a);
     d = wave_max(d);
-    __shared__ float sm[4];
+    __shared__ float sm[16];
     if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = d;
     __syncthreads();
-    if (threadIdx.x == 0) atomic_max_nonneg(P.disp, fmaxf(fmaxf(sm[0], sm[1]), fmaxf(sm[2], sm[3])));
+    if (threadIdx.x < 64) {
+        d = threadIdx.x < 16 ? sm[threadIdx.x] : 0.f;
+        d = wave_max(d);
+        if (threadIdx.x == 0) atomic_max_nonneg(&P.disp[(blockIdx.x % kDispSlots) * kDispStride], d);
+    }
 }
 
 // Batched prep: atom a of the concatenated replicas.  A wavefront whose atoms
@@ -159,11 +163,12 @@ __global__ __launch_bounds__(256) void k_batch_prep(const ReplicaDesc *__restric
     }
     const int r0 = __shfl(r, 0);
     const bool uniform = __all(r == r0);
+    const size_t slot = (size_t)((blockIdx.x * 4 + (threadIdx.x >> 6)) % kDispSlots) * kDispStride;
     if (uniform) {
         d = wave_max(d);
-        if ((threadIdx.x & 63) == 0 && r0 >= 0) atomic_max_nonneg(reps[r0].prep.disp, d);
+        if ((threadIdx.x & 63) == 0 && r0 >= 0) atomic_max_nonneg(&reps[r0].prep.disp[slot], d);
     } else if (r >= 0) {
-        atomic_max_nonneg(reps[r].prep.disp, d);
+        atomic_max_nonneg(&reps[r].prep.disp[slot], d);
     }
 }
 
@@ -203,7 +208,13 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     {
         // keep tile ci of an entry unless byte_ci > bound (see kPruneSteps)
         float bound = 3.0e38f;
-        if (A.disp) bound = fmaf(__uint_as_float(*A.disp), A.prune_scale, A.prune_bias);
+        if (A.disp) {
+            float d = lane < kDispSlots ? __uint_as_float(A.disp[lane * kDispStride]) : 0.f;
+            for (int o = 1; o < kDispSlots; o <<= 1) d = fmaxf(d, __shfl_xor(d, o));
+            // lane 0 holds the largest of the slots
+            const float dmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(d)));
+            bound = fmaf(dmax, A.prune_scale, A.prune_bias);
+        }
         const uint32_t *lb = reinterpret_cast<const uint32_t *>(ent_lb + (size_t)e0 * 64);
         uint32_t lbv[kMaxRun / 4];
 #pragma unroll
@@ -479,7 +490,7 @@ PrepArgs prep_args(gm_sys *s, const double *x, const double *y, const double *z)
 // Slot-order fixed-point positions and the displacement bound; precedes every
 // launch_forces on the same positions.
 hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double *z) {
-    if (s->n > 0) k_prep<<<(unsigned)((s->n + 255) / 256), 256, 0, s->stream>>>(prep_args(s, x, y, z));
+    if (s->n > 0) k_prep<<<(unsigned)((s->n + 1023) / 1024), 1024, 0, s->stream>>>(prep_args(s, x, y, z));
     return hipGetLastError();
 }
 

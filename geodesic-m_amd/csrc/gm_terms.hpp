@@ -194,8 +194,8 @@ struct TileArgs {
     const int2 *ent_meta;              // (j cluster, i-cluster mask)
     const uint2 *ent_dmin;             // per entry, one byte per tile: closest listed pair at build
     const uint8_t *ent_lb;             // per-lane mask bytes per entry
-    const unsigned *disp;              // largest displacement since the build (f32 bits, A);
-                                       // null: no tile pruning
+    const unsigned *disp;              // largest displacement since the build (f32 bits, A),
+                                       // kDispSlots words; null: no tile pruning
     float prune_scale, prune_bias;     // keep a tile iff byte <= disp * scale + bias
     unsigned long long *F;             // int64 fixed-point accumulator, slot order
     int n_pad;                         // slots
