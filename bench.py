@@ -256,6 +256,32 @@ def bench_surrogate(frames=20000, steps=400, traj_cycles=25, min_steps=300):
             "pair_interactions_per_s_equivalent": n_cut / (mts_ms * 1e-3)}
 
 
+def pruning_study(g, s, x, y, z, e, args):
+    """Tile pruning skips tiles that lie beyond r_c at the current positions,
+    bounded by the largest displacement since the build: the same evaluation
+    with pruning off, and at positions drifted as after a whole 20-step
+    rebuild interval of 0.02 A Gaussian moves per coordinate (the 250k
+    workload's protocol), where the bound has grown the most."""
+    g.set_tile_pruning(False)
+    ms_off, prof_off, e_off, _ = time_resident(g, args.steps, args.warmup)
+    g.set_tile_pruning(True)
+    assert e_off.total == e.total
+    rng = np.random.default_rng(20)
+    d = rng.normal(0.0, 0.02 * 20 ** 0.5, (3, s.n))
+    g.set_positions(x + d[0], y + d[1], z + d[2])
+    ms_drift, prof_drift, _, _ = time_resident(g, args.steps, args.warmup)
+    g.set_tile_pruning(False)
+    ms_drift_off, _, _, _ = time_resident(g, args.steps, args.warmup)
+    g.set_tile_pruning(True)
+    g.set_positions(x, y, z)
+    return {"ms_per_step_pruning_off": ms_off,
+            "kernel_ms_pruning_off": prof_off["ms_nonbonded_total"] / max(prof_off["n_evals"], 1),
+            "drift_max_displacement_A": float(np.sqrt((d ** 2).sum(axis=0)).max()),
+            "ms_per_step_drifted": ms_drift,
+            "kernel_ms_drifted": prof_drift["ms_nonbonded_total"] / max(prof_drift["n_evals"], 1),
+            "ms_per_step_drifted_pruning_off": ms_drift_off}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,6 +292,8 @@ def main():
     ap.add_argument("--no-250k", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-pruning-study", action="store_true",
+                    help="skip the pruning-off and drifted-position timings (profiling)")
     ap.add_argument("--only", choices=("250k", "batched", "surrogate"), default=None,
                     help="run one secondary section alone and print its JSON (profiling)")
     args = ap.parse_args()
@@ -309,29 +337,7 @@ def main():
     nb_ms = prof["ms_nonbonded_total"] / max(prof["n_evals"], 1)
     ev_ms = prof["ms_eval_total"] / max(prof["n_evals"], 1)
     prep_ms = prof["ms_prep_total"] / max(prof["n_evals"], 1)
-    # Tile pruning skips tiles that lie beyond r_c at the current positions,
-    # bounded by the largest displacement since the build: the same evaluation
-    # with pruning off, and at positions drifted as after a whole 20-step
-    # rebuild interval of 0.02 A Gaussian moves per coordinate (the 250k
-    # workload's protocol), where the bound has grown the most.
-    g.set_tile_pruning(False)
-    ms_off, prof_off, e_off, _ = time_resident(g, args.steps, args.warmup)
-    g.set_tile_pruning(True)
-    assert e_off.total == e.total
-    rng = np.random.default_rng(20)
-    d = rng.normal(0.0, 0.02 * 20 ** 0.5, (3, s.n))
-    g.set_positions(x + d[0], y + d[1], z + d[2])
-    ms_drift, prof_drift, _, _ = time_resident(g, args.steps, args.warmup)
-    g.set_tile_pruning(False)
-    ms_drift_off, _, _, _ = time_resident(g, args.steps, args.warmup)
-    g.set_tile_pruning(True)
-    g.set_positions(x, y, z)
-    pruning = {"ms_per_step_pruning_off": ms_off,
-               "kernel_ms_pruning_off": prof_off["ms_nonbonded_total"] / max(prof_off["n_evals"], 1),
-               "drift_max_displacement_A": float(np.sqrt((d ** 2).sum(axis=0)).max()),
-               "ms_per_step_drifted": ms_drift,
-               "kernel_ms_drifted": prof_drift["ms_nonbonded_total"] / max(prof_drift["n_evals"], 1),
-               "ms_per_step_drifted_pruning_off": ms_drift_off}
+    pruning = None if args.no_pruning_study else pruning_study(g, s, x, y, z, e, args)
     achieved = n_cut * FLOP_PER_PAIR / (nb_ms * 1e-3) / 1e12
 
     out = {

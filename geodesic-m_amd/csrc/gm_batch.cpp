@@ -5,9 +5,9 @@
 // Each replica is an ordinary gm_sys (own box, topology, Verlet list) sharing
 // the batch's stream.  After the lists are built, a device table of replica
 // descriptors and two item lists (replica, bonded group) and (replica, tile
-// work item) are written.  k_batch_bonded runs the bonded terms on a side
-// branch while k_batch_prep writes every replica's slot-order fixed-point
-// positions and displacement bound and the persistent k_batch_tiles walks the
+// work item) are written.  k_batch_prep writes every replica's slot-order
+// fixed-point positions and displacement bound; then k_batch_bonded runs the
+// bonded terms on a side branch while the persistent k_batch_tiles walks the
 // tile items with a fixed grid of wavefronts (static, XCD-contiguous ranges); k_batch_finish
 // then reduces per-replica f64 energies and writes the concatenated f64
 // forces.  The three kernels are captured once per list build into a
@@ -89,13 +89,15 @@ gm_status make_plan(gm_batch *b) {
 
     drop_graph(b);
     const int grid = std::max(1, b->waves_per_cu * (b->ctx->n_cu > 0 ? b->ctx->n_cu : 256));
-    // graph: fork -> {bonded terms on the side branch | persistent tiles} -> join -> finish
+    // graph: prep -> fork -> {bonded terms on the side branch | persistent tiles} -> join -> finish
+    // (the prep runs alone: sharing the machine with the bonded terms would
+    // stretch it, and the tiles wait for it)
     GM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    hipError_t e1 = hipEventRecord(b->fork, st);
+    hipError_t e1 = launch_batch_prep(b->d_reps.p, b->atom_rep.p, b->atom_off.p, b->off.back(), st);
+    if (e1 == hipSuccess) e1 = hipEventRecord(b->fork, st);
     if (e1 == hipSuccess) e1 = hipStreamWaitEvent(b->side, b->fork, 0);
     if (e1 == hipSuccess) e1 = launch_batch_bonded(b->d_reps.p, b->d_items.p, b->n_bonded_items, b->side);
     if (e1 == hipSuccess) e1 = hipEventRecord(b->join, b->side);
-    if (e1 == hipSuccess) e1 = launch_batch_prep(b->d_reps.p, b->atom_rep.p, b->atom_off.p, b->off.back(), st);
     if (e1 == hipSuccess)
         e1 = launch_batch_tiles(b->d_reps.p, b->d_items.p + b->n_bonded_items, b->n_items, grid, st);
     if (e1 == hipSuccess) e1 = hipStreamWaitEvent(st, b->join, 0);

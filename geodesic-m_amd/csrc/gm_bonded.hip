@@ -20,26 +20,37 @@ struct Segs {
 };
 
 // Fixed-order f64 sum of one segment by a 1024-thread block: thread t adds
-// items t, t+1024, ... in order, then a fixed LDS tree.
+// items t, t + 1024, ... into four running sums (items 4k + 0..3 apart by
+// 1024, so four loads are in flight), combines them, each wavefront sums its
+// lanes with DPP and permlane steps, and thread 0 adds the 16 wave sums in
+// order.  The result is valid in thread 0.
 __device__ inline double block_sum(const double *ptr, long long len, double *sh) {
     const int t = threadIdx.x;
-    double a = 0.0;
-    for (long long i = t; i < len; i += 1024) a += ptr[i];
-    sh[t] = a;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if (t < w) sh[t] = sh[t] + sh[t + w];
-        __syncthreads();
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    long long i = t;
+    for (; i + 3072 < len; i += 4096) {
+        a0 += ptr[i];
+        a1 += ptr[i + 1024];
+        a2 += ptr[i + 2048];
+        a3 += ptr[i + 3072];
     }
-    return sh[0];
+    for (; i < len; i += 1024) a0 += ptr[i];
+    double a = wave_sum_f64((a0 + a1) + (a2 + a3));
+    if ((t & 63) == 0) sh[t >> 6] = a;
+    __syncthreads();
+    double r = 0.0;
+    if (t == 0)
+        for (int w = 0; w < 16; w++) r += sh[w];
+    return r;
 }
 
-// Blocks [0, nseg): block b sums segment b.  Blocks >= nseg: one thread per atom.
-// Block 0 also clears the displacement bound the prep kernel raised.
-__global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, int n, const int *atom_slot,
+// Blocks [0, nseg): block b sums segment b.  Blocks >= nseg: one thread per
+// slot (reads of the accumulator are contiguous; the f64 force goes to the
+// slot's atom).  Block 0 also clears the displacement bound the prep kernel raised.
+__global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, const int *slot_atom,
                                                  unsigned long long *F, size_t n_pad, double *fx,
                                                  double *fy, double *fz, unsigned *disp) {
-    __shared__ double sh[1024];
+    __shared__ double sh[16];
     const int b = blockIdx.x, t = threadIdx.x;
     if (b == 0 && t < kDispSlots) disp[t * kDispStride] = 0u;
     if (b < sg.nseg) {
@@ -47,15 +58,17 @@ __global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, int n, c
         if (t == 0) eout[b] = v;
         return;
     }
-    const int a = (b - sg.nseg) * 1024 + t;
-    if (a >= n) return;
-    const size_t s = (size_t)atom_slot[a];
-    fx[a] = (double)(long long)F[s] * kFixedInv;
-    fy[a] = (double)(long long)F[n_pad + s] * kFixedInv;
-    fz[a] = (double)(long long)F[2 * n_pad + s] * kFixedInv;
+    const size_t s = (size_t)(b - sg.nseg) * 1024 + t;
+    if (s >= n_pad) return;
+    const int a = slot_atom[s];
+    const long long qx = (long long)F[s], qy = (long long)F[n_pad + s], qz = (long long)F[2 * n_pad + s];
     F[s] = 0ull;
     F[n_pad + s] = 0ull;
     F[2 * n_pad + s] = 0ull;
+    if (a < 0) return;
+    fx[a] = (double)qx * kFixedInv;
+    fy[a] = (double)qy * kFixedInv;
+    fz[a] = (double)qz * kFixedInv;
 }
 
 // Batched end of evaluation: blocks [0, 5 R) sum replica b/5's segment b%5
@@ -125,9 +138,10 @@ hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz, bool lj_
     sg.ptr[3] = eb + s->nb + s->na + s->nd; sg.len[3] = lj_only ? 0 : n14;
     sg.ptr[4] = s->e_wi.p;                  sg.len[4] = s->n_wi;
     sg.nseg = 5;
-    const unsigned grid = 5u + (unsigned)((s->n + 1023) / 1024);
-    k_finish<<<grid, 1024, 0, s->stream>>>(sg, s->e_out.p, (int)s->n, s->atom_slot.p, s->F.p,
-                                            (size_t)s->nsc * 64, fx, fy, fz, s->disp.p);
+    const size_t n_pad = (size_t)s->nsc * 64;
+    const unsigned grid = 5u + (unsigned)((n_pad + 1023) / 1024);
+    k_finish<<<grid, 1024, 0, s->stream>>>(sg, s->e_out.p, s->slot_atom.p, s->F.p, n_pad, fx, fy, fz,
+                                            s->disp.p);
     return hipGetLastError();
 }
 

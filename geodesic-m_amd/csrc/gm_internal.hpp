@@ -80,6 +80,37 @@ __device__ inline int quantize_fixed(double x, double inv_q) {
 // minimum-image displacement in fixed-point steps, rounded once.
 __device__ inline float fixdiff(int a, int b) { return (float)(int)((unsigned)a - (unsigned)b); }
 
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// v[l] + v[l ^ 16] (rows) or v[l] + v[l ^ 32], by the gfx950 permlane swaps.
+__device__ inline double swap_self_f64(double v, bool rows) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)b, hi = (unsigned)(b >> 32);
+    const auto l = rows ? __builtin_amdgcn_permlane16_swap(lo, lo, false, false)
+                        : __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h = rows ? __builtin_amdgcn_permlane16_swap(hi, hi, false, false)
+                        : __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    return __longlong_as_double(((long long)h[0] << 32) | l[0]) +
+           __longlong_as_double(((long long)h[1] << 32) | l[1]);
+}
+
+// Sum over a full wavefront in a fixed order (lane bits 0, 1, 2, 3, 4, 5);
+// every lane receives the total.
+__device__ inline double wave_sum_f64(double v) {
+    v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);  // row_half_mirror: the other quad of the 8
+    v += dpp_f64<0x128>(v);  // row_ror:8: the other 8 of the row
+    v = swap_self_f64(v, true);
+    return swap_self_f64(v, false);
+}
+
 // Tile pruning.  The list build stores, per tile, a byte b with
 //   (closest listed pair of the tile at build) >= r_cutoff + b * step.
 // If no atom has moved farther than d since the build, every pair of the tile

@@ -71,36 +71,6 @@ __device__ inline float swap_add16(float a, float b) {
     return __uint_as_float(r[0]) + __uint_as_float(r[1]);
 }
 
-template <int CTRL>
-__device__ inline double dpp_f64(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
-__device__ inline double swap_self_f64(double v, bool rows) {
-    const long long b = __double_as_longlong(v);
-    const unsigned lo = (unsigned)b, hi = (unsigned)(b >> 32);
-    const auto l = rows ? __builtin_amdgcn_permlane16_swap(lo, lo, false, false)
-                        : __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
-    const auto h = rows ? __builtin_amdgcn_permlane16_swap(hi, hi, false, false)
-                        : __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
-    return __longlong_as_double(((long long)h[0] << 32) | l[0]) +
-           __longlong_as_double(((long long)h[1] << 32) | l[1]);
-}
-
-// Sum over the wavefront in a fixed order (lane bits 0, 1, 2, 3, 4, 5); every
-// lane receives the total.
-__device__ inline double wave_sum_f64(double v) {
-    v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
-    v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
-    v += dpp_f64<0x141>(v);  // row_half_mirror: the other quad of the 8
-    v += dpp_f64<0x128>(v);  // row_ror:8: the other 8 of the row
-    v = swap_self_f64(v, true);
-    return swap_self_f64(v, false);
-}
-
 // A wave-uniform value copied into a vector register (a plain move, no load).
 __device__ inline float in_vgpr(float v) {
     float r;
@@ -149,26 +119,24 @@ __global__ __launch_bounds__(1024) void k_prep(PrepArgs P) {
 }
 
 // Batched prep: atom a of the concatenated replicas.  A wavefront whose atoms
-// all belong to one replica raises that replica's bound once; one that
-// straddles a boundary raises it per lane.
+// all belong to one replica (nearly all of them) reads that replica's
+// descriptor once, with scalar loads, and raises its bound once; one that
+// straddles a boundary reads the descriptor and raises the bound per lane.
 __global__ __launch_bounds__(256) void k_batch_prep(const ReplicaDesc *__restrict__ reps,
                                                     const int *__restrict__ atom_rep,
                                                     const int64_t *__restrict__ atom_off, int64_t n_atoms) {
     const int64_t a = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    int r = -1;
-    float d = 0.f;
-    if (a < n_atoms) {
-        r = atom_rep[a];
-        d = prep_atom(reps[r].prep, (int)(a - atom_off[r]));
-    }
-    const int r0 = __shfl(r, 0);
-    const bool uniform = __all(r == r0);
+    const int r = a < n_atoms ? atom_rep[a] : -1;
+    const int r0 = __builtin_amdgcn_readfirstlane(r);
     const size_t slot = (size_t)((blockIdx.x * 4 + (threadIdx.x >> 6)) % kDispSlots) * kDispStride;
-    if (uniform) {
-        d = wave_max(d);
-        if ((threadIdx.x & 63) == 0 && r0 >= 0) atomic_max_nonneg(&reps[r0].prep.disp[slot], d);
+    if (__all(r == r0)) {
+        if (r0 < 0) return;
+        const PrepArgs &P = reps[r0].prep;
+        const float d = wave_max(prep_atom(P, (int)(a - atom_off[r0])));
+        if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&P.disp[slot], d);
     } else if (r >= 0) {
-        atomic_max_nonneg(&reps[r].prep.disp[slot], d);
+        const PrepArgs &P = reps[r].prep;
+        atomic_max_nonneg(&P.disp[slot], prep_atom(P, (int)(a - atom_off[r])));
     }
 }
 
