@@ -76,11 +76,11 @@ bool indices_ok(const uint32_t *a, int64_t cnt, int64_t n) {
 // items fill the tail.
 gm_status make_work_items(gm_sys *s) {
     const std::vector<int> &start = s->sc_start_host;
-    const std::vector<uint8_t> &nt = s->ent_tiles_host;
-    constexpr double kEntryCost = 0.5, kItemCost = 4.0;
+    const uint32_t *im = s->imask_host;  // per entry i-cluster mask: popcount = tiles
+    constexpr int kEntryCost2 = 1, kItemCost2 = 8;  // twice the per-entry and per-item cost, in tiles
     struct Item {
         int sc, e0, e1;
-        double cost;
+        int cost;  // twice the estimated cost, in tiles
     };
     int64_t chunk = s->chunk;
     if (chunk <= 0) {
@@ -90,19 +90,27 @@ gm_status make_work_items(gm_sys *s) {
     }
     chunk = std::min<int64_t>(chunk, kMaxRun);
     std::vector<Item> items;
+    items.reserve((size_t)(s->nent / chunk + s->nsc + 1));
     for (int64_t k = 0; k < s->nsc; k++)
         for (int64_t e0 = start[(size_t)k]; e0 < start[(size_t)k + 1]; e0 += chunk) {
             const int64_t e1 = std::min<int64_t>(e0 + chunk, start[(size_t)k + 1]);
-            double c = kItemCost;
-            for (int64_t e = e0; e < e1; e++) c += (double)nt[(size_t)e] + kEntryCost;
+            int c = kItemCost2;
+            for (int64_t e = e0; e < e1; e++) c += 2 * __builtin_popcount(im[e] & 0xffu) + kEntryCost2;
             items.push_back({(int)k, (int)e0, (int)e1, c});
         }
-    std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
-    std::vector<int> wsc, wst, wen;
+    // stable counting sort by decreasing cost (costs are small integers)
+    int cmax = 0;
+    for (const Item &it : items) cmax = std::max(cmax, it.cost);
+    std::vector<int> first((size_t)cmax + 2, 0);
+    for (const Item &it : items) first[(size_t)(cmax - it.cost) + 1]++;
+    for (size_t c = 1; c < first.size(); c++) first[c] += first[c - 1];
+    const size_t ni = items.size();
+    std::vector<int> wsc(ni), wst(ni), wen(ni);
     for (const Item &it : items) {
-        wsc.push_back(it.sc);
-        wst.push_back(it.e0);
-        wen.push_back(it.e1);
+        const size_t o = (size_t)first[(size_t)(cmax - it.cost)]++;
+        wsc[o] = it.sc;
+        wst[o] = it.e0;
+        wen[o] = it.e1;
     }
     s->n_wi = (int64_t)wsc.size();
     hipStream_t st = s->stream;
@@ -161,13 +169,19 @@ gm_status build_on_device(gm_sys *s, bool wrap) {
     s->sc_start_host = start;
     GM_HIP(launch_pair_fill_and_masks(s));
     {
-        // tiles per entry, for the cost of each work item
-        std::vector<uint32_t> im((size_t)total);
+        // tiles per entry, for the cost of each work item (read back into pinned memory)
+        if (s->imask_host_n < (size_t)total) {
+            if (s->imask_host) (void)hipHostFree(s->imask_host);
+            s->imask_host = nullptr;
+            s->imask_host_n = 0;
+            GM_HIP(hipHostMalloc((void **)&s->imask_host, sizeof(uint32_t) * (size_t)std::max<int64_t>(total, 1),
+                                 hipHostMallocDefault));
+            s->imask_host_n = (size_t)std::max<int64_t>(total, 1);
+        }
         if (total > 0)
-            GM_HIP(hipMemcpyAsync(im.data(), s->ent_imask.p, sizeof(uint32_t) * im.size(), hipMemcpyDeviceToHost, st));
+            GM_HIP(hipMemcpyAsync(s->imask_host, s->ent_imask.p, sizeof(uint32_t) * (size_t)total,
+                                  hipMemcpyDeviceToHost, st));
         GM_HIP(hipStreamSynchronize(st));
-        s->ent_tiles_host.resize((size_t)total);
-        for (size_t e = 0; e < im.size(); e++) s->ent_tiles_host[e] = (uint8_t)__builtin_popcount(im[e] & 0xffu);
     }
     gm_status rc = make_work_items(s);
     if (rc != GM_OK) return rc;
@@ -587,6 +601,8 @@ gm_status gm_system_destroy(gm_sys *s) {
     s->ent_mask.release();
     s->ent_meta.release();
     s->ent_dmin.release();
+    if (s->imask_host) (void)hipHostFree(s->imask_host);
+    s->imask_host = nullptr;
     s->qpos.release();
     s->qeps.release();
     s->disp.release();

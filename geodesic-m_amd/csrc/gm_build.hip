@@ -36,27 +36,72 @@ __global__ void k_fill_u64(unsigned long long *p, size_t n) {
     if (i < n) p[i] = 0ull;
 }
 
+// Minimum over a full wavefront (DPP within rows, permlane swaps across them);
+// every lane receives it.  All 64 lanes must be active.
+__device__ inline int wave_min_i32(int v) {
+    v = min(v, __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = min(v, __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = min(v, __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = min(v, __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false));  // row_ror:8
+    const auto r = __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false);
+    v = min((int)r[0], (int)r[1]);
+    const auto q = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
+    return min((int)q[0], (int)q[1]);
+}
+
+// counter[key] += 1 for every active lane, with one atomic per distinct key
+// in the wavefront (atoms in input order are often spatially coherent, and 64
+// same-address atomics in one instruction serialise).  Returns the counter's
+// old value plus the lane's rank among the active lanes with its key: the
+// lane's slot in its bucket.  Every lane of the wavefront must call it.
+__device__ inline int aggregated_fetch_add(int *counter, int key, bool active) {
+    const int lane = threadIdx.x & 63;
+    bool todo = active;
+    int old = 0;
+    while (true) {
+        const unsigned long long pend = __ballot(todo);
+        if (pend == 0) break;
+        const int leader = __ffsll((long long)pend) - 1;
+        const int k0 = __shfl(key, leader);
+        const bool mine = todo && key == k0;
+        const unsigned long long grp = __ballot(mine);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&counter[k0], __popcll(grp));
+        base = __shfl(base, leader);
+        if (mine) {
+            old = base + __popcll(grp & ((1ull << lane) - 1ull));
+            todo = false;
+        }
+    }
+    return old;
+}
+
 // neighbor.rs:13-15 + 78-88: wrap (periodic builds only), snapshot the
 // reference positions (neighbor.rs:48-50), quantise, bin into xy columns.
 __global__ void k_wrap_quantize(int n, double *x, double *y, double *z, double *rx, double *ry,
                                 double *rz, uint32_t *q, int *col_count, BuildParams bp, int wrap) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double xi = x[i], yi = y[i], zi = z[i];
-    if (wrap) {
-        xi = wrap_coord(xi, bp.box[0]);
-        yi = wrap_coord(yi, bp.box[1]);
-        zi = wrap_coord(zi, bp.box[2]);
-        x[i] = xi; y[i] = yi; z[i] = zi;
+    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = i0 < n;
+    const int i = active ? i0 : 0;
+    int col = 0;
+    if (active) {
+        double xi = x[i], yi = y[i], zi = z[i];
+        if (wrap) {
+            xi = wrap_coord(xi, bp.box[0]);
+            yi = wrap_coord(yi, bp.box[1]);
+            zi = wrap_coord(zi, bp.box[2]);
+            x[i] = xi; y[i] = yi; z[i] = zi;
+        }
+        rx[i] = xi; ry[i] = yi; rz[i] = zi;
+        uint32_t qx = quantize(xi, bp.inv_q[0]);
+        uint32_t qy = quantize(yi, bp.inv_q[1]);
+        uint32_t qz = quantize(zi, bp.inv_q[2]);
+        q[i] = qx; q[n + i] = qy; q[2 * n + i] = qz;
+        int cx = (int)(((uint64_t)qx * (uint64_t)bp.ncx) >> 32);
+        int cy = (int)(((uint64_t)qy * (uint64_t)bp.ncy) >> 32);
+        col = cx * bp.ncy + cy;
     }
-    rx[i] = xi; ry[i] = yi; rz[i] = zi;
-    uint32_t qx = quantize(xi, bp.inv_q[0]);
-    uint32_t qy = quantize(yi, bp.inv_q[1]);
-    uint32_t qz = quantize(zi, bp.inv_q[2]);
-    q[i] = qx; q[n + i] = qy; q[2 * n + i] = qz;
-    int cx = (int)(((uint64_t)qx * (uint64_t)bp.ncx) >> 32);
-    int cy = (int)(((uint64_t)qy * (uint64_t)bp.ncy) >> 32);
-    atomicAdd(&col_count[cx * bp.ncy + cy], 1);
+    (void)aggregated_fetch_add(col_count, col, active);
 }
 
 // Exclusive scans over columns: atom offsets and supercluster offsets; total
@@ -97,12 +142,16 @@ __global__ __launch_bounds__(1024) void k_scan_columns(int ncol, const int *cnt,
 
 __global__ void k_scatter(int n, const uint32_t *q, const int *col_start, int *col_fill, int *bucket,
                           BuildParams bp) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int cx = (int)(((uint64_t)q[i] * (uint64_t)bp.ncx) >> 32);
-    int cy = (int)(((uint64_t)q[n + i] * (uint64_t)bp.ncy) >> 32);
-    int c = cx * bp.ncy + cy;
-    bucket[col_start[c] + atomicAdd(&col_fill[c], 1)] = i;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = i < n;
+    int c = 0;
+    if (active) {
+        const int cx = (int)(((uint64_t)q[i] * (uint64_t)bp.ncx) >> 32);
+        const int cy = (int)(((uint64_t)q[n + i] * (uint64_t)bp.ncy) >> 32);
+        c = cx * bp.ncy + cy;
+    }
+    const int slot = aggregated_fetch_add(col_fill, c, active);
+    if (active) bucket[col_start[c] + slot] = i;
 }
 
 // One workgroup per column: bitonic sort of (z, atom) in LDS, which fixes the
@@ -436,7 +485,7 @@ __global__ __launch_bounds__(256) void k_masks(int nent, const int *ent_sc, cons
                 const float st = floorf((dl - bp.prune_rc) * bp.prune_inv_step);
                 db = st <= 0.f ? 0 : (st >= 255.f ? 255 : (int)st);
             }
-            for (int o = 1; o < 64; o <<= 1) db = min(db, __shfl_xor(db, o));
+            db = wave_min_i32(db);
             if (ci < 4) dlo |= (uint32_t)db << (8 * ci);
             else dhi |= (uint32_t)db << (8 * (ci - 4));
         }

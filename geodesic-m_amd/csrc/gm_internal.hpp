@@ -230,7 +230,8 @@ struct gm_sys {
     gm::NbParams nbp_shell{}; // shell-mode constants (set by a surrogate that owns this system)
     struct gm_dyn *dyn = nullptr;  // masses, constraints and dynamics buffers (gm_dynamics_set)
     std::vector<int> sc_start_host;
-    std::vector<uint8_t> ent_tiles_host;  // tiles per list entry (work item costs)
+    uint32_t *imask_host = nullptr;       // pinned readback of the entries' tile masks (work item costs)
+    size_t imask_host_n = 0;
 
     bool built = false;
     int64_t nsc = 0, ncl = 0, nent = 0, n_wi = 0;
