@@ -126,10 +126,101 @@ gm_status make_work_items(gm_sys *s) {
 constexpr int64_t kProfMax = 8192;
 constexpr int kProfStride = 4;  // events per profiled evaluation
 
+void refresh_shell(gm_sys *s) {
+    NbParams q = s->nbp;
+    const double w_in = s->shell_w;
+    const double step = s->qbox[0] / 4294967296.0;
+    q.rin_sw = (float)s->shell_rsw;
+    q.inv_w_in = (float)(1.0 / w_in);
+    q.ds_coef_in = (float)(-30.0 / w_in);
+    q.rin_sw_u = (float)(s->shell_rsw / step);
+    q.inv_w_in_u = (float)(step / w_in);
+    q.ds_coef_in_u = (float)(-30.0 * step / w_in);
+    q.nrw_in = (float)(-s->shell_rsw / w_in);
+    s->nbp_shell = q;
+}
+
+void set_quant_span(gm_sys *s, const double L[3], int ncx, int ncy) {
+    const double two32 = 4294967296.0;
+    NbParams &p = s->nbp;
+    BuildParams &bp = s->bp;
+    for (int a = 0; a < 3; a++) {
+        s->qbox[a] = L[a];
+        bp.inv_q[a] = two32 / L[a];
+        bp.sq[a] = (float)(L[a] / two32);
+    }
+    p.sx = (float)(L[0] / two32);
+    p.sy = (float)(L[1] / two32);
+    p.sz = (float)(L[2] / two32);
+    {
+        const double step = L[0] / two32;  // fixed-point step, A (cubic spans)
+        p.inv_s = (float)(1.0 / step);
+        p.rc2_u = (float)((s->rc / step) * (s->rc / step));
+        p.rsw2_u = (float)((s->rsw / step) * (s->rsw / step));
+        p.rsw_u = (float)(s->rsw / step);
+        p.inv_w_u = (float)(step / (s->rc - s->rsw));
+        p.ds_coef_u = (float)(-30.0 * step / (s->rc - s->rsw));
+        p.fscale_u = (float)(two32 / step);
+    }
+    bp.ncx = ncx;
+    bp.ncy = ncy;
+    bp.ncol = ncx * ncy;
+    {
+        // The f32 estimate of r2 from fixed-point displacements errs by a few
+        // ulp plus 2 r (quantisation step) sqrt(3); the band is far wider.
+        const double qmax = std::max(L[0], std::max(L[1], L[2])) / two32;
+        bp.band_skin = (float)(1e-4 * bp.r_skin2 + 16.0 * s->rskin * qmax);
+        bp.band_cut = (float)(1e-4 * bp.r_cut2 + 16.0 * s->rc * qmax);
+    }
+    bp.wcol[0] = (float)(L[0] / bp.ncx);
+    bp.wcol[1] = (float)(L[1] / bp.ncy);
+    bp.kx = (int)std::min<double>(bp.ncx, std::floor(bp.r_search / bp.wcol[0]) + 1.0);
+    bp.ky = (int)std::min<double>(bp.ncy, std::floor(bp.r_search / bp.wcol[1]) + 1.0);
+    // the pruning bound needs every pair's nearest image in the span to stay
+    // the nearest one under displacements below (r_skin - r_cutoff) / 2: edges >= 2 r_skin
+    s->prune_geom = std::min(L[0], std::min(L[1], L[2])) >= 2.0 * s->rskin * (1.0 + 1e-6);
+    if (s->has_shell) refresh_shell(s);
+}
+
+// Non-periodic systems: the fixed-point span is a cube of edge the power of
+// two at or above 2 (extent + 2 r_skin), so that no two atoms of the build
+// nor any listed pair until the next rebuild (each atom moves less than
+// (r_skin - r_cutoff) / 2) are half a span apart: the wrapped 32-bit
+// difference is the plain difference.  When that cube does not fit in the
+// smallest box edge, or a position is not finite, the span is the box.
+gm_status fit_span_to_atoms(gm_sys *s) {
+    double ext[8];
+    GM_HIP(launch_extent(s, s->ext.p));
+    GM_HIP(hipMemcpyAsync(ext, s->ext.p, sizeof(ext), hipMemcpyDeviceToHost, s->stream));
+    GM_HIP(hipStreamSynchronize(s->stream));
+    const double e[3] = {ext[3] - ext[0], ext[4] - ext[1], ext[5] - ext[2]};
+    const double emax = std::max(e[0], std::max(e[1], e[2]));
+    const double need = 2.0 * (emax + 2.0 * s->rskin);
+    const double bmin = std::min(s->box[0], std::min(s->box[1], s->box[2]));
+    const bool finite = ext[6] == 0.0 && std::isfinite(emax);
+    if (!finite || !(need <= bmin)) {
+        set_quant_span(s, s->box, s->box_ncx, s->box_ncy);
+        return GM_OK;
+    }
+    const double L = std::min(std::exp2(std::ceil(std::log2(need))), bmin);
+    // columns about the width of a cube holding 64 atoms at the density of the
+    // atoms' bounding box (each edge at least r_skin)
+    const double vol = std::max(e[0], s->rskin) * std::max(e[1], s->rskin) * std::max(e[2], s->rskin);
+    const double side = std::cbrt(64.0 * vol / (double)s->n);
+    const int nc = (int)std::max(1.0, std::min((double)s->ncx_cap, std::floor(L / side + 0.5)));
+    const double Lc[3] = {L, L, L};
+    set_quant_span(s, Lc, nc, nc);
+    return GM_OK;
+}
+
 gm_status build_on_device(gm_sys *s, bool wrap) {
     hipStream_t st = s->stream;
     s->built = false;
     GM_HIP(hipEventRecord(s->ev_b0, st));
+    if (s->fit_span && s->n > 0) {
+        gm_status rc = fit_span_to_atoms(s);
+        if (rc != GM_OK) return rc;
+    }
     GM_HIP(launch_build(s, wrap));
     int meta[4] = {0, 0, 0, 0};
     GM_HIP(hipMemcpyAsync(meta, s->meta.p, sizeof(meta), hipMemcpyDeviceToHost, st));
@@ -413,10 +504,6 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     s->nd = d->n_dihedrals;
     s->n14 = d->n_pairs14;
 
-    const double two32 = 4294967296.0;
-    s->nbp.sx = (float)(d->box[0] / two32);
-    s->nbp.sy = (float)(d->box[1] / two32);
-    s->nbp.sz = (float)(d->box[2] / two32);
     s->nbp.rc2 = (float)(d->r_cutoff * d->r_cutoff);
     s->nbp.rsw2 = (float)(d->r_switch * d->r_switch);
     s->nbp.rsw = (float)d->r_switch;
@@ -424,54 +511,35 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     s->nbp.ds_coef = (float)(-30.0 / (d->r_cutoff - d->r_switch));
     s->nbp.nrw = (float)(-d->r_switch / (d->r_cutoff - d->r_switch));
     s->nbp.nrw_in = 0.f;
-    {
-        const double step = d->box[0] / two32;  // fixed-point step, A (cubic boxes)
-        s->nbp.inv_s = (float)(1.0 / step);
-        s->nbp.rc2_u = (float)((d->r_cutoff / step) * (d->r_cutoff / step));
-        s->nbp.rsw2_u = (float)((d->r_switch / step) * (d->r_switch / step));
-        s->nbp.rsw_u = (float)(d->r_switch / step);
-        s->nbp.inv_w_u = (float)(step / (d->r_cutoff - d->r_switch));
-        s->nbp.ds_coef_u = (float)(-30.0 * step / (d->r_cutoff - d->r_switch));
-        s->nbp.fscale_u = (float)(two32 / step);
-    }
 
     BuildParams &bp = s->bp;
-    for (int a = 0; a < 3; a++) {
-        bp.box[a] = d->box[a];
-        bp.inv_q[a] = two32 / d->box[a];
-        bp.sq[a] = (float)(d->box[a] / two32);
-    }
+    for (int a = 0; a < 3; a++) bp.box[a] = d->box[a];
     bp.r_skin2 = d->r_skin * d->r_skin;
     bp.r_cut2 = d->r_cutoff * d->r_cutoff;
     bp.r_search = (float)(d->r_skin * (1.0 + 1e-5) + 1e-3);
+    bp.r_skin2f = (float)bp.r_skin2;
+    bp.r_cut2f = (float)bp.r_cut2;
+    bp.prune_rc = (float)d->r_cutoff;
+    bp.prune_inv_step = (float)(kPruneSteps / (d->r_skin - d->r_cutoff));
+    bp.periodic = s->periodic;
+    bp.n_atoms = (int)n;
     // xy columns about the width of a cube holding 64 atoms at the mean density
     const double vol = d->box[0] * d->box[1] * d->box[2];
     const double rho = n > 0 ? (double)n / vol : 1.0;
     const double side = std::cbrt(64.0 / rho);
-    bp.ncx = (int)std::max(1.0, std::min(512.0, std::floor(d->box[0] / side + 0.5)));
-    bp.ncy = (int)std::max(1.0, std::min(512.0, std::floor(d->box[1] / side + 0.5)));
-    bp.ncol = bp.ncx * bp.ncy;
-    bp.r_skin2f = (float)bp.r_skin2;
-    bp.r_cut2f = (float)bp.r_cut2;
-    {
-        // The f32 estimate of r2 from fixed-point displacements errs by a few
-        // ulp plus 2 r (quantisation step) sqrt(3); the band is far wider.
-        const double qmax = std::max(d->box[0], std::max(d->box[1], d->box[2])) / two32;
-        bp.band_skin = (float)(1e-4 * bp.r_skin2 + 16.0 * d->r_skin * qmax);
-        bp.band_cut = (float)(1e-4 * bp.r_cut2 + 16.0 * d->r_cutoff * qmax);
+    const int ncx = (int)std::max(1.0, std::min(512.0, std::floor(d->box[0] / side + 0.5)));
+    const int ncy = (int)std::max(1.0, std::min(512.0, std::floor(d->box[1] / side + 0.5)));
+    int ncol_max = ncx * ncy;
+    s->box_ncx = ncx;
+    s->box_ncy = ncy;
+    s->fit_span = !s->periodic;
+    if (s->fit_span) {
+        // a fitted span holds the atoms in about half its columns per axis
+        s->ncx_cap = (int)std::min(512.0, 4.0 * std::ceil(std::cbrt((double)n / 64.0)) + 4.0);
+        ncol_max = std::max(ncol_max, s->ncx_cap * s->ncx_cap);
     }
-    bp.wcol[0] = (float)(d->box[0] / bp.ncx);
-    bp.wcol[1] = (float)(d->box[1] / bp.ncy);
-    bp.kx = (int)std::min<double>(bp.ncx, std::floor(bp.r_search / bp.wcol[0]) + 1.0);
-    bp.ky = (int)std::min<double>(bp.ncy, std::floor(bp.r_search / bp.wcol[1]) + 1.0);
-    bp.prune_rc = (float)d->r_cutoff;
-    bp.prune_inv_step = (float)(kPruneSteps / (d->r_skin - d->r_cutoff));
-    // the pruning bound needs every pair's minimum image to stay the nearest
-    // one under displacements below (r_skin - r_cutoff) / 2: box edges >= 2 r_skin
-    s->prune_geom = std::min(d->box[0], std::min(d->box[1], d->box[2])) >= 2.0 * d->r_skin * (1.0 + 1e-6);
-    bp.periodic = s->periodic;
-    bp.n_atoms = (int)n;
-    s->max_sc = (n + 63) / 64 + bp.ncol;
+    set_quant_span(s, d->box, ncx, ncy);
+    s->max_sc = (n + 63) / 64 + ncol_max;
     bp.max_sc = (int)s->max_sc;
 
     const size_t N = (size_t)std::max<int64_t>(n, 1);
@@ -541,7 +609,8 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
         TRY(b->ensure(N));
     TRY(s->qatom.ensure(3 * N));
     for (DevBuf<int> *b : {&s->col_count, &s->col_start, &s->col_sc, &s->col_fill})
-        TRY(b->ensure((size_t)bp.ncol));
+        TRY(b->ensure((size_t)ncol_max));
+    TRY(s->ext.ensure(8));
     TRY(s->bucket.ensure(N));
     TRY(s->sorted_atoms.ensure(N));
     TRY(s->atom_slot.ensure(N));
@@ -577,7 +646,8 @@ gm_status gm_system_destroy(gm_sys *s) {
     s->dyn = nullptr;
     DevBuf<double> *dbl[] = {&s->sigma, &s->eps, &s->bk, &s->br0, &s->akth, &s->ath0, &s->dkphi,
                              &s->ddelta, &s->x, &s->y, &s->z, &s->refx, &s->refy, &s->refz,
-                             &s->fx, &s->fy, &s->fz, &s->sx, &s->sy, &s->sz, &s->e_items, &s->e_wi, &s->e_out};
+                             &s->fx, &s->fy, &s->fz, &s->sx, &s->sy, &s->sz, &s->e_items, &s->e_wi, &s->e_out,
+                             &s->ext};
     for (auto *b : dbl) b->release();
     DevBuf<uint32_t> *u32[] = {&s->bi, &s->bj, &s->ai, &s->aj, &s->ak, &s->di, &s->dj, &s->dk,
                                &s->dl, &s->dn, &s->p14i, &s->p14j, &s->excl_nbr, &s->qatom,

@@ -523,9 +523,46 @@ __global__ void k_needs_rebuild(int n, const double *x, const double *y, const d
 
 inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
 
+// Bounding box of the positions (one block): out[0..2] min, out[3..5] max,
+// out[6] = 1 if a coordinate is not finite.
+__global__ __launch_bounds__(1024) void k_extent(int n, const double *x, const double *y, const double *z,
+                                                double *out) {
+    __shared__ double sm[6][1024];
+    __shared__ int bad;
+    const int t = threadIdx.x;
+    if (t == 0) bad = 0;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    bool nf = false;
+    for (int i = t; i < n; i += 1024) {
+        const double v[3] = {x[i], y[i], z[i]};
+        for (int a = 0; a < 3; a++) {
+            nf = nf || !isfinite(v[a]);
+            lo[a] = fmin(lo[a], v[a]);
+            hi[a] = fmax(hi[a], v[a]);
+        }
+    }
+    for (int a = 0; a < 3; a++) {
+        sm[a][t] = lo[a];
+        sm[3 + a][t] = hi[a];
+    }
+    __syncthreads();
+    if (nf) bad = 1;
+    for (int off = 512; off > 0; off >>= 1) {
+        if (t < off)
+            for (int a = 0; a < 3; a++) {
+                sm[a][t] = fmin(sm[a][t], sm[a][t + off]);
+                sm[3 + a][t] = fmax(sm[3 + a][t], sm[3 + a][t + off]);
+            }
+        __syncthreads();
+    }
+    if (t < 6) out[t] = sm[t][0];
+    if (t == 6) out[6] = bad ? 1.0 : 0.0;
+    if (t == 7) out[7] = 0.0;
+}
+
 // sigma/2 in the force kernel's length unit: fixed-point steps for cubic boxes, else A
 inline float kernel_sigma_scale(const gm_sys *s) {
-    return (s->box[0] == s->box[1] && s->box[1] == s->box[2]) ? s->nbp.inv_s : 1.0f;
+    return span_is_cubic(s) ? s->nbp.inv_s : 1.0f;
 }
 
 }  // namespace
@@ -546,6 +583,11 @@ hipError_t launch_fill_u64(unsigned long long *p, size_t n, hipStream_t st) {
 
 // Wrap/quantise, bin, sort, split and box.  meta[0] (supercluster count),
 // meta[1] (column overflow) are valid once the stream reaches this point.
+hipError_t launch_extent(gm_sys *s, double *out) {
+    k_extent<<<1, 1024, 0, s->stream>>>((int)s->n, s->x.p, s->y.p, s->z.p, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_build(gm_sys *s, bool wrap) {
     hipStream_t st = s->stream;
     const int n = (int)s->n;

@@ -157,6 +157,15 @@ struct gm_sys {
     int64_t n = 0;
     double box[3] = {0, 0, 0};
     int periodic = 0;
+    // Span of the fixed-point coordinates per axis.  Periodic systems: the box.
+    // Non-periodic systems: a cube fitted at each list build to the atoms'
+    // extent (a power of two at least 2 (extent + 2 r_skin), when that fits in
+    // the smallest box edge), so the 32-bit step stays fine in a large box.
+    double qbox[3] = {0, 0, 0};
+    bool fit_span = false;   // non-periodic: refit qbox at each list build
+    int ncx_cap = 1;         // largest column count per axis the buffers hold (fitted spans)
+    int box_ncx = 1, box_ncy = 1;  // column counts of a span equal to the box
+    DevBuf<double> ext;      // atoms' bounding box (min xyz, max xyz) and a non-finite flag
     double rc = 0, rsw = 0, rskin = 0, scale14 = 0;
     int bonded_mi = 0;
     gm::NbParams nbp{};
@@ -228,6 +237,8 @@ struct gm_sys {
     int prof_every = 1;
     int64_t last_timed = -1;  // pool index of the last profiled evaluation, -1 if unprofiled
     gm::NbParams nbp_shell{}; // shell-mode constants (set by a surrogate that owns this system)
+    bool has_shell = false;   // nbp_shell in use: rebuilt from nbp whenever the span changes
+    double shell_rsw = 0, shell_w = 0;  // inner switch start and width of the shell mode
     struct gm_dyn *dyn = nullptr;  // masses, constraints and dynamics buffers (gm_dynamics_set)
     std::vector<int> sc_start_host;
     uint32_t *imask_host = nullptr;       // pinned readback of the entries' tile masks (work item costs)
@@ -265,12 +276,21 @@ gm_status compute_on_device(gm_sys *s, const double *x, const double *y, const d
 // Waits for the system's stream and reads the last evaluation's energies.
 gm_status collect(gm_sys *s, gm_energy_terms *e);
 gm_status make_work_items(gm_sys *s);
+// Fixed-point span L (per axis) and xy column counts: every constant that
+// depends on the length of one fixed-point step.
+void set_quant_span(gm_sys *s, const double L[3], int ncx, int ncy);
+// nbp_shell from nbp and the shell's inner switch (shell_rsw, shell_w).
+void refresh_shell(gm_sys *s);
+// cubic fixed-point span (the fast tile variant works in step units)
+inline bool span_is_cubic(const gm_sys *s) { return s->qbox[0] == s->qbox[1] && s->qbox[1] == s->qbox[2]; }
 }  // namespace gm
 
 // ---- kernel launchers (defined in the .hip files) --------------------------
 namespace gm {
 // gm_build.hip
 hipError_t launch_build(gm_sys *s, bool wrap);
+// out[0..2] min x y z, out[3..5] max x y z, out[6] 1 if a coordinate is not finite
+hipError_t launch_extent(gm_sys *s, double *out);
 hipError_t launch_pair_count(gm_sys *s);
 hipError_t launch_pair_fill_and_masks(gm_sys *s);
 hipError_t launch_needs_rebuild(gm_sys *s, const double *x, const double *y, const double *z);

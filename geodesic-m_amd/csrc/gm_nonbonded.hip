@@ -436,7 +436,7 @@ TileArgs tile_args(gm_sys *s) {
     A.n_pad = (int)(s->nsc * 64);
     A.e_wi = s->e_wi.p;
     A.p = s->nbp;
-    A.cubic = (s->box[0] == s->box[1] && s->box[1] == s->box[2]) ? 1 : 0;
+    A.cubic = span_is_cubic(s) ? 1 : 0;
     return A;
 }
 

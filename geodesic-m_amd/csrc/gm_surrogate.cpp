@@ -353,20 +353,12 @@ gm_status gm_surrogate_create(gm_ctx *ctx, const gm_system_desc *exact, const gm
     in.r_skin = d->r_inner_skin;
     rc = system_create_impl(ctx, &in, s->stream, &s->in);
     if (rc != GM_OK) return fail(rc);
-    {
-        // shell mode of the exact system: weight S(r) - S_in(r)
-        NbParams q = s->ex->nbp;
-        const double w_in = d->r_inner_cutoff - d->r_inner_switch;
-        const double step = exact->box[0] / 4294967296.0;
-        q.rin_sw = (float)d->r_inner_switch;
-        q.inv_w_in = (float)(1.0 / w_in);
-        q.ds_coef_in = (float)(-30.0 / w_in);
-        q.rin_sw_u = (float)(d->r_inner_switch / step);
-        q.inv_w_in_u = (float)(step / w_in);
-        q.ds_coef_in_u = (float)(-30.0 * step / w_in);
-        q.nrw_in = (float)(-d->r_inner_switch / w_in);
-        s->ex->nbp_shell = q;
-    }
+    // shell mode of the exact system: weight S(r) - S_in(r), rebuilt with its
+    // step-unit constants whenever the system's fixed-point span changes
+    s->ex->has_shell = true;
+    s->ex->shell_rsw = d->r_inner_switch;
+    s->ex->shell_w = d->r_inner_cutoff - d->r_inner_switch;
+    refresh_shell(s->ex);
     rc = alloc_state(s);
     if (rc != GM_OK) return fail(rc);
     if ((e = s->cls.ensure((size_t)std::max<int64_t>(s->n, 1))) != hipSuccess) return fail(from_hip(e, "hipMalloc"));

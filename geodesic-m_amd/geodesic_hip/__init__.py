@@ -23,10 +23,13 @@ LIB_PATH = os.environ.get("GEODESIC_HIP_LIB") or os.path.join(os.path.dirname(_H
 HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "geodesic_hip.h")
 
 GM_OK = 0
+GM_ERR_INVALID_ARG = 5
+GM_ERR_CONVERGENCE = 9
 STATUS_NAMES = {
     0: "GM_OK", 1: "GM_ERR_DEVICE_LOST", 2: "GM_ERR_KERNEL_LOAD", 3: "GM_ERR_OOM",
     4: "GM_ERR_NO_DEVICE", 5: "GM_ERR_INVALID_ARG", 6: "GM_ERR_KERNEL", 7: "GM_ERR_NOT_BUILT",
-    8: "GM_ERR_CAPACITY",
+    8: "GM_ERR_CAPACITY", 9: "GM_ERR_CONVERGENCE", 10: "GMH_ERR_IO", 11: "GMH_ERR_CONFIG",
+    12: "GMH_ERR_NUMERICAL",
 }
 
 _P = C.c_void_p
@@ -729,18 +732,24 @@ class GpuBackend:
 
     `try_new(atoms, topology, params)` raises GmError with the BackendError
     code on failure, like GpuBackend::try_new.  `state` objects need float64
-    `pos_x/pos_y/pos_z` arrays (SimState, state.rs:3-23).  geodesic_drift is
-    the integrator (outside the force path) and is not provided here."""
+    `pos_x/pos_y/pos_z` (and, for geodesic_drift, `vel_*` and `step`) arrays
+    (SimState, state.rs:3-23).  max_constr_iter / constr_tol are the SimParams
+    fields the constraint solver reads (params.rs:13-14)."""
 
-    def __init__(self, system: System):
+    def __init__(self, system: System, atoms=None, topology=None, max_constr_iter=100, constr_tol=1e-10):
         self._sys = system
         self._reduced = None
         self._potential = 0.0
         self._terms = None
+        self._atoms, self._topo = atoms, topology
+        self._max_iter, self._tol = max_constr_iter, constr_tol
+        self._dyn = False
 
     @classmethod
-    def try_new(cls, atoms, topology, params: Params, ctx: Context | None = None):
-        return cls(System(atoms.sigma, atoms.epsilon, topology, params, ctx))
+    def try_new(cls, atoms, topology, params: Params, ctx: Context | None = None, max_constr_iter=100,
+                constr_tol=1e-10):
+        return cls(System(atoms.sigma, atoms.epsilon, topology, params, ctx), atoms, topology, max_constr_iter,
+                   constr_tol)
 
     def build_neighbor_list(self, state, params=None):
         self._sys.build_neighbor_list(state.pos_x, state.pos_y, state.pos_z)
@@ -770,4 +779,18 @@ class GpuBackend:
         return 1
 
     def geodesic_drift(self, state, dt):
-        raise NotImplementedError("geodesic_drift (integrator + constraints) is outside the GPU force path")
+        """integrator.rs:40-71 (drift, SHAKE, velocity resync) on the device, in
+        place on `state`; GmError with status 9 (ConvergenceError) when the
+        solver does not converge."""
+        if self._atoms is None:
+            raise GmError(GM_ERR_INVALID_ARG, "geodesic_drift needs the backend built by try_new")
+        if not self._dyn:
+            t = self._topo
+            self._sys.dynamics_set(self._atoms.mass, getattr(t, "constr_i", ()), getattr(t, "constr_j", ()),
+                                   getattr(t, "constr_dsq", ()))
+            self._dyn = True
+        names = ("pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z")
+        d = [DeviceArray.from_host(np.ascontiguousarray(getattr(state, k), np.float64)) for k in names]
+        self._sys.drift_and_constrain_device(*d, dt, self._max_iter, self._tol, step=getattr(state, "step", 0))
+        for k, a in zip(names, d):
+            getattr(state, k)[:] = a.to_host()
