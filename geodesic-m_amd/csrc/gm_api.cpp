@@ -594,6 +594,17 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     TRY(upload(s->excl_off, off.data(), off.size(), st));
     TRY(upload(s->excl_nbr, nbr.data(), nbr.size(), st));
     TRY(upload(s->excl_rng, rng.data(), rng.size(), st));
+    // per atom, a bit mask of its partners i+1 .. i+64 (bit k: partner i+1+k)
+    std::vector<uint2> bits((size_t)N, make_uint2(0u, 0u));
+    for (int64_t i = 0; i < n; i++) {
+        uint2 &v = bits[(size_t)i];
+        for (int64_t k = off[(size_t)i]; k < off[(size_t)i + 1]; k++) {
+            const uint64_t d = (uint64_t)nbr[(size_t)k] - (uint64_t)i - 1u;
+            if (d < 32u) v.x |= 1u << d;
+            else if (d < 64u) v.y |= 1u << (d - 32u);
+        }
+    }
+    TRY(upload(s->excl_bits, bits.data(), bits.size(), st));
     std::vector<uint2> rng_any((size_t)N, make_uint2(0xffffffffu, 0u));
     for (uint64_t k : keys) {
         const uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)(k & 0xffffffffu);
@@ -611,6 +622,7 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     for (DevBuf<int> *b : {&s->col_count, &s->col_start, &s->col_sc, &s->col_fill})
         TRY(b->ensure((size_t)ncol_max));
     TRY(s->ext.ensure(8));
+    TRY(s->mask_parts.ensure((size_t)kMaskBlocks * 4 * 4));
     TRY(s->bucket.ensure(N));
     TRY(s->sorted_atoms.ensure(N));
     TRY(s->atom_slot.ensure(N));
@@ -618,6 +630,7 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     TRY(s->meta.ensure(24));
     TRY(s->slot_atom.ensure(slots));
     TRY(s->sq.ensure(slots));
+    TRY(s->sxb.ensure(slots));
     TRY(s->sp.ensure(slots));
     TRY(s->qpos.ensure(slots));
     TRY(s->qeps.ensure(slots));
@@ -660,10 +673,13 @@ gm_status gm_system_destroy(gm_sys *s) {
     for (auto *b : i32) b->release();
     s->excl_off.release();
     s->excl_rng.release();
+    s->excl_bits.release();
+    s->mask_parts.release();
     s->excl_rng_any.release();
     s->atom_par.release();
     s->sp.release();
     s->sq.release();
+    s->sxb.release();
     s->cl_ref.release();
     s->sc_ref.release();
     s->cl_half.release();

@@ -231,8 +231,8 @@ __global__ __launch_bounds__(64) void k_sc_split(const int *meta, int n, const u
 // every slot, padding included, so the per-evaluation prep kernel only needs
 // to overwrite the atoms' slots.
 __global__ void k_slot_setup(int nslots, int n, const int *slot_atom, const uint32_t *q,
-                             const float2 *atom_par, int *atom_slot, int4 *sq, float2 *sp,
-                             int4 *qpos, float *qeps, float inv_s) {
+                             const float2 *atom_par, const uint2 *excl_bits, int *atom_slot, int4 *sq,
+                             float2 *sp, int4 *qpos, float *qeps, uint2 *sxb, float inv_s) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
     int a = slot_atom[s];
@@ -244,11 +244,13 @@ __global__ void k_slot_setup(int nslots, int n, const int *slot_atom, const uint
         sp[s] = p;
         qpos[s] = make_int4(v.x, v.y, v.z, __float_as_int(p.x * inv_s));
         qeps[s] = 2.0f * p.y;
+        sxb[s] = excl_bits[a];
     } else {
         sq[s] = make_int4(0, 0, 0, 0);
         sp[s] = make_float2(0.f, 0.f);
         qpos[s] = make_int4(0, 0, 0, 0);
         qeps[s] = 0.f;
+        sxb[s] = make_uint2(0u, 0u);
     }
 }
 
@@ -388,8 +390,8 @@ __device__ inline bool excluded(const int64_t *off, const uint32_t *nbr, uint32_
     return a < off[lo + 1] && nbr[a] == hi;
 }
 
-// One wavefront per entry (grid-stride over entries, four wavefronts per
-// workgroup): exact pair masks.  Lane = (jl = lane>>3, il = lane&7).
+// One wavefront per entry (a strided walk over an XCD's range of entries,
+// four wavefronts per workgroup): exact pair masks.  Lane = (jl = lane>>3, il = lane&7).
 // The inclusion test is neighbor.rs:31-38: min-image r2 <= r_skin^2 in f64 on
 // the wrapped positions, with i = min(atom), j = max(atom), then the exclusion
 // set (neighbor.rs:33-35).  r2 is first estimated in f32 from the exact
@@ -397,7 +399,8 @@ __device__ inline bool excluded(const int64_t *off, const uint32_t *nbr, uint32_
 // band of a threshold is decided by the f64 expression itself, and the band
 // is many times the estimate's worst error, so every decision is the f64 one.
 // The exclusion lookup runs only for pairs inside r_skin, and only when the
-// partner lies in the atom's (smallest, largest) exclusion-partner range.
+// a bit of the atom's mask of partners lo+1..lo+64, or for a partner farther
+// up, a binary search when it lies in the atom's exclusion-partner range.
 // Per tile, the byte floor((d_lo - r_cutoff) / step) (0 if negative, 255 for
 // a tile without listed pairs) goes to ent_dmin, d_lo a lower bound on the
 // distance of the tile's closest listed pair (the f32 estimate less a
@@ -408,88 +411,152 @@ __global__ __launch_bounds__(256) void k_masks(int nent, const int *ent_sc, cons
                                                const int *slot_atom, const int4 *sq, const double *x,
                                                const double *y, const double *z,
                                                const int64_t *excl_off, const uint32_t *excl_nbr,
-                                               const uint2 *excl_rng, BuildParams bp,
-                                               unsigned long long *counters) {
+                                               const uint2 *excl_rng, const uint2 *sxb,
+                                               BuildParams bp,
+                                               unsigned long long *parts) {
     const int t = threadIdx.x & 63;
     const int il = t & 7, jl = t >> 3;
-    const int nwaves = (int)gridDim.x * 4;
+    // Workgroups are dealt to the 8 XCDs in turn (workgroup g runs on XCD
+    // g % 8); group xg of the P = min(8, grid) residues walks the contiguous
+    // entry range [xg n / P, (xg + 1) n / P), so an XCD's L2 holds the slot data
+    // of about an eighth of the system.
+    const int P = (int)min(gridDim.x, 8u);
+    const int xg = (int)(blockIdx.x % (unsigned)P);
+    const int nwx = (int)((gridDim.x - (unsigned)xg + (unsigned)P - 1u) / (unsigned)P) * 4;  // wavefronts of group xg
+    const int l = (int)(blockIdx.x / (unsigned)P) * 4 + (int)(threadIdx.x >> 6);
+    const int e_lo = (int)((int64_t)nent * xg / P), e_hi = (int)((int64_t)nent * (xg + 1) / P);
     unsigned long long tot_pairs = 0, tot_tiles = 0, tot_ctiles = 0, tot_cpairs = 0;
-    for (int e = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); e < nent; e += nwaves) {
+    for (int e = e_lo + l; e < e_hi; e += nwx) {
         const int sc = ent_sc[e], cj = ent_cj[e];
         const uint32_t im = ent_imask[e];
         const size_t js = (size_t)cj * 8 + jl;
         const int b = slot_atom[js];
         const int4 qb = sq[js];
-        int a[8];
-        int4 qa[8];
+        const uint2 xbj = sxb[js];
+        // per-cluster values in arrays of scalars (no vector structs), which
+        // stay in registers once the loops are unrolled; the exclusion masks
+        // of the slots' atoms arrive with their coordinates
+        int a[8], qax[8], qay[8], qaz[8];
+        uint32_t exl[8], exh[8];
 #pragma unroll
         for (int ci = 0; ci < 8; ci++) {
             a[ci] = -1;
-            qa[ci] = make_int4(0, 0, 0, 0);
+            qax[ci] = qay[ci] = qaz[ci] = 0;
+            exl[ci] = exh[ci] = 0u;
             if (im & (1u << ci)) {
                 const size_t is = (size_t)sc * 64 + ci * 8 + il;
                 a[ci] = slot_atom[is];
-                qa[ci] = sq[is];
+                const int4 q = sq[is];
+                qax[ci] = q.x;
+                qay[ci] = q.y;
+                qaz[ci] = q.z;
+                const uint2 xb = sxb[is];
+                exl[ci] = xb.x;
+                exh[ci] = xb.y;
             }
         }
-        uint32_t inb = 0, cutb = 0;
+        // exclusions, reduced at once to one bit per tile (the masks' registers
+        // are then free): a partner within 64 indices is a bit of the lower
+        // atom's mask; a farther one (far) is looked up later if listed
+        uint32_t xhit = 0, far = 0;
+#pragma unroll
+        for (int ci = 0; ci < 8; ci++) {
+            if (a[ci] >= 0 && b >= 0) {
+                const uint32_t lo = (uint32_t)min(a[ci], b), hi = (uint32_t)max(a[ci], b);
+                const uint32_t k = hi - lo - 1u;
+                const uint32_t w = k < 32u ? (a[ci] < b ? exl[ci] : xbj.x) : (a[ci] < b ? exh[ci] : xbj.y);
+                if (k >= 64u) far |= 1u << ci;
+                else if (((w >> (k & 31u)) & 1u) != 0u) xhit |= 1u << ci;
+            }
+        }
+        uint32_t inb = 0, cutb = 0, band = 0;
         float r2v[8];
 #pragma unroll
         for (int ci = 0; ci < 8; ci++) {
             const int cg = sc * 8 + ci;
             r2v[ci] = 0.f;
             if (a[ci] >= 0 && b >= 0 && (cg < cj || il < jl)) {
-                const float dx = (float)wrapdiff(qb.x, qa[ci].x) * bp.sq[0];
-                const float dy = (float)wrapdiff(qb.y, qa[ci].y) * bp.sq[1];
-                const float dz = (float)wrapdiff(qb.z, qa[ci].z) * bp.sq[2];
+                const float dx = (float)wrapdiff(qb.x, qax[ci]) * bp.sq[0];
+                const float dy = (float)wrapdiff(qb.y, qay[ci]) * bp.sq[1];
+                const float dz = (float)wrapdiff(qb.z, qaz[ci]) * bp.sq[2];
                 const float r2f = dx * dx + dy * dy + dz * dz;
                 r2v[ci] = r2f;
-                bool in = r2f <= bp.r_skin2f, cut = r2f <= bp.r_cut2f;
-                if (fabsf(r2f - bp.r_skin2f) <= bp.band_skin || fabsf(r2f - bp.r_cut2f) <= bp.band_cut) {
-                    const uint32_t lo = (uint32_t)min(a[ci], b), hi = (uint32_t)max(a[ci], b);
-                    const double ddx = min_image(x[hi] - x[lo], bp.box[0]);
-                    const double ddy = min_image(y[hi] - y[lo], bp.box[1]);
-                    const double ddz = min_image(z[hi] - z[lo], bp.box[2]);
-                    const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    in = r2 <= bp.r_skin2;
-                    cut = r2 <= bp.r_cut2;
-                }
+                const bool in = r2f <= bp.r_skin2f, cut = r2f <= bp.r_cut2f;
+                if (fabsf(r2f - bp.r_skin2f) <= bp.band_skin || fabsf(r2f - bp.r_cut2f) <= bp.band_cut)
+                    band |= 1u << ci;
                 inb |= in ? (1u << ci) : 0u;
                 cutb |= (in && cut) ? (1u << ci) : 0u;
             }
         }
-        for (int ci = 0; ci < 8; ci++) {
-            if (inb & (1u << ci)) {
-                const uint32_t lo = (uint32_t)min(a[ci], b), hi = (uint32_t)max(a[ci], b);
-                const uint2 rg = excl_rng[lo];
-                if (hi >= rg.x && hi <= rg.y && excluded(excl_off, excl_nbr, lo, hi)) {
-                    inb &= ~(1u << ci);
-                    cutb &= ~(1u << ci);
-                }
+        // pairs near a threshold (rare): the f64 expression decides, one at a
+        // time, outside the unrolled loop
+        while (band) {
+            const int ci = __ffs(band) - 1;
+            band &= band - 1u;
+            int aci = a[0];
+#pragma unroll
+            for (int c = 1; c < 8; c++) aci = ci == c ? a[c] : aci;
+            const uint32_t lo = (uint32_t)min(aci, b), hi = (uint32_t)max(aci, b);
+            const double ddx = min_image(x[hi] - x[lo], bp.box[0]);
+            const double ddy = min_image(y[hi] - y[lo], bp.box[1]);
+            const double ddz = min_image(z[hi] - z[lo], bp.box[2]);
+            const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            const bool in = r2 <= bp.r_skin2, cut = r2 <= bp.r_cut2;
+            inb = (inb & ~(1u << ci)) | (in ? (1u << ci) : 0u);
+            cutb = (cutb & ~(1u << ci)) | ((in && cut) ? (1u << ci) : 0u);
+        }
+        inb &= ~xhit;
+        cutb &= ~xhit;
+        far &= inb;
+        while (far) {
+            // the register arrays are read through selects, never by a variable index
+            const int ci = __ffs(far) - 1;
+            far &= far - 1u;
+            int aci = a[0];
+#pragma unroll
+            for (int c = 1; c < 8; c++) aci = ci == c ? a[c] : aci;
+            const uint32_t lo = (uint32_t)min(aci, b), hi = (uint32_t)max(aci, b);
+            const uint2 rg = excl_rng[lo];
+            if (hi <= rg.y && excluded(excl_off, excl_nbr, lo, hi)) {
+                inb &= ~(1u << ci);
+                cutb &= ~(1u << ci);
             }
         }
         uint32_t newmask = 0;
         unsigned long long npairs = 0, ncut_tiles = 0, ncut_pairs = 0;
         uint32_t dlo = 0, dhi = 0;
+        // lane ci collects tile ci's pair mask: one 64-byte store per entry (a
+        // store per tile would wait for the previous one to free its registers)
+        unsigned long long mlane = 0;
+        // lane ci also collects the smallest r2 estimate of tile ci's listed
+        // pairs (f32 bits of non-negative values order as integers; +inf for
+        // a tile without one); the distance byte is a non-decreasing function
+        // of r2, so converting the minimum gives the minimum byte
+        int r2min = 0x7f800000;
 #pragma unroll
         for (int ci = 0; ci < 8; ci++) {
             const unsigned long long m = __ballot((inb >> ci) & 1u);
             const unsigned long long mc = __ballot((cutb >> ci) & 1u);
-            if (t == 0) ent_mask[(size_t)e * 8 + ci] = m;
+            mlane = t == ci ? m : mlane;
             if (m) { newmask |= 1u << ci; npairs += __popcll(m); }
             if (mc) ncut_tiles++;
             ncut_pairs += __popcll(mc);
-            int db = 255;
-            if ((inb >> ci) & 1u) {
-                const float dl = sqrtf(r2v[ci]) * (1.0f - 2e-6f) - 1e-5f;
-                const float st = floorf((dl - bp.prune_rc) * bp.prune_inv_step);
-                db = st <= 0.f ? 0 : (st >= 255.f ? 255 : (int)st);
-            }
-            db = wave_min_i32(db);
-            if (ci < 4) dlo |= (uint32_t)db << (8 * ci);
-            else dhi |= (uint32_t)db << (8 * (ci - 4));
+            const int rb = wave_min_i32(((inb >> ci) & 1u) ? __float_as_int(r2v[ci]) : 0x7f800000);
+            r2min = t == ci ? rb : r2min;
+        }
+        int db = 255;
+        if (r2min != 0x7f800000) {
+            const float dl = sqrtf(__int_as_float(r2min)) * (1.0f - 2e-6f) - 1e-5f;
+            const float st = floorf((dl - bp.prune_rc) * bp.prune_inv_step);
+            db = st <= 0.f ? 0 : (st >= 255.f ? 255 : (int)st);
+        }
+#pragma unroll
+        for (int ci = 0; ci < 4; ci++) {
+            dlo |= (uint32_t)__builtin_amdgcn_readlane(db, ci) << (8 * ci);
+            dhi |= (uint32_t)__builtin_amdgcn_readlane(db, ci + 4) << (8 * ci);
         }
         ent_lb[(size_t)e * 64 + t] = (uint8_t)inb;
+        if (t < 8) ent_mask[(size_t)e * 8 + t] = mlane;
         if (t == 0) {
             ent_imask[e] = newmask;
             ent_meta[e] = make_int2(cj, (int)newmask);
@@ -500,13 +567,33 @@ __global__ __launch_bounds__(256) void k_masks(int nent, const int *ent_sc, cons
         tot_ctiles += ncut_tiles;
         tot_cpairs += ncut_pairs;
     }
-    // one set of counter atomics per wavefront, not per entry
-    if (t == 0 && tot_tiles) {
-        atomicAdd(&counters[0], tot_pairs);
-        atomicAdd(&counters[1], tot_tiles);
-        atomicAdd(&counters[2], tot_ctiles);
-        atomicAdd(&counters[3], tot_cpairs);
+    // per-wavefront partial counts, summed by k_sum_parts: same-address
+    // atomics from every wavefront would serialise (about 10 ns each)
+    if (t == 0) {
+        unsigned long long *pw = parts + ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 4;
+        pw[0] = tot_pairs;
+        pw[1] = tot_tiles;
+        pw[2] = tot_ctiles;
+        pw[3] = tot_cpairs;
     }
+}
+
+// out[c] = sum over the n partial records of parts[4 k + c], in a fixed order.
+__global__ __launch_bounds__(1024) void k_sum_parts(const unsigned long long *parts, int n,
+                                                   unsigned long long *out) {
+    __shared__ unsigned long long sm[4][1024];
+    const int t = threadIdx.x;
+    unsigned long long a[4] = {0, 0, 0, 0};
+    for (int k = t; k < n; k += 1024)
+        for (int c = 0; c < 4; c++) a[c] += parts[(size_t)k * 4 + c];
+    for (int c = 0; c < 4; c++) sm[c][t] = a[c];
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (t < off)
+            for (int c = 0; c < 4; c++) sm[c][t] += sm[c][t + off];
+        __syncthreads();
+    }
+    if (t < 4) out[t] = sm[t][0];
 }
 
 // neighbor.rs:60-72, same f64 expression; any thread past the threshold sets the flag.
@@ -612,8 +699,8 @@ hipError_t launch_build(gm_sys *s, bool wrap) {
     k_sc_split<<<(unsigned)s->max_sc, 64, 0, st>>>(s->meta.p, n, s->qatom.p, s->sorted_atoms.p,
                                                    s->sc_base.p, s->sc_m.p, s->slot_atom.p);
     k_slot_setup<<<grid_for((size_t)s->max_sc * 64, 256), 256, 0, st>>>(
-        (int)(s->max_sc * 64), n, s->slot_atom.p, s->qatom.p, s->atom_par.p, s->atom_slot.p,
-        s->sq.p, s->sp.p, s->qpos.p, s->qeps.p, kernel_sigma_scale(s));
+        (int)(s->max_sc * 64), n, s->slot_atom.p, s->qatom.p, s->atom_par.p, s->excl_bits.p,
+        s->atom_slot.p, s->sq.p, s->sp.p, s->qpos.p, s->qeps.p, s->sxb.p, kernel_sigma_scale(s));
     k_bbox<<<(unsigned)s->max_sc, 64, 0, st>>>(s->meta.p, s->slot_atom.p, s->sq.p, bp, s->cl_ref.p,
                                                s->cl_half.p, s->sc_ref.p, s->sc_half.p);
     return hipGetLastError();
@@ -634,15 +721,15 @@ hipError_t launch_pair_fill_and_masks(gm_sys *s) {
             s->meta.p, s->bp, s->sc_col.p, s->col_sc.p, s->col_count.p, s->cl_ref.p, s->cl_half.p,
             s->sc_ref.p, s->sc_half.p, nullptr, s->sc_start.p, s->ent_sc.p, s->ent_cj.p,
             s->ent_imask.p);
-    hipError_t e = launch_fill_u64((unsigned long long *)(s->meta.p + 8), 4, st);
-    if (e != hipSuccess) return e;
-    if (s->nent > 0)
-        k_masks<<<(unsigned)std::min<int64_t>((s->nent + 3) / 4, 2048), 256, 0, st>>>((int)s->nent, s->ent_sc.p, s->ent_cj.p,
-                                                  s->ent_imask.p, s->ent_mask.p, s->ent_meta.p,
-                                                  s->ent_dmin.p, s->ent_lb.p, s->slot_atom.p,
-                                                  s->sq.p, s->x.p, s->y.p, s->z.p, s->excl_off.p,
-                                                  s->excl_nbr.p, s->excl_rng.p, s->bp,
-                                                  (unsigned long long *)(s->meta.p + 8));
+    unsigned long long *counters = (unsigned long long *)(s->meta.p + 8);
+    if (s->nent == 0) return launch_fill_u64(counters, 4, st);
+    const int blocks = (int)std::min<int64_t>((s->nent + 3) / 4, kMaskBlocks);
+    k_masks<<<(unsigned)blocks, 256, 0, st>>>((int)s->nent, s->ent_sc.p, s->ent_cj.p, s->ent_imask.p,
+                                              s->ent_mask.p, s->ent_meta.p, s->ent_dmin.p, s->ent_lb.p,
+                                              s->slot_atom.p, s->sq.p, s->x.p, s->y.p, s->z.p,
+                                              s->excl_off.p, s->excl_nbr.p, s->excl_rng.p, s->sxb.p,
+                                              s->bp, s->mask_parts.p);
+    k_sum_parts<<<1, 1024, 0, st>>>(s->mask_parts.p, blocks * 4, counters);
     return hipGetLastError();
 }
 

@@ -22,6 +22,8 @@ constexpr int kScSlots = 64;
 constexpr int kColumnCap = 8192;
 // Longest run of list entries one nonbonded wavefront processes.
 constexpr int kMaxRun = 32;
+// list build: workgroups (of four wavefronts) of the pair-mask kernel, at most
+constexpr int kMaskBlocks = 2048;
 // Fixed-point force unit: 2^-32 kcal/mol/A per integer step.
 constexpr double kFixedScale = 4294967296.0;
 constexpr double kFixedInv = 1.0 / 4294967296.0;
@@ -180,6 +182,8 @@ struct gm_sys {
     DevBuf<int64_t> excl_off;
     DevBuf<uint32_t> excl_nbr;
     DevBuf<uint2> excl_rng;  // per atom (smallest, largest) exclusion partner above it; (1, 0) if none
+    DevBuf<uint2> excl_bits; // per atom, mask of exclusion partners i+1..i+64 (bit k: partner i+1+k)
+    DevBuf<unsigned long long> mask_parts;  // list statistics, 4 counts per pair-mask wavefront
     DevBuf<uint2> excl_rng_any;  // per atom (smallest, largest) exclusion partner, either side; (1, 0) if none
 
     // working positions (device, original order) and rebuild reference
@@ -199,6 +203,7 @@ struct gm_sys {
     // slots
     DevBuf<int> slot_atom, atom_slot;
     DevBuf<int4> sq;         // fixed-point coords per slot (x, y, z, 0) at the list build
+    DevBuf<uint2> sxb;       // per slot, the exclusion mask (excl_bits) of its atom; 0 for padding
     DevBuf<float2> sp;       // (sigma/2, sqrt eps) per slot
     DevBuf<int4> qpos;       // per evaluation: fixed-point coords per slot + bits of sigma/2
     DevBuf<float> qeps;      // 2 sqrt(eps) per slot

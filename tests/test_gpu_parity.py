@@ -247,6 +247,37 @@ def test_small_periodic_boxes_use_the_single_nearest_image(n):
     check_forces(fg, f, f"n={n}", synthetic=True)
 
 
+def test_exclusions_at_every_index_gap():
+    """Exclusion partners near in index (a bit of the atom's mask of partners
+    i+1..i+64) and far (a binary search), including the gaps 63, 64 and 65
+    on either side of the mask's end: the list equals the oracle's."""
+    say("[gpu] exclusions at near and far index gaps")
+    s = systems.protein_solvent(2000, (2000 / 0.1) ** (1 / 3), 250, seed=5)
+    x, y, z = s.x.copy(), s.y.copy(), s.z.copy()
+    orc.wrap_positions(x, y, z, s.params.box)
+    pi, pj = orc.neighbor_build(x, y, z, s.params.box, s.params.r_skin, s.topo.excl_i, s.topo.excl_j,
+                                method="brute")
+    gap = pj.astype(np.int64) - pi.astype(np.int64)
+    rng = np.random.default_rng(17)
+    pick = []
+    for g0, g1 in ((63, 64), (64, 65), (65, 66), (1, 63), (66, 2000)):
+        idx = np.nonzero((gap >= g0) & (gap < g1))[0]
+        pick.append(rng.choice(idx, size=min(len(idx), 300), replace=False))
+    pick = np.concatenate(pick)
+    assert len(pick) > 600 and {63, 64, 65} <= set(gap[pick].tolist())
+    topo = s.topo
+    topo.excl_i = np.concatenate([topo.excl_i, pj[pick]]).astype(np.uint32)  # either order
+    topo.excl_j = np.concatenate([topo.excl_j, pi[pick]]).astype(np.uint32)
+    (wx, wy, wz), (ri, rj), t, f = oracle_reference(s.x, s.y, s.z, s.sigma, s.epsilon, topo, s.params,
+                                                    method="brute")
+    assert len(ri) == len(pi) - len(pick)
+    g, _, e, fg = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, topo, s.params)
+    qi, qj = g.pair_list()
+    assert np.array_equal(qi, ri) and np.array_equal(qj, rj)
+    check_terms(e, t, "exclusion gaps")
+    check_forces(fg, f, "exclusion gaps", synthetic=True)
+
+
 def test_empty_and_single_atom_systems():
     say("[gpu] empty / single")
     z32, zf = np.zeros(0, np.uint32), np.zeros(0)
