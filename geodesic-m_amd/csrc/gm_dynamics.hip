@@ -55,21 +55,35 @@ __global__ void k_resync(int64_t n, const double *x, const double *y, const doub
     vz[i] = (z[i] - rz[i]) / dt;
 }
 
-__device__ inline void atomic_max_abs(unsigned long long *slot, double v) {
-    // non-negative doubles order like their bit patterns
-    __hip_atomic_fetch_max(slot, (unsigned long long)__double_as_longlong(fabs(v)), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+// |v| as the bit pattern that orders like the value (a NaN above everything,
+// so it reaches the convergence test as before).
+__device__ inline unsigned long long abs_bits(double v) {
+    return (unsigned long long)__double_as_longlong(fabs(v));
+}
+
+// The largest of the wavefront's abs_bits values into *slot: one atomic per
+// wavefront (same-address atomics from every thread would serialise).  Every
+// lane calls it.
+__device__ inline void wave_atomic_max_bits(unsigned long long *slot, unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long w = __shfl_xor(v, o);
+        v = w > v ? w : v;
+    }
+    if ((threadIdx.x & 63) == 0 && v != 0ull)
+        __hip_atomic_fetch_max(slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // One SHAKE iteration (constraint.rs:97-150) from positions `in` to `out`.
 // Skipped entirely once an earlier iteration has converged (flag[0] != 0).
 __global__ void k_shake_iter(DynArgs D, Vec3 in, Vec3 out, Vec3C ref, unsigned long long *maxslot,
                              double *lambda, const int *flag) {
-    if (flag[0]) return;
+    if (flag[0]) return;  // the same for every thread
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= D.n) return;
+    const bool act = a < D.n;
     double ddx = 0.0, ddy = 0.0, ddz = 0.0;
-    for (int64_t k = D.inc_off[a]; k < D.inc_off[a + 1]; k++) {
+    unsigned long long mx = 0ull;
+    const int64_t k0 = act ? D.inc_off[a] : 0, k1 = act ? D.inc_off[a + 1] : 0;
+    for (int64_t k = k0; k < k1; k++) {
         const int64_t c = D.inc_con[k];
         const uint32_t i = D.ci[c], j = D.cj[c];
         const double dx = in.x[i] - in.x[j];
@@ -89,13 +103,15 @@ __global__ void k_shake_iter(DynArgs D, Vec3 in, Vec3 out, Vec3C ref, unsigned l
             ddy -= lam * inv_mi * ry;
             ddz -= lam * inv_mi * rz;
             lambda[c] = lam;
-            atomic_max_abs(maxslot, lam);
+            mx = max(mx, abs_bits(lam));
         } else {
             ddx += lam * inv_mj * rx;
             ddy += lam * inv_mj * ry;
             ddz += lam * inv_mj * rz;
         }
     }
+    wave_atomic_max_bits(maxslot, mx);
+    if (!act) return;
     out.x[a] = in.x[a] + ddx;
     out.y[a] = in.y[a] + ddy;
     out.z[a] = in.z[a] + ddz;
@@ -104,11 +120,13 @@ __global__ void k_shake_iter(DynArgs D, Vec3 in, Vec3 out, Vec3C ref, unsigned l
 // One RATTLE velocity iteration (constraint.rs:196-250) from velocities `in` to `out`.
 __global__ void k_rattle_iter(DynArgs D, Vec3C pos, Vec3 in, Vec3 out, unsigned long long *maxslot,
                               double *lambda, const int *flag) {
-    if (flag[0]) return;
+    if (flag[0]) return;  // the same for every thread
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= D.n) return;
+    const bool act = a < D.n;
     double ddx = 0.0, ddy = 0.0, ddz = 0.0;
-    for (int64_t k = D.inc_off[a]; k < D.inc_off[a + 1]; k++) {
+    unsigned long long mx = 0ull;
+    const int64_t k0 = act ? D.inc_off[a] : 0, k1 = act ? D.inc_off[a + 1] : 0;
+    for (int64_t k = k0; k < k1; k++) {
         const int64_t c = D.inc_con[k];
         const uint32_t i = D.ci[c], j = D.cj[c];
         const double rx = pos.x[i] - pos.x[j];
@@ -128,13 +146,15 @@ __global__ void k_rattle_iter(DynArgs D, Vec3C pos, Vec3 in, Vec3 out, unsigned 
             ddy -= mu * inv_mi * ry;
             ddz -= mu * inv_mi * rz;
             lambda[c] = mu;
-            atomic_max_abs(maxslot, mu);
+            mx = max(mx, abs_bits(mu));
         } else {
             ddx += mu * inv_mj * rx;
             ddy += mu * inv_mj * ry;
             ddz += mu * inv_mj * rz;
         }
     }
+    wave_atomic_max_bits(maxslot, mx);
+    if (!act) return;
     out.x[a] = in.x[a] + ddx;
     out.y[a] = in.y[a] + ddy;
     out.z[a] = in.z[a] + ddz;
