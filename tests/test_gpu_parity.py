@@ -278,6 +278,78 @@ def test_exclusions_at_every_index_gap():
     check_forces(fg, f, "exclusion gaps", synthetic=True)
 
 
+def _vacuum(s, shift, box=1000.0):
+    """The synthetic system moved into a large non-periodic box."""
+    x, y, z = (v + d for v, d in zip((s.x, s.y, s.z), shift))
+    p = gh.Params(box=(box,) * 3, periodic=False, r_cutoff=s.params.r_cutoff, r_switch=s.params.r_switch,
+                  r_skin=s.params.r_skin)
+    return x, y, z, p
+
+
+def test_non_periodic_system_in_a_large_box():
+    """A dense 6,000-atom cluster in a 1000 A non-periodic box (partly at
+    negative coordinates): the fixed-point span is fitted to the atoms at each
+    build.  List, energies and forces against the oracle, reruns byte-identical,
+    and a stale list after displacements up to 0.35 A against the oracle on
+    the same list."""
+    say("[gpu] non-periodic 6000-atom cluster, 1000 A box")
+    s = systems.protein_solvent(6000, (6000 / 0.1) ** (1 / 3), 60, seed=21)
+    x, y, z, p = _vacuum(s, (-20.0, 350.0, -480.0))
+    (wx, wy, wz), (pi, pj), t, f = oracle_reference(x, y, z, s.sigma, s.epsilon, s.topo, p, method="brute")
+    g, (gx, gy, gz), e, fg = run_gpu(x, y, z, s.sigma, s.epsilon, s.topo, p)
+    assert gx.tobytes() == x.tobytes()  # not wrapped
+    qi, qj = g.pair_list()
+    assert np.array_equal(qi, pi) and np.array_equal(qj, pj)
+    check_terms(e, t, "vacuum cluster")
+    check_forces(fg, f, "vacuum cluster", synthetic=True)
+    e2, *f2 = g.compute_forces(x, y, z)
+    assert all(u.tobytes() == v.tobytes() for u, v in zip(fg, f2)) and e2.total == e.total
+    rng = np.random.default_rng(4)
+    d = [rng.uniform(-0.2, 0.2, len(x)) for _ in range(3)]
+    x2, y2, z2 = x + d[0], y + d[1], z + d[2]
+    assert not g.needs_rebuild(x2, y2, z2)
+    e3, *f3 = g.compute_forces(x2, y2, z2)
+    t3, *r3 = orc.cpu_backend_compute(x2, y2, z2, s.sigma, s.epsilon, s.topo, pi, pj, p.box, p.r_cutoff,
+                                      p.r_switch, n_threads=8, periodic=False)
+    check_terms(e3, t3, "vacuum cluster, stale list")
+    check_forces(f3, r3, "vacuum cluster, stale list", synthetic=True)
+
+
+def test_non_periodic_atoms_spread_over_the_box():
+    """Two clusters 600 A apart in a 1000 A non-periodic box: no cube of twice
+    the extent fits, so the span is the box; the reference's min image with
+    the box then applies to every pair, as in the oracle."""
+    say("[gpu] non-periodic, two clusters 600 A apart")
+    a = systems.protein_solvent(1000, (1000 / 0.1) ** (1 / 3), 100, seed=8)
+    b = systems.protein_solvent(1000, (1000 / 0.1) ** (1 / 3), 100, seed=9)
+    n = 1000
+    x = np.concatenate([a.x + 10.0, b.x + 610.0])
+    y = np.concatenate([a.y - 5.0, b.y + 20.0])
+    z = np.concatenate([a.z, b.z + 3.0])
+    sig = np.concatenate([a.sigma, b.sigma])
+    eps = np.concatenate([a.epsilon, b.epsilon])
+    ta, tb = a.topo, b.topo
+    cat = lambda u, v: np.concatenate([u, v + n]).astype(np.uint32)  # noqa: E731
+    topo = systems.Topology(cat(ta.bond_i, tb.bond_i), cat(ta.bond_j, tb.bond_j),
+                            np.concatenate([ta.bond_k, tb.bond_k]), np.concatenate([ta.bond_r0, tb.bond_r0]),
+                            cat(ta.angle_i, tb.angle_i), cat(ta.angle_j, tb.angle_j), cat(ta.angle_k, tb.angle_k),
+                            np.concatenate([ta.angle_kth, tb.angle_kth]),
+                            np.concatenate([ta.angle_th0, tb.angle_th0]),
+                            cat(ta.dihed_i, tb.dihed_i), cat(ta.dihed_j, tb.dihed_j), cat(ta.dihed_k, tb.dihed_k),
+                            cat(ta.dihed_l, tb.dihed_l), np.concatenate([ta.dihed_kphi, tb.dihed_kphi]),
+                            np.concatenate([ta.dihed_n, tb.dihed_n]).astype(np.uint32),
+                            np.concatenate([ta.dihed_delta, tb.dihed_delta]),
+                            cat(ta.excl_i, tb.excl_i), cat(ta.excl_j, tb.excl_j))
+    p = gh.Params(box=(1000.0,) * 3, periodic=False, r_cutoff=9.0, r_switch=7.5, r_skin=10.5)
+    (wx, wy, wz), (pi, pj), t, f = oracle_reference(x, y, z, sig, eps, topo, p, method="brute")
+    g, _, e, fg = run_gpu(x, y, z, sig, eps, topo, p)
+    qi, qj = g.pair_list()
+    assert np.array_equal(qi, pi) and np.array_equal(qj, pj)
+    assert np.all((pi < n) == (pj < n))  # no pair across the clusters
+    check_terms(e, t, "two clusters")
+    check_forces(fg, f, "two clusters", synthetic=True)
+
+
 def test_empty_and_single_atom_systems():
     say("[gpu] empty / single")
     z32, zf = np.zeros(0, np.uint32), np.zeros(0)
