@@ -105,18 +105,15 @@ gm_status make_work_items(gm_sys *s) {
     for (const Item &it : items) first[(size_t)(cmax - it.cost) + 1]++;
     for (size_t c = 1; c < first.size(); c++) first[c] += first[c - 1];
     const size_t ni = items.size();
-    std::vector<int> wsc(ni), wst(ni), wen(ni);
+    std::vector<int4> wi(ni);
     for (const Item &it : items) {
         const size_t o = (size_t)first[(size_t)(cmax - it.cost)]++;
-        wsc[o] = it.sc;
-        wst[o] = it.e0;
-        wen[o] = it.e1;
+        wi[o] = make_int4(it.sc, it.e0, it.e1, 0);
     }
-    s->n_wi = (int64_t)wsc.size();
+    s->n_wi = (int64_t)wi.size();
     hipStream_t st = s->stream;
-    GM_HIP(upload(s->wi_sc, wsc.data(), wsc.size(), st));
-    GM_HIP(upload(s->wi_start, wst.data(), wst.size(), st));
-    GM_HIP(upload(s->wi_end, wen.data(), wen.size(), st));
+    GM_HIP(upload(s->wi, wi.data(), wi.size(), st));
+    GM_HIP(launch_work_item_cj0(s));  // .w = ent_cj[first entry], from the device list
     GM_HIP(s->e_wi.ensure((size_t)std::max<int64_t>(s->n_wi, 1)));
     GM_HIP(hipStreamSynchronize(st));
     s->stats.n_work_items = s->n_wi;
@@ -668,8 +665,7 @@ gm_status gm_system_destroy(gm_sys *s) {
     for (auto *b : u32) b->release();
     DevBuf<int> *i32[] = {&s->col_count, &s->col_start, &s->col_sc, &s->col_fill, &s->bucket,
                           &s->sorted_atoms, &s->sc_base, &s->sc_m, &s->sc_col, &s->meta, &s->slot_atom,
-                          &s->atom_slot, &s->sc_count, &s->sc_start, &s->ent_sc, &s->ent_cj,
-                          &s->wi_sc, &s->wi_start, &s->wi_end};
+                          &s->atom_slot, &s->sc_count, &s->sc_start, &s->ent_sc, &s->ent_cj};
     for (auto *b : i32) b->release();
     s->excl_off.release();
     s->excl_rng.release();
@@ -680,6 +676,7 @@ gm_status gm_system_destroy(gm_sys *s) {
     s->sp.release();
     s->sq.release();
     s->sxb.release();
+    s->wi.release();
     s->cl_ref.release();
     s->sc_ref.release();
     s->cl_half.release();

@@ -225,7 +225,7 @@ struct gm_sys {
     DevBuf<uint2> ent_dmin;               // per entry, per tile: closest listed pair byte
     DevBuf<uint8_t> ent_lb;               // per entry, per lane: mask bits over the 8 i clusters
     bool F_dirty = true;                  // accumulator not known to be zero
-    DevBuf<int> wi_sc, wi_start, wi_end;
+    DevBuf<int4> wi;          // work items (supercluster, first entry, end entry, first j-cluster)
     int chunk = 0;  // entries per work item (0 = automatic)
 
     // accumulators
@@ -303,6 +303,8 @@ hipError_t launch_needs_rebuild(gm_sys *s, const double *x, const double *y, con
 // (shell: tiles only, with nbp_shell)
 // (after launch_prep: slot-order fixed-point positions and the displacement bound)
 hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double *z);
+// fills the work items' first j-cluster from the list entries
+hipError_t launch_work_item_cj0(gm_sys *s);
 hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z, bool shell = false);
 // gm_bonded.hip: energy reduction, f64 output, accumulator clear (lj_only:
 // the bonded energy segments are empty, as after a shell-mode launch)

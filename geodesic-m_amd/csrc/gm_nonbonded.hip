@@ -78,9 +78,28 @@ __device__ inline float in_vgpr(float v) {
     return r;
 }
 
+template <int CTRL>
+__device__ inline float dpp_max(float v) {
+    return fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false)));
+}
+
+// Largest value of a row of 16 lanes, in every lane of the row (non-negative
+// values; every lane active).
+__device__ inline float row_max16(float v) {
+    v = dpp_max<0xB1>(v);      // quad_perm [1,0,3,2]
+    v = dpp_max<0x4E>(v);      // quad_perm [2,3,0,1]
+    v = dpp_max<0x141>(v);     // row_half_mirror
+    return dpp_max<0x128>(v);  // row_ror:8
+}
+
+// Largest value of the wavefront, in every lane (non-negative values; every
+// lane active): rows by DPP, then across rows by the gfx950 permlane swaps.
 __device__ inline float wave_max(float v) {
-    for (int o = 1; o < 64; o <<= 1) v = fmaxf(v, __shfl_xor(v, o));
-    return v;
+    v = row_max16(v);
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    v = fmaxf(__uint_as_float(r[0]), __uint_as_float(r[1]));
+    const auto q = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return fmaxf(__uint_as_float(q[0]), __uint_as_float(q[1]));
 }
 
 // One atom of the prep pass: quantise, store x, y, z to the slot (sigma/2,
@@ -171,15 +190,19 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     __shared__ int2 s_meta[kMaxRun];
     __shared__ float jf[3][kMaxRun * 8];
     const int il = lane & 7, jl = lane >> 3;
-    const int sc = A.wi_sc[w];
-    const int e0 = A.wi_start[w], ne = A.wi_end[w] - e0;
+    // (supercluster, first entry, end entry, j-cluster of the first entry):
+    // the first j positions load with the run's list data
+    const int4 wd = A.wi[w];
+    const int sc = wd.x;
+    const int e0 = wd.y, ne = wd.z - e0;
+    int4 cq = qpos[(size_t)wd.w * 8 + jl];
+    float ce = qeps[(size_t)wd.w * 8 + jl];
     {
         // keep tile ci of an entry unless byte_ci > bound (see kPruneSteps)
         float bound = 3.0e38f;
         if (A.disp) {
             float d = lane < kDispSlots ? __uint_as_float(A.disp[lane * kDispStride]) : 0.f;
-            for (int o = 1; o < kDispSlots; o <<= 1) d = fmaxf(d, __shfl_xor(d, o));
-            // lane 0 holds the largest of the slots
+            d = row_max16(d);  // lane 0 holds the largest of the kDispSlots (16) slots
             const float dmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(d)));
             bound = fmaf(dmax, A.prune_scale, A.prune_bias);
         }
@@ -226,8 +249,6 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     const float inv_w_in = CUBIC ? p.inv_w_in_u : p.inv_w_in, nrw_in = in_vgpr(p.nrw_in);
     const float ds_coef_in = CUBIC ? p.ds_coef_in_u : p.ds_coef_in;
     double e_acc = 0.0;
-    int4 cq = qpos[(size_t)s_meta[0].x * 8 + jl];
-    float ce = qeps[(size_t)s_meta[0].x * 8 + jl];
     for (int k = 0; k < ne; k++) {
         const int cjn = __builtin_amdgcn_readfirstlane(s_meta[min(k + 1, ne - 1)].x);
         const int4 nq = qpos[(size_t)cjn * 8 + jl];
@@ -352,6 +373,12 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     __syncthreads();  // LDS is reused by the next work item of a persistent wave
 }
 
+// Work-item descriptors: the j-cluster of each item's first entry.
+__global__ void k_wi_cj0(int n, int4 *wi, const int *ent_cj) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < n) wi[w].w = ent_cj[wi[w].y];
+}
+
 template <bool CUBIC, bool SHELL>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_forces(TileArgs A,
                                                                                        TermArgs ta) {
@@ -426,7 +453,7 @@ TileArgs tile_args(gm_sys *s) {
     TileArgs A{};
     A.qpos = s->qpos.p;
     A.qeps = s->qeps.p;
-    A.wi_sc = s->wi_sc.p; A.wi_start = s->wi_start.p; A.wi_end = s->wi_end.p;
+    A.wi = s->wi.p;
     A.ent_meta = s->ent_meta.p; A.ent_dmin = s->ent_dmin.p; A.ent_lb = s->ent_lb.p;
     A.disp = (s->prune && s->prune_geom) ? s->disp.p : nullptr;
     // keep iff byte <= (2 disp + 1e-4 r_c) / step + 1 (see kPruneSteps)
@@ -478,6 +505,12 @@ hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const doub
                           : (A.cubic ? k_forces<true, false> : k_forces<false, false>);
         kern<<<(unsigned)grid, 64, 0, s->stream>>>(A, ta);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_work_item_cj0(gm_sys *s) {
+    if (s->n_wi > 0)
+        k_wi_cj0<<<(unsigned)((s->n_wi + 255) / 256), 256, 0, s->stream>>>((int)s->n_wi, s->wi.p, s->ent_cj.p);
     return hipGetLastError();
 }
 

@@ -190,7 +190,7 @@ struct TileArgs {
     const int4 *qpos;                  // per slot: fixed-point x, y, z and the bits of sigma/2
                                        // (kernel length unit), written by the prep kernel
     const float *qeps;                 // per slot: 2 sqrt(eps)
-    const int *wi_sc, *wi_start, *wi_end;
+    const int4 *wi;  // work items: (supercluster, first entry, end entry, j-cluster of the first entry)
     const int2 *ent_meta;              // (j cluster, i-cluster mask)
     const uint2 *ent_dmin;             // per entry, one byte per tile: closest listed pair at build
     const uint8_t *ent_lb;             // per-lane mask bytes per entry
