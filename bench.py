@@ -369,7 +369,7 @@ def main():
                      "kernel_ms_min": prof["ms_nonbonded_min"], "flop_per_pair": FLOP_PER_PAIR,
                      "timed_launches": prof["n_evals"], "events_every": PROFILE_EVERY},
         "device_ms_per_eval": ev_ms,
-        "prep_kernel_ms": prep_ms,
+        "ms_to_force_kernel_start": prep_ms,
         "tile_pruning": pruning,
         "ms_per_step_events_every_step": ms_all_events,
         "ms_per_eval_host_buffers": ms_host,

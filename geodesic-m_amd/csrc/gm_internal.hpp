@@ -305,7 +305,9 @@ hipError_t launch_needs_rebuild(gm_sys *s, const double *x, const double *y, con
 hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double *z);
 // fills the work items' first j-cluster from the list entries
 hipError_t launch_work_item_cj0(gm_sys *s);
-hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z, bool shell = false);
+// ev_start / ev_stop (optional): stamped by the dispatch at the kernel's start and end
+hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z, bool shell = false,
+                         hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
 // gm_bonded.hip: energy reduction, f64 output, accumulator clear (lj_only:
 // the bonded energy segments are empty, as after a shell-mode launch)
 hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz, bool lj_only = false);

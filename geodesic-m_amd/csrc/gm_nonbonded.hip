@@ -34,6 +34,8 @@
 //    integer atomics, so the result does not depend on scheduling order.
 // Bonded path (the first ceil(n_terms/64) workgroups): one thread per bonded term (f64,
 // gm_terms.hpp), running beside the tiles instead of in launches of its own.
+#include <hip/hip_ext.h>
+
 #include "gm_internal.hpp"
 #include "gm_terms.hpp"
 
@@ -491,7 +493,8 @@ hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double
 
 // One launch: ceil(n_terms / 64) bonded workgroups, then the tile work items.
 // Shell mode: tile work items only.
-hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z, bool shell) {
+hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z, bool shell,
+                         hipEvent_t ev_start, hipEvent_t ev_stop) {
     TermArgs ta = term_args(s, x, y, z);
     TileArgs A = tile_args(s);
     if (shell) {
@@ -503,7 +506,15 @@ hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const doub
     if (grid > 0) {
         auto kern = shell ? (A.cubic ? k_forces<true, true> : k_forces<false, true>)
                           : (A.cubic ? k_forces<true, false> : k_forces<false, false>);
-        kern<<<(unsigned)grid, 64, 0, s->stream>>>(A, ta);
+        if (ev_start || ev_stop)  // the dispatch itself stamps the events: the kernel's own span
+            hipExtLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64), 0, s->stream, ev_start, ev_stop, 0, A, ta);
+        else
+            kern<<<(unsigned)grid, 64, 0, s->stream>>>(A, ta);
+    } else {
+        hipError_t e = hipSuccess;
+        if (ev_start) e = hipEventRecord(ev_start, s->stream);
+        if (e == hipSuccess && ev_stop) e = hipEventRecord(ev_stop, s->stream);
+        if (e != hipSuccess) return e;
     }
     return hipGetLastError();
 }

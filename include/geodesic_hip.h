@@ -179,10 +179,12 @@ gm_status gm_get_stats(gm_sys *sys, gm_stats *out);
 typedef struct {
     int64_t n_evals;            /* evaluations timed */
     int64_t n_untimed;          /* evaluations past the event pool */
-    double ms_nonbonded_total;  /* sum over timed evaluations, nonbonded kernel */
+    double ms_nonbonded_total;  /* sum over timed evaluations, nonbonded kernel (its dispatch's own
+                                   start and end stamps) */
     double ms_nonbonded_min, ms_nonbonded_max;
     double ms_eval_total;       /* sum over timed evaluations, whole evaluation */
-    double ms_prep_total;       /* sum over timed evaluations, position prep kernel */
+    double ms_prep_total;       /* sum over timed evaluations, evaluation start to the nonbonded
+                                   kernel's start (accumulator clear, position prep kernel) */
 } gm_profile;
 gm_status gm_profile_begin(gm_sys *sys, int32_t every);
 gm_status gm_profile_end(gm_sys *sys, gm_profile *out);
