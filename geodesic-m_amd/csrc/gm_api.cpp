@@ -618,7 +618,6 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     for (DevBuf<int> *b : {&s->col_count, &s->col_start, &s->col_sc, &s->col_fill})
         TRY(b->ensure((size_t)ncol_max));
     TRY(s->ext.ensure(8));
-    TRY(s->mask_parts.ensure((size_t)kMaskBlocks * 4 * 4));
     TRY(s->bucket.ensure(N));
     TRY(s->sorted_atoms.ensure(N));
     TRY(s->atom_slot.ensure(N));

@@ -724,6 +724,8 @@ hipError_t launch_pair_fill_and_masks(gm_sys *s) {
     unsigned long long *counters = (unsigned long long *)(s->meta.p + 8);
     if (s->nent == 0) return launch_fill_u64(counters, 4, st);
     const int blocks = (int)std::min<int64_t>((s->nent + 3) / 4, kMaskBlocks);
+    hipError_t e = s->mask_parts.ensure((size_t)blocks * 4 * 4);  // 4 counts per wavefront
+    if (e != hipSuccess) return e;
     k_masks<<<(unsigned)blocks, 256, 0, st>>>((int)s->nent, s->ent_sc.p, s->ent_cj.p, s->ent_imask.p,
                                               s->ent_mask.p, s->ent_meta.p, s->ent_dmin.p, s->ent_lb.p,
                                               s->slot_atom.p, s->sq.p, s->x.p, s->y.p, s->z.p,

@@ -23,7 +23,7 @@ constexpr int kColumnCap = 8192;
 // Longest run of list entries one nonbonded wavefront processes.
 constexpr int kMaxRun = 32;
 // list build: workgroups (of four wavefronts) of the pair-mask kernel, at most
-constexpr int kMaskBlocks = 2048;
+constexpr int kMaskBlocks = 8192;
 // Fixed-point force unit: 2^-32 kcal/mol/A per integer step.
 constexpr double kFixedScale = 4294967296.0;
 constexpr double kFixedInv = 1.0 / 4294967296.0;
