@@ -199,6 +199,7 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     const int e0 = wd.y, ne = wd.z - e0;
     int4 cq = qpos[(size_t)wd.w * 8 + jl];
     float ce = qeps[(size_t)wd.w * 8 + jl];
+    unsigned long long live = 0;
     {
         // keep tile ci of an entry unless byte_ci > bound (see kPruneSteps)
         float bound = 3.0e38f;
@@ -227,10 +228,17 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
             }
             mv.y &= (int)keep;
         }
+        // Entries with every tile pruned contribute nothing: the run's
+        // metadata is compacted to its live entries (in order), so the loop
+        // below neither visits nor prefetches them.  Entry e goes to its rank
+        // among them, its own index above the 8 mask bits (its pair-mask bytes
+        // stay in place).
 #pragma unroll
         for (int r = 0; r < kMaxRun / 4; r++) s_lb[lane + 64 * r] = lbv[r];
-        if (lane < kMaxRun) s_meta[lane] = mv;
+        live = __ballot(mv.y != 0);
+        if (mv.y != 0) s_meta[__popcll(live & ((1ull << lane) - 1ull))] = make_int2(mv.x, mv.y | (lane << 8));
     }
+    const int nl = __popcll(live);  // live entries of the run
     // The supercluster's 64 i atoms live in LDS (fixed-point position + sigma/2
     // in one int4, 2 sqrt(eps) beside it): every tile reads 8 distinct entries,
     // broadcast to 8 lanes each, which keeps 40 registers free for occupancy.
@@ -251,14 +259,19 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     const float inv_w_in = CUBIC ? p.inv_w_in_u : p.inv_w_in, nrw_in = in_vgpr(p.nrw_in);
     const float ds_coef_in = CUBIC ? p.ds_coef_in_u : p.ds_coef_in;
     double e_acc = 0.0;
-    for (int k = 0; k < ne; k++) {
-        const int cjn = __builtin_amdgcn_readfirstlane(s_meta[min(k + 1, ne - 1)].x);
+    if (nl > 0 && !(live & 1ull)) {  // the first entry was pruned: start from the first live one
+        cq = qpos[(size_t)s_meta[0].x * 8 + jl];
+        ce = qeps[(size_t)s_meta[0].x * 8 + jl];
+    }
+    for (int k = 0; k < nl; k++) {
+        const int cjn = __builtin_amdgcn_readfirstlane(s_meta[min(k + 1, nl - 1)].x);
         const int4 nq = qpos[(size_t)cjn * 8 + jl];
         const float nce = qeps[(size_t)cjn * 8 + jl];
-        const uint32_t im = (uint32_t)__builtin_amdgcn_readfirstlane(s_meta[k].y);
+        const uint32_t mk = (uint32_t)__builtin_amdgcn_readfirstlane(s_meta[k].y);
+        const uint32_t im = mk & 255u;
         float fjx = 0.f, fjy = 0.f, fjz = 0.f;
         if (im) {
-            const uint32_t bits = s_lb8[k * 64 + lane];
+            const uint32_t bits = s_lb8[(mk >> 8) * 64 + lane];
             const float shj = __int_as_float(cq.w), ej = ce;
             float ee = 0.f;
 #pragma unroll
@@ -333,9 +346,8 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     }
     __syncthreads();
     const float fscale = CUBIC ? p.fscale_u : 4294967296.0f;
-    for (int idx = lane; idx < ne * 8; idx += 64) {
+    for (int idx = lane; idx < nl * 8; idx += 64) {
         const int2 m = s_meta[idx >> 3];
-        if (m.y == 0) continue;
         const size_t js = (size_t)m.x * 8 + (idx & 7);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -364,7 +376,7 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
         const float send = odd ? h2[k][0] : h2[k][1];
         fo[k] = keep + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(send), 0x128, 0xF, 0xF, false));
     }
-    {
+    if (nl > 0) {  // (a run with every tile pruned has nothing to add)
         const size_t s = (size_t)sc * 64 + lane;
         fixed_add(&F[s], fo[0], fscale);
         fixed_add(&F[(size_t)n_pad + s], fo[1], fscale);
