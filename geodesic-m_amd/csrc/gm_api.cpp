@@ -111,8 +111,23 @@ gm_status make_work_items(gm_sys *s) {
         wi[o] = make_int4(it.sc, it.e0, it.e1, 0);
     }
     s->n_wi = (int64_t)wi.size();
+    // each supercluster's work items, ascending (the finish kernel adds their i forces)
+    std::vector<int> off((size_t)s->nsc + 1, 0), lst(ni);
+    for (const int4 &v : wi) off[(size_t)v.x + 1]++;
+    for (size_t k = 1; k < off.size(); k++) off[k] += off[k - 1];
+    {
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        for (size_t w = 0; w < ni; w++) lst[(size_t)fill[(size_t)wi[w].x]++] = (int)w;
+    }
     hipStream_t st = s->stream;
     GM_HIP(upload(s->wi, wi.data(), wi.size(), st));
+    GM_HIP(upload(s->sc_wi_off, off.data(), off.size(), st));
+    GM_HIP(upload(s->sc_wi, lst.data(), lst.size(), st));
+    // Small work items: many i-force updates per slot, which as atomics (past
+    // the L2) cost the force kernel more than plain per-item records cost the
+    // finish kernel; large ones: few, and atomics are cheaper.
+    s->fi_records = chunk < 16;
+    if (s->fi_records) GM_HIP(s->fi_items.ensure((size_t)std::max<int64_t>(s->n_wi, 1) * 192));
     GM_HIP(launch_work_item_cj0(s));  // .w = ent_cj[first entry], from the device list
     GM_HIP(s->e_wi.ensure((size_t)std::max<int64_t>(s->n_wi, 1)));
     GM_HIP(hipStreamSynchronize(st));
@@ -675,6 +690,9 @@ gm_status gm_system_destroy(gm_sys *s) {
     s->sq.release();
     s->sxb.release();
     s->wi.release();
+    s->fi_items.release();
+    s->sc_wi_off.release();
+    s->sc_wi.release();
     s->cl_ref.release();
     s->sc_ref.release();
     s->cl_half.release();

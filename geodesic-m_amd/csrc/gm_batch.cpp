@@ -35,6 +35,8 @@ struct gm_batch {
     hipEvent_t fork = nullptr, join = nullptr;
     DevBuf<int> atom_rep;
     DevBuf<int64_t> atom_off;
+    DevBuf<int2> scl;      // every (replica, supercluster) pair, for the finish kernel
+    int64_t n_scl = 0;
     DevBuf<double> fx, fy, fz, E;
     int n_items = 0;
     int chunk = 0;         // entries per tile work item (0 = automatic)
@@ -78,6 +80,13 @@ gm_status make_plan(gm_batch *b) {
     }
     std::vector<int2> items(bonded_items);
     items.insert(items.end(), tile_items.begin(), tile_items.end());
+    std::vector<int2> scl;
+    for (int r = 0; r < R; r++)
+        for (int64_t k = 0; k < b->reps[(size_t)r]->nsc; k++) scl.push_back(make_int2(r, (int)k));
+    b->n_scl = (int64_t)scl.size();
+    GM_HIP(b->scl.ensure(std::max<size_t>(scl.size(), 1)));
+    if (!scl.empty())
+        GM_HIP(hipMemcpyAsync(b->scl.p, scl.data(), sizeof(int2) * scl.size(), hipMemcpyHostToDevice, st));
     b->n_bonded_items = (int)bonded_items.size();
     b->n_items = (int)tile_items.size();
     GM_HIP(b->d_reps.ensure((size_t)R));
@@ -102,8 +111,8 @@ gm_status make_plan(gm_batch *b) {
         e1 = launch_batch_tiles(b->d_reps.p, b->d_items.p + b->n_bonded_items, b->n_items, grid, st);
     if (e1 == hipSuccess) e1 = hipStreamWaitEvent(st, b->join, 0);
     if (e1 == hipSuccess)
-        e1 = launch_batch_finish(b->d_reps.p, R, b->atom_rep.p, b->atom_off.p, b->off.back(), b->E.p,
-                                 b->fx.p, b->fy.p, b->fz.p, st);
+        e1 = launch_batch_finish(b->d_reps.p, R, b->scl.p, b->n_scl, b->atom_off.p, b->E.p, b->fx.p, b->fy.p,
+                                 b->fz.p, st);
     hipGraph_t g = nullptr;
     hipError_t e3 = hipStreamEndCapture(st, &g);
     if (e1 != hipSuccess) { if (g) (void)hipGraphDestroy(g); return from_hip(e1, "batched graph capture"); }
@@ -181,6 +190,7 @@ gm_status gm_batch_destroy(gm_batch *b) {
     b->d_items.release();
     b->atom_rep.release();
     b->atom_off.release();
+    b->scl.release();
     b->fx.release();
     b->fy.release();
     b->fz.release();

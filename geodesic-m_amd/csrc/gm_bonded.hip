@@ -44,11 +44,63 @@ __device__ inline double block_sum(const double *ptr, long long len, double *sh)
     return r;
 }
 
-// Blocks [0, nseg): block b sums segment b.  Blocks >= nseg: one thread per
-// slot (reads of the accumulator are contiguous; the f64 force goes to the
-// slot's atom).  Block 0 also clears the displacement bound the prep kernel raised.
+// Slot s's force in fixed point: the accumulator (j forces, bonded terms)
+// plus the i-force records of its supercluster's work items.  Integer sums:
+// the order does not change the result.
+__device__ inline long long to_fixed(float v, float scale) { return (long long)__builtin_rintf(v * scale); }
+
+// Forces of one supercluster's 64 slots, by G (4, 8 or 16) wavefronts of a
+// 1024-thread workgroup (wavefronts G g .. G g + G - 1 for the workgroup's
+// supercluster g): wavefront q adds the i-force records of the
+// supercluster's work items q, q + G, ... in fixed point; the first then
+// adds the others' sums and the accumulator (j forces, bonded terms), clears
+// the accumulator and writes each slot's f64 force to its atom (atom index +
+// atom_base).  Integer sums: the order does not change the result.  Every
+// thread of the workgroup calls it (it holds a barrier); valid says whether
+// group g has a supercluster.
+__device__ inline void sc_finish(bool valid, int sc, int G, unsigned long long *F, size_t n_pad,
+                                 const float *fi_items, const int *sc_wi_off, const int *sc_wi, float fi_scale,
+                                 const int *slot_atom, int64_t atom_base, double *fx, double *fy, double *fz) {
+    __shared__ long long red[16][3][64];
+    const int t = threadIdx.x, wv = t >> 6, l = t & 63, q = wv & (G - 1);
+    long long ax = 0, ay = 0, az = 0;
+    if (valid && fi_items) {  // (null: the i forces are already in the accumulator)
+        const int k0 = sc_wi_off[sc], k1 = sc_wi_off[sc + 1];
+#pragma unroll 2
+        for (int k = k0 + q; k < k1; k += G) {
+            const float *fw = fi_items + (size_t)sc_wi[k] * 192 + l;
+            ax += to_fixed(fw[0], fi_scale);
+            ay += to_fixed(fw[64], fi_scale);
+            az += to_fixed(fw[128], fi_scale);
+        }
+    }
+    red[wv][0][l] = ax;
+    red[wv][1][l] = ay;
+    red[wv][2][l] = az;
+    __syncthreads();
+    if (!valid || q != 0) return;
+    const size_t s = (size_t)sc * 64 + l;
+    unsigned long long f[3];
+    for (int c = 0; c < 3; c++) {
+        unsigned long long v = F[(size_t)c * n_pad + s];
+        for (int o = 0; o < G; o++) v += (unsigned long long)red[wv + o][c][l];
+        f[c] = v;
+        F[(size_t)c * n_pad + s] = 0ull;
+    }
+    const int a = slot_atom[s];
+    if (a < 0) return;
+    fx[atom_base + a] = (double)(long long)f[0] * kFixedInv;
+    fy[atom_base + a] = (double)(long long)f[1] * kFixedInv;
+    fz[atom_base + a] = (double)(long long)f[2] * kFixedInv;
+}
+
+// Blocks [0, nseg): block b sums segment b.  Blocks >= nseg: 16 / G
+// superclusters each (sc_finish).  Block 0 also clears the displacement bound
+// the prep kernel raised.
 __global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, const int *slot_atom,
-                                                 unsigned long long *F, size_t n_pad, double *fx,
+                                                 unsigned long long *F, size_t n_pad,
+                                                 const float *fi_items, const int *sc_wi_off,
+                                                 const int *sc_wi, float fi_scale, int G, double *fx,
                                                  double *fy, double *fz, unsigned *disp) {
     __shared__ double sh[16];
     const int b = blockIdx.x, t = threadIdx.x;
@@ -58,28 +110,19 @@ __global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, const in
         if (t == 0) eout[b] = v;
         return;
     }
-    const size_t s = (size_t)(b - sg.nseg) * 1024 + t;
-    if (s >= n_pad) return;
-    const int a = slot_atom[s];
-    const long long qx = (long long)F[s], qy = (long long)F[n_pad + s], qz = (long long)F[2 * n_pad + s];
-    F[s] = 0ull;
-    F[n_pad + s] = 0ull;
-    F[2 * n_pad + s] = 0ull;
-    if (a < 0) return;
-    fx[a] = (double)qx * kFixedInv;
-    fy[a] = (double)qy * kFixedInv;
-    fz[a] = (double)qz * kFixedInv;
+    const int sc = (b - sg.nseg) * (16 / G) + (t >> 6) / G;
+    sc_finish((size_t)sc * 64 < n_pad, sc, G, F, n_pad, fi_items, sc_wi_off, sc_wi, fi_scale, slot_atom, 0,
+              fx, fy, fz);
 }
 
 // Batched end of evaluation: blocks [0, 5 R) sum replica b/5's segment b%5
 // (bond, angle, dihedral, 1-4, LJ) exactly like k_finish; the remaining
-// blocks convert every atom's force of every replica and clear the
-// accumulators.
+// blocks finish four (replica, supercluster) pairs of scl each, exactly like
+// k_finish, writing forces at the replica's atom offset.
 __global__ __launch_bounds__(1024) void k_batch_finish(const ReplicaDesc *__restrict__ reps, int n_reps,
-                                                       const int *__restrict__ atom_rep,
-                                                       const int64_t *__restrict__ atom_off,
-                                                       int64_t n_atoms, double *eout, double *fx,
-                                                       double *fy, double *fz) {
+                                                       const int2 *__restrict__ scl, int64_t n_scl,
+                                                       const int64_t *__restrict__ atom_off, double *eout,
+                                                       double *fx, double *fy, double *fz) {
     __shared__ double sh[1024];
     const int b = blockIdx.x, t = threadIdx.x;
     if (b < 5 * n_reps) {
@@ -100,30 +143,24 @@ __global__ __launch_bounds__(1024) void k_batch_finish(const ReplicaDesc *__rest
         if (t == 0) eout[b] = v;
         return;
     }
-    const int64_t a = (int64_t)(b - 5 * n_reps) * 1024 + t;
-    if (a >= n_atoms) return;
-    const int r = atom_rep[a];
-    const ReplicaDesc &R = reps[r];
-    unsigned long long *F = R.terms.F;
-    const size_t n_pad = R.terms.n_pad;
-    const size_t s = (size_t)R.terms.atom_slot[a - atom_off[r]];
-    fx[a] = (double)(long long)F[s] * kFixedInv;
-    fy[a] = (double)(long long)F[n_pad + s] * kFixedInv;
-    fz[a] = (double)(long long)F[2 * n_pad + s] * kFixedInv;
-    F[s] = 0ull;
-    F[n_pad + s] = 0ull;
-    F[2 * n_pad + s] = 0ull;
+    // the rest: four (replica, supercluster) pairs per workgroup
+    const int64_t pidx = (int64_t)(b - 5 * n_reps) * 4 + (t >> 8);
+    const bool valid = pidx < n_scl;
+    const int2 rs = valid ? scl[pidx] : make_int2(0, 0);
+    const ReplicaDesc &R = reps[rs.x];
+    sc_finish(valid, rs.y, 4, R.terms.F, R.terms.n_pad, R.tile.fi_items, R.tile.sc_wi_off, R.tile.sc_wi,
+              R.tile.fi_scale, R.tile.slot_atom, atom_off[rs.x], fx, fy, fz);
 }
 
 }  // namespace
 
 namespace gm {
 
-hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int *atom_rep,
-                               const int64_t *atom_off, int64_t n_atoms, double *eout,
-                               double *fx, double *fy, double *fz, hipStream_t st) {
-    const unsigned grid = 5u * (unsigned)n_reps + (unsigned)((n_atoms + 1023) / 1024);
-    k_batch_finish<<<grid, 1024, 0, st>>>(reps, n_reps, atom_rep, atom_off, n_atoms, eout, fx, fy, fz);
+hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int2 *scl, int64_t n_scl,
+                               const int64_t *atom_off, double *eout, double *fx, double *fy, double *fz,
+                               hipStream_t st) {
+    const unsigned grid = 5u * (unsigned)n_reps + (unsigned)((n_scl + 3) / 4);
+    k_batch_finish<<<grid, 1024, 0, st>>>(reps, n_reps, scl, n_scl, atom_off, eout, fx, fy, fz);
     return hipGetLastError();
 }
 
@@ -139,9 +176,14 @@ hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz, bool lj_
     sg.ptr[4] = s->e_wi.p;                  sg.len[4] = s->n_wi;
     sg.nseg = 5;
     const size_t n_pad = (size_t)s->nsc * 64;
-    const unsigned grid = 5u + (unsigned)((n_pad + 1023) / 1024);
-    k_finish<<<grid, 1024, 0, s->stream>>>(sg, s->e_out.p, s->slot_atom.p, s->F.p, n_pad, fx, fy, fz,
-                                            s->disp.p);
+    // wavefronts per supercluster: about one work-item record each, 4 to 16
+    const int64_t per_sc = (s->fi_records && s->nsc > 0) ? (s->n_wi + s->nsc - 1) / s->nsc : 1;
+    const int G = per_sc > 8 ? 16 : (per_sc > 4 ? 8 : 4);
+    const unsigned grid = 5u + (unsigned)((s->nsc + 16 / G - 1) / (16 / G));
+    const float fi_scale = span_is_cubic(s) ? s->nbp.fscale_u : 4294967296.0f;
+    k_finish<<<grid, 1024, 0, s->stream>>>(sg, s->e_out.p, s->slot_atom.p, s->F.p, n_pad,
+                                            s->fi_records ? s->fi_items.p : nullptr,
+                                            s->sc_wi_off.p, s->sc_wi.p, fi_scale, G, fx, fy, fz, s->disp.p);
     return hipGetLastError();
 }
 

@@ -226,6 +226,9 @@ struct gm_sys {
     DevBuf<uint8_t> ent_lb;               // per entry, per lane: mask bits over the 8 i clusters
     bool F_dirty = true;                  // accumulator not known to be zero
     DevBuf<int4> wi;          // work items (supercluster, first entry, end entry, first j-cluster)
+    DevBuf<float> fi_items;               // i forces per work item (192 each, kernel force unit)
+    bool fi_records = false;              // i forces as per-item records (small items) or atomics
+    DevBuf<int> sc_wi_off, sc_wi;         // per supercluster, its work items (CSR)
     int chunk = 0;  // entries per work item (0 = automatic)
 
     // accumulators
@@ -325,9 +328,8 @@ hipError_t launch_batch_prep(const ReplicaDesc *reps, const int *atom_rep, const
 hipError_t launch_batch_tiles(const ReplicaDesc *reps, const int2 *items, int n_items, int grid,
                               hipStream_t st);
 hipError_t launch_batch_bonded(const ReplicaDesc *reps, const int2 *items, int n_items, hipStream_t st);
-hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int *atom_rep,
-                               const int64_t *atom_off, int64_t n_atoms, double *eout,
-                               double *fx, double *fy, double *fz,
+hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int2 *scl, int64_t n_scl,
+                               const int64_t *atom_off, double *eout, double *fx, double *fy, double *fz,
                                hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long *p, size_t n, hipStream_t st);
 hipError_t launch_fill_i32(int *p, size_t n, int v, hipStream_t st);

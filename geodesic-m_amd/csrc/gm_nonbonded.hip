@@ -376,7 +376,14 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
         const float send = odd ? h2[k][0] : h2[k][1];
         fo[k] = keep + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(send), 0x128, 0xF, 0xF, false));
     }
-    if (nl > 0) {  // (a run with every tile pruned has nothing to add)
+    if (A.fi_items) {
+        // the item's own record (plain stores; the finish kernel adds the
+        // records of a supercluster's items): no atomics on the i side
+        float *fw = A.fi_items + (size_t)w * 192 + lane;
+        fw[0] = fo[0];
+        fw[64] = fo[1];
+        fw[128] = fo[2];
+    } else if (nl > 0) {  // large work items: into the accumulator (nothing to add for an empty run)
         const size_t s = (size_t)sc * 64 + lane;
         fixed_add(&F[s], fo[0], fscale);
         fixed_add(&F[(size_t)n_pad + s], fo[1], fscale);
@@ -476,6 +483,11 @@ TileArgs tile_args(gm_sys *s) {
     A.F = s->F.p;
     A.n_pad = (int)(s->nsc * 64);
     A.e_wi = s->e_wi.p;
+    A.fi_items = s->fi_records ? s->fi_items.p : nullptr;
+    A.sc_wi_off = s->sc_wi_off.p;
+    A.sc_wi = s->sc_wi.p;
+    A.fi_scale = span_is_cubic(s) ? s->nbp.fscale_u : 4294967296.0f;  // the tile kernel's fscale
+    A.slot_atom = s->slot_atom.p;
     A.p = s->nbp;
     A.cubic = span_is_cubic(s) ? 1 : 0;
     return A;
