@@ -49,7 +49,7 @@ __device__ inline double block_sum(const double *ptr, long long len, double *sh)
 // the order does not change the result.
 __device__ inline long long to_fixed(float v, float scale) { return (long long)__builtin_rintf(v * scale); }
 
-// Forces of one supercluster's 64 slots, by G (4, 8 or 16) wavefronts of a
+// Forces of one supercluster's 64 slots, by G (1, 4, 8 or 16) wavefronts of a
 // 1024-thread workgroup (wavefronts G g .. G g + G - 1 for the workgroup's
 // supercluster g): wavefront q adds the i-force records of the
 // supercluster's work items q, q + G, ... in fixed point; the first then
@@ -117,10 +117,11 @@ __global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, const in
 
 // Batched end of evaluation: blocks [0, 5 R) sum replica b/5's segment b%5
 // (bond, angle, dihedral, 1-4, LJ) exactly like k_finish; the remaining
-// blocks finish four (replica, supercluster) pairs of scl each, exactly like
-// k_finish, writing forces at the replica's atom offset.
+// blocks finish 16 / G (replica, supercluster) pairs of scl each, exactly
+// like k_finish, writing forces at the replica's atom offset (G = 1 when no
+// replica keeps i-force records).
 __global__ __launch_bounds__(1024) void k_batch_finish(const ReplicaDesc *__restrict__ reps, int n_reps,
-                                                       const int2 *__restrict__ scl, int64_t n_scl,
+                                                       const int2 *__restrict__ scl, int64_t n_scl, int G,
                                                        const int64_t *__restrict__ atom_off, double *eout,
                                                        double *fx, double *fy, double *fz) {
     __shared__ double sh[1024];
@@ -143,12 +144,12 @@ __global__ __launch_bounds__(1024) void k_batch_finish(const ReplicaDesc *__rest
         if (t == 0) eout[b] = v;
         return;
     }
-    // the rest: four (replica, supercluster) pairs per workgroup
-    const int64_t pidx = (int64_t)(b - 5 * n_reps) * 4 + (t >> 8);
+    // the rest: 16 / G (replica, supercluster) pairs per workgroup
+    const int64_t pidx = (int64_t)(b - 5 * n_reps) * (16 / G) + (t >> 6) / G;
     const bool valid = pidx < n_scl;
     const int2 rs = valid ? scl[pidx] : make_int2(0, 0);
     const ReplicaDesc &R = reps[rs.x];
-    sc_finish(valid, rs.y, 4, R.terms.F, R.terms.n_pad, R.tile.fi_items, R.tile.sc_wi_off, R.tile.sc_wi,
+    sc_finish(valid, rs.y, G, R.terms.F, R.terms.n_pad, R.tile.fi_items, R.tile.sc_wi_off, R.tile.sc_wi,
               R.tile.fi_scale, R.tile.slot_atom, atom_off[rs.x], fx, fy, fz);
 }
 
@@ -156,11 +157,12 @@ __global__ __launch_bounds__(1024) void k_batch_finish(const ReplicaDesc *__rest
 
 namespace gm {
 
-hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int2 *scl, int64_t n_scl,
+hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int2 *scl, int64_t n_scl, int G,
                                const int64_t *atom_off, double *eout, double *fx, double *fy, double *fz,
                                hipStream_t st) {
-    const unsigned grid = 5u * (unsigned)n_reps + (unsigned)((n_scl + 3) / 4);
-    k_batch_finish<<<grid, 1024, 0, st>>>(reps, n_reps, scl, n_scl, atom_off, eout, fx, fy, fz);
+    const int per = 16 / G;
+    const unsigned grid = 5u * (unsigned)n_reps + (unsigned)((n_scl + per - 1) / per);
+    k_batch_finish<<<grid, 1024, 0, st>>>(reps, n_reps, scl, n_scl, G, atom_off, eout, fx, fy, fz);
     return hipGetLastError();
 }
 
