@@ -124,19 +124,16 @@ __device__ inline void atomic_max_nonneg(unsigned *slot, float v) {
     __hip_atomic_fetch_max(slot, __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ __launch_bounds__(1024) void k_prep(PrepArgs P) {
-    const int a = blockIdx.x * 1024 + threadIdx.x;
+// One thread per atom, 256 per workgroup (spread over many CUs: the kernel is
+// a load round trip and a store); each wavefront raises one of the
+// kDispSlots displacement words with its largest value.
+__global__ __launch_bounds__(256) void k_prep(PrepArgs P) {
+    const int a = blockIdx.x * 256 + threadIdx.x;
     float d = 0.f;
     if (a < P.n) d = prep_atom(P, a);
     d = wave_max(d);
-    __shared__ float sm[16];
-    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = d;
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        d = threadIdx.x < 16 ? sm[threadIdx.x] : 0.f;
-        d = wave_max(d);
-        if (threadIdx.x == 0) atomic_max_nonneg(&P.disp[(blockIdx.x % kDispSlots) * kDispStride], d);
-    }
+    if ((threadIdx.x & 63) == 0)
+        atomic_max_nonneg(&P.disp[((blockIdx.x * 4 + (threadIdx.x >> 6)) % kDispSlots) * kDispStride], d);
 }
 
 // Batched prep: atom a of the concatenated replicas.  A wavefront whose atoms
@@ -511,7 +508,7 @@ PrepArgs prep_args(gm_sys *s, const double *x, const double *y, const double *z)
 // Slot-order fixed-point positions and the displacement bound; precedes every
 // launch_forces on the same positions.
 hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double *z) {
-    if (s->n > 0) k_prep<<<(unsigned)((s->n + 1023) / 1024), 1024, 0, s->stream>>>(prep_args(s, x, y, z));
+    if (s->n > 0) k_prep<<<(unsigned)((s->n + 255) / 256), 256, 0, s->stream>>>(prep_args(s, x, y, z));
     return hipGetLastError();
 }
 
