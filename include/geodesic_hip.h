@@ -191,8 +191,10 @@ gm_status gm_profile_end(gm_sys *sys, gm_profile *out);
 
 /* Tuning: list entries per nonbonded work item (one wavefront each, at most
  * 32); 0 picks a size that gives about 1.5 work items per resident wavefront
- * slot.  Work items are issued heaviest first.  Takes effect immediately on a
- * built list (no rebuild). */
+ * slot.  Work items are issued heaviest first.  Items of fewer than 16
+ * entries leave their i forces as per-item records that the finish kernel
+ * adds; larger ones use atomics (same result either way).  Takes effect
+ * immediately on a built list (no rebuild). */
 gm_status gm_set_work_chunk(gm_sys *sys, int32_t entries_per_item);
 
 /* Tile pruning (default on): each evaluation bounds how far any atom has moved
