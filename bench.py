@@ -256,6 +256,96 @@ def bench_surrogate(frames=20000, steps=400, traj_cycles=25, min_steps=300):
             "pair_interactions_per_s_equivalent": n_cut / (mts_ms * 1e-3)}
 
 
+KB_KCAL = 0.0019872041      # kcal/mol/K
+KCAL_PER_AMU = 418.4        # (kcal/mol)/amu in A^2/ps^2
+
+
+def md_system(k=10, spacing=16.0):
+    """k^3 copies of the ala_dipeptide fixture (prmtop read by the host
+    layer, bonds to hydrogen promoted to constraints) in a periodic box."""
+    from geodesic_hip import host, systems
+    fix = os.path.join(ROOT, "tests", "golden", "fixtures")
+    t = host.Topology(os.path.join(fix, "ala_dipeptide.prmtop"))
+    t.promote_hydrogen_bonds()
+    fields = t.as_dict()
+    pos, _ = host.read_inpcrd(os.path.join(fix, "ala_dipeptide.inpcrd"), len(fields["sigma"]))
+    return systems.replicated_molecule(fields, pos, k, spacing)
+
+
+def md_velocities(mass, temperature=300.0, seed=7):
+    """Seeded Maxwell-Boltzmann velocities (A/ps)."""
+    from geodesic_hip import systems
+    rng = systems.Rng(seed)
+    sd = np.sqrt(KB_KCAL * temperature * KCAL_PER_AMU / mass)
+    return [np.ascontiguousarray(rng.normal(len(mass)) * sd) for _ in range(3)]
+
+
+def bench_md(steps=200, warmup=20, frame_interval=50, cpu_steps=20, cpu=True):
+    """Device-resident MD (gm_md_run: the reference run loop, lib.rs:255-330,
+    with B / A + SHAKE / rebuild when needed / forces / B / RATTLE on device
+    arrays): 1,000 ala_dipeptide molecules, 22,000 atoms, 12,000 constraints,
+    160 A periodic box, r_c 12 / r_sw 10 / list 14 A, dt 1 fs, 300 K start.
+    Beside it (cpu=True), the oracle's restatement of the same loop on the
+    host cores: the reference's CPU path for this row."""
+    import geodesic_hip as gh
+    x, y, z, sig, eps, mass, topo, box = md_system()
+    n, nc = len(x), len(topo.constr_i)
+    p = gh.Params(box=(box,) * 3, periodic=True, r_cutoff=12.0, r_switch=10.0, r_skin=14.0)
+    g = gh.System(sig, eps, topo, p)
+    g.dynamics_set(mass, topo.constr_i, topo.constr_j, topo.constr_dsq)
+    vx, vy, vz = md_velocities(mass)
+    d = [gh.DeviceArray.from_host(np.ascontiguousarray(a, np.float64)) for a in (x, y, z, vx, vy, vz)]
+    g.md_run(*d, n_steps=warmup, dt=0.001, frame_interval=warmup, max_iter=100, tol=1e-8)
+    t0 = time.perf_counter()
+    frames, epot, ekin = g.md_run(*d, n_steps=steps, dt=0.001, frame_interval=frame_interval, max_iter=100,
+                                  tol=1e-8)
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    etot = epot + ekin
+    out = {"workload": "device-resident MD, 1,000 ala_dipeptide copies (22,000 atoms, 12,000 H-bond constraints), "
+                       "160 A periodic box, r_c 12 / r_sw 10 / list 14 A, dt 1 fs, 300 K start",
+           "n_atoms": n, "n_constraints": nc, "steps": steps, "ms_per_step": ms, "steps_per_s": 1e3 / ms,
+           "frames": int(len(frames)), "total_energy_first_last": [float(etot[0]), float(etot[-1])]}
+    if cpu:
+        out["cpu_baseline"] = md_cpu_baseline(x, y, z, vx, vy, vz, sig, eps, mass, topo, box, cpu_steps)
+        out["speedup_vs_cpu"] = out["cpu_baseline"]["ms_per_step"] / ms
+    return out
+
+
+def md_cpu_baseline(x, y, z, vx, vy, vz, sig, eps, mass, topo, box, steps):
+    """The oracle's restatement of the reference loop (integrator.rs,
+    constraint.rs, cpu_backend.rs with T static strips) on the same system:
+    ms per step over `steps` steps after the first list build and forces."""
+    from oracle import oracle as orc
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    x, y, z, vx, vy, vz = (a.copy() for a in (x, y, z, vx, vy, vz))
+    bx = (box,) * 3
+    rc, rs, rsw, dt, mi, tol = 12.0, 14.0, 10.0, 0.001, 100, 1e-8
+    orc.wrap_positions(x, y, z, bx)
+    pi, pj = orc.neighbor_build(x, y, z, bx, rs, topo.excl_i, topo.excl_j, method="cells")
+    rx, ry, rz = x.copy(), y.copy(), z.copy()
+    t, fx, fy, fz = orc.cpu_backend_compute(x, y, z, sig, eps, topo, pi, pj, bx, rc, rsw, n_threads=cores,
+                                            periodic=True)
+    orc.constrain_velocities(x, y, z, vx, vy, vz, topo, mass, mi, tol)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        orc.half_kick(vx, vy, vz, fx, fy, fz, mass, dt / 2)
+        orc.drift_and_constrain(x, y, z, vx, vy, vz, topo, mass, dt, mi, tol)
+        if orc.needs_rebuild(x, y, z, rx, ry, rz, rs, rc):
+            orc.wrap_positions(x, y, z, bx)
+            pi, pj = orc.neighbor_build(x, y, z, bx, rs, topo.excl_i, topo.excl_j, method="cells")
+            rx, ry, rz = x.copy(), y.copy(), z.copy()
+        t, fx, fy, fz = orc.cpu_backend_compute(x, y, z, sig, eps, topo, pi, pj, bx, rc, rsw,
+                                                n_threads=cores, periodic=True)
+        orc.half_kick(vx, vy, vz, fx, fy, fz, mass, dt / 2)
+        orc.constrain_velocities(x, y, z, vx, vy, vz, topo, mass, mi, tol)
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    return {"ms_per_step": ms, "cores": cores, "kind": "port",
+            "sample": f"{steps} steps of the same loop after the first list build and forces"}
+
+
 def pruning_study(g, s, x, y, z, e, args):
     """Tile pruning skips tiles that lie beyond r_c at the current positions,
     bounded by the largest displacement since the build: the same evaluation
@@ -292,9 +382,10 @@ def main():
     ap.add_argument("--no-250k", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-md", action="store_true")
     ap.add_argument("--no-pruning-study", action="store_true",
                     help="skip the pruning-off and drifted-position timings (profiling)")
-    ap.add_argument("--only", choices=("250k", "batched", "surrogate"), default=None,
+    ap.add_argument("--only", choices=("250k", "batched", "surrogate", "md"), default=None,
                     help="run one secondary section alone and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -310,6 +401,9 @@ def main():
         return
     if args.only == "surrogate":
         print(json.dumps(bench_surrogate()))
+        return
+    if args.only == "md":
+        print(json.dumps(bench_md(cpu=not args.no_cpu_baseline)))
         return
 
     import geodesic_hip as gh
@@ -386,6 +480,8 @@ def main():
         out["batched"] = bb
     if not args.no_surrogate:
         out["surrogate"] = bench_surrogate()
+    if not args.no_md:
+        out["md"] = bench_md(cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(s, n_cut)
         out["cpu_baseline"] = cb

@@ -4,11 +4,13 @@
 // root) with forces from the GPU force path and frames streamed to the host.
 //
 // The SHAKE / RATTLE iterations run without a host round trip per iteration:
-// after every iteration a one-thread kernel compares the largest multiplier
-// with the tolerance and raises a device flag, which makes the remaining
-// launched iterations return at once; the host reads the flag every
-// kCheckEvery iterations.  The result is the reference's: same iterations,
-// same stopping point.
+// each iteration's kernel first compares the previous iteration's largest
+// multiplier with the tolerance and raises a device flag, which makes the
+// remaining launched iterations return at once (before a read-back, a
+// one-thread kernel makes the same test for the last launched iteration).  The host first reads the flag after
+// as many iterations as the previous solve of the same kind needed (plus
+// two), then every kCheckEvery iterations: about one round trip per solve.
+// The result is the reference's: same iterations, same stopping point.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -29,6 +31,7 @@ struct gm_dyn {
     DevBuf<unsigned long long> maxslot;
     DevBuf<int> flag;
     int max_iter_alloc = 0;
+    int last_iters[2] = {0, 0};  // iterations the previous position / velocity solve took
 };
 
 void gm_dyn_free(gm_dyn *d) {
@@ -107,11 +110,13 @@ gm_status project(gm_sys *s, bool velocity, Vec3 buf0, Vec3C other, int max_iter
     for (int it = 0; it < max_iter; it++) {
         const Vec3 in = (it & 1) ? buf1 : buf0, out = (it & 1) ? buf0 : buf1;
         if (velocity)
-            GM_HIP(launch_rattle_iter(D, other, in, out, d->maxslot.p + it, d->lambda.p, d->flag.p, st));
+            GM_HIP(launch_rattle_iter(D, other, in, out, d->maxslot.p, it, tol, d->lambda.p, d->flag.p, st));
         else
-            GM_HIP(launch_shake_iter(D, in, out, other, d->maxslot.p + it, d->lambda.p, d->flag.p, st));
-        GM_HIP(launch_check(d->maxslot.p, tol, it, d->flag.p, st));
-        if ((it + 1) % kCheckEvery == 0 || it + 1 == max_iter) {
+            GM_HIP(launch_shake_iter(D, in, out, other, d->maxslot.p, it, tol, d->lambda.p, d->flag.p, st));
+        const int first = d->last_iters[velocity ? 1 : 0] + 2;  // the first read-back
+        if (it + 1 == first || (it + 1 > first && (it + 1 - first) % kCheckEvery == 0) || it + 1 == max_iter) {
+            // the test of this iteration (the next one's kernel would make it)
+            GM_HIP(launch_check(d->maxslot.p, tol, it, d->flag.p, st));
             GM_HIP(hipMemcpyAsync(flag, d->flag.p, sizeof(flag), hipMemcpyDeviceToHost, st));
             GM_HIP(hipStreamSynchronize(st));
             if (flag[0]) break;
@@ -124,6 +129,7 @@ gm_status project(gm_sys *s, bool velocity, Vec3 buf0, Vec3C other, int max_iter
         GM_HIP(hipMemcpyAsync(buf0.z, buf1.z, b, hipMemcpyDeviceToDevice, st));
     }
     if (iters) *iters = flag[1];
+    d->last_iters[velocity ? 1 : 0] = flag[1];
     if (!flag[0]) {
         std::vector<double> lam((size_t)d->nc);
         std::vector<uint32_t> hi(1), hj(1);

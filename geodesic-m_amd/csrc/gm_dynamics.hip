@@ -73,11 +73,29 @@ __device__ inline void wave_atomic_max_bits(unsigned long long *slot, unsigned l
         __hip_atomic_fetch_max(slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The convergence test of the previous iteration (k_check's, made by the
+// next iteration's kernel instead of a launch of its own): true if this
+// iteration is not to run, because an earlier one converged (flag[0]) or the
+// previous one did (its largest |multiplier| below tol; then the first
+// thread records it as k_check would).  The same for every thread.
+__device__ inline bool iteration_skipped(const unsigned long long *prev, double tol, int it, int *flag) {
+    if (flag[0]) return true;
+    if (prev && __longlong_as_double((long long)*prev) < tol) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            flag[1] = it;
+            flag[2] = it & 1;
+            flag[0] = 1;
+        }
+        return true;
+    }
+    return false;
+}
+
 // One SHAKE iteration (constraint.rs:97-150) from positions `in` to `out`.
-// Skipped entirely once an earlier iteration has converged (flag[0] != 0).
+// Skipped entirely once an earlier iteration has converged.
 __global__ void k_shake_iter(DynArgs D, Vec3 in, Vec3 out, Vec3C ref, unsigned long long *maxslot,
-                             double *lambda, const int *flag) {
-    if (flag[0]) return;  // the same for every thread
+                             const unsigned long long *prev, double tol, int it, double *lambda, int *flag) {
+    if (iteration_skipped(prev, tol, it, flag)) return;  // the same for every thread
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = a < D.n;
     double ddx = 0.0, ddy = 0.0, ddz = 0.0;
@@ -119,8 +137,8 @@ __global__ void k_shake_iter(DynArgs D, Vec3 in, Vec3 out, Vec3C ref, unsigned l
 
 // One RATTLE velocity iteration (constraint.rs:196-250) from velocities `in` to `out`.
 __global__ void k_rattle_iter(DynArgs D, Vec3C pos, Vec3 in, Vec3 out, unsigned long long *maxslot,
-                              double *lambda, const int *flag) {
-    if (flag[0]) return;  // the same for every thread
+                              const unsigned long long *prev, double tol, int it, double *lambda, int *flag) {
+    if (iteration_skipped(prev, tol, it, flag)) return;  // the same for every thread
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = a < D.n;
     double ddx = 0.0, ddy = 0.0, ddz = 0.0;
@@ -179,6 +197,10 @@ __global__ void k_kinetic_terms(int64_t n, const double *vx, const double *vy, c
 }
 
 inline unsigned grid_of(int64_t n) { return (unsigned)((n + 255) / 256); }
+// the constraint iterations: one wavefront per workgroup, so a system of a
+// few tens of thousands of atoms spreads over every CU (each thread is a
+// short chain of dependent loads)
+inline unsigned grid_of64(int64_t n) { return (unsigned)((n + 63) / 64); }
 
 }  // namespace
 
@@ -207,17 +229,19 @@ hipError_t launch_resync(int64_t n, const double *x, const double *y, const doub
     return hipGetLastError();
 }
 
-hipError_t launch_shake_iter(const DynArgs &D, Vec3 in, Vec3 out, Vec3C ref, unsigned long long *maxslot,
-                             double *lambda, const int *flag, hipStream_t st) {
+hipError_t launch_shake_iter(const DynArgs &D, Vec3 in, Vec3 out, Vec3C ref, unsigned long long *maxslot, int it,
+                             double tol, double *lambda, int *flag, hipStream_t st) {
     if (D.n == 0) return hipSuccess;
-    k_shake_iter<<<grid_of(D.n), 256, 0, st>>>(D, in, out, ref, maxslot, lambda, flag);
+    k_shake_iter<<<grid_of64(D.n), 64, 0, st>>>(D, in, out, ref, maxslot + it, it > 0 ? maxslot + it - 1 : nullptr,
+                                                tol, it, lambda, flag);
     return hipGetLastError();
 }
 
-hipError_t launch_rattle_iter(const DynArgs &D, Vec3C pos, Vec3 in, Vec3 out, unsigned long long *maxslot,
-                              double *lambda, const int *flag, hipStream_t st) {
+hipError_t launch_rattle_iter(const DynArgs &D, Vec3C pos, Vec3 in, Vec3 out, unsigned long long *maxslot, int it,
+                              double tol, double *lambda, int *flag, hipStream_t st) {
     if (D.n == 0) return hipSuccess;
-    k_rattle_iter<<<grid_of(D.n), 256, 0, st>>>(D, pos, in, out, maxslot, lambda, flag);
+    k_rattle_iter<<<grid_of64(D.n), 64, 0, st>>>(D, pos, in, out, maxslot + it, it > 0 ? maxslot + it - 1 : nullptr,
+                                                 tol, it, lambda, flag);
     return hipGetLastError();
 }
 

@@ -32,10 +32,12 @@ hipError_t launch_drift(int64_t n, double *x, double *y, double *z, const double
 hipError_t launch_resync(int64_t n, const double *x, const double *y, const double *z, const double *rx,
                          const double *ry, const double *rz, double *vx, double *vy, double *vz, double dt,
                          hipStream_t st);
-hipError_t launch_shake_iter(const DynArgs &D, Vec3 in, Vec3 out, Vec3C ref, unsigned long long *maxslot,
-                             double *lambda, const int *flag, hipStream_t st);
-hipError_t launch_rattle_iter(const DynArgs &D, Vec3C pos, Vec3 in, Vec3 out, unsigned long long *maxslot,
-                              double *lambda, const int *flag, hipStream_t st);
+// One iteration `it` (maxslot: the per-iteration maxima, from iteration 0);
+// it first makes the convergence test of iteration it - 1.
+hipError_t launch_shake_iter(const DynArgs &D, Vec3 in, Vec3 out, Vec3C ref, unsigned long long *maxslot, int it,
+                             double tol, double *lambda, int *flag, hipStream_t st);
+hipError_t launch_rattle_iter(const DynArgs &D, Vec3C pos, Vec3 in, Vec3 out, unsigned long long *maxslot, int it,
+                              double tol, double *lambda, int *flag, hipStream_t st);
 hipError_t launch_check(const unsigned long long *maxslot, double tol, int it, int *flag, hipStream_t st);
 hipError_t launch_kinetic_terms(int64_t n, const double *vx, const double *vy, const double *vz,
                                 const double *mass, double *out, hipStream_t st);

@@ -220,3 +220,38 @@ def replica_set(n_replicas=512, lo=500, hi=20000, seed=3, density=0.1):
         out.append(protein_solvent(int(n), float(box), int(n) // 8, seed=seed * 100003 + k,
                                    name=f"replica{k}"))
     return out
+
+
+_TOPO_INDEX = ("bond_i", "bond_j", "angle_i", "angle_j", "angle_k", "dihed_i", "dihed_j", "dihed_k", "dihed_l",
+               "excl_i", "excl_j", "constr_i", "constr_j", "pair14_i", "pair14_j")
+_TOPO_VALUE = ("bond_k", "bond_r0", "angle_kth", "angle_th0", "dihed_kphi", "dihed_n", "dihed_delta",
+               "constr_dsq")
+
+
+def replicated_molecule(fields, pos, k, spacing):
+    """k^3 copies of one molecule on a cubic grid of the given spacing (A),
+    in a periodic box of edge k * spacing.  `fields`: the molecule's AtomData
+    and BondedTopology arrays by field name (sigma, epsilon, mass and the
+    topology fields, as the host layer's prmtop reader gives them); `pos`:
+    its positions, 3 x n.  Returns (x, y, z, sigma, epsilon, mass, Topology,
+    box).  Atom indices of copy c are the molecule's plus c n; every copy sits
+    inside its own cell, so no bonded term crosses the box boundary."""
+    n = len(fields["sigma"])
+    nc = k ** 3
+    p = np.asarray(pos, np.float64)
+    centre = 0.5 * (p.min(axis=1) + p.max(axis=1))
+    g = np.stack(np.meshgrid(np.arange(k), np.arange(k), np.arange(k), indexing="ij"), -1).reshape(-1, 3)
+    shift = (g + 0.5) * spacing - centre  # (nc, 3)
+    x, y, z = (np.ascontiguousarray((p[a][None, :] + shift[:, a:a + 1]).reshape(-1)) for a in range(3))
+    rep = lambda v: np.ascontiguousarray(np.tile(np.asarray(v), nc))  # noqa: E731
+    off = lambda v: np.ascontiguousarray(  # noqa: E731
+        (np.asarray(v, np.int64)[None, :] + (np.arange(nc, dtype=np.int64) * n)[:, None]).reshape(-1).astype(np.uint32))
+    t = {}
+    for name in _TOPO_INDEX:
+        t[name] = off(fields.get(name, np.zeros(0, np.uint32)))
+    for name in _TOPO_VALUE:
+        v = fields.get(name, np.zeros(0))
+        t[name] = rep(v).astype(np.uint32 if name == "dihed_n" else np.float64)
+    topo = Topology(**t)
+    return (x, y, z, rep(fields["sigma"]).astype(np.float64), rep(fields["epsilon"]).astype(np.float64),
+            rep(fields["mass"]).astype(np.float64), topo, float(k * spacing))
