@@ -99,6 +99,31 @@ def test_graph_replays_are_byte_identical(batch_case):
         assert [t.total for t in e] == [t.total for t in e0]
 
 
+@pytest.mark.parametrize("entries", [6, 32])
+def test_work_shape_equals_each_replica_alone(batch_case, entries):
+    """Work items of 6 entries leave i forces as per-item records, 32 as
+    atomics (and the finish kernel then takes one wavefront per
+    supercluster): either way each replica's forces are byte-identical to
+    the replica evaluated alone with the same work items."""
+    reps, b, (x, y, z), e0, f0 = batch_case
+    say(f"[gpu] batch shape {entries} entries per item")
+    b.set_shape(entries, 16)
+    b.set_positions(x, y, z)
+    b.compute()
+    e, _ = b.sync()
+    fx, fy, fz = b.get_forces()
+    for k, r in enumerate(reps):
+        g = gh.System(r.sigma, r.epsilon, r.topo, r.params)
+        px, py, pz = r.x.copy(), r.y.copy(), r.z.copy()
+        g.build_neighbor_list(px, py, pz)
+        g.set_work_chunk(entries)
+        ep, gx, gy, gz = g.compute_forces(px, py, pz)
+        o0, o1 = b.offsets[k], b.offsets[k + 1]
+        assert gx.tobytes() == fx[o0:o1].tobytes() and gy.tobytes() == fy[o0:o1].tobytes()
+        assert gz.tobytes() == fz[o0:o1].tobytes() and ep.total == e[k].total
+    b.set_shape(0, 16)  # back to the automatic shape for the tests after this one
+
+
 def test_autotuned_shape_is_pinned_and_reproducible(batch_case):
     reps, b, (x, y, z), e0, f0 = batch_case
     say("[gpu] autotune")
