@@ -154,6 +154,59 @@ __global__ void k_scatter(int n, const uint32_t *q, const int *col_start, int *c
     if (active) bucket[col_start[c] + slot] = i;
 }
 
+// The same two passes with a per-workgroup column histogram in LDS (up to
+// kLdsColumns columns): one global atomic per (workgroup, column) instead of
+// one per (wavefront, column), all issued at once rather than one returning
+// atomic after another.
+__global__ __launch_bounds__(1024) void k_wrap_quantize_lds(int n, double *x, double *y, double *z, double *rx,
+                                                           double *ry, double *rz, uint32_t *q, int *col_count,
+                                                           BuildParams bp, int wrap) {
+    extern __shared__ int hist[];
+    const int t = threadIdx.x, i = blockIdx.x * 1024 + t;
+    for (int c = t; c < bp.ncol; c += 1024) hist[c] = 0;
+    __syncthreads();
+    if (i < n) {
+        double xi = x[i], yi = y[i], zi = z[i];
+        if (wrap) {
+            xi = wrap_coord(xi, bp.box[0]);
+            yi = wrap_coord(yi, bp.box[1]);
+            zi = wrap_coord(zi, bp.box[2]);
+            x[i] = xi; y[i] = yi; z[i] = zi;
+        }
+        rx[i] = xi; ry[i] = yi; rz[i] = zi;
+        const uint32_t qx = quantize(xi, bp.inv_q[0]), qy = quantize(yi, bp.inv_q[1]), qz = quantize(zi, bp.inv_q[2]);
+        q[i] = qx; q[n + i] = qy; q[2 * n + i] = qz;
+        const int cx = (int)(((uint64_t)qx * (uint64_t)bp.ncx) >> 32);
+        const int cy = (int)(((uint64_t)qy * (uint64_t)bp.ncy) >> 32);
+        atomicAdd(&hist[cx * bp.ncy + cy], 1);
+    }
+    __syncthreads();
+    for (int c = t; c < bp.ncol; c += 1024)
+        if (hist[c]) atomicAdd(&col_count[c], hist[c]);
+}
+
+__global__ __launch_bounds__(1024) void k_scatter_lds(int n, const uint32_t *q, const int *col_start, int *col_fill,
+                                                     int *bucket, BuildParams bp) {
+    extern __shared__ int hist[];  // counts, then each column's base for this workgroup
+    const int t = threadIdx.x, i = blockIdx.x * 1024 + t;
+    for (int c = t; c < bp.ncol; c += 1024) hist[c] = 0;
+    __syncthreads();
+    int c = 0, rank = 0;
+    if (i < n) {
+        const int cx = (int)(((uint64_t)q[i] * (uint64_t)bp.ncx) >> 32);
+        const int cy = (int)(((uint64_t)q[n + i] * (uint64_t)bp.ncy) >> 32);
+        c = cx * bp.ncy + cy;
+        rank = atomicAdd(&hist[c], 1);
+    }
+    __syncthreads();
+    for (int k = t; k < bp.ncol; k += 1024) {
+        const int h = hist[k];
+        if (h) hist[k] = atomicAdd(&col_fill[k], h);
+    }
+    __syncthreads();
+    if (i < n) bucket[col_start[c] + hist[c] + rank] = i;
+}
+
 // One workgroup per column: bitonic sort of (z, atom) in LDS, which fixes the
 // order independently of the scatter's atomics; cut into superclusters of 64.
 __global__ __launch_bounds__(1024) void k_column_sort(int n, const uint32_t *q, const int *bucket,
@@ -683,15 +736,26 @@ hipError_t launch_build(gm_sys *s, bool wrap) {
     if ((e = launch_fill_i32(s->col_count.p, bp.ncol, 0, st)) != hipSuccess) return e;
     if ((e = launch_fill_i32(s->col_fill.p, bp.ncol, 0, st)) != hipSuccess) return e;
     if ((e = launch_fill_i32(s->meta.p, 8, 0, st)) != hipSuccess) return e;
-    if (n > 0)
-        k_wrap_quantize<<<grid_for(n, 256), 256, 0, st>>>(n, s->x.p, s->y.p, s->z.p, s->refx.p,
-                                                            s->refy.p, s->refz.p, s->qatom.p,
-                                                            s->col_count.p, bp, wrap ? 1 : 0);
+    if (n > 0) {
+        if (bp.ncol <= kLdsColumns)
+            k_wrap_quantize_lds<<<grid_for(n, 1024), 1024, sizeof(int) * (size_t)bp.ncol, st>>>(
+                n, s->x.p, s->y.p, s->z.p, s->refx.p, s->refy.p, s->refz.p, s->qatom.p, s->col_count.p, bp,
+                wrap ? 1 : 0);
+        else
+            k_wrap_quantize<<<grid_for(n, 256), 256, 0, st>>>(n, s->x.p, s->y.p, s->z.p, s->refx.p,
+                                                                s->refy.p, s->refz.p, s->qatom.p,
+                                                                s->col_count.p, bp, wrap ? 1 : 0);
+    }
     k_scan_columns<<<1, 1024, 0, st>>>(bp.ncol, s->col_count.p, s->col_start.p, s->col_sc.p,
                                        s->meta.p, kColumnCap);
-    if (n > 0)
-        k_scatter<<<grid_for(n, 256), 256, 0, st>>>(n, s->qatom.p, s->col_start.p, s->col_fill.p,
-                                                      s->bucket.p, bp);
+    if (n > 0) {
+        if (bp.ncol <= kLdsColumns)
+            k_scatter_lds<<<grid_for(n, 1024), 1024, sizeof(int) * (size_t)bp.ncol, st>>>(
+                n, s->qatom.p, s->col_start.p, s->col_fill.p, s->bucket.p, bp);
+        else
+            k_scatter<<<grid_for(n, 256), 256, 0, st>>>(n, s->qatom.p, s->col_start.p, s->col_fill.p,
+                                                          s->bucket.p, bp);
+    }
     k_column_sort<<<bp.ncol, 1024, kColumnCap * sizeof(unsigned long long), st>>>(
         n, s->qatom.p, s->bucket.p, s->col_start.p, s->col_count.p, s->col_sc.p,
         s->sorted_atoms.p, s->sc_base.p, s->sc_m.p, s->sc_col.p);

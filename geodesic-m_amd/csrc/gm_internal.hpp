@@ -24,6 +24,8 @@ constexpr int kColumnCap = 8192;
 constexpr int kMaxRun = 32;
 // list build: workgroups (of four wavefronts) of the pair-mask kernel, at most
 constexpr int kMaskBlocks = 8192;
+// list build: column histograms in LDS (32 KB) up to this many xy columns
+constexpr int kLdsColumns = 8192;
 // Fixed-point force unit: 2^-32 kcal/mol/A per integer step.
 constexpr double kFixedScale = 4294967296.0;
 constexpr double kFixedInv = 1.0 / 4294967296.0;
