@@ -332,16 +332,15 @@ gm_status compute_on_device(gm_sys *s, const double *x, const double *y, const d
         }
     }
     // Unprofiled evaluations record no events: the stream holds only kernels.
-    // Profiled ones: an event at the start and one after the finish kernel;
-    // the force kernel's dispatch stamps its own start and end (ev[1], ev[2]).
-    if (ev[0]) GM_HIP(hipEventRecord(ev[0], st));
+    // Profiled ones carry no separate event packets either: the dispatches
+    // stamp them, ev[0] at the prep kernel's start, ev[1] / ev[2] at the force
+    // kernel's start and end, ev[3] at the finish kernel's end.
     if (s->F_dirty) GM_HIP(launch_fill_u64(s->F.p, (size_t)s->nsc * 64 * 3, st));
     s->F_dirty = true;
-    GM_HIP(launch_prep(s, x, y, z));
+    GM_HIP(launch_prep(s, x, y, z, ev[0]));
     GM_HIP(launch_forces(s, x, y, z, shell, ev[1], ev[2]));
-    GM_HIP(launch_finish(s, fx, fy, fz, shell));  // also clears the accumulator
+    GM_HIP(launch_finish(s, fx, fy, fz, shell, ev[3]));  // also clears the accumulator
     s->F_dirty = false;
-    if (ev[3]) GM_HIP(hipEventRecord(ev[3], st));
     s->last_timed = ev[0] ? s->prof_used - 1 : -1;
     return GM_OK;
 }

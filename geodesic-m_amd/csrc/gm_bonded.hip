@@ -6,6 +6,8 @@
 // terms, then per-thread LJ partials, in a fixed order: cpu_backend.rs:93-166;
 // paths relative to the reference repository root).  Forces leave the
 // fixed-point accumulator as value * 2^-32 kcal/mol/A.
+#include <hip/hip_ext.h>
+
 #include "gm_internal.hpp"
 #include "gm_terms.hpp"
 
@@ -166,7 +168,7 @@ hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int2 *
     return hipGetLastError();
 }
 
-hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz, bool lj_only) {
+hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz, bool lj_only, hipEvent_t ev_stop) {
     Segs sg{};
     const double *eb = s->e_items.p;
     const int64_t n14 = (s->scale14 != 0.0) ? s->n14 : 0;
@@ -183,9 +185,14 @@ hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz, bool lj_
     const int G = per_sc > 8 ? 16 : (per_sc > 4 ? 8 : 4);
     const unsigned grid = 5u + (unsigned)((s->nsc + 16 / G - 1) / (16 / G));
     const float fi_scale = span_is_cubic(s) ? s->nbp.fscale_u : 4294967296.0f;
-    k_finish<<<grid, 1024, 0, s->stream>>>(sg, s->e_out.p, s->slot_atom.p, s->F.p, n_pad,
-                                            s->fi_records ? s->fi_items.p : nullptr,
-                                            s->sc_wi_off.p, s->sc_wi.p, fi_scale, G, fx, fy, fz, s->disp.p);
+    const float *fi = s->fi_records ? s->fi_items.p : nullptr;
+    if (ev_stop)  // stamped by the dispatch at the kernel's end
+        hipExtLaunchKernelGGL(k_finish, dim3(grid), dim3(1024), 0, s->stream, nullptr, ev_stop, 0, sg, s->e_out.p,
+                              (const int *)s->slot_atom.p, s->F.p, n_pad, fi, (const int *)s->sc_wi_off.p,
+                              (const int *)s->sc_wi.p, fi_scale, G, fx, fy, fz, s->disp.p);
+    else
+        k_finish<<<grid, 1024, 0, s->stream>>>(sg, s->e_out.p, s->slot_atom.p, s->F.p, n_pad, fi, s->sc_wi_off.p,
+                                                s->sc_wi.p, fi_scale, G, fx, fy, fz, s->disp.p);
     return hipGetLastError();
 }
 

@@ -307,7 +307,7 @@ hipError_t launch_needs_rebuild(gm_sys *s, const double *x, const double *y, con
 // gm_nonbonded.hip: tiles + bonded terms in one launch
 // (shell: tiles only, with nbp_shell)
 // (after launch_prep: slot-order fixed-point positions and the displacement bound)
-hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double *z);
+hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double *z, hipEvent_t ev_start = nullptr);
 // fills the work items' first j-cluster from the list entries
 hipError_t launch_work_item_cj0(gm_sys *s);
 // ev_start / ev_stop (optional): stamped by the dispatch at the kernel's start and end
@@ -315,7 +315,8 @@ hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const doub
                          hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
 // gm_bonded.hip: energy reduction, f64 output, accumulator clear (lj_only:
 // the bonded energy segments are empty, as after a shell-mode launch)
-hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz, bool lj_only = false);
+hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz, bool lj_only = false,
+                         hipEvent_t ev_stop = nullptr);
 }  // namespace gm
 namespace gm {
 struct ReplicaDesc;

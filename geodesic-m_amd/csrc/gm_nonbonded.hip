@@ -507,8 +507,13 @@ PrepArgs prep_args(gm_sys *s, const double *x, const double *y, const double *z)
 
 // Slot-order fixed-point positions and the displacement bound; precedes every
 // launch_forces on the same positions.
-hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double *z) {
-    if (s->n > 0) k_prep<<<(unsigned)((s->n + 255) / 256), 256, 0, s->stream>>>(prep_args(s, x, y, z));
+hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double *z, hipEvent_t ev_start) {
+    if (s->n == 0) return ev_start ? hipEventRecord(ev_start, s->stream) : hipSuccess;
+    const unsigned grid = (unsigned)((s->n + 255) / 256);
+    if (ev_start)  // stamped by the dispatch at the kernel's start
+        hipExtLaunchKernelGGL(k_prep, dim3(grid), dim3(256), 0, s->stream, ev_start, nullptr, 0, prep_args(s, x, y, z));
+    else
+        k_prep<<<grid, 256, 0, s->stream>>>(prep_args(s, x, y, z));
     return hipGetLastError();
 }
 

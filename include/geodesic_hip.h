@@ -171,20 +171,23 @@ typedef struct {
 } gm_stats;
 gm_status gm_get_stats(gm_sys *sys, gm_stats *out);
 
-/* Per-evaluation device timings, recorded with HIP events on the system's
- * stream between gm_profile_begin and gm_profile_end: the first evaluation
- * and every `every`-th one after it are timed (at most 8192; later ones are
- * counted in n_untimed), the others record nothing.  Outside a profiling
- * session an evaluation records no events at all. */
+/* Per-evaluation device timings between gm_profile_begin and gm_profile_end:
+ * the first evaluation and every `every`-th one after it are timed (at most
+ * 8192; later ones are counted in n_untimed), the others record nothing.
+ * Timed evaluations add no event packets to the stream: the kernels'
+ * dispatches stamp the HIP events themselves (the prep kernel's start, the
+ * nonbonded kernel's start and end, the finish kernel's end).  Outside a
+ * profiling session an evaluation records no events at all. */
 typedef struct {
     int64_t n_evals;            /* evaluations timed */
     int64_t n_untimed;          /* evaluations past the event pool */
     double ms_nonbonded_total;  /* sum over timed evaluations, nonbonded kernel (its dispatch's own
                                    start and end stamps) */
     double ms_nonbonded_min, ms_nonbonded_max;
-    double ms_eval_total;       /* sum over timed evaluations, whole evaluation */
-    double ms_prep_total;       /* sum over timed evaluations, evaluation start to the nonbonded
-                                   kernel's start (accumulator clear, position prep kernel) */
+    double ms_eval_total;       /* sum over timed evaluations, prep kernel's start to the finish
+                                   kernel's end (the accumulator clear before it is not included) */
+    double ms_prep_total;       /* sum over timed evaluations, prep kernel's start to the nonbonded
+                                   kernel's start */
 } gm_profile;
 gm_status gm_profile_begin(gm_sys *sys, int32_t every);
 gm_status gm_profile_end(gm_sys *sys, gm_profile *out);
