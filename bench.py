@@ -29,7 +29,7 @@ import numpy as np  # noqa: E402
 FLOP_PER_PAIR = 40          # canonical count per pair interaction (FMA = 2)
 PEAK_FP32_TFLOPS = 157.3    # MI355X published FP32 peak
 PEAK_HBM_GBS = 8000.0
-PROFILE_EVERY = 10          # timed region: HIP events on every 10th evaluation
+PROFILE_EVERY = 20          # timed region: HIP events on every 20th evaluation
 
 
 def traffic_from_profiles(name):

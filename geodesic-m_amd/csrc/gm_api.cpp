@@ -89,6 +89,8 @@ gm_status make_work_items(gm_sys *s) {
         chunk = std::max<int64_t>(4, chunk);
     }
     chunk = std::min<int64_t>(chunk, kMaxRun);
+    if (s->nsc > (int64_t)kWiScMask + 1)
+        return set_error(GM_ERR_CAPACITY, "work items: more than 2^26 superclusters");
     std::vector<Item> items;
     items.reserve((size_t)(s->nent / chunk + s->nsc + 1));
     for (int64_t k = 0; k < s->nsc; k++)
@@ -108,21 +110,23 @@ gm_status make_work_items(gm_sys *s) {
     std::vector<int4> wi(ni);
     for (const Item &it : items) {
         const size_t o = (size_t)first[(size_t)(cmax - it.cost)]++;
-        wi[o] = make_int4(it.sc, it.e0, it.e1, 0);
+        // (supercluster | entries << 26, first entry, record slot, j-cluster of the first entry)
+        wi[o] = make_int4(it.sc | ((it.e1 - it.e0) << kWiEntryShift), it.e0, 0, 0);
     }
     s->n_wi = (int64_t)wi.size();
-    // each supercluster's work items, ascending (the finish kernel adds their i forces)
-    std::vector<int> off((size_t)s->nsc + 1, 0), lst(ni);
-    for (const int4 &v : wi) off[(size_t)v.x + 1]++;
+    // Each supercluster's work items in ascending order own consecutive record
+    // slots, so the finish kernel reads a supercluster's i-force records as
+    // one contiguous range.
+    std::vector<int> off((size_t)s->nsc + 1, 0);
+    for (const Item &it : items) off[(size_t)it.sc + 1]++;
     for (size_t k = 1; k < off.size(); k++) off[k] += off[k - 1];
     {
         std::vector<int> fill(off.begin(), off.end() - 1);
-        for (size_t w = 0; w < ni; w++) lst[(size_t)fill[(size_t)wi[w].x]++] = (int)w;
+        for (size_t w = 0; w < ni; w++) wi[w].z = fill[(size_t)(wi[w].x & kWiScMask)]++;
     }
     hipStream_t st = s->stream;
     GM_HIP(upload(s->wi, wi.data(), wi.size(), st));
     GM_HIP(upload(s->sc_wi_off, off.data(), off.size(), st));
-    GM_HIP(upload(s->sc_wi, lst.data(), lst.size(), st));
     // Small work items: many i-force updates per slot, which as atomics (past
     // the L2) cost the force kernel more than plain per-item records cost the
     // finish kernel; large ones: few, and atomics are cheaper.
@@ -691,7 +695,6 @@ gm_status gm_system_destroy(gm_sys *s) {
     s->wi.release();
     s->fi_items.release();
     s->sc_wi_off.release();
-    s->sc_wi.release();
     s->cl_ref.release();
     s->sc_ref.release();
     s->cl_half.release();

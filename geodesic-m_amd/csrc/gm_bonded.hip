@@ -21,23 +21,32 @@ struct Segs {
     int nseg;
 };
 
-// Fixed-order f64 sum of one segment by a 1024-thread block: thread t adds
-// items t, t + 1024, ... into four running sums (items 4k + 0..3 apart by
-// 1024, so four loads are in flight), combines them, each wavefront sums its
-// lanes with DPP and permlane steps, and thread 0 adds the 16 wave sums in
-// order.  The result is valid in thread 0.
+// Fixed-order f64 sum of one segment by a 1024-thread block: in round r,
+// thread t adds item 16384 r + 1024 k + t to running sum k (k = 0..15; an item
+// past the end adds 0), so sixteen loads are in flight per round (the loads
+// are unconditional, at an index clamped into the segment).  The sixteen
+// sums combine as a pairwise tree, each wavefront sums its lanes with DPP and
+// permlane steps, and thread 0 adds the 16 wave sums in order.  The result is
+// valid in thread 0.
 __device__ inline double block_sum(const double *ptr, long long len, double *sh) {
+    constexpr int K = 8;
     const int t = threadIdx.x;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    long long i = t;
-    for (; i + 3072 < len; i += 4096) {
-        a0 += ptr[i];
-        a1 += ptr[i + 1024];
-        a2 += ptr[i + 2048];
-        a3 += ptr[i + 3072];
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] = 0.0;
+    for (long long base = 0; base < len; base += K * 1024) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const long long i = base + k * 1024 + t;
+            const double v = ptr[i < len ? i : len - 1];
+            acc[k] += i < len ? v : 0.0;
+        }
     }
-    for (; i < len; i += 1024) a0 += ptr[i];
-    double a = wave_sum_f64((a0 + a1) + (a2 + a3));
+#pragma unroll
+    for (int w = K / 2; w >= 1; w /= 2)
+#pragma unroll
+        for (int k = 0; k < w; k++) acc[k] = acc[k] + acc[k + w];
+    double a = wave_sum_f64(acc[0]);
     if ((t & 63) == 0) sh[t >> 6] = a;
     __syncthreads();
     double r = 0.0;
@@ -61,16 +70,26 @@ __device__ inline long long to_fixed(float v, float scale) { return (long long)_
 // thread of the workgroup calls it (it holds a barrier); valid says whether
 // group g has a supercluster.
 __device__ inline void sc_finish(bool valid, int sc, int G, unsigned long long *F, size_t n_pad,
-                                 const float *fi_items, const int *sc_wi_off, const int *sc_wi, float fi_scale,
+                                 const float *fi_items, const int *sc_wi_off, float fi_scale,
                                  const int *slot_atom, int64_t atom_base, double *fx, double *fy, double *fz) {
     __shared__ long long red[16][3][64];
     const int t = threadIdx.x, wv = t >> 6, l = t & 63, q = wv & (G - 1);
+    const size_t s = (size_t)sc * 64 + l;
+    // the summing wavefront's accumulator words and atom index load with the records
+    unsigned long long f[3] = {0ull, 0ull, 0ull};
+    int a = -1;
+    if (valid && q == 0) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) f[c] = F[(size_t)c * n_pad + s];
+        a = slot_atom[s];
+    }
     long long ax = 0, ay = 0, az = 0;
     if (valid && fi_items) {  // (null: the i forces are already in the accumulator)
+        // the supercluster's records are consecutive: wavefront q adds every G-th
         const int k0 = sc_wi_off[sc], k1 = sc_wi_off[sc + 1];
 #pragma unroll 2
         for (int k = k0 + q; k < k1; k += G) {
-            const float *fw = fi_items + (size_t)sc_wi[k] * 192 + l;
+            const float *fw = fi_items + (size_t)k * 192 + l;
             ax += to_fixed(fw[0], fi_scale);
             ay += to_fixed(fw[64], fi_scale);
             az += to_fixed(fw[128], fi_scale);
@@ -81,15 +100,11 @@ __device__ inline void sc_finish(bool valid, int sc, int G, unsigned long long *
     red[wv][2][l] = az;
     __syncthreads();
     if (!valid || q != 0) return;
-    const size_t s = (size_t)sc * 64 + l;
-    unsigned long long f[3];
+#pragma unroll
     for (int c = 0; c < 3; c++) {
-        unsigned long long v = F[(size_t)c * n_pad + s];
-        for (int o = 0; o < G; o++) v += (unsigned long long)red[wv + o][c][l];
-        f[c] = v;
+        for (int o = 0; o < G; o++) f[c] += (unsigned long long)red[wv + o][c][l];
         F[(size_t)c * n_pad + s] = 0ull;
     }
-    const int a = slot_atom[s];
     if (a < 0) return;
     fx[atom_base + a] = (double)(long long)f[0] * kFixedInv;
     fy[atom_base + a] = (double)(long long)f[1] * kFixedInv;
@@ -102,7 +117,7 @@ __device__ inline void sc_finish(bool valid, int sc, int G, unsigned long long *
 __global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, const int *slot_atom,
                                                  unsigned long long *F, size_t n_pad,
                                                  const float *fi_items, const int *sc_wi_off,
-                                                 const int *sc_wi, float fi_scale, int G, double *fx,
+                                                 float fi_scale, int G, double *fx,
                                                  double *fy, double *fz, unsigned *disp) {
     __shared__ double sh[16];
     const int b = blockIdx.x, t = threadIdx.x;
@@ -113,7 +128,7 @@ __global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, const in
         return;
     }
     const int sc = (b - sg.nseg) * (16 / G) + (t >> 6) / G;
-    sc_finish((size_t)sc * 64 < n_pad, sc, G, F, n_pad, fi_items, sc_wi_off, sc_wi, fi_scale, slot_atom, 0,
+    sc_finish((size_t)sc * 64 < n_pad, sc, G, F, n_pad, fi_items, sc_wi_off, fi_scale, slot_atom, 0,
               fx, fy, fz);
 }
 
@@ -151,7 +166,7 @@ __global__ __launch_bounds__(1024) void k_batch_finish(const ReplicaDesc *__rest
     const bool valid = pidx < n_scl;
     const int2 rs = valid ? scl[pidx] : make_int2(0, 0);
     const ReplicaDesc &R = reps[rs.x];
-    sc_finish(valid, rs.y, G, R.terms.F, R.terms.n_pad, R.tile.fi_items, R.tile.sc_wi_off, R.tile.sc_wi,
+    sc_finish(valid, rs.y, G, R.terms.F, R.terms.n_pad, R.tile.fi_items, R.tile.sc_wi_off,
               R.tile.fi_scale, R.tile.slot_atom, atom_off[rs.x], fx, fy, fz);
 }
 
@@ -180,19 +195,20 @@ hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz, bool lj_
     sg.ptr[4] = s->e_wi.p;                  sg.len[4] = s->n_wi;
     sg.nseg = 5;
     const size_t n_pad = (size_t)s->nsc * 64;
-    // wavefronts per supercluster: about one work-item record each, 4 to 16
+    // wavefronts per supercluster: about one work-item record each, 4 to 16;
+    // one when there are no records to add
     const int64_t per_sc = (s->fi_records && s->nsc > 0) ? (s->n_wi + s->nsc - 1) / s->nsc : 1;
-    const int G = per_sc > 8 ? 16 : (per_sc > 4 ? 8 : 4);
+    const int G = !s->fi_records ? 1 : (per_sc > 8 ? 16 : (per_sc > 4 ? 8 : 4));
     const unsigned grid = 5u + (unsigned)((s->nsc + 16 / G - 1) / (16 / G));
     const float fi_scale = span_is_cubic(s) ? s->nbp.fscale_u : 4294967296.0f;
     const float *fi = s->fi_records ? s->fi_items.p : nullptr;
     if (ev_stop)  // stamped by the dispatch at the kernel's end
         hipExtLaunchKernelGGL(k_finish, dim3(grid), dim3(1024), 0, s->stream, nullptr, ev_stop, 0, sg, s->e_out.p,
                               (const int *)s->slot_atom.p, s->F.p, n_pad, fi, (const int *)s->sc_wi_off.p,
-                              (const int *)s->sc_wi.p, fi_scale, G, fx, fy, fz, s->disp.p);
+                              fi_scale, G, fx, fy, fz, s->disp.p);
     else
         k_finish<<<grid, 1024, 0, s->stream>>>(sg, s->e_out.p, s->slot_atom.p, s->F.p, n_pad, fi, s->sc_wi_off.p,
-                                                s->sc_wi.p, fi_scale, G, fx, fy, fz, s->disp.p);
+                                                fi_scale, G, fx, fy, fz, s->disp.p);
     return hipGetLastError();
 }
 

@@ -22,6 +22,10 @@ constexpr int kScSlots = 64;
 constexpr int kColumnCap = 8192;
 // Longest run of list entries one nonbonded wavefront processes.
 constexpr int kMaxRun = 32;
+// Work-item descriptor word 0: supercluster in the low 26 bits, entry count above.
+constexpr int kWiEntryShift = 26;
+constexpr int kWiScMask = (1 << kWiEntryShift) - 1;
+static_assert(kMaxRun < (1 << (32 - kWiEntryShift)), "entry count must fit above the supercluster");
 // list build: workgroups (of four wavefronts) of the pair-mask kernel, at most
 constexpr int kMaskBlocks = 8192;
 // list build: column histograms in LDS (32 KB) up to this many xy columns
@@ -230,7 +234,7 @@ struct gm_sys {
     DevBuf<int4> wi;          // work items (supercluster, first entry, end entry, first j-cluster)
     DevBuf<float> fi_items;               // i forces per work item (192 each, kernel force unit)
     bool fi_records = false;              // i forces as per-item records (small items) or atomics
-    DevBuf<int> sc_wi_off, sc_wi;         // per supercluster, its work items (CSR)
+    DevBuf<int> sc_wi_off;                // per supercluster, its first record slot (CSR offsets)
     int chunk = 0;  // entries per work item (0 = automatic)
 
     // accumulators

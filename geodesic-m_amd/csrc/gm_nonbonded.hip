@@ -80,6 +80,14 @@ __device__ inline float in_vgpr(float v) {
     return r;
 }
 
+// An SGPR holding the f32 bit pattern k, opaque to the optimiser (a constant
+// operand of a class test would be folded back into a mask-and-compare).
+__device__ inline float sgpr_bits(unsigned k) {
+    float r;
+    asm("s_mov_b32 %0, %1" : "=s"(r) : "i"(k));
+    return r;
+}
+
 template <int CTRL>
 __device__ inline float dpp_max(float v) {
     return fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false)));
@@ -189,11 +197,11 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     __shared__ int2 s_meta[kMaxRun];
     __shared__ float jf[3][kMaxRun * 8];
     const int il = lane & 7, jl = lane >> 3;
-    // (supercluster, first entry, end entry, j-cluster of the first entry):
-    // the first j positions load with the run's list data
+    // (supercluster | entry count << 26, first entry, record slot, j-cluster
+    // of the first entry): the first j positions load with the run's list data
     const int4 wd = A.wi[w];
-    const int sc = wd.x;
-    const int e0 = wd.y, ne = wd.z - e0;
+    const int sc = (int)((unsigned)wd.x & (unsigned)kWiScMask);
+    const int e0 = wd.y, ne = (int)((unsigned)wd.x >> kWiEntryShift);
     int4 cq = qpos[(size_t)wd.w * 8 + jl];
     float ce = qeps[(size_t)wd.w * 8 + jl];
     unsigned long long live = 0;
@@ -255,6 +263,13 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     const float ds_coef = CUBIC ? p.ds_coef_u : p.ds_coef;
     const float inv_w_in = CUBIC ? p.inv_w_in_u : p.inv_w_in, nrw_in = in_vgpr(p.nrw_in);
     const float ds_coef_in = CUBIC ? p.ds_coef_in_u : p.ds_coef_in;
+    // Tile ci's list bit is bit ci of the lane's mask byte, tested by one
+    // class compare: a class test's mask bit c selects IEEE class c, and
+    // kcls[ci] is a value of class ci (signalling NaN, quiet NaN, -inf,
+    // -normal, -denormal, -0, +0, +denormal).
+    const float kcls[8] = {sgpr_bits(0x7f800001u), sgpr_bits(0x7fc00000u), sgpr_bits(0xff800000u),
+                           sgpr_bits(0xbf800000u), sgpr_bits(0x80000001u), sgpr_bits(0x80000000u),
+                           sgpr_bits(0x00000000u), sgpr_bits(0x00000001u)};
     double e_acc = 0.0;
     if (nl > 0 && !(live & 1ull)) {  // the first entry was pruned: start from the first live one
         cq = qpos[(size_t)s_meta[0].x * 8 + jl];
@@ -286,7 +301,7 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
                     if (!CUBIC) { dx *= p.sx; dy *= p.sy; dz *= p.sz; }
                     const float r2 = dx * dx + dy * dy + dz * dz;
                     // 0 < r2 <= rc2 as one unsigned compare of the bit patterns
-                    const bool ok = ((bits >> ci) & 1u) &&
+                    const bool ok = __builtin_amdgcn_classf(kcls[ci], (int)bits) &&
                                     ((unsigned)__float_as_int(r2) - 1u) < rc2_bits;
                     const float inv_r = __builtin_amdgcn_rsqf(r2);
                     // one Newton step brings rsq^2 (~2.5 ulp) to ~1 ulp, the
@@ -376,7 +391,7 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     if (A.fi_items) {
         // the item's own record (plain stores; the finish kernel adds the
         // records of a supercluster's items): no atomics on the i side
-        float *fw = A.fi_items + (size_t)w * 192 + lane;
+        float *fw = A.fi_items + (size_t)wd.z * 192 + lane;
         fw[0] = fo[0];
         fw[64] = fo[1];
         fw[128] = fo[2];
@@ -482,7 +497,6 @@ TileArgs tile_args(gm_sys *s) {
     A.e_wi = s->e_wi.p;
     A.fi_items = s->fi_records ? s->fi_items.p : nullptr;
     A.sc_wi_off = s->sc_wi_off.p;
-    A.sc_wi = s->sc_wi.p;
     A.fi_scale = span_is_cubic(s) ? s->nbp.fscale_u : 4294967296.0f;  // the tile kernel's fscale
     A.slot_atom = s->slot_atom.p;
     A.p = s->nbp;

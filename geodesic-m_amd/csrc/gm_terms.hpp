@@ -200,13 +200,14 @@ struct TileArgs {
     unsigned long long *F;             // int64 fixed-point accumulator, slot order (j forces, bonded)
     int n_pad;                         // slots
     double *e_wi;                      // per work item LJ energy
-    // i forces per work item, [item][x, y, z][64 slots of its supercluster],
-    // f32 in the kernel's force unit: plain stores; the finish kernel rounds
-    // each to fixed point (scaled by fi_scale) and adds them per slot over the
-    // supercluster's work items (sc_wi_off / sc_wi, CSR, ascending item).
+    // i forces per work item, [record slot][x, y, z][64 slots of its
+    // supercluster], f32 in the kernel's force unit: plain stores.  A
+    // supercluster's items, ascending, own the record slots
+    // [sc_wi_off[sc], sc_wi_off[sc + 1]); the finish kernel rounds each record
+    // to fixed point (scaled by fi_scale) and adds them per slot in that order.
     // Null: the i forces go to the accumulator F with atomics instead.
     float *fi_items;
-    const int *sc_wi_off, *sc_wi;
+    const int *sc_wi_off;
     float fi_scale;
     const int *slot_atom;              // slot -> atom (-1 for padding), for the finish kernels
     NbParams p;
