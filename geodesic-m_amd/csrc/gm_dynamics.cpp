@@ -3,14 +3,24 @@
 // (geodesic/src/lib.rs:255-330; paths relative to the reference repository
 // root) with forces from the GPU force path and frames streamed to the host.
 //
-// The SHAKE / RATTLE iterations run without a host round trip per iteration:
+// SHAKE / RATTLE: when every connected component of the constraint graph
+// fits a constraint group (GroupArgs), a solve is two launches and one read-
+// back: each group iterates alone in LDS to its own first iteration below the
+// tolerance, then every group continues to the largest of those counts, which
+// is the reference's global stopping point unless some group is back above
+// the tolerance there (not seen in practice; the per-iteration kernels below
+// then take over from that state).
+//
+// The per-iteration kernels run without a host round trip per iteration:
 // each iteration's kernel first compares the previous iteration's largest
 // multiplier with the tolerance and raises a device flag, which makes the
 // remaining launched iterations return at once (before a read-back, a
-// one-thread kernel makes the same test for the last launched iteration).  The host first reads the flag after
-// as many iterations as the previous solve of the same kind needed (plus
-// two), then every kCheckEvery iterations: about one round trip per solve.
-// The result is the reference's: same iterations, same stopping point.
+// one-thread kernel makes the same test for the last launched iteration).
+// The host first reads the flag after as many iterations as the previous
+// solve of the same kind needed (plus two), then every kCheckEvery
+// iterations: about one round trip per solve.
+// Either way the result is the reference's: same iterations, same stopping
+// point, same arithmetic.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -32,6 +42,15 @@ struct gm_dyn {
     DevBuf<int> flag;
     int max_iter_alloc = 0;
     int last_iters[2] = {0, 0};  // iterations the previous position / velocity solve took
+    // constraint groups (none: a component exceeds a group's capacity)
+    int n_groups = 0;
+    int64_t n_free = 0;
+    DevBuf<int> g_atom_off, g_con_off, g_inc_off, g_iters, g_conv, g_ctl;
+    DevBuf<uint32_t> g_atoms, g_cons, g_con_loc, g_free;
+    DevBuf<uint16_t> g_inc;
+    bool groups_on = true;
+    int group_cap = 0;      // > 0: phase 0 stops there (the per-iteration kernels continue)
+    int *h_ctl = nullptr;   // pinned: g_ctl[0..3] and a rebuild flag, read back together
 };
 
 void gm_dyn_free(gm_dyn *d) {
@@ -44,6 +63,11 @@ void gm_dyn_free(gm_dyn *d) {
     d->inc_con.release();
     d->maxslot.release();
     d->flag.release();
+    for (DevBuf<int> *b : {&d->g_atom_off, &d->g_con_off, &d->g_inc_off, &d->g_iters, &d->g_conv, &d->g_ctl})
+        b->release();
+    for (DevBuf<uint32_t> *b : {&d->g_atoms, &d->g_cons, &d->g_con_loc, &d->g_free}) b->release();
+    d->g_inc.release();
+    if (d->h_ctl) (void)hipHostFree(d->h_ctl);
     delete d;
 }
 
@@ -68,6 +92,105 @@ DynArgs dyn_args(const gm_dyn *d) {
     return D;
 }
 
+GroupArgs group_args(gm_dyn *d, bool velocity) {
+    GroupArgs G{};
+    G.n_groups = d->n_groups;
+    G.atom_off = d->g_atom_off.p;
+    G.atoms = d->g_atoms.p;
+    G.con_off = d->g_con_off.p;
+    G.cons = d->g_cons.p;
+    G.con_loc = d->g_con_loc.p;
+    G.inc_off = d->g_inc_off.p;
+    G.inc = d->g_inc.p;
+    G.free_atoms = d->g_free.p;
+    G.n_free = d->n_free;
+    G.iters = d->g_iters.p;
+    G.conv = d->g_conv.p;
+    G.ctl = d->g_ctl.p + (velocity ? 2 : 0);  // position and velocity solves: separate words
+    return G;
+}
+
+// Host-side layout of the constraint groups: connected components (union-
+// find over constraint endpoints) in order of their lowest atom, packed whole
+// and in that order into groups of at most kGroupAtoms atoms and kGroupCons
+// constraints.  Returns false (no groups) when a component alone is larger.
+struct GroupLayout {
+    std::vector<int> atom_off{0}, con_off{0}, inc_off{0};
+    std::vector<uint32_t> atoms, cons, con_loc, free_atoms;
+    std::vector<uint16_t> inc;
+};
+
+bool make_groups(int64_t n, int64_t nc, const uint32_t *ci, const uint32_t *cj, const std::vector<int64_t> &off,
+                 const std::vector<int64_t> &con, GroupLayout &L) {
+    std::vector<int64_t> parent((size_t)n);
+    for (int64_t a = 0; a < n; a++) parent[(size_t)a] = a;
+    auto find = [&](int64_t a) {
+        while (parent[(size_t)a] != a) a = parent[(size_t)a] = parent[(size_t)parent[(size_t)a]];
+        return a;
+    };
+    for (int64_t c = 0; c < nc; c++) {
+        const int64_t ra = find(ci[c]), rb = find(cj[c]);
+        if (ra != rb) parent[(size_t)std::max(ra, rb)] = std::min(ra, rb);
+    }
+    std::vector<int64_t> comp((size_t)n, -1);  // root -> component
+    std::vector<std::vector<uint32_t>> catoms, ccons;
+    for (int64_t a = 0; a < n; a++) {
+        if (off[(size_t)a + 1] == off[(size_t)a]) {
+            L.free_atoms.push_back((uint32_t)a);
+            continue;
+        }
+        const int64_t r = find(a);
+        if (comp[(size_t)r] < 0) {
+            comp[(size_t)r] = (int64_t)catoms.size();
+            catoms.emplace_back();
+            ccons.emplace_back();
+        }
+        catoms[(size_t)comp[(size_t)r]].push_back((uint32_t)a);
+    }
+    for (int64_t c = 0; c < nc; c++) ccons[(size_t)comp[(size_t)find(ci[c])]].push_back((uint32_t)c);
+    std::vector<int> loc((size_t)n, -1), cloc((size_t)nc, -1);
+    size_t k = 0;
+    while (k < catoms.size()) {
+        if (catoms[k].size() > (size_t)kGroupAtoms || ccons[k].size() > (size_t)kGroupCons) return false;
+        size_t na = 0, ncg = 0, k1 = k;
+        while (k1 < catoms.size() && na + catoms[k1].size() <= (size_t)kGroupAtoms &&
+               ncg + ccons[k1].size() <= (size_t)kGroupCons) {
+            na += catoms[k1].size();
+            ncg += ccons[k1].size();
+            k1++;
+        }
+        // each component's atoms and constraints are ascending, and no atom's
+        // constraints span components: per atom, local order = constraint order
+        const size_t a_base = L.atoms.size(), c_base = L.cons.size();
+        for (size_t q = k; q < k1; q++) {
+            for (uint32_t a : catoms[q]) {
+                loc[a] = (int)(L.atoms.size() - a_base);
+                L.atoms.push_back(a);
+            }
+            for (uint32_t c : ccons[q]) {
+                cloc[c] = (int)(L.cons.size() - c_base);
+                L.cons.push_back(c);
+            }
+        }
+        for (size_t q = c_base; q < L.cons.size(); q++) {
+            const uint32_t c = L.cons[q];
+            L.con_loc.push_back((uint32_t)loc[ci[c]] | ((uint32_t)loc[cj[c]] << 16));
+        }
+        for (size_t q = a_base; q < L.atoms.size(); q++) {
+            const uint32_t a = L.atoms[q];
+            for (int64_t e = off[a]; e < off[(size_t)a + 1]; e++) {
+                const int64_t c = con[(size_t)e];
+                L.inc.push_back((uint16_t)((cloc[(size_t)c] << 1) | (cj[c] == a ? 1 : 0)));
+            }
+            L.inc_off.push_back((int)L.inc.size());
+        }
+        L.atom_off.push_back((int)L.atoms.size());
+        L.con_off.push_back((int)L.cons.size());
+        k = k1;
+    }
+    return true;
+}
+
 gm_status ensure_iter_buffers(gm_dyn *d, int max_iter) {
     if (d->max_iter_alloc < max_iter) {
         GM_HIP(d->maxslot.ensure((size_t)max_iter));
@@ -88,48 +211,68 @@ void worst_of(const std::vector<double> &lam, int64_t &idx, double &val) {
         }
 }
 
-// Jacobi iterations of SHAKE (velocity = false) or RATTLE (velocity = true)
-// on the 3n-block `buf0` (the caller's arrays), with the scratch buffer as the
-// second half of the double buffer.  On return the result is in buf0.
-gm_status project(gm_sys *s, bool velocity, Vec3 buf0, Vec3C other, int max_iter, double tol, int64_t step,
-                  int32_t *iters) {
+bool groups_active(const gm_dyn *d) { return d->nc > 0 && d->n_groups > 0 && d->groups_on; }
+
+// Enqueue a group solve of one kind (fill of its control words, phase 0,
+// phase 1), without a read-back: its control words end in
+// g_ctl[2 velocity ..] on the device.
+gm_status group_launch(gm_sys *s, bool velocity, Vec3 buf0, Vec3C other, int max_iter, double tol,
+                       const int *guard) {
     gm_dyn *d = s->dyn;
     hipStream_t st = s->stream;
-    if (d->nc == 0) {
-        if (iters) *iters = 0;
-        return GM_OK;
-    }
+    const GroupArgs G = group_args(d, velocity);
+    const DynArgs D = dyn_args(d);
+    const int cap = d->group_cap > 0 ? std::min(d->group_cap, max_iter) : max_iter;
+    GM_HIP(launch_fill_i32(G.ctl, 2, 0, st));
+    GM_HIP(launch_group_solve(G, D, velocity, buf0, other, cap, tol, 0, d->lambda.p, st, guard));
+    GM_HIP(launch_group_solve(G, D, velocity, buf0, other, cap, tol, 1, d->lambda.p, st, guard));
+    return GM_OK;
+}
+
+// The per-iteration kernels, as the reference's iterations start .. max_iter - 1
+// from the state in buf0 (iteration start + j is their iteration j), to the
+// first iteration below tol; GM_ERR_CONVERGENCE with the reference's report
+// when there is none.  The scratch buffer is the second half of the double
+// buffer; on return the result is in buf0.
+gm_status iterate_from(gm_sys *s, bool velocity, Vec3 buf0, Vec3C other, int start, int max_iter, double tol,
+                       int64_t step, int32_t *iters) {
+    gm_dyn *d = s->dyn;
+    hipStream_t st = s->stream;
     gm_status rc = ensure_iter_buffers(d, max_iter);
     if (rc != GM_OK) return rc;
     const int64_t n = d->n;
-    Vec3 buf1{d->scratch.p, d->scratch.p + n, d->scratch.p + 2 * n};
-    GM_HIP(launch_fill_i32(d->flag.p, 3, 0, st));
-    GM_HIP(launch_fill_u64(d->maxslot.p, (size_t)max_iter, st));
     const DynArgs D = dyn_args(d);
+    Vec3 buf1{d->scratch.p, d->scratch.p + n, d->scratch.p + 2 * n};
     int flag[3] = {0, 0, 0};
-    for (int it = 0; it < max_iter; it++) {
-        const Vec3 in = (it & 1) ? buf1 : buf0, out = (it & 1) ? buf0 : buf1;
-        if (velocity)
-            GM_HIP(launch_rattle_iter(D, other, in, out, d->maxslot.p, it, tol, d->lambda.p, d->flag.p, st));
-        else
-            GM_HIP(launch_shake_iter(D, in, out, other, d->maxslot.p, it, tol, d->lambda.p, d->flag.p, st));
-        const int first = d->last_iters[velocity ? 1 : 0] + 2;  // the first read-back
-        if (it + 1 == first || (it + 1 > first && (it + 1 - first) % kCheckEvery == 0) || it + 1 == max_iter) {
-            // the test of this iteration (the next one's kernel would make it)
-            GM_HIP(launch_check(d->maxslot.p, tol, it, d->flag.p, st));
-            GM_HIP(hipMemcpyAsync(flag, d->flag.p, sizeof(flag), hipMemcpyDeviceToHost, st));
-            GM_HIP(hipStreamSynchronize(st));
-            if (flag[0]) break;
+    const int left = max_iter - start;
+    if (left > 0) {
+        GM_HIP(launch_fill_i32(d->flag.p, 3, 0, st));
+        GM_HIP(launch_fill_u64(d->maxslot.p, (size_t)left, st));
+        // the first read-back after as many iterations as the previous solve needed, plus two
+        const int first = std::max(d->last_iters[velocity ? 1 : 0] - start, 0) + 2;
+        for (int it = 0; it < left; it++) {
+            const Vec3 in = (it & 1) ? buf1 : buf0, out = (it & 1) ? buf0 : buf1;
+            if (velocity)
+                GM_HIP(launch_rattle_iter(D, other, in, out, d->maxslot.p, it, tol, d->lambda.p, d->flag.p, st));
+            else
+                GM_HIP(launch_shake_iter(D, in, out, other, d->maxslot.p, it, tol, d->lambda.p, d->flag.p, st));
+            if (it + 1 == first || (it + 1 > first && (it + 1 - first) % kCheckEvery == 0) || it + 1 == left) {
+                // the test of this iteration (the next one's kernel would make it)
+                GM_HIP(launch_check(d->maxslot.p, tol, it, d->flag.p, st));
+                GM_HIP(hipMemcpyAsync(flag, d->flag.p, sizeof(flag), hipMemcpyDeviceToHost, st));
+                GM_HIP(hipStreamSynchronize(st));
+                if (flag[0]) break;
+            }
+        }
+        if (flag[2] == 1) {
+            const size_t b = sizeof(double) * (size_t)n;
+            GM_HIP(hipMemcpyAsync(buf0.x, buf1.x, b, hipMemcpyDeviceToDevice, st));
+            GM_HIP(hipMemcpyAsync(buf0.y, buf1.y, b, hipMemcpyDeviceToDevice, st));
+            GM_HIP(hipMemcpyAsync(buf0.z, buf1.z, b, hipMemcpyDeviceToDevice, st));
         }
     }
-    if (flag[2] == 1) {
-        const size_t b = sizeof(double) * (size_t)n;
-        GM_HIP(hipMemcpyAsync(buf0.x, buf1.x, b, hipMemcpyDeviceToDevice, st));
-        GM_HIP(hipMemcpyAsync(buf0.y, buf1.y, b, hipMemcpyDeviceToDevice, st));
-        GM_HIP(hipMemcpyAsync(buf0.z, buf1.z, b, hipMemcpyDeviceToDevice, st));
-    }
-    if (iters) *iters = flag[1];
-    d->last_iters[velocity ? 1 : 0] = flag[1];
+    if (iters) *iters = start + flag[1];
+    d->last_iters[velocity ? 1 : 0] = start + flag[1];
     if (!flag[0]) {
         std::vector<double> lam((size_t)d->nc);
         std::vector<uint32_t> hi(1), hj(1);
@@ -148,6 +291,47 @@ gm_status project(gm_sys *s, bool velocity, Vec3 buf0, Vec3C other, int max_iter
                              std::to_string(max_iter) + " iterations");
     }
     return GM_OK;
+}
+
+// The end of a group solve of one kind whose control words are in h_ctl:
+// the count every group reached is the reference's stopping point when no
+// group is above tol there; otherwise the per-iteration kernels continue
+// from that state.
+gm_status group_finish(gm_sys *s, bool velocity, Vec3 buf0, Vec3C other, int max_iter, double tol, int64_t step,
+                       int32_t *iters) {
+    gm_dyn *d = s->dyn;
+    const int *c = d->h_ctl + (velocity ? 2 : 0);
+    if (!c[1]) {
+        if (iters) *iters = c[0];
+        d->last_iters[velocity ? 1 : 0] = c[0];
+        return GM_OK;
+    }
+    return iterate_from(s, velocity, buf0, other, c[0], max_iter, tol, step, iters);
+}
+
+gm_status read_ctl(gm_sys *s, const int *rebuild_flag) {
+    gm_dyn *d = s->dyn;
+    GM_HIP(hipMemcpyAsync(d->h_ctl, d->g_ctl.p, 4 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    if (rebuild_flag)
+        GM_HIP(hipMemcpyAsync(d->h_ctl + 4, rebuild_flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    GM_HIP(hipStreamSynchronize(s->stream));
+    return GM_OK;
+}
+
+// SHAKE (velocity = false) or RATTLE (velocity = true) on the 3n-block
+// `buf0` (the caller's arrays): the group solve when there are groups, the
+// per-iteration kernels otherwise.  On return the result is in buf0.
+gm_status project(gm_sys *s, bool velocity, Vec3 buf0, Vec3C other, int max_iter, double tol, int64_t step,
+                  int32_t *iters) {
+    gm_dyn *d = s->dyn;
+    if (d->nc == 0) {
+        if (iters) *iters = 0;
+        return GM_OK;
+    }
+    if (!groups_active(d)) return iterate_from(s, velocity, buf0, other, 0, max_iter, tol, step, iters);
+    gm_status rc = group_launch(s, velocity, buf0, other, max_iter, tol, nullptr);
+    if (rc != GM_OK || (rc = read_ctl(s, nullptr)) != GM_OK) return rc;
+    return group_finish(s, velocity, buf0, other, max_iter, tol, step, iters);
 }
 
 gm_status drift_and_constrain(gm_sys *s, double *x, double *y, double *z, double *vx, double *vy, double *vz,
@@ -241,8 +425,37 @@ gm_status gm_dynamics_set(gm_sys *s, const double *mass, int64_t n_constr, const
             (e = hipMemcpy(d->dsq.p, dsq, 8 * (size_t)n_constr, hipMemcpyHostToDevice)) != hipSuccess ||
             (e = hipMemcpy(d->inc_con.p, con.data(), 8 * con.size(), hipMemcpyHostToDevice)) != hipSuccess)
             return fail(e);
+        GroupLayout L;
+        if (make_groups(n, n_constr, ci, cj, off, con, L)) {
+            const int G = (int)L.atom_off.size() - 1;
+            auto up = [&](auto &buf, const auto &v) {
+                hipError_t r = buf.ensure(std::max<size_t>(v.size(), 1));
+                if (r == hipSuccess && !v.empty())
+                    r = hipMemcpy(buf.p, v.data(), sizeof(v[0]) * v.size(), hipMemcpyHostToDevice);
+                return r;
+            };
+            if ((e = up(d->g_atom_off, L.atom_off)) != hipSuccess || (e = up(d->g_con_off, L.con_off)) != hipSuccess ||
+                (e = up(d->g_inc_off, L.inc_off)) != hipSuccess || (e = up(d->g_atoms, L.atoms)) != hipSuccess ||
+                (e = up(d->g_cons, L.cons)) != hipSuccess || (e = up(d->g_con_loc, L.con_loc)) != hipSuccess ||
+                (e = up(d->g_inc, L.inc)) != hipSuccess || (e = up(d->g_free, L.free_atoms)) != hipSuccess ||
+                (e = d->g_iters.ensure((size_t)G)) != hipSuccess || (e = d->g_conv.ensure((size_t)G)) != hipSuccess ||
+                (e = d->g_ctl.ensure(4)) != hipSuccess ||
+                (e = hipHostMalloc((void **)&d->h_ctl, 8 * sizeof(int), hipHostMallocDefault)) != hipSuccess)
+                return fail(e);
+            d->n_groups = G;
+            d->n_free = (int64_t)L.free_atoms.size();
+        }
     }
     s->dyn = d;
+    return GM_OK;
+}
+
+gm_status gm_set_constraint_groups(gm_sys *s, int32_t enable, int32_t iteration_cap, int32_t *n_groups) {
+    GM_REQUIRE(s && s->dyn && (enable == 0 || enable == 1) && iteration_cap >= 0,
+               "gm_set_constraint_groups: no dynamics attached or bad argument");
+    s->dyn->groups_on = enable != 0;
+    s->dyn->group_cap = iteration_cap;
+    if (n_groups) *n_groups = s->dyn->n_groups;
     return GM_OK;
 }
 
@@ -308,18 +521,72 @@ gm_status gm_md_run(gm_sys *s, double *x, double *y, double *z, double *vx, doub
     if ((rc = project(s, true, Vec3{vx, vy, vz}, Vec3C{x, y, z}, md->max_iter, md->tol, 0, nullptr)) != GM_OK)
         return rc;
     if ((rc = record(e.total)) != GM_OK) return rc;
-    // lib.rs:283-322: B, A (drift + SHAKE), rebuild if needed, forces, B, RATTLE
+    // lib.rs:283-322: B, A (drift + SHAKE), rebuild if needed, forces, B, RATTLE.
+    // With constraint groups a step has one read-back: a step's RATTLE is
+    // checked together with the next step's SHAKE and rebuild test.  The next
+    // step's kernels up to that point are guarded by RATTLE's "continue"
+    // word, so when RATTLE must continue per iteration none of them has
+    // changed anything: RATTLE continues, and the step runs again.
+    gm_dyn *d = s->dyn;
+    hipStream_t st = s->stream;
+    double *rx = d->ref.p, *ry = rx + n, *rz = rx + 2 * n;
+    const bool deferred = groups_active(d);
+    bool v_pending = false;  // the previous step's RATTLE is launched, not yet checked
     for (int64_t step = 0; step < md->n_steps; step++) {
-        GM_HIP(launch_half_kick(n, vx, vy, vz, fx, fy, fz, s->dyn->mass.p, dt / 2.0, s->stream));
-        if ((rc = drift_and_constrain(s, x, y, z, vx, vy, vz, dt, md->max_iter, md->tol, step, nullptr)) != GM_OK)
-            return rc;
         int32_t rebuild = 0;
-        if ((rc = gm_needs_rebuild_device(s, x, y, z, &rebuild)) != GM_OK) return rc;
+        bool stepped = false;
+        if (deferred) {
+            const int *guard = v_pending ? d->g_ctl.p + 3 : nullptr;
+            GM_HIP(launch_half_kick(n, vx, vy, vz, fx, fy, fz, d->mass.p, dt / 2.0, st, guard));
+            GM_HIP(launch_drift(n, x, y, z, vx, vy, vz, rx, ry, rz, dt, st, guard));
+            if ((rc = group_launch(s, false, Vec3{x, y, z}, Vec3C{rx, ry, rz}, md->max_iter, md->tol, guard)) !=
+                GM_OK)
+                return rc;
+            GM_HIP(launch_resync(n, x, y, z, rx, ry, rz, vx, vy, vz, dt, st, guard));
+            GM_HIP(launch_needs_rebuild(s, x, y, z));
+            if ((rc = read_ctl(s, s->meta.p + 5)) != GM_OK) return rc;
+            const bool replay = v_pending && d->h_ctl[3];
+            if (v_pending) {
+                rc = group_finish(s, true, Vec3{vx, vy, vz}, Vec3C{x, y, z}, md->max_iter, md->tol, step - 1, nullptr);
+                if (rc != GM_OK) return rc;
+                v_pending = false;
+            }
+            if (!replay) {
+                if (d->h_ctl[1]) {  // SHAKE continues per iteration: resync and test again after it
+                    rc = group_finish(s, false, Vec3{x, y, z}, Vec3C{rx, ry, rz}, md->max_iter, md->tol, step,
+                                      nullptr);
+                    if (rc != GM_OK) return rc;
+                    GM_HIP(launch_resync(n, x, y, z, rx, ry, rz, vx, vy, vz, dt, st));
+                    if ((rc = gm_needs_rebuild_device(s, x, y, z, &rebuild)) != GM_OK) return rc;
+                } else {
+                    rebuild = d->h_ctl[4] ? 1 : 0;
+                }
+                stepped = true;
+            }
+        }
+        if (!stepped) {
+            GM_HIP(launch_half_kick(n, vx, vy, vz, fx, fy, fz, d->mass.p, dt / 2.0, st));
+            if ((rc = drift_and_constrain(s, x, y, z, vx, vy, vz, dt, md->max_iter, md->tol, step, nullptr)) != GM_OK)
+                return rc;
+            if ((rc = gm_needs_rebuild_device(s, x, y, z, &rebuild)) != GM_OK) return rc;
+        }
         if (rebuild && (rc = gm_build_neighbor_list_device(s, x, y, z)) != GM_OK) return rc;
         if ((rc = compute_on_device(s, x, y, z, fx, fy, fz)) != GM_OK) return rc;
-        GM_HIP(launch_half_kick(n, vx, vy, vz, fx, fy, fz, s->dyn->mass.p, dt / 2.0, s->stream));
-        if ((rc = project(s, true, Vec3{vx, vy, vz}, Vec3C{x, y, z}, md->max_iter, md->tol, step, nullptr)) != GM_OK)
+        GM_HIP(launch_half_kick(n, vx, vy, vz, fx, fy, fz, d->mass.p, dt / 2.0, st));
+        if (deferred) {
+            rc = group_launch(s, true, Vec3{vx, vy, vz}, Vec3C{x, y, z}, md->max_iter, md->tol, nullptr);
+            if (rc != GM_OK) return rc;
+            v_pending = true;
+            if ((step + 1) % md->frame_interval == 0 || step + 1 == md->n_steps) {  // checked before a frame
+                if ((rc = read_ctl(s, nullptr)) != GM_OK) return rc;
+                rc = group_finish(s, true, Vec3{vx, vy, vz}, Vec3C{x, y, z}, md->max_iter, md->tol, step, nullptr);
+                if (rc != GM_OK) return rc;
+                v_pending = false;
+            }
+        } else if ((rc = project(s, true, Vec3{vx, vy, vz}, Vec3C{x, y, z}, md->max_iter, md->tol, step,
+                                 nullptr)) != GM_OK) {
             return rc;
+        }
         if ((step + 1) % md->frame_interval == 0) {
             if ((rc = collect(s, &e)) != GM_OK) return rc;
             if ((rc = record(e.total)) != GM_OK) return rc;

@@ -191,6 +191,7 @@ def lib():
             "gm_needs_rebuild_device": ([_P, _P, _P, _P, C.POINTER(C.c_int32)], C.c_int),
             "gm_surrogate_get_hessian": ([_P, _P], C.c_int),
             "gm_dynamics_set": ([_P, _P, _I64, _P, _P, _P], C.c_int),
+            "gm_set_constraint_groups": ([_P, C.c_int32, C.c_int32, _P], C.c_int),
             "gm_half_kick_device": ([_P, _P, _P, _P, _P, _P, _P, C.c_double], C.c_int),
             "gm_drift_and_constrain_device": ([_P, _P, _P, _P, _P, _P, _P, C.c_double, C.c_int32, C.c_double,
                                                _I64, C.POINTER(C.c_int32)], C.c_int),
@@ -449,6 +450,14 @@ class System:
         cj = np.ascontiguousarray(constr_j, np.uint32)
         dsq = np.ascontiguousarray(constr_dsq, np.float64)
         _check(lib().gm_dynamics_set(self.h, _p(m), len(ci), _p(ci), _p(cj), _p(dsq)))
+
+    def set_constraint_groups(self, enable=True, iteration_cap=0):
+        """SHAKE / RATTLE by constraint groups (enable) or per-iteration launches;
+        iteration_cap > 0 hands over to the latter after that many group
+        iterations.  Returns the number of groups (0: none fit)."""
+        ng = C.c_int32(0)
+        _check(lib().gm_set_constraint_groups(self.h, 1 if enable else 0, int(iteration_cap), C.byref(ng)))
+        return ng.value
 
     def half_kick_device(self, dvx, dvy, dvz, dfx, dfy, dfz, dt_half):
         _check(lib().gm_half_kick_device(self.h, *(_P(a.ptr) for a in (dvx, dvy, dvz, dfx, dfy, dfz)),

@@ -219,6 +219,17 @@ gm_status gm_set_tile_pruning(gm_sys *sys, int32_t enable);
  * topology.rs:26-32) are attached to a system once. */
 gm_status gm_dynamics_set(gm_sys *sys, const double *mass, int64_t n_constr, const uint32_t *constr_i,
                           const uint32_t *constr_j, const double *constr_dsq);
+/* Tuning: the SHAKE / RATTLE solver.  When every connected component of the
+ * constraint graph has at most 256 atoms and 256 constraints (X-H bond
+ * constraints, rigid waters), the components are packed into groups that
+ * one workgroup each iterates in LDS: a solve is two launches and one read-
+ * back.  Otherwise (enable 0, or a larger component) every iteration is a
+ * launch over all atoms.  iteration_cap > 0 stops the group iterations after
+ * that many, leaving the rest to the per-iteration launches.  Same
+ * iterations, stopping point and arithmetic (hence results) every way.
+ * *n_groups (may be NULL) receives the group count, 0 when there are none.
+ * No reference counterpart (constraint.rs:62-259 iterates every constraint). */
+gm_status gm_set_constraint_groups(gm_sys *sys, int32_t enable, int32_t iteration_cap, int32_t *n_groups);
 /* v += dt_half F / m * 20.455^2 (ordered on the system's stream). */
 gm_status gm_half_kick_device(gm_sys *sys, double *d_vx, double *d_vy, double *d_vz, const double *d_fx,
                               const double *d_fy, const double *d_fz, double dt_half);

@@ -36,6 +36,19 @@ def host(*arrs):
     return [a.to_host() for a in arrs]
 
 
+# Constraint solver modes (gm_set_constraint_groups): groups of whole
+# components iterated in LDS; one launch per iteration over all atoms; and
+# groups handing over to per-iteration launches after two iterations.  All
+# three must reproduce the oracle's bytes and iteration counts.
+SOLVERS = {"groups": (True, 0), "iterations": (False, 0), "handover": (True, 2)}
+
+
+def use_solver(g, name):
+    enable, cap = SOLVERS[name]
+    n_groups = g.set_constraint_groups(enable, cap)
+    return n_groups
+
+
 def ala_system():
     (x, y, z), atoms, topo = load("ala_dipeptide")
     amber.promote_hydrogen_bonds(topo, atoms)
@@ -45,9 +58,11 @@ def ala_system():
     return (x, y, z), atoms, topo, params, g
 
 
-def test_half_kick_drift_rattle_match_oracle_bytes():
-    say("[gpu] dynamics primitives, ala_dipeptide")
+@pytest.mark.parametrize("solver", list(SOLVERS))
+def test_half_kick_drift_rattle_match_oracle_bytes(solver):
+    say(f"[gpu] dynamics primitives, ala_dipeptide, {solver}")
     (x, y, z), atoms, topo, params, g = ala_system()
+    assert use_solver(g, solver) == 1  # the 12 X-H constraints fit one group
     rng = np.random.default_rng(3)
     vx, vy, vz = (rng.normal(0, 2.0, 22) for _ in range(3))
     pi, pj = orc.neighbor_build(x, y, z, BIG, 14.0, topo.excl_i, topo.excl_j)
@@ -73,11 +88,13 @@ def test_half_kick_drift_rattle_match_oracle_bytes():
     assert np.stack(host(*d[3:6])).tobytes() == np.stack([ovx, ovy, ovz]).tobytes()
 
 
-def test_shake_rattle_many_constraints_match_oracle_bytes():
+@pytest.mark.parametrize("solver", list(SOLVERS))
+def test_shake_rattle_many_constraints_match_oracle_bytes(solver):
     """3,000 atoms; every triplet is a water-like unit (heavy centre, two
     light arms held at their current lengths), so the centre atoms take two
-    constraints each: the per-atom Jacobi sums still match byte for byte."""
-    say("[gpu] dynamics primitives, constrained triplets")
+    constraints each: the per-atom Jacobi sums still match byte for byte.
+    The 875 triplets pack into groups of 85 (255 atoms)."""
+    say(f"[gpu] dynamics primitives, constrained triplets, {solver}")
     n = 3000
     s = systems.protein_solvent(n, (n / 0.1) ** (1 / 3), n // 8, seed=51)
     x, y, z = s.x.copy(), s.y.copy(), s.z.copy()
@@ -92,6 +109,7 @@ def test_shake_rattle_many_constraints_match_oracle_bytes():
     mass[tri], mass[tri + 1], mass[tri + 2] = 16.0, 1.008, 1.008
     g = gh.System(s.sigma, s.epsilon, s.topo, s.params)
     g.dynamics_set(mass, ci, cj, dsq)
+    assert use_solver(g, solver) == -(-len(tri) // 85)
     rng = np.random.default_rng(5)
     vx, vy, vz = (rng.normal(0, 3.0, n) for _ in range(3))
     d = [dev(a) for a in (x, y, z, vx, vy, vz)]
@@ -108,10 +126,46 @@ def test_shake_rattle_many_constraints_match_oracle_bytes():
 
 def test_constraint_solver_reports_non_convergence():
     (x, y, z), atoms, topo, params, g = ala_system()
-    d = [dev(a) for a in (x, y, z, np.full(22, 5.0), np.zeros(22), np.zeros(22))]
-    with pytest.raises(gh.GmError) as ei:
-        g.drift_and_constrain_device(*d, 0.01, 1, 0.0)
-    assert ei.value.status == 9 and "did not converge" in str(ei.value)
+    msgs = []
+    for solver in SOLVERS:
+        use_solver(g, solver)
+        d = [dev(a) for a in (x, y, z, np.full(22, 5.0), np.zeros(22), np.zeros(22))]
+        with pytest.raises(gh.GmError) as ei:
+            g.drift_and_constrain_device(*d, 0.01, 3, 0.0)
+        assert ei.value.status == 9 and "did not converge" in str(ei.value)
+        msgs.append(str(ei.value))
+    # the same worst constraint, residual and count whichever solver ran
+    assert msgs[0] == msgs[1] == msgs[2]
+
+
+def test_component_larger_than_a_group_uses_iterations():
+    """A 300-atom chain with every bond constrained is one component of 300
+    atoms: no groups, the per-iteration launches, the oracle's bytes."""
+    say("[gpu] dynamics primitives, 300-atom constrained chain")
+    n = 320
+    s = systems.protein_solvent(n, (n / 0.1) ** (1 / 3), 300, seed=53)
+    x, y, z = s.x.copy(), s.y.copy(), s.z.copy()
+    ci = np.arange(0, 299, dtype=np.uint32)
+    cj = ci + 1
+    dsq = (x[ci] - x[cj]) ** 2 + (y[ci] - y[cj]) ** 2 + (z[ci] - z[cj]) ** 2
+    z32, zf = np.zeros(0, np.uint32), np.zeros(0)
+    topo = amber.Topology(z32, z32, zf, zf, z32, z32, z32, zf, zf, z32, z32, z32, z32, zf, z32, zf, z32, z32,
+                          constr_i=ci, constr_j=cj, constr_dsq=dsq)
+    mass = np.where(np.arange(n) % 2 == 0, 12.0, 14.0)
+    g = gh.System(s.sigma, s.epsilon, s.topo, s.params)
+    g.dynamics_set(mass, ci, cj, dsq)
+    assert g.set_constraint_groups(True, 0) == 0
+    rng = np.random.default_rng(7)
+    vx, vy, vz = (rng.normal(0, 1.0, n) for _ in range(3))
+    d = [dev(a) for a in (x, y, z, vx, vy, vz)]
+    it = g.drift_and_constrain_device(*d, 0.001, 500, 1e-8)
+    oit = orc.drift_and_constrain(x, y, z, vx, vy, vz, topo, mass, 0.001, 500, 1e-8)
+    assert it == oit and it > 2
+    assert np.stack(host(*d[0:3])).tobytes() == np.stack([x, y, z]).tobytes()
+    it = g.constrain_velocities_device(*d, 500, 1e-8)
+    oit = orc.constrain_velocities(x, y, z, vx, vy, vz, topo, mass, 500, 1e-8)
+    assert it == oit
+    assert np.stack(host(*d[3:6])).tobytes() == np.stack([vx, vy, vz]).tobytes()
 
 
 def test_md_run_reproduces_golden_trajectory():
@@ -136,6 +190,27 @@ def test_md_run_reproduces_golden_trajectory():
                                     periodic=False)
     assert abs(epot[0] - t.total) <= 1e-6 * abs(t.total)
     assert abs((epot[-1] + ekin[-1]) - (epot[0] + ekin[0])) <= 1e-2
+
+
+def test_md_run_same_bytes_with_every_solver():
+    """The run loop checks a step's RATTLE with the next step's SHAKE (one
+    read-back a step) and replays the step when RATTLE must continue per
+    iteration; caps of 1..6 group iterations send some or all solves of both
+    kinds down that path.  Every way, the run's bytes are those of the
+    per-iteration solver."""
+    say("[gpu] md run, solver modes")
+    (x, y, z), atoms, topo, params, g = ala_system()
+    rng = np.random.default_rng(11)
+    v0 = [rng.normal(0, 3.0, 22) for _ in range(3)]
+    runs = {}
+    for name, (enable, cap) in [("iterations", (False, 0)), ("groups", (True, 0))] + \
+            [(f"cap{c}", (True, c)) for c in range(1, 7)]:
+        g.set_constraint_groups(enable, cap)
+        d = [dev(a) for a in (x, y, z, *v0)]
+        frames, epot, ekin = g.md_run(*d, n_steps=30, dt=0.001, frame_interval=7, max_iter=100, tol=1e-8)
+        runs[name] = (frames.tobytes(), epot.tobytes(), ekin.tobytes(), np.stack(host(*d)).tobytes())
+    for name, r in runs.items():
+        assert r == runs["iterations"], name
 
 
 def test_jacobi_factor():
