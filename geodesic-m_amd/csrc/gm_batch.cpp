@@ -37,7 +37,7 @@ struct gm_batch {
     DevBuf<int64_t> atom_off;
     DevBuf<int2> scl;      // every (replica, supercluster) pair, for the finish kernel
     int64_t n_scl = 0;
-    int finish_G = 1;      // wavefronts per supercluster in the finish kernel
+    int64_t n_scl_rec = 0;  // the first n_scl_rec pairs belong to replicas that keep i-force records
     DevBuf<double> fx, fy, fz, E;
     int n_items = 0;
     int chunk = 0;         // entries per tile work item (0 = automatic)
@@ -81,14 +81,17 @@ gm_status make_plan(gm_batch *b) {
     }
     std::vector<int2> items(bonded_items);
     items.insert(items.end(), tile_items.begin(), tile_items.end());
+    // finish: the superclusters of replicas that keep i-force records first
+    // (four wavefronts each add the records), then the others (one wavefront each)
     std::vector<int2> scl;
-    for (int r = 0; r < R; r++)
-        for (int64_t k = 0; k < b->reps[(size_t)r]->nsc; k++) scl.push_back(make_int2(r, (int)k));
+    for (int pass = 0; pass < 2; pass++)
+        for (int r = 0; r < R; r++)
+            if (b->reps[(size_t)r]->fi_records == (pass == 0))
+                for (int64_t k = 0; k < b->reps[(size_t)r]->nsc; k++) scl.push_back(make_int2(r, (int)k));
     b->n_scl = (int64_t)scl.size();
-    // finish: four wavefronts per supercluster if any replica keeps i-force records, else one
-    b->finish_G = 1;
+    b->n_scl_rec = 0;
     for (gm_sys *s : b->reps)
-        if (s->fi_records) b->finish_G = 4;
+        if (s->fi_records) b->n_scl_rec += s->nsc;
     GM_HIP(b->scl.ensure(std::max<size_t>(scl.size(), 1)));
     if (!scl.empty())
         GM_HIP(hipMemcpyAsync(b->scl.p, scl.data(), sizeof(int2) * scl.size(), hipMemcpyHostToDevice, st));
@@ -116,8 +119,8 @@ gm_status make_plan(gm_batch *b) {
         e1 = launch_batch_tiles(b->d_reps.p, b->d_items.p + b->n_bonded_items, b->n_items, grid, st);
     if (e1 == hipSuccess) e1 = hipStreamWaitEvent(st, b->join, 0);
     if (e1 == hipSuccess)
-        e1 = launch_batch_finish(b->d_reps.p, R, b->scl.p, b->n_scl, b->finish_G, b->atom_off.p, b->E.p,
-                                 b->fx.p, b->fy.p, b->fz.p, st);
+        e1 = launch_batch_finish(b->d_reps.p, R, b->scl.p, b->n_scl_rec, b->n_scl - b->n_scl_rec, b->atom_off.p,
+                                 b->E.p, b->fx.p, b->fy.p, b->fz.p, st);
     hipGraph_t g = nullptr;
     hipError_t e3 = hipStreamEndCapture(st, &g);
     if (e1 != hipSuccess) { if (g) (void)hipGraphDestroy(g); return from_hip(e1, "batched graph capture"); }

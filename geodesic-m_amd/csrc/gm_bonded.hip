@@ -134,11 +134,12 @@ __global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, const in
 
 // Batched end of evaluation: blocks [0, 5 R) sum replica b/5's segment b%5
 // (bond, angle, dihedral, 1-4, LJ) exactly like k_finish; the remaining
-// blocks finish 16 / G (replica, supercluster) pairs of scl each, exactly
-// like k_finish, writing forces at the replica's atom offset (G = 1 when no
-// replica keeps i-force records).
+// blocks finish (replica, supercluster) pairs of scl exactly like k_finish,
+// writing forces at the replica's atom offset: the first n_rec pairs (of
+// replicas that keep i-force records) four per block with four wavefronts
+// each, the next n_atm sixteen per block with one wavefront each.
 __global__ __launch_bounds__(1024) void k_batch_finish(const ReplicaDesc *__restrict__ reps, int n_reps,
-                                                       const int2 *__restrict__ scl, int64_t n_scl, int G,
+                                                       const int2 *__restrict__ scl, int64_t n_rec, int64_t n_atm,
                                                        const int64_t *__restrict__ atom_off, double *eout,
                                                        double *fx, double *fy, double *fz) {
     __shared__ double sh[1024];
@@ -161,9 +162,11 @@ __global__ __launch_bounds__(1024) void k_batch_finish(const ReplicaDesc *__rest
         if (t == 0) eout[b] = v;
         return;
     }
-    // the rest: 16 / G (replica, supercluster) pairs per workgroup
-    const int64_t pidx = (int64_t)(b - 5 * n_reps) * (16 / G) + (t >> 6) / G;
-    const bool valid = pidx < n_scl;
+    // the rest: four (replica, supercluster) pairs with records, or sixteen without, per workgroup
+    const int64_t b_rec = (n_rec + 3) / 4, bb = b - 5 * n_reps;
+    const int G = bb < b_rec ? 4 : 1;
+    const int64_t pidx = bb < b_rec ? bb * 4 + (t >> 6) / 4 : n_rec + (bb - b_rec) * 16 + (t >> 6);
+    const bool valid = bb < b_rec ? pidx < n_rec : pidx < n_rec + n_atm;
     const int2 rs = valid ? scl[pidx] : make_int2(0, 0);
     const ReplicaDesc &R = reps[rs.x];
     sc_finish(valid, rs.y, G, R.terms.F, R.terms.n_pad, R.tile.fi_items, R.tile.sc_wi_off,
@@ -174,12 +177,11 @@ __global__ __launch_bounds__(1024) void k_batch_finish(const ReplicaDesc *__rest
 
 namespace gm {
 
-hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int2 *scl, int64_t n_scl, int G,
+hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int2 *scl, int64_t n_rec, int64_t n_atm,
                                const int64_t *atom_off, double *eout, double *fx, double *fy, double *fz,
                                hipStream_t st) {
-    const int per = 16 / G;
-    const unsigned grid = 5u * (unsigned)n_reps + (unsigned)((n_scl + per - 1) / per);
-    k_batch_finish<<<grid, 1024, 0, st>>>(reps, n_reps, scl, n_scl, G, atom_off, eout, fx, fy, fz);
+    const unsigned grid = 5u * (unsigned)n_reps + (unsigned)((n_rec + 3) / 4 + (n_atm + 15) / 16);
+    k_batch_finish<<<grid, 1024, 0, st>>>(reps, n_reps, scl, n_rec, n_atm, atom_off, eout, fx, fy, fz);
     return hipGetLastError();
 }
 

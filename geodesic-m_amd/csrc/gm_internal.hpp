@@ -335,7 +335,7 @@ hipError_t launch_batch_prep(const ReplicaDesc *reps, const int *atom_rep, const
 hipError_t launch_batch_tiles(const ReplicaDesc *reps, const int2 *items, int n_items, int grid,
                               hipStream_t st);
 hipError_t launch_batch_bonded(const ReplicaDesc *reps, const int2 *items, int n_items, hipStream_t st);
-hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int2 *scl, int64_t n_scl, int G,
+hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int2 *scl, int64_t n_rec, int64_t n_atm,
                                const int64_t *atom_off, double *eout, double *fx, double *fy, double *fz,
                                hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long *p, size_t n, hipStream_t st);
