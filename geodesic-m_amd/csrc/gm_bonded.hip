@@ -132,8 +132,8 @@ __global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, const in
               fx, fy, fz);
 }
 
-// Batched end of evaluation: blocks [0, 5 R) sum replica b/5's segment b%5
-// (bond, angle, dihedral, 1-4, LJ) exactly like k_finish; the remaining
+// Batched end of evaluation: blocks [0, R) sum replica b's five segments
+// (bond, angle, dihedral, 1-4, LJ) in turn, each exactly like k_finish; the remaining
 // blocks finish (replica, supercluster) pairs of scl exactly like k_finish,
 // writing forces at the replica's atom offset: the first n_rec pairs (of
 // replicas that keep i-force records) four per block with four wavefronts
@@ -144,26 +144,22 @@ __global__ __launch_bounds__(1024) void k_batch_finish(const ReplicaDesc *__rest
                                                        double *fx, double *fy, double *fz) {
     __shared__ double sh[1024];
     const int b = blockIdx.x, t = threadIdx.x;
-    if (b < 5 * n_reps) {
-        const ReplicaDesc &R = reps[b / 5];
-        const int seg = b % 5;
-        if (seg == 0 && t < kDispSlots) R.prep.disp[t * kDispStride] = 0u;  // bound of the next evaluation
+    if (b < n_reps) {  // the replica's five segments in turn
+        const ReplicaDesc &R = reps[b];
+        if (t < kDispSlots) R.prep.disp[t * kDispStride] = 0u;  // bound of the next evaluation
         const double *eb = R.terms.e_items;
-        const double *ptr;
-        long long len;
-        switch (seg) {
-            case 0: ptr = eb; len = R.terms.nb; break;
-            case 1: ptr = eb + R.terms.nb; len = R.terms.na; break;
-            case 2: ptr = eb + R.terms.nb + R.terms.na; len = R.terms.nd; break;
-            case 3: ptr = eb + R.terms.nb + R.terms.na + R.terms.nd; len = R.terms.n14; break;
-            default: ptr = R.tile.e_wi; len = R.n_wi; break;
+        const double *ptr[5] = {eb, eb + R.terms.nb, eb + R.terms.nb + R.terms.na,
+                                eb + R.terms.nb + R.terms.na + R.terms.nd, R.tile.e_wi};
+        const long long len[5] = {R.terms.nb, R.terms.na, R.terms.nd, R.terms.n14, R.n_wi};
+        for (int seg = 0; seg < 5; seg++) {
+            const double v = block_sum(ptr[seg], len[seg], sh);
+            if (t == 0) eout[5 * b + seg] = v;
+            __syncthreads();  // sh is reused by the next segment
         }
-        const double v = block_sum(ptr, len, sh);
-        if (t == 0) eout[b] = v;
         return;
     }
     // the rest: four (replica, supercluster) pairs with records, or sixteen without, per workgroup
-    const int64_t b_rec = (n_rec + 3) / 4, bb = b - 5 * n_reps;
+    const int64_t b_rec = (n_rec + 3) / 4, bb = b - n_reps;
     const int G = bb < b_rec ? 4 : 1;
     const int64_t pidx = bb < b_rec ? bb * 4 + (t >> 6) / 4 : n_rec + (bb - b_rec) * 16 + (t >> 6);
     const bool valid = bb < b_rec ? pidx < n_rec : pidx < n_rec + n_atm;
@@ -180,7 +176,7 @@ namespace gm {
 hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int2 *scl, int64_t n_rec, int64_t n_atm,
                                const int64_t *atom_off, double *eout, double *fx, double *fy, double *fz,
                                hipStream_t st) {
-    const unsigned grid = 5u * (unsigned)n_reps + (unsigned)((n_rec + 3) / 4 + (n_atm + 15) / 16);
+    const unsigned grid = (unsigned)n_reps + (unsigned)((n_rec + 3) / 4 + (n_atm + 15) / 16);
     k_batch_finish<<<grid, 1024, 0, st>>>(reps, n_reps, scl, n_rec, n_atm, atom_off, eout, fx, fy, fz);
     return hipGetLastError();
 }
