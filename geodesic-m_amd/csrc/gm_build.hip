@@ -799,14 +799,15 @@ hipError_t launch_pair_fill_and_masks(gm_sys *s) {
     return hipGetLastError();
 }
 
-hipError_t launch_needs_rebuild(gm_sys *s, const double *x, const double *y, const double *z) {
+hipError_t launch_needs_rebuild(gm_sys *s, const double *x, const double *y, const double *z, int *flag) {
     const double half = (s->rskin - s->rc) / 2.0;
     const double thr = half * half;
-    hipError_t e = launch_fill_i32(s->meta.p + 5, 1, 0, s->stream);
+    if (!flag) flag = s->meta.p + 5;
+    hipError_t e = launch_fill_i32(flag, 1, 0, s->stream);
     if (e != hipSuccess) return e;
     if (s->n > 0)
         k_needs_rebuild<<<grid_for(s->n, 256), 256, 0, s->stream>>>(
-            (int)s->n, x, y, z, s->refx.p, s->refy.p, s->refz.p, thr, s->meta.p + 5);
+            (int)s->n, x, y, z, s->refx.p, s->refy.p, s->refz.p, thr, flag);
     return hipGetLastError();
 }
 

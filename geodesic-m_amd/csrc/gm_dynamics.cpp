@@ -50,7 +50,7 @@ struct gm_dyn {
     DevBuf<uint16_t> g_inc;
     bool groups_on = true;
     int group_cap = 0;      // > 0: phase 0 stops there (the per-iteration kernels continue)
-    int *h_ctl = nullptr;   // pinned: g_ctl[0..3] and a rebuild flag, read back together
+    int *h_ctl = nullptr;   // pinned copy of g_ctl: [0..1] SHAKE, [2..3] RATTLE, [4] the run loop's rebuild test
 };
 
 void gm_dyn_free(gm_dyn *d) {
@@ -309,11 +309,10 @@ gm_status group_finish(gm_sys *s, bool velocity, Vec3 buf0, Vec3C other, int max
     return iterate_from(s, velocity, buf0, other, c[0], max_iter, tol, step, iters);
 }
 
-gm_status read_ctl(gm_sys *s, const int *rebuild_flag) {
+// g_ctl[0..3] (and g_ctl[4], where the run loop's rebuild test writes) into h_ctl, one copy.
+gm_status read_ctl(gm_sys *s) {
     gm_dyn *d = s->dyn;
-    GM_HIP(hipMemcpyAsync(d->h_ctl, d->g_ctl.p, 4 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
-    if (rebuild_flag)
-        GM_HIP(hipMemcpyAsync(d->h_ctl + 4, rebuild_flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    GM_HIP(hipMemcpyAsync(d->h_ctl, d->g_ctl.p, 5 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     GM_HIP(hipStreamSynchronize(s->stream));
     return GM_OK;
 }
@@ -330,7 +329,7 @@ gm_status project(gm_sys *s, bool velocity, Vec3 buf0, Vec3C other, int max_iter
     }
     if (!groups_active(d)) return iterate_from(s, velocity, buf0, other, 0, max_iter, tol, step, iters);
     gm_status rc = group_launch(s, velocity, buf0, other, max_iter, tol, nullptr);
-    if (rc != GM_OK || (rc = read_ctl(s, nullptr)) != GM_OK) return rc;
+    if (rc != GM_OK || (rc = read_ctl(s)) != GM_OK) return rc;
     return group_finish(s, velocity, buf0, other, max_iter, tol, step, iters);
 }
 
@@ -439,7 +438,7 @@ gm_status gm_dynamics_set(gm_sys *s, const double *mass, int64_t n_constr, const
                 (e = up(d->g_cons, L.cons)) != hipSuccess || (e = up(d->g_con_loc, L.con_loc)) != hipSuccess ||
                 (e = up(d->g_inc, L.inc)) != hipSuccess || (e = up(d->g_free, L.free_atoms)) != hipSuccess ||
                 (e = d->g_iters.ensure((size_t)G)) != hipSuccess || (e = d->g_conv.ensure((size_t)G)) != hipSuccess ||
-                (e = d->g_ctl.ensure(4)) != hipSuccess ||
+                (e = d->g_ctl.ensure(5)) != hipSuccess ||
                 (e = hipHostMalloc((void **)&d->h_ctl, 8 * sizeof(int), hipHostMallocDefault)) != hipSuccess)
                 return fail(e);
             d->n_groups = G;
@@ -543,8 +542,8 @@ gm_status gm_md_run(gm_sys *s, double *x, double *y, double *z, double *vx, doub
                 GM_OK)
                 return rc;
             GM_HIP(launch_resync(n, x, y, z, rx, ry, rz, vx, vy, vz, dt, st, guard));
-            GM_HIP(launch_needs_rebuild(s, x, y, z));
-            if ((rc = read_ctl(s, s->meta.p + 5)) != GM_OK) return rc;
+            GM_HIP(launch_needs_rebuild(s, x, y, z, d->g_ctl.p + 4));
+            if ((rc = read_ctl(s)) != GM_OK) return rc;
             const bool replay = v_pending && d->h_ctl[3];
             if (v_pending) {
                 rc = group_finish(s, true, Vec3{vx, vy, vz}, Vec3C{x, y, z}, md->max_iter, md->tol, step - 1, nullptr);
@@ -578,7 +577,7 @@ gm_status gm_md_run(gm_sys *s, double *x, double *y, double *z, double *vx, doub
             if (rc != GM_OK) return rc;
             v_pending = true;
             if ((step + 1) % md->frame_interval == 0 || step + 1 == md->n_steps) {  // checked before a frame
-                if ((rc = read_ctl(s, nullptr)) != GM_OK) return rc;
+                if ((rc = read_ctl(s)) != GM_OK) return rc;
                 rc = group_finish(s, true, Vec3{vx, vy, vz}, Vec3C{x, y, z}, md->max_iter, md->tol, step, nullptr);
                 if (rc != GM_OK) return rc;
                 v_pending = false;

@@ -307,7 +307,8 @@ hipError_t launch_build(gm_sys *s, bool wrap);
 hipError_t launch_extent(gm_sys *s, double *out);
 hipError_t launch_pair_count(gm_sys *s);
 hipError_t launch_pair_fill_and_masks(gm_sys *s);
-hipError_t launch_needs_rebuild(gm_sys *s, const double *x, const double *y, const double *z);
+// flag: where the answer goes (null: the system's own word, meta[5])
+hipError_t launch_needs_rebuild(gm_sys *s, const double *x, const double *y, const double *z, int *flag = nullptr);
 // gm_nonbonded.hip: tiles + bonded terms in one launch
 // (shell: tiles only, with nbp_shell)
 // (after launch_prep: slot-order fixed-point positions and the displacement bound)
