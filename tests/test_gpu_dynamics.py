@@ -138,14 +138,16 @@ def test_constraint_solver_reports_non_convergence():
     assert msgs[0] == msgs[1] == msgs[2]
 
 
-def test_component_larger_than_a_group_uses_iterations():
-    """A 300-atom chain with every bond constrained is one component of 300
-    atoms: no groups, the per-iteration launches, the oracle's bytes."""
-    say("[gpu] dynamics primitives, 300-atom constrained chain")
+@pytest.mark.parametrize("chain,groups", [(300, 0), (256, 1), (257, 0)])
+def test_constrained_chain_at_and_past_group_capacity(chain, groups):
+    """A chain with every bond constrained is one component of `chain` atoms:
+    256 atoms (255 constraints) still fit one group, 257 or 300 do not and
+    take the per-iteration launches.  The oracle's bytes either way."""
+    say(f"[gpu] dynamics primitives, {chain}-atom constrained chain")
     n = 320
     s = systems.protein_solvent(n, (n / 0.1) ** (1 / 3), 300, seed=53)
     x, y, z = s.x.copy(), s.y.copy(), s.z.copy()
-    ci = np.arange(0, 299, dtype=np.uint32)
+    ci = np.arange(0, chain - 1, dtype=np.uint32)
     cj = ci + 1
     dsq = (x[ci] - x[cj]) ** 2 + (y[ci] - y[cj]) ** 2 + (z[ci] - z[cj]) ** 2
     z32, zf = np.zeros(0, np.uint32), np.zeros(0)
@@ -154,7 +156,7 @@ def test_component_larger_than_a_group_uses_iterations():
     mass = np.where(np.arange(n) % 2 == 0, 12.0, 14.0)
     g = gh.System(s.sigma, s.epsilon, s.topo, s.params)
     g.dynamics_set(mass, ci, cj, dsq)
-    assert g.set_constraint_groups(True, 0) == 0
+    assert g.set_constraint_groups(True, 0) == groups
     rng = np.random.default_rng(7)
     vx, vy, vz = (rng.normal(0, 1.0, n) for _ in range(3))
     d = [dev(a) for a in (x, y, z, vx, vy, vz)]
