@@ -50,7 +50,12 @@ struct gm_dyn {
     DevBuf<uint16_t> g_inc;
     bool groups_on = true;
     int group_cap = 0;      // > 0: phase 0 stops there (the per-iteration kernels continue)
-    int *h_ctl = nullptr;   // pinned copy of g_ctl: [0..1] SHAKE, [2..3] RATTLE, [4] the run loop's rebuild test
+    // Control words: g_ctl[4 kind + 2 buf + {0, 1}] (kind 0 SHAKE, 1 RATTLE;
+    // two buffers per kind, alternating, each cleared by the other's phase 1),
+    // g_ctl[8] the run loop's rebuild test; h_ctl is their pinned copy.
+    int *h_ctl = nullptr;
+    int ctl_buf[2] = {0, 0};   // buffer the next solve of each kind uses
+    int ctl_last[2] = {0, 0};  // buffer the last launched solve used
 };
 
 void gm_dyn_free(gm_dyn *d) {
@@ -106,7 +111,9 @@ GroupArgs group_args(gm_dyn *d, bool velocity) {
     G.n_free = d->n_free;
     G.iters = d->g_iters.p;
     G.conv = d->g_conv.p;
-    G.ctl = d->g_ctl.p + (velocity ? 2 : 0);  // position and velocity solves: separate words
+    const int k = velocity ? 1 : 0;
+    G.ctl = d->g_ctl.p + 4 * k + 2 * d->ctl_buf[k];
+    G.ctl_next = d->g_ctl.p + 4 * k + 2 * (d->ctl_buf[k] ^ 1);
     return G;
 }
 
@@ -223,9 +230,11 @@ gm_status group_launch(gm_sys *s, bool velocity, Vec3 buf0, Vec3C other, int max
     const GroupArgs G = group_args(d, velocity);
     const DynArgs D = dyn_args(d);
     const int cap = d->group_cap > 0 ? std::min(d->group_cap, max_iter) : max_iter;
-    GM_HIP(launch_fill_i32(G.ctl, 2, 0, st));
     GM_HIP(launch_group_solve(G, D, velocity, buf0, other, cap, tol, 0, d->lambda.p, st, guard));
     GM_HIP(launch_group_solve(G, D, velocity, buf0, other, cap, tol, 1, d->lambda.p, st, guard));
+    const int k = velocity ? 1 : 0;
+    d->ctl_last[k] = d->ctl_buf[k];
+    d->ctl_buf[k] ^= 1;
     return GM_OK;
 }
 
@@ -300,7 +309,8 @@ gm_status iterate_from(gm_sys *s, bool velocity, Vec3 buf0, Vec3C other, int sta
 gm_status group_finish(gm_sys *s, bool velocity, Vec3 buf0, Vec3C other, int max_iter, double tol, int64_t step,
                        int32_t *iters) {
     gm_dyn *d = s->dyn;
-    const int *c = d->h_ctl + (velocity ? 2 : 0);
+    const int k = velocity ? 1 : 0;
+    const int *c = d->h_ctl + 4 * k + 2 * d->ctl_last[k];
     if (!c[1]) {
         if (iters) *iters = c[0];
         d->last_iters[velocity ? 1 : 0] = c[0];
@@ -309,10 +319,10 @@ gm_status group_finish(gm_sys *s, bool velocity, Vec3 buf0, Vec3C other, int max
     return iterate_from(s, velocity, buf0, other, c[0], max_iter, tol, step, iters);
 }
 
-// g_ctl[0..3] (and g_ctl[4], where the run loop's rebuild test writes) into h_ctl, one copy.
+// Every control word (and g_ctl[8], where the run loop's rebuild test writes) into h_ctl, one copy.
 gm_status read_ctl(gm_sys *s) {
     gm_dyn *d = s->dyn;
-    GM_HIP(hipMemcpyAsync(d->h_ctl, d->g_ctl.p, 5 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    GM_HIP(hipMemcpyAsync(d->h_ctl, d->g_ctl.p, 9 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     GM_HIP(hipStreamSynchronize(s->stream));
     return GM_OK;
 }
@@ -438,8 +448,9 @@ gm_status gm_dynamics_set(gm_sys *s, const double *mass, int64_t n_constr, const
                 (e = up(d->g_cons, L.cons)) != hipSuccess || (e = up(d->g_con_loc, L.con_loc)) != hipSuccess ||
                 (e = up(d->g_inc, L.inc)) != hipSuccess || (e = up(d->g_free, L.free_atoms)) != hipSuccess ||
                 (e = d->g_iters.ensure((size_t)G)) != hipSuccess || (e = d->g_conv.ensure((size_t)G)) != hipSuccess ||
-                (e = d->g_ctl.ensure(5)) != hipSuccess ||
-                (e = hipHostMalloc((void **)&d->h_ctl, 8 * sizeof(int), hipHostMallocDefault)) != hipSuccess)
+                (e = d->g_ctl.ensure(9)) != hipSuccess ||
+                (e = hipMemset(d->g_ctl.p, 0, 9 * sizeof(int))) != hipSuccess ||  // every buffer starts clear
+                (e = hipHostMalloc((void **)&d->h_ctl, 9 * sizeof(int), hipHostMallocDefault)) != hipSuccess)
                 return fail(e);
             d->n_groups = G;
             d->n_free = (int64_t)L.free_atoms.size();
@@ -535,30 +546,33 @@ gm_status gm_md_run(gm_sys *s, double *x, double *y, double *z, double *vx, doub
         int32_t rebuild = 0;
         bool stepped = false;
         if (deferred) {
-            const int *guard = v_pending ? d->g_ctl.p + 3 : nullptr;
+            const int *guard = v_pending ? d->g_ctl.p + 4 + 2 * d->ctl_last[1] + 1 : nullptr;
             GM_HIP(launch_half_kick(n, vx, vy, vz, fx, fy, fz, d->mass.p, dt / 2.0, st, guard));
             GM_HIP(launch_drift(n, x, y, z, vx, vy, vz, rx, ry, rz, dt, st, guard));
             if ((rc = group_launch(s, false, Vec3{x, y, z}, Vec3C{rx, ry, rz}, md->max_iter, md->tol, guard)) !=
                 GM_OK)
                 return rc;
             GM_HIP(launch_resync(n, x, y, z, rx, ry, rz, vx, vy, vz, dt, st, guard));
-            GM_HIP(launch_needs_rebuild(s, x, y, z, d->g_ctl.p + 4));
+            GM_HIP(launch_needs_rebuild(s, x, y, z, d->g_ctl.p + 8));
             if ((rc = read_ctl(s)) != GM_OK) return rc;
-            const bool replay = v_pending && d->h_ctl[3];
+            const bool replay = v_pending && d->h_ctl[4 + 2 * d->ctl_last[1] + 1];
+            // the guarded SHAKE did nothing and cleared nothing: its buffer, still clear, is used again
+            if (replay) d->ctl_buf[0] = d->ctl_last[0];
             if (v_pending) {
                 rc = group_finish(s, true, Vec3{vx, vy, vz}, Vec3C{x, y, z}, md->max_iter, md->tol, step - 1, nullptr);
                 if (rc != GM_OK) return rc;
                 v_pending = false;
             }
             if (!replay) {
-                if (d->h_ctl[1]) {  // SHAKE continues per iteration: resync and test again after it
+                // SHAKE continues per iteration: resync and test again after it
+                if (d->h_ctl[2 * d->ctl_last[0] + 1]) {
                     rc = group_finish(s, false, Vec3{x, y, z}, Vec3C{rx, ry, rz}, md->max_iter, md->tol, step,
                                       nullptr);
                     if (rc != GM_OK) return rc;
                     GM_HIP(launch_resync(n, x, y, z, rx, ry, rz, vx, vy, vz, dt, st));
                     if ((rc = gm_needs_rebuild_device(s, x, y, z, &rebuild)) != GM_OK) return rc;
                 } else {
-                    rebuild = d->h_ctl[4] ? 1 : 0;
+                    rebuild = d->h_ctl[8] ? 1 : 0;
                 }
                 stepped = true;
             }

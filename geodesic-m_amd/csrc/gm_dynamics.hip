@@ -229,6 +229,10 @@ __global__ __launch_bounds__(256) void k_group_solve(GroupArgs A, DynArgs D, Vec
     }
     int it = 0, target = cap;
     if (phase == 1) {
+        if (g == 0 && t == 0) {
+            A.ctl_next[0] = 0;
+            A.ctl_next[1] = 0;
+        }
         target = A.ctl[0];
         it = A.iters[g];
         if (it == target) {  // this group set the count (or reached it): converged there or not

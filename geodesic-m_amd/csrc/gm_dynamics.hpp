@@ -47,6 +47,7 @@ struct GroupArgs {
     int *iters;                // per group: iterations done
     int *conv;                 // per group: converged at iters
     int *ctl;                  // [0] largest iters over groups, [1] some group not converged at ctl[0]
+    int *ctl_next;             // the next solve's words, cleared by phase 1 (no fill launch)
 };
 
 // A guard (nullable) makes a kernel do nothing while *guard is nonzero: the
