@@ -81,14 +81,17 @@ gm_status make_plan(gm_batch *b) {
     }
     std::vector<int2> items(bonded_items);
     items.insert(items.end(), tile_items.begin(), tile_items.end());
-    // finish: one wavefront per (replica, supercluster), sixteen per workgroup,
-    // whether or not the replica keeps i-force records (a batch's replicas
-    // have few records per supercluster; fewer, fuller workgroups finish sooner)
+    // finish: the superclusters of replicas that keep i-force records first
+    // (four wavefronts each share the records), then the others (one each)
     std::vector<int2> scl;
-    for (int r = 0; r < R; r++)
-        for (int64_t k = 0; k < b->reps[(size_t)r]->nsc; k++) scl.push_back(make_int2(r, (int)k));
+    for (int pass = 0; pass < 2; pass++)
+        for (int r = 0; r < R; r++)
+            if (b->reps[(size_t)r]->fi_records == (pass == 0))
+                for (int64_t k = 0; k < b->reps[(size_t)r]->nsc; k++) scl.push_back(make_int2(r, (int)k));
     b->n_scl = (int64_t)scl.size();
-    b->n_scl_rec = 0;  // the kernel's four-wavefront range stays empty
+    b->n_scl_rec = 0;
+    for (gm_sys *s : b->reps)
+        if (s->fi_records) b->n_scl_rec += s->nsc;
     GM_HIP(b->scl.ensure(std::max<size_t>(scl.size(), 1)));
     if (!scl.empty())
         GM_HIP(hipMemcpyAsync(b->scl.p, scl.data(), sizeof(int2) * scl.size(), hipMemcpyHostToDevice, st));
