@@ -19,6 +19,12 @@ thread_local std::string g_last_error;
 
 namespace gm {
 
+gm_status nonfinite_status(const gm_sys *s) {
+    if (s->last_bad == 0x7fffffff) return GM_OK;
+    return set_error(GM_ERR_NUMERICAL, "force not finite at atom " + std::to_string(s->last_bad) +
+                                           " (forces written; non-finite atoms hold NaN)");
+}
+
 gm_status set_error(gm_status s, const std::string &msg) {
     g_last_error = msg;
     return s;
@@ -68,29 +74,37 @@ bool indices_ok(const uint32_t *a, int64_t cnt, int64_t n) {
 }
 
 // Work items: runs of at most `chunk` consecutive list entries of one
-// supercluster, one wavefront each.  Automatic chunk: about 1.5 work items per
-// resident wavefront slot (5 per SIMD), 4 to 32 entries.  Items are issued
-// heaviest first (estimated cost: tiles plus per-entry and per-item overhead,
-// in tile units): workgroups are dispatched in index order as slots free up,
-// so this is longest-processing-time-first list scheduling and the light
-// items fill the tail.
+// supercluster (whole fixed-point blocks of kBlock entries, counted from the
+// supercluster's first entry), one wavefront each.  Automatic chunk: about
+// 1.5 work items per resident wavefront slot (5 per SIMD), rounded up to whole
+// blocks, 8 to 32 entries.  The chunk only groups blocks into items: every
+// block's i forces and energy are rounded on their own, so forces and
+// energies do not depend on it.  Items are issued heaviest first (estimated
+// cost: tiles plus per-entry and per-item overhead, in tile units):
+// workgroups are dispatched in index order as slots free up, so this is
+// longest-processing-time-first list scheduling and the light items fill the
+// tail.
 gm_status make_work_items(gm_sys *s) {
     const std::vector<int> &start = s->sc_start_host;
     const uint32_t *im = s->imask_host;  // per entry i-cluster mask: popcount = tiles
     constexpr int kEntryCost2 = 1, kItemCost2 = 8;  // twice the per-entry and per-item cost, in tiles
     struct Item {
-        int sc, e0, e1;
+        int sc, e0, e1, blk0;
         int cost;  // twice the estimated cost, in tiles
     };
     int64_t chunk = s->chunk;
     if (chunk <= 0) {
         const int64_t slots = 20LL * (s->ctx->n_cu > 0 ? s->ctx->n_cu : 256);
         chunk = (2 * s->nent + 3 * slots - 1) / (3 * slots);
-        chunk = std::max<int64_t>(4, chunk);
     }
-    chunk = std::min<int64_t>(chunk, kMaxRun);
+    chunk = std::min<int64_t>(std::max<int64_t>(kBlock, (chunk + kBlock - 1) / kBlock * kBlock), kMaxRun);
     if (s->nsc > (int64_t)kWiScMask + 1)
         return set_error(GM_ERR_CAPACITY, "work items: more than 2^26 superclusters");
+    // blocks: per supercluster ceil(entries / kBlock), consecutive
+    std::vector<int> boff((size_t)s->nsc + 1, 0);
+    for (int64_t k = 0; k < s->nsc; k++)
+        boff[(size_t)k + 1] = boff[(size_t)k] + (start[(size_t)k + 1] - start[(size_t)k] + kBlock - 1) / kBlock;
+    s->n_blk = boff[(size_t)s->nsc];
     std::vector<Item> items;
     items.reserve((size_t)(s->nent / chunk + s->nsc + 1));
     for (int64_t k = 0; k < s->nsc; k++)
@@ -98,7 +112,8 @@ gm_status make_work_items(gm_sys *s) {
             const int64_t e1 = std::min<int64_t>(e0 + chunk, start[(size_t)k + 1]);
             int c = kItemCost2;
             for (int64_t e = e0; e < e1; e++) c += 2 * __builtin_popcount(im[e] & 0xffu) + kEntryCost2;
-            items.push_back({(int)k, (int)e0, (int)e1, c});
+            const int blk0 = boff[(size_t)k] + (int)((e0 - start[(size_t)k]) / kBlock);
+            items.push_back({(int)k, (int)e0, (int)e1, blk0, c});
         }
     // stable counting sort by decreasing cost (costs are small integers)
     int cmax = 0;
@@ -110,30 +125,20 @@ gm_status make_work_items(gm_sys *s) {
     std::vector<int4> wi(ni);
     for (const Item &it : items) {
         const size_t o = (size_t)first[(size_t)(cmax - it.cost)]++;
-        // (supercluster | entries << 26, first entry, record slot, j-cluster of the first entry)
-        wi[o] = make_int4(it.sc | ((it.e1 - it.e0) << kWiEntryShift), it.e0, 0, 0);
+        // (supercluster | entries << 26, first entry, first block, j-cluster of the first entry)
+        wi[o] = make_int4(it.sc | ((it.e1 - it.e0) << kWiEntryShift), it.e0, it.blk0, 0);
     }
     s->n_wi = (int64_t)wi.size();
-    // Each supercluster's work items in ascending order own consecutive record
-    // slots, so the finish kernel reads a supercluster's i-force records as
-    // one contiguous range.
-    std::vector<int> off((size_t)s->nsc + 1, 0);
-    for (const Item &it : items) off[(size_t)it.sc + 1]++;
-    for (size_t k = 1; k < off.size(); k++) off[k] += off[k - 1];
-    {
-        std::vector<int> fill(off.begin(), off.end() - 1);
-        for (size_t w = 0; w < ni; w++) wi[w].z = fill[(size_t)(wi[w].x & kWiScMask)]++;
-    }
     hipStream_t st = s->stream;
     GM_HIP(upload(s->wi, wi.data(), wi.size(), st));
-    GM_HIP(upload(s->sc_wi_off, off.data(), off.size(), st));
+    GM_HIP(upload(s->sc_blk_off, boff.data(), boff.size(), st));
     // Small work items: many i-force updates per slot, which as atomics (past
-    // the L2) cost the force kernel more than plain per-item records cost the
+    // the L2) cost the force kernel more than plain per-block records cost the
     // finish kernel; large ones: few, and atomics are cheaper.
     s->fi_records = chunk < 16;
-    if (s->fi_records) GM_HIP(s->fi_items.ensure((size_t)std::max<int64_t>(s->n_wi, 1) * 192));
+    if (s->fi_records) GM_HIP(s->fi_blk.ensure((size_t)std::max<int64_t>(s->n_blk, 1) * 192));
     GM_HIP(launch_work_item_cj0(s));  // .w = ent_cj[first entry], from the device list
-    GM_HIP(s->e_wi.ensure((size_t)std::max<int64_t>(s->n_wi, 1)));
+    GM_HIP(s->e_blk.ensure((size_t)std::max<int64_t>(s->n_blk, 1)));
     GM_HIP(hipStreamSynchronize(st));
     s->stats.n_work_items = s->n_wi;
     return GM_OK;
@@ -351,8 +356,9 @@ gm_status compute_on_device(gm_sys *s, const double *x, const double *y, const d
 
 gm_status collect(gm_sys *s, gm_energy_terms *e) {
     GM_HIP(hipStreamSynchronize(s->stream));
-    double t[5] = {0, 0, 0, 0, 0};
+    double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     GM_HIP(hipMemcpy(t, s->e_out.p, sizeof(t), hipMemcpyDeviceToHost));
+    std::memcpy(&s->last_bad, &t[6], sizeof(int));
     if (s->last_timed >= 0) {
         const size_t k = kProfStride * (size_t)s->last_timed;
         float a = 0.f, b = 0.f;
@@ -654,9 +660,12 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     TRY(s->sc_ref.ensure((size_t)s->max_sc));
     TRY(s->sc_half.ensure((size_t)s->max_sc));
     TRY(s->F.ensure(3 * slots));
+    TRY(s->bad.ensure(slots));
+    TRY(launch_fill_i32(reinterpret_cast<int *>(s->bad.p), slots, 0, st));
     TRY(s->e_items.ensure((size_t)(s->nb + s->na + s->nd + s->n14)));
     TRY(s->e_out.ensure(8));
-    TRY(s->e_wi.ensure(1));
+    TRY(launch_fill_i32(bad_first_word(s), 1, 0x7fffffff, st));
+    TRY(s->e_blk.ensure(1));
     TRY(hipStreamSynchronize(st));
 #undef TRY
     *out = s;
@@ -672,7 +681,7 @@ gm_status gm_system_destroy(gm_sys *s) {
     s->dyn = nullptr;
     DevBuf<double> *dbl[] = {&s->sigma, &s->eps, &s->bk, &s->br0, &s->akth, &s->ath0, &s->dkphi,
                              &s->ddelta, &s->x, &s->y, &s->z, &s->refx, &s->refy, &s->refz,
-                             &s->fx, &s->fy, &s->fz, &s->sx, &s->sy, &s->sz, &s->e_items, &s->e_wi, &s->e_out,
+                             &s->fx, &s->fy, &s->fz, &s->sx, &s->sy, &s->sz, &s->e_items, &s->e_blk, &s->e_out,
                              &s->ext};
     for (auto *b : dbl) b->release();
     DevBuf<uint32_t> *u32[] = {&s->bi, &s->bj, &s->ai, &s->aj, &s->ak, &s->di, &s->dj, &s->dk,
@@ -693,8 +702,9 @@ gm_status gm_system_destroy(gm_sys *s) {
     s->sq.release();
     s->sxb.release();
     s->wi.release();
-    s->fi_items.release();
-    s->sc_wi_off.release();
+    s->fi_blk.release();
+    s->sc_blk_off.release();
+    s->bad.release();
     s->cl_ref.release();
     s->sc_ref.release();
     s->cl_half.release();
@@ -719,6 +729,7 @@ gm_status gm_system_destroy(gm_sys *s) {
 
 gm_status gm_build_neighbor_list(gm_sys *s, double *x, double *y, double *z) {
     GM_REQUIRE(s, "NULL system");
+    GM_REQUIRE(!s->batch_owned, "gm_build_neighbor_list: a batch replica's list is built by gm_batch_build_neighbor_lists");
     GM_REQUIRE(s->n == 0 || (x && y && z), "NULL positions");
     const size_t bytes = sizeof(double) * (size_t)s->n;
     hipStream_t st = s->stream;
@@ -740,6 +751,8 @@ gm_status gm_build_neighbor_list(gm_sys *s, double *x, double *y, double *z) {
 
 gm_status gm_build_neighbor_list_device(gm_sys *s, double *dx, double *dy, double *dz) {
     GM_REQUIRE(s, "NULL system");
+    GM_REQUIRE(!s->batch_owned,
+               "gm_build_neighbor_list_device: a batch replica's list is built by gm_batch_build_neighbor_lists");
     GM_REQUIRE(s->n == 0 || (dx && dy && dz), "NULL positions");
     const size_t bytes = sizeof(double) * (size_t)s->n;
     hipStream_t st = s->stream;
@@ -810,7 +823,8 @@ gm_status gm_compute_forces(gm_sys *s, const double *x, const double *y, const d
         GM_HIP(hipMemcpyAsync(fy, s->fy.p, bytes, hipMemcpyDeviceToHost, st));
         GM_HIP(hipMemcpyAsync(fz, s->fz.p, bytes, hipMemcpyDeviceToHost, st));
     }
-    return collect(s, e);
+    rc = collect(s, e);
+    return rc != GM_OK ? rc : nonfinite_status(s);
 }
 
 gm_status gm_compute_forces_device(gm_sys *s, const double *dx, const double *dy, const double *dz,
@@ -855,7 +869,8 @@ gm_status gm_get_forces(gm_sys *s, double *fx, double *fy, double *fz) {
 
 gm_status gm_sync(gm_sys *s, gm_energy_terms *e) {
     GM_REQUIRE(s, "NULL system");
-    return collect(s, e);
+    gm_status rc = collect(s, e);
+    return rc != GM_OK ? rc : nonfinite_status(s);
 }
 
 gm_status gm_get_pair_list(gm_sys *s, uint32_t *pi, uint32_t *pj, int64_t cap, int64_t *nout) {
@@ -942,6 +957,7 @@ gm_status gm_profile_end(gm_sys *s, gm_profile *out) {
 
 gm_status gm_set_work_chunk(gm_sys *s, int32_t entries_per_item) {
     GM_REQUIRE(s && entries_per_item >= 0, "bad argument");
+    GM_REQUIRE(!s->batch_owned, "gm_set_work_chunk: a batch replica's work shape is set by gm_batch_set_shape");
     s->chunk = entries_per_item;
     if (!s->built) return GM_OK;
     return make_work_items(s);

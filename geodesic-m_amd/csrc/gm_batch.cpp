@@ -8,7 +8,7 @@
 // work item) are written.  k_batch_prep writes every replica's slot-order
 // fixed-point positions and displacement bound; then k_batch_bonded runs the
 // bonded terms on a side branch while the persistent k_batch_tiles walks the
-// tile items with a fixed grid of wavefronts (static, XCD-contiguous ranges); k_batch_finish
+// tile items with a fixed grid of wavefronts (per-XCD work counters); k_batch_finish
 // then reduces per-replica f64 energies and writes the concatenated f64
 // forces.  The three kernels are captured once per list build into a
 // hipGraph, so a batched evaluation is one graph launch (the four kernels).  Every item runs the
@@ -39,6 +39,8 @@ struct gm_batch {
     int64_t n_scl = 0;
     int64_t n_scl_rec = 0;  // pairs finished by four wavefronts each (the first n_scl_rec of scl)
     DevBuf<double> fx, fy, fz, E;
+    DevBuf<int> bad_first;  // per replica, the first atom whose force is not finite (INT_MAX if none)
+    DevBuf<unsigned> ctr;   // the tile kernel's per-XCD work counters
     int n_items = 0;
     int chunk = 0;         // entries per tile work item (0 = automatic)
     int waves_per_cu = 16; // persistent wavefronts per compute unit
@@ -76,6 +78,8 @@ gm_status make_plan(gm_batch *b) {
         d.terms = term_args(s, s->x.p, s->y.p, s->z.p);
         d.n_bonded_groups = (int)((d.terms.nb + d.terms.na + d.terms.nd + d.terms.n14 + 63) / 64);
         d.n_wi = (int)s->n_wi;
+        d.n_blk = (int)s->n_blk;
+        d.tile.bad_first = d.prep.bad_first = b->bad_first.p + r;
         for (int g = 0; g < d.n_bonded_groups; g++) bonded_items.push_back(make_int2(r, g));
         for (int w = 0; w < d.n_wi; w++) tile_items.push_back(make_int2(r, w));
     }
@@ -110,13 +114,13 @@ gm_status make_plan(gm_batch *b) {
     // (the prep runs alone: sharing the machine with the bonded terms would
     // stretch it, and the tiles wait for it)
     GM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    hipError_t e1 = launch_batch_prep(b->d_reps.p, b->atom_rep.p, b->atom_off.p, b->off.back(), st);
+    hipError_t e1 = launch_batch_prep(b->d_reps.p, b->atom_rep.p, b->atom_off.p, b->off.back(), b->ctr.p, st);
     if (e1 == hipSuccess) e1 = hipEventRecord(b->fork, st);
     if (e1 == hipSuccess) e1 = hipStreamWaitEvent(b->side, b->fork, 0);
     if (e1 == hipSuccess) e1 = launch_batch_bonded(b->d_reps.p, b->d_items.p, b->n_bonded_items, b->side);
     if (e1 == hipSuccess) e1 = hipEventRecord(b->join, b->side);
     if (e1 == hipSuccess)
-        e1 = launch_batch_tiles(b->d_reps.p, b->d_items.p + b->n_bonded_items, b->n_items, grid, st);
+        e1 = launch_batch_tiles(b->d_reps.p, b->d_items.p + b->n_bonded_items, b->n_items, grid, b->ctr.p, st);
     if (e1 == hipSuccess) e1 = hipStreamWaitEvent(st, b->join, 0);
     if (e1 == hipSuccess)
         e1 = launch_batch_finish(b->d_reps.p, R, b->scl.p, b->n_scl_rec, b->n_scl - b->n_scl_rec, b->atom_off.p,
@@ -168,6 +172,7 @@ gm_status gm_batch_create(gm_ctx *ctx, const gm_system_desc *descs, int32_t n_re
         gm_sys *s = nullptr;
         gm_status rc = system_create_impl(ctx, &descs[r], b->stream, &s);
         if (rc != GM_OK) return fail(rc);
+        s->batch_owned = true;
         b->reps.push_back(s);
         b->off.push_back(b->off.back() + descs[r].n_atoms);
     }
@@ -177,8 +182,12 @@ gm_status gm_batch_create(gm_ctx *ctx, const gm_system_desc *descs, int32_t n_re
         for (int64_t a = b->off[(size_t)r]; a < b->off[(size_t)r + 1]; a++) rep[(size_t)a] = r;
     if ((e = b->atom_rep.ensure((size_t)std::max<int64_t>(n, 1))) != hipSuccess ||
         (e = b->atom_off.ensure(b->off.size())) != hipSuccess ||
-        (e = b->E.ensure((size_t)n_replicas * 5)) != hipSuccess)
+        (e = b->E.ensure((size_t)n_replicas * 5)) != hipSuccess ||
+        (e = b->bad_first.ensure((size_t)n_replicas)) != hipSuccess ||
+        (e = b->ctr.ensure((size_t)kCtrStride * 8)) != hipSuccess)
         return fail(from_hip(e, "hipMalloc"));
+    if ((e = launch_fill_i32(b->bad_first.p, (size_t)n_replicas, 0x7fffffff, b->stream)) != hipSuccess)
+        return fail(from_hip(e, "fill"));
     for (DevBuf<double> *d : {&b->fx, &b->fy, &b->fz})
         if ((e = d->ensure((size_t)std::max<int64_t>(n, 1))) != hipSuccess) return fail(from_hip(e, "hipMalloc"));
     if (n > 0 && (e = hipMemcpy(b->atom_rep.p, rep.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice)) != hipSuccess)
@@ -203,6 +212,8 @@ gm_status gm_batch_destroy(gm_batch *b) {
     b->fy.release();
     b->fz.release();
     b->E.release();
+    b->bad_first.release();
+    b->ctr.release();
     for (hipEvent_t ev : {b->ev0, b->ev1, b->fork, b->join})
         if (ev) (void)hipEventDestroy(ev);
     if (b->side) (void)hipStreamDestroy(b->side);
@@ -281,8 +292,10 @@ gm_status gm_batch_sync(gm_batch *b, gm_energy_terms *e, float *ms) {
         GM_HIP(hipEventElapsedTime(&t, b->ev0, b->ev1));
         *ms = t;
     }
+    const size_t R = b->reps.size();
+    std::vector<int> bad(R);
+    GM_HIP(hipMemcpy(bad.data(), b->bad_first.p, sizeof(int) * R, hipMemcpyDeviceToHost));
     if (e) {
-        const size_t R = b->reps.size();
         std::vector<double> E(R * 5);
         GM_HIP(hipMemcpy(E.data(), b->E.p, sizeof(double) * R * 5, hipMemcpyDeviceToHost));
         for (size_t r = 0; r < R; r++) {
@@ -301,6 +314,10 @@ gm_status gm_batch_sync(gm_batch *b, gm_energy_terms *e, float *ms) {
             e[r].total = tot;
         }
     }
+    for (size_t r = 0; r < R; r++)
+        if (bad[r] != 0x7fffffff)
+            return set_error(GM_ERR_NUMERICAL, "force not finite in replica " + std::to_string(r) + " at atom " +
+                                                   std::to_string(bad[r]) + " (forces written; those atoms hold NaN)");
     return GM_OK;
 }
 
@@ -331,8 +348,8 @@ gm_status gm_batch_set_shape(gm_batch *b, int32_t entries_per_item, int32_t wave
 gm_status gm_batch_autotune(gm_batch *b, int32_t *entries_per_item, int32_t *waves_per_cu, float *best_ms) {
     GM_REQUIRE(b, "NULL batch");
     if (!b->built) return set_error(GM_ERR_NOT_BUILT, "gm_batch_autotune before gm_batch_build_neighbor_lists");
-    static const int chunks[] = {4, 6, 8, 12, 16};
-    static const int waves[] = {8, 16, 24};
+    static const int chunks[] = {8, 16, 24, 32};  // whole fixed-point blocks
+    static const int waves[] = {8, 12, 16, 20};  // at most 5 per SIMD are resident
     float best = 1e30f;
     int bc = b->chunk, bw = b->waves_per_cu;
     for (int c : chunks) {

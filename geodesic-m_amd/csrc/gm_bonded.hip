@@ -55,49 +55,61 @@ __device__ inline double block_sum(const double *ptr, long long len, double *sh)
     return r;
 }
 
-// Slot s's force in fixed point: the accumulator (j forces, bonded terms)
-// plus the i-force records of its supercluster's work items.  Integer sums:
-// the order does not change the result.
+// v * scale rounded to an integer, as the force kernel rounds the i forces
+// it adds itself (so records and atomics give the same integers).
 __device__ inline long long to_fixed(float v, float scale) { return (long long)__builtin_rintf(v * scale); }
 
 // Forces of one supercluster's 64 slots, by G (1, 4, 8 or 16) wavefronts of a
 // 1024-thread workgroup (wavefronts G g .. G g + G - 1 for the workgroup's
 // supercluster g): wavefront q adds the i-force records of the
-// supercluster's work items q, q + G, ... in fixed point; the first then
-// adds the others' sums and the accumulator (j forces, bonded terms), clears
-// the accumulator and writes each slot's f64 force to its atom (atom index +
-// atom_base).  Integer sums: the order does not change the result.  Every
+// supercluster's blocks q, q + G, ... in fixed point; the first then adds the
+// others' sums and the accumulator (j forces, bonded terms), clears the
+// accumulator and writes each slot's f64 force to its atom (atom index +
+// atom_base).  Integer sums: the order does not change the result.  A slot
+// marked in bad (a contribution that was not finite or out of the fixed-point
+// range, or a position that was not finite), or with a record that is, gets
+// NaN forces, and its atom is raised into *bad_first (atomic minimum).  Every
 // thread of the workgroup calls it (it holds a barrier); valid says whether
 // group g has a supercluster.
 __device__ inline void sc_finish(bool valid, int sc, int G, unsigned long long *F, size_t n_pad,
-                                 const float *fi_items, const int *sc_wi_off, float fi_scale,
-                                 const int *slot_atom, int64_t atom_base, double *fx, double *fy, double *fz) {
+                                 const float *fi_blk, const int *sc_blk_off, float fi_scale,
+                                 const int *slot_atom, int64_t atom_base, double *fx, double *fy, double *fz,
+                                 unsigned *bad, int *bad_first) {
     __shared__ long long red[16][3][64];
+    __shared__ unsigned redb[16][64];
     const int t = threadIdx.x, wv = t >> 6, l = t & 63, q = wv & (G - 1);
     const size_t s = (size_t)sc * 64 + l;
     // the summing wavefront's accumulator words and atom index load with the records
     unsigned long long f[3] = {0ull, 0ull, 0ull};
     int a = -1;
+    unsigned bd = 0u;
     if (valid && q == 0) {
 #pragma unroll
         for (int c = 0; c < 3; c++) f[c] = F[(size_t)c * n_pad + s];
         a = slot_atom[s];
+        bd = bad[s];
     }
     long long ax = 0, ay = 0, az = 0;
-    if (valid && fi_items) {  // (null: the i forces are already in the accumulator)
+    unsigned rb = 0u;
+    if (valid && fi_blk) {  // (null: the i forces are already in the accumulator)
         // the supercluster's records are consecutive: wavefront q adds every G-th
-        const int k0 = sc_wi_off[sc], k1 = sc_wi_off[sc + 1];
+        const int k0 = sc_blk_off[sc], k1 = sc_blk_off[sc + 1];
 #pragma unroll 2
         for (int k = k0 + q; k < k1; k += G) {
-            const float *fw = fi_items + (size_t)k * 192 + l;
-            ax += to_fixed(fw[0], fi_scale);
-            ay += to_fixed(fw[64], fi_scale);
-            az += to_fixed(fw[128], fi_scale);
+            const float *fw = fi_blk + (size_t)k * 192 + l;
+            const float vx = fw[0], vy = fw[64], vz = fw[128];
+            const float m = fmaxf(fabsf(vx), fmaxf(fabsf(vy), fabsf(vz)));
+            rb |= (!(m * fi_scale < kFixedLimitF) || __builtin_isnan(vx) || __builtin_isnan(vy) ||
+                   __builtin_isnan(vz)) ? 1u : 0u;
+            ax += to_fixed(vx, fi_scale);
+            ay += to_fixed(vy, fi_scale);
+            az += to_fixed(vz, fi_scale);
         }
     }
     red[wv][0][l] = ax;
     red[wv][1][l] = ay;
     red[wv][2][l] = az;
+    redb[wv][l] = rb;
     __syncthreads();
     if (!valid || q != 0) return;
 #pragma unroll
@@ -105,7 +117,16 @@ __device__ inline void sc_finish(bool valid, int sc, int G, unsigned long long *
         for (int o = 0; o < G; o++) f[c] += (unsigned long long)red[wv + o][c][l];
         F[(size_t)c * n_pad + s] = 0ull;
     }
+    for (int o = 0; o < G; o++) bd |= redb[wv + o][l];
+    if (bd) bad[s] = 0u;
     if (a < 0) return;
+    if (bd) {
+        fx[atom_base + a] = __builtin_nan("");
+        fy[atom_base + a] = __builtin_nan("");
+        fz[atom_base + a] = __builtin_nan("");
+        atomicMin(bad_first, a);
+        return;
+    }
     fx[atom_base + a] = (double)(long long)f[0] * kFixedInv;
     fy[atom_base + a] = (double)(long long)f[1] * kFixedInv;
     fz[atom_base + a] = (double)(long long)f[2] * kFixedInv;
@@ -116,9 +137,10 @@ __device__ inline void sc_finish(bool valid, int sc, int G, unsigned long long *
 // the prep kernel raised.
 __global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, const int *slot_atom,
                                                  unsigned long long *F, size_t n_pad,
-                                                 const float *fi_items, const int *sc_wi_off,
+                                                 const float *fi_blk, const int *sc_blk_off,
                                                  float fi_scale, int G, double *fx,
-                                                 double *fy, double *fz, unsigned *disp) {
+                                                 double *fy, double *fz, unsigned *disp, unsigned *bad,
+                                                 int *bad_first) {
     __shared__ double sh[16];
     const int b = blockIdx.x, t = threadIdx.x;
     if (b == 0 && t < kDispSlots) disp[t * kDispStride] = 0u;
@@ -128,8 +150,8 @@ __global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, const in
         return;
     }
     const int sc = (b - sg.nseg) * (16 / G) + (t >> 6) / G;
-    sc_finish((size_t)sc * 64 < n_pad, sc, G, F, n_pad, fi_items, sc_wi_off, fi_scale, slot_atom, 0,
-              fx, fy, fz);
+    sc_finish((size_t)sc * 64 < n_pad, sc, G, F, n_pad, fi_blk, sc_blk_off, fi_scale, slot_atom, 0,
+              fx, fy, fz, bad, bad_first);
 }
 
 // Batched end of evaluation: blocks [0, R) sum replica b's five segments
@@ -149,8 +171,8 @@ __global__ __launch_bounds__(1024) void k_batch_finish(const ReplicaDesc *__rest
         if (t < kDispSlots) R.prep.disp[t * kDispStride] = 0u;  // bound of the next evaluation
         const double *eb = R.terms.e_items;
         const double *ptr[5] = {eb, eb + R.terms.nb, eb + R.terms.nb + R.terms.na,
-                                eb + R.terms.nb + R.terms.na + R.terms.nd, R.tile.e_wi};
-        const long long len[5] = {R.terms.nb, R.terms.na, R.terms.nd, R.terms.n14, R.n_wi};
+                                eb + R.terms.nb + R.terms.na + R.terms.nd, R.tile.e_blk};
+        const long long len[5] = {R.terms.nb, R.terms.na, R.terms.nd, R.terms.n14, R.n_blk};
         for (int seg = 0; seg < 5; seg++) {
             const double v = block_sum(ptr[seg], len[seg], sh);
             if (t == 0) eout[5 * b + seg] = v;
@@ -165,8 +187,8 @@ __global__ __launch_bounds__(1024) void k_batch_finish(const ReplicaDesc *__rest
     const bool valid = bb < b_rec ? pidx < n_rec : pidx < n_rec + n_atm;
     const int2 rs = valid ? scl[pidx] : make_int2(0, 0);
     const ReplicaDesc &R = reps[rs.x];
-    sc_finish(valid, rs.y, G, R.terms.F, R.terms.n_pad, R.tile.fi_items, R.tile.sc_wi_off,
-              R.tile.fi_scale, R.tile.slot_atom, atom_off[rs.x], fx, fy, fz);
+    sc_finish(valid, rs.y, G, R.terms.F, R.terms.n_pad, R.tile.fi_blk, R.tile.sc_blk_off,
+              R.tile.fi_scale, R.tile.slot_atom, atom_off[rs.x], fx, fy, fz, R.tile.bad, R.tile.bad_first);
 }
 
 }  // namespace
@@ -190,23 +212,24 @@ hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz, bool lj_
     sg.ptr[1] = eb + s->nb;                 sg.len[1] = na;
     sg.ptr[2] = eb + s->nb + s->na;         sg.len[2] = nd;
     sg.ptr[3] = eb + s->nb + s->na + s->nd; sg.len[3] = lj_only ? 0 : n14;
-    sg.ptr[4] = s->e_wi.p;                  sg.len[4] = s->n_wi;
+    sg.ptr[4] = s->e_blk.p;                 sg.len[4] = s->n_blk;
     sg.nseg = 5;
     const size_t n_pad = (size_t)s->nsc * 64;
-    // wavefronts per supercluster: about one work-item record each, 4 to 16;
+    // wavefronts per supercluster: about one block record each, 4 to 16;
     // one when there are no records to add
-    const int64_t per_sc = (s->fi_records && s->nsc > 0) ? (s->n_wi + s->nsc - 1) / s->nsc : 1;
+    const int64_t per_sc = (s->fi_records && s->nsc > 0) ? (s->n_blk + s->nsc - 1) / s->nsc : 1;
     const int G = !s->fi_records ? 1 : (per_sc > 8 ? 16 : (per_sc > 4 ? 8 : 4));
     const unsigned grid = 5u + (unsigned)((s->nsc + 16 / G - 1) / (16 / G));
     const float fi_scale = span_is_cubic(s) ? s->nbp.fscale_u : 4294967296.0f;
-    const float *fi = s->fi_records ? s->fi_items.p : nullptr;
+    const float *fi = s->fi_records ? s->fi_blk.p : nullptr;
+    int *bf = bad_first_word(s);
     if (ev_stop)  // stamped by the dispatch at the kernel's end
         hipExtLaunchKernelGGL(k_finish, dim3(grid), dim3(1024), 0, s->stream, nullptr, ev_stop, 0, sg, s->e_out.p,
-                              (const int *)s->slot_atom.p, s->F.p, n_pad, fi, (const int *)s->sc_wi_off.p,
-                              fi_scale, G, fx, fy, fz, s->disp.p);
+                              (const int *)s->slot_atom.p, s->F.p, n_pad, fi, (const int *)s->sc_blk_off.p,
+                              fi_scale, G, fx, fy, fz, s->disp.p, s->bad.p, bf);
     else
-        k_finish<<<grid, 1024, 0, s->stream>>>(sg, s->e_out.p, s->slot_atom.p, s->F.p, n_pad, fi, s->sc_wi_off.p,
-                                                fi_scale, G, fx, fy, fz, s->disp.p);
+        k_finish<<<grid, 1024, 0, s->stream>>>(sg, s->e_out.p, s->slot_atom.p, s->F.p, n_pad, fi, s->sc_blk_off.p,
+                                                fi_scale, G, fx, fy, fz, s->disp.p, s->bad.p, bf);
     return hipGetLastError();
 }
 

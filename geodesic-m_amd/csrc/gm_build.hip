@@ -283,15 +283,19 @@ __global__ __launch_bounds__(64) void k_sc_split(const int *meta, int n, const u
 // Slot tables.  qpos/qeps (the force kernel's per-slot inputs) are filled for
 // every slot, padding included, so the per-evaluation prep kernel only needs
 // to overwrite the atoms' slots.
+// sq.w = 1 marks an atom whose position is not finite: such an atom is in no
+// listed pair (the reference's r2 <= r_skin^2 is false for a NaN distance).
 __global__ void k_slot_setup(int nslots, int n, const int *slot_atom, const uint32_t *q,
                              const float2 *atom_par, const uint2 *excl_bits, int *atom_slot, int4 *sq,
-                             float2 *sp, int4 *qpos, float *qeps, uint2 *sxb, float inv_s) {
+                             float2 *sp, int4 *qpos, float *qeps, uint2 *sxb, float inv_s,
+                             const double *x, const double *y, const double *z) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
     int a = slot_atom[s];
     if (a >= 0) {
         atom_slot[a] = s;
-        const int4 v = make_int4((int)q[a], (int)q[n + a], (int)q[2 * n + a], 0);
+        const bool fin = isfinite(x[a]) && isfinite(y[a]) && isfinite(z[a]);
+        const int4 v = make_int4((int)q[a], (int)q[n + a], (int)q[2 * n + a], fin ? 0 : 1);
         const float2 p = atom_par[a];
         sq[s] = v;
         sp[s] = p;
@@ -483,8 +487,8 @@ __global__ __launch_bounds__(256) void k_masks(int nent, const int *ent_sc, cons
         const int sc = ent_sc[e], cj = ent_cj[e];
         const uint32_t im = ent_imask[e];
         const size_t js = (size_t)cj * 8 + jl;
-        const int b = slot_atom[js];
         const int4 qb = sq[js];
+        const int b = qb.w ? -1 : slot_atom[js];  // (a non-finite position is never listed)
         const uint2 xbj = sxb[js];
         // per-cluster values in arrays of scalars (no vector structs), which
         // stay in registers once the loops are unrolled; the exclusion masks
@@ -498,8 +502,8 @@ __global__ __launch_bounds__(256) void k_masks(int nent, const int *ent_sc, cons
             exl[ci] = exh[ci] = 0u;
             if (im & (1u << ci)) {
                 const size_t is = (size_t)sc * 64 + ci * 8 + il;
-                a[ci] = slot_atom[is];
                 const int4 q = sq[is];
+                a[ci] = q.w ? -1 : slot_atom[is];
                 qax[ci] = q.x;
                 qay[ci] = q.y;
                 qaz[ci] = q.z;
@@ -649,16 +653,30 @@ __global__ __launch_bounds__(1024) void k_sum_parts(const unsigned long long *pa
     if (t < 4) out[t] = sm[t][0];
 }
 
-// neighbor.rs:60-72, same f64 expression; any thread past the threshold sets the flag.
+// Clears the rebuild test's words: flag[0] (rebuild) = 0, flag[1] (first
+// atom with a non-finite position) = INT_MAX, and copies the first atom with
+// a non-finite force of the last evaluation into flag[2].
+__global__ void k_rebuild_words(int *flag, const int *bad_first) {
+    flag[0] = 0;
+    flag[1] = 0x7fffffff;
+    flag[2] = *bad_first;
+}
+
+// neighbor.rs:60-72, same f64 expression; any thread past the threshold sets
+// flag[0].  A position that is not finite (never past the threshold, as in
+// the reference) is raised into flag[1] (atomic minimum of the atom index):
+// the run loop's check_positions_finite (geodesic/src/lib.rs:94-107).
 __global__ void k_needs_rebuild(int n, const double *x, const double *y, const double *z,
                                 const double *rx, const double *ry, const double *rz,
                                 double threshold_sq, int *flag) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double dx = x[i] - rx[i];
-    double dy = y[i] - ry[i];
-    double dz = z[i] - rz[i];
-    if (dx * dx + dy * dy + dz * dz > threshold_sq) *flag = 1;
+    const double xi = x[i], yi = y[i], zi = z[i];
+    double dx = xi - rx[i];
+    double dy = yi - ry[i];
+    double dz = zi - rz[i];
+    if (dx * dx + dy * dy + dz * dz > threshold_sq) flag[0] = 1;
+    if (!(isfinite(xi) && isfinite(yi) && isfinite(zi))) atomicMin(&flag[1], i);
 }
 
 inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
@@ -764,7 +782,8 @@ hipError_t launch_build(gm_sys *s, bool wrap) {
                                                    s->sc_base.p, s->sc_m.p, s->slot_atom.p);
     k_slot_setup<<<grid_for((size_t)s->max_sc * 64, 256), 256, 0, st>>>(
         (int)(s->max_sc * 64), n, s->slot_atom.p, s->qatom.p, s->atom_par.p, s->excl_bits.p,
-        s->atom_slot.p, s->sq.p, s->sp.p, s->qpos.p, s->qeps.p, s->sxb.p, kernel_sigma_scale(s));
+        s->atom_slot.p, s->sq.p, s->sp.p, s->qpos.p, s->qeps.p, s->sxb.p, kernel_sigma_scale(s), s->x.p, s->y.p,
+        s->z.p);
     k_bbox<<<(unsigned)s->max_sc, 64, 0, st>>>(s->meta.p, s->slot_atom.p, s->sq.p, bp, s->cl_ref.p,
                                                s->cl_half.p, s->sc_ref.p, s->sc_half.p);
     return hipGetLastError();
@@ -803,7 +822,8 @@ hipError_t launch_needs_rebuild(gm_sys *s, const double *x, const double *y, con
     const double half = (s->rskin - s->rc) / 2.0;
     const double thr = half * half;
     if (!flag) flag = s->meta.p + 5;
-    hipError_t e = launch_fill_i32(flag, 1, 0, s->stream);
+    k_rebuild_words<<<1, 1, 0, s->stream>>>(flag, bad_first_word(s));
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (s->n > 0)
         k_needs_rebuild<<<grid_for(s->n, 256), 256, 0, s->stream>>>(

@@ -32,6 +32,8 @@
 using namespace gm;
 using namespace gm::dyn;
 
+constexpr int kCtlWords = 11;  // control words: 4 per solver kind, then 3 rebuild test words
+
 struct gm_dyn {
     int64_t n = 0, nc = 0;
     DevBuf<double> mass, dsq;
@@ -52,10 +54,14 @@ struct gm_dyn {
     int group_cap = 0;      // > 0: phase 0 stops there (the per-iteration kernels continue)
     // Control words: g_ctl[4 kind + 2 buf + {0, 1}] (kind 0 SHAKE, 1 RATTLE;
     // two buffers per kind, alternating, each cleared by the other's phase 1),
-    // g_ctl[8] the run loop's rebuild test; h_ctl is their pinned copy.
+    // g_ctl[8..10] the run loop's rebuild test words (rebuild, first atom with
+    // a non-finite position, first atom with a non-finite force in the last
+    // evaluation); h_ctl is their pinned copy.
     int *h_ctl = nullptr;
     int ctl_buf[2] = {0, 0};   // buffer the next solve of each kind uses
     int ctl_last[2] = {0, 0};  // buffer the last launched solve used
+    bool forces_valid = false; // `force` holds the forces of the positions a run left (gm_md_run resume)
+    gm_md_failure fail{};      // the last gm_md_run's NanInPos / NanInForce report
 };
 
 void gm_dyn_free(gm_dyn *d) {
@@ -319,10 +325,10 @@ gm_status group_finish(gm_sys *s, bool velocity, Vec3 buf0, Vec3C other, int max
     return iterate_from(s, velocity, buf0, other, c[0], max_iter, tol, step, iters);
 }
 
-// Every control word (and g_ctl[8], where the run loop's rebuild test writes) into h_ctl, one copy.
+// Every control word (and g_ctl[8..10], where the run loop's rebuild test writes) into h_ctl, one copy.
 gm_status read_ctl(gm_sys *s) {
     gm_dyn *d = s->dyn;
-    GM_HIP(hipMemcpyAsync(d->h_ctl, d->g_ctl.p, 9 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    GM_HIP(hipMemcpyAsync(d->h_ctl, d->g_ctl.p, kCtlWords * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     GM_HIP(hipStreamSynchronize(s->stream));
     return GM_OK;
 }
@@ -390,6 +396,7 @@ gm_status gm_dynamics_set(gm_sys *s, const double *mass, int64_t n_constr, const
                           const double *dsq) {
     GM_REQUIRE(s && (s->n == 0 || mass) && n_constr >= 0 && (n_constr == 0 || (ci && cj && dsq)),
                "gm_dynamics_set: bad argument");
+    GM_REQUIRE(!s->batch_owned, "gm_dynamics_set: not available on a batch replica");
     const int64_t n = s->n;
     for (int64_t i = 0; i < n; i++)
         GM_REQUIRE(std::isfinite(mass[i]) && mass[i] > 0.0, "gm_dynamics_set: masses must be positive");
@@ -448,9 +455,9 @@ gm_status gm_dynamics_set(gm_sys *s, const double *mass, int64_t n_constr, const
                 (e = up(d->g_cons, L.cons)) != hipSuccess || (e = up(d->g_con_loc, L.con_loc)) != hipSuccess ||
                 (e = up(d->g_inc, L.inc)) != hipSuccess || (e = up(d->g_free, L.free_atoms)) != hipSuccess ||
                 (e = d->g_iters.ensure((size_t)G)) != hipSuccess || (e = d->g_conv.ensure((size_t)G)) != hipSuccess ||
-                (e = d->g_ctl.ensure(9)) != hipSuccess ||
-                (e = hipMemset(d->g_ctl.p, 0, 9 * sizeof(int))) != hipSuccess ||  // every buffer starts clear
-                (e = hipHostMalloc((void **)&d->h_ctl, 9 * sizeof(int), hipHostMallocDefault)) != hipSuccess)
+                (e = d->g_ctl.ensure(kCtlWords)) != hipSuccess ||
+                (e = hipMemset(d->g_ctl.p, 0, kCtlWords * sizeof(int))) != hipSuccess ||  // every buffer starts clear
+                (e = hipHostMalloc((void **)&d->h_ctl, kCtlWords * sizeof(int), hipHostMallocDefault)) != hipSuccess)
                 return fail(e);
             d->n_groups = G;
             d->n_free = (int64_t)L.free_atoms.size();
@@ -505,12 +512,24 @@ gm_status gm_md_run(gm_sys *s, double *x, double *y, double *z, double *vx, doub
                     const gm_md_desc *md, float *h_frames, double *h_epot, double *h_ekin, int64_t *n_frames) {
     GM_REQUIRE(s && s->dyn && md, "gm_md_run: no dynamics attached (gm_dynamics_set) or NULL description");
     GM_REQUIRE(s->n == 0 || (x && y && z && vx && vy && vz), "gm_md_run: NULL array");
-    GM_REQUIRE(md->n_steps >= 0 && md->frame_interval > 0 && md->max_iter > 0 && md->dt != 0.0,
-               "gm_md_run: need n_steps >= 0, frame_interval > 0, max_iter > 0, dt != 0");
+    GM_REQUIRE(md->n_steps >= 0 && md->frame_interval > 0 && md->max_iter > 0 && md->dt != 0.0 &&
+                   md->first_step >= 0 && (md->resume == 0 || md->resume == 1),
+               "gm_md_run: need n_steps >= 0, frame_interval > 0, max_iter > 0, dt != 0, first_step >= 0");
+    GM_REQUIRE(!md->resume || (s->built && s->dyn->forces_valid),
+               "gm_md_run: resume needs a previous run on this system");
     const int64_t n = s->n;
-    double *fx = s->dyn->force.p, *fy = fx + n, *fz = fx + 2 * n;
+    gm_dyn *d = s->dyn;
+    d->fail = gm_md_failure{};
+    d->forces_valid = false;
+    double *fx = d->force.p, *fy = fx + n, *fz = fx + 2 * n;
     const double dt = md->dt;
+    const int64_t s0 = md->first_step;
     int64_t frames = 0;
+    if (n_frames) *n_frames = 0;
+    auto done = [&](gm_status r) {
+        if (n_frames) *n_frames = frames;
+        return r;
+    };
     auto record = [&](double epot) -> gm_status {
         double ke = 0.0;
         gm_status r = kinetic_energy(s, vx, vy, vz, &ke);
@@ -522,27 +541,56 @@ gm_status gm_md_run(gm_sys *s, double *x, double *y, double *z, double *vx, doub
         frames++;
         return GM_OK;
     };
-    // lib.rs:255-276: list, forces, velocity projection, frame 0
-    gm_status rc = gm_build_neighbor_list_device(s, x, y, z);
-    if (rc != GM_OK) return rc;
-    if ((rc = compute_on_device(s, x, y, z, fx, fy, fz)) != GM_OK) return rc;
+    // NumericalError (error.rs) with the reference's message (lib.rs:79-107)
+    auto nan_in_force = [&](int64_t step, int atom) {
+        d->fail = gm_md_failure{2, 0, step, atom};  // a flagged atom's force is NaN in x, y and z
+        return set_error(GM_ERR_NUMERICAL, "NaN in force_X at step " + std::to_string(step) + ", atom " +
+                                               std::to_string(atom));
+    };
+    auto nan_in_pos = [&](int64_t step, int atom) -> gm_status {
+        double p[3];
+        const double *src[3] = {x, y, z};
+        for (int c = 0; c < 3; c++) GM_HIP(hipMemcpy(&p[c], src[c] + atom, sizeof(double), hipMemcpyDeviceToHost));
+        const int comp = !std::isfinite(p[0]) ? 0 : (!std::isfinite(p[1]) ? 1 : 2);
+        d->fail = gm_md_failure{1, comp, step, atom};
+        return set_error(GM_ERR_NUMERICAL, std::string("NaN in position_") + "XYZ"[comp] + " at step " +
+                                               std::to_string(step) + ", atom " + std::to_string(atom));
+    };
+    constexpr int kNone = 0x7fffffff;
     gm_energy_terms e{};
-    if ((rc = collect(s, &e)) != GM_OK) return rc;
-    if ((rc = project(s, true, Vec3{vx, vy, vz}, Vec3C{x, y, z}, md->max_iter, md->tol, 0, nullptr)) != GM_OK)
-        return rc;
-    if ((rc = record(e.total)) != GM_OK) return rc;
-    // lib.rs:283-322: B, A (drift + SHAKE), rebuild if needed, forces, B, RATTLE.
-    // With constraint groups a step has one read-back: a step's RATTLE is
-    // checked together with the next step's SHAKE and rebuild test.  The next
-    // step's kernels up to that point are guarded by RATTLE's "continue"
-    // word, so when RATTLE must continue per iteration none of them has
-    // changed anything: RATTLE continues, and the step runs again.
-    gm_dyn *d = s->dyn;
+    gm_status rc;
+    if (!md->resume) {
+        // lib.rs:261-283: list, forces, NaN check, velocity projection, frame 0
+        if ((rc = gm_build_neighbor_list_device(s, x, y, z)) != GM_OK) return rc;
+        if ((rc = compute_on_device(s, x, y, z, fx, fy, fz)) != GM_OK) return rc;
+        if ((rc = collect(s, &e)) != GM_OK) return rc;
+        if (s->last_bad != kNone) return done(nan_in_force(s0, s->last_bad));
+        if ((rc = project(s, true, Vec3{vx, vy, vz}, Vec3C{x, y, z}, md->max_iter, md->tol, s0, nullptr)) != GM_OK)
+            return done(rc);
+        if ((rc = record(e.total)) != GM_OK) return done(rc);
+    }
+    // lib.rs:292-333: B, A (drift + SHAKE), NaN check of positions, rebuild if
+    // needed, forces, NaN check of forces, B, RATTLE.
+    // With constraint groups a step has one read-back: a step's RATTLE and
+    // force check are read together with the next step's SHAKE, position
+    // check and rebuild test.  The next step's kernels up to that point are
+    // guarded by RATTLE's "continue" word, so when RATTLE must continue per
+    // iteration none of them has changed anything: RATTLE continues, and the
+    // step runs again.  Failures are reported in the reference's order: the
+    // previous step's forces, its RATTLE, this step's SHAKE, its positions.
     hipStream_t st = s->stream;
     double *rx = d->ref.p, *ry = rx + n, *rz = rx + 2 * n;
     const bool deferred = groups_active(d);
     bool v_pending = false;  // the previous step's RATTLE is launched, not yet checked
-    for (int64_t step = 0; step < md->n_steps; step++) {
+    int words[3];            // rebuild test words of the per-iteration path
+    auto rebuild_test = [&]() -> gm_status {
+        GM_HIP(launch_needs_rebuild(s, x, y, z));
+        GM_HIP(hipMemcpyAsync(words, s->meta.p + 5, sizeof(words), hipMemcpyDeviceToHost, st));
+        GM_HIP(hipStreamSynchronize(st));
+        return GM_OK;
+    };
+    for (int64_t k = 0; k < md->n_steps; k++) {
+        const int64_t step = s0 + k;
         int32_t rebuild = 0;
         bool stepped = false;
         if (deferred) {
@@ -551,16 +599,17 @@ gm_status gm_md_run(gm_sys *s, double *x, double *y, double *z, double *vx, doub
             GM_HIP(launch_drift(n, x, y, z, vx, vy, vz, rx, ry, rz, dt, st, guard));
             if ((rc = group_launch(s, false, Vec3{x, y, z}, Vec3C{rx, ry, rz}, md->max_iter, md->tol, guard)) !=
                 GM_OK)
-                return rc;
+                return done(rc);
             GM_HIP(launch_resync(n, x, y, z, rx, ry, rz, vx, vy, vz, dt, st, guard));
             GM_HIP(launch_needs_rebuild(s, x, y, z, d->g_ctl.p + 8));
-            if ((rc = read_ctl(s)) != GM_OK) return rc;
+            if ((rc = read_ctl(s)) != GM_OK) return done(rc);
+            if (v_pending && d->h_ctl[10] != kNone) return done(nan_in_force(step - 1, d->h_ctl[10]));
             const bool replay = v_pending && d->h_ctl[4 + 2 * d->ctl_last[1] + 1];
             // the guarded SHAKE did nothing and cleared nothing: its buffer, still clear, is used again
             if (replay) d->ctl_buf[0] = d->ctl_last[0];
             if (v_pending) {
                 rc = group_finish(s, true, Vec3{vx, vy, vz}, Vec3C{x, y, z}, md->max_iter, md->tol, step - 1, nullptr);
-                if (rc != GM_OK) return rc;
+                if (rc != GM_OK) return done(rc);
                 v_pending = false;
             }
             if (!replay) {
@@ -568,10 +617,13 @@ gm_status gm_md_run(gm_sys *s, double *x, double *y, double *z, double *vx, doub
                 if (d->h_ctl[2 * d->ctl_last[0] + 1]) {
                     rc = group_finish(s, false, Vec3{x, y, z}, Vec3C{rx, ry, rz}, md->max_iter, md->tol, step,
                                       nullptr);
-                    if (rc != GM_OK) return rc;
+                    if (rc != GM_OK) return done(rc);
                     GM_HIP(launch_resync(n, x, y, z, rx, ry, rz, vx, vy, vz, dt, st));
-                    if ((rc = gm_needs_rebuild_device(s, x, y, z, &rebuild)) != GM_OK) return rc;
+                    if ((rc = rebuild_test()) != GM_OK) return done(rc);
+                    if (words[1] != kNone) return done(nan_in_pos(step, words[1]));
+                    rebuild = words[0];
                 } else {
+                    if (d->h_ctl[9] != kNone) return done(nan_in_pos(step, d->h_ctl[9]));
                     rebuild = d->h_ctl[8] ? 1 : 0;
                 }
                 stepped = true;
@@ -580,33 +632,45 @@ gm_status gm_md_run(gm_sys *s, double *x, double *y, double *z, double *vx, doub
         if (!stepped) {
             GM_HIP(launch_half_kick(n, vx, vy, vz, fx, fy, fz, d->mass.p, dt / 2.0, st));
             if ((rc = drift_and_constrain(s, x, y, z, vx, vy, vz, dt, md->max_iter, md->tol, step, nullptr)) != GM_OK)
-                return rc;
-            if ((rc = gm_needs_rebuild_device(s, x, y, z, &rebuild)) != GM_OK) return rc;
+                return done(rc);
+            if ((rc = rebuild_test()) != GM_OK) return done(rc);
+            if (words[1] != kNone) return done(nan_in_pos(step, words[1]));
+            rebuild = words[0];
         }
-        if (rebuild && (rc = gm_build_neighbor_list_device(s, x, y, z)) != GM_OK) return rc;
-        if ((rc = compute_on_device(s, x, y, z, fx, fy, fz)) != GM_OK) return rc;
+        if (rebuild && (rc = gm_build_neighbor_list_device(s, x, y, z)) != GM_OK) return done(rc);
+        if ((rc = compute_on_device(s, x, y, z, fx, fy, fz)) != GM_OK) return done(rc);
+        const bool frame = (step + 1) % md->frame_interval == 0, last = k + 1 == md->n_steps;
+        // forces checked before the velocity projection can fail: at once where
+        // the host waits anyway, otherwise with the next step's read-back
+        if (frame || last || !deferred) {
+            if ((rc = collect(s, &e)) != GM_OK) return done(rc);
+            if (s->last_bad != kNone) return done(nan_in_force(step, s->last_bad));
+        }
         GM_HIP(launch_half_kick(n, vx, vy, vz, fx, fy, fz, d->mass.p, dt / 2.0, st));
         if (deferred) {
             rc = group_launch(s, true, Vec3{vx, vy, vz}, Vec3C{x, y, z}, md->max_iter, md->tol, nullptr);
-            if (rc != GM_OK) return rc;
+            if (rc != GM_OK) return done(rc);
             v_pending = true;
-            if ((step + 1) % md->frame_interval == 0 || step + 1 == md->n_steps) {  // checked before a frame
-                if ((rc = read_ctl(s)) != GM_OK) return rc;
+            if (frame || last) {  // checked before a frame and at the end
+                if ((rc = read_ctl(s)) != GM_OK) return done(rc);
                 rc = group_finish(s, true, Vec3{vx, vy, vz}, Vec3C{x, y, z}, md->max_iter, md->tol, step, nullptr);
-                if (rc != GM_OK) return rc;
+                if (rc != GM_OK) return done(rc);
                 v_pending = false;
             }
         } else if ((rc = project(s, true, Vec3{vx, vy, vz}, Vec3C{x, y, z}, md->max_iter, md->tol, step,
                                  nullptr)) != GM_OK) {
-            return rc;
+            return done(rc);
         }
-        if ((step + 1) % md->frame_interval == 0) {
-            if ((rc = collect(s, &e)) != GM_OK) return rc;
-            if ((rc = record(e.total)) != GM_OK) return rc;
-        }
+        if (frame && (rc = record(e.total)) != GM_OK) return done(rc);
     }
     GM_HIP(hipStreamSynchronize(s->stream));
-    if (n_frames) *n_frames = frames;
+    d->forces_valid = true;
+    return done(GM_OK);
+}
+
+gm_status gm_md_last_failure(gm_sys *s, gm_md_failure *out) {
+    GM_REQUIRE(s && s->dyn && out, "gm_md_last_failure: no dynamics attached or NULL argument");
+    *out = s->dyn->fail;
     return GM_OK;
 }
 

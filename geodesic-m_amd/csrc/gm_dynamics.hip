@@ -59,8 +59,11 @@ __global__ void k_resync(int64_t n, const double *x, const double *y, const doub
 
 // |v| as the bit pattern that orders like the value (a NaN above everything,
 // so it reaches the convergence test as before).
+// |v| as ordered bits, NaN as 0: the reference's `lambda_abs > max_lambda`
+// never takes a NaN (constraint.rs:127-131), so a NaN multiplier does not
+// hold the solve from converging.
 __device__ inline unsigned long long abs_bits(double v) {
-    return (unsigned long long)__double_as_longlong(fabs(v));
+    return __builtin_isnan(v) ? 0ull : (unsigned long long)__double_as_longlong(fabs(v));
 }
 
 // The largest of the wavefront's abs_bits values into *slot: one atomic per

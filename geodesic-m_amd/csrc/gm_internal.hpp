@@ -22,6 +22,17 @@ constexpr int kScSlots = 64;
 constexpr int kColumnCap = 8192;
 // Longest run of list entries one nonbonded wavefront processes.
 constexpr int kMaxRun = 32;
+// Fixed-point block: a supercluster's list entries are cut into blocks of
+// kBlock, counted from its first entry.  A block's i forces are summed in f32
+// and converted to fixed point once, and its LJ energy is one f64 value, so a
+// work item (a run of whole blocks) only decides which wavefront evaluates
+// which blocks: results do not depend on the work shape.
+constexpr int kBlock = 8;
+static_assert(kMaxRun % kBlock == 0, "work items hold whole blocks");
+// A fixed-point contribution of |v| * 2^32 at or above this (or not finite)
+// marks its slot's force non-finite instead of being added.
+constexpr float kFixedLimitF = 4.611686018427388e18f;  // 2^62
+constexpr double kFixedLimit = 4.611686018427388e18;   // 2^62
 // Work-item descriptor word 0: supercluster in the low 26 bits, entry count above.
 constexpr int kWiEntryShift = 26;
 constexpr int kWiScMask = (1 << kWiEntryShift) - 1;
@@ -131,6 +142,8 @@ constexpr int kPruneSteps = 254;  // bytes 0..254 span [r_cutoff, r_skin]
 // block raises word blockIdx % kDispSlots) so that same-address atomics do not
 // serialise; readers take the largest.
 constexpr int kDispSlots = 16, kDispStride = 16;
+// Work counters of the batched tile kernel (one per XCD), this many words apart.
+constexpr int kCtrStride = 32;
 
 }  // namespace gm
 
@@ -201,7 +214,7 @@ struct gm_sys {
     DevBuf<int> col_count, col_start, col_sc, col_fill, bucket, sorted_atoms;
     DevBuf<int> sc_base, sc_m, sc_col;  // per supercluster: first sorted atom, atom count, column
     DevBuf<uint32_t> qatom;  // 3*n fixed-point coordinates, atom order
-    DevBuf<int> meta;        // [0] nsc, [1] overflow, [2] max column, [5] rebuild flag;
+    DevBuf<int> meta;        // [0] nsc, [1] overflow, [2] max column, [5..7] rebuild test words;
                              // ints 8..15 hold u64 counters: list pairs, tiles, tiles with a pair
                              // within r_cutoff, pairs within r_cutoff (f64, build positions)
     int64_t max_sc = 0;
@@ -231,17 +244,20 @@ struct gm_sys {
     DevBuf<uint2> ent_dmin;               // per entry, per tile: closest listed pair byte
     DevBuf<uint8_t> ent_lb;               // per entry, per lane: mask bits over the 8 i clusters
     bool F_dirty = true;                  // accumulator not known to be zero
-    DevBuf<int4> wi;          // work items (supercluster, first entry, end entry, first j-cluster)
-    DevBuf<float> fi_items;               // i forces per work item (192 each, kernel force unit)
-    bool fi_records = false;              // i forces as per-item records (small items) or atomics
-    DevBuf<int> sc_wi_off;                // per supercluster, its first record slot (CSR offsets)
-    int chunk = 0;  // entries per work item (0 = automatic)
+    DevBuf<int4> wi;          // work items (supercluster | entries << 26, first entry, first block,
+                              // j-cluster of the first entry)
+    DevBuf<float> fi_blk;                 // i forces per block (192 each, kernel force unit)
+    bool fi_records = false;              // i forces as per-block records (small items) or atomics
+    DevBuf<int> sc_blk_off;               // per supercluster, its first block (CSR offsets)
+    int chunk = 0;  // entries per work item (0 = automatic; a multiple of kBlock)
+    DevBuf<unsigned> bad;                 // per slot: force not finite this evaluation (cleared by the finish)
 
     // accumulators
     DevBuf<unsigned long long> F;  // 3 * n_pad, int64 fixed point, slot order
     DevBuf<double> e_items;        // per bonded term energies
-    DevBuf<double> e_wi;           // per work item LJ energies
-    DevBuf<double> e_out;          // 5 reduced terms
+    DevBuf<double> e_blk;          // per block LJ energies
+    DevBuf<double> e_out;          // 5 reduced terms; the int at e_out + 6 is the first atom whose
+                                   // force is not finite (INT_MAX if none), reset by the prep kernel
 
     // profiling: three events per timed evaluation (start, nonbonded end, end);
     // one evaluation in every prof_every is timed
@@ -259,7 +275,9 @@ struct gm_sys {
     size_t imask_host_n = 0;
 
     bool built = false;
-    int64_t nsc = 0, ncl = 0, nent = 0, n_wi = 0;
+    int64_t nsc = 0, ncl = 0, nent = 0, n_wi = 0, n_blk = 0;
+    int last_bad = 0x7fffffff;  // first atom with a non-finite force in the last collected evaluation
+    bool batch_owned = false;  // a replica of a gm_batch: its list and work items belong to the batch
     gm_stats stats{};
 };
 
@@ -287,8 +305,12 @@ gm_status build_on_device(gm_sys *s, bool wrap);
 // (nbp_shell), no bonded terms.
 gm_status compute_on_device(gm_sys *s, const double *x, const double *y, const double *z, double *fx,
                             double *fy, double *fz, bool shell = false);
-// Waits for the system's stream and reads the last evaluation's energies.
+// Waits for the system's stream and reads the last evaluation's energies
+// and first non-finite atom (last_bad).
 gm_status collect(gm_sys *s, gm_energy_terms *e);
+// GM_ERR_NUMERICAL naming the first atom with a non-finite force of the last
+// collected evaluation, or GM_OK.
+gm_status nonfinite_status(const gm_sys *s);
 gm_status make_work_items(gm_sys *s);
 // Fixed-point span L (per axis) and xy column counts: every constant that
 // depends on the length of one fixed-point step.
@@ -297,6 +319,8 @@ void set_quant_span(gm_sys *s, const double L[3], int ncx, int ncy);
 void refresh_shell(gm_sys *s);
 // cubic fixed-point span (the fast tile variant works in step units)
 inline bool span_is_cubic(const gm_sys *s) { return s->qbox[0] == s->qbox[1] && s->qbox[1] == s->qbox[2]; }
+// the first atom whose force was not finite in the last evaluation (INT_MAX if none)
+inline int *bad_first_word(gm_sys *s) { return reinterpret_cast<int *>(s->e_out.p + 6); }
 }  // namespace gm
 
 // ---- kernel launchers (defined in the .hip files) --------------------------
@@ -307,7 +331,10 @@ hipError_t launch_build(gm_sys *s, bool wrap);
 hipError_t launch_extent(gm_sys *s, double *out);
 hipError_t launch_pair_count(gm_sys *s);
 hipError_t launch_pair_fill_and_masks(gm_sys *s);
-// flag: where the answer goes (null: the system's own word, meta[5])
+// flag: where the answer goes (null: the system's own words, meta[5..7]):
+// flag[0] rebuild needed, flag[1] the first atom whose position is not finite
+// (INT_MAX if none), flag[2] the first atom whose force was not finite in the
+// last evaluation (INT_MAX if none)
 hipError_t launch_needs_rebuild(gm_sys *s, const double *x, const double *y, const double *z, int *flag = nullptr);
 // gm_nonbonded.hip: tiles + bonded terms in one launch
 // (shell: tiles only, with nbp_shell)
@@ -331,9 +358,10 @@ TermArgs term_args(gm_sys *s, const double *x, const double *y, const double *z)
 TileArgs tile_args(gm_sys *s);
 struct PrepArgs;
 PrepArgs prep_args(gm_sys *s, const double *x, const double *y, const double *z);
+// ctr: kCtrStride * 8 words (the tile kernel's per-XCD counters, cleared by the prep)
 hipError_t launch_batch_prep(const ReplicaDesc *reps, const int *atom_rep, const int64_t *atom_off,
-                             int64_t n_atoms, hipStream_t st);
-hipError_t launch_batch_tiles(const ReplicaDesc *reps, const int2 *items, int n_items, int grid,
+                             int64_t n_atoms, unsigned *ctr, hipStream_t st);
+hipError_t launch_batch_tiles(const ReplicaDesc *reps, const int2 *items, int n_items, int grid, unsigned *ctr,
                               hipStream_t st);
 hipError_t launch_batch_bonded(const ReplicaDesc *reps, const int2 *items, int n_items, hipStream_t st);
 hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int2 *scl, int64_t n_rec, int64_t n_atm,

@@ -36,6 +36,8 @@
 // gm_terms.hpp), running beside the tiles instead of in launches of its own.
 #include <hip/hip_ext.h>
 
+#include <algorithm>
+
 #include "gm_internal.hpp"
 #include "gm_terms.hpp"
 
@@ -53,9 +55,20 @@ __device__ inline float dpp_sum8(float v) {
     return v;
 }
 
-__device__ inline void fixed_add(unsigned long long *p, float v, float scale) {
-    const long long q = (long long)__builtin_rintf(v * scale);
-    __hip_atomic_fetch_add(p, (unsigned long long)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// v * scale rounded to an integer: the fixed-point image of one partial sum.
+// The finish kernel rounds i-force records with the same expression, so the
+// integers do not depend on where the rounding happens.
+__device__ inline long long to_fixed_f(float v, float scale) { return (long long)__builtin_rintf(v * scale); }
+
+// The fixed-point image of v into the accumulator word p; a value that is not
+// finite or reaches 2^62 in fixed point marks the slot instead (*bad = 1).
+__device__ inline void fixed_add(unsigned long long *p, float v, float scale, unsigned *bad) {
+    if (!(fabsf(v * scale) < kFixedLimitF)) {
+        *bad = 1u;
+        return;
+    }
+    __hip_atomic_fetch_add(p, (unsigned long long)to_fixed_f(v, scale), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Exchange-and-add across lane bit 5: with a, b one value each per lane,
@@ -112,15 +125,25 @@ __device__ inline float wave_max(float v) {
     return fmaxf(__uint_as_float(q[0]), __uint_as_float(q[1]));
 }
 
-// One atom of the prep pass: quantise, store x, y, z to the slot (sigma/2,
-// the slot's fourth word, is written at the list build), return the
-// displacement from the build position (A).  Every load is by atom index.
+// One atom of the prep pass: quantise, store x, y, z and sigma/2 to the slot,
+// return the displacement from the build position (A).  Every load is by
+// atom index.  A position that is not finite marks the slot (its force is
+// reported as NaN), stores sigma/2 as NaN (the force kernel then marks every
+// listed partner of the atom the same way) and returns an infinite
+// displacement, which turns tile pruning off for the evaluation.
 __device__ inline float prep_atom(const PrepArgs &P, int a) {
     const int s = P.atom_slot[a];
-    const int qx = quantize_fixed(P.x[a], P.iq[0]);
-    const int qy = quantize_fixed(P.y[a], P.iq[1]);
-    const int qz = quantize_fixed(P.z[a], P.iq[2]);
-    *reinterpret_cast<int3 *>(&P.qpos[s]) = make_int3(qx, qy, qz);
+    const double x = P.x[a], y = P.y[a], z = P.z[a];
+    const float sh = P.atom_par[a].x * P.sig_scale;
+    const int qx = quantize_fixed(x, P.iq[0]);
+    const int qy = quantize_fixed(y, P.iq[1]);
+    const int qz = quantize_fixed(z, P.iq[2]);
+    const bool fin = isfinite(x) && isfinite(y) && isfinite(z);
+    P.qpos[s] = make_int4(qx, qy, qz, fin ? __float_as_int(sh) : 0x7fc00000);
+    if (!fin) {
+        P.bad[s] = 1u;
+        return INFINITY;
+    }
     const float dx = fixdiff(qx, (int)P.qref[a]) * P.sq_len[0];
     const float dy = fixdiff(qy, (int)P.qref[P.n + a]) * P.sq_len[1];
     const float dz = fixdiff(qz, (int)P.qref[2 * (size_t)P.n + a]) * P.sq_len[2];
@@ -137,6 +160,7 @@ __device__ inline void atomic_max_nonneg(unsigned *slot, float v) {
 // kDispSlots displacement words with its largest value.
 __global__ __launch_bounds__(256) void k_prep(PrepArgs P) {
     const int a = blockIdx.x * 256 + threadIdx.x;
+    if (a == 0) *P.bad_first = 0x7fffffff;  // read by the finish kernel of this evaluation
     float d = 0.f;
     if (a < P.n) d = prep_atom(P, a);
     d = wave_max(d);
@@ -150,9 +174,12 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs P) {
 // straddles a boundary reads the descriptor and raises the bound per lane.
 __global__ __launch_bounds__(256) void k_batch_prep(const ReplicaDesc *__restrict__ reps,
                                                     const int *__restrict__ atom_rep,
-                                                    const int64_t *__restrict__ atom_off, int64_t n_atoms) {
+                                                    const int64_t *__restrict__ atom_off, int64_t n_atoms,
+                                                    unsigned *ctr) {
     const int64_t a = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int r = a < n_atoms ? atom_rep[a] : -1;
+    if (r >= 0 && a == atom_off[r]) *reps[r].prep.bad_first = 0x7fffffff;  // the replica's first atom
+    if (a < 8) ctr[a * kCtrStride] = 0u;  // the tile kernel's work counters, one per XCD
     const int r0 = __builtin_amdgcn_readfirstlane(r);
     const size_t slot = (size_t)((blockIdx.x * 4 + (threadIdx.x >> 6)) % kDispSlots) * kDispStride;
     if (__all(r == r0)) {
@@ -197,11 +224,11 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     __shared__ int2 s_meta[kMaxRun];
     __shared__ float jf[3][kMaxRun * 8];
     const int il = lane & 7, jl = lane >> 3;
-    // (supercluster | entry count << 26, first entry, record slot, j-cluster
+    // (supercluster | entry count << 26, first entry, first block, j-cluster
     // of the first entry): the first j positions load with the run's list data
     const int4 wd = A.wi[w];
     const int sc = (int)((unsigned)wd.x & (unsigned)kWiScMask);
-    const int e0 = wd.y, ne = (int)((unsigned)wd.x >> kWiEntryShift);
+    const int e0 = wd.y, ne = (int)((unsigned)wd.x >> kWiEntryShift), blk0 = wd.z;
     int4 cq = qpos[(size_t)wd.w * 8 + jl];
     float ce = qeps[(size_t)wd.w * 8 + jl];
     unsigned long long live = 0;
@@ -249,8 +276,11 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     // broadcast to 8 lanes each, which keeps 40 registers free for occupancy.
     __shared__ int4 s_iq[64];
     __shared__ float s_ie[64];
-    s_iq[lane] = qpos[(size_t)sc * 64 + lane];
+    const int4 qown = qpos[(size_t)sc * 64 + lane];
+    s_iq[lane] = qown;
     s_ie[lane] = qeps[(size_t)sc * 64 + lane];
+    // an i atom whose position is not finite (sigma/2 stored as NaN)?
+    bool anybad = __ballot(__builtin_isnan(__int_as_float(qown.w))) != 0ull;
     float fix[8], fiy[8], fiz[8];
 #pragma unroll
     for (int ci = 0; ci < 8; ci++) { fix[ci] = 0.f; fiy[ci] = 0.f; fiz[ci] = 0.f; }
@@ -263,6 +293,7 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     const float ds_coef = CUBIC ? p.ds_coef_u : p.ds_coef;
     const float inv_w_in = CUBIC ? p.inv_w_in_u : p.inv_w_in, nrw_in = in_vgpr(p.nrw_in);
     const float ds_coef_in = CUBIC ? p.ds_coef_in_u : p.ds_coef_in;
+    const float fscale = CUBIC ? p.fscale_u : 4294967296.0f;
     // Tile ci's list bit is bit ci of the lane's mask byte, tested by one
     // class compare: a class test's mask bit c selects IEEE class c, and
     // kcls[ci] is a value of class ci (signalling NaN, quiet NaN, -inf,
@@ -270,139 +301,188 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     const float kcls[8] = {sgpr_bits(0x7f800001u), sgpr_bits(0x7fc00000u), sgpr_bits(0xff800000u),
                            sgpr_bits(0xbf800000u), sgpr_bits(0x80000001u), sgpr_bits(0x80000000u),
                            sgpr_bits(0x00000000u), sgpr_bits(0x00000001u)};
-    double e_acc = 0.0;
     if (nl > 0 && !(live & 1ull)) {  // the first entry was pruned: start from the first live one
         cq = qpos[(size_t)s_meta[0].x * 8 + jl];
         ce = qeps[(size_t)s_meta[0].x * 8 + jl];
     }
-    for (int k = 0; k < nl; k++) {
-        const int cjn = __builtin_amdgcn_readfirstlane(s_meta[min(k + 1, nl - 1)].x);
-        const int4 nq = qpos[(size_t)cjn * 8 + jl];
-        const float nce = qeps[(size_t)cjn * 8 + jl];
-        const uint32_t mk = (uint32_t)__builtin_amdgcn_readfirstlane(s_meta[k].y);
-        const uint32_t im = mk & 255u;
-        float fjx = 0.f, fjy = 0.f, fjz = 0.f;
-        if (im) {
-            const uint32_t bits = s_lb8[(mk >> 8) * 64 + lane];
-            const float shj = __int_as_float(cq.w), ej = ce;
-            float ee = 0.f;
+    // Blocks: entries [b kBlock, (b + 1) kBlock) of the run are block blk0 + b.
+    // A block's i forces and energy leave once, after its last entry: records
+    // (plain stores) or, without records, fixed point summed over the item's
+    // blocks in registers and added with atomics at the end.
+    const int nblk = (ne + kBlock - 1) / kBlock;
+    long long ai[3] = {0, 0, 0};
+    bool ai_ok = true;
+    int k = 0;
+    for (int b = 0; b < nblk; b++) {
+        const int kend = __popcll(live & ((1ull << ((b + 1) * kBlock)) - 1ull));  // live entries up to block b
+        const bool any = k < kend;
+        double e_acc = 0.0;
+        for (; k < kend; k++) {
+            const int cjn = __builtin_amdgcn_readfirstlane(s_meta[min(k + 1, nl - 1)].x);
+            const int4 nq = qpos[(size_t)cjn * 8 + jl];
+            const float nce = qeps[(size_t)cjn * 8 + jl];
+            const uint32_t mk = (uint32_t)__builtin_amdgcn_readfirstlane(s_meta[k].y);
+            const uint32_t im = mk & 255u;
+            float fjx = 0.f, fjy = 0.f, fjz = 0.f;
+            if (im) {
+                const uint32_t bits = s_lb8[(mk >> 8) * 64 + lane];
+                const float shj = __int_as_float(cq.w), ej = ce;
+                float ee = 0.f;
 #pragma unroll
-            for (int ci = 0; ci < 8; ci++) {
-                if (im & (1u << ci)) {
-                    // Branch-free tile: every lane evaluates its pair; pairs not
-                    // in the list, beyond r_cutoff or coincident get zero force
-                    // and energy through one select each (which also discards
-                    // the inf/NaN an r2 of 0 produces).  Below r_switch u
-                    // clamps to 0, where S = 1 and dS/dr = 0 exactly.
-                    const int4 qi = s_iq[ci * 8 + il];
-                    float dx = fixdiff(cq.x, qi.x);
-                    float dy = fixdiff(cq.y, qi.y);
-                    float dz = fixdiff(cq.z, qi.z);
-                    if (!CUBIC) { dx *= p.sx; dy *= p.sy; dz *= p.sz; }
-                    const float r2 = dx * dx + dy * dy + dz * dz;
-                    // 0 < r2 <= rc2 as one unsigned compare of the bit patterns
-                    const bool ok = __builtin_amdgcn_classf(kcls[ci], (int)bits) &&
-                                    ((unsigned)__float_as_int(r2) - 1u) < rc2_bits;
-                    const float inv_r = __builtin_amdgcn_rsqf(r2);
-                    // one Newton step brings rsq^2 (~2.5 ulp) to ~1 ulp, the
-                    // accuracy (sigma/r)^12 needs, for two FMAs instead of a rcp
-                    float inv_r2 = inv_r * inv_r;
-                    inv_r2 = fmaf(inv_r2, fmaf(-r2, inv_r2, 1.0f), inv_r2);
-                    const float sig = __int_as_float(qi.w) + shj;
-                    const float s2 = sig * sig * inv_r2;
-                    const float s6 = s2 * s2 * s2;
-                    const float lj_a = fmaf(s6, s6, -s6);                 // s12 - s6
-                    const float lj_b = s6 * fmaf(s6, 12.0f, -6.0f);       // 6 (2 s12 - s6)
-                    const float e4 = s_ie[ci * 8 + il] * ej;
-                    const float r = r2 * inv_r;
-                    const float u = __builtin_amdgcn_fmed3f(fmaf(r, inv_w, nrw), 0.0f, 1.0f);
-                    const float u2 = u * u;
-                    const float pu = fmaf(u, fmaf(u, -6.0f, 15.0f), -10.0f);
-                    const float qu = u - u2;
-                    float sw, dsdr;
-                    if (SHELL) {
-                        const float ui = __builtin_amdgcn_fmed3f(fmaf(r, inv_w_in, nrw_in), 0.0f, 1.0f);
-                        const float ui2 = ui * ui;
-                        const float pi = fmaf(ui, fmaf(ui, -6.0f, 15.0f), -10.0f);
-                        const float qi2 = ui - ui2;
-                        sw = fmaf(u2 * u, pu, -(ui2 * ui * pi));           // S - S_in
-                        dsdr = fmaf(ds_coef * qu, qu, -(ds_coef_in * qi2 * qi2));
-                    } else {
-                        sw = fmaf(u2 * u, pu, 1.0f);
-                        dsdr = ds_coef * qu * qu;
+                for (int ci = 0; ci < 8; ci++) {
+                    if (im & (1u << ci)) {
+                        // Branch-free tile: every lane evaluates its pair; pairs not
+                        // in the list, beyond r_cutoff or coincident get zero force
+                        // and energy through one select each (which also discards
+                        // the inf/NaN an r2 of 0 produces).  Below r_switch u
+                        // clamps to 0, where S = 1 and dS/dr = 0 exactly.
+                        const int4 qi = s_iq[ci * 8 + il];
+                        float dx = fixdiff(cq.x, qi.x);
+                        float dy = fixdiff(cq.y, qi.y);
+                        float dz = fixdiff(cq.z, qi.z);
+                        if (!CUBIC) { dx *= p.sx; dy *= p.sy; dz *= p.sz; }
+                        const float r2 = dx * dx + dy * dy + dz * dz;
+                        // 0 < r2 <= rc2 as one unsigned compare of the bit patterns
+                        const bool ok = __builtin_amdgcn_classf(kcls[ci], (int)bits) &&
+                                        ((unsigned)__float_as_int(r2) - 1u) < rc2_bits;
+                        const float inv_r = __builtin_amdgcn_rsqf(r2);
+                        // one Newton step brings rsq^2 (~2.5 ulp) to ~1 ulp, the
+                        // accuracy (sigma/r)^12 needs, for two FMAs instead of a rcp
+                        float inv_r2 = inv_r * inv_r;
+                        inv_r2 = fmaf(inv_r2, fmaf(-r2, inv_r2, 1.0f), inv_r2);
+                        const float sig = __int_as_float(qi.w) + shj;
+                        const float s2 = sig * sig * inv_r2;
+                        const float s6 = s2 * s2 * s2;
+                        const float lj_a = fmaf(s6, s6, -s6);                 // s12 - s6
+                        const float lj_b = s6 * fmaf(s6, 12.0f, -6.0f);       // 6 (2 s12 - s6)
+                        const float e4 = s_ie[ci * 8 + il] * ej;
+                        const float r = r2 * inv_r;
+                        const float u = __builtin_amdgcn_fmed3f(fmaf(r, inv_w, nrw), 0.0f, 1.0f);
+                        const float u2 = u * u;
+                        const float pu = fmaf(u, fmaf(u, -6.0f, 15.0f), -10.0f);
+                        const float qu = u - u2;
+                        float sw, dsdr;
+                        if (SHELL) {
+                            const float ui = __builtin_amdgcn_fmed3f(fmaf(r, inv_w_in, nrw_in), 0.0f, 1.0f);
+                            const float ui2 = ui * ui;
+                            const float pi = fmaf(ui, fmaf(ui, -6.0f, 15.0f), -10.0f);
+                            const float qi2 = ui - ui2;
+                            sw = fmaf(u2 * u, pu, -(ui2 * ui * pi));           // S - S_in
+                            dsdr = fmaf(ds_coef * qu, qu, -(ds_coef_in * qi2 * qi2));
+                        } else {
+                            sw = fmaf(u2 * u, pu, 1.0f);
+                            dsdr = ds_coef * qu * qu;
+                        }
+                        const float inner = fmaf(-(lj_a * dsdr), r, lj_b * sw);
+                        const float fr = ok ? e4 * inv_r2 * inner : 0.0f;
+                        const float v = ok ? e4 * lj_a * sw : 0.0f;
+                        fix[ci] = fmaf(-fr, dx, fix[ci]);
+                        fiy[ci] = fmaf(-fr, dy, fiy[ci]);
+                        fiz[ci] = fmaf(-fr, dz, fiz[ci]);
+                        fjx = fmaf(fr, dx, fjx);
+                        fjy = fmaf(fr, dy, fjy);
+                        fjz = fmaf(fr, dz, fjz);
+                        ee += v;
                     }
-                    const float inner = fmaf(-(lj_a * dsdr), r, lj_b * sw);
-                    const float fr = ok ? e4 * inv_r2 * inner : 0.0f;
-                    const float v = ok ? e4 * lj_a * sw : 0.0f;
-                    fix[ci] = fmaf(-fr, dx, fix[ci]);
-                    fiy[ci] = fmaf(-fr, dy, fiy[ci]);
-                    fiz[ci] = fmaf(-fr, dz, fiz[ci]);
-                    fjx = fmaf(fr, dx, fjx);
-                    fjy = fmaf(fr, dy, fjy);
-                    fjz = fmaf(fr, dz, fjz);
-                    ee += v;
                 }
+                fjx = dpp_sum8(fjx);
+                fjy = dpp_sum8(fjy);
+                fjz = dpp_sum8(fjz);
+                e_acc += (double)ee;
+                anybad = anybad || __ballot(__builtin_isnan(shj)) != 0ull;  // a j atom, likewise
             }
-            fjx = dpp_sum8(fjx);
-            fjy = dpp_sum8(fjy);
-            fjz = dpp_sum8(fjz);
-            e_acc += (double)ee;
+            if (il == 0) {
+                jf[0][k * 8 + jl] = fjx;
+                jf[1][k * 8 + jl] = fjy;
+                jf[2][k * 8 + jl] = fjz;
+            }
+            cq = nq;
+            ce = nce;
         }
-        if (il == 0) {
-            jf[0][k * 8 + jl] = fjx;
-            jf[1][k * 8 + jl] = fjy;
-            jf[2][k * 8 + jl] = fjz;
+        // i forces: lane (jl, il) holds partial sums for atom il of all 8 clusters;
+        // the 8 jl lanes of each atom are reduced by a transpose: exchange halves
+        // across lane bit 5, then bit 4 (gfx950 permlane swaps), then bit 3 (DPP),
+        // keeping the half that belongs to the lane's own bit each time.  Lane
+        // (jl, il) ends with the total of cluster jl, atom il: slot sc * 64 + lane.
+        float fo[3] = {0.f, 0.f, 0.f};
+        if (any) {
+            const bool odd = (lane & 8) != 0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {  // one component at a time: few temporaries
+                float *f = c == 0 ? fix : (c == 1 ? fiy : fiz);
+                float h4[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) h4[q] = swap_add32(f[q], f[q + 4]);
+                const float h20 = swap_add16(h4[0], h4[2]);
+                const float h21 = swap_add16(h4[1], h4[3]);
+                const float keep = odd ? h21 : h20;
+                const float send = odd ? h20 : h21;
+                fo[c] = keep + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(send), 0x128, 0xF, 0xF, false));
+#pragma unroll
+                for (int ci = 0; ci < 8; ci++) f[ci] = 0.f;
+            }
         }
-        cq = nq;
-        ce = nce;
+        if (A.fi_blk) {
+            // the block's own record (plain stores; the finish kernel adds the
+            // records of a supercluster's blocks): no atomics on the i side
+            float *fw = A.fi_blk + (size_t)(blk0 + b) * 192 + lane;
+            fw[0] = fo[0];
+            fw[64] = fo[1];
+            fw[128] = fo[2];
+        } else if (any) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                ai_ok = ai_ok && fabsf(fo[c] * fscale) < kFixedLimitF;
+                ai[c] += to_fixed_f(fo[c], fscale);
+            }
+        }
+        const double eb = any ? wave_sum_f64(e_acc) : 0.0;
+        if (lane == 0) A.e_blk[blk0 + b] = eb;
     }
     __syncthreads();
-    const float fscale = CUBIC ? p.fscale_u : 4294967296.0f;
     for (int idx = lane; idx < nl * 8; idx += 64) {
         const int2 m = s_meta[idx >> 3];
         const size_t js = (size_t)m.x * 8 + (idx & 7);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const float v = jf[c][idx];
-            if (v != 0.0f) fixed_add(&F[(size_t)c * n_pad + js], v, fscale);
+            if (v != 0.0f) fixed_add(&F[(size_t)c * n_pad + js], v, fscale, &A.bad[js]);
         }
     }
-    // i forces: lane (jl, il) holds partial sums for atom il of all 8 clusters;
-    // the 8 jl lanes of each atom are reduced by a transpose: exchange halves
-    // across lane bit 5, then bit 4 (gfx950 permlane swaps), then bit 3 (DPP),
-    // keeping the half that belongs to the lane's own bit each time.  Lane
-    // (jl, il) ends with the total of cluster jl, atom il: slot sc * 64 + lane.
-    float h4[3][4], h2[3][2], fo[3];
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        h4[0][c] = swap_add32(fix[c], fix[c + 4]);
-        h4[1][c] = swap_add32(fiy[c], fiy[c + 4]);
-        h4[2][c] = swap_add32(fiz[c], fiz[c + 4]);
-    }
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        h2[k][0] = swap_add16(h4[k][0], h4[k][2]);
-        h2[k][1] = swap_add16(h4[k][1], h4[k][3]);
-        const bool odd = (lane & 8) != 0;
-        const float keep = odd ? h2[k][1] : h2[k][0];
-        const float send = odd ? h2[k][0] : h2[k][1];
-        fo[k] = keep + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(send), 0x128, 0xF, 0xF, false));
-    }
-    if (A.fi_items) {
-        // the item's own record (plain stores; the finish kernel adds the
-        // records of a supercluster's items): no atomics on the i side
-        float *fw = A.fi_items + (size_t)wd.z * 192 + lane;
-        fw[0] = fo[0];
-        fw[64] = fo[1];
-        fw[128] = fo[2];
-    } else if (nl > 0) {  // large work items: into the accumulator (nothing to add for an empty run)
+    if (!A.fi_blk && nl > 0) {  // large work items: into the accumulator (nothing to add for an empty run)
         const size_t s = (size_t)sc * 64 + lane;
-        fixed_add(&F[s], fo[0], fscale);
-        fixed_add(&F[(size_t)n_pad + s], fo[1], fscale);
-        fixed_add(&F[2 * (size_t)n_pad + s], fo[2], fscale);
+        if (!ai_ok) {
+            A.bad[s] = 1u;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                __hip_atomic_fetch_add(&F[(size_t)c * n_pad + s], (unsigned long long)ai[c], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
-    e_acc = wave_sum_f64(e_acc);
-    if (lane == 0) A.e_wi[w] = e_acc;
+    // A listed pair with an atom whose position is not finite: the other
+    // atom's force is reported as NaN too, as every listed pair of such an
+    // atom gives NaN in f64 (rare; a pass of its own after everything else).
+    // The energy is NaN then as well: the item's first block energy is
+    // overwritten (the same wavefront stored it above).
+    if (__builtin_expect(anybad, 0)) {
+        const unsigned long long ib = __ballot(__builtin_isnan(__int_as_float(s_iq[lane].w)));
+        bool hit = false;
+        for (int kk = 0; kk < nl; kk++) {
+            const int2 m = s_meta[kk];
+            const size_t js = (size_t)m.x * 8 + jl;
+            const bool jb = __builtin_isnan(__int_as_float(qpos[js].w));
+            const uint32_t bits = s_lb8[((uint32_t)m.y >> 8) * 64 + lane] & (uint32_t)m.y;
+            for (int ci = 0; ci < 8; ci++) {
+                if (!((bits >> ci) & 1u)) continue;
+                const bool ibb = (ib >> (ci * 8 + il)) & 1ull;
+                if (ibb) A.bad[js] = 1u;
+                if (jb) A.bad[(size_t)sc * 64 + ci * 8 + il] = 1u;
+                hit = hit || ibb || jb;
+            }
+        }
+        if (__ballot(hit) != 0ull && lane == 0) A.e_blk[blk0] = __builtin_nan("");
+    }
     __syncthreads();  // LDS is reused by the next work item of a persistent wave
 }
 
@@ -428,23 +508,36 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
     tile_work<CUBIC, SHELL>((int)blockIdx.x - n_bonded_groups, lane, A);
 }
 
-// Persistent batched tile kernel: a fixed grid of wavefronts walks the tile
-// work items (replica, work item) in a static order, with no shared counter.
-// Workgroups are dealt to the 8 XCDs in turn, so wavefront w runs on XCD
-// w % 8; XCD x owns the contiguous item range [x n / 8, (x + 1) n / 8), which
-// keeps neighbouring items (same replica, nearby superclusters) in one L2.
-// Each item's arithmetic is the single-system kernel's, so results do not
-// depend on which wavefront takes which item.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_batch_tiles(
-    const ReplicaDesc *__restrict__ reps, const int2 *__restrict__ items, int n_items) {
+// Persistent batched tile kernel: a fixed grid of wavefronts takes tile work
+// items (replica, work item) from eight counters, one per XCD.  Workgroups
+// are dealt to the 8 XCDs in turn, so wavefront w runs on XCD w % 8 and takes
+// items of its XCD's contiguous range [x n / 8, (x + 1) n / 8) first (it
+// keeps neighbouring items, same replica and nearby superclusters, in one
+// L2), then helps the other XCDs' ranges in turn.  The counters (kCtrStride
+// words apart) are cleared by the batched prep kernel.  The next ticket is
+// fetched before the current item is evaluated, so its latency overlaps the
+// item.  Each item's arithmetic is the single-system kernel's, so results do
+// not depend on which wavefront takes which item.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_batch_tiles(
+    const ReplicaDesc *__restrict__ reps, const int2 *__restrict__ items, int n_items, unsigned *ctr) {
     const int lane = threadIdx.x;
-    const int w = blockIdx.x, nw = gridDim.x;
-    const int x = w & 7, l = w >> 3;
-    const int nwx = (nw - x + 7) >> 3;  // wavefronts on XCD x
-    const int lo = (int)(((int64_t)n_items * x) >> 3), hi = (int)(((int64_t)n_items * (x + 1)) >> 3);
-    for (int it = lo + l; it < hi; it += nwx) {
-        const int2 rw = items[it];
-        tile_work<true, false>(rw.y, lane, reps[rw.x].tile);  // replicas are cubic
+    const int x = blockIdx.x & 7;
+    for (int d = 0; d < 8; d++) {
+        const int xx = (x + d) & 7;
+        const int lo = (int)(((int64_t)n_items * xx) >> 3), hi = (int)(((int64_t)n_items * (xx + 1)) >> 3);
+        unsigned *c = ctr + xx * kCtrStride;
+        unsigned t = 0;
+        if (lane == 0) t = __hip_atomic_fetch_add(c, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int it = lo + (int)__builtin_amdgcn_readfirstlane(t);
+        while (it < hi) {
+            unsigned tn = 0;
+            if (lane == 0) tn = __hip_atomic_fetch_add(c, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // (replica, item) is the same for every lane: scalar registers
+            const int rep = __builtin_amdgcn_readfirstlane(items[it].x);
+            const int wi = __builtin_amdgcn_readfirstlane(items[it].y);
+            tile_work<true, false>(wi, lane, reps[rep].tile);  // replicas are cubic
+            it = lo + (int)__builtin_amdgcn_readfirstlane(tn);
+        }
     }
 }
 
@@ -467,6 +560,7 @@ TermArgs term_args(gm_sys *s, const double *x, const double *y, const double *z)
     for (int a = 0; a < 3; a++) t.box[a] = s->box[a];
     t.mi = s->bonded_mi;
     t.F = s->F.p;
+    t.bad = s->bad.p;
     t.atom_slot = s->atom_slot.p;
     t.n_pad = (size_t)s->nsc * 64;
     t.bi = s->bi.p; t.bj = s->bj.p; t.bk = s->bk.p; t.br0 = s->br0.p;
@@ -493,10 +587,12 @@ TileArgs tile_args(gm_sys *s) {
     A.prune_scale = 2.0f * s->bp.prune_inv_step;
     A.prune_bias = (float)(1e-4 * s->rc) * s->bp.prune_inv_step + 1.0f;
     A.F = s->F.p;
+    A.bad = s->bad.p;
+    A.bad_first = bad_first_word(s);
     A.n_pad = (int)(s->nsc * 64);
-    A.e_wi = s->e_wi.p;
-    A.fi_items = s->fi_records ? s->fi_items.p : nullptr;
-    A.sc_wi_off = s->sc_wi_off.p;
+    A.e_blk = s->e_blk.p;
+    A.fi_blk = s->fi_records ? s->fi_blk.p : nullptr;
+    A.sc_blk_off = s->sc_blk_off.p;
     A.fi_scale = span_is_cubic(s) ? s->nbp.fscale_u : 4294967296.0f;  // the tile kernel's fscale
     A.slot_atom = s->slot_atom.p;
     A.p = s->nbp;
@@ -510,7 +606,11 @@ PrepArgs prep_args(gm_sys *s, const double *x, const double *y, const double *z)
     P.atom_slot = s->atom_slot.p;
     P.qref = s->qatom.p;
     P.qpos = s->qpos.p;
+    P.atom_par = s->atom_par.p;
+    P.sig_scale = span_is_cubic(s) ? s->nbp.inv_s : 1.0f;  // as the list build's sigma/2
     P.disp = s->disp.p;
+    P.bad = s->bad.p;
+    P.bad_first = bad_first_word(s);
     for (int a = 0; a < 3; a++) {
         P.iq[a] = s->bp.inv_q[a];
         P.sq_len[a] = s->bp.sq[a];
@@ -566,15 +666,16 @@ hipError_t launch_work_item_cj0(gm_sys *s) {
 }
 
 hipError_t launch_batch_prep(const ReplicaDesc *reps, const int *atom_rep, const int64_t *atom_off,
-                             int64_t n_atoms, hipStream_t st) {
-    if (n_atoms > 0)
-        k_batch_prep<<<(unsigned)((n_atoms + 255) / 256), 256, 0, st>>>(reps, atom_rep, atom_off, n_atoms);
+                             int64_t n_atoms, unsigned *ctr, hipStream_t st) {
+    // at least one block, which clears the tile kernel's counters
+    k_batch_prep<<<(unsigned)std::max<int64_t>((n_atoms + 255) / 256, 1), 256, 0, st>>>(reps, atom_rep, atom_off,
+                                                                                         n_atoms, ctr);
     return hipGetLastError();
 }
 
-hipError_t launch_batch_tiles(const ReplicaDesc *reps, const int2 *items, int n_items, int grid,
+hipError_t launch_batch_tiles(const ReplicaDesc *reps, const int2 *items, int n_items, int grid, unsigned *ctr,
                               hipStream_t st) {
-    if (n_items > 0 && grid > 0) k_batch_tiles<<<(unsigned)grid, 64, 0, st>>>(reps, items, n_items);
+    if (n_items > 0 && grid > 0) k_batch_tiles<<<(unsigned)grid, 64, 0, st>>>(reps, items, n_items, ctr);
     return hipGetLastError();
 }
 

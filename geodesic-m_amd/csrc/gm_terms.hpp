@@ -21,8 +21,17 @@ __device__ inline double term_diff(double a, double b, int mi, double L) {
     return mi ? term_min_image(d, L) : d;
 }
 
-__device__ inline void fixed_add_d(unsigned long long *p, double v) {
-    const long long q = __double2ll_rn(v * 4294967296.0);
+// v (kcal/mol/A) into the accumulator word p, in units of 2^-32.  A value
+// that is not finite or whose fixed-point image reaches 2^62 is not added: it
+// marks the slot (*bad = 1), and the finish kernel reports that atom's force
+// as NaN.
+__device__ inline void fixed_add_d(unsigned long long *p, double v, unsigned *bad) {
+    const double t = v * 4294967296.0;
+    if (!(fabs(t) < kFixedLimit)) {
+        *bad = 1u;
+        return;
+    }
+    const long long q = __double2ll_rn(t);
     __hip_atomic_fetch_add(p, (unsigned long long)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -33,6 +42,7 @@ struct TermArgs {
     double box[3];
     int mi;  // min-image bonded displacements
     unsigned long long *F;
+    unsigned *bad;  // per slot: a contribution was not finite or out of the fixed-point range
     const int *atom_slot;
     size_t n_pad;
     const uint32_t *bi, *bj;
@@ -50,9 +60,9 @@ struct TermArgs {
 
 __device__ inline void term_add(const TermArgs &t, uint32_t atom, double fx, double fy, double fz) {
     const size_t s = (size_t)t.atom_slot[atom];
-    fixed_add_d(&t.F[s], fx);
-    fixed_add_d(&t.F[t.n_pad + s], fy);
-    fixed_add_d(&t.F[2 * t.n_pad + s], fz);
+    fixed_add_d(&t.F[s], fx, &t.bad[s]);
+    fixed_add_d(&t.F[t.n_pad + s], fy, &t.bad[s]);
+    fixed_add_d(&t.F[2 * t.n_pad + s], fz, &t.bad[s]);
 }
 
 // force/bonded.rs:5-40
@@ -198,16 +208,18 @@ struct TileArgs {
                                        // kDispSlots words; null: no tile pruning
     float prune_scale, prune_bias;     // keep a tile iff byte <= disp * scale + bias
     unsigned long long *F;             // int64 fixed-point accumulator, slot order (j forces, bonded)
+    unsigned *bad;                     // per slot: force not finite this evaluation
+    int *bad_first;                    // first atom whose force is not finite (atomic minimum)
     int n_pad;                         // slots
-    double *e_wi;                      // per work item LJ energy
-    // i forces per work item, [record slot][x, y, z][64 slots of its
-    // supercluster], f32 in the kernel's force unit: plain stores.  A
-    // supercluster's items, ascending, own the record slots
-    // [sc_wi_off[sc], sc_wi_off[sc + 1]); the finish kernel rounds each record
-    // to fixed point (scaled by fi_scale) and adds them per slot in that order.
-    // Null: the i forces go to the accumulator F with atomics instead.
-    float *fi_items;
-    const int *sc_wi_off;
+    double *e_blk;                     // per block LJ energy (blocks of kBlock entries)
+    // i forces per block, [block][x, y, z][64 slots of its supercluster], f32
+    // in the kernel's force unit: plain stores.  A supercluster's blocks are
+    // [sc_blk_off[sc], sc_blk_off[sc + 1]); the finish kernel rounds each
+    // record to fixed point (scaled by fi_scale) and adds them per slot.
+    // Null: each block's i forces are rounded to fixed point in the force
+    // kernel and go to the accumulator F with atomics instead (same integers).
+    float *fi_blk;
+    const int *sc_blk_off;
     float fi_scale;
     const int *slot_atom;              // slot -> atom (-1 for padding), for the finish kernels
     NbParams p;
@@ -219,8 +231,12 @@ struct PrepArgs {
     const double *x, *y, *z;           // positions, atom order
     const int *atom_slot;
     const uint32_t *qref;              // fixed-point positions at the list build, 3 x n, atom order
-    int4 *qpos;                        // x, y, z written here; w (sigma/2) set by the build
+    int4 *qpos;                        // x, y, z and w = sigma/2 (NaN for a non-finite position)
+    const float2 *atom_par;            // per atom (sigma/2, sqrt eps), A
+    float sig_scale;                   // sigma/2 in the force kernel's length unit: A * sig_scale
     unsigned *disp;
+    unsigned *bad;                     // per slot: set for an atom whose position is not finite
+    int *bad_first;                    // reset to INT_MAX at the start of every evaluation
     double iq[3];                      // 2^32 / box
     float sq_len[3];                   // box / 2^32, A
     int n;
@@ -233,6 +249,7 @@ struct ReplicaDesc {
     PrepArgs prep;
     int n_bonded_groups;               // items [0, n) are bonded groups, then work items
     int n_wi;
+    int n_blk;                         // fixed-point blocks (LJ energy segment length)
 };
 
 // Term k of the flat range [bonds | angles | dihedrals | 1-4]; writes its energy.

@@ -29,8 +29,9 @@ STATUS_NAMES = {
     0: "GM_OK", 1: "GM_ERR_DEVICE_LOST", 2: "GM_ERR_KERNEL_LOAD", 3: "GM_ERR_OOM",
     4: "GM_ERR_NO_DEVICE", 5: "GM_ERR_INVALID_ARG", 6: "GM_ERR_KERNEL", 7: "GM_ERR_NOT_BUILT",
     8: "GM_ERR_CAPACITY", 9: "GM_ERR_CONVERGENCE", 10: "GMH_ERR_IO", 11: "GMH_ERR_CONFIG",
-    12: "GMH_ERR_NUMERICAL",
+    12: "GM_ERR_NUMERICAL",
 }
+GM_ERR_NUMERICAL = 12
 
 _P = C.c_void_p
 _I64 = C.c_int64
@@ -44,6 +45,17 @@ class GmError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
         self.status = status
+        self.result = None  # GM_ERR_NUMERICAL: the outputs, written all the same
+
+
+def _check_numerical(rc, result):
+    """_check, except that GM_ERR_NUMERICAL (a non-finite force; outputs are
+    written, those atoms hold NaN) raises with the outputs attached."""
+    if rc == GM_ERR_NUMERICAL:
+        err = GmError(rc, lib().gm_last_error().decode())
+        err.result = result
+        raise err
+    _check(rc)
 
 
 class SystemDesc(C.Structure):
@@ -111,7 +123,15 @@ class TrainReport(C.Structure):
 
 class MdDesc(C.Structure):
     _fields_ = [("n_steps", _I64), ("dt", _D), ("frame_interval", C.c_int32), ("max_iter", C.c_int32),
-                ("tol", _D)]
+                ("tol", _D), ("first_step", _I64), ("resume", C.c_int32)]
+
+
+class MdFailure(C.Structure):
+    """gm_md_failure: kind 0 none, 1 NanInPos, 2 NanInForce; component 0..2 = X, Y, Z."""
+    _fields_ = [("kind", C.c_int32), ("component", C.c_int32), ("step", _I64), ("atom", _I64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 SUR_NPARAM = 29315
@@ -199,6 +219,7 @@ def lib():
                                                 C.POINTER(C.c_int32)], C.c_int),
             "gm_md_run": ([_P, _P, _P, _P, _P, _P, _P, C.POINTER(MdDesc), _P, _P, _P, C.POINTER(_I64)], C.c_int),
             "gm_jacobi_factor": ([_P, C.c_double, C.POINTER(C.c_double)], C.c_int),
+            "gm_md_last_failure": ([_P, C.POINTER(MdFailure)], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -406,7 +427,8 @@ class System:
         x, y, z = (np.ascontiguousarray(a, np.float64) for a in (x, y, z))
         fx, fy, fz = np.empty(self.n), np.empty(self.n), np.empty(self.n)
         e = EnergyTerms()
-        _check(lib().gm_compute_forces(self.h, _p(x), _p(y), _p(z), _p(fx), _p(fy), _p(fz), C.byref(e)))
+        rc = lib().gm_compute_forces(self.h, _p(x), _p(y), _p(z), _p(fx), _p(fy), _p(fz), C.byref(e))
+        _check_numerical(rc, (e, fx, fy, fz))
         return e, fx, fy, fz
 
     def pair_list(self):
@@ -439,7 +461,7 @@ class System:
 
     def sync(self):
         e = EnergyTerms()
-        _check(lib().gm_sync(self.h, C.byref(e)))
+        _check_numerical(lib().gm_sync(self.h, C.byref(e)), e)
         return e
 
     # ---- device-resident dynamics (integrator.rs, constraint.rs, lib.rs run loop)
@@ -475,18 +497,29 @@ class System:
                                                     int(max_iter), float(tol), int(step), C.byref(it)))
         return it.value
 
-    def md_run(self, dx, dy, dz, dvx, dvy, dvz, n_steps, dt, frame_interval=1, max_iter=100, tol=1e-8):
-        """Returns (frames [n_frames, 3, n] f32, potential, kinetic)."""
+    def md_run(self, dx, dy, dz, dvx, dvy, dvz, n_steps, dt, frame_interval=1, max_iter=100, tol=1e-8,
+               first_step=0, resume=False):
+        """Returns (frames [n_frames, 3, n] f32, potential, kinetic).  On a
+        failure the GmError carries the frames taken before it in .result."""
         md = MdDesc(n_steps=int(n_steps), dt=float(dt), frame_interval=int(frame_interval),
-                    max_iter=int(max_iter), tol=float(tol))
-        cap = int(n_steps) // int(frame_interval) + 1
+                    max_iter=int(max_iter), tol=float(tol), first_step=int(first_step), resume=1 if resume else 0)
+        cap = int(n_steps) // int(frame_interval) + 2
         frames = np.empty((cap, 3, self.n), np.float32)
         epot, ekin = np.empty(cap), np.empty(cap)
         nf = _I64(0)
-        _check(lib().gm_md_run(self.h, *(_P(a.ptr) for a in (dx, dy, dz, dvx, dvy, dvz)), C.byref(md),
-                               _p(frames), _p(epot), _p(ekin), C.byref(nf)))
+        rc = lib().gm_md_run(self.h, *(_P(a.ptr) for a in (dx, dy, dz, dvx, dvy, dvz)), C.byref(md),
+                             _p(frames), _p(epot), _p(ekin), C.byref(nf))
         k = nf.value
+        if rc != GM_OK:
+            err = GmError(rc, lib().gm_last_error().decode())
+            err.result = (frames[:k], epot[:k], ekin[:k])
+            raise err
         return frames[:k], epot[:k], ekin[:k]
+
+    def md_last_failure(self):
+        f = MdFailure()
+        _check(lib().gm_md_last_failure(self.h, C.byref(f)))
+        return f.as_dict()
 
     def jacobi_factor(self, total_energy):
         out = C.c_double(0.0)
@@ -571,7 +604,8 @@ class Batch:
         """Returns (list of EnergyTerms per replica, device ms of the last compute)."""
         e = (EnergyTerms * len(self.sizes))()
         ms = C.c_float(0.0)
-        _check(lib().gm_batch_sync(self.h, C.cast(e, _P), C.byref(ms)))
+        rc = lib().gm_batch_sync(self.h, C.cast(e, _P), C.byref(ms))
+        _check_numerical(rc, ([e[k] for k in range(len(self.sizes))], ms.value))
         return [e[k] for k in range(len(self.sizes))], ms.value
 
     def get_forces(self):

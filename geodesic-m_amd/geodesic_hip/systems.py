@@ -7,6 +7,8 @@ gives the same system on every platform):
   protein_solvent(24000, 62.0, 3000)    24k atoms, 62 A periodic box
   protein_solvent(250000, 136.0, 30000) 250k atoms, 136 A periodic box
   replica_set(512)                      log-uniform 500..20000-atom replicas
+  lj_lattice(10000)                     the reference's own benchmark system
+                                        (geodesic/benches/benchmarks.rs:66-95)
 
 Layout: a jittered cubic lattice (about 0.1 atoms/A^3, jitter <= 0.3 A per
 coordinate) walked in a space-filling snake.  The first n_chain sites form a
@@ -255,3 +257,28 @@ def replicated_molecule(fields, pos, k, spacing):
     topo = Topology(**t)
     return (x, y, z, rep(fields["sigma"]).astype(np.float64), rep(fields["epsilon"]).astype(np.float64),
             rep(fields["mass"]).astype(np.float64), topo, float(k * spacing))
+
+
+def _empty_topology():
+    z32, zf = np.zeros(0, np.uint32), np.zeros(0, np.float64)
+    return Topology(z32, z32, zf, zf, z32, z32, z32, zf, zf, z32, z32, z32, z32, zf, z32, zf, z32, z32)
+
+
+def lj_lattice(n=10000):
+    """The workload of the reference's bench_lj_inner_loop and
+    bench_neighbor_rebuild (geodesic/benches/benchmarks.rs:49-95; paths
+    relative to the reference repository root): n atoms on a cubic lattice of
+    2.0 A spacing, side ceil(n^(1/3)) sites, filled x-major then y then z
+    (the last plane partly), in a periodic box of side * 2.0 A; sigma 3.0,
+    epsilon 0.1 for every atom; no topology; r_cutoff 5, r_skin 6, r_switch 4
+    (SimParams of lj_params).  Lattice distances hit the thresholds exactly:
+    pairs at r = 6.0 A (r2 == r_skin^2, in the list) and r = 4.0 A (r ==
+    r_switch, unswitched)."""
+    side = int(np.ceil(round(n ** (1.0 / 3.0), 12)))
+    ix, iy, iz = np.meshgrid(np.arange(side), np.arange(side), np.arange(side), indexing="ij")
+    sites = np.stack([ix.reshape(-1), iy.reshape(-1), iz.reshape(-1)], 1)[:n]
+    x, y, z = (np.ascontiguousarray(sites[:, a].astype(np.float64) * 2.0) for a in range(3))
+    box = side * 2.0
+    params = Params(box=(box, box, box), periodic=True, r_cutoff=5.0, r_switch=4.0, r_skin=6.0)
+    return SyntheticSystem(x, y, z, np.full(n, 3.0), np.full(n, 0.1), np.zeros(n, np.int64), _empty_topology(),
+                           params, 0, "lj_lattice")

@@ -108,10 +108,13 @@ void GpuBackend::build_neighbor_list(SimState &state, const SimParams &) {
           "gm_build_neighbor_list");
 }
 
+// Infallible as the trait's compute_forces is: a non-finite force
+// (GM_ERR_NUMERICAL) is returned as NaN in the buffer, for the caller's
+// finiteness check (lib.rs:79-92) to report.
 const ForceBuffer &GpuBackend::compute_forces(const SimState &state) {
-    check(gm_compute_forces(sys_, state.pos_x.data(), state.pos_y.data(), state.pos_z.data(), forces_.fx.data(),
-                            forces_.fy.data(), forces_.fz.data(), &terms_),
-          "gm_compute_forces");
+    const gm_status st = gm_compute_forces(sys_, state.pos_x.data(), state.pos_y.data(), state.pos_z.data(),
+                                           forces_.fx.data(), forces_.fy.data(), forces_.fz.data(), &terms_);
+    if (st != GM_ERR_NUMERICAL) check(st, "gm_compute_forces");
     return forces_;
 }
 
@@ -152,11 +155,12 @@ void GpuBackend::geodesic_drift(SimState &state, double dt) {
     download_state(state);
 }
 
-MdRecord GpuBackend::run_md(SimState &state, uint64_t n_steps, double dt, uint32_t frame_interval) {
+MdRecord GpuBackend::run_md(SimState &state, uint64_t first_step, uint64_t n_steps, double dt,
+                            uint32_t frame_interval, bool resume) {
     const size_t n = state.pos_x.size();
     if (frame_interval == 0) throw ConfigError("invalid value for key 'run.frame_interval': '0' (must be >= 1)");
     MdRecord r;
-    const uint64_t cap = n_steps / frame_interval + 1;
+    const uint64_t cap = n_steps / frame_interval + 2;
     r.frames.resize((size_t)cap * 3 * n);
     r.epot.resize((size_t)cap);
     r.ekin.resize((size_t)cap);
@@ -168,11 +172,18 @@ MdRecord GpuBackend::run_md(SimState &state, uint64_t n_steps, double dt, uint32
     md.frame_interval = (int32_t)frame_interval;
     md.max_iter = (int32_t)params_.max_constr_iter;
     md.tol = params_.constr_tol;
-    check(gm_md_run(sys_, d_[0], d_[1], d_[2], d_[3], d_[4], d_[5], &md, r.frames.data(), r.epot.data(),
-                    r.ekin.data(), &r.n_frames),
-          "gm_md_run");
+    md.first_step = (int64_t)first_step;
+    md.resume = resume ? 1 : 0;
+    const gm_status st = gm_md_run(sys_, d_[0], d_[1], d_[2], d_[3], d_[4], d_[5], &md, r.frames.data(),
+                                   r.epot.data(), r.ekin.data(), &r.n_frames);
+    if (st == GM_ERR_NUMERICAL || st == GM_ERR_CONVERGENCE) {
+        r.status = st;
+        r.error = gm_last_error();
+    } else {
+        check(st, "gm_md_run");
+    }
     download_state(state);
-    state.step += n_steps;
+    state.step = first_step + n_steps;
     if (r.n_frames > 0) {
         state.potential_energy = r.epot[(size_t)r.n_frames - 1];
         state.kinetic_energy = r.ekin[(size_t)r.n_frames - 1];

@@ -132,11 +132,16 @@ public:
     virtual size_t n_threads() const = 0;
 };
 
-// Frames and energies of a device-resident run (GpuBackend::run_md).
+// Frames and energies of a device-resident run segment (GpuBackend::run_md),
+// and how the segment ended: status GM_OK, or GM_ERR_NUMERICAL /
+// GM_ERR_CONVERGENCE with the reference's message (the frames taken before
+// the failure are kept).
 struct MdRecord {
     int64_t n_frames = 0;
     std::vector<float> frames;  // n_frames x (x block, y block, z block) x n_atoms
     std::vector<double> epot, ekin;
+    gm_status status = GM_OK;
+    std::string error;
 };
 
 // gpu_backend.rs:11-100 over libgeodesic_hip.so.  Owns one device context,
@@ -164,9 +169,14 @@ public:
 
     // Inherent: per-term energies of the last evaluation (EnergyReport fields).
     const gm_energy_terms &energy_terms() const { return terms_; }
-    // Inherent: the whole run loop of lib.rs:255-330 on the device
-    // (gm_md_run); positions and velocities advance in `state`.
-    MdRecord run_md(SimState &state, uint64_t n_steps, double dt, uint32_t frame_interval);
+    // Inherent: the run loop of lib.rs:255-330 on the device (gm_md_run),
+    // steps [first_step, first_step + n_steps); resume continues the previous
+    // segment (which must end on a frame step); positions and velocities
+    // advance in `state`.  A NaN or convergence failure ends the segment
+    // early and is returned in the record (the caller writes its frames, then
+    // raises it); other failures throw.
+    MdRecord run_md(SimState &state, uint64_t first_step, uint64_t n_steps, double dt, uint32_t frame_interval,
+                    bool resume);
     gm_sys *raw() const { return sys_; }
 
 private:

@@ -14,8 +14,10 @@
 // step returned to the host.  Frames go to the DCD and rows to the CSV log in
 // step order; energy-drift monitoring (lib.rs:365-386) is applied per frame
 // in the same order, so a Stop leaves the same files the reference would.
-// Non-finite positions are detected at frames (the reference checks every
-// step).
+// Non-finite positions and forces are checked on the device at every step
+// (lib.rs:79-107) and stop the run there, after the frames before it are
+// written.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -103,38 +105,52 @@ RunSummary run(const RunConfig &c) {
     DcdWriter dcd(c.trajectory, n, c.frame_interval, c.dt);
     EnergyLogWriter csv(c.energy_log, n);
 
-    MdRecord rec = backend->run_md(state, c.n_steps, c.dt, c.frame_interval);
-    const double e0 = rec.n_frames > 0 ? rec.epot[0] + rec.ekin[0] : 0.0;
-    for (int64_t k = 0; k < rec.n_frames; k++) {
-        const float *fr = rec.frames.data() + (size_t)k * 3 * n;
-        const uint64_t step = (uint64_t)k * c.frame_interval;
-        for (size_t i = 0; i < 3 * n; i++)
-            if (!std::isfinite(fr[i])) {
-                std::ostringstream m;
-                m << "NaN in position_" << "XYZ"[i / (n ? n : 1)] << " at step " << step << ", atom " << i % (n ? n : 1);
-                throw NumericalError(m.str());
-            }
-        dcd.write_frame(fr, fr + n, fr + 2 * n, n);
-        csv.write_row(step, (double)step * c.dt, rec.epot[(size_t)k], rec.ekin[(size_t)k]);
-        if (k > 0 && c.monitoring) {
-            const double drift = std::fabs(rec.epot[(size_t)k] + rec.ekin[(size_t)k] - e0);
-            if (drift > c.energy_drift_threshold_kcal) {
-                std::ostringstream m;
-                if (c.energy_drift_stop) {
-                    dcd.close();
-                    m << "energy drift " << drift << " kcal/mol exceeds threshold " << c.energy_drift_threshold_kcal
-                      << " kcal/mol at step " << step;
-                    throw NumericalError(m.str());
+    // The device loop runs in segments of whole frame intervals; each
+    // segment's frames are written before the next starts, so host memory is
+    // bounded and a failure leaves every frame taken before it on disk.
+    const uint64_t frame_bytes = 12 * (uint64_t)(n > 0 ? n : 1);
+    const uint64_t seg_frames = std::max<uint64_t>(1, std::min<uint64_t>(4096, (256ull << 20) / frame_bytes));
+    const uint64_t seg_steps = seg_frames * c.frame_interval;
+    uint64_t done = 0, frame_no = 0;
+    double e0 = 0.0;
+    bool first = true;
+    while (first || done < c.n_steps) {
+        const uint64_t ns = std::min<uint64_t>(seg_steps, c.n_steps - done);
+        MdRecord rec = backend->run_md(state, done, ns, c.dt, c.frame_interval, !first);
+        for (int64_t k = 0; k < rec.n_frames; k++, frame_no++) {
+            const float *fr = rec.frames.data() + (size_t)k * 3 * n;
+            const uint64_t step = frame_no * c.frame_interval;
+            dcd.write_frame(fr, fr + n, fr + 2 * n, n);
+            csv.write_row(step, (double)step * c.dt, rec.epot[(size_t)k], rec.ekin[(size_t)k]);
+            if (frame_no == 0) {
+                e0 = rec.epot[(size_t)k] + rec.ekin[(size_t)k];
+            } else if (c.monitoring) {
+                const double drift = std::fabs(rec.epot[(size_t)k] + rec.ekin[(size_t)k] - e0);
+                if (drift > c.energy_drift_threshold_kcal) {
+                    std::ostringstream m;
+                    if (c.energy_drift_stop) {
+                        dcd.close();
+                        m << "energy drift " << drift << " kcal/mol exceeds threshold "
+                          << c.energy_drift_threshold_kcal << " kcal/mol at step " << step;
+                        throw NumericalError(m.str());
+                    }
+                    std::fprintf(stderr, "warning: energy drift %.4f kcal/mol exceeds threshold %.4f at step %llu\n",
+                                 drift, c.energy_drift_threshold_kcal, (unsigned long long)step);
                 }
-                std::fprintf(stderr, "warning: energy drift %.4f kcal/mol exceeds threshold %.4f at step %llu\n",
-                             drift, c.energy_drift_threshold_kcal, (unsigned long long)step);
             }
         }
+        if (rec.status != GM_OK) {
+            dcd.close();
+            if (rec.status == GM_ERR_CONVERGENCE) throw ConvergenceError(rec.error);
+            throw NumericalError(rec.error);
+        }
+        done += ns;
+        first = false;
     }
     dcd.close();
     RunSummary s;
     s.n_steps = c.n_steps;
-    s.n_frames = (size_t)rec.n_frames;
+    s.n_frames = (size_t)frame_no;
     s.final_potential = state.potential_energy;
     s.final_kinetic = state.kinetic_energy;
     s.trajectory = c.trajectory;

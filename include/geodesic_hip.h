@@ -48,6 +48,9 @@ enum {
     GM_ERR_NOT_BUILT = 7,     /* compute before build_neighbor_list */
     GM_ERR_CAPACITY = 8,      /* output buffer too small / fixed-point or LDS capacity exceeded */
     GM_ERR_CONVERGENCE = 9,   /* ConvergenceError::ConstraintSolver (error.rs): SHAKE/RATTLE did not converge */
+    /* 10, 11: host-layer codes (geodesic_host.h) */
+    GM_ERR_NUMERICAL = 12,    /* NumericalError::NanInForce / NanInPos (error.rs; geodesic/src/lib.rs:79-107):
+                                 a force or position is not finite.  Outputs are still written. */
 };
 
 typedef struct gm_ctx gm_ctx; /* one per device */
@@ -243,21 +246,44 @@ gm_status gm_constrain_velocities_device(gm_sys *sys, const double *d_x, const d
                                          double *d_vx, double *d_vy, double *d_vz, int32_t max_iter, double tol,
                                          int64_t step, int32_t *iters);
 typedef struct {
-    int64_t n_steps;         /* SimParams::n_steps */
+    int64_t n_steps;         /* SimParams::n_steps (of this call) */
     double dt;               /* ps */
     int32_t frame_interval;  /* steps between frames */
     int32_t max_iter;        /* constraint iterations */
     double tol;              /* constraint tolerance */
+    int64_t first_step;      /* [ext] step number of this call's first step (reports, frame schedule) */
+    int32_t resume;          /* [ext] 1: continue the previous call's run (no list build, initial forces,
+                                velocity projection or frame at first_step) */
 } gm_md_desc;
-/* The reference run loop on the device: list, forces and velocity projection,
- * then per step B, A (drift + SHAKE), list rebuild when needs_rebuild fires,
- * forces, B, RATTLE.  Every frame_interval steps (and at step 0) the f32
- * positions (3 x n: x block, y block, z block) go to h_frames, the potential
- * and kinetic energy (kcal/mol) to h_epot / h_ekin (each may be NULL);
- * *n_frames receives the count.  Positions and velocities advance in place. */
+/* The reference run loop on the device (geodesic/src/lib.rs:255-330): list,
+ * forces and velocity projection, then per step B, A (drift + SHAKE), list
+ * rebuild when needs_rebuild fires, forces, B, RATTLE.  A frame is taken at
+ * the start (unless resuming) and after every step s with (s + 1) %
+ * frame_interval == 0 (s counted from first_step): the f32 positions (3 x n:
+ * x block, y block, z block) go to h_frames, the potential and kinetic energy
+ * (kcal/mol) to h_epot / h_ekin (each may be NULL).  *n_frames receives the
+ * count, also when the run stops early.  Positions and velocities advance in
+ * place.  A long run can be cut into calls that end on frame steps (resume =
+ * 1, first_step advanced), with the same results as one call.
+ * Failures stop the run at the step where the reference stops:
+ *   - GM_ERR_NUMERICAL: a position not finite after the drift of step s
+ *     (NanInPos, lib.rs:94-107, checked at :296) or a force not finite after
+ *     the evaluation of step s (NanInForce, lib.rs:79-92, checked at :269 and
+ *     :307); gm_md_last_failure gives step, atom and component;
+ *   - GM_ERR_CONVERGENCE: SHAKE or RATTLE did not converge. */
 gm_status gm_md_run(gm_sys *sys, double *d_x, double *d_y, double *d_z, double *d_vx, double *d_vy,
                     double *d_vz, const gm_md_desc *md, float *h_frames, double *h_epot, double *h_ekin,
                     int64_t *n_frames);
+/* The NumericalError of the last gm_md_run on this system: kind 0 none,
+ * 1 NanInPos, 2 NanInForce; component 0 X, 1 Y, 2 Z (error.rs NumericalError).
+ * An atom whose force is not finite holds NaN in all three components. */
+typedef struct {
+    int32_t kind;
+    int32_t component;
+    int64_t step;
+    int64_t atom;
+} gm_md_failure;
+gm_status gm_md_last_failure(gm_sys *sys, gm_md_failure *out);
 /* Jacobi-metric factor 2 (E - V(x)) in f64 for total energy E (SimParams
  * total_energy, params.rs:21-22) and the potential of the last evaluation. */
 gm_status gm_jacobi_factor(gm_sys *sys, double total_energy, double *out);

@@ -29,7 +29,7 @@ extern "C" {
 enum {
     GMH_ERR_IO = 10,        /* IoError */
     GMH_ERR_CONFIG = 11,    /* ConfigError (parse errors, invalid values) */
-    GMH_ERR_NUMERICAL = 12, /* NumericalError (NaN positions, energy drift with action Stop) */
+    GMH_ERR_NUMERICAL = GM_ERR_NUMERICAL, /* NumericalError (NaN positions or forces, energy drift with action Stop) */
 };
 
 const char *gmh_last_error(void);

@@ -56,6 +56,10 @@ int main(void) {
   F(gm_train_report, loss_taylor) F(gm_train_report, ms_total) F(gm_train_report, steps)
   printf("gm_md_desc %zu\n", sizeof(gm_md_desc));
   F(gm_md_desc, dt) F(gm_md_desc, frame_interval) F(gm_md_desc, max_iter) F(gm_md_desc, tol)
+  F(gm_md_desc, first_step) F(gm_md_desc, resume)
+  printf("gm_md_failure %zu\n", sizeof(gm_md_failure));
+  F(gm_md_failure, component) F(gm_md_failure, step) F(gm_md_failure, atom)
+  printf("GM_ERR_NUMERICAL %d\n", (int)GM_ERR_NUMERICAL);
   printf("GM_ERR_CONVERGENCE %d\n", (int)GM_ERR_CONVERGENCE);
   printf("GM_SUR_NPARAM %d\n", (int)GM_SUR_NPARAM);
   printf("GM_SUR_NFEAT %d\n", (int)GM_SUR_NFEAT);
@@ -79,8 +83,11 @@ def test_ctypes_structs_match_c_layout(tmp_path):
     assert int(out["gm_train_report"]) == C.sizeof(gh.TrainReport)
     assert int(out["GM_SUR_NPARAM"]) == gh.SUR_NPARAM and int(out["GM_SUR_NFEAT"]) == gh.SUR_NFEAT
     assert int(out["gm_md_desc"]) == C.sizeof(gh.MdDesc) and int(out["GM_ERR_CONVERGENCE"]) == 9
+    assert int(out["gm_md_failure"]) == C.sizeof(gh.MdFailure)
+    assert int(out["GM_ERR_NUMERICAL"]) == gh.GM_ERR_NUMERICAL
     classes = {"gm_system_desc": gh.SystemDesc, "gm_stats": gh.Stats, "gm_surrogate_desc": gh.SurrogateDesc,
-               "gm_train_desc": gh.TrainDesc, "gm_train_report": gh.TrainReport, "gm_md_desc": gh.MdDesc}
+               "gm_train_desc": gh.TrainDesc, "gm_train_report": gh.TrainReport, "gm_md_desc": gh.MdDesc,
+               "gm_md_failure": gh.MdFailure}
     for key, val in out.items():
         if "." in key:
             t, m = key.split(".")
