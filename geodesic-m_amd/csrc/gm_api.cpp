@@ -320,12 +320,24 @@ gm_status build_on_device(gm_sys *s, bool wrap) {
     return GM_OK;
 }
 
+EvalBufs own_bufs(gm_sys *s) {
+    EvalBufs b{};
+    b.F = s->F.p;
+    b.bad = s->bad.p;
+    b.disp = s->disp.p;
+    b.e_items = s->e_items.p;
+    b.e_blk = s->e_blk.p;
+    b.fi_blk = s->fi_records ? s->fi_blk.p : nullptr;
+    return b;
+}
+
 gm_status compute_on_device(gm_sys *s, const double *x, const double *y, const double *z,
-                            double *fx, double *fy, double *fz, bool shell) {
+                            double *fx, double *fy, double *fz, bool shell, const EvalBufs *bufs) {
     if (!s->built) return set_error(GM_ERR_NOT_BUILT, "compute_forces before build_neighbor_list");
     hipStream_t st = s->stream;
     hipEvent_t ev[kProfStride] = {nullptr, nullptr, nullptr, nullptr};
-    if (s->profiling && (s->prof_seen++ % s->prof_every) == 0) {
+    s->ev = bufs ? *bufs : own_bufs(s);
+    if (!bufs && s->profiling && (s->prof_seen++ % s->prof_every) == 0) {
         if (s->prof_used < kProfMax) {
             const size_t need = kProfStride * (size_t)(s->prof_used + 1);
             while (s->prof_pool.size() < need) {
@@ -344,12 +356,14 @@ gm_status compute_on_device(gm_sys *s, const double *x, const double *y, const d
     // Profiled ones carry no separate event packets either: the dispatches
     // stamp them, ev[0] at the prep kernel's start, ev[1] / ev[2] at the force
     // kernel's start and end, ev[3] at the finish kernel's end.
-    if (s->F_dirty) GM_HIP(launch_fill_u64(s->F.p, (size_t)s->nsc * 64 * 3, st));
-    s->F_dirty = true;
+    if (!bufs) {
+        if (s->F_dirty) GM_HIP(launch_fill_u64(s->F.p, (size_t)s->nsc * 64 * 3, st));
+        s->F_dirty = true;
+    }
     GM_HIP(launch_prep(s, x, y, z, ev[0]));
     GM_HIP(launch_forces(s, x, y, z, shell, ev[1], ev[2]));
     GM_HIP(launch_finish(s, fx, fy, fz, shell, ev[3]));  // also clears the accumulator
-    s->F_dirty = false;
+    if (!bufs) s->F_dirty = false;
     s->last_timed = ev[0] ? s->prof_used - 1 : -1;
     return GM_OK;
 }
@@ -415,6 +429,11 @@ gm_status gm_init(int device_ordinal, gm_ctx **out) {
     c->device = device_ordinal;
     c->n_cu = prop.multiProcessorCount;
     c->arch = arch;
+    const gm_status rc = scratch_init(c);  // the device is current
+    if (rc != GM_OK) {
+        delete c;
+        return rc;
+    }
     *out = c;
     return GM_OK;
 }

@@ -73,6 +73,7 @@ gm_status make_plan(gm_batch *b) {
     for (int r = 0; r < R; r++) {
         gm_sys *s = b->reps[(size_t)r];
         ReplicaDesc &d = desc[(size_t)r];
+        s->ev = own_bufs(s);
         d.tile = tile_args(s);
         d.prep = prep_args(s, s->x.p, s->y.p, s->z.p);
         d.terms = term_args(s, s->x.p, s->y.p, s->z.p);

@@ -205,14 +205,14 @@ hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int2 *
 
 hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz, bool lj_only, hipEvent_t ev_stop) {
     Segs sg{};
-    const double *eb = s->e_items.p;
+    const double *eb = s->ev.e_items;
     const int64_t n14 = (s->scale14 != 0.0) ? s->n14 : 0;
     const int64_t nb = lj_only ? 0 : s->nb, na = lj_only ? 0 : s->na, nd = lj_only ? 0 : s->nd;
     sg.ptr[0] = eb;                         sg.len[0] = nb;
     sg.ptr[1] = eb + s->nb;                 sg.len[1] = na;
     sg.ptr[2] = eb + s->nb + s->na;         sg.len[2] = nd;
     sg.ptr[3] = eb + s->nb + s->na + s->nd; sg.len[3] = lj_only ? 0 : n14;
-    sg.ptr[4] = s->e_blk.p;                 sg.len[4] = s->n_blk;
+    sg.ptr[4] = s->ev.e_blk;                sg.len[4] = s->n_blk;
     sg.nseg = 5;
     const size_t n_pad = (size_t)s->nsc * 64;
     // wavefronts per supercluster: about one block record each, 4 to 16;
@@ -221,15 +221,15 @@ hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz, bool lj_
     const int G = !s->fi_records ? 1 : (per_sc > 8 ? 16 : (per_sc > 4 ? 8 : 4));
     const unsigned grid = 5u + (unsigned)((s->nsc + 16 / G - 1) / (16 / G));
     const float fi_scale = span_is_cubic(s) ? s->nbp.fscale_u : 4294967296.0f;
-    const float *fi = s->fi_records ? s->fi_blk.p : nullptr;
+    const float *fi = s->ev.fi_blk;
     int *bf = bad_first_word(s);
     if (ev_stop)  // stamped by the dispatch at the kernel's end
         hipExtLaunchKernelGGL(k_finish, dim3(grid), dim3(1024), 0, s->stream, nullptr, ev_stop, 0, sg, s->e_out.p,
-                              (const int *)s->slot_atom.p, s->F.p, n_pad, fi, (const int *)s->sc_blk_off.p,
-                              fi_scale, G, fx, fy, fz, s->disp.p, s->bad.p, bf);
+                              (const int *)s->slot_atom.p, s->ev.F, n_pad, fi, (const int *)s->sc_blk_off.p,
+                              fi_scale, G, fx, fy, fz, s->ev.disp, s->ev.bad, bf);
     else
-        k_finish<<<grid, 1024, 0, s->stream>>>(sg, s->e_out.p, s->slot_atom.p, s->F.p, n_pad, fi, s->sc_blk_off.p,
-                                                fi_scale, G, fx, fy, fz, s->disp.p, s->bad.p, bf);
+        k_finish<<<grid, 1024, 0, s->stream>>>(sg, s->e_out.p, s->slot_atom.p, s->ev.F, n_pad, fi, s->sc_blk_off.p,
+                                                fi_scale, G, fx, fy, fz, s->ev.disp, s->ev.bad, bf);
     return hipGetLastError();
 }
 

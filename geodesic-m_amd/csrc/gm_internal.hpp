@@ -142,6 +142,20 @@ constexpr int kPruneSteps = 254;  // bytes 0..254 span [r_cutoff, r_skin]
 // block raises word blockIdx % kDispSlots) so that same-address atomics do not
 // serialise; readers take the largest.
 constexpr int kDispSlots = 16, kDispStride = 16;
+
+// The device memory one force evaluation writes besides its outputs (its
+// scratch): the int64 fixed-point accumulator (3 x slots), per-slot
+// non-finite flags, the displacement words, bonded energies, per-block LJ
+// energies and i-force records.  A system's own buffers, or a region of the
+// library's static scratch array (gm_compute_forces_device_noalloc).
+struct EvalBufs {
+    unsigned long long *F;
+    unsigned *bad;
+    unsigned *disp;
+    double *e_items;
+    double *e_blk;
+    float *fi_blk;  // null when the i forces go to F with atomics
+};
 // Work counters of the batched tile kernel (one per XCD), this many words apart.
 constexpr int kCtrStride = 32;
 
@@ -166,6 +180,8 @@ struct gm_ctx {
     int device = 0;
     int n_cu = 0;
     std::string arch;
+    void *scratch = nullptr;   // this device's copy of the static scratch array (gm_noalloc.hip)
+    size_t scratch_bytes = 0;
 };
 
 struct gm_sys {
@@ -274,6 +290,7 @@ struct gm_sys {
     uint32_t *imask_host = nullptr;       // pinned readback of the entries' tile masks (work item costs)
     size_t imask_host_n = 0;
 
+    gm::EvalBufs ev{};  // the buffers the launchers use (own_bufs unless a caller gives others)
     bool built = false;
     int64_t nsc = 0, ncl = 0, nent = 0, n_wi = 0, n_blk = 0;
     int last_bad = 0x7fffffff;  // first atom with a non-finite force in the last collected evaluation
@@ -303,8 +320,16 @@ gm_status build_on_device(gm_sys *s, bool wrap);
 // Forces at device positions into device arrays (f64, atom order), on the
 // system's stream.  shell: LJ only, each pair weighted by S(r) - S_in(r)
 // (nbp_shell), no bonded terms.
+// bufs: the evaluation's scratch (null: the system's own); with bufs the
+// caller has zeroed its accumulator, flags and displacement words, and no
+// profiling events are recorded.
 gm_status compute_on_device(gm_sys *s, const double *x, const double *y, const double *z, double *fx,
-                            double *fy, double *fz, bool shell = false);
+                            double *fy, double *fz, bool shell = false, const EvalBufs *bufs = nullptr);
+// The system's own evaluation buffers.
+EvalBufs own_bufs(gm_sys *s);
+// gm_noalloc.hip: this device's address of the static scratch array into ctx
+// (called by gm_init with the device current).
+gm_status scratch_init(gm_ctx *ctx);
 // Waits for the system's stream and reads the last evaluation's energies
 // and first non-finite atom (last_bad).
 gm_status collect(gm_sys *s, gm_energy_terms *e);

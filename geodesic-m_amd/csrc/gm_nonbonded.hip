@@ -559,8 +559,8 @@ TermArgs term_args(gm_sys *s, const double *x, const double *y, const double *z)
     t.x = x; t.y = y; t.z = z;
     for (int a = 0; a < 3; a++) t.box[a] = s->box[a];
     t.mi = s->bonded_mi;
-    t.F = s->F.p;
-    t.bad = s->bad.p;
+    t.F = s->ev.F;
+    t.bad = s->ev.bad;
     t.atom_slot = s->atom_slot.p;
     t.n_pad = (size_t)s->nsc * 64;
     t.bi = s->bi.p; t.bj = s->bj.p; t.bk = s->bk.p; t.br0 = s->br0.p;
@@ -572,7 +572,7 @@ TermArgs term_args(gm_sys *s, const double *x, const double *y, const double *z)
     t.scale14 = s->scale14;
     t.nb = s->nb; t.na = s->na; t.nd = s->nd;
     t.n14 = (s->scale14 != 0.0) ? s->n14 : 0;
-    t.e_items = s->e_items.p;
+    t.e_items = s->ev.e_items;
     return t;
 }
 
@@ -582,16 +582,16 @@ TileArgs tile_args(gm_sys *s) {
     A.qeps = s->qeps.p;
     A.wi = s->wi.p;
     A.ent_meta = s->ent_meta.p; A.ent_dmin = s->ent_dmin.p; A.ent_lb = s->ent_lb.p;
-    A.disp = (s->prune && s->prune_geom) ? s->disp.p : nullptr;
+    A.disp = (s->prune && s->prune_geom) ? s->ev.disp : nullptr;
     // keep iff byte <= (2 disp + 1e-4 r_c) / step + 1 (see kPruneSteps)
     A.prune_scale = 2.0f * s->bp.prune_inv_step;
     A.prune_bias = (float)(1e-4 * s->rc) * s->bp.prune_inv_step + 1.0f;
-    A.F = s->F.p;
-    A.bad = s->bad.p;
+    A.F = s->ev.F;
+    A.bad = s->ev.bad;
     A.bad_first = bad_first_word(s);
     A.n_pad = (int)(s->nsc * 64);
-    A.e_blk = s->e_blk.p;
-    A.fi_blk = s->fi_records ? s->fi_blk.p : nullptr;
+    A.e_blk = s->ev.e_blk;
+    A.fi_blk = s->ev.fi_blk;
     A.sc_blk_off = s->sc_blk_off.p;
     A.fi_scale = span_is_cubic(s) ? s->nbp.fscale_u : 4294967296.0f;  // the tile kernel's fscale
     A.slot_atom = s->slot_atom.p;
@@ -608,8 +608,8 @@ PrepArgs prep_args(gm_sys *s, const double *x, const double *y, const double *z)
     P.qpos = s->qpos.p;
     P.atom_par = s->atom_par.p;
     P.sig_scale = span_is_cubic(s) ? s->nbp.inv_s : 1.0f;  // as the list build's sigma/2
-    P.disp = s->disp.p;
-    P.bad = s->bad.p;
+    P.disp = s->ev.disp;
+    P.bad = s->ev.bad;
     P.bad_first = bad_first_word(s);
     for (int a = 0; a < 3; a++) {
         P.iq[a] = s->bp.inv_q[a];

@@ -220,6 +220,7 @@ def lib():
             "gm_md_run": ([_P, _P, _P, _P, _P, _P, _P, C.POINTER(MdDesc), _P, _P, _P, C.POINTER(_I64)], C.c_int),
             "gm_jacobi_factor": ([_P, C.c_double, C.POINTER(C.c_double)], C.c_int),
             "gm_md_last_failure": ([_P, C.POINTER(MdFailure)], C.c_int),
+            "gm_compute_forces_device_noalloc": ([_P, _P, _P, _P, _P, _P, _P], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -442,6 +443,10 @@ class System:
     # device-resident variants: integer device pointers (e.g. tensor.data_ptr())
     def build_neighbor_list_device(self, dx, dy, dz):
         _check(lib().gm_build_neighbor_list_device(self.h, _P(dx), _P(dy), _P(dz)))
+
+    def compute_forces_device_noalloc(self, dx, dy, dz, dfx, dfy, dfz):
+        """gm_compute_forces_device with the library's static scratch array."""
+        _check(lib().gm_compute_forces_device_noalloc(self.h, _P(dx), _P(dy), _P(dz), _P(dfx), _P(dfy), _P(dfz)))
 
     def compute_forces_device(self, dx, dy, dz, dfx, dfy, dfz):
         _check(lib().gm_compute_forces_device(self.h, _P(dx), _P(dy), _P(dz), _P(dfx), _P(dfy), _P(dfz)))

@@ -133,6 +133,18 @@ gm_status gm_get_pair_list(gm_sys *sys, uint32_t *pair_i, uint32_t *pair_j, int6
 gm_status gm_build_neighbor_list_device(gm_sys *sys, double *d_x, double *d_y, double *d_z);
 gm_status gm_compute_forces_device(gm_sys *sys, const double *d_x, const double *d_y,
                                    const double *d_z, double *d_fx, double *d_fy, double *d_fz);
+/* No counterpart in the reference: for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * gm_compute_forces_device, bit for bit, except that the evaluation's scratch
+ * (fixed-point accumulator, non-finite flags, displacement words, energy
+ * partials, i-force records) lives in a static device array of 32 MiB the
+ * library holds on each GPU until the process ends; it calls no allocator.
+ * The system must be built (list, work items) beforehand.  GM_ERR_CAPACITY if
+ * the list needs more than 32 MiB (about 200k atoms with the automatic work
+ * shape).  All calls on one GPU share that array: calls that overlap on one
+ * GPU (two streams or host threads) race. */
+gm_status gm_compute_forces_device_noalloc(gm_sys *sys, const double *d_x, const double *d_y, const double *d_z,
+                                           double *d_fx, double *d_fy, double *d_fz);
 /* ComputeBackend::needs_rebuild at device positions (synchronous). */
 gm_status gm_needs_rebuild_device(gm_sys *sys, const double *d_x, const double *d_y, const double *d_z,
                                   int32_t *out);
