@@ -99,6 +99,82 @@ def time_resident(g, steps, warmup, every=PROFILE_EVERY):
     return (t1 - t0) * 1e3 / steps, prof, e, (t3 - t2) * 1e3 / steps
 
 
+ROOF_LAUNCHES = 200         # the roofline pass times this many force-kernel launches
+
+
+def roofline_pass(g, n_pairs, launches=ROOF_LAUNCHES):
+    """A fixed-length pass beside the timed region: every evaluation's force
+    kernel timed by its own dispatch-stamped start and end events.  Returns
+    the kernel's average and minimum ms over `launches` launches and the
+    achieved FP32 rate (canonical 40 FLOP per pair interaction)."""
+    g.profile_begin(1)
+    for _ in range(launches):
+        g.compute_forces_resident()
+    g.sync()
+    prof = g.profile_end()
+    avg = prof["ms_nonbonded_total"] / max(prof["n_evals"], 1)
+    achieved = n_pairs * FLOP_PER_PAIR / (avg * 1e-3) / 1e12
+    return {"bound": "valu", "kernel_ms_avg": avg, "kernel_ms_min": prof["ms_nonbonded_min"],
+            "timed_launches": prof["n_evals"], "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+            "eval_ms_avg": prof["ms_eval_total"] / max(prof["n_evals"], 1)}
+
+
+def bench_reference_workloads(cpu=True):
+    """The reference's own benchmarks (geodesic/benches/benchmarks.rs:97-135):
+    bench_lj_inner_loop (nonbonded throughput in atom-pairs/s over the Verlet
+    list) and bench_neighbor_rebuild (list build time), on its 10,000-atom
+    lattice (2.0 A spacing, sigma 3, eps 0.1, r_c/r_skin/r_sw 5/6/4), with the
+    oracle's restatement of the reference's single-threaded routines beside
+    them (compute_pair_forces over the list; the serial O(N^2) neighbor::build)."""
+    import geodesic_hip as gh
+    from geodesic_hip import systems
+    s = systems.lj_lattice(10000)
+    g = gh.System(s.sigma, s.epsilon, s.topo, s.params)
+    x, y, z = s.x.copy(), s.y.copy(), s.z.copy()
+    g.build_neighbor_list(x, y, z)
+    builds = []
+    for _ in range(5):
+        g.build_neighbor_list(x, y, z)
+        builds.append(g.stats()["ms_build"])
+    t0 = time.perf_counter()
+    for _ in range(5):
+        g.build_neighbor_list(x, y, z)
+    build_wall_ms = (time.perf_counter() - t0) * 1e3 / 5
+    st = g.stats()
+    n_list, n_cut = st["n_list_pairs"], st["n_pairs_cutoff"]
+    ms, _, _, _ = time_resident(g, 200, 20)
+    out = {"workload": "10,000-atom lattice (2.0 A spacing), 44 A periodic box, r_c/r_skin/r_sw 5/6/4, "
+                       "sigma 3 / eps 0.1, no topology",
+           "n_list_pairs": n_list, "n_pairs_cutoff": n_cut,
+           "bench_lj_inner_loop": {"ms_per_eval": ms, "atom_pairs_per_s": n_list / (ms * 1e-3),
+                                   "pair_interactions_per_s": n_cut / (ms * 1e-3)},
+           "bench_neighbor_rebuild": {"ms_device": statistics.median(builds), "ms_wall": build_wall_ms}}
+    if cpu:
+        from oracle import oracle as orc
+        pi, pj = orc.neighbor_build(x, y, z, s.params.box, s.params.r_skin, s.topo.excl_i, s.topo.excl_j,
+                                    method="brute")
+        t0 = time.perf_counter()
+        orc.neighbor_build(x, y, z, s.params.box, s.params.r_skin, s.topo.excl_i, s.topo.excl_j, method="brute")
+        cpu_build_s = time.perf_counter() - t0
+        times = []
+        for k in range(8):
+            t0 = time.perf_counter()
+            orc.pair_forces(x, y, z, s.sigma, s.epsilon, pi, pj, s.params.r_cutoff, s.params.r_switch,
+                            s.params.box)
+            if k >= 2:
+                times.append(time.perf_counter() - t0)
+        cms = statistics.median(times) * 1e3
+        out["cpu_baseline"] = {"cores": 1, "kind": "port",
+                               "sample": "6 evaluations (median) of compute_pair_forces over the list after 2 "
+                                         "warm-ups; one serial O(N^2) list build",
+                               "bench_lj_inner_loop_ms": cms, "atom_pairs_per_s": n_list / (cms * 1e-3),
+                               "bench_neighbor_rebuild_ms": cpu_build_s * 1e3}
+        out["speedup_lj_inner_loop"] = cms / ms
+        out["speedup_neighbor_rebuild_wall"] = cpu_build_s * 1e3 / build_wall_ms
+    return out
+
+
 def bench_250k(steps=100, warmup=10):
     """250,000 atoms, 136 A box: force evaluation at resident positions and the
     list build, amortised over the 20-step rebuild interval."""
@@ -116,9 +192,10 @@ def bench_250k(steps=100, warmup=10):
     ms, prof, e, ms_all_events = time_resident(g, steps, warmup)
     build_ms = statistics.median(builds)
     n_cut = st["n_pairs_cutoff"]
+    roof = roofline_pass(g, n_cut, 50)
     return {"workload": "force evaluation, 250,000 atoms, 136 A periodic box, rebuild every 20 steps",
             "n_pairs_cutoff": n_cut, "ms_per_eval": ms, "ms_per_eval_events_every_eval": ms_all_events,
-            "kernel_ms_avg": prof["ms_nonbonded_total"] / max(prof["n_evals"], 1),
+            "kernel_ms_avg": roof["kernel_ms_avg"], "roofline": roof,
             "list_build_ms": build_ms, "ms_per_step_amortised": ms + build_ms / 20.0,
             "pair_interactions_per_s": n_cut / (ms * 1e-3),
             "pair_interactions_per_s_amortised": n_cut / ((ms + build_ms / 20.0) * 1e-3)}
@@ -149,20 +226,35 @@ def bench_batched(n_replicas=512, steps=20):
         b.compute()
     _, last_ms = b.sync()
     batched_ms = (time.perf_counter() - t0) * 1e3 / steps
-    # plain path: each replica's own launches (prep, forces, finish), back to back on one stream
-    plain = [b.replica(k) for k in range(len(reps))]
+    # plain path: every replica as its own system (own stream, automatic work
+    # shape) with its own launches (prep, forces, finish): all 512 evaluations
+    # launched, then all waited for
+    plain = []
+    for r in reps:
+        g = gh.System(r.sigma, r.epsilon, r.topo, r.params)
+        px, py, pz = r.x.copy(), r.y.copy(), r.z.copy()
+        g.build_neighbor_list(px, py, pz)
+        plain.append(g)
     for g in plain:
         g.compute_forces_resident()
-    plain[-1].sync()
+    for g in plain:
+        g.sync()
     sweeps = 3
     t0 = time.perf_counter()
     for _ in range(sweeps):
         for g in plain:
             g.compute_forces_resident()
-    plain[-1].sync()
+    for g in plain:
+        g.sync()
     plain_ms = (time.perf_counter() - t0) * 1e3 / sweeps
+    for g in plain:
+        g.close()
+    step_tflops = n_cut * FLOP_PER_PAIR / (last_ms * 1e-3) / 1e12
     return {"workload": f"{len(reps)} replicas, {b.n} atoms, log-uniform 500..20000 atoms, "
                         "one persistent launch per step from a captured hipGraph",
+            "roofline": {"bound": "valu", "achieved": step_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                         "frac": step_tflops / PEAK_FP32_TFLOPS,
+                         "timing": "device time of the whole graph step (prep, bonded, tiles, finish)"},
             "n_atoms": b.n, "n_pairs_cutoff": n_cut, "ms_per_batched_step": batched_ms,
             "device_ms_last_step": last_ms, "ms_plain_path_all_replicas": plain_ms,
             "pair_interactions_per_s": n_cut / (batched_ms * 1e-3),
@@ -380,12 +472,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--chunk", type=int, default=-1, help="entries per work item (tuning)")
     ap.add_argument("--no-250k", action="store_true")
+    ap.add_argument("--no-reference-workloads", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-md", action="store_true")
     ap.add_argument("--no-pruning-study", action="store_true",
                     help="skip the pruning-off and drifted-position timings (profiling)")
-    ap.add_argument("--only", choices=("250k", "batched", "surrogate", "md"), default=None,
+    ap.add_argument("--only", choices=("250k", "batched", "surrogate", "md", "reference"), default=None,
                     help="run one secondary section alone and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -404,6 +497,9 @@ def main():
         return
     if args.only == "md":
         print(json.dumps(bench_md(cpu=not args.no_cpu_baseline)))
+        return
+    if args.only == "reference":
+        print(json.dumps(bench_reference_workloads(cpu=not args.no_cpu_baseline)))
         return
 
     import geodesic_hip as gh
@@ -433,11 +529,10 @@ def main():
     for _ in range(n_host):
         g.compute_forces(hx, hy, hz)
     ms_host = (time.perf_counter() - t0) * 1e3 / n_host
-    nb_ms = prof["ms_nonbonded_total"] / max(prof["n_evals"], 1)
     ev_ms = prof["ms_eval_total"] / max(prof["n_evals"], 1)
     prep_ms = prof["ms_prep_total"] / max(prof["n_evals"], 1)
+    roof = roofline_pass(g, n_cut)
     pruning = None if args.no_pruning_study else pruning_study(g, s, x, y, z, e, args)
-    achieved = n_cut * FLOP_PER_PAIR / (nb_ms * 1e-3) / 1e12
 
     out = {
         "metric": "pair-interactions/s (24k-atom force evaluation)",
@@ -455,13 +550,14 @@ def main():
         "config": {"workload": "force evaluation, 24,000 atoms, 62 A periodic box, r_c 9 / r_sw 7.5 / list 10.5 A",
                    "n_atoms": s.n, "n_list_pairs": n_list, "n_pairs_cutoff": n_cut,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix work in this kernel)",
-                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS,
+        "roofline": {"bound": "valu", "compute": "fp32 VALU (no matrix work in this kernel)",
+                     "achieved": roof["achieved"], "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": roof["frac"],
                      "traffic": traffic_from_profiles("nonbonded_24k_traffic.json"),
-                     "kernel": "k_forces<cubic> (tiles + bonded workgroups)", "kernel_ms_avg": nb_ms,
-                     "kernel_ms_min": prof["ms_nonbonded_min"], "flop_per_pair": FLOP_PER_PAIR,
-                     "timed_launches": prof["n_evals"], "events_every": PROFILE_EVERY},
+                     "kernel": "k_forces<cubic> (tiles + bonded workgroups)", "kernel_ms_avg": roof["kernel_ms_avg"],
+                     "kernel_ms_min": roof["kernel_ms_min"], "flop_per_pair": FLOP_PER_PAIR,
+                     "timed_launches": roof["timed_launches"],
+                     "timing": f"dispatch-stamped events on each of {ROOF_LAUNCHES} launches after the timed region"},
         "device_ms_per_eval": ev_ms,
         "ms_to_force_kernel_start": prep_ms,
         "tile_pruning": pruning,
@@ -473,6 +569,8 @@ def main():
                                     "n_work_items")},
         "energy_total": e.total,
     }
+    if not args.no_reference_workloads:
+        out["reference_benchmarks"] = bench_reference_workloads(cpu=not args.no_cpu_baseline)
     if not args.no_250k:
         out["system_250k"] = bench_250k()
     if not args.no_batched:
