@@ -128,9 +128,10 @@ __device__ inline float wave_max(float v) {
 // One atom of the prep pass: quantise, store x, y, z and sigma/2 to the slot,
 // return the displacement from the build position (A).  Every load is by
 // atom index.  A position that is not finite marks the slot (its force is
-// reported as NaN), stores sigma/2 as NaN (the force kernel then marks every
-// listed partner of the atom the same way) and returns an infinite
-// displacement, which turns tile pruning off for the evaluation.
+// reported as NaN), stores sigma/2 as -0.0 (its sign bit tells the force
+// kernel to mark every listed partner of the atom the same way; as a value it
+// keeps the pairs the kernel does not evaluate at exactly zero) and returns an
+// infinite displacement, which turns tile pruning off for the evaluation.
 __device__ inline float prep_atom(const PrepArgs &P, int a) {
     const int s = P.atom_slot[a];
     const double x = P.x[a], y = P.y[a], z = P.z[a];
@@ -139,7 +140,7 @@ __device__ inline float prep_atom(const PrepArgs &P, int a) {
     const int qy = quantize_fixed(y, P.iq[1]);
     const int qz = quantize_fixed(z, P.iq[2]);
     const bool fin = isfinite(x) && isfinite(y) && isfinite(z);
-    P.qpos[s] = make_int4(qx, qy, qz, fin ? __float_as_int(sh) : 0x7fc00000);
+    P.qpos[s] = make_int4(qx, qy, qz, fin ? __float_as_int(sh) : (int)0x80000000);
     if (!fin) {
         P.bad[s] = 1u;
         return INFINITY;
@@ -279,8 +280,8 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     const int4 qown = qpos[(size_t)sc * 64 + lane];
     s_iq[lane] = qown;
     s_ie[lane] = qeps[(size_t)sc * 64 + lane];
-    // an i atom whose position is not finite (sigma/2 stored as NaN)?
-    bool anybad = __ballot(__builtin_isnan(__int_as_float(qown.w))) != 0ull;
+    // an i atom whose position is not finite (sigma/2 stored as -0.0)?
+    bool anybad = __ballot(qown.w < 0) != 0ull;
     float fix[8], fiy[8], fiz[8];
 #pragma unroll
     for (int ci = 0; ci < 8; ci++) { fix[ci] = 0.f; fiy[ci] = 0.f; fiz[ci] = 0.f; }
@@ -341,10 +342,16 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
                         float dy = fixdiff(cq.y, qi.y);
                         float dz = fixdiff(cq.z, qi.z);
                         if (!CUBIC) { dx *= p.sx; dy *= p.sy; dz *= p.sz; }
-                        const float r2 = dx * dx + dy * dy + dz * dz;
+                        const float r2p = dx * dx + dy * dy + dz * dz;
                         // 0 < r2 <= rc2 as one unsigned compare of the bit patterns
                         const bool ok = __builtin_amdgcn_classf(kcls[ci], (int)bits) &&
-                                        ((unsigned)__float_as_int(r2) - 1u) < rc2_bits;
+                                        ((unsigned)__float_as_int(r2p) - 1u) < rc2_bits;
+                        // A pair that is not evaluated continues at r2 = 1e36 (far
+                        // beyond any pair distance in either length unit): there
+                        // (sigma/r)^6 underflows to +0 and u clamps to 1, where
+                        // S = dS/dr = 0, so its force and energy are exactly +0,
+                        // as a select of both would give.
+                        const float r2 = ok ? r2p : 1.0e36f;
                         const float inv_r = __builtin_amdgcn_rsqf(r2);
                         // one Newton step brings rsq^2 (~2.5 ulp) to ~1 ulp, the
                         // accuracy (sigma/r)^12 needs, for two FMAs instead of a rcp
@@ -374,8 +381,8 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
                             dsdr = ds_coef * qu * qu;
                         }
                         const float inner = fmaf(-(lj_a * dsdr), r, lj_b * sw);
-                        const float fr = ok ? e4 * inv_r2 * inner : 0.0f;
-                        const float v = ok ? e4 * lj_a * sw : 0.0f;
+                        const float fr = e4 * inv_r2 * inner;
+                        const float v = e4 * lj_a * sw;
                         fix[ci] = fmaf(-fr, dx, fix[ci]);
                         fiy[ci] = fmaf(-fr, dy, fiy[ci]);
                         fiz[ci] = fmaf(-fr, dz, fiz[ci]);
@@ -389,7 +396,7 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
                 fjy = dpp_sum8(fjy);
                 fjz = dpp_sum8(fjz);
                 e_acc += (double)ee;
-                anybad = anybad || __ballot(__builtin_isnan(shj)) != 0ull;  // a j atom, likewise
+                anybad = anybad || __ballot(cq.w < 0) != 0ull;  // a j atom, likewise
             }
             if (il == 0) {
                 jf[0][k * 8 + jl] = fjx;
@@ -466,12 +473,12 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     // The energy is NaN then as well: the item's first block energy is
     // overwritten (the same wavefront stored it above).
     if (__builtin_expect(anybad, 0)) {
-        const unsigned long long ib = __ballot(__builtin_isnan(__int_as_float(s_iq[lane].w)));
+        const unsigned long long ib = __ballot(s_iq[lane].w < 0);
         bool hit = false;
         for (int kk = 0; kk < nl; kk++) {
             const int2 m = s_meta[kk];
             const size_t js = (size_t)m.x * 8 + jl;
-            const bool jb = __builtin_isnan(__int_as_float(qpos[js].w));
+            const bool jb = qpos[js].w < 0;
             const uint32_t bits = s_lb8[((uint32_t)m.y >> 8) * 64 + lane] & (uint32_t)m.y;
             for (int ci = 0; ci < 8; ci++) {
                 if (!((bits >> ci) & 1u)) continue;
