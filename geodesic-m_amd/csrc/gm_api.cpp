@@ -75,72 +75,56 @@ bool indices_ok(const uint32_t *a, int64_t cnt, int64_t n) {
 
 // Work items: runs of at most `chunk` consecutive list entries of one
 // supercluster (whole fixed-point blocks of kBlock entries, counted from the
-// supercluster's first entry), one wavefront each.  Automatic chunk: about
-// 1.5 work items per resident wavefront slot (5 per SIMD), rounded up to whole
-// blocks, 8 to 32 entries.  The chunk only groups blocks into items: every
-// block's i forces and energy are rounded on their own, so forces and
+// supercluster's first entry), one wavefront each, built on the device
+// (k_sc_scan, k_wi_make, k_cost_scan, k_wi_scatter).  Automatic chunk: about
+// 1.5 work items per resident wavefront slot (5 per SIMD), rounded up to
+// whole blocks, 8 to 32 entries.  The chunk only groups blocks into items:
+// every block's i forces and energy are rounded on their own, so forces and
 // energies do not depend on it.  Items are issued heaviest first (estimated
-// cost: tiles plus per-entry and per-item overhead, in tile units):
-// workgroups are dispatched in index order as slots free up, so this is
-// longest-processing-time-first list scheduling and the light items fill the
-// tail.
-gm_status make_work_items(gm_sys *s) {
-    const std::vector<int> &start = s->sc_start_host;
-    const uint32_t *im = s->imask_host;  // per entry i-cluster mask: popcount = tiles
-    constexpr int kEntryCost2 = 1, kItemCost2 = 8;  // twice the per-entry and per-item cost, in tiles
-    struct Item {
-        int sc, e0, e1, blk0;
-        int cost;  // twice the estimated cost, in tiles
-    };
-    int64_t chunk = s->chunk;
-    if (chunk <= 0) {
-        const int64_t slots = 20LL * (s->ctx->n_cu > 0 ? s->ctx->n_cu : 256);
-        chunk = (2 * s->nent + 3 * slots - 1) / (3 * slots);
-    }
-    chunk = std::min<int64_t>(std::max<int64_t>(kBlock, (chunk + kBlock - 1) / kBlock * kBlock), kMaxRun);
-    if (s->nsc > (int64_t)kWiScMask + 1)
-        return set_error(GM_ERR_CAPACITY, "work items: more than 2^26 superclusters");
-    // blocks: per supercluster ceil(entries / kBlock), consecutive
-    std::vector<int> boff((size_t)s->nsc + 1, 0);
-    for (int64_t k = 0; k < s->nsc; k++)
-        boff[(size_t)k + 1] = boff[(size_t)k] + (start[(size_t)k + 1] - start[(size_t)k] + kBlock - 1) / kBlock;
-    s->n_blk = boff[(size_t)s->nsc];
-    std::vector<Item> items;
-    items.reserve((size_t)(s->nent / chunk + s->nsc + 1));
-    for (int64_t k = 0; k < s->nsc; k++)
-        for (int64_t e0 = start[(size_t)k]; e0 < start[(size_t)k + 1]; e0 += chunk) {
-            const int64_t e1 = std::min<int64_t>(e0 + chunk, start[(size_t)k + 1]);
-            int c = kItemCost2;
-            for (int64_t e = e0; e < e1; e++) c += 2 * __builtin_popcount(im[e] & 0xffu) + kEntryCost2;
-            const int blk0 = boff[(size_t)k] + (int)((e0 - start[(size_t)k]) / kBlock);
-            items.push_back({(int)k, (int)e0, (int)e1, blk0, c});
-        }
-    // stable counting sort by decreasing cost (costs are small integers)
-    int cmax = 0;
-    for (const Item &it : items) cmax = std::max(cmax, it.cost);
-    std::vector<int> first((size_t)cmax + 2, 0);
-    for (const Item &it : items) first[(size_t)(cmax - it.cost) + 1]++;
-    for (size_t c = 1; c < first.size(); c++) first[c] += first[c - 1];
-    const size_t ni = items.size();
-    std::vector<int4> wi(ni);
-    for (const Item &it : items) {
-        const size_t o = (size_t)first[(size_t)(cmax - it.cost)]++;
-        // (supercluster | entries << 26, first entry, first block, j-cluster of the first entry)
-        wi[o] = make_int4(it.sc | ((it.e1 - it.e0) << kWiEntryShift), it.e0, it.blk0, 0);
-    }
-    s->n_wi = (int64_t)wi.size();
-    hipStream_t st = s->stream;
-    GM_HIP(upload(s->wi, wi.data(), wi.size(), st));
-    GM_HIP(upload(s->sc_blk_off, boff.data(), boff.size(), st));
-    // Small work items: many i-force updates per slot, which as atomics (past
-    // the L2) cost the force kernel more than plain per-block records cost the
-    // finish kernel; large ones: few, and atomics are cheaper.
-    s->fi_records = chunk < 16;
+// cost: tiles plus per-entry and per-item overhead): workgroups are
+// dispatched in index order as slots free up, so this is longest-processing-
+// time-first list scheduling and the light items fill the tail.
+//
+// The host learns the counts from the build words the device wrote
+// (meta): work items, blocks, entries per item.
+gm_status finish_work_items(gm_sys *s, const int *m) {
+    s->n_wi = m[kMetaNwi];
+    s->n_blk = m[kMetaNblk];
+    s->fi_records = m[kMetaChunk] < 16;
     if (s->fi_records) GM_HIP(s->fi_blk.ensure((size_t)std::max<int64_t>(s->n_blk, 1) * 192));
-    GM_HIP(launch_work_item_cj0(s));  // .w = ent_cj[first entry], from the device list
     GM_HIP(s->e_blk.ensure((size_t)std::max<int64_t>(s->n_blk, 1)));
-    GM_HIP(hipStreamSynchronize(st));
     s->stats.n_work_items = s->n_wi;
+    return GM_OK;
+}
+
+// New work items for a built list (gm_set_work_chunk, a batch's shape).
+gm_status make_work_items(gm_sys *s) {
+    GM_HIP(launch_sc_scan(s, s->ent_cap, s->chunk));
+    GM_HIP(launch_work_items(s));
+    int m[24];
+    GM_HIP(hipMemcpyAsync(m, s->meta.p, sizeof(m), hipMemcpyDeviceToHost, s->stream));
+    GM_HIP(hipStreamSynchronize(s->stream));
+    return finish_work_items(s, m);
+}
+
+// Entry and work-item buffers for about nent entries (a quarter more, so
+// that later builds of a moving system fit without resizing).
+gm_status size_list_buffers(gm_sys *s, int64_t nent) {
+    const int64_t cap = nent + nent / 4 + 1024;
+    if (cap > (int64_t)0x7fffffff) return set_error(GM_ERR_CAPACITY, "list build: too many entries");
+    GM_HIP(s->ent_sc.ensure((size_t)cap));
+    GM_HIP(s->ent_cj.ensure((size_t)cap));
+    GM_HIP(s->ent_imask.ensure((size_t)cap));
+    GM_HIP(s->ent_mask.ensure((size_t)cap * 8));
+    GM_HIP(s->ent_meta.ensure((size_t)cap));
+    GM_HIP(s->ent_dmin.ensure((size_t)cap));
+    GM_HIP(s->ent_lb.ensure((size_t)cap * 64));
+    const int64_t icap = cap / kBlock + s->max_sc + 1;  // items of at least kBlock entries
+    GM_HIP(s->wi.ensure((size_t)icap));
+    GM_HIP(s->wi_tmp.ensure((size_t)icap));
+    GM_HIP(s->wi_cost.ensure((size_t)icap));
+    s->ent_cap = cap;
+    s->item_cap = icap;
     return GM_OK;
 }
 
@@ -234,6 +218,12 @@ gm_status fit_span_to_atoms(gm_sys *s) {
     return GM_OK;
 }
 
+// The list build on the device: binning, sort, split, pair search, scans,
+// pair masks and work items, with one read-back at the end (the build words
+// and counts).  The first build of a system also reads the entry count after
+// the pair search, to size the entry buffers; later builds use that size,
+// and a build whose entries overflow it (the kernels then skip the rest)
+// sizes the buffers anew and runs again.
 gm_status build_on_device(gm_sys *s, bool wrap) {
     hipStream_t st = s->stream;
     s->built = false;
@@ -243,80 +233,68 @@ gm_status build_on_device(gm_sys *s, bool wrap) {
         if (rc != GM_OK) return rc;
     }
     GM_HIP(launch_build(s, wrap));
-    int meta[4] = {0, 0, 0, 0};
-    GM_HIP(hipMemcpyAsync(meta, s->meta.p, sizeof(meta), hipMemcpyDeviceToHost, st));
-    GM_HIP(hipStreamSynchronize(st));
-    if (meta[1])
-        return set_error(GM_ERR_CAPACITY, "list build: an xy column holds " + std::to_string(meta[2]) +
+    GM_HIP(launch_pair_count(s));
+    int m[24];
+    auto column_overflow = [&]() {
+        return set_error(GM_ERR_CAPACITY, "list build: an xy column holds " + std::to_string(m[2]) +
                                               " atoms, more than the LDS sort capacity " +
                                               std::to_string(kColumnCap));
-    s->nsc = meta[0];
-    s->ncl = s->nsc * kClustersPerSc;
-    if (s->nsc > s->max_sc) return set_error(GM_ERR_KERNEL, "list build: supercluster count overflow");
-
-    std::vector<int> cnt((size_t)s->nsc), start((size_t)s->nsc + 1, 0);
-    if (s->nsc > 0) {
-        GM_HIP(launch_pair_count(s));
-        GM_HIP(hipMemcpyAsync(cnt.data(), s->sc_count.p, sizeof(int) * cnt.size(),
-                              hipMemcpyDeviceToHost, st));
+    };
+    if (s->ent_cap == 0) {
+        GM_HIP(launch_sc_scan(s, 0x7fffffff, s->chunk));
+        GM_HIP(hipMemcpyAsync(m, s->meta.p, sizeof(m), hipMemcpyDeviceToHost, st));
         GM_HIP(hipStreamSynchronize(st));
+        if (m[1]) return column_overflow();
+        gm_status rc = size_list_buffers(s, m[kMetaNent]);
+        if (rc != GM_OK) return rc;
     }
-    int64_t total = 0;
-    for (int64_t k = 0; k < s->nsc; k++) {
-        start[(size_t)k] = (int)total;
-        total += cnt[(size_t)k];
-    }
-    start[(size_t)s->nsc] = (int)total;
-    if (total > (int64_t)0x7fffffff) return set_error(GM_ERR_CAPACITY, "list build: too many entries");
-    s->nent = total;
-    GM_HIP(s->ent_sc.ensure((size_t)total));
-    GM_HIP(s->ent_cj.ensure((size_t)total));
-    GM_HIP(s->ent_imask.ensure((size_t)total));
-    GM_HIP(s->ent_mask.ensure((size_t)total * 8));
-    GM_HIP(s->ent_meta.ensure((size_t)total));
-    GM_HIP(s->ent_dmin.ensure((size_t)total));
-    GM_HIP(s->ent_lb.ensure((size_t)total * 64));
-    GM_HIP(upload(s->sc_start, start.data(), start.size(), st));
-
-    s->sc_start_host = start;
+    GM_HIP(launch_sc_scan(s, s->ent_cap, s->chunk));
     GM_HIP(launch_pair_fill_and_masks(s));
-    {
-        // tiles per entry, for the cost of each work item (read back into pinned memory)
-        if (s->imask_host_n < (size_t)total) {
-            if (s->imask_host) (void)hipHostFree(s->imask_host);
-            s->imask_host = nullptr;
-            s->imask_host_n = 0;
-            GM_HIP(hipHostMalloc((void **)&s->imask_host, sizeof(uint32_t) * (size_t)std::max<int64_t>(total, 1),
-                                 hipHostMallocDefault));
-            s->imask_host_n = (size_t)std::max<int64_t>(total, 1);
-        }
-        if (total > 0)
-            GM_HIP(hipMemcpyAsync(s->imask_host, s->ent_imask.p, sizeof(uint32_t) * (size_t)total,
-                                  hipMemcpyDeviceToHost, st));
-        GM_HIP(hipStreamSynchronize(st));
-    }
-    gm_status rc = make_work_items(s);
-    if (rc != GM_OK) return rc;
+    GM_HIP(launch_work_items(s));
     // the slot count changed: start the next evaluation from a zero accumulator
     GM_HIP(launch_fill_u64(s->F.p, (size_t)s->max_sc * 64 * 3, st));
-    s->F_dirty = false;
     GM_HIP(hipEventRecord(s->ev_b1, st));
-    unsigned long long counters[4] = {0, 0, 0, 0};
-    GM_HIP(hipMemcpyAsync(counters, s->meta.p + 8, sizeof(counters), hipMemcpyDeviceToHost, st));
+    GM_HIP(hipMemcpyAsync(m, s->meta.p, sizeof(m), hipMemcpyDeviceToHost, st));
     GM_HIP(hipStreamSynchronize(st));
+    if (m[1]) return column_overflow();
+    if (m[kMetaEntOver]) {
+        // more entries than the buffers hold: size them for this list and build
+        // again (the positions are already wrapped; wrapping is idempotent)
+        s->ent_cap = 0;
+        return build_on_device(s, false);
+    }
+    s->nsc = m[0];
+    s->ncl = s->nsc * kClustersPerSc;
+    s->nent = m[kMetaNent];
+    if (s->nsc > s->max_sc) return set_error(GM_ERR_KERNEL, "list build: supercluster count overflow");
+    gm_status rc = finish_work_items(s, m);
+    if (rc != GM_OK) return rc;
+    s->F_dirty = false;
+    s->stats_pending = true;  // pair and tile counts: summed when asked for (gm_get_stats)
     float ms = 0.f;
     GM_HIP(hipEventElapsedTime(&ms, s->ev_b0, s->ev_b1));
     s->stats.n_atoms = s->n;
     s->stats.n_superclusters = s->nsc;
     s->stats.n_clusters = s->ncl;
     s->stats.n_entries = s->nent;
+    s->stats.ms_build = ms;
+    s->built = true;
+    return GM_OK;
+}
+
+// The list statistics of the last build (pairs, tiles, pairs and tiles within
+// r_cutoff), summed from the pair-mask kernel's partial counts on demand.
+gm_status list_stats(gm_sys *s) {
+    if (!s->stats_pending) return GM_OK;
+    GM_HIP(launch_list_stats(s));
+    unsigned long long counters[4] = {0, 0, 0, 0};
+    GM_HIP(hipMemcpyAsync(counters, s->meta.p + 8, sizeof(counters), hipMemcpyDeviceToHost, s->stream));
+    GM_HIP(hipStreamSynchronize(s->stream));
     s->stats.n_list_pairs = (int64_t)counters[0];
     s->stats.n_tiles = (int64_t)counters[1];
     s->stats.n_tiles_cutoff = (int64_t)counters[2];
     s->stats.n_pairs_cutoff = (int64_t)counters[3];
-    s->stats.n_work_items = s->n_wi;
-    s->stats.ms_build = ms;
-    s->built = true;
+    s->stats_pending = false;
     return GM_OK;
 }
 
@@ -664,7 +642,11 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     TRY(s->bucket.ensure(N));
     TRY(s->sorted_atoms.ensure(N));
     TRY(s->atom_slot.ensure(N));
-    for (DevBuf<int> *b : {&s->sc_base, &s->sc_m, &s->sc_col, &s->sc_count}) TRY(b->ensure((size_t)s->max_sc + 1));
+    for (DevBuf<int> *b : {&s->sc_base, &s->sc_m, &s->sc_col, &s->sc_count, &s->sc_start, &s->item_off,
+                           &s->sc_blk_off})
+        TRY(b->ensure((size_t)s->max_sc + 1));
+    TRY(s->cost_bins.ensure(2 * (size_t)kCostBins + 2));
+    if (s->max_sc > (int64_t)kWiScMask + 1) return fail(set_error(GM_ERR_CAPACITY, "more than 2^26 superclusters"));
     TRY(s->meta.ensure(24));
     TRY(s->slot_atom.ensure(slots));
     TRY(s->sq.ensure(slots));
@@ -731,8 +713,10 @@ gm_status gm_system_destroy(gm_sys *s) {
     s->ent_mask.release();
     s->ent_meta.release();
     s->ent_dmin.release();
-    if (s->imask_host) (void)hipHostFree(s->imask_host);
-    s->imask_host = nullptr;
+    s->item_off.release();
+    s->wi_tmp.release();
+    s->wi_cost.release();
+    s->cost_bins.release();
     s->qpos.release();
     s->qeps.release();
     s->disp.release();
@@ -910,6 +894,8 @@ gm_status gm_get_pair_list(gm_sys *s, uint32_t *pi, uint32_t *pj, int64_t cap, i
                               hipMemcpyDeviceToHost, st));
     }
     GM_HIP(hipStreamSynchronize(st));
+    gm_status rs = list_stats(s);
+    if (rs != GM_OK) return rs;
     std::vector<uint64_t> pairs;
     pairs.reserve((size_t)s->stats.n_list_pairs);
     for (size_t e = 0; e < ne; e++) {
@@ -991,6 +977,8 @@ gm_status gm_set_tile_pruning(gm_sys *s, int32_t enable) {
 
 gm_status gm_get_stats(gm_sys *s, gm_stats *out) {
     GM_REQUIRE(s && out, "NULL argument");
+    gm_status rc = list_stats(s);
+    if (rc != GM_OK) return rc;
     *out = s->stats;
     return GM_OK;
 }

@@ -377,11 +377,11 @@ __global__ __launch_bounds__(64) void k_pair_search(const int *meta, BuildParams
                                                    const int4 *cl_ref, const float4 *cl_half,
                                                    const int4 *sc_ref, const float4 *sc_half,
                                                    int *sc_count, const int *sc_start, int *ent_sc,
-                                                   int *ent_cj, uint32_t *ent_imask) {
+                                                   int *ent_cj, uint32_t *ent_imask, int fill) {
     __shared__ int4 cir[8];
     __shared__ float4 cih[8];
     const int sc = blockIdx.x;
-    if (meta[1] || sc >= meta[0]) return;
+    if (meta[1] || sc >= meta[0] || (fill && meta[kMetaEntOver])) return;
     const int t = threadIdx.x;
     if (t < 8) { cir[t] = cl_ref[sc * 8 + t]; cih[t] = cl_half[sc * 8 + t]; }
     __syncthreads();
@@ -462,7 +462,7 @@ __device__ inline bool excluded(const int64_t *off, const uint32_t *nbr, uint32_
 // a tile without listed pairs) goes to ent_dmin, d_lo a lower bound on the
 // distance of the tile's closest listed pair (the f32 estimate less a
 // relative 2e-6 and 1e-5 A, far above its error).
-__global__ __launch_bounds__(256) void k_masks(int nent, const int *ent_sc, const int *ent_cj,
+__global__ __launch_bounds__(256) void k_masks(const int *meta, const int *ent_sc, const int *ent_cj,
                                                uint32_t *ent_imask, unsigned long long *ent_mask,
                                                int2 *ent_meta, uint2 *ent_dmin, uint8_t *ent_lb,
                                                const int *slot_atom, const int4 *sq, const double *x,
@@ -473,6 +473,8 @@ __global__ __launch_bounds__(256) void k_masks(int nent, const int *ent_sc, cons
                                                unsigned long long *parts) {
     const int t = threadIdx.x & 63;
     const int il = t & 7, jl = t >> 3;
+    // the entries the pair search wrote (none when they overflowed the buffers)
+    const int nent = (meta[1] || meta[kMetaEntOver]) ? 0 : meta[kMetaNent];
     // Workgroups are dealt to the 8 XCDs in turn (workgroup g runs on XCD
     // g % 8); group xg of the P = min(8, grid) residues walks the contiguous
     // entry range [xg n / P, (xg + 1) n / P), so an XCD's L2 holds the slot data
@@ -662,6 +664,124 @@ __global__ void k_rebuild_words(int *flag, const int *bad_first) {
     flag[2] = *bad_first;
 }
 
+// Exclusive scan of n ints by one 1024-thread block (in place of the host
+// loop): out[k] = sum of in[0..k), out[n] = total, returned to every thread.
+// f(k) gives the k-th input (so a count can be transformed on the fly).
+template <typename F>
+__device__ inline int block_exclusive_scan(int n, F f, int *out, int *sh) {
+    const int t = threadIdx.x;
+    int carry = 0;
+    for (int base = 0; base < n; base += 1024) {
+        const int k = base + t;
+        const int v = k < n ? f(k) : 0;
+        // inclusive scan within the block: wavefront scan by DPP-free shuffles, then the 16 wave totals
+        int x = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if ((t & 63) >= o) x += y;
+        }
+        if ((t & 63) == 63) sh[t >> 6] = x;
+        __syncthreads();
+        if (t < 16) {
+            int w = sh[t];
+            for (int o = 1; o < 16; o <<= 1) {
+                const int y = __shfl_up(w, o, 16);
+                if ((t & 15) >= o) w += y;
+            }
+            sh[16 + t] = w;  // inclusive over waves
+        }
+        __syncthreads();
+        const int wave_base = (t >> 6) ? sh[16 + (t >> 6) - 1] : 0;
+        if (k < n) out[k] = carry + wave_base + x - v;
+        carry += sh[16 + 15];
+        __syncthreads();  // sh is rewritten by the next tile
+    }
+    if (t == 0) out[n] = carry;
+    return carry;
+}
+
+// One block: entry offsets per supercluster (sc_start, exclusive scan of the
+// pair-search counts), the total entry count and whether it exceeds the entry
+// buffers' capacity, the work shape (entries per work item: chunk_req, or for
+// 0 about 1.5 items per resident wavefront slot, in whole blocks of kBlock,
+// 8..kMaxRun), and per supercluster its first work item and first block.
+// Also clears the bins of the work-item sort.
+__global__ __launch_bounds__(1024) void k_sc_scan(int *meta, const int *sc_count, int *sc_start, int *item_off,
+                                                  int *blk_off, int ent_cap, int chunk_req, int slots,
+                                                  int *bins) {
+    __shared__ int sh[32];
+    __shared__ int s_chunk;
+    const int t = threadIdx.x;
+    for (int k = t; k < 2 * kCostBins; k += 1024) bins[k] = 0;
+    const int nsc = meta[1] ? 0 : meta[0];
+    const int nent = block_exclusive_scan(nsc, [&](int k) { return sc_count[k]; }, sc_start, sh);
+    if (t == 0) {
+        meta[kMetaNent] = nent;
+        meta[kMetaEntOver] = nent > ent_cap ? 1 : 0;
+        int c = chunk_req;
+        if (c <= 0) c = (int)((2LL * nent + 3LL * slots - 1) / (3LL * slots));
+        c = min(max(kBlock, (c + kBlock - 1) / kBlock * kBlock), kMaxRun);
+        s_chunk = c;
+        meta[kMetaChunk] = c;
+    }
+    __syncthreads();
+    const int c = s_chunk;
+    const int nwi = block_exclusive_scan(nsc, [&](int k) { return (sc_count[k] + c - 1) / c; }, item_off, sh);
+    const int nblk = block_exclusive_scan(nsc, [&](int k) { return (sc_count[k] + kBlock - 1) / kBlock; }, blk_off, sh);
+    if (t == 0) {
+        meta[kMetaNwi] = nwi;
+        meta[kMetaNblk] = nblk;
+    }
+}
+
+// One thread per work item: a run of `chunk` consecutive entries of one
+// supercluster (whole blocks, counted from its first entry), with its first
+// block and the j-cluster of its first entry, and its estimated cost in
+// half-tile units (tiles plus 0.5 per entry plus 4 per item).
+__global__ void k_wi_make(const int *meta, const int *sc_start, const int *item_off, const int *blk_off,
+                          const uint32_t *ent_imask, const int *ent_cj, int4 *wi_tmp, int *cost,
+                          int *bins) {
+    const int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (meta[kMetaEntOver] || it >= meta[kMetaNwi]) return;
+    const int nsc = meta[1] ? 0 : meta[0];
+    const int chunk = meta[kMetaChunk];
+    int lo = 0, hi = nsc - 1;  // the last supercluster whose first item is <= it
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (item_off[mid] <= it) lo = mid; else hi = mid - 1;
+    }
+    const int sc = lo, k = it - item_off[sc];
+    const int e0 = sc_start[sc] + k * chunk, e1 = min(e0 + chunk, sc_start[sc + 1]);
+    int c = 8;
+    for (int e = e0; e < e1; e++) c += 2 * __popc(ent_imask[e] & 0xffu) + 1;
+    wi_tmp[it] = make_int4(sc | ((e1 - e0) << kWiEntryShift), e0, blk_off[sc] + k * (chunk / kBlock), ent_cj[e0]);
+    cost[it] = c;
+    atomicAdd(&bins[c], 1);
+}
+
+// One block: bin offsets in decreasing cost order (heaviest items first).
+__global__ __launch_bounds__(1024) void k_cost_scan(int *bins) {
+    __shared__ int sh[32];
+    // bins[0..kCostBins) counts -> bins[kCostBins + b] = items costlier than b
+    block_exclusive_scan(kCostBins, [&](int k) { return bins[kCostBins - 1 - k]; }, bins + kCostBins, sh);
+    __syncthreads();
+    // reversed offsets: start of cost b = bins[kCostBins + (kCostBins - 1 - b)]; keep starts in bins[0..)
+    const int t = threadIdx.x;
+    int v[(kCostBins + 1023) / 1024];
+    for (int r = 0; r * 1024 + t < kCostBins; r++) v[r] = bins[kCostBins + (kCostBins - 1 - (r * 1024 + t))];
+    __syncthreads();
+    for (int r = 0; r * 1024 + t < kCostBins; r++) bins[r * 1024 + t] = v[r];
+}
+
+// Items to their place in the heaviest-first order (the order within one cost
+// is whatever the atomics give: results do not depend on the item order).
+__global__ void k_wi_scatter(const int *meta, const int4 *wi_tmp, const int *cost, int *bins, int4 *wi) {
+    const int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (meta[kMetaEntOver] || it >= meta[kMetaNwi]) return;
+    const int pos = atomicAdd(&bins[cost[it]], 1);
+    wi[pos] = wi_tmp[it];
+}
+
 // neighbor.rs:60-72, same f64 expression; any thread past the threshold sets
 // flag[0].  A position that is not finite (never past the threshold, as in
 // the reference) is raised into flag[1] (atomic minimum of the atom index):
@@ -789,32 +909,58 @@ hipError_t launch_build(gm_sys *s, bool wrap) {
     return hipGetLastError();
 }
 
+// Grids of the list kernels cover the capacities (the supercluster bound
+// max_sc, the entry buffers' ent_cap); each kernel reads the actual counts
+// from meta and leaves the rest alone, so the build needs no host round trip
+// between its kernels.
 hipError_t launch_pair_count(gm_sys *s) {
-    if (s->nsc == 0) return hipSuccess;
-    k_pair_search<0><<<(unsigned)s->nsc, 64, 0, s->stream>>>(
+    if (s->max_sc == 0) return hipSuccess;
+    k_pair_search<0><<<(unsigned)s->max_sc, 64, 0, s->stream>>>(
         s->meta.p, s->bp, s->sc_col.p, s->col_sc.p, s->col_count.p, s->cl_ref.p, s->cl_half.p,
-        s->sc_ref.p, s->sc_half.p, s->sc_count.p, nullptr, nullptr, nullptr, nullptr);
+        s->sc_ref.p, s->sc_half.p, s->sc_count.p, nullptr, nullptr, nullptr, nullptr, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_sc_scan(gm_sys *s, int64_t ent_cap, int chunk_req) {
+    const int slots = 20 * (s->ctx->n_cu > 0 ? s->ctx->n_cu : 256);
+    k_sc_scan<<<1, 1024, 0, s->stream>>>(s->meta.p, s->sc_count.p, s->sc_start.p, s->item_off.p, s->sc_blk_off.p,
+                                         (int)std::min<int64_t>(ent_cap, 0x7fffffff), chunk_req, slots,
+                                         s->cost_bins.p);
     return hipGetLastError();
 }
 
 hipError_t launch_pair_fill_and_masks(gm_sys *s) {
     hipStream_t st = s->stream;
-    if (s->nsc > 0)
-        k_pair_search<1><<<(unsigned)s->nsc, 64, 0, st>>>(
+    if (s->max_sc > 0)
+        k_pair_search<1><<<(unsigned)s->max_sc, 64, 0, st>>>(
             s->meta.p, s->bp, s->sc_col.p, s->col_sc.p, s->col_count.p, s->cl_ref.p, s->cl_half.p,
             s->sc_ref.p, s->sc_half.p, nullptr, s->sc_start.p, s->ent_sc.p, s->ent_cj.p,
-            s->ent_imask.p);
-    unsigned long long *counters = (unsigned long long *)(s->meta.p + 8);
-    if (s->nent == 0) return launch_fill_u64(counters, 4, st);
-    const int blocks = (int)std::min<int64_t>((s->nent + 3) / 4, kMaskBlocks);
+            s->ent_imask.p, 1);
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((s->ent_cap + 3) / 4, kMaskBlocks));
     hipError_t e = s->mask_parts.ensure((size_t)blocks * 4 * 4);  // 4 counts per wavefront
     if (e != hipSuccess) return e;
-    k_masks<<<(unsigned)blocks, 256, 0, st>>>((int)s->nent, s->ent_sc.p, s->ent_cj.p, s->ent_imask.p,
+    s->mask_waves = blocks * 4;
+    k_masks<<<(unsigned)blocks, 256, 0, st>>>(s->meta.p, s->ent_sc.p, s->ent_cj.p, s->ent_imask.p,
                                               s->ent_mask.p, s->ent_meta.p, s->ent_dmin.p, s->ent_lb.p,
                                               s->slot_atom.p, s->sq.p, s->x.p, s->y.p, s->z.p,
                                               s->excl_off.p, s->excl_nbr.p, s->excl_rng.p, s->sxb.p,
                                               s->bp, s->mask_parts.p);
-    k_sum_parts<<<1, 1024, 0, st>>>(s->mask_parts.p, blocks * 4, counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_work_items(gm_sys *s) {
+    hipStream_t st = s->stream;
+    const unsigned g = (unsigned)std::max<int64_t>(1, (s->item_cap + 255) / 256);
+    k_wi_make<<<g, 256, 0, st>>>(s->meta.p, s->sc_start.p, s->item_off.p, s->sc_blk_off.p, s->ent_imask.p,
+                                 s->ent_cj.p, s->wi_tmp.p, s->wi_cost.p, s->cost_bins.p);
+    k_cost_scan<<<1, 1024, 0, st>>>(s->cost_bins.p);
+    k_wi_scatter<<<g, 256, 0, st>>>(s->meta.p, s->wi_tmp.p, s->wi_cost.p, s->cost_bins.p, s->wi.p);
+    return hipGetLastError();
+}
+
+hipError_t launch_list_stats(gm_sys *s) {
+    unsigned long long *counters = (unsigned long long *)(s->meta.p + 8);
+    k_sum_parts<<<1, 1024, 0, s->stream>>>(s->mask_parts.p, s->mask_waves, counters);
     return hipGetLastError();
 }
 

@@ -41,6 +41,11 @@ static_assert(kMaxRun < (1 << (32 - kWiEntryShift)), "entry count must fit above
 constexpr int kMaskBlocks = 8192;
 // list build: column histograms in LDS (32 KB) up to this many xy columns
 constexpr int kLdsColumns = 8192;
+// list build words in gm_sys::meta (ints): entries, entry-capacity overflow,
+// entries per work item, work items, blocks
+constexpr int kMetaNent = 16, kMetaEntOver = 17, kMetaChunk = 18, kMetaNwi = 19, kMetaNblk = 20;
+// work-item costs (half tiles) lie in [8, 8 + kMaxRun * 17]: one sort bin each
+constexpr int kCostBins = 8 + kMaxRun * 17 + 1;
 // Fixed-point force unit: 2^-32 kcal/mol/A per integer step.
 constexpr double kFixedScale = 4294967296.0;
 constexpr double kFixedInv = 1.0 / 4294967296.0;
@@ -286,9 +291,15 @@ struct gm_sys {
     bool has_shell = false;   // nbp_shell in use: rebuilt from nbp whenever the span changes
     double shell_rsw = 0, shell_w = 0;  // inner switch start and width of the shell mode
     struct gm_dyn *dyn = nullptr;  // masses, constraints and dynamics buffers (gm_dynamics_set)
-    std::vector<int> sc_start_host;
-    uint32_t *imask_host = nullptr;       // pinned readback of the entries' tile masks (work item costs)
-    size_t imask_host_n = 0;
+    // list build on the device: per-supercluster scans, work items, their sort
+    DevBuf<int> item_off;                 // per supercluster, its first work item (CSR offsets)
+    DevBuf<int4> wi_tmp;                  // work items in supercluster order (before the sort)
+    DevBuf<int> wi_cost;                  // their estimated costs
+    DevBuf<int> cost_bins;                // sort bins: counts, then offsets (heaviest first)
+    int64_t ent_cap = 0;                  // entries the entry buffers hold (0: size them at the next build)
+    int64_t item_cap = 0;                 // work items the work-item buffers hold
+    int mask_waves = 0;                   // pair-mask wavefronts of the last build (their partial counts)
+    bool stats_pending = false;           // list statistics not yet summed (gm_get_stats sums them)
 
     gm::EvalBufs ev{};  // the buffers the launchers use (own_bufs unless a caller gives others)
     bool built = false;
@@ -355,7 +366,14 @@ hipError_t launch_build(gm_sys *s, bool wrap);
 // out[0..2] min x y z, out[3..5] max x y z, out[6] 1 if a coordinate is not finite
 hipError_t launch_extent(gm_sys *s, double *out);
 hipError_t launch_pair_count(gm_sys *s);
+// entry offsets, entry count (and capacity overflow), work shape (chunk_req
+// entries per item, 0 automatic), work-item and block offsets: into meta
+hipError_t launch_sc_scan(gm_sys *s, int64_t ent_cap, int chunk_req);
 hipError_t launch_pair_fill_and_masks(gm_sys *s);
+// heaviest-first work items from the scanned list (after launch_sc_scan)
+hipError_t launch_work_items(gm_sys *s);
+// list statistics (pairs, tiles, within r_cutoff) into meta[8..15] from the pair-mask partials
+hipError_t launch_list_stats(gm_sys *s);
 // flag: where the answer goes (null: the system's own words, meta[5..7]):
 // flag[0] rebuild needed, flag[1] the first atom whose position is not finite
 // (INT_MAX if none), flag[2] the first atom whose force was not finite in the
@@ -365,8 +383,6 @@ hipError_t launch_needs_rebuild(gm_sys *s, const double *x, const double *y, con
 // (shell: tiles only, with nbp_shell)
 // (after launch_prep: slot-order fixed-point positions and the displacement bound)
 hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double *z, hipEvent_t ev_start = nullptr);
-// fills the work items' first j-cluster from the list entries
-hipError_t launch_work_item_cj0(gm_sys *s);
 // ev_start / ev_stop (optional): stamped by the dispatch at the kernel's start and end
 hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z, bool shell = false,
                          hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);

@@ -493,12 +493,6 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
     __syncthreads();  // LDS is reused by the next work item of a persistent wave
 }
 
-// Work-item descriptors: the j-cluster of each item's first entry.
-__global__ void k_wi_cj0(int n, int4 *wi, const int *ent_cj) {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w < n) wi[w].w = ent_cj[wi[w].y];
-}
-
 template <bool CUBIC, bool SHELL>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_forces(TileArgs A,
                                                                                        TermArgs ta) {
@@ -663,12 +657,6 @@ hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const doub
         if (e == hipSuccess && ev_stop) e = hipEventRecord(ev_stop, s->stream);
         if (e != hipSuccess) return e;
     }
-    return hipGetLastError();
-}
-
-hipError_t launch_work_item_cj0(gm_sys *s) {
-    if (s->n_wi > 0)
-        k_wi_cj0<<<(unsigned)((s->n_wi + 255) / 256), 256, 0, s->stream>>>((int)s->n_wi, s->wi.p, s->ent_cj.p);
     return hipGetLastError();
 }
 
