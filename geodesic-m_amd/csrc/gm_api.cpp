@@ -115,7 +115,6 @@ gm_status size_list_buffers(gm_sys *s, int64_t nent) {
     GM_HIP(s->ent_sc.ensure((size_t)cap));
     GM_HIP(s->ent_cj.ensure((size_t)cap));
     GM_HIP(s->ent_imask.ensure((size_t)cap));
-    GM_HIP(s->ent_mask.ensure((size_t)cap * 8));
     GM_HIP(s->ent_meta.ensure((size_t)cap));
     GM_HIP(s->ent_dmin.ensure((size_t)cap));
     GM_HIP(s->ent_lb.ensure((size_t)cap * 64));
@@ -193,9 +192,9 @@ void set_quant_span(gm_sys *s, const double L[3], int ncx, int ncy) {
 // (r_skin - r_cutoff) / 2) are half a span apart: the wrapped 32-bit
 // difference is the plain difference.  When that cube does not fit in the
 // smallest box edge, or a position is not finite, the span is the box.
-gm_status fit_span_to_atoms(gm_sys *s) {
+gm_status fit_span_to_atoms(gm_sys *s, const double *px, const double *py, const double *pz) {
     double ext[8];
-    GM_HIP(launch_extent(s, s->ext.p));
+    GM_HIP(launch_extent(s, px, py, pz, s->ext.p));
     GM_HIP(hipMemcpyAsync(ext, s->ext.p, sizeof(ext), hipMemcpyDeviceToHost, s->stream));
     GM_HIP(hipStreamSynchronize(s->stream));
     const double e[3] = {ext[3] - ext[0], ext[4] - ext[1], ext[5] - ext[2]};
@@ -218,48 +217,107 @@ gm_status fit_span_to_atoms(gm_sys *s) {
     return GM_OK;
 }
 
+// The kernels of a build whose buffers are sized (all but the first build of
+// a system), in stream order.
+hipError_t launch_sized_build(gm_sys *s, bool wrap, double *px, double *py, double *pz, bool clear_cols) {
+    hipError_t e;
+    if ((e = launch_build(s, wrap, px, py, pz, clear_cols)) != hipSuccess) return e;
+    if ((e = launch_pair_count(s, s->ent_cap, s->chunk)) != hipSuccess) return e;
+    if ((e = launch_pair_fill_and_masks(s)) != hipSuccess) return e;
+    return launch_work_items(s);
+}
+
+// What the captured build depends on: its launch arguments (positions,
+// buffers that are resized, the work chunk, the span parameters).
+std::vector<unsigned char> build_graph_key(gm_sys *s, bool wrap, double *px, double *py, double *pz) {
+    const void *ptrs[] = {px, py, pz, s->ent_sc.p, s->ent_cj.p, s->ent_imask.p, s->ent_meta.p, s->ent_dmin.p,
+                          s->ent_lb.p, s->wi.p, s->wi_tmp.p, s->wi_cost.p};
+    const int64_t ints[] = {wrap ? 1 : 0, s->chunk, s->ent_cap, s->item_cap, s->n, s->max_sc};
+    std::vector<unsigned char> k(sizeof(ptrs) + sizeof(ints) + sizeof(BuildParams));
+    std::memcpy(k.data(), ptrs, sizeof(ptrs));
+    std::memcpy(k.data() + sizeof(ptrs), ints, sizeof(ints));
+    std::memcpy(k.data() + sizeof(ptrs) + sizeof(ints), &s->bp, sizeof(BuildParams));
+    return k;
+}
+
+// A sized build of a system with a fixed span (periodic) runs as one captured
+// graph (the same kernels with the same arguments as the stream launches),
+// captured again when anything it depends on changes: the kernels go to the
+// device at once instead of one host launch each.
+gm_status launch_build_graph(gm_sys *s, bool wrap, double *px, double *py, double *pz) {
+    std::vector<unsigned char> key = build_graph_key(s, wrap, px, py, pz);
+    if (!s->bexec || key != s->bkey) {
+        if (s->bexec) (void)hipGraphExecDestroy(s->bexec);
+        if (s->bgraph) (void)hipGraphDestroy(s->bgraph);
+        s->bexec = nullptr;
+        s->bgraph = nullptr;
+        s->bkey.clear();
+        GM_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t e1 = launch_sized_build(s, wrap, px, py, pz, false);
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(s->stream, &g);
+        if (e1 != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return from_hip(e1 != hipSuccess ? e1 : e2, "list build graph capture");
+        }
+        s->bgraph = g;
+        GM_HIP(hipGraphInstantiate(&s->bexec, s->bgraph, nullptr, nullptr, 0));
+        s->bkey = std::move(key);
+    }
+    GM_HIP(hipGraphLaunch(s->bexec, s->stream));
+    return GM_OK;
+}
+
 // The list build on the device: binning, sort, split, pair search, scans,
 // pair masks and work items, with one read-back at the end (the build words
 // and counts).  The first build of a system also reads the entry count after
 // the pair search, to size the entry buffers; later builds use that size,
 // and a build whose entries overflow it (the kernels then skip the rest)
 // sizes the buffers anew and runs again.
-gm_status build_on_device(gm_sys *s, bool wrap) {
+gm_status build_on_device(gm_sys *s, bool wrap, double *px, double *py, double *pz) {
     hipStream_t st = s->stream;
     s->built = false;
-    GM_HIP(hipEventRecord(s->ev_b0, st));
-    if (s->fit_span && s->n > 0) {
-        gm_status rc = fit_span_to_atoms(s);
-        if (rc != GM_OK) return rc;
-    }
-    GM_HIP(launch_build(s, wrap));
-    GM_HIP(launch_pair_count(s));
+    const bool clean = s->cols_clean;
+    s->cols_clean = false;
+    if (!px) { px = s->x.p; py = s->y.p; pz = s->z.p; }
     int m[24];
     auto column_overflow = [&]() {
         return set_error(GM_ERR_CAPACITY, "list build: an xy column holds " + std::to_string(m[2]) +
                                               " atoms, more than the LDS sort capacity " +
                                               std::to_string(kColumnCap));
     };
-    if (s->ent_cap == 0) {
-        GM_HIP(launch_sc_scan(s, 0x7fffffff, s->chunk));
-        GM_HIP(hipMemcpyAsync(m, s->meta.p, sizeof(m), hipMemcpyDeviceToHost, st));
-        GM_HIP(hipStreamSynchronize(st));
-        if (m[1]) return column_overflow();
-        gm_status rc = size_list_buffers(s, m[kMetaNent]);
+    GM_HIP(hipEventRecord(s->ev_b0, st));
+    if (s->ent_cap > 0 && clean && !s->fit_span && !s->batch_owned) {
+        gm_status rc = launch_build_graph(s, wrap, px, py, pz);
         if (rc != GM_OK) return rc;
+    } else {
+        if (s->fit_span && s->n > 0) {
+            gm_status rc = fit_span_to_atoms(s, px, py, pz);
+            if (rc != GM_OK) return rc;
+        }
+        if (s->ent_cap == 0) {
+            GM_HIP(launch_build(s, wrap, px, py, pz, !clean));
+            GM_HIP(launch_pair_count(s, 0x7fffffff, s->chunk));
+            GM_HIP(hipMemcpyAsync(m, s->meta.p, sizeof(m), hipMemcpyDeviceToHost, st));
+            GM_HIP(hipStreamSynchronize(st));
+            if (m[1]) return column_overflow();
+            gm_status rc = size_list_buffers(s, m[kMetaNent]);
+            if (rc != GM_OK) return rc;
+            GM_HIP(launch_sc_scan(s, s->ent_cap, s->chunk));
+            GM_HIP(launch_pair_fill_and_masks(s));
+            GM_HIP(launch_work_items(s));
+        } else {
+            GM_HIP(launch_sized_build(s, wrap, px, py, pz, !clean));
+        }
     }
-    GM_HIP(launch_sc_scan(s, s->ent_cap, s->chunk));
-    GM_HIP(launch_pair_fill_and_masks(s));
-    GM_HIP(launch_work_items(s));
-    // the slot count changed: start the next evaluation from a zero accumulator
-    GM_HIP(launch_fill_u64(s->F.p, (size_t)s->max_sc * 64 * 3, st));
+    s->cols_clean = true;  // the pair-mask kernel is on the stream: it clears them
     GM_HIP(hipEventRecord(s->ev_b1, st));
     GM_HIP(hipMemcpyAsync(m, s->meta.p, sizeof(m), hipMemcpyDeviceToHost, st));
     GM_HIP(hipStreamSynchronize(st));
     if (m[1]) return column_overflow();
     if (m[kMetaEntOver]) {
         // more entries than the buffers hold: size them for this list and build
-        // again (the positions are already wrapped; wrapping is idempotent)
+        // again from the system's copy of the (already wrapped) positions
         s->ent_cap = 0;
         return build_on_device(s, false);
     }
@@ -612,9 +670,11 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     TRY(upload(s->excl_nbr, nbr.data(), nbr.size(), st));
     TRY(upload(s->excl_rng, rng.data(), rng.size(), st));
     // per atom, a bit mask of its partners i+1 .. i+64 (bit k: partner i+1+k)
-    std::vector<uint2> bits((size_t)N, make_uint2(0u, 0u));
+    // and its largest partner when that is farther up (0 if none)
+    std::vector<uint4> bits((size_t)N, make_uint4(0u, 0u, 0u, 0u));
     for (int64_t i = 0; i < n; i++) {
-        uint2 &v = bits[(size_t)i];
+        uint4 &v = bits[(size_t)i];
+        if (rng[(size_t)i].y > (uint64_t)i + 64u) v.z = rng[(size_t)i].y;
         for (int64_t k = off[(size_t)i]; k < off[(size_t)i + 1]; k++) {
             const uint64_t d = (uint64_t)nbr[(size_t)k] - (uint64_t)i - 1u;
             if (d < 32u) v.x |= 1u << d;
@@ -636,8 +696,8 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
                               &s->sx, &s->sy, &s->sz})
         TRY(b->ensure(N));
     TRY(s->qatom.ensure(3 * N));
-    for (DevBuf<int> *b : {&s->col_count, &s->col_start, &s->col_sc, &s->col_fill})
-        TRY(b->ensure((size_t)ncol_max));
+    for (DevBuf<int> *b : {&s->col_count, &s->col_start, &s->col_fill}) TRY(b->ensure((size_t)ncol_max));
+    TRY(s->col_sc.ensure((size_t)ncol_max + 1));
     TRY(s->ext.ensure(8));
     TRY(s->bucket.ensure(N));
     TRY(s->sorted_atoms.ensure(N));
@@ -648,6 +708,7 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     TRY(s->cost_bins.ensure(2 * (size_t)kCostBins + 2));
     if (s->max_sc > (int64_t)kWiScMask + 1) return fail(set_error(GM_ERR_CAPACITY, "more than 2^26 superclusters"));
     TRY(s->meta.ensure(24));
+    TRY(launch_fill_i32(s->meta.p, 24, 0, st));
     TRY(s->slot_atom.ensure(slots));
     TRY(s->sq.ensure(slots));
     TRY(s->sxb.ensure(slots));
@@ -710,9 +771,12 @@ gm_status gm_system_destroy(gm_sys *s) {
     s->sc_ref.release();
     s->cl_half.release();
     s->sc_half.release();
-    s->ent_mask.release();
     s->ent_meta.release();
     s->ent_dmin.release();
+    if (s->bexec) (void)hipGraphExecDestroy(s->bexec);
+    if (s->bgraph) (void)hipGraphDestroy(s->bgraph);
+    s->bexec = nullptr;
+    s->bgraph = nullptr;
     s->item_off.release();
     s->wi_tmp.release();
     s->wi_cost.release();
@@ -757,22 +821,10 @@ gm_status gm_build_neighbor_list_device(gm_sys *s, double *dx, double *dy, doubl
     GM_REQUIRE(!s->batch_owned,
                "gm_build_neighbor_list_device: a batch replica's list is built by gm_batch_build_neighbor_lists");
     GM_REQUIRE(s->n == 0 || (dx && dy && dz), "NULL positions");
-    const size_t bytes = sizeof(double) * (size_t)s->n;
-    hipStream_t st = s->stream;
-    if (s->n > 0) {
-        GM_HIP(hipMemcpyAsync(s->x.p, dx, bytes, hipMemcpyDeviceToDevice, st));
-        GM_HIP(hipMemcpyAsync(s->y.p, dy, bytes, hipMemcpyDeviceToDevice, st));
-        GM_HIP(hipMemcpyAsync(s->z.p, dz, bytes, hipMemcpyDeviceToDevice, st));
-    }
-    gm_status rc = build_on_device(s, s->periodic != 0);
-    if (rc != GM_OK) return rc;
-    if (s->periodic && s->n > 0) {
-        GM_HIP(hipMemcpyAsync(dx, s->x.p, bytes, hipMemcpyDeviceToDevice, st));
-        GM_HIP(hipMemcpyAsync(dy, s->y.p, bytes, hipMemcpyDeviceToDevice, st));
-        GM_HIP(hipMemcpyAsync(dz, s->z.p, bytes, hipMemcpyDeviceToDevice, st));
-    }
-    GM_HIP(hipStreamSynchronize(st));
-    return GM_OK;
+    // the build reads the caller's arrays, wraps them in place (periodic) and
+    // keeps its own copy; it ends with a synchronisation of the stream
+    if (s->n == 0) return build_on_device(s, s->periodic != 0);
+    return build_on_device(s, s->periodic != 0, dx, dy, dz);
 }
 
 gm_status gm_needs_rebuild(gm_sys *s, const double *x, const double *y, const double *z, int32_t *out) {
@@ -884,14 +936,13 @@ gm_status gm_get_pair_list(gm_sys *s, uint32_t *pi, uint32_t *pj, int64_t cap, i
     const size_t nslots = (size_t)s->nsc * 64, ne = (size_t)s->nent;
     std::vector<int> slot(nslots), esc(ne), ecj(ne);
     std::vector<uint32_t> eim(ne);
-    std::vector<unsigned long long> em(ne * 8);
+    std::vector<uint8_t> lb(ne * 64);
     if (nslots) GM_HIP(hipMemcpyAsync(slot.data(), s->slot_atom.p, nslots * sizeof(int), hipMemcpyDeviceToHost, st));
     if (ne) {
         GM_HIP(hipMemcpyAsync(esc.data(), s->ent_sc.p, ne * sizeof(int), hipMemcpyDeviceToHost, st));
         GM_HIP(hipMemcpyAsync(ecj.data(), s->ent_cj.p, ne * sizeof(int), hipMemcpyDeviceToHost, st));
         GM_HIP(hipMemcpyAsync(eim.data(), s->ent_imask.p, ne * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        GM_HIP(hipMemcpyAsync(em.data(), s->ent_mask.p, ne * 8 * sizeof(unsigned long long),
-                              hipMemcpyDeviceToHost, st));
+        GM_HIP(hipMemcpyAsync(lb.data(), s->ent_lb.p, ne * 64, hipMemcpyDeviceToHost, st));
     }
     GM_HIP(hipStreamSynchronize(st));
     gm_status rs = list_stats(s);
@@ -901,10 +952,8 @@ gm_status gm_get_pair_list(gm_sys *s, uint32_t *pi, uint32_t *pj, int64_t cap, i
     for (size_t e = 0; e < ne; e++) {
         for (int ci = 0; ci < 8; ci++) {
             if (!(eim[e] & (1u << ci))) continue;
-            unsigned long long m = em[e * 8 + ci];
-            while (m) {
-                const int bit = __builtin_ctzll(m);
-                m &= m - 1;
+            for (int bit = 0; bit < 64; bit++) {  // lane = (jl, il): bit ci of its mask byte
+                if (!((lb[e * 64 + (size_t)bit] >> ci) & 1u)) continue;
                 const int il = bit & 7, jl = bit >> 3;
                 const int a = slot[(size_t)esc[e] * 64 + ci * 8 + il];
                 const int b = slot[(size_t)ecj[e] * 8 + jl];

@@ -36,19 +36,6 @@ __global__ void k_fill_u64(unsigned long long *p, size_t n) {
     if (i < n) p[i] = 0ull;
 }
 
-// Minimum over a full wavefront (DPP within rows, permlane swaps across them);
-// every lane receives it.  All 64 lanes must be active.
-__device__ inline int wave_min_i32(int v) {
-    v = min(v, __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
-    v = min(v, __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
-    v = min(v, __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
-    v = min(v, __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false));  // row_ror:8
-    const auto r = __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false);
-    v = min((int)r[0], (int)r[1]);
-    const auto q = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
-    return min((int)q[0], (int)q[1]);
-}
-
 // counter[key] += 1 for every active lane, with one atomic per distinct key
 // in the wavefront (atoms in input order are often spatially coherent, and 64
 // same-address atomics in one instruction serialise).  Returns the counter's
@@ -78,20 +65,24 @@ __device__ inline int aggregated_fetch_add(int *counter, int key, bool active) {
 
 // neighbor.rs:13-15 + 78-88: wrap (periodic builds only), snapshot the
 // reference positions (neighbor.rs:48-50), quantise, bin into xy columns.
-__global__ void k_wrap_quantize(int n, double *x, double *y, double *z, double *rx, double *ry,
-                                double *rz, uint32_t *q, int *col_count, BuildParams bp, int wrap) {
+// The positions are read from the caller's arrays (px, py, pz), wrapped in
+// place there, and copied into the system's own (x, y, z).
+__global__ void k_wrap_quantize(int n, double *px, double *py, double *pz, double *x, double *y, double *z,
+                                double *rx, double *ry, double *rz, uint32_t *q, int *col_count, BuildParams bp,
+                                int wrap) {
     const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = i0 < n;
     const int i = active ? i0 : 0;
     int col = 0;
     if (active) {
-        double xi = x[i], yi = y[i], zi = z[i];
+        double xi = px[i], yi = py[i], zi = pz[i];
         if (wrap) {
             xi = wrap_coord(xi, bp.box[0]);
             yi = wrap_coord(yi, bp.box[1]);
             zi = wrap_coord(zi, bp.box[2]);
-            x[i] = xi; y[i] = yi; z[i] = zi;
+            px[i] = xi; py[i] = yi; pz[i] = zi;
         }
+        x[i] = xi; y[i] = yi; z[i] = zi;
         rx[i] = xi; ry[i] = yi; rz[i] = zi;
         uint32_t qx = quantize(xi, bp.inv_q[0]);
         uint32_t qy = quantize(yi, bp.inv_q[1]);
@@ -104,12 +95,14 @@ __global__ void k_wrap_quantize(int n, double *x, double *y, double *z, double *
     (void)aggregated_fetch_add(col_count, col, active);
 }
 
-// Exclusive scans over columns: atom offsets and supercluster offsets; total
-// supercluster count, column overflow flag and largest column into meta.
-__global__ __launch_bounds__(1024) void k_scan_columns(int ncol, const int *cnt, int *start,
-                                                      int *colsc, int *meta, int cap) {
+// Exclusive scans over columns (one block of 1024): atom offsets and
+// supercluster offsets; total supercluster count, column overflow flag and
+// largest column into meta.  Also clears the scatter's column fill counters.
+__device__ inline void scan_columns_block(int ncol, const int *cnt, int *start, int *colsc, int *meta, int cap,
+                                          int *col_fill) {
     __shared__ int s1[1024], s2[1024], sm[1024];
     int t = threadIdx.x;
+    for (int i = t; i < ncol; i += 1024) col_fill[i] = 0;
     int per = (ncol + 1023) / 1024;
     int b = t * per, e = min(b + per, ncol);
     int a1 = 0, a2 = 0, mx = 0;
@@ -134,10 +127,16 @@ __global__ __launch_bounds__(1024) void k_scan_columns(int ncol, const int *cnt,
         ex1 += c; ex2 += (c + 63) / 64;
     }
     if (t == 1023) {
+        colsc[ncol] = s2[1023];  // (column c's superclusters: colsc[c] .. colsc[c + 1])
         meta[0] = s2[1023];
         meta[1] = sm[1023] > cap ? 1 : 0;
         meta[2] = sm[1023];
     }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_columns(int ncol, const int *cnt, int *start, int *colsc, int *meta,
+                                                      int cap, int *col_fill) {
+    scan_columns_block(ncol, cnt, start, colsc, meta, cap, col_fill);
 }
 
 __global__ void k_scatter(int n, const uint32_t *q, const int *col_start, int *col_fill, int *bucket,
@@ -158,21 +157,22 @@ __global__ void k_scatter(int n, const uint32_t *q, const int *col_start, int *c
 // kLdsColumns columns): one global atomic per (workgroup, column) instead of
 // one per (wavefront, column), all issued at once rather than one returning
 // atomic after another.
-__global__ __launch_bounds__(1024) void k_wrap_quantize_lds(int n, double *x, double *y, double *z, double *rx,
-                                                           double *ry, double *rz, uint32_t *q, int *col_count,
-                                                           BuildParams bp, int wrap) {
+__global__ __launch_bounds__(1024) void k_wrap_quantize_lds(int n, double *px, double *py, double *pz, double *x,
+                                                           double *y, double *z, double *rx, double *ry, double *rz,
+                                                           uint32_t *q, int *col_count, BuildParams bp, int wrap) {
     extern __shared__ int hist[];
     const int t = threadIdx.x, i = blockIdx.x * 1024 + t;
     for (int c = t; c < bp.ncol; c += 1024) hist[c] = 0;
     __syncthreads();
     if (i < n) {
-        double xi = x[i], yi = y[i], zi = z[i];
+        double xi = px[i], yi = py[i], zi = pz[i];
         if (wrap) {
             xi = wrap_coord(xi, bp.box[0]);
             yi = wrap_coord(yi, bp.box[1]);
             zi = wrap_coord(zi, bp.box[2]);
-            x[i] = xi; y[i] = yi; z[i] = zi;
+            px[i] = xi; py[i] = yi; pz[i] = zi;
         }
+        x[i] = xi; y[i] = yi; z[i] = zi;
         rx[i] = xi; ry[i] = yi; rz[i] = zi;
         const uint32_t qx = quantize(xi, bp.inv_q[0]), qy = quantize(yi, bp.inv_q[1]), qz = quantize(zi, bp.inv_q[2]);
         q[i] = qx; q[n + i] = qy; q[2 * n + i] = qz;
@@ -251,51 +251,64 @@ __global__ __launch_bounds__(1024) void k_column_sort(int n, const uint32_t *q, 
     }
 }
 
-// Split a full supercluster (64 atoms, z-sorted) into 2x2x2 clusters of 8:
-// halves by x, quarters by y, eighths by z (ties broken by atom index).
-// Partial superclusters keep z order and are cut into consecutive eights.
-__global__ __launch_bounds__(64) void k_sc_split(const int *meta, int n, const uint32_t *q,
-                                                 const int *sorted_atoms, const int *sc_base,
-                                                 const int *sc_m, int *slot_atom) {
+// One wavefront per supercluster slot range (all max_sc of them):
+//  - split: a full supercluster (64 atoms, z-sorted) into 2x2x2 clusters of
+//    8: halves by x, quarters by y, eighths by z (ties broken by atom index);
+//    a partial one keeps z order and is cut into consecutive eights;
+//  - slot tables: qpos/qeps (the force kernel's per-slot inputs) are filled
+//    for every slot, padding included, so the per-evaluation prep kernel only
+//    needs to overwrite the atoms' slots; sq.w = 1 marks an atom whose
+//    position is not finite: such an atom is in no listed pair (the
+//    reference's r2 <= r_skin^2 is false for a NaN distance);
+//  - bounding boxes (fixed-point centre, half extents in A) of the 8 clusters
+//    and the supercluster.  A cluster is valid iff its first slot holds an
+//    atom.  Offsets are taken relative to the group's first atom (minimum
+//    image), so min <= 0 <= max; spans can exceed half the box in small
+//    boxes, hence computed in float;
+//  - the fixed-point force accumulator's words of these slots are cleared
+//    (the slot count changed: the next evaluation starts from zero).
+// Superclusters past the count (or all, after a column overflow) get padding.
+__global__ __launch_bounds__(64) void k_sc_setup(const int *meta, int n, int max_sc, const uint32_t *q,
+                                                 const int *sorted_atoms, const int *sc_base, const int *sc_m,
+                                                 int *slot_atom, const float2 *atom_par, const uint4 *excl_bits,
+                                                 int *atom_slot, int4 *sq, float2 *sp, int4 *qpos, float *qeps,
+                                                 uint4 *sxb, float inv_s, const double *x, const double *y,
+                                                 const double *z, BuildParams bp, int4 *cl_ref, float4 *cl_half,
+                                                 int4 *sc_ref, float4 *sc_half, unsigned long long *F) {
     __shared__ unsigned long long key[64];
+    __shared__ int4 lsq[64];
     const int sc = blockIdx.x;
-    if (meta[1] || sc >= meta[0]) return;  // a column overflowed: sc_base is not set
     const int t = threadIdx.x;
-    const int m = sc_m[sc];
-    const int a = t < m ? sorted_atoms[sc_base[sc] + t] : -1;
-    int pos = t;
-    if (m == 64) {
-        for (int level = 0; level < 3; level++) {
-            const int gsize = 64 >> level;
-            const unsigned long long mine = ((unsigned long long)q[level * n + a] << 32) | (unsigned)a;
-            key[pos] = mine;
-            __syncthreads();
-            const int g = pos / gsize;
-            int r = 0;
-            for (int u = 0; u < gsize; u++) r += key[g * gsize + u] < mine ? 1 : 0;
-            __syncthreads();
-            pos = g * gsize + r;
+    const size_t npad = (size_t)max_sc * 64;
+    F[(size_t)sc * 64 + t] = 0ull;
+    F[npad + (size_t)sc * 64 + t] = 0ull;
+    F[2 * npad + (size_t)sc * 64 + t] = 0ull;
+    const bool live = !meta[1] && sc < meta[0];  // (a column overflowed: sc_base is not set)
+    int a = -1, pos = t;
+    if (live) {
+        const int m = sc_m[sc];
+        a = t < m ? sorted_atoms[sc_base[sc] + t] : -1;
+        if (m == 64) {
+            for (int level = 0; level < 3; level++) {
+                const int gsize = 64 >> level;
+                const unsigned long long mine = ((unsigned long long)q[level * n + a] << 32) | (unsigned)a;
+                key[pos] = mine;
+                __syncthreads();
+                const int g = pos / gsize;
+                int r = 0;
+                for (int u = 0; u < gsize; u++) r += key[g * gsize + u] < mine ? 1 : 0;
+                __syncthreads();
+                pos = g * gsize + r;
+            }
         }
     }
-    slot_atom[(size_t)sc * 64 + pos] = a;
-}
-
-// Slot tables.  qpos/qeps (the force kernel's per-slot inputs) are filled for
-// every slot, padding included, so the per-evaluation prep kernel only needs
-// to overwrite the atoms' slots.
-// sq.w = 1 marks an atom whose position is not finite: such an atom is in no
-// listed pair (the reference's r2 <= r_skin^2 is false for a NaN distance).
-__global__ void k_slot_setup(int nslots, int n, const int *slot_atom, const uint32_t *q,
-                             const float2 *atom_par, const uint2 *excl_bits, int *atom_slot, int4 *sq,
-                             float2 *sp, int4 *qpos, float *qeps, uint2 *sxb, float inv_s,
-                             const double *x, const double *y, const double *z) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslots) return;
-    int a = slot_atom[s];
+    const size_t s = (size_t)sc * 64 + pos;  // this thread's atom's slot
+    slot_atom[s] = a;
+    int4 v = make_int4(0, 0, 0, 0);
     if (a >= 0) {
-        atom_slot[a] = s;
+        atom_slot[a] = (int)s;
         const bool fin = isfinite(x[a]) && isfinite(y[a]) && isfinite(z[a]);
-        const int4 v = make_int4((int)q[a], (int)q[n + a], (int)q[2 * n + a], fin ? 0 : 1);
+        v = make_int4((int)q[a], (int)q[n + a], (int)q[2 * n + a], fin ? 0 : 1);
         const float2 p = atom_par[a];
         sq[s] = v;
         sp[s] = p;
@@ -303,32 +316,22 @@ __global__ void k_slot_setup(int nslots, int n, const int *slot_atom, const uint
         qeps[s] = 2.0f * p.y;
         sxb[s] = excl_bits[a];
     } else {
-        sq[s] = make_int4(0, 0, 0, 0);
+        sq[s] = v;
         sp[s] = make_float2(0.f, 0.f);
-        qpos[s] = make_int4(0, 0, 0, 0);
+        qpos[s] = v;
         qeps[s] = 0.f;
-        sxb[s] = make_uint2(0u, 0u);
+        sxb[s] = make_uint4(0u, 0u, 0u, 0u);
     }
-}
-
-// Bounding boxes (fixed-point centre, half extents in A) of every cluster and
-// supercluster.  A cluster is valid iff its first slot holds an atom.  Offsets
-// are taken relative to the group's first atom (minimum image), so min <= 0 <=
-// max; spans can exceed half the box in small boxes, hence computed in float.
-__global__ __launch_bounds__(64) void k_bbox(const int *meta, const int *slot_atom, const int4 *sq,
-                                             BuildParams bp, int4 *cl_ref, float4 *cl_half,
-                                             int4 *sc_ref, float4 *sc_half) {
-    const int sc = blockIdx.x;
-    if (sc >= meta[0]) return;
-    const int t = threadIdx.x;
-    const size_t s = (size_t)sc * 64 + t;
-    const bool valid = slot_atom[s] >= 0;
-    const int4 v = sq[s];
-    const size_t sr = (size_t)sc * 64 + (t & ~7);
-    const int4 r = sq[sr];
-    const bool cvalid = slot_atom[sr] >= 0;
-    int dx = valid ? wrapdiff(v.x, r.x) : 0, dy = valid ? wrapdiff(v.y, r.y) : 0,
-        dz = valid ? wrapdiff(v.z, r.z) : 0;
+    if (!live) return;  // the same for the whole block
+    // bounding boxes over the slots, from LDS (slot t's values; w: 1 = atom)
+    lsq[pos] = make_int4(v.x, v.y, v.z, a >= 0 ? 1 : 0);
+    __syncthreads();
+    const int4 vs = lsq[t];
+    const bool valid = vs.w != 0;
+    const int4 r = lsq[t & ~7];
+    const bool cvalid = r.w != 0;
+    int dx = valid ? wrapdiff(vs.x, r.x) : 0, dy = valid ? wrapdiff(vs.y, r.y) : 0,
+        dz = valid ? wrapdiff(vs.z, r.z) : 0;
     int mnx = dx, mxx = dx, mny = dy, mxy = dy, mnz = dz, mxz = dz;
     for (int o = 1; o < 8; o <<= 1) {
         mnx = min(mnx, __shfl_xor(mnx, o)); mxx = max(mxx, __shfl_xor(mxx, o));
@@ -344,10 +347,10 @@ __global__ __launch_bounds__(64) void k_bbox(const int *meta, const int *slot_at
                                  (((float)mxy - (float)mny) * 0.5f + 1.0f) * bp.sq[1],
                                  (((float)mxz - (float)mnz) * 0.5f + 1.0f) * bp.sq[2], 0.f);
     }
-    const int4 r0 = sq[(size_t)sc * 64];
-    dx = valid ? wrapdiff(v.x, r0.x) : 0;
-    dy = valid ? wrapdiff(v.y, r0.y) : 0;
-    dz = valid ? wrapdiff(v.z, r0.z) : 0;
+    const int4 r0 = lsq[0];
+    dx = valid ? wrapdiff(vs.x, r0.x) : 0;
+    dy = valid ? wrapdiff(vs.y, r0.y) : 0;
+    dz = valid ? wrapdiff(vs.z, r0.z) : 0;
     mnx = mxx = dx; mny = mxy = dy; mnz = mxz = dz;
     for (int o = 1; o < 64; o <<= 1) {
         mnx = min(mnx, __shfl_xor(mnx, o)); mxx = max(mxx, __shfl_xor(mxx, o));
@@ -364,78 +367,216 @@ __global__ __launch_bounds__(64) void k_bbox(const int *meta, const int *slot_at
     }
 }
 
-// One wave per supercluster: scan the j-clusters cj >= its first cluster that
-// lie in xy columns within reach of its own column (columns are periodic in
-// the fixed-point frame, so every pair within r_skin is in reach).  An entry
-// is (sc, cj) with a nonzero 8-bit i-cluster mask (ci <= cj, both valid,
-// boxes within r_skin).  Columns are visited in ascending index, and clusters
-// are numbered column by column, so entries come in ascending cj.  Pass 0
-// counts, pass 1 writes entries at sc_start[sc].
+// Exclusive scan of n ints by one block of NT threads: out[k] = sum of
+// in[0..k), out[n] = total, returned to every thread.  Thread t supplies
+// input base + t of each tile of NT as f(base + t) (so a count can be
+// transformed on the fly).  sh: 2 * (NT / 64) ints of LDS.
+template <int NT, typename F>
+__device__ inline int block_exclusive_scan(int n, F f, int *out, int *sh) {
+    constexpr int NW = NT / 64;
+    const int t = threadIdx.x;
+    int carry = 0;
+    for (int base = 0; base < n; base += NT) {
+        const int k = base + t;
+        const int v = k < n ? f(k) : 0;
+        // inclusive scan within each wavefront, then over the wave totals
+        int x = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if ((t & 63) >= o) x += y;
+        }
+        if ((t & 63) == 63) sh[t >> 6] = x;
+        __syncthreads();
+        if (t < NW) {
+            int w = sh[t];
+            for (int o = 1; o < NW; o <<= 1) {
+                const int y = __shfl_up(w, o, NW);
+                if (t >= o) w += y;
+            }
+            sh[NW + t] = w;  // inclusive over waves
+        }
+        __syncthreads();
+        const int wave_base = (t >> 6) ? sh[NW + (t >> 6) - 1] : 0;
+        if (k < n) out[k] = carry + wave_base + x - v;
+        carry += sh[2 * NW - 1];
+        __syncthreads();  // sh is rewritten by the next tile
+    }
+    if (t == 0) out[n] = carry;
+    return carry;
+}
+
+// Arguments of the per-supercluster scans.
+struct ScScanArgs {
+    int *sc_start, *item_off, *blk_off, *bins;
+    int ent_cap, chunk_req, slots;
+};
+
+// One block of NT: entry offsets per supercluster (sc_start, exclusive scan of
+// the pair-search counts), the total entry count and whether it exceeds the
+// entry buffers' capacity, the work shape (entries per work item: chunk_req,
+// or for 0 about 1.5 items per resident wavefront slot, in whole blocks of
+// kBlock, 8..kMaxRun), and per supercluster its first work item and first
+// block.  Also clears the bins of the work-item sort.
+template <int NT>
+__device__ inline void sc_scan_block(int *meta, const int *sc_count, const ScScanArgs &A) {
+    __shared__ int sh[2 * (NT / 64)];
+    __shared__ int s_chunk;
+    const int t = threadIdx.x;
+    for (int k = t; k < 2 * kCostBins; k += NT) A.bins[k] = 0;
+    const int nsc = meta[1] ? 0 : meta[0];
+    const int nent = block_exclusive_scan<NT>(nsc, [&](int k) { return sc_count[k]; }, A.sc_start, sh);
+    if (t == 0) {
+        meta[kMetaNent] = nent;
+        meta[kMetaEntOver] = nent > A.ent_cap ? 1 : 0;
+        int c = A.chunk_req;
+        if (c <= 0) c = (int)((2LL * nent + 3LL * A.slots - 1) / (3LL * A.slots));
+        c = min(max(kBlock, (c + kBlock - 1) / kBlock * kBlock), kMaxRun);
+        s_chunk = c;
+        meta[kMetaChunk] = c;
+    }
+    __syncthreads();
+    const int c = s_chunk;
+    const int nwi = block_exclusive_scan<NT>(nsc, [&](int k) { return (sc_count[k] + c - 1) / c; }, A.item_off, sh);
+    const int nblk =
+        block_exclusive_scan<NT>(nsc, [&](int k) { return (sc_count[k] + kBlock - 1) / kBlock; }, A.blk_off, sh);
+    if (t == 0) {
+        meta[kMetaNwi] = nwi;
+        meta[kMetaNblk] = nblk;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_sc_scan(int *meta, const int *sc_count, ScScanArgs A) {
+    sc_scan_block<1024>(meta, sc_count, A);
+}
+
+// The ix-th column of at most two ascending pieces of a ring.
+__device__ inline int piece_at(int2 p0, int2 p1, int ix) {
+    const int n0 = p0.y - p0.x + 1;
+    return ix < n0 ? p0.x + ix : p1.x + (ix - n0);
+}
+
+// One workgroup of 256 per supercluster: test the j-clusters cj >= its first
+// cluster that lie in xy columns within reach of its own column (columns are
+// periodic in the fixed-point frame, so every pair within r_skin is in
+// reach).  An entry is (sc, cj) with a nonzero 8-bit i-cluster mask (ci <= cj,
+// both valid, boxes within r_skin).  Columns are taken in ascending index,
+// 256 at a time; their candidate clusters are laid end to end (a scan of the
+// per-column counts) and tested 256 per round, one per thread, so a round
+// waits for one load of cluster boxes.  Clusters are numbered column by
+// column, so entries come in ascending cj.  Pass 0 counts, pass 1 writes
+// entries at sc_start[sc], each round's in thread order.
 template <int FILL>
-__global__ __launch_bounds__(64) void k_pair_search(const int *meta, BuildParams bp, const int *sc_col,
-                                                   const int *col_sc, const int *col_count,
-                                                   const int4 *cl_ref, const float4 *cl_half,
-                                                   const int4 *sc_ref, const float4 *sc_half,
-                                                   int *sc_count, const int *sc_start, int *ent_sc,
-                                                   int *ent_cj, uint32_t *ent_imask, int fill) {
+__device__ inline int pair_search_sc(const int *meta, const BuildParams &bp, const int *sc_col,
+                                     const int *col_sc, const int4 *cl_ref,
+                                     const float4 *cl_half, const int4 *sc_ref, const float4 *sc_half,
+                                     const int *sc_start, int *ent_sc, int *ent_cj, uint32_t *ent_imask) {
     __shared__ int4 cir[8];
     __shared__ float4 cih[8];
+    __shared__ int s_lo[256];       // first candidate cluster of each column of the chunk
+    __shared__ int s_off[257];      // exclusive scan of the columns' candidate counts
+    __shared__ int s_sh[8];
+    __shared__ int s_wc[4];         // entries per wavefront in the round
     const int sc = blockIdx.x;
-    if (meta[1] || sc >= meta[0] || (fill && meta[kMetaEntOver])) return;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (t < 8) { cir[t] = cl_ref[sc * 8 + t]; cih[t] = cl_half[sc * 8 + t]; }
-    __syncthreads();
     const int4 sr = sc_ref[sc];
     const float4 sh = sc_half[sc];
     const int first = sc * 8;
     const int base = FILL ? sc_start[sc] : 0;
     const int c0 = sc_col[sc];
     const int cx0 = c0 / bp.ncy, cy0 = c0 - (c0 / bp.ncy) * bp.ncy;
-    int2 px[2], py[2];
-    const int npx = ring_pieces(cx0, bp.kx, bp.ncx, px);
-    const int npy = ring_pieces(cy0, bp.ky, bp.ncy, py);
+    int2 px0, px1, py0, py1;
+    ring_pieces(cx0, bp.kx, bp.ncx, px0, px1);
+    ring_pieces(cy0, bp.ky, bp.ncy, py0, py1);
+    const int NX = (px0.y - px0.x + 1) + (px1.y - px1.x + 1);  // an absent piece is [0, -1]
+    const int NY = (py0.y - py0.x + 1) + (py1.y - py1.x + 1);
+    const int ncols = NX * NY;  // ascending (ix, iy) visits ascending column index
     const float rs2 = bp.r_search * bp.r_search;
+    const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int count = 0;
-    const unsigned long long lt_mask = (t == 0) ? 0ull : (~0ull >> (64 - t));
-    for (int a = 0; a < npx; a++) {
-        for (int cx = px[a].x; cx <= px[a].y; cx++) {
+    __syncthreads();  // cir, cih
+    for (int k0 = 0; k0 < ncols; k0 += 256) {
+        int lo = 0, n = 0;
+        const int k = k0 + t;
+        if (k < ncols) {
+            const int ix = k / NY, iy = k - (k / NY) * NY;
+            const int cx = piece_at(px0, px1, ix), cy = piece_at(py0, py1, iy);
             const float gx = column_gap(cx - cx0, bp.ncx, bp.wcol[0]);
-            for (int b = 0; b < npy; b++) {
-                for (int cy = py[b].x; cy <= py[b].y; cy++) {
-                    const float gy = column_gap(cy - cy0, bp.ncy, bp.wcol[1]);
-                    const int c = cx * bp.ncy + cy;
-                    if (c < c0 || gx * gx + gy * gy > rs2) continue;
-                    const int m = col_count[c];
-                    const int lo = (c == c0) ? first : col_sc[c] * 8;
-                    const int hi = col_sc[c] * 8 + ((m + 63) / 64) * 8;
-                    for (int j0 = lo; j0 < hi; j0 += 64) {
-                        const int cj = j0 + t;
-                        uint32_t imask = 0;
-                        if (cj < hi) {
-                            const int4 jr = cl_ref[cj];
-                            const float4 jh = cl_half[cj];
-                            if (jr.w && boxes_close(sr, sh, jr, jh, bp)) {
-                                for (int ci = 0; ci < 8; ci++) {
-                                    if (first + ci <= cj && cir[ci].w &&
-                                        boxes_close(cir[ci], cih[ci], jr, jh, bp))
-                                        imask |= 1u << ci;
-                                }
-                            }
-                        }
-                        const unsigned long long bl = __ballot(imask != 0);
-                        if (FILL && imask) {
-                            const int off = base + count + __popcll(bl & lt_mask);
-                            ent_sc[off] = sc;
-                            ent_cj[off] = cj;
-                            ent_imask[off] = imask;
-                        }
-                        count += __popcll(bl);
+            const float gy = column_gap(cy - cy0, bp.ncy, bp.wcol[1]);
+            const int c = cx * bp.ncy + cy;
+            if (c >= c0 && gx * gx + gy * gy <= rs2) {
+                lo = (c == c0) ? first : col_sc[c] * 8;
+                n = max(col_sc[c + 1] * 8 - lo, 0);
+            }
+        }
+        s_lo[t] = lo;
+        const int ncand = block_exclusive_scan<256>(256, [&](int) { return n; }, s_off, s_sh);
+        const int nc = min(256, ncols - k0);
+        __syncthreads();  // s_lo, s_off
+        for (int q0 = 0; q0 < ncand; q0 += 256) {
+            const int q = q0 + t;
+            uint32_t imask = 0;
+            int cj = 0;
+            if (q < ncand) {
+                int a = 0, b = nc - 1;  // the last column whose candidates start at or before q
+                while (a < b) {
+                    const int mid = (a + b + 1) >> 1;
+                    if (s_off[mid] <= q) a = mid; else b = mid - 1;
+                }
+                cj = s_lo[a] + (q - s_off[a]);
+                const int4 jr = cl_ref[cj];
+                const float4 jh = cl_half[cj];
+                if (jr.w && boxes_close(sr, sh, jr, jh, bp)) {
+                    for (int ci = 0; ci < 8; ci++) {
+                        if (first + ci <= cj && cir[ci].w && boxes_close(cir[ci], cih[ci], jr, jh, bp))
+                            imask |= 1u << ci;
                     }
                 }
             }
+            const unsigned long long bl = __ballot(imask != 0);
+            if (lane == 0) s_wc[w] = __popcll(bl);
+            __syncthreads();
+            int woff = 0, rtot = 0;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int cv = s_wc[v];
+                woff += v < w ? cv : 0;
+                rtot += cv;
+            }
+            if (FILL && imask) {
+                const int off = base + count + woff + __popcll(bl & lt_mask);
+                ent_sc[off] = sc;
+                ent_cj[off] = cj;
+                ent_imask[off] = imask;
+            }
+            count += rtot;
+            __syncthreads();  // s_wc is rewritten by the next round
         }
     }
-    if (!FILL && t == 0) sc_count[sc] = count;
+    return count;
+}
+
+// Pass 0 counts, pass 1 writes the entries.
+__global__ __launch_bounds__(256) void k_pair_count(const int *meta, BuildParams bp, const int *sc_col,
+                                                    const int *col_sc, const int4 *cl_ref,
+                                                    const float4 *cl_half, const int4 *sc_ref,
+                                                    const float4 *sc_half, int *sc_count) {
+    const int sc = blockIdx.x;
+    if (meta[1] || sc >= meta[0]) return;  // the same for the whole block
+    const int count = pair_search_sc<0>(meta, bp, sc_col, col_sc, cl_ref, cl_half, sc_ref, sc_half, nullptr,
+                                        nullptr, nullptr, nullptr);
+    if (threadIdx.x == 0) sc_count[sc] = count;
+}
+
+__global__ __launch_bounds__(256) void k_pair_fill(const int *meta, BuildParams bp, const int *sc_col,
+                                                   const int *col_sc, const int4 *cl_ref,
+                                                   const float4 *cl_half, const int4 *sc_ref,
+                                                   const float4 *sc_half, const int *sc_start, int *ent_sc,
+                                                   int *ent_cj, uint32_t *ent_imask) {
+    const int sc = blockIdx.x;
+    if (meta[1] || sc >= meta[0] || meta[kMetaEntOver]) return;  // the same for the whole block
+    (void)pair_search_sc<1>(meta, bp, sc_col, col_sc, cl_ref, cl_half, sc_ref, sc_half, sc_start,
+                            ent_sc, ent_cj, ent_imask);
 }
 
 __device__ inline bool excluded(const int64_t *off, const uint32_t *nbr, uint32_t lo, uint32_t hi) {
@@ -447,30 +588,79 @@ __device__ inline bool excluded(const int64_t *off, const uint32_t *nbr, uint32_
     return a < off[lo + 1] && nbr[a] == hi;
 }
 
+// Per tile ci (one value per tile in every lane), the minimum over the
+// wavefront's 64 lanes, returned in the 8 lanes of the tile (lane >> 3 == ci).
+// A transposed reduction: each exchange halves the tiles a lane carries, so
+// the 8 minima cost about as much as 2 separate ones.
+__device__ inline int wave_min8_i32(const int (&v)[8]) {
+    const int lane = (int)__lane_id();
+    // lanes L and L + 32: the lower half keeps tiles 0-3, the upper tiles 4-7
+    int w[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const auto r = __builtin_amdgcn_permlane32_swap((unsigned)v[k], (unsigned)v[k + 4], false, false);
+        w[k] = min((int)r[0], (int)r[1]);
+    }
+    // lanes L and L + 16 within each half: bit 4 clear keeps w[0..1], set w[2..3]
+    int u[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const auto r = __builtin_amdgcn_permlane16_swap((unsigned)w[k], (unsigned)w[k + 2], false, false);
+        u[k] = min((int)r[0], (int)r[1]);
+    }
+    // lanes L and L ^ 8: bit 3 clear keeps u[0], set u[1]
+    const bool b3 = (lane & 8) != 0;
+    const int keep = b3 ? u[1] : u[0], send = b3 ? u[0] : u[1];
+    int x = min(keep, __builtin_amdgcn_update_dpp(0, send, 0x128, 0xF, 0xF, false));  // row_ror:8
+    // the 8 lanes of the tile
+    x = min(x, __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    x = min(x, __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    x = min(x, __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    return x;
+}
+
+// Bitwise OR over the wavefront's 64 lanes, in every lane.
+__device__ inline int wave_or_i32(int v) {
+    v |= __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+    v |= __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+    v |= __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);
+    v |= __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false);
+    const auto r = __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false);
+    v = (int)(r[0] | r[1]);
+    const auto q = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
+    return (int)(q[0] | q[1]);
+}
+
 // One wavefront per entry (a strided walk over an XCD's range of entries,
 // four wavefronts per workgroup): exact pair masks.  Lane = (jl = lane>>3, il = lane&7).
 // The inclusion test is neighbor.rs:31-38: min-image r2 <= r_skin^2 in f64 on
 // the wrapped positions, with i = min(atom), j = max(atom), then the exclusion
 // set (neighbor.rs:33-35).  r2 is first estimated in f32 from the exact
 // fixed-point min-image displacement; a pair whose estimate lies within the
-// band of a threshold is decided by the f64 expression itself, and the band
+// band of the threshold is decided by the f64 expression itself, and the band
 // is many times the estimate's worst error, so every decision is the f64 one.
-// The exclusion lookup runs only for pairs inside r_skin, and only when the
-// a bit of the atom's mask of partners lo+1..lo+64, or for a partner farther
-// up, a binary search when it lies in the atom's exclusion-partner range.
+// Exclusions: a partner within 64 indices above the lower atom is a bit of
+// that atom's mask; a partner farther up is looked up (binary search in the
+// lower atom's sorted partners) only for pairs inside r_skin whose upper atom
+// is at or below the lower atom's largest partner.
 // Per tile, the byte floor((d_lo - r_cutoff) / step) (0 if negative, 255 for
 // a tile without listed pairs) goes to ent_dmin, d_lo a lower bound on the
 // distance of the tile's closest listed pair (the f32 estimate less a
-// relative 2e-6 and 1e-5 A, far above its error).
-__global__ __launch_bounds__(256) void k_masks(const int *meta, const int *ent_sc, const int *ent_cj,
-                                               uint32_t *ent_imask, unsigned long long *ent_mask,
-                                               int2 *ent_meta, uint2 *ent_dmin, uint8_t *ent_lb,
-                                               const int *slot_atom, const int4 *sq, const double *x,
-                                               const double *y, const double *z,
-                                               const int64_t *excl_off, const uint32_t *excl_nbr,
-                                               const uint2 *excl_rng, const uint2 *sxb,
-                                               BuildParams bp,
-                                               unsigned long long *parts) {
+// relative 2e-6 and 1e-5 A, far above its error).  The list statistics are
+// counted separately, on demand (k_list_counts).
+__global__ __launch_bounds__(256) void k_masks(const int *__restrict__ meta, const int *__restrict__ ent_sc,
+                                               const int *__restrict__ ent_cj, uint32_t *__restrict__ ent_imask,
+                                               int2 *__restrict__ ent_meta, uint2 *__restrict__ ent_dmin,
+                                               uint8_t *__restrict__ ent_lb, const int *__restrict__ slot_atom,
+                                               const int4 *__restrict__ sq, const double *__restrict__ x,
+                                               const double *__restrict__ y, const double *__restrict__ z,
+                                               const int64_t *__restrict__ excl_off,
+                                               const uint32_t *__restrict__ excl_nbr,
+                                               const uint4 *__restrict__ sxb, BuildParams bp,
+                                               int *__restrict__ col_count) {
+    // the column counters (last read by the column sort) start the next build
+    // at zero
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < bp.ncol; c += gridDim.x * 256) col_count[c] = 0;
     const int t = threadIdx.x & 63;
     const int il = t & 7, jl = t >> 3;
     // the entries the pair search wrote (none when they overflowed the buffers)
@@ -482,73 +672,84 @@ __global__ __launch_bounds__(256) void k_masks(const int *meta, const int *ent_s
     const int P = (int)min(gridDim.x, 8u);
     const int xg = (int)(blockIdx.x % (unsigned)P);
     const int nwx = (int)((gridDim.x - (unsigned)xg + (unsigned)P - 1u) / (unsigned)P) * 4;  // wavefronts of group xg
-    const int l = (int)(blockIdx.x / (unsigned)P) * 4 + (int)(threadIdx.x >> 6);
+    const int l = __builtin_amdgcn_readfirstlane((int)(blockIdx.x / (unsigned)P) * 4 + (int)(threadIdx.x >> 6));
     const int e_lo = (int)((int64_t)nent * xg / P), e_hi = (int)((int64_t)nent * (xg + 1) / P);
-    unsigned long long tot_pairs = 0, tot_tiles = 0, tot_ctiles = 0, tot_cpairs = 0;
-    for (int e = e_lo + l; e < e_hi; e += nwx) {
-        const int sc = ent_sc[e], cj = ent_cj[e];
-        const uint32_t im = ent_imask[e];
+    // The entry header (supercluster, j cluster, i-cluster mask) is the same in
+    // every lane: it lives in scalar registers and is loaded one entry ahead.
+    // The slot data of all 8 i clusters is loaded at once, whether the mask
+    // lists a cluster or not (every slot of a supercluster exists), so that an
+    // entry waits for one round of loads rather than one per cluster.
+    int e = e_lo + l;
+    int n_sc = 0, n_cj = 0;
+    uint32_t n_im = 0;
+    if (e < e_hi) { n_sc = ent_sc[e]; n_cj = ent_cj[e]; n_im = ent_imask[e]; }
+    for (; e < e_hi; e += nwx) {
+        const int sc = __builtin_amdgcn_readfirstlane(n_sc), cj = __builtin_amdgcn_readfirstlane(n_cj);
+        const uint32_t im = (uint32_t)__builtin_amdgcn_readfirstlane((int)n_im);
+        if (e + nwx < e_hi) { n_sc = ent_sc[e + nwx]; n_cj = ent_cj[e + nwx]; n_im = ent_imask[e + nwx]; }
         const size_t js = (size_t)cj * 8 + jl;
         const int4 qb = sq[js];
-        const int b = qb.w ? -1 : slot_atom[js];  // (a non-finite position is never listed)
-        const uint2 xbj = sxb[js];
+        const int bslot = slot_atom[js];
+        const uint4 xbj = sxb[js];
         // per-cluster values in arrays of scalars (no vector structs), which
         // stay in registers once the loops are unrolled; the exclusion masks
         // of the slots' atoms arrive with their coordinates
         int a[8], qax[8], qay[8], qaz[8];
-        uint32_t exl[8], exh[8];
+        uint32_t exl[8], exh[8], exf[8];
+        const int4 *sqi = sq + (size_t)sc * 64 + il;
+        const int *sai = slot_atom + (size_t)sc * 64 + il;
+        const uint4 *sxi = sxb + (size_t)sc * 64 + il;
 #pragma unroll
         for (int ci = 0; ci < 8; ci++) {
-            a[ci] = -1;
-            qax[ci] = qay[ci] = qaz[ci] = 0;
-            exl[ci] = exh[ci] = 0u;
-            if (im & (1u << ci)) {
-                const size_t is = (size_t)sc * 64 + ci * 8 + il;
-                const int4 q = sq[is];
-                a[ci] = q.w ? -1 : slot_atom[is];
-                qax[ci] = q.x;
-                qay[ci] = q.y;
-                qaz[ci] = q.z;
-                const uint2 xb = sxb[is];
-                exl[ci] = xb.x;
-                exh[ci] = xb.y;
-            }
+            const int4 q = sqi[ci * 8];
+            const int at = sai[ci * 8];
+            const uint4 xb = sxi[ci * 8];
+            a[ci] = (((im >> ci) & 1u) && q.w == 0) ? at : -1;
+            qax[ci] = q.x;
+            qay[ci] = q.y;
+            qaz[ci] = q.z;
+            exl[ci] = xb.x;
+            exh[ci] = xb.y;
+            exf[ci] = xb.z;
         }
+        const int b = qb.w ? -1 : bslot;  // (a non-finite position is never listed)
         // exclusions, reduced at once to one bit per tile (the masks' registers
-        // are then free): a partner within 64 indices is a bit of the lower
-        // atom's mask; a farther one (far) is looked up later if listed
+        // are then free): pairs within 64 indices (near) test the lower atom's
+        // mask, which a tile does only if it has such a pair; a farther pair
+        // up to the lower atom's largest partner (far) is looked up later if
+        // it is listed
         uint32_t xhit = 0, far = 0;
 #pragma unroll
         for (int ci = 0; ci < 8; ci++) {
-            if (a[ci] >= 0 && b >= 0) {
-                const uint32_t lo = (uint32_t)min(a[ci], b), hi = (uint32_t)max(a[ci], b);
-                const uint32_t k = hi - lo - 1u;
-                const uint32_t w = k < 32u ? (a[ci] < b ? exl[ci] : xbj.x) : (a[ci] < b ? exh[ci] : xbj.y);
-                if (k >= 64u) far |= 1u << ci;
-                else if (((w >> (k & 31u)) & 1u) != 0u) xhit |= 1u << ci;
+            const bool ilo = a[ci] < b;
+            const int lo = ilo ? a[ci] : b, hi = ilo ? b : a[ci];
+            const uint32_t k = (uint32_t)(hi - lo - 1);
+            const bool valid = lo >= 0;  // both slots hold atoms
+            const bool near = valid && k < 64u;
+            if (valid && k >= 64u && (uint32_t)hi <= (ilo ? exf[ci] : xbj.z)) far |= 1u << ci;
+            if (__ballot(near)) {
+                const uint32_t w = k < 32u ? (ilo ? exl[ci] : xbj.x) : (ilo ? exh[ci] : xbj.y);
+                if (near && ((w >> (k & 31u)) & 1u) != 0u) xhit |= 1u << ci;
             }
         }
-        uint32_t inb = 0, cutb = 0, band = 0;
-        float r2v[8];
+        uint32_t inb = 0, band = 0;
+        int r2v[8];  // f32 bits of the listed pairs' r2 estimates (+inf for the others)
 #pragma unroll
         for (int ci = 0; ci < 8; ci++) {
             const int cg = sc * 8 + ci;
-            r2v[ci] = 0.f;
+            r2v[ci] = 0x7f800000;
             if (a[ci] >= 0 && b >= 0 && (cg < cj || il < jl)) {
                 const float dx = (float)wrapdiff(qb.x, qax[ci]) * bp.sq[0];
                 const float dy = (float)wrapdiff(qb.y, qay[ci]) * bp.sq[1];
                 const float dz = (float)wrapdiff(qb.z, qaz[ci]) * bp.sq[2];
                 const float r2f = dx * dx + dy * dy + dz * dz;
-                r2v[ci] = r2f;
-                const bool in = r2f <= bp.r_skin2f, cut = r2f <= bp.r_cut2f;
-                if (fabsf(r2f - bp.r_skin2f) <= bp.band_skin || fabsf(r2f - bp.r_cut2f) <= bp.band_cut)
-                    band |= 1u << ci;
-                inb |= in ? (1u << ci) : 0u;
-                cutb |= (in && cut) ? (1u << ci) : 0u;
+                r2v[ci] = __float_as_int(r2f);
+                if (fabsf(r2f - bp.r_skin2f) <= bp.band_skin) band |= 1u << ci;
+                if (r2f <= bp.r_skin2f) inb |= 1u << ci;
             }
         }
-        // pairs near a threshold (rare): the f64 expression decides, one at a
-        // time, outside the unrolled loop
+        // pairs near the threshold (rare): the f64 expression decides, one at
+        // a time, outside the unrolled loop
         while (band) {
             const int ci = __ffs(band) - 1;
             band &= band - 1u;
@@ -560,12 +761,9 @@ __global__ __launch_bounds__(256) void k_masks(const int *meta, const int *ent_s
             const double ddy = min_image(y[hi] - y[lo], bp.box[1]);
             const double ddz = min_image(z[hi] - z[lo], bp.box[2]);
             const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            const bool in = r2 <= bp.r_skin2, cut = r2 <= bp.r_cut2;
-            inb = (inb & ~(1u << ci)) | (in ? (1u << ci) : 0u);
-            cutb = (cutb & ~(1u << ci)) | ((in && cut) ? (1u << ci) : 0u);
+            inb = (inb & ~(1u << ci)) | (r2 <= bp.r_skin2 ? (1u << ci) : 0u);
         }
         inb &= ~xhit;
-        cutb &= ~xhit;
         far &= inb;
         while (far) {
             // the register arrays are read through selects, never by a variable index
@@ -575,65 +773,102 @@ __global__ __launch_bounds__(256) void k_masks(const int *meta, const int *ent_s
 #pragma unroll
             for (int c = 1; c < 8; c++) aci = ci == c ? a[c] : aci;
             const uint32_t lo = (uint32_t)min(aci, b), hi = (uint32_t)max(aci, b);
-            const uint2 rg = excl_rng[lo];
-            if (hi <= rg.y && excluded(excl_off, excl_nbr, lo, hi)) {
-                inb &= ~(1u << ci);
-                cutb &= ~(1u << ci);
-            }
+            if (excluded(excl_off, excl_nbr, lo, hi)) inb &= ~(1u << ci);
         }
-        uint32_t newmask = 0;
-        unsigned long long npairs = 0, ncut_tiles = 0, ncut_pairs = 0;
-        uint32_t dlo = 0, dhi = 0;
-        // lane ci collects tile ci's pair mask: one 64-byte store per entry (a
-        // store per tile would wait for the previous one to free its registers)
-        unsigned long long mlane = 0;
-        // lane ci also collects the smallest r2 estimate of tile ci's listed
-        // pairs (f32 bits of non-negative values order as integers; +inf for
-        // a tile without one); the distance byte is a non-decreasing function
+        // per tile, the smallest r2 estimate of its listed pairs (f32 bits of
+        // non-negative values order as integers; +inf for a tile without one),
+        // in the tile's 8 lanes; the distance byte is a non-decreasing function
         // of r2, so converting the minimum gives the minimum byte
-        int r2min = 0x7f800000;
 #pragma unroll
-        for (int ci = 0; ci < 8; ci++) {
-            const unsigned long long m = __ballot((inb >> ci) & 1u);
-            const unsigned long long mc = __ballot((cutb >> ci) & 1u);
-            mlane = t == ci ? m : mlane;
-            if (m) { newmask |= 1u << ci; npairs += __popcll(m); }
-            if (mc) ncut_tiles++;
-            ncut_pairs += __popcll(mc);
-            const int rb = wave_min_i32(((inb >> ci) & 1u) ? __float_as_int(r2v[ci]) : 0x7f800000);
-            r2min = t == ci ? rb : r2min;
-        }
+        for (int ci = 0; ci < 8; ci++) r2v[ci] = ((inb >> ci) & 1u) ? r2v[ci] : 0x7f800000;
+        const int r2min = wave_min8_i32(r2v);
         int db = 255;
         if (r2min != 0x7f800000) {
             const float dl = sqrtf(__int_as_float(r2min)) * (1.0f - 2e-6f) - 1e-5f;
             const float st = floorf((dl - bp.prune_rc) * bp.prune_inv_step);
             db = st <= 0.f ? 0 : (st >= 255.f ? 255 : (int)st);
         }
+        uint32_t dlo = 0, dhi = 0;
 #pragma unroll
         for (int ci = 0; ci < 4; ci++) {
-            dlo |= (uint32_t)__builtin_amdgcn_readlane(db, ci) << (8 * ci);
-            dhi |= (uint32_t)__builtin_amdgcn_readlane(db, ci + 4) << (8 * ci);
+            dlo |= (uint32_t)__builtin_amdgcn_readlane(db, ci * 8) << (8 * ci);
+            dhi |= (uint32_t)__builtin_amdgcn_readlane(db, ci * 8 + 32) << (8 * ci);
         }
+        const uint32_t newmask = (uint32_t)__builtin_amdgcn_readfirstlane(wave_or_i32((int)inb));
         ent_lb[(size_t)e * 64 + t] = (uint8_t)inb;
-        if (t < 8) ent_mask[(size_t)e * 8 + t] = mlane;
         if (t == 0) {
             ent_imask[e] = newmask;
             ent_meta[e] = make_int2(cj, (int)newmask);
             ent_dmin[e] = make_uint2(dlo, dhi);
         }
-        tot_pairs += npairs;
-        tot_tiles += (unsigned long long)__popc(newmask);
-        tot_ctiles += ncut_tiles;
-        tot_cpairs += ncut_pairs;
     }
-    // per-wavefront partial counts, summed by k_sum_parts: same-address
-    // atomics from every wavefront would serialise (about 10 ns each)
+}
+
+// List statistics of the last build, on demand: listed pairs, tiles with a
+// listed pair, and of those the pairs and tiles within r_cutoff (the f32
+// estimate, and the f64 expression within the band of the threshold, as in
+// k_masks).  One wavefront per entry; per-wavefront partial counts.
+__global__ __launch_bounds__(256) void k_list_counts(const int *__restrict__ meta, const int *__restrict__ ent_sc,
+                                                     const int2 *__restrict__ ent_meta,
+                                                     const uint8_t *__restrict__ ent_lb,
+                                                     const int *__restrict__ slot_atom, const int4 *__restrict__ sq,
+                                                     const double *__restrict__ x, const double *__restrict__ y,
+                                                     const double *__restrict__ z, BuildParams bp,
+                                                     unsigned long long *__restrict__ parts) {
+    const int t = threadIdx.x & 63;
+    const int il = t & 7, jl = t >> 3;
+    const int nent = (meta[1] || meta[kMetaEntOver]) ? 0 : meta[kMetaNent];
+    const int wv = __builtin_amdgcn_readfirstlane((int)blockIdx.x * 4 + (int)(threadIdx.x >> 6));
+    const int nw = (int)gridDim.x * 4;
+    unsigned long long n_pairs = 0, n_tiles = 0, n_ctiles = 0, n_cpairs = 0;
+    for (int e = wv; e < nent; e += nw) {
+        const int sc = __builtin_amdgcn_readfirstlane(ent_sc[e]);
+        const int2 em = ent_meta[e];
+        const int cj = __builtin_amdgcn_readfirstlane(em.x);
+        const uint32_t im = (uint32_t)__builtin_amdgcn_readfirstlane(em.y);
+        const uint32_t lb = ent_lb[(size_t)e * 64 + t];
+        const int4 qb = sq[(size_t)cj * 8 + jl];
+        const int b = slot_atom[(size_t)cj * 8 + jl];
+        uint32_t cut = 0;
+#pragma unroll
+        for (int ci = 0; ci < 8; ci++) {
+            if (!((im >> ci) & 1u)) continue;
+            const size_t is = (size_t)sc * 64 + ci * 8 + il;
+            const int4 q = sq[is];
+            const int aci = slot_atom[is];
+            if ((lb >> ci) & 1u) {
+                const float dx = (float)wrapdiff(qb.x, q.x) * bp.sq[0];
+                const float dy = (float)wrapdiff(qb.y, q.y) * bp.sq[1];
+                const float dz = (float)wrapdiff(qb.z, q.z) * bp.sq[2];
+                const float r2f = dx * dx + dy * dy + dz * dz;
+                bool c = r2f <= bp.r_cut2f;
+                if (fabsf(r2f - bp.r_cut2f) <= bp.band_cut) {
+                    const uint32_t lo = (uint32_t)min(aci, b), hi = (uint32_t)max(aci, b);
+                    const double ddx = min_image(x[hi] - x[lo], bp.box[0]);
+                    const double ddy = min_image(y[hi] - y[lo], bp.box[1]);
+                    const double ddz = min_image(z[hi] - z[lo], bp.box[2]);
+                    c = ddx * ddx + ddy * ddy + ddz * ddz <= bp.r_cut2;
+                }
+                if (c) cut |= 1u << ci;
+            }
+        }
+        n_pairs += (unsigned long long)__popc(lb);
+        n_cpairs += (unsigned long long)__popc(cut);
+        if (t == 0) n_tiles += (unsigned long long)__popc(im);
+        const uint32_t cutor = (uint32_t)__builtin_amdgcn_readfirstlane(wave_or_i32((int)cut));
+        if (t == 0) n_ctiles += (unsigned long long)__popc(cutor);
+    }
+    // per-wavefront sums in a fixed order (lane sums by a tree, then lane 0 writes)
+    for (int o = 32; o > 0; o >>= 1) {
+        n_pairs += __shfl_xor(n_pairs, o);
+        n_cpairs += __shfl_xor(n_cpairs, o);
+    }
     if (t == 0) {
         unsigned long long *pw = parts + ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 4;
-        pw[0] = tot_pairs;
-        pw[1] = tot_tiles;
-        pw[2] = tot_ctiles;
-        pw[3] = tot_cpairs;
+        pw[0] = n_pairs;
+        pw[1] = n_tiles;
+        pw[2] = n_ctiles;
+        pw[3] = n_cpairs;
     }
 }
 
@@ -664,122 +899,72 @@ __global__ void k_rebuild_words(int *flag, const int *bad_first) {
     flag[2] = *bad_first;
 }
 
-// Exclusive scan of n ints by one 1024-thread block (in place of the host
-// loop): out[k] = sum of in[0..k), out[n] = total, returned to every thread.
-// f(k) gives the k-th input (so a count can be transformed on the fly).
-template <typename F>
-__device__ inline int block_exclusive_scan(int n, F f, int *out, int *sh) {
-    const int t = threadIdx.x;
-    int carry = 0;
-    for (int base = 0; base < n; base += 1024) {
-        const int k = base + t;
-        const int v = k < n ? f(k) : 0;
-        // inclusive scan within the block: wavefront scan by DPP-free shuffles, then the 16 wave totals
-        int x = v;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(x, o, 64);
-            if ((t & 63) >= o) x += y;
-        }
-        if ((t & 63) == 63) sh[t >> 6] = x;
-        __syncthreads();
-        if (t < 16) {
-            int w = sh[t];
-            for (int o = 1; o < 16; o <<= 1) {
-                const int y = __shfl_up(w, o, 16);
-                if ((t & 15) >= o) w += y;
-            }
-            sh[16 + t] = w;  // inclusive over waves
-        }
-        __syncthreads();
-        const int wave_base = (t >> 6) ? sh[16 + (t >> 6) - 1] : 0;
-        if (k < n) out[k] = carry + wave_base + x - v;
-        carry += sh[16 + 15];
-        __syncthreads();  // sh is rewritten by the next tile
-    }
-    if (t == 0) out[n] = carry;
-    return carry;
-}
-
-// One block: entry offsets per supercluster (sc_start, exclusive scan of the
-// pair-search counts), the total entry count and whether it exceeds the entry
-// buffers' capacity, the work shape (entries per work item: chunk_req, or for
-// 0 about 1.5 items per resident wavefront slot, in whole blocks of kBlock,
-// 8..kMaxRun), and per supercluster its first work item and first block.
-// Also clears the bins of the work-item sort.
-__global__ __launch_bounds__(1024) void k_sc_scan(int *meta, const int *sc_count, int *sc_start, int *item_off,
-                                                  int *blk_off, int ent_cap, int chunk_req, int slots,
-                                                  int *bins) {
-    __shared__ int sh[32];
-    __shared__ int s_chunk;
-    const int t = threadIdx.x;
-    for (int k = t; k < 2 * kCostBins; k += 1024) bins[k] = 0;
-    const int nsc = meta[1] ? 0 : meta[0];
-    const int nent = block_exclusive_scan(nsc, [&](int k) { return sc_count[k]; }, sc_start, sh);
-    if (t == 0) {
-        meta[kMetaNent] = nent;
-        meta[kMetaEntOver] = nent > ent_cap ? 1 : 0;
-        int c = chunk_req;
-        if (c <= 0) c = (int)((2LL * nent + 3LL * slots - 1) / (3LL * slots));
-        c = min(max(kBlock, (c + kBlock - 1) / kBlock * kBlock), kMaxRun);
-        s_chunk = c;
-        meta[kMetaChunk] = c;
-    }
-    __syncthreads();
-    const int c = s_chunk;
-    const int nwi = block_exclusive_scan(nsc, [&](int k) { return (sc_count[k] + c - 1) / c; }, item_off, sh);
-    const int nblk = block_exclusive_scan(nsc, [&](int k) { return (sc_count[k] + kBlock - 1) / kBlock; }, blk_off, sh);
-    if (t == 0) {
-        meta[kMetaNwi] = nwi;
-        meta[kMetaNblk] = nblk;
-    }
-}
-
 // One thread per work item: a run of `chunk` consecutive entries of one
 // supercluster (whole blocks, counted from its first entry), with its first
 // block and the j-cluster of its first entry, and its estimated cost in
-// half-tile units (tiles plus 0.5 per entry plus 4 per item).
-__global__ void k_wi_make(const int *meta, const int *sc_start, const int *item_off, const int *blk_off,
-                          const uint32_t *ent_imask, const int *ent_cj, int4 *wi_tmp, int *cost,
-                          int *bins) {
-    const int it = blockIdx.x * blockDim.x + threadIdx.x;
-    if (meta[kMetaEntOver] || it >= meta[kMetaNwi]) return;
-    const int nsc = meta[1] ? 0 : meta[0];
-    const int chunk = meta[kMetaChunk];
-    int lo = 0, hi = nsc - 1;  // the last supercluster whose first item is <= it
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (item_off[mid] <= it) lo = mid; else hi = mid - 1;
+// half-tile units (tiles plus 0.5 per entry plus 4 per item).  The costs are
+// counted per bin in LDS first, so a workgroup adds one global count per
+// distinct cost.
+__global__ __launch_bounds__(256) void k_wi_make(const int *meta, const int *sc_start, const int *item_off,
+                                                 const int *blk_off, const uint32_t *ent_imask,
+                                                 const int *ent_cj, int4 *wi_tmp, int *cost, int *bins) {
+    __shared__ int lb[kCostBins];
+    for (int k = threadIdx.x; k < kCostBins; k += 256) lb[k] = 0;
+    __syncthreads();
+    const int it = blockIdx.x * 256 + threadIdx.x;
+    if (!meta[kMetaEntOver] && it < meta[kMetaNwi]) {
+        const int nsc = meta[1] ? 0 : meta[0];
+        const int chunk = meta[kMetaChunk];
+        int lo = 0, hi = nsc - 1;  // the last supercluster whose first item is <= it
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (item_off[mid] <= it) lo = mid; else hi = mid - 1;
+        }
+        const int sc = lo, k = it - item_off[sc];
+        const int e0 = sc_start[sc] + k * chunk, e1 = min(e0 + chunk, sc_start[sc + 1]);
+        // every load issued before any is used: the index is clamped into the run
+        int c = 8;
+#pragma unroll
+        for (int r = 0; r < kMaxRun; r++) {
+            const uint32_t m = ent_imask[min(e0 + r, e1 - 1)];
+            c += e0 + r < e1 ? 2 * __popc(m & 0xffu) + 1 : 0;
+        }
+        wi_tmp[it] = make_int4(sc | ((e1 - e0) << kWiEntryShift), e0, blk_off[sc] + k * (chunk / kBlock),
+                               ent_cj[e0]);
+        cost[it] = c;
+        atomicAdd(&lb[c], 1);
     }
-    const int sc = lo, k = it - item_off[sc];
-    const int e0 = sc_start[sc] + k * chunk, e1 = min(e0 + chunk, sc_start[sc + 1]);
-    int c = 8;
-    for (int e = e0; e < e1; e++) c += 2 * __popc(ent_imask[e] & 0xffu) + 1;
-    wi_tmp[it] = make_int4(sc | ((e1 - e0) << kWiEntryShift), e0, blk_off[sc] + k * (chunk / kBlock), ent_cj[e0]);
-    cost[it] = c;
-    atomicAdd(&bins[c], 1);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kCostBins; k += 256)
+        if (lb[k]) atomicAdd(&bins[k], lb[k]);
 }
 
-// One block: bin offsets in decreasing cost order (heaviest items first).
-__global__ __launch_bounds__(1024) void k_cost_scan(int *bins) {
-    __shared__ int sh[32];
-    // bins[0..kCostBins) counts -> bins[kCostBins + b] = items costlier than b
-    block_exclusive_scan(kCostBins, [&](int k) { return bins[kCostBins - 1 - k]; }, bins + kCostBins, sh);
+// Items to their place in the heaviest-first order: each workgroup scans the
+// cost counts itself (bins[0..kCostBins)) for the start of every cost in
+// decreasing order, ranks its own items in LDS, then reserves one range per
+// distinct cost from the running counts (bins[kCostBins + cost], cleared by the
+// scan).  The order within one cost is whatever the atomics give: results do
+// not depend on the item order.
+__global__ __launch_bounds__(256) void k_wi_scatter(const int *meta, const int4 *wi_tmp, const int *cost,
+                                                    int *bins, int4 *wi) {
+    __shared__ int lb[kCostBins];
+    __shared__ int st[kCostBins + 1];  // st[kCostBins - 1 - c]: items costlier than c
+    __shared__ int sh[8];
+    for (int k = threadIdx.x; k < kCostBins; k += 256) lb[k] = 0;
+    block_exclusive_scan<256>(kCostBins, [&](int k) { return bins[kCostBins - 1 - k]; }, st, sh);
     __syncthreads();
-    // reversed offsets: start of cost b = bins[kCostBins + (kCostBins - 1 - b)]; keep starts in bins[0..)
-    const int t = threadIdx.x;
-    int v[(kCostBins + 1023) / 1024];
-    for (int r = 0; r * 1024 + t < kCostBins; r++) v[r] = bins[kCostBins + (kCostBins - 1 - (r * 1024 + t))];
+    const int it = blockIdx.x * 256 + threadIdx.x;
+    const bool on = !meta[kMetaEntOver] && it < meta[kMetaNwi];
+    int c = 0, r = 0;
+    if (on) {
+        c = cost[it];
+        r = atomicAdd(&lb[c], 1);
+    }
     __syncthreads();
-    for (int r = 0; r * 1024 + t < kCostBins; r++) bins[r * 1024 + t] = v[r];
-}
-
-// Items to their place in the heaviest-first order (the order within one cost
-// is whatever the atomics give: results do not depend on the item order).
-__global__ void k_wi_scatter(const int *meta, const int4 *wi_tmp, const int *cost, int *bins, int4 *wi) {
-    const int it = blockIdx.x * blockDim.x + threadIdx.x;
-    if (meta[kMetaEntOver] || it >= meta[kMetaNwi]) return;
-    const int pos = atomicAdd(&bins[cost[it]], 1);
-    wi[pos] = wi_tmp[it];
+    for (int k = threadIdx.x; k < kCostBins; k += 256)
+        if (lb[k]) lb[k] = st[kCostBins - 1 - k] + atomicAdd(&bins[kCostBins + k], lb[k]);
+    __syncthreads();
+    if (on) wi[lb[c] + r] = wi_tmp[it];
 }
 
 // neighbor.rs:60-72, same f64 expression; any thread past the threshold sets
@@ -861,33 +1046,32 @@ hipError_t launch_fill_u64(unsigned long long *p, size_t n, hipStream_t st) {
 
 // Wrap/quantise, bin, sort, split and box.  meta[0] (supercluster count),
 // meta[1] (column overflow) are valid once the stream reaches this point.
-hipError_t launch_extent(gm_sys *s, double *out) {
-    k_extent<<<1, 1024, 0, s->stream>>>((int)s->n, s->x.p, s->y.p, s->z.p, out);
+hipError_t launch_extent(gm_sys *s, const double *px, const double *py, const double *pz, double *out) {
+    k_extent<<<1, 1024, 0, s->stream>>>((int)s->n, px, py, pz, out);
     return hipGetLastError();
 }
 
-hipError_t launch_build(gm_sys *s, bool wrap) {
+hipError_t launch_build(gm_sys *s, bool wrap, double *px, double *py, double *pz, bool clear_cols) {
     hipStream_t st = s->stream;
     const int n = (int)s->n;
     const BuildParams &bp = s->bp;
     hipError_t e;
-    if ((e = launch_fill_i32(s->col_count.p, bp.ncol, 0, st)) != hipSuccess) return e;
-    if ((e = launch_fill_i32(s->col_fill.p, bp.ncol, 0, st)) != hipSuccess) return e;
-    if ((e = launch_fill_i32(s->meta.p, 8, 0, st)) != hipSuccess) return e;
+    if (clear_cols && (e = launch_fill_i32(s->col_count.p, bp.ncol, 0, st)) != hipSuccess) return e;
+    const bool lds = bp.ncol <= kLdsColumns;
     if (n > 0) {
-        if (bp.ncol <= kLdsColumns)
+        if (lds)
             k_wrap_quantize_lds<<<grid_for(n, 1024), 1024, sizeof(int) * (size_t)bp.ncol, st>>>(
-                n, s->x.p, s->y.p, s->z.p, s->refx.p, s->refy.p, s->refz.p, s->qatom.p, s->col_count.p, bp,
-                wrap ? 1 : 0);
+                n, px, py, pz, s->x.p, s->y.p, s->z.p, s->refx.p, s->refy.p, s->refz.p, s->qatom.p,
+                s->col_count.p, bp, wrap ? 1 : 0);
         else
-            k_wrap_quantize<<<grid_for(n, 256), 256, 0, st>>>(n, s->x.p, s->y.p, s->z.p, s->refx.p,
+            k_wrap_quantize<<<grid_for(n, 256), 256, 0, st>>>(n, px, py, pz, s->x.p, s->y.p, s->z.p, s->refx.p,
                                                                 s->refy.p, s->refz.p, s->qatom.p,
                                                                 s->col_count.p, bp, wrap ? 1 : 0);
     }
-    k_scan_columns<<<1, 1024, 0, st>>>(bp.ncol, s->col_count.p, s->col_start.p, s->col_sc.p,
-                                       s->meta.p, kColumnCap);
+    k_scan_columns<<<1, 1024, 0, st>>>(bp.ncol, s->col_count.p, s->col_start.p, s->col_sc.p, s->meta.p,
+                                       kColumnCap, s->col_fill.p);
     if (n > 0) {
-        if (bp.ncol <= kLdsColumns)
+        if (lds)
             k_scatter_lds<<<grid_for(n, 1024), 1024, sizeof(int) * (size_t)bp.ncol, st>>>(
                 n, s->qatom.p, s->col_start.p, s->col_fill.p, s->bucket.p, bp);
         else
@@ -897,54 +1081,56 @@ hipError_t launch_build(gm_sys *s, bool wrap) {
     k_column_sort<<<bp.ncol, 1024, kColumnCap * sizeof(unsigned long long), st>>>(
         n, s->qatom.p, s->bucket.p, s->col_start.p, s->col_count.p, s->col_sc.p,
         s->sorted_atoms.p, s->sc_base.p, s->sc_m.p, s->sc_col.p);
-    if ((e = launch_fill_i32(s->slot_atom.p, (size_t)s->max_sc * 64, -1, st)) != hipSuccess) return e;
-    k_sc_split<<<(unsigned)s->max_sc, 64, 0, st>>>(s->meta.p, n, s->qatom.p, s->sorted_atoms.p,
-                                                   s->sc_base.p, s->sc_m.p, s->slot_atom.p);
-    k_slot_setup<<<grid_for((size_t)s->max_sc * 64, 256), 256, 0, st>>>(
-        (int)(s->max_sc * 64), n, s->slot_atom.p, s->qatom.p, s->atom_par.p, s->excl_bits.p,
-        s->atom_slot.p, s->sq.p, s->sp.p, s->qpos.p, s->qeps.p, s->sxb.p, kernel_sigma_scale(s), s->x.p, s->y.p,
-        s->z.p);
-    k_bbox<<<(unsigned)s->max_sc, 64, 0, st>>>(s->meta.p, s->slot_atom.p, s->sq.p, bp, s->cl_ref.p,
-                                               s->cl_half.p, s->sc_ref.p, s->sc_half.p);
+    k_sc_setup<<<(unsigned)s->max_sc, 64, 0, st>>>(
+        s->meta.p, n, (int)s->max_sc, s->qatom.p, s->sorted_atoms.p, s->sc_base.p, s->sc_m.p, s->slot_atom.p,
+        s->atom_par.p, s->excl_bits.p, s->atom_slot.p, s->sq.p, s->sp.p, s->qpos.p, s->qeps.p, s->sxb.p,
+        kernel_sigma_scale(s), s->x.p, s->y.p, s->z.p, bp, s->cl_ref.p, s->cl_half.p, s->sc_ref.p, s->sc_half.p,
+        s->F.p);
     return hipGetLastError();
 }
+
+namespace {
+ScScanArgs sc_scan_args(gm_sys *s, int64_t ent_cap, int chunk_req) {
+    ScScanArgs A;
+    A.sc_start = s->sc_start.p;
+    A.item_off = s->item_off.p;
+    A.blk_off = s->sc_blk_off.p;
+    A.bins = s->cost_bins.p;
+    A.ent_cap = (int)std::min<int64_t>(ent_cap, 0x7fffffff);
+    A.chunk_req = chunk_req;
+    A.slots = 20 * (s->ctx->n_cu > 0 ? s->ctx->n_cu : 256);
+    return A;
+}
+}  // namespace
 
 // Grids of the list kernels cover the capacities (the supercluster bound
 // max_sc, the entry buffers' ent_cap); each kernel reads the actual counts
 // from meta and leaves the rest alone, so the build needs no host round trip
 // between its kernels.
-hipError_t launch_pair_count(gm_sys *s) {
-    if (s->max_sc == 0) return hipSuccess;
-    k_pair_search<0><<<(unsigned)s->max_sc, 64, 0, s->stream>>>(
-        s->meta.p, s->bp, s->sc_col.p, s->col_sc.p, s->col_count.p, s->cl_ref.p, s->cl_half.p,
-        s->sc_ref.p, s->sc_half.p, s->sc_count.p, nullptr, nullptr, nullptr, nullptr, 0);
-    return hipGetLastError();
+hipError_t launch_pair_count(gm_sys *s, int64_t ent_cap, int chunk_req) {
+    if (s->max_sc > 0)
+        k_pair_count<<<(unsigned)s->max_sc, 256, 0, s->stream>>>(s->meta.p, s->bp, s->sc_col.p, s->col_sc.p,
+                                                                 s->cl_ref.p, s->cl_half.p, s->sc_ref.p,
+                                                                 s->sc_half.p, s->sc_count.p);
+    return launch_sc_scan(s, ent_cap, chunk_req);
 }
 
 hipError_t launch_sc_scan(gm_sys *s, int64_t ent_cap, int chunk_req) {
-    const int slots = 20 * (s->ctx->n_cu > 0 ? s->ctx->n_cu : 256);
-    k_sc_scan<<<1, 1024, 0, s->stream>>>(s->meta.p, s->sc_count.p, s->sc_start.p, s->item_off.p, s->sc_blk_off.p,
-                                         (int)std::min<int64_t>(ent_cap, 0x7fffffff), chunk_req, slots,
-                                         s->cost_bins.p);
+    k_sc_scan<<<1, 1024, 0, s->stream>>>(s->meta.p, s->sc_count.p, sc_scan_args(s, ent_cap, chunk_req));
     return hipGetLastError();
 }
 
 hipError_t launch_pair_fill_and_masks(gm_sys *s) {
     hipStream_t st = s->stream;
     if (s->max_sc > 0)
-        k_pair_search<1><<<(unsigned)s->max_sc, 64, 0, st>>>(
-            s->meta.p, s->bp, s->sc_col.p, s->col_sc.p, s->col_count.p, s->cl_ref.p, s->cl_half.p,
-            s->sc_ref.p, s->sc_half.p, nullptr, s->sc_start.p, s->ent_sc.p, s->ent_cj.p,
-            s->ent_imask.p, 1);
+        k_pair_fill<<<(unsigned)s->max_sc, 256, 0, st>>>(s->meta.p, s->bp, s->sc_col.p, s->col_sc.p,
+                                                         s->cl_ref.p, s->cl_half.p, s->sc_ref.p, s->sc_half.p,
+                                                         s->sc_start.p, s->ent_sc.p, s->ent_cj.p, s->ent_imask.p);
     const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((s->ent_cap + 3) / 4, kMaskBlocks));
-    hipError_t e = s->mask_parts.ensure((size_t)blocks * 4 * 4);  // 4 counts per wavefront
-    if (e != hipSuccess) return e;
-    s->mask_waves = blocks * 4;
-    k_masks<<<(unsigned)blocks, 256, 0, st>>>(s->meta.p, s->ent_sc.p, s->ent_cj.p, s->ent_imask.p,
-                                              s->ent_mask.p, s->ent_meta.p, s->ent_dmin.p, s->ent_lb.p,
-                                              s->slot_atom.p, s->sq.p, s->x.p, s->y.p, s->z.p,
-                                              s->excl_off.p, s->excl_nbr.p, s->excl_rng.p, s->sxb.p,
-                                              s->bp, s->mask_parts.p);
+    k_masks<<<(unsigned)blocks, 256, 0, st>>>(s->meta.p, s->ent_sc.p, s->ent_cj.p, s->ent_imask.p, s->ent_meta.p,
+                                              s->ent_dmin.p, s->ent_lb.p, s->slot_atom.p, s->sq.p, s->x.p, s->y.p,
+                                              s->z.p, s->excl_off.p, s->excl_nbr.p, s->sxb.p, s->bp,
+                                              s->col_count.p);
     return hipGetLastError();
 }
 
@@ -953,14 +1139,19 @@ hipError_t launch_work_items(gm_sys *s) {
     const unsigned g = (unsigned)std::max<int64_t>(1, (s->item_cap + 255) / 256);
     k_wi_make<<<g, 256, 0, st>>>(s->meta.p, s->sc_start.p, s->item_off.p, s->sc_blk_off.p, s->ent_imask.p,
                                  s->ent_cj.p, s->wi_tmp.p, s->wi_cost.p, s->cost_bins.p);
-    k_cost_scan<<<1, 1024, 0, st>>>(s->cost_bins.p);
     k_wi_scatter<<<g, 256, 0, st>>>(s->meta.p, s->wi_tmp.p, s->wi_cost.p, s->cost_bins.p, s->wi.p);
     return hipGetLastError();
 }
 
 hipError_t launch_list_stats(gm_sys *s) {
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((s->nent + 3) / 4, kMaskBlocks));
+    hipError_t e = s->mask_parts.ensure((size_t)blocks * 4 * 4);  // 4 counts per wavefront
+    if (e != hipSuccess) return e;
     unsigned long long *counters = (unsigned long long *)(s->meta.p + 8);
-    k_sum_parts<<<1, 1024, 0, s->stream>>>(s->mask_parts.p, s->mask_waves, counters);
+    k_list_counts<<<(unsigned)blocks, 256, 0, s->stream>>>(s->meta.p, s->ent_sc.p, s->ent_meta.p, s->ent_lb.p,
+                                                            s->slot_atom.p, s->sq.p, s->x.p, s->y.p, s->z.p, s->bp,
+                                                            s->mask_parts.p);
+    k_sum_parts<<<1, 1024, 0, s->stream>>>(s->mask_parts.p, blocks * 4, counters);
     return hipGetLastError();
 }
 

@@ -21,14 +21,15 @@ __device__ inline bool boxes_close(int4 a, float4 ha, int4 b, float4 hb, const B
 }
 
 // Columns of a periodic ring within `k` of `c`, as at most two ascending
-// [first, last] pieces, so that visiting them in order visits ascending indices.
-__device__ inline int ring_pieces(int c, int k, int n, int2 *pc) {
-    if (2 * k + 1 >= n) { pc[0] = make_int2(0, n - 1); return 1; }
+// [first, last] pieces p0, p1 (an absent piece is [0, -1]), so that visiting
+// them in order visits ascending indices.
+__device__ inline void ring_pieces(int c, int k, int n, int2 &p0, int2 &p1) {
+    p1 = make_int2(0, -1);
     const int lo = c - k, hi = c + k;
-    if (lo < 0) { pc[0] = make_int2(0, hi); pc[1] = make_int2(lo + n, n - 1); return 2; }
-    if (hi >= n) { pc[0] = make_int2(0, hi - n); pc[1] = make_int2(lo, n - 1); return 2; }
-    pc[0] = make_int2(lo, hi);
-    return 1;
+    if (2 * k + 1 >= n) p0 = make_int2(0, n - 1);
+    else if (lo < 0) { p0 = make_int2(0, hi); p1 = make_int2(lo + n, n - 1); }
+    else if (hi >= n) { p0 = make_int2(0, hi - n); p1 = make_int2(lo, n - 1); }
+    else p0 = make_int2(lo, hi);
 }
 
 // Smallest possible distance along one axis between atoms of two columns

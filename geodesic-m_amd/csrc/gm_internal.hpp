@@ -222,7 +222,8 @@ struct gm_sys {
     DevBuf<int64_t> excl_off;
     DevBuf<uint32_t> excl_nbr;
     DevBuf<uint2> excl_rng;  // per atom (smallest, largest) exclusion partner above it; (1, 0) if none
-    DevBuf<uint2> excl_bits; // per atom, mask of exclusion partners i+1..i+64 (bit k: partner i+1+k)
+    DevBuf<uint4> excl_bits; // per atom, mask of exclusion partners i+1..i+64 (bit k: partner i+1+k),
+                             // then its largest partner if farther up (0 if none)
     DevBuf<unsigned long long> mask_parts;  // list statistics, 4 counts per pair-mask wavefront
     DevBuf<uint2> excl_rng_any;  // per atom (smallest, largest) exclusion partner, either side; (1, 0) if none
 
@@ -232,7 +233,8 @@ struct gm_sys {
     DevBuf<double> sx, sy, sz;  // staging for needs_rebuild positions
 
     // spatial sort
-    DevBuf<int> col_count, col_start, col_sc, col_fill, bucket, sorted_atoms;
+    DevBuf<int> col_count, col_start, col_sc, col_fill, bucket, sorted_atoms;  // col_sc: ncol + 1 offsets;
+                                                                               // col_count zero between builds
     DevBuf<int> sc_base, sc_m, sc_col;  // per supercluster: first sorted atom, atom count, column
     DevBuf<uint32_t> qatom;  // 3*n fixed-point coordinates, atom order
     DevBuf<int> meta;        // [0] nsc, [1] overflow, [2] max column, [5..7] rebuild test words;
@@ -243,7 +245,7 @@ struct gm_sys {
     // slots
     DevBuf<int> slot_atom, atom_slot;
     DevBuf<int4> sq;         // fixed-point coords per slot (x, y, z, 0) at the list build
-    DevBuf<uint2> sxb;       // per slot, the exclusion mask (excl_bits) of its atom; 0 for padding
+    DevBuf<uint4> sxb;       // per slot, excl_bits of its atom; 0 for padding
     DevBuf<float2> sp;       // (sigma/2, sqrt eps) per slot
     DevBuf<int4> qpos;       // per evaluation: fixed-point coords per slot + bits of sigma/2
     DevBuf<float> qeps;      // 2 sqrt(eps) per slot
@@ -260,7 +262,6 @@ struct gm_sys {
     DevBuf<int> sc_count, sc_start;
     DevBuf<int> ent_sc, ent_cj;
     DevBuf<uint32_t> ent_imask;
-    DevBuf<unsigned long long> ent_mask;  // per entry 8 x 64-bit pair masks (export)
     DevBuf<int2> ent_meta;                // per entry (j cluster, i-cluster mask)
     DevBuf<uint2> ent_dmin;               // per entry, per tile: closest listed pair byte
     DevBuf<uint8_t> ent_lb;               // per entry, per lane: mask bits over the 8 i clusters
@@ -298,8 +299,11 @@ struct gm_sys {
     DevBuf<int> cost_bins;                // sort bins: counts, then offsets (heaviest first)
     int64_t ent_cap = 0;                  // entries the entry buffers hold (0: size them at the next build)
     int64_t item_cap = 0;                 // work items the work-item buffers hold
-    int mask_waves = 0;                   // pair-mask wavefronts of the last build (their partial counts)
     bool stats_pending = false;           // list statistics not yet summed (gm_get_stats sums them)
+    bool cols_clean = false;              // column counters left at zero by the last build
+    hipGraph_t bgraph = nullptr;          // the captured sized build and what it was captured with
+    hipGraphExec_t bexec = nullptr;
+    std::vector<unsigned char> bkey;
 
     gm::EvalBufs ev{};  // the buffers the launchers use (own_bufs unless a caller gives others)
     bool built = false;
@@ -327,7 +331,10 @@ void gm_dyn_free(gm_dyn *d);  // gm_dynamics.cpp
 // ---- host helpers shared by the single-system and batch APIs (gm_api.cpp) --
 namespace gm {
 gm_status system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream_t ext_stream, gm_sys **out);
-gm_status build_on_device(gm_sys *s, bool wrap);
+// positions from px, py, pz (the system's own x, y, z if null; wrapped there in
+// place when wrap) into the system's x, y, z
+gm_status build_on_device(gm_sys *s, bool wrap, double *px = nullptr, double *py = nullptr,
+                          double *pz = nullptr);
 // Forces at device positions into device arrays (f64, atom order), on the
 // system's stream.  shell: LJ only, each pair weighted by S(r) - S_in(r)
 // (nbp_shell), no bonded terms.
@@ -362,17 +369,20 @@ inline int *bad_first_word(gm_sys *s) { return reinterpret_cast<int *>(s->e_out.
 // ---- kernel launchers (defined in the .hip files) --------------------------
 namespace gm {
 // gm_build.hip
-hipError_t launch_build(gm_sys *s, bool wrap);
+// clear_cols: clear the column counters first (they are left at zero by the
+// pair-mask kernel of a completed build)
+hipError_t launch_build(gm_sys *s, bool wrap, double *px, double *py, double *pz, bool clear_cols);
 // out[0..2] min x y z, out[3..5] max x y z, out[6] 1 if a coordinate is not finite
-hipError_t launch_extent(gm_sys *s, double *out);
-hipError_t launch_pair_count(gm_sys *s);
+hipError_t launch_extent(gm_sys *s, const double *px, const double *py, const double *pz, double *out);
+// entries per supercluster, then the per-supercluster scans (launch_sc_scan)
+hipError_t launch_pair_count(gm_sys *s, int64_t ent_cap, int chunk_req);
 // entry offsets, entry count (and capacity overflow), work shape (chunk_req
 // entries per item, 0 automatic), work-item and block offsets: into meta
 hipError_t launch_sc_scan(gm_sys *s, int64_t ent_cap, int chunk_req);
 hipError_t launch_pair_fill_and_masks(gm_sys *s);
 // heaviest-first work items from the scanned list (after launch_sc_scan)
 hipError_t launch_work_items(gm_sys *s);
-// list statistics (pairs, tiles, within r_cutoff) into meta[8..15] from the pair-mask partials
+// list statistics (pairs, tiles, within r_cutoff) into meta[8..15], counted on demand
 hipError_t launch_list_stats(gm_sys *s);
 // flag: where the answer goes (null: the system's own words, meta[5..7]):
 // flag[0] rebuild needed, flag[1] the first atom whose position is not finite

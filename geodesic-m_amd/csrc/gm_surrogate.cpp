@@ -152,7 +152,6 @@ gm_status features_on_device(gm_surrogate *s, const double *x, const double *y, 
     A.slot_atom = e->slot_atom.p;
     A.sc_col = e->sc_col.p;
     A.col_sc = e->col_sc.p;
-    A.col_count = e->col_count.p;
     A.cl_ref = e->cl_ref.p;
     A.cl_half = e->cl_half.p;
     A.sc_ref = e->sc_ref.p;

@@ -142,8 +142,9 @@ __global__ __launch_bounds__(64 * kFeatWaves) void k_features(FeatArgs A, int ns
     const int c0 = A.sc_col[sc];
     const int cx0 = c0 / bp.ncy, cy0 = c0 - (c0 / bp.ncy) * bp.ncy;
     int2 px[2], py[2];
-    const int npx = ring_pieces(cx0, bp.kx, bp.ncx, px);
-    const int npy = ring_pieces(cy0, bp.ky, bp.ncy, py);
+    ring_pieces(cx0, bp.kx, bp.ncx, px[0], px[1]);
+    ring_pieces(cy0, bp.ky, bp.ncy, py[0], py[1]);
+    const int npx = 2, npy = 2;  // an absent piece is [0, -1]: no columns
     const float rs2 = bp.r_search * bp.r_search;
     int seen = 0;
     for (int a = 0; a < npx; a++) {
@@ -154,8 +155,7 @@ __global__ __launch_bounds__(64 * kFeatWaves) void k_features(FeatArgs A, int ns
                     const float gy = column_gap(cy - cy0, bp.ncy, bp.wcol[1]);
                     if (gx * gx + gy * gy > rs2) continue;
                     const int c = cx * bp.ncy + cy;
-                    const int m = A.col_count[c];
-                    const int lo = A.col_sc[c] * 8, hi = lo + ((m + 63) / 64) * 8;
+                    const int lo = A.col_sc[c] * 8, hi = A.col_sc[c + 1] * 8;
                     for (int j0 = lo; j0 < hi; j0 += 64) {
                         const int cj = j0 + lane;
                         bool ok = false;

@@ -56,7 +56,7 @@ struct FeatArgs {
     const int4 *qs;                  // per slot: fixed-point x, y, z and class (-1: empty slot)
     const float2 *sp;                // per slot: (sigma/2, sqrt(eps))
     const int *slot_atom;
-    const int *sc_col, *col_sc, *col_count;
+    const int *sc_col, *col_sc;  // col_sc[c] .. col_sc[c + 1]: column c's superclusters
     const int4 *cl_ref;
     const float4 *cl_half;
     const int4 *sc_ref;
