@@ -192,6 +192,36 @@ def bench_reference_workloads(cpu=True):
     return out
 
 
+def bench_build(repeats=20):
+    """The Verlet list build alone, at steady state (buffers sized, the build
+    replayed as one graph), on the 24,000- and 250,000-atom systems: device
+    time between the events around the build's kernels, and wall time of
+    build_neighbor_list_device on resident positions (the kernels plus the
+    one read-back of the counts)."""
+    import geodesic_hip as gh
+    from geodesic_hip import systems
+    out = {}
+    for name, mk in (("24k", systems.system_24k), ("250k", systems.system_250k)):
+        s = mk()
+        g = gh.System(s.sigma, s.epsilon, s.topo, s.params)
+        d = [gh.DeviceArray.from_host(a) for a in (s.x, s.y, s.z)]
+        for _ in range(3):
+            g.build_neighbor_list_device(*(a.ptr for a in d))
+        wall, dev = [], []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            g.build_neighbor_list_device(*(a.ptr for a in d))
+            wall.append((time.perf_counter() - t0) * 1e3)
+            dev.append(g.stats()["ms_build"])
+        st = g.stats()
+        out[name] = {"n_atoms": s.n, "ms_device": statistics.median(dev), "ms_wall": statistics.median(wall),
+                     "n_superclusters": st["n_superclusters"], "n_entries": st["n_entries"],
+                     "n_list_pairs": st["n_list_pairs"]}
+        for a in d:
+            a.free()
+    return out
+
+
 def bench_250k(steps=100, warmup=10):
     """250,000 atoms, 136 A box: force evaluation at resident positions and the
     list build, amortised over the 20-step rebuild interval."""
@@ -497,7 +527,7 @@ def main():
     ap.add_argument("--no-md", action="store_true")
     ap.add_argument("--no-pruning-study", action="store_true",
                     help="skip the pruning-off and drifted-position timings (profiling)")
-    ap.add_argument("--only", choices=("250k", "batched", "surrogate", "md", "reference"), default=None,
+    ap.add_argument("--only", choices=("250k", "batched", "surrogate", "md", "reference", "build"), default=None,
                     help="run one secondary section alone and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -519,6 +549,9 @@ def main():
         return
     if args.only == "reference":
         print(json.dumps(bench_reference_workloads(cpu=not args.no_cpu_baseline)))
+        return
+    if args.only == "build":
+        print(json.dumps(bench_build()))
         return
 
     import geodesic_hip as gh

@@ -829,13 +829,20 @@ __global__ __launch_bounds__(256) void k_list_counts(const int *__restrict__ met
         const uint32_t lb = ent_lb[(size_t)e * 64 + t];
         const int4 qb = sq[(size_t)cj * 8 + jl];
         const int b = slot_atom[(size_t)cj * 8 + jl];
+        // every cluster's slot data is loaded at once (every slot exists)
+        int4 qi[8];
+        int ai[8];
+#pragma unroll
+        for (int ci = 0; ci < 8; ci++) {
+            const size_t is = (size_t)sc * 64 + ci * 8 + il;
+            qi[ci] = sq[is];
+            ai[ci] = slot_atom[is];
+        }
         uint32_t cut = 0;
 #pragma unroll
         for (int ci = 0; ci < 8; ci++) {
-            if (!((im >> ci) & 1u)) continue;
-            const size_t is = (size_t)sc * 64 + ci * 8 + il;
-            const int4 q = sq[is];
-            const int aci = slot_atom[is];
+            const int4 q = qi[ci];
+            const int aci = ai[ci];
             if ((lb >> ci) & 1u) {
                 const float dx = (float)wrapdiff(qb.x, q.x) * bp.sq[0];
                 const float dy = (float)wrapdiff(qb.y, q.y) * bp.sq[1];
