@@ -217,6 +217,77 @@ gm_status fit_span_to_atoms(gm_sys *s, const double *px, const double *py, const
     return GM_OK;
 }
 
+// Host arrays to and from device arrays through the system's pinned stage
+// (3 n doubles each way, allocated at the first use).  Pageable transfers ran
+// at about 2.4 GB/s here; a copy into pinned memory and one DMA of all three
+// arrays (the system's position and force triples are one allocation each)
+// take a fraction of that.  Three separate device arrays go one by one, the
+// copy of array k + 1 overlapping the transfer of array k.  The stage is
+// reused only after the previous call's transfers out of it ended.
+gm_status stage_ensure(gm_sys *s) {
+    const size_t n = (size_t)s->n;
+    if (s->stage && s->stage_n >= n) return GM_OK;
+    if (s->stage) {
+        if (s->stage_in_pending) GM_HIP(hipEventSynchronize(s->stage_ev[3]));
+        GM_HIP(hipHostFree(s->stage));
+    }
+    s->stage = nullptr;
+    s->stage_n = 0;
+    s->stage_in_pending = false;
+    GM_HIP(hipHostMalloc((void **)&s->stage, sizeof(double) * 6 * n, hipHostMallocDefault));
+    s->stage_n = n;
+    for (hipEvent_t &ev : s->stage_ev)
+        if (!ev) GM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    return GM_OK;
+}
+
+gm_status host_to_device(gm_sys *s, const double *const src[3], double *const dst[3]) {
+    const size_t n = (size_t)s->n;
+    if (n == 0) return GM_OK;
+    gm_status rc = stage_ensure(s);
+    if (rc != GM_OK) return rc;
+    if (s->stage_in_pending) GM_HIP(hipEventSynchronize(s->stage_ev[3]));  // the previous call's transfers
+    if (dst[1] == dst[0] + n && dst[2] == dst[0] + 2 * n) {  // one allocation: one transfer
+        for (int k = 0; k < 3; k++) std::memcpy(s->stage + k * n, src[k], sizeof(double) * n);
+        GM_HIP(hipMemcpyAsync(dst[0], s->stage, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s->stream));
+    } else {
+        for (int k = 0; k < 3; k++) {
+            std::memcpy(s->stage + k * n, src[k], sizeof(double) * n);
+            GM_HIP(hipMemcpyAsync(dst[k], s->stage + k * n, sizeof(double) * n, hipMemcpyHostToDevice,
+                                  s->stream));
+        }
+    }
+    GM_HIP(hipEventRecord(s->stage_ev[3], s->stream));
+    s->stage_in_pending = true;
+    return GM_OK;
+}
+
+// Device arrays to host arrays through the stage's second half (one transfer
+// for a contiguous triple; otherwise array k is copied out as soon as its own
+// transfer ended).  Returns after the stream reached the last transfer.
+gm_status device_to_host(gm_sys *s, const double *const src[3], double *const dst[3]) {
+    const size_t n = (size_t)s->n;
+    if (n == 0) return GM_OK;
+    gm_status rc = stage_ensure(s);
+    if (rc != GM_OK) return rc;
+    double *out = s->stage + 3 * n;
+    if (src[1] == src[0] + n && src[2] == src[0] + 2 * n) {  // one allocation: one transfer
+        GM_HIP(hipMemcpyAsync(out, src[0], sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s->stream));
+        GM_HIP(hipStreamSynchronize(s->stream));
+        for (int k = 0; k < 3; k++) std::memcpy(dst[k], out + k * n, sizeof(double) * n);
+        return GM_OK;
+    }
+    for (int k = 0; k < 3; k++) {
+        GM_HIP(hipMemcpyAsync(out + k * n, src[k], sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
+        GM_HIP(hipEventRecord(s->stage_ev[k], s->stream));
+    }
+    for (int k = 0; k < 3; k++) {
+        GM_HIP(hipEventSynchronize(s->stage_ev[k]));
+        std::memcpy(dst[k], out + k * n, sizeof(double) * n);
+    }
+    return GM_OK;
+}
+
 // The kernels of a build whose buffers are sized (all but the first build of
 // a system), in stream order.
 hipError_t launch_sized_build(gm_sys *s, bool wrap, double *px, double *py, double *pz, bool clear_cols) {
@@ -405,9 +476,10 @@ gm_status compute_on_device(gm_sys *s, const double *x, const double *y, const d
 }
 
 gm_status collect(gm_sys *s, gm_energy_terms *e) {
-    GM_HIP(hipStreamSynchronize(s->stream));
     double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    GM_HIP(hipMemcpy(t, s->e_out.p, sizeof(t), hipMemcpyDeviceToHost));
+    GM_HIP(hipMemcpyAsync(s->pinned_e, s->e_out.p, sizeof(t), hipMemcpyDeviceToHost, s->stream));
+    GM_HIP(hipStreamSynchronize(s->stream));
+    std::memcpy(t, s->pinned_e, sizeof(t));
     std::memcpy(&s->last_bad, &t[6], sizeof(int));
     if (s->last_timed >= 0) {
         const size_t k = kProfStride * (size_t)s->last_timed;
@@ -692,9 +764,17 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
         if (r.x > r.y) r = make_uint2(1u, 0u);
     TRY(upload(s->excl_rng_any, rng_any.data(), rng_any.size(), st));
 
-    for (DevBuf<double> *b : {&s->x, &s->y, &s->z, &s->refx, &s->refy, &s->refz, &s->fx, &s->fy, &s->fz,
-                              &s->sx, &s->sy, &s->sz})
-        TRY(b->ensure(N));
+    for (DevBuf<double> *b : {&s->refx, &s->refy, &s->refz, &s->sx, &s->sy, &s->sz}) TRY(b->ensure(N));
+    TRY(s->pos3.ensure(3 * N));
+    TRY(s->frc3.ensure(3 * N));
+    DevBuf<double> *xyz[3] = {&s->x, &s->y, &s->z}, *fxyz[3] = {&s->fx, &s->fy, &s->fz};
+    for (int k = 0; k < 3; k++) {
+        xyz[k]->p = s->pos3.p + k * N;
+        xyz[k]->n = N;
+        fxyz[k]->p = s->frc3.p + k * N;
+        fxyz[k]->n = N;
+    }
+    TRY(hipHostMalloc((void **)&s->pinned_e, 8 * sizeof(double), hipHostMallocDefault));
     TRY(s->qatom.ensure(3 * N));
     for (DevBuf<int> *b : {&s->col_count, &s->col_start, &s->col_fill}) TRY(b->ensure((size_t)ncol_max));
     TRY(s->col_sc.ensure((size_t)ncol_max + 1));
@@ -741,11 +821,16 @@ gm_status gm_system_destroy(gm_sys *s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     gm_dyn_free(s->dyn);
     s->dyn = nullptr;
+    for (DevBuf<double> *v : {&s->x, &s->y, &s->z, &s->fx, &s->fy, &s->fz}) {  // views of pos3, frc3
+        v->p = nullptr;
+        v->n = 0;
+    }
     DevBuf<double> *dbl[] = {&s->sigma, &s->eps, &s->bk, &s->br0, &s->akth, &s->ath0, &s->dkphi,
-                             &s->ddelta, &s->x, &s->y, &s->z, &s->refx, &s->refy, &s->refz,
-                             &s->fx, &s->fy, &s->fz, &s->sx, &s->sy, &s->sz, &s->e_items, &s->e_blk, &s->e_out,
-                             &s->ext};
+                             &s->ddelta, &s->pos3, &s->frc3, &s->refx, &s->refy, &s->refz,
+                             &s->sx, &s->sy, &s->sz, &s->e_items, &s->e_blk, &s->e_out, &s->ext};
     for (auto *b : dbl) b->release();
+    if (s->pinned_e) (void)hipHostFree(s->pinned_e);
+    s->pinned_e = nullptr;
     DevBuf<uint32_t> *u32[] = {&s->bi, &s->bj, &s->ai, &s->aj, &s->ak, &s->di, &s->dj, &s->dk,
                                &s->dl, &s->dn, &s->p14i, &s->p14j, &s->excl_nbr, &s->qatom,
                                &s->ent_imask};
@@ -777,6 +862,12 @@ gm_status gm_system_destroy(gm_sys *s) {
     if (s->bgraph) (void)hipGraphDestroy(s->bgraph);
     s->bexec = nullptr;
     s->bgraph = nullptr;
+    if (s->stage) (void)hipHostFree(s->stage);
+    s->stage = nullptr;
+    for (hipEvent_t &ev : s->stage_ev) {
+        if (ev) (void)hipEventDestroy(ev);
+        ev = nullptr;
+    }
     s->item_off.release();
     s->wi_tmp.release();
     s->wi_cost.release();
@@ -798,21 +889,18 @@ gm_status gm_build_neighbor_list(gm_sys *s, double *x, double *y, double *z) {
     GM_REQUIRE(s, "NULL system");
     GM_REQUIRE(!s->batch_owned, "gm_build_neighbor_list: a batch replica's list is built by gm_batch_build_neighbor_lists");
     GM_REQUIRE(s->n == 0 || (x && y && z), "NULL positions");
-    const size_t bytes = sizeof(double) * (size_t)s->n;
-    hipStream_t st = s->stream;
-    if (s->n > 0) {
-        GM_HIP(hipMemcpyAsync(s->x.p, x, bytes, hipMemcpyHostToDevice, st));
-        GM_HIP(hipMemcpyAsync(s->y.p, y, bytes, hipMemcpyHostToDevice, st));
-        GM_HIP(hipMemcpyAsync(s->z.p, z, bytes, hipMemcpyHostToDevice, st));
-    }
-    gm_status rc = build_on_device(s, s->periodic != 0);
+    const double *const in[3] = {x, y, z};
+    double *const dev[3] = {s->x.p, s->y.p, s->z.p};
+    gm_status rc = host_to_device(s, in, dev);
     if (rc != GM_OK) return rc;
-    if (s->periodic && s->n > 0) {
-        GM_HIP(hipMemcpyAsync(x, s->x.p, bytes, hipMemcpyDeviceToHost, st));
-        GM_HIP(hipMemcpyAsync(y, s->y.p, bytes, hipMemcpyDeviceToHost, st));
-        GM_HIP(hipMemcpyAsync(z, s->z.p, bytes, hipMemcpyDeviceToHost, st));
+    rc = build_on_device(s, s->periodic != 0);
+    if (rc != GM_OK) return rc;
+    if (s->periodic) {  // the wrapped positions back into the caller's arrays (neighbor.rs:13-15)
+        double *const out[3] = {x, y, z};
+        rc = device_to_host(s, dev, out);
+        if (rc != GM_OK) return rc;
     }
-    GM_HIP(hipStreamSynchronize(st));
+    GM_HIP(hipStreamSynchronize(s->stream));
     return GM_OK;
 }
 
@@ -831,13 +919,11 @@ gm_status gm_needs_rebuild(gm_sys *s, const double *x, const double *y, const do
     GM_REQUIRE(s && out, "NULL argument");
     GM_REQUIRE(s->n == 0 || (x && y && z), "NULL positions");
     if (!s->built) return set_error(GM_ERR_NOT_BUILT, "needs_rebuild before build_neighbor_list");
-    const size_t bytes = sizeof(double) * (size_t)s->n;
     hipStream_t st = s->stream;
-    if (s->n > 0) {
-        GM_HIP(hipMemcpyAsync(s->sx.p, x, bytes, hipMemcpyHostToDevice, st));
-        GM_HIP(hipMemcpyAsync(s->sy.p, y, bytes, hipMemcpyHostToDevice, st));
-        GM_HIP(hipMemcpyAsync(s->sz.p, z, bytes, hipMemcpyHostToDevice, st));
-    }
+    const double *const in[3] = {x, y, z};
+    double *const dev[3] = {s->sx.p, s->sy.p, s->sz.p};
+    gm_status rc = host_to_device(s, in, dev);
+    if (rc != GM_OK) return rc;
     GM_HIP(launch_needs_rebuild(s, s->sx.p, s->sy.p, s->sz.p));
     int flag = 0;
     GM_HIP(hipMemcpyAsync(&flag, s->meta.p + 5, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -864,20 +950,16 @@ gm_status gm_compute_forces(gm_sys *s, const double *x, const double *y, const d
                             double *fx, double *fy, double *fz, gm_energy_terms *e) {
     GM_REQUIRE(s, "NULL system");
     GM_REQUIRE(s->n == 0 || (x && y && z && fx && fy && fz), "NULL array");
-    const size_t bytes = sizeof(double) * (size_t)s->n;
-    hipStream_t st = s->stream;
-    if (s->n > 0) {
-        GM_HIP(hipMemcpyAsync(s->x.p, x, bytes, hipMemcpyHostToDevice, st));
-        GM_HIP(hipMemcpyAsync(s->y.p, y, bytes, hipMemcpyHostToDevice, st));
-        GM_HIP(hipMemcpyAsync(s->z.p, z, bytes, hipMemcpyHostToDevice, st));
-    }
-    gm_status rc = compute_on_device(s, s->x.p, s->y.p, s->z.p, s->fx.p, s->fy.p, s->fz.p);
+    const double *const in[3] = {x, y, z};
+    double *const dev[3] = {s->x.p, s->y.p, s->z.p};
+    gm_status rc = host_to_device(s, in, dev);
     if (rc != GM_OK) return rc;
-    if (s->n > 0) {
-        GM_HIP(hipMemcpyAsync(fx, s->fx.p, bytes, hipMemcpyDeviceToHost, st));
-        GM_HIP(hipMemcpyAsync(fy, s->fy.p, bytes, hipMemcpyDeviceToHost, st));
-        GM_HIP(hipMemcpyAsync(fz, s->fz.p, bytes, hipMemcpyDeviceToHost, st));
-    }
+    rc = compute_on_device(s, s->x.p, s->y.p, s->z.p, s->fx.p, s->fy.p, s->fz.p);
+    if (rc != GM_OK) return rc;
+    const double *const fdev[3] = {s->fx.p, s->fy.p, s->fz.p};
+    double *const out[3] = {fx, fy, fz};
+    rc = device_to_host(s, fdev, out);
+    if (rc != GM_OK) return rc;
     rc = collect(s, e);
     return rc != GM_OK ? rc : nonfinite_status(s);
 }
@@ -892,14 +974,11 @@ gm_status gm_compute_forces_device(gm_sys *s, const double *dx, const double *dy
 gm_status gm_set_positions(gm_sys *s, const double *x, const double *y, const double *z) {
     GM_REQUIRE(s, "NULL system");
     GM_REQUIRE(s->n == 0 || (x && y && z), "NULL positions");
-    const size_t bytes = sizeof(double) * (size_t)s->n;
-    hipStream_t st = s->stream;
-    if (s->n > 0) {
-        GM_HIP(hipMemcpyAsync(s->x.p, x, bytes, hipMemcpyHostToDevice, st));
-        GM_HIP(hipMemcpyAsync(s->y.p, y, bytes, hipMemcpyHostToDevice, st));
-        GM_HIP(hipMemcpyAsync(s->z.p, z, bytes, hipMemcpyHostToDevice, st));
-    }
-    GM_HIP(hipStreamSynchronize(st));
+    const double *const in[3] = {x, y, z};
+    double *const dev[3] = {s->x.p, s->y.p, s->z.p};
+    gm_status rc = host_to_device(s, in, dev);
+    if (rc != GM_OK) return rc;
+    GM_HIP(hipStreamSynchronize(s->stream));
     return GM_OK;
 }
 
@@ -911,15 +990,9 @@ gm_status gm_compute_forces_resident(gm_sys *s) {
 gm_status gm_get_forces(gm_sys *s, double *fx, double *fy, double *fz) {
     GM_REQUIRE(s, "NULL system");
     GM_REQUIRE(s->n == 0 || (fx && fy && fz), "NULL array");
-    const size_t bytes = sizeof(double) * (size_t)s->n;
-    hipStream_t st = s->stream;
-    if (s->n > 0) {
-        GM_HIP(hipMemcpyAsync(fx, s->fx.p, bytes, hipMemcpyDeviceToHost, st));
-        GM_HIP(hipMemcpyAsync(fy, s->fy.p, bytes, hipMemcpyDeviceToHost, st));
-        GM_HIP(hipMemcpyAsync(fz, s->fz.p, bytes, hipMemcpyDeviceToHost, st));
-    }
-    GM_HIP(hipStreamSynchronize(st));
-    return GM_OK;
+    const double *const dev[3] = {s->fx.p, s->fy.p, s->fz.p};
+    double *const out[3] = {fx, fy, fz};
+    return device_to_host(s, dev, out);
 }
 
 gm_status gm_sync(gm_sys *s, gm_energy_terms *e) {

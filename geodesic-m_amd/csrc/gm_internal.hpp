@@ -230,6 +230,10 @@ struct gm_sys {
     // working positions (device, original order) and rebuild reference
     DevBuf<double> x, y, z, refx, refy, refz;
     DevBuf<double> fx, fy, fz;  // resident f64 forces (atom order)
+    // x, y, z and fx, fy, fz are views of one allocation each (x at 0, y at n,
+    // z at 2 n), so that a host array set moves in one transfer
+    DevBuf<double> pos3, frc3;
+    double *pinned_e = nullptr;  // 8 doubles of pinned host memory: the energy words read back
     DevBuf<double> sx, sy, sz;  // staging for needs_rebuild positions
 
     // spatial sort
@@ -301,6 +305,13 @@ struct gm_sys {
     int64_t item_cap = 0;                 // work items the work-item buffers hold
     bool stats_pending = false;           // list statistics not yet summed (gm_get_stats sums them)
     bool cols_clean = false;              // column counters left at zero by the last build
+    // host arrays in and out: a pinned staging buffer (3 n doubles each way) and
+    // one event per array, so that the copy of one array into or out of the stage
+    // overlaps the transfer of another
+    double *stage = nullptr;
+    size_t stage_n = 0;
+    hipEvent_t stage_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool stage_in_pending = false;  // stage_ev[3] marks the end of transfers out of the stage
     hipGraph_t bgraph = nullptr;          // the captured sized build and what it was captured with
     hipGraphExec_t bexec = nullptr;
     std::vector<unsigned char> bkey;
