@@ -242,13 +242,15 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
             const float dmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(d)));
             bound = fmaf(dmax, A.prune_scale, A.prune_bias);
         }
+        // the run's mask bytes, 4 entries per row of 64 dwords: only the rows
+        // that hold its entries are loaded (a test the whole wavefront shares);
+        // the last row may run past the run into later entries, which the
+        // entry buffers' spare capacity (1024 entries or more) keeps in bounds
+        // and nothing reads
         const uint32_t *lb = reinterpret_cast<const uint32_t *>(ent_lb + (size_t)e0 * 64);
         uint32_t lbv[kMaxRun / 4];
 #pragma unroll
-        for (int r = 0; r < kMaxRun / 4; r++) {
-            const int k = lane + 64 * r;
-            lbv[r] = k < ne * 16 ? lb[k] : 0u;
-        }
+        for (int r = 0; r < kMaxRun / 4; r++) lbv[r] = r * 4 < ne ? lb[lane + 64 * r] : 0u;
         int2 mv = make_int2(0, 0);
         if (lane < ne) {
             mv = A.ent_meta[e0 + lane];
@@ -267,7 +269,8 @@ __device__ __attribute__((always_inline)) inline void tile_work(const int w, con
         // among them, its own index above the 8 mask bits (its pair-mask bytes
         // stay in place).
 #pragma unroll
-        for (int r = 0; r < kMaxRun / 4; r++) s_lb[lane + 64 * r] = lbv[r];
+        for (int r = 0; r < kMaxRun / 4; r++)
+            if (r * 4 < ne) s_lb[lane + 64 * r] = lbv[r];
         live = __ballot(mv.y != 0);
         if (mv.y != 0) s_meta[__popcll(live & ((1ull << lane) - 1ull))] = make_int2(mv.x, mv.y | (lane << 8));
     }
