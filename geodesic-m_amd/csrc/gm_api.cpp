@@ -314,26 +314,47 @@ std::vector<unsigned char> build_graph_key(gm_sys *s, bool wrap, double *px, dou
 // A sized build of a system with a fixed span (periodic) runs as one captured
 // graph (the same kernels with the same arguments as the stream launches),
 // captured again when anything it depends on changes: the kernels go to the
-// device at once instead of one host launch each.
+// device at once instead of one host launch each.  A system whose arguments
+// change on two builds in a row (positions from alternating arrays, say), or
+// whose capture fails, goes back to stream launches for good.
+void drop_build_graph(gm_sys *s) {
+    if (s->bexec) (void)hipGraphExecDestroy(s->bexec);
+    if (s->bgraph) (void)hipGraphDestroy(s->bgraph);
+    s->bexec = nullptr;
+    s->bgraph = nullptr;
+    s->bkey.clear();
+}
+
 gm_status launch_build_graph(gm_sys *s, bool wrap, double *px, double *py, double *pz) {
     std::vector<unsigned char> key = build_graph_key(s, wrap, px, py, pz);
-    if (!s->bexec || key != s->bkey) {
-        if (s->bexec) (void)hipGraphExecDestroy(s->bexec);
-        if (s->bgraph) (void)hipGraphDestroy(s->bgraph);
-        s->bexec = nullptr;
-        s->bgraph = nullptr;
-        s->bkey.clear();
-        GM_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-        const hipError_t e1 = launch_sized_build(s, wrap, px, py, pz, false);
-        hipGraph_t g = nullptr;
-        const hipError_t e2 = hipStreamEndCapture(s->stream, &g);
-        if (e1 != hipSuccess || e2 != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            return from_hip(e1 != hipSuccess ? e1 : e2, "list build graph capture");
+    if (s->bexec && key == s->bkey) {
+        s->bgraph_misses = 0;
+    } else {
+        if (s->bexec && ++s->bgraph_misses >= 2) s->bgraph_off = true;
+        drop_build_graph(s);
+        if (!s->bgraph_off) {
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                const hipError_t e1 = launch_sized_build(s, wrap, px, py, pz, false);
+                const hipError_t e2 = hipStreamEndCapture(s->stream, &g);
+                e = e1 != hipSuccess ? e1 : e2;
+            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&s->bexec, g, nullptr, nullptr, 0);
+            if (e == hipSuccess) {
+                s->bgraph = g;
+                s->bkey = std::move(key);
+            } else {
+                (void)hipGetLastError();  // clear a capture error
+                if (g) (void)hipGraphDestroy(g);
+                s->bexec = nullptr;
+                s->bgraph_off = true;
+            }
         }
-        s->bgraph = g;
-        GM_HIP(hipGraphInstantiate(&s->bexec, s->bgraph, nullptr, nullptr, 0));
-        s->bkey = std::move(key);
+        if (s->bgraph_off) {
+            GM_HIP(launch_sized_build(s, wrap, px, py, pz, false));
+            return GM_OK;
+        }
     }
     GM_HIP(hipGraphLaunch(s->bexec, s->stream));
     return GM_OK;
@@ -358,7 +379,7 @@ gm_status build_on_device(gm_sys *s, bool wrap, double *px, double *py, double *
                                               std::to_string(kColumnCap));
     };
     GM_HIP(hipEventRecord(s->ev_b0, st));
-    if (s->ent_cap > 0 && clean && !s->fit_span && !s->batch_owned) {
+    if (s->ent_cap > 0 && clean && !s->fit_span && !s->batch_owned) {  // (stream launches if graphs are off)
         gm_status rc = launch_build_graph(s, wrap, px, py, pz);
         if (rc != GM_OK) return rc;
     } else {
@@ -858,10 +879,7 @@ gm_status gm_system_destroy(gm_sys *s) {
     s->sc_half.release();
     s->ent_meta.release();
     s->ent_dmin.release();
-    if (s->bexec) (void)hipGraphExecDestroy(s->bexec);
-    if (s->bgraph) (void)hipGraphDestroy(s->bgraph);
-    s->bexec = nullptr;
-    s->bgraph = nullptr;
+    drop_build_graph(s);
     if (s->stage) (void)hipHostFree(s->stage);
     s->stage = nullptr;
     for (hipEvent_t &ev : s->stage_ev) {

@@ -315,6 +315,8 @@ struct gm_sys {
     hipGraph_t bgraph = nullptr;          // the captured sized build and what it was captured with
     hipGraphExec_t bexec = nullptr;
     std::vector<unsigned char> bkey;
+    int bgraph_misses = 0;                // consecutive builds that needed a new capture
+    bool bgraph_off = false;              // stream launches only (capture failed, or arguments keep changing)
 
     gm::EvalBufs ev{};  // the buffers the launchers use (own_bufs unless a caller gives others)
     bool built = false;

@@ -463,6 +463,32 @@ def test_device_resident_path_equals_host_path(sys24k):
     assert e3.total == e.total
 
 
+def test_builds_from_alternating_device_arrays(sys24k):
+    """Steady-state builds from two caller-owned sets of device arrays used in
+    turn (each build's launch arguments differ from the previous one's), and
+    then from one set again: every list and every evaluation is byte-identical
+    to the host-array build, and the caller's arrays hold the wrapped
+    positions."""
+    s, _ = sys24k
+    say("[gpu] builds from alternating device arrays, 24k")
+    g, (x, y, z), e, f = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params)
+    pi, pj = g.pair_list()
+    sets = [[gh.DeviceArray.from_host(v) for v in (s.x, s.y, s.z)] for _ in range(2)]
+    out = [gh.DeviceArray(len(x)) for _ in range(3)]
+    for k in (0, 1, 0, 1, 0, 0, 0):
+        d = sets[k]
+        g.build_neighbor_list_device(*(a.ptr for a in d))
+        for a, w in zip(d, (x, y, z)):
+            assert a.to_host().tobytes() == w.tobytes()
+        qi, qj = g.pair_list()
+        assert np.array_equal(qi, pi) and np.array_equal(qj, pj), k
+        g.compute_forces_device(*(a.ptr for a in d), *(o.ptr for o in out))
+        e2 = g.sync()
+        for a, b in zip(out, f):
+            assert a.to_host().tobytes() == b.tobytes(), k
+        assert e2.total == e.total
+
+
 def test_250k_full_size_properties():
     """At the full 250k size: the list equals the oracle's, energies and forces
     within the bars, and reruns byte-identical."""
