@@ -489,6 +489,39 @@ def test_builds_from_alternating_device_arrays(sys24k):
         assert e2.total == e.total
 
 
+def test_rebuild_past_the_entry_capacity():
+    """A rebuild whose list needs more entries than the buffers the first build
+    sized (a quarter more than its own entries, plus 1024): the atoms start
+    spread over a box 1.5 times wider, then return to their normal density in
+    one corner of it (20,000 atoms: the xy columns, sized for the mean
+    density, are then fine enough for the corner to hold several per axis).  The build finds its entry count past the capacity, sizes
+    the buffers anew and runs again: list, energies and forces against the
+    oracle, and a second build at the same positions byte-identical."""
+    s = systems.protein_solvent(20000, (20000 / 0.1) ** (1 / 3), 2000, seed=31)
+    L = s.params.box[0] * 1.5
+    p = gh.Params(box=(L,) * 3, periodic=True, r_cutoff=s.params.r_cutoff, r_switch=s.params.r_switch,
+                  r_skin=s.params.r_skin, scale_14=s.params.scale_14, bonded_min_image=s.params.bonded_min_image)
+    say("[gpu] rebuild past the entry capacity")
+    g = gh.System(s.sigma, s.epsilon, s.topo, p)
+    x1, y1, z1 = s.x * 1.5, s.y * 1.5, s.z * 1.5
+    g.build_neighbor_list(x1, y1, z1)
+    ent1 = g.stats()["n_entries"]
+    x, y, z = s.x.copy(), s.y.copy(), s.z.copy()
+    g.build_neighbor_list(x, y, z)
+    ent2 = g.stats()["n_entries"]
+    assert ent2 > ent1 + ent1 // 4 + 1024, (ent1, ent2)  # the capacity of the first build was exceeded
+    (wx, wy, wz), (pi, pj), t, f = oracle_reference(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, p, method="cells")
+    assert x.tobytes() == wx.tobytes()
+    qi, qj = g.pair_list()
+    assert np.array_equal(qi, pi) and np.array_equal(qj, pj)
+    e, *fg = g.compute_forces(x, y, z)
+    check_terms(e, t, "grown list")
+    check_forces(fg, f, "grown list", synthetic=True)
+    g.build_neighbor_list(x, y, z)
+    e2, *fg2 = g.compute_forces(x, y, z)
+    assert e2.total == e.total and all(a.tobytes() == b.tobytes() for a, b in zip(fg, fg2))
+
+
 def test_250k_full_size_properties():
     """At the full 250k size: the list equals the oracle's, energies and forces
     within the bars, and reruns byte-identical."""
