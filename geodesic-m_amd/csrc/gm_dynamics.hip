@@ -57,9 +57,7 @@ __global__ void k_resync(int64_t n, const double *x, const double *y, const doub
     vz[i] = (z[i] - rz[i]) / dt;
 }
 
-// |v| as the bit pattern that orders like the value (a NaN above everything,
-// so it reaches the convergence test as before).
-// |v| as ordered bits, NaN as 0: the reference's `lambda_abs > max_lambda`
+// |v| as the bit pattern that orders like the value, NaN as 0: the reference's `lambda_abs > max_lambda`
 // never takes a NaN (constraint.rs:127-131), so a NaN multiplier does not
 // hold the solve from converging.
 __device__ inline unsigned long long abs_bits(double v) {
