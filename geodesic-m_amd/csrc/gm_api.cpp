@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "gm_internal.hpp"
@@ -92,16 +93,16 @@ gm_status finish_work_items(gm_sys *s, const int *m) {
     s->n_blk = m[kMetaNblk];
     s->fi_records = m[kMetaChunk] < 16;
     if (s->fi_records) GM_HIP(s->fi_blk.ensure((size_t)std::max<int64_t>(s->n_blk, 1) * 192));
-    GM_HIP(s->e_blk.ensure((size_t)std::max<int64_t>(s->n_blk, 1)));
-    s->stats.n_work_items = s->n_wi;
+    GM_HIP(s->e_blk.ensure((size_t)std::max<int64_t>(s->n_blk, 1)));  // LJ energy records, per block
+    if (!s->use_pl) s->stats.n_work_items = s->n_wi;
     return GM_OK;
 }
 
-// New work items for a built list (gm_set_work_chunk, a batch's shape).
+// New work items for a built tile list (gm_set_work_chunk, a batch's shape).
 gm_status make_work_items(gm_sys *s) {
     GM_HIP(launch_sc_scan(s, s->ent_cap, s->chunk));
     GM_HIP(launch_work_items(s));
-    int m[24];
+    int m[kMetaWords];
     GM_HIP(hipMemcpyAsync(m, s->meta.p, sizeof(m), hipMemcpyDeviceToHost, s->stream));
     GM_HIP(hipStreamSynchronize(s->stream));
     return finish_work_items(s, m);
@@ -124,6 +125,26 @@ gm_status size_list_buffers(gm_sys *s, int64_t nent) {
     GM_HIP(s->wi_cost.ensure((size_t)icap));
     s->ent_cap = cap;
     s->item_cap = icap;
+    return GM_OK;
+}
+
+// Partner-list buffers: full entries for about nfent (a quarter more), list
+// entries for about npl (a quarter more, plus one group's worth).
+gm_status size_fent_buffers(gm_sys *s, int64_t nfent) {
+    const int64_t cap = nfent + nfent / 4 + 1024;
+    if (cap > (int64_t)0x7fffffff / 64) return set_error(GM_ERR_CAPACITY, "list build: too many full entries");
+    GM_HIP(s->fent_sc.ensure((size_t)cap));
+    GM_HIP(s->fent_cj.ensure((size_t)cap));
+    GM_HIP(s->fent_w.ensure((size_t)cap * 64));
+    s->fent_cap = cap;
+    return GM_OK;
+}
+
+gm_status size_pl_buffer(gm_sys *s, int64_t npl) {
+    const int64_t cap = npl + npl / 4 + 1024;
+    if (cap + kPlSlack > (int64_t)0x7fffffff) return set_error(GM_ERR_CAPACITY, "list build: partner lists too long");
+    GM_HIP(s->pl_nbr.ensure((size_t)(cap + kPlSlack)));
+    s->pl_cap = cap;
     return GM_OK;
 }
 
@@ -175,6 +196,8 @@ void set_quant_span(gm_sys *s, const double L[3], int ncx, int ncy) {
         const double qmax = std::max(L[0], std::max(L[1], L[2])) / two32;
         bp.band_skin = (float)(1e-4 * bp.r_skin2 + 16.0 * s->rskin * qmax);
         bp.band_cut = (float)(1e-4 * bp.r_cut2 + 16.0 * s->rc * qmax);
+        const double r_in = s->rc + kPlInner;
+        bp.band_in = (float)(1e-4 * r_in * r_in + 16.0 * r_in * qmax);
     }
     bp.wcol[0] = (float)(L[0] / bp.ncx);
     bp.wcol[1] = (float)(L[1] / bp.ncy);
@@ -293,17 +316,29 @@ gm_status device_to_host(gm_sys *s, const double *const src[3], double *const ds
 hipError_t launch_sized_build(gm_sys *s, bool wrap, double *px, double *py, double *pz, bool clear_cols) {
     hipError_t e;
     if ((e = launch_build(s, wrap, px, py, pz, clear_cols)) != hipSuccess) return e;
+    if (s->use_pl) {
+        if ((e = launch_pl_count(s)) != hipSuccess) return e;
+        return launch_pl_fill(s);
+    }
     if ((e = launch_pair_count(s, s->ent_cap, s->chunk)) != hipSuccess) return e;
     if ((e = launch_pair_fill_and_masks(s)) != hipSuccess) return e;
     return launch_work_items(s);
 }
 
+// A build's list buffers are sized (the entry buffers of the layout in use).
+bool build_sized(const gm_sys *s) { return s->use_pl ? s->fent_cap > 0 : s->ent_cap > 0; }
+
 // What the captured build depends on: its launch arguments (positions,
 // buffers that are resized, the work chunk, the span parameters).
 std::vector<unsigned char> build_graph_key(gm_sys *s, bool wrap, double *px, double *py, double *pz) {
-    const void *ptrs[] = {px, py, pz, s->ent_sc.p, s->ent_cj.p, s->ent_imask.p, s->ent_meta.p, s->ent_dmin.p,
-                          s->ent_lb.p, s->wi.p, s->wi_tmp.p, s->wi_cost.p};
-    const int64_t ints[] = {wrap ? 1 : 0, s->chunk, s->ent_cap, s->item_cap, s->n, s->max_sc};
+    const bool pl = s->use_pl;  // (the other layout's buffers are not launch arguments)
+    const void *ptrs[] = {px, py, pz, pl ? nullptr : s->ent_sc.p, pl ? nullptr : s->ent_cj.p,
+                          pl ? nullptr : s->ent_imask.p, pl ? nullptr : s->ent_meta.p, pl ? nullptr : s->ent_dmin.p,
+                          pl ? nullptr : s->ent_lb.p, pl ? nullptr : s->wi.p, pl ? nullptr : s->wi_tmp.p,
+                          pl ? nullptr : s->wi_cost.p, pl ? s->fent_sc.p : nullptr, pl ? s->fent_cj.p : nullptr,
+                          pl ? s->fent_w.p : nullptr, pl ? s->pl_nbr.p : nullptr};
+    const int64_t ints[] = {wrap ? 1 : 0, pl ? 0 : s->chunk, pl ? 0 : s->ent_cap, pl ? 0 : s->item_cap, s->n,
+                            s->max_sc, pl ? 1 : 0, pl ? s->fent_cap : 0, pl ? s->pl_cap : 0};
     std::vector<unsigned char> k(sizeof(ptrs) + sizeof(ints) + sizeof(BuildParams));
     std::memcpy(k.data(), ptrs, sizeof(ptrs));
     std::memcpy(k.data() + sizeof(ptrs), ints, sizeof(ints));
@@ -360,54 +395,135 @@ gm_status launch_build_graph(gm_sys *s, bool wrap, double *px, double *py, doubl
     return GM_OK;
 }
 
+gm_status read_meta(gm_sys *s, int *m) {
+    GM_HIP(hipMemcpyAsync(m, s->meta.p, sizeof(int) * kMetaWords, hipMemcpyDeviceToHost, s->stream));
+    GM_HIP(hipStreamSynchronize(s->stream));
+    return GM_OK;
+}
+
+// Partner lists longer than their buffers (the kernels then skip the rest):
+// size the buffers from the build words of the last pass (m) and make the
+// lists again.  After a full-entry overflow the list length is not known
+// yet, so a second pass may size the list buffer.
+gm_status fit_partner_lists(gm_sys *s, int *m) {
+    for (int pass = 0; pass < 3 && (m[kMetaFentOver] || m[kMetaPlOver]); pass++) {
+        gm_status rc = GM_OK;
+        if (m[kMetaFentOver] && (rc = size_fent_buffers(s, m[kMetaNfent])) != GM_OK) return rc;
+        if (m[kMetaPlOver] && (rc = size_pl_buffer(s, m[kMetaNpl])) != GM_OK) return rc;
+        GM_HIP(launch_pl_count(s));
+        GM_HIP(launch_pl_fill(s));
+        if ((rc = read_meta(s, m)) != GM_OK) return rc;
+    }
+    if (m[kMetaFentOver] || m[kMetaPlOver]) return set_error(GM_ERR_KERNEL, "list build: partner lists overflow");
+    GM_HIP(s->e_blk.ensure((size_t)std::max<int64_t>(s->max_sc * (64 / kPlAtoms), 1)));  // per group
+    s->nfent = m[kMetaNfent];
+    s->npl = m[kMetaNpl];
+    s->pl_built = true;
+    return GM_OK;
+}
+
+// Partner lists for the current build (the partner layout chosen after it).
+// The kernels read the build's binning and build-time positions: the lists
+// are the ones the build would have made.
+gm_status make_partner_lists(gm_sys *s) {
+    int m[kMetaWords];
+    gm_status rc;
+    if (s->fent_cap == 0) {
+        GM_HIP(launch_pl_count(s));
+        if ((rc = read_meta(s, m)) != GM_OK) return rc;
+        if ((rc = size_fent_buffers(s, m[kMetaNfent])) != GM_OK) return rc;
+    }
+    GM_HIP(launch_pl_count(s));
+    GM_HIP(launch_pl_fill(s));
+    if ((rc = read_meta(s, m)) != GM_OK) return rc;
+    return fit_partner_lists(s, m);
+}
+
+// The tile list and work items of the current build.  A build with partner
+// lists leaves them out; they are made here when something asks for them
+// (pair-list export, list statistics, shell-mode evaluations, the tile
+// layout chosen after the build), from the build's binning and build-time
+// positions: the same list the build would have made.
+gm_status ensure_tiles(gm_sys *s) {
+    if (s->tiles_built) return GM_OK;
+    if (!s->built) return set_error(GM_ERR_NOT_BUILT, "tile list before build_neighbor_list");
+    int m[kMetaWords];
+    gm_status rc;
+    for (int pass = 0; pass < 2; pass++) {
+        if (s->ent_cap == 0) {
+            GM_HIP(launch_pair_count(s, 0x7fffffff, s->chunk));
+            if ((rc = read_meta(s, m)) != GM_OK) return rc;
+            if ((rc = size_list_buffers(s, m[kMetaNent])) != GM_OK) return rc;
+            GM_HIP(launch_sc_scan(s, s->ent_cap, s->chunk));
+        } else {
+            GM_HIP(launch_pair_count(s, s->ent_cap, s->chunk));
+        }
+        GM_HIP(launch_pair_fill_and_masks(s));
+        GM_HIP(launch_work_items(s));
+        if ((rc = read_meta(s, m)) != GM_OK) return rc;
+        if (!m[kMetaEntOver]) break;
+        s->ent_cap = 0;  // more entries than the buffers hold: size them for this list
+    }
+    if (m[kMetaEntOver]) return set_error(GM_ERR_KERNEL, "list build: tile entries overflow");
+    s->nent = m[kMetaNent];
+    s->stats.n_entries = s->nent;
+    if ((rc = finish_work_items(s, m)) != GM_OK) return rc;
+    s->tiles_built = true;
+    return GM_OK;
+}
+
 // The list build on the device: binning, sort, split, pair search, scans,
-// pair masks and work items, with one read-back at the end (the build words
-// and counts).  The first build of a system also reads the entry count after
-// the pair search, to size the entry buffers; later builds use that size,
-// and a build whose entries overflow it (the kernels then skip the rest)
-// sizes the buffers anew and runs again.
+// then the partner lists (use_pl) or the tile list's pair masks and work
+// items, with one read-back at the end (the build words and counts).  The
+// first build of a system also reads the entry count after the pair search,
+// to size the entry buffers; later builds use that size, and a build whose
+// entries overflow it (the kernels then skip the rest) sizes the buffers
+// anew and runs again.
 gm_status build_on_device(gm_sys *s, bool wrap, double *px, double *py, double *pz) {
     hipStream_t st = s->stream;
     s->built = false;
+    s->pl_built = false;
+    s->tiles_built = false;
     const bool clean = s->cols_clean;
     s->cols_clean = false;
     if (!px) { px = s->x.p; py = s->y.p; pz = s->z.p; }
-    int m[24];
+    int m[kMetaWords];
+    gm_status rc;
     auto column_overflow = [&]() {
         return set_error(GM_ERR_CAPACITY, "list build: an xy column holds " + std::to_string(m[2]) +
                                               " atoms, more than the LDS sort capacity " +
                                               std::to_string(kColumnCap));
     };
     GM_HIP(hipEventRecord(s->ev_b0, st));
-    if (s->ent_cap > 0 && clean && !s->fit_span && !s->batch_owned) {  // (stream launches if graphs are off)
-        gm_status rc = launch_build_graph(s, wrap, px, py, pz);
-        if (rc != GM_OK) return rc;
+    if (build_sized(s) && clean && !s->fit_span && !s->batch_owned) {  // (stream launches if graphs are off)
+        if ((rc = launch_build_graph(s, wrap, px, py, pz)) != GM_OK) return rc;
     } else {
-        if (s->fit_span && s->n > 0) {
-            gm_status rc = fit_span_to_atoms(s, px, py, pz);
-            if (rc != GM_OK) return rc;
-        }
-        if (s->ent_cap == 0) {
+        if (s->fit_span && s->n > 0 && (rc = fit_span_to_atoms(s, px, py, pz)) != GM_OK) return rc;
+        if (!build_sized(s)) {
             GM_HIP(launch_build(s, wrap, px, py, pz, !clean));
-            GM_HIP(launch_pair_count(s, 0x7fffffff, s->chunk));
-            GM_HIP(hipMemcpyAsync(m, s->meta.p, sizeof(m), hipMemcpyDeviceToHost, st));
-            GM_HIP(hipStreamSynchronize(st));
+            if (s->use_pl) GM_HIP(launch_pl_count(s));
+            else GM_HIP(launch_pair_count(s, 0x7fffffff, s->chunk));
+            if ((rc = read_meta(s, m)) != GM_OK) return rc;
             if (m[1]) return column_overflow();
-            gm_status rc = size_list_buffers(s, m[kMetaNent]);
-            if (rc != GM_OK) return rc;
-            GM_HIP(launch_sc_scan(s, s->ent_cap, s->chunk));
-            GM_HIP(launch_pair_fill_and_masks(s));
-            GM_HIP(launch_work_items(s));
+            if (s->use_pl) {
+                if ((rc = size_fent_buffers(s, m[kMetaNfent])) != GM_OK) return rc;
+                GM_HIP(launch_pl_count(s));
+                GM_HIP(launch_pl_fill(s));
+            } else {
+                if ((rc = size_list_buffers(s, m[kMetaNent])) != GM_OK) return rc;
+                GM_HIP(launch_sc_scan(s, s->ent_cap, s->chunk));
+                GM_HIP(launch_pair_fill_and_masks(s));
+                GM_HIP(launch_work_items(s));
+            }
         } else {
             GM_HIP(launch_sized_build(s, wrap, px, py, pz, !clean));
         }
     }
     s->cols_clean = true;  // the pair-mask kernel is on the stream: it clears them
     GM_HIP(hipEventRecord(s->ev_b1, st));
-    GM_HIP(hipMemcpyAsync(m, s->meta.p, sizeof(m), hipMemcpyDeviceToHost, st));
-    GM_HIP(hipStreamSynchronize(st));
+    if ((rc = read_meta(s, m)) != GM_OK) return rc;
     if (m[1]) return column_overflow();
-    if (m[kMetaEntOver]) {
+    if (!s->use_pl && m[kMetaEntOver]) {
         // more entries than the buffers hold: size them for this list and build
         // again from the system's copy of the (already wrapped) positions
         s->ent_cap = 0;
@@ -415,10 +531,21 @@ gm_status build_on_device(gm_sys *s, bool wrap, double *px, double *py, double *
     }
     s->nsc = m[0];
     s->ncl = s->nsc * kClustersPerSc;
-    s->nent = m[kMetaNent];
     if (s->nsc > s->max_sc) return set_error(GM_ERR_KERNEL, "list build: supercluster count overflow");
-    gm_status rc = finish_work_items(s, m);
-    if (rc != GM_OK) return rc;
+    s->built = true;  // (the binning is complete: the on-demand list kernels may run)
+    if (s->use_pl) {
+        const bool over = m[kMetaFentOver] || m[kMetaPlOver];
+        if ((rc = fit_partner_lists(s, m)) != GM_OK) { s->built = false; return rc; }
+        if (over) drop_build_graph(s);  // its buffers were resized
+        s->nent = 0;
+        s->stats.n_entries = 0;
+        s->stats.n_work_items = s->nsc * (64 / kPlAtoms);
+    } else {
+        s->nent = m[kMetaNent];
+        s->stats.n_entries = s->nent;
+        if ((rc = finish_work_items(s, m)) != GM_OK) { s->built = false; return rc; }
+        s->tiles_built = true;
+    }
     s->F_dirty = false;
     s->stats_pending = true;  // pair and tile counts: summed when asked for (gm_get_stats)
     float ms = 0.f;
@@ -426,9 +553,10 @@ gm_status build_on_device(gm_sys *s, bool wrap, double *px, double *py, double *
     s->stats.n_atoms = s->n;
     s->stats.n_superclusters = s->nsc;
     s->stats.n_clusters = s->ncl;
-    s->stats.n_entries = s->nent;
     s->stats.ms_build = ms;
-    s->built = true;
+    s->stats.pair_layout = s->use_pl ? GM_LAYOUT_PARTNERS : GM_LAYOUT_TILES;
+    s->stats.n_lj_types = s->n_types;
+    s->stats.n_partner_entries = s->use_pl ? s->npl : 0;
     return GM_OK;
 }
 
@@ -436,6 +564,8 @@ gm_status build_on_device(gm_sys *s, bool wrap, double *px, double *py, double *
 // r_cutoff), summed from the pair-mask kernel's partial counts on demand.
 gm_status list_stats(gm_sys *s) {
     if (!s->stats_pending) return GM_OK;
+    gm_status rc = ensure_tiles(s);
+    if (rc != GM_OK) return rc;
     GM_HIP(launch_list_stats(s));
     unsigned long long counters[4] = {0, 0, 0, 0};
     GM_HIP(hipMemcpyAsync(counters, s->meta.p + 8, sizeof(counters), hipMemcpyDeviceToHost, s->stream));
@@ -455,13 +585,17 @@ EvalBufs own_bufs(gm_sys *s) {
     b.disp = s->disp.p;
     b.e_items = s->e_items.p;
     b.e_blk = s->e_blk.p;
-    b.fi_blk = s->fi_records ? s->fi_blk.p : nullptr;
+    b.fi_blk = (s->fi_records && !pl_active(s)) ? s->fi_blk.p : nullptr;
     return b;
 }
 
 gm_status compute_on_device(gm_sys *s, const double *x, const double *y, const double *z,
                             double *fx, double *fy, double *fz, bool shell, const EvalBufs *bufs) {
     if (!s->built) return set_error(GM_ERR_NOT_BUILT, "compute_forces before build_neighbor_list");
+    if (shell || !pl_active(s)) {
+        gm_status rc = ensure_tiles(s);
+        if (rc != GM_OK) return rc;
+    }
     hipStream_t st = s->stream;
     hipEvent_t ev[kProfStride] = {nullptr, nullptr, nullptr, nullptr};
     s->ev = bufs ? *bufs : own_bufs(s);
@@ -488,7 +622,7 @@ gm_status compute_on_device(gm_sys *s, const double *x, const double *y, const d
         if (s->F_dirty) GM_HIP(launch_fill_u64(s->F.p, (size_t)s->nsc * 64 * 3, st));
         s->F_dirty = true;
     }
-    GM_HIP(launch_prep(s, x, y, z, ev[0]));
+    GM_HIP(launch_prep(s, x, y, z, ev[0], !shell && pl_active(s)));
     GM_HIP(launch_forces(s, x, y, z, shell, ev[1], ev[2]));
     GM_HIP(launch_finish(s, fx, fy, fz, shell, ev[3]));  // also clears the accumulator
     if (!bufs) s->F_dirty = false;
@@ -689,6 +823,7 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     bp.r_cut2f = (float)bp.r_cut2;
     bp.prune_rc = (float)d->r_cutoff;
     bp.prune_inv_step = (float)(kPruneSteps / (d->r_skin - d->r_cutoff));
+    bp.r_in2f = (float)((d->r_cutoff + kPlInner) * (d->r_cutoff + kPlInner));
     bp.periodic = s->periodic;
     bp.n_atoms = (int)n;
     // xy columns about the width of a cube holding 64 atoms at the mean density
@@ -718,6 +853,31 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     for (int64_t k = 0; k < n; k++)
         par[(size_t)k] = make_float2((float)(0.5 * d->sigma[k]), (float)std::sqrt(d->epsilon[k]));
     TRY(upload(s->atom_par, par.data(), par.size(), st));
+    {
+        // distinct (sigma/2, sqrt eps) pairs in f32, the values the kernels use
+        std::map<std::pair<uint32_t, uint32_t>, int> ids;
+        std::vector<int> wt((size_t)n);
+        std::vector<float2> tp;
+        for (int64_t k = 0; k < n && (int)ids.size() <= kMaxLjTypes; k++) {
+            uint32_t a, b;
+            std::memcpy(&a, &par[(size_t)k].x, 4);
+            std::memcpy(&b, &par[(size_t)k].y, 4);
+            auto it = ids.emplace(std::make_pair(a, b), (int)ids.size()).first;
+            if (it->second == (int)tp.size()) tp.push_back(make_float2(par[(size_t)k].x, 2.0f * par[(size_t)k].y));
+            wt[(size_t)k] = it->second * 8;
+        }
+        s->n_types = (int)ids.size() <= kMaxLjTypes ? (int)tp.size() : 0;
+        if (s->n_types > 0) {
+            TRY(upload(s->atom_wtype, wt.data(), wt.size(), st));
+            TRY(upload(s->type_par, tp.data(), tp.size(), st));
+        }
+        if (n == 0) {  // (a one-entry table, never read)
+            tp.assign(1, make_float2(0.f, 0.f));
+            TRY(upload(s->type_par, tp.data(), tp.size(), st));
+            s->n_types = 1;
+        }
+        if (s->n_types == 0) s->use_pl = false;
+    }
     TRY(upload(s->bi, d->bond_i, (size_t)s->nb, st));
     TRY(upload(s->bj, d->bond_j, (size_t)s->nb, st));
     TRY(upload(s->bk, d->bond_k, (size_t)s->nb, st));
@@ -804,12 +964,16 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     TRY(s->sorted_atoms.ensure(N));
     TRY(s->atom_slot.ensure(N));
     for (DevBuf<int> *b : {&s->sc_base, &s->sc_m, &s->sc_col, &s->sc_count, &s->sc_start, &s->item_off,
-                           &s->sc_blk_off})
+                           &s->sc_blk_off, &s->fsc_count, &s->fsc_start, &s->pl_scoff})
         TRY(b->ensure((size_t)s->max_sc + 1));
+    TRY(s->pl_cnt.ensure(slots));
+    TRY(s->pl_glen.ensure(slots / kPlAtoms));
+    TRY(s->pl_goff.ensure(slots / kPlAtoms + 1));
+    TRY(s->pl_gloc.ensure(slots / kPlAtoms));
     TRY(s->cost_bins.ensure(2 * (size_t)kCostBins + 2));
     if (s->max_sc > (int64_t)kWiScMask + 1) return fail(set_error(GM_ERR_CAPACITY, "more than 2^26 superclusters"));
-    TRY(s->meta.ensure(24));
-    TRY(launch_fill_i32(s->meta.p, 24, 0, st));
+    TRY(s->meta.ensure(kMetaWords));
+    TRY(launch_fill_i32(s->meta.p, kMetaWords, 0, st));
     TRY(s->slot_atom.ensure(slots));
     TRY(s->sq.ensure(slots));
     TRY(s->sxb.ensure(slots));
@@ -879,6 +1043,14 @@ gm_status gm_system_destroy(gm_sys *s) {
     s->sc_half.release();
     s->ent_meta.release();
     s->ent_dmin.release();
+    // partner lists and the LJ type table
+    for (DevBuf<int> *b : {&s->fsc_count, &s->fsc_start, &s->fent_sc, &s->fent_cj, &s->pl_goff, &s->pl_scoff,
+                           &s->pl_gloc, &s->pl_nbr, &s->atom_wtype})
+        b->release();
+    s->fent_w.release();
+    s->pl_cnt.release();
+    s->pl_glen.release();
+    s->type_par.release();
     drop_build_graph(s);
     if (s->stage) (void)hipHostFree(s->stage);
     s->stage = nullptr;
@@ -1023,6 +1195,8 @@ gm_status gm_get_pair_list(gm_sys *s, uint32_t *pi, uint32_t *pj, int64_t cap, i
     GM_REQUIRE(s && nout, "NULL argument");
     GM_REQUIRE(cap == 0 || (pi && pj), "NULL output with nonzero capacity");
     if (!s->built) return set_error(GM_ERR_NOT_BUILT, "get_pair_list before build_neighbor_list");
+    gm_status rt = ensure_tiles(s);
+    if (rt != GM_OK) return rt;
     hipStream_t st = s->stream;
     const size_t nslots = (size_t)s->nsc * 64, ne = (size_t)s->nent;
     std::vector<int> slot(nslots), esc(ne), ecj(ne);
@@ -1104,8 +1278,24 @@ gm_status gm_set_work_chunk(gm_sys *s, int32_t entries_per_item) {
     GM_REQUIRE(s && entries_per_item >= 0, "bad argument");
     GM_REQUIRE(!s->batch_owned, "gm_set_work_chunk: a batch replica's work shape is set by gm_batch_set_shape");
     s->chunk = entries_per_item;
-    if (!s->built) return GM_OK;
+    if (!s->built || !s->tiles_built) return GM_OK;  // (applies when the tile list is made)
     return make_work_items(s);
+}
+
+gm_status gm_set_pair_layout(gm_sys *s, int32_t layout) {
+    GM_REQUIRE(s && (layout == GM_LAYOUT_TILES || layout == GM_LAYOUT_PARTNERS), "bad argument");
+    GM_REQUIRE(!s->batch_owned, "gm_set_pair_layout: a batch evaluates its replicas' tile lists");
+    const bool pl = layout == GM_LAYOUT_PARTNERS && s->n_types > 0;
+    if (pl == s->use_pl) return GM_OK;
+    s->use_pl = pl;
+    drop_build_graph(s);
+    if (!s->built) return GM_OK;
+    s->stats.pair_layout = pl ? GM_LAYOUT_PARTNERS : GM_LAYOUT_TILES;
+    if (!pl) return ensure_tiles(s);
+    gm_status rc = make_partner_lists(s);
+    if (rc != GM_OK) return rc;
+    s->stats.n_partner_entries = s->npl;
+    return GM_OK;
 }
 
 

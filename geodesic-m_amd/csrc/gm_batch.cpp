@@ -174,6 +174,7 @@ gm_status gm_batch_create(gm_ctx *ctx, const gm_system_desc *descs, int32_t n_re
         gm_status rc = system_create_impl(ctx, &descs[r], b->stream, &s);
         if (rc != GM_OK) return fail(rc);
         s->batch_owned = true;
+        s->use_pl = false;  // the batch evaluates its replicas' tile lists
         b->reps.push_back(s);
         b->off.push_back(b->off.back() + descs[r].n_atoms);
     }

@@ -212,16 +212,16 @@ hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz, bool lj_
     sg.ptr[1] = eb + s->nb;                 sg.len[1] = na;
     sg.ptr[2] = eb + s->nb + s->na;         sg.len[2] = nd;
     sg.ptr[3] = eb + s->nb + s->na + s->nd; sg.len[3] = lj_only ? 0 : n14;
-    sg.ptr[4] = s->ev.e_blk;                sg.len[4] = s->n_blk;
+    sg.ptr[4] = s->ev.e_blk;                sg.len[4] = lj_only ? s->n_blk : lj_records(s);
     sg.nseg = 5;
     const size_t n_pad = (size_t)s->nsc * 64;
     // wavefronts per supercluster: about one block record each, 4 to 16;
     // one when there are no records to add
-    const int64_t per_sc = (s->fi_records && s->nsc > 0) ? (s->n_blk + s->nsc - 1) / s->nsc : 1;
-    const int G = !s->fi_records ? 1 : (per_sc > 8 ? 16 : (per_sc > 4 ? 8 : 4));
+    const float *fi = s->ev.fi_blk;  // (null: no records to add)
+    const int64_t per_sc = (fi && s->nsc > 0) ? (s->n_blk + s->nsc - 1) / s->nsc : 1;
+    const int G = !fi ? 1 : (per_sc > 8 ? 16 : (per_sc > 4 ? 8 : 4));
     const unsigned grid = 5u + (unsigned)((s->nsc + 16 / G - 1) / (16 / G));
     const float fi_scale = span_is_cubic(s) ? s->nbp.fscale_u : 4294967296.0f;
-    const float *fi = s->ev.fi_blk;
     int *bf = bad_first_word(s);
     if (ev_stop)  // stamped by the dispatch at the kernel's end
         hipExtLaunchKernelGGL(k_finish, dim3(grid), dim3(1024), 0, s->stream, nullptr, ev_stop, 0, sg, s->e_out.p,

@@ -465,7 +465,7 @@ __device__ inline int piece_at(int2 p0, int2 p1, int ix) {
 // waits for one load of cluster boxes.  Clusters are numbered column by
 // column, so entries come in ascending cj.  Pass 0 counts, pass 1 writes
 // entries at sc_start[sc], each round's in thread order.
-template <int FILL>
+template <int FILL, bool FULL = false>
 __device__ inline int pair_search_sc(const int *meta, const BuildParams &bp, const int *sc_col,
                                      const int *col_sc, const int4 *cl_ref,
                                      const float4 *cl_half, const int4 *sc_ref, const float4 *sc_half,
@@ -504,8 +504,8 @@ __device__ inline int pair_search_sc(const int *meta, const BuildParams &bp, con
             const float gx = column_gap(cx - cx0, bp.ncx, bp.wcol[0]);
             const float gy = column_gap(cy - cy0, bp.ncy, bp.wcol[1]);
             const int c = cx * bp.ncy + cy;
-            if (c >= c0 && gx * gx + gy * gy <= rs2) {
-                lo = (c == c0) ? first : col_sc[c] * 8;
+            if ((FULL || c >= c0) && gx * gx + gy * gy <= rs2) {
+                lo = (c == c0 && !FULL) ? first : col_sc[c] * 8;
                 n = max(col_sc[c + 1] * 8 - lo, 0);
             }
         }
@@ -528,7 +528,7 @@ __device__ inline int pair_search_sc(const int *meta, const BuildParams &bp, con
                 const float4 jh = cl_half[cj];
                 if (jr.w && boxes_close(sr, sh, jr, jh, bp)) {
                     for (int ci = 0; ci < 8; ci++) {
-                        if (first + ci <= cj && cir[ci].w && boxes_close(cir[ci], cih[ci], jr, jh, bp))
+                        if ((FULL || first + ci <= cj) && cir[ci].w && boxes_close(cir[ci], cih[ci], jr, jh, bp))
                             imask |= 1u << ci;
                     }
                 }
@@ -547,7 +547,7 @@ __device__ inline int pair_search_sc(const int *meta, const BuildParams &bp, con
                 const int off = base + count + woff + __popcll(bl & lt_mask);
                 ent_sc[off] = sc;
                 ent_cj[off] = cj;
-                ent_imask[off] = imask;
+                if (!FULL) ent_imask[off] = imask;
             }
             count += rtot;
             __syncthreads();  // s_wc is rewritten by the next round
@@ -1030,6 +1030,371 @@ __global__ __launch_bounds__(1024) void k_extent(int n, const double *x, const d
     if (t == 7) out[7] = 0.0;
 }
 
+// ---- Partner lists ----------------------------------------------------------
+// The per-slot lists the default nonbonded path walks: each slot's listed
+// partners (the pairs of neighbor::build, neighbor.rs:31-43, in both
+// directions), the partners within r_cutoff + kPlInner at the build first.
+// Built from the full pair search (every cluster near a supercluster, not
+// only those at or above it), in five steps without atomics, so the order of
+// every list is fixed by the data: partners in ascending full-entry order (j
+// cluster), then ascending slot within the cluster.
+
+// Full pair search, pass 0: full entries per supercluster.
+__global__ __launch_bounds__(256) void k_fpair_count(const int *meta, BuildParams bp, const int *sc_col,
+                                                     const int *col_sc, const int4 *cl_ref,
+                                                     const float4 *cl_half, const int4 *sc_ref,
+                                                     const float4 *sc_half, int *fsc_count) {
+    const int sc = blockIdx.x;
+    if (meta[1] || sc >= meta[0]) return;  // the same for the whole block
+    const int count = pair_search_sc<0, true>(meta, bp, sc_col, col_sc, cl_ref, cl_half, sc_ref, sc_half,
+                                              nullptr, nullptr, nullptr, nullptr);
+    if (threadIdx.x == 0) fsc_count[sc] = count;
+}
+
+// Full-entry offsets per supercluster, their total and whether it exceeds the
+// full-entry buffers' capacity.
+__global__ __launch_bounds__(1024) void k_fsc_scan(int *meta, const int *fsc_count, int *fsc_start, int fent_cap) {
+    __shared__ int sh[2 * (1024 / 64)];
+    const int nsc = meta[1] ? 0 : meta[0];
+    const int tot = block_exclusive_scan<1024>(nsc, [&](int k) { return fsc_count[k]; }, fsc_start, sh);
+    if (threadIdx.x == 0) {
+        meta[kMetaNfent] = tot;
+        meta[kMetaFentOver] = tot > fent_cap ? 1 : 0;
+    }
+}
+
+// Full pair search, pass 1: the full entries (supercluster, j cluster).
+__global__ __launch_bounds__(256) void k_fpair_fill(const int *meta, BuildParams bp, const int *sc_col,
+                                                    const int *col_sc, const int4 *cl_ref,
+                                                    const float4 *cl_half, const int4 *sc_ref,
+                                                    const float4 *sc_half, const int *fsc_start, int *fent_sc,
+                                                    int *fent_cj) {
+    const int sc = blockIdx.x;
+    if (meta[1] || sc >= meta[0] || meta[kMetaFentOver]) return;  // the same for the whole block
+    (void)pair_search_sc<1, true>(meta, bp, sc_col, col_sc, cl_ref, cl_half, sc_ref, sc_half, fsc_start,
+                                  fent_sc, fent_cj, nullptr);
+}
+
+// Full entries of a build, in the pair-mask kernels' walk: workgroups dealt to
+// the 8 XCDs in turn, each XCD's wavefronts over a contiguous eighth.
+struct EntryWalk {
+    int e, e_hi, step;
+};
+__device__ inline EntryWalk entry_walk(int n) {
+    const int P = (int)min(gridDim.x, 8u);
+    const int xg = (int)(blockIdx.x % (unsigned)P);
+    const int nwx = (int)((gridDim.x - (unsigned)xg + (unsigned)P - 1u) / (unsigned)P) * 4;
+    const int l = __builtin_amdgcn_readfirstlane((int)(blockIdx.x / (unsigned)P) * 4 + (int)(threadIdx.x >> 6));
+    return EntryWalk{(int)((int64_t)n * xg / P) + l, (int)((int64_t)n * (xg + 1) / P), nwx};
+}
+
+// A supercluster's full-entry words are stored per quad of its slots: the
+// word of slot t (0..63) for the supercluster's f-th full entry (of nf, from
+// f0) is here, so that each quad's words over the entries are contiguous
+// (8 bytes per entry).
+__device__ inline size_t fent_word_at(int f0, int nf, int t, int f) {
+    return (size_t)f0 * 64 + (size_t)(t >> 2) * nf * 4 + (size_t)f * 4 + (t & 3);
+}
+
+// One wavefront per full entry (sc, cj), four per workgroup: lane t is slot
+// i = sc * 64 + t, and the 8 slots j of cluster cj follow one another (the
+// same in every lane: scalar loads).  Bit jl of the lane's word: the pair
+// {i, j} is listed, by exactly the decision k_masks makes for it (i != j,
+// both positions finite, the f32 estimate of r2 or the f64 expression within
+// the band of r_skin^2, not excluded); bit 8 + jl: listed and the estimate at
+// most r_in2f + band_in (the list's first part; a pair in the second part is
+// farther than r_cutoff + kPlInner at the build).  The estimate of a pair does
+// not depend on which slot is i, so both directions get the same decision.
+__global__ __launch_bounds__(256) void k_pl_masks(const int *__restrict__ meta, const int *__restrict__ fent_sc,
+                                                  const int *__restrict__ fent_cj, const int *__restrict__ fsc_start,
+                                                  uint16_t *__restrict__ fent_w,
+                                                  const int *__restrict__ slot_atom, const int4 *__restrict__ sq,
+                                                  const double *__restrict__ x, const double *__restrict__ y,
+                                                  const double *__restrict__ z, const int64_t *__restrict__ excl_off,
+                                                  const uint32_t *__restrict__ excl_nbr,
+                                                  const uint4 *__restrict__ sxb, BuildParams bp,
+                                                  int *__restrict__ col_count) {
+    // the column counters (last read by the column sort) start the next build
+    // at zero (the tile list's mask kernel, when it runs, clears them again)
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < bp.ncol; c += gridDim.x * 256) col_count[c] = 0;
+    const int t = threadIdx.x & 63;
+    const int n = (meta[1] || meta[kMetaFentOver]) ? 0 : meta[kMetaNfent];
+    for (EntryWalk it = entry_walk(n); it.e < it.e_hi; it.e += it.step) {
+        const int e = it.e;
+        const int sc = __builtin_amdgcn_readfirstlane(fent_sc[e]);
+        const int cj = __builtin_amdgcn_readfirstlane(fent_cj[e]);
+        const int f0 = fsc_start[sc], nf = fsc_start[sc + 1] - f0;
+        const size_t si = (size_t)sc * 64 + t;
+        const int4 qa = sq[si];
+        const int a = qa.w ? -1 : slot_atom[si];  // (a non-finite position is never listed)
+        const uint4 xa = sxb[si];
+        uint32_t inb = 0, first = 0, band = 0, xhit = 0, far = 0;
+        int bat[8];
+#pragma unroll
+        for (int jl = 0; jl < 8; jl++) {
+            const size_t sj = (size_t)cj * 8 + jl;
+            const int4 qb = sq[sj];
+            const int b = qb.w ? -1 : slot_atom[sj];
+            const uint4 xb = sxb[sj];
+            bat[jl] = b;
+            if (a >= 0 && b >= 0 && a != b) {
+                const bool ilo = a < b;
+                const int lo = ilo ? a : b, hi = ilo ? b : a;
+                const uint32_t k = (uint32_t)(hi - lo - 1);
+                if (k < 64u) {
+                    const uint32_t w = k < 32u ? (ilo ? xa.x : xb.x) : (ilo ? xa.y : xb.y);
+                    if ((w >> (k & 31u)) & 1u) xhit |= 1u << jl;
+                } else if ((uint32_t)hi <= (ilo ? xa.z : xb.z)) {
+                    far |= 1u << jl;
+                }
+                const float dx = (float)wrapdiff(qb.x, qa.x) * bp.sq[0];
+                const float dy = (float)wrapdiff(qb.y, qa.y) * bp.sq[1];
+                const float dz = (float)wrapdiff(qb.z, qa.z) * bp.sq[2];
+                const float r2f = dx * dx + dy * dy + dz * dz;
+                if (fabsf(r2f - bp.r_skin2f) <= bp.band_skin) band |= 1u << jl;
+                if (r2f <= bp.r_skin2f) inb |= 1u << jl;
+                if (r2f <= bp.r_in2f + bp.band_in) first |= 1u << jl;
+            }
+        }
+        // pairs near the threshold (rare): the f64 expression decides
+        while (band) {
+            const int jl = __ffs(band) - 1;
+            band &= band - 1u;
+            int b = bat[0];
+#pragma unroll
+            for (int c = 1; c < 8; c++) b = jl == c ? bat[c] : b;
+            const uint32_t lo = (uint32_t)min(a, b), hi = (uint32_t)max(a, b);
+            const double ddx = min_image(x[hi] - x[lo], bp.box[0]);
+            const double ddy = min_image(y[hi] - y[lo], bp.box[1]);
+            const double ddz = min_image(z[hi] - z[lo], bp.box[2]);
+            const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            inb = (inb & ~(1u << jl)) | (r2 <= bp.r_skin2 ? (1u << jl) : 0u);
+        }
+        inb &= ~xhit;
+        far &= inb;
+        while (far) {
+            const int jl = __ffs(far) - 1;
+            far &= far - 1u;
+            int b = bat[0];
+#pragma unroll
+            for (int c = 1; c < 8; c++) b = jl == c ? bat[c] : b;
+            const uint32_t lo = (uint32_t)min(a, b), hi = (uint32_t)max(a, b);
+            if (excluded(excl_off, excl_nbr, lo, hi)) inb &= ~(1u << jl);
+        }
+        fent_w[fent_word_at(f0, nf, t, e - f0)] = (uint16_t)(inb | ((inb & first) << 8));
+    }
+}
+
+// One workgroup of 16 wavefronts per supercluster, lane t = slot sc * 64 + t:
+// wavefront w sums the lane's partners of both parts over the full entries
+// f0 + w, f0 + w + 16, ...; the 16 sums are added per slot through LDS in
+// order, per group of kPlAtoms slots the parts' lengths are padded to the
+// group's longest, rounded up to kPlPad, with each group's list offset in
+// the supercluster (gloc), and the supercluster's lists' total length goes to
+// sc_len.
+__global__ __launch_bounds__(1024) void k_pl_count(const int *__restrict__ meta, const int *__restrict__ fsc_start,
+                                                   const uint16_t *__restrict__ fent_w, int2 *__restrict__ pl_cnt,
+                                                   int2 *__restrict__ pl_glen, int *__restrict__ gloc,
+                                                   int *__restrict__ sc_len) {
+    __shared__ int2 part[16][64];
+    const int sc = blockIdx.x;
+    if (meta[1] || meta[kMetaFentOver] || sc >= meta[0]) return;  // the same for the whole block
+    const int t = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int f0 = fsc_start[sc], nf = fsc_start[sc + 1] - f0;
+    int ni = 0, no = 0;
+#pragma unroll 4
+    for (int f = wv; f < nf; f += 16) {
+        const uint32_t v = fent_w[fent_word_at(f0, nf, t, f)];
+        const uint32_t fi = (v >> 8) & 255u;
+        ni += __popc(fi);
+        no += __popc(v & 255u & ~fi);
+    }
+    part[wv][t] = make_int2(ni, no);
+    __syncthreads();
+    if (wv != 0) return;
+    int si1 = 0, so1 = 0;
+    for (int u = 0; u < 16; u++) {
+        si1 += part[u][t].x;
+        so1 += part[u][t].y;
+    }
+    pl_cnt[(size_t)sc * 64 + t] = make_int2(si1, so1);
+    // group maxima over the group's lanes (a group is kPlAtoms consecutive slots)
+    static_assert(kPlAtoms == 2 || kPlAtoms == 4, "groups of 2 or 4 slots");
+    int mi = si1, mo = so1;
+    mi = max(mi, __builtin_amdgcn_update_dpp(0, mi, 0xB1, 0xF, 0xF, false));  // quad_perm [1,0,3,2]
+    mo = max(mo, __builtin_amdgcn_update_dpp(0, mo, 0xB1, 0xF, 0xF, false));
+    if (kPlAtoms == 4) {
+        mi = max(mi, __builtin_amdgcn_update_dpp(0, mi, 0x4E, 0xF, 0xF, false));  // quad_perm [2,3,0,1]
+        mo = max(mo, __builtin_amdgcn_update_dpp(0, mo, 0x4E, 0xF, 0xF, false));
+    }
+    const int pi = (mi + kPlPad - 1) / kPlPad * kPlPad, po = (mo + kPlPad - 1) / kPlPad * kPlPad;
+    if ((t % kPlAtoms) == 0) pl_glen[(size_t)sc * (64 / kPlAtoms) + t / kPlAtoms] = make_int2(pi, pi + po);
+    // groups' offsets in the supercluster and its total: each group's
+    // kPlAtoms lists counted once (by its first lane), scanned over the lanes
+    const int own = (t % kPlAtoms) == 0 ? (pi + po) * kPlAtoms : 0;
+    int inc = own;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(inc, o, 64);
+        if (t >= o) inc += u;
+    }
+    if ((t % kPlAtoms) == 0) gloc[(size_t)sc * (64 / kPlAtoms) + t / kPlAtoms] = inc - own;
+    if (t == 63) sc_len[sc] = inc;
+}
+
+// One block: list offsets per supercluster (exclusive scan of the
+// superclusters' list lengths), the lists' total length and whether it
+// exceeds the list buffer (pl_cap entries before its slack; -1 when there is
+// no buffer), and the buffer's slack after the lists (valid slots, read ahead
+// by the force kernel and never used).
+__global__ __launch_bounds__(1024) void k_pl_scan(int *meta, const int *sc_len, int *sc_off, int pl_cap,
+                                                  int *pl_nbr) {
+    __shared__ int sh[2 * (1024 / 64)];
+    __shared__ long long sh64[1024 / 64];
+    const bool ok = !(meta[1] || meta[kMetaFentOver]);
+    const int nsc = ok ? meta[0] : 0;
+    // the total in 64 bits first: lists too long for 32-bit offsets are
+    // reported (length INT_MAX, overflow) before any offset is used
+    long long part = 0;
+    for (int k = threadIdx.x; k < nsc; k += 1024) part += sc_len[k];
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+    if ((threadIdx.x & 63) == 0) sh64[threadIdx.x >> 6] = part;
+    __syncthreads();
+    long long tot64 = 0;
+    for (int w = 0; w < 1024 / 64; w++) tot64 += sh64[w];
+    if (tot64 > 0x7fffffffLL - kPlSlack) {
+        if (threadIdx.x == 0) {
+            meta[kMetaNpl] = 0x7fffffff;
+            meta[kMetaPlOver] = 1;
+        }
+        return;
+    }
+    const int tot = block_exclusive_scan<1024>(nsc, [&](int k) { return sc_len[k]; }, sc_off, sh);
+    const bool over = ok && tot > pl_cap;
+    if (threadIdx.x == 0) {
+        meta[kMetaNpl] = tot;
+        meta[kMetaPlOver] = over ? 1 : 0;
+    }
+    if (ok && !over && threadIdx.x < kPlSlack) pl_nbr[tot + threadIdx.x] = 0;
+}
+
+// One workgroup of 64 threads per 4 slots (16 threads each): the
+// supercluster's full entries (j cluster and the 4 slots' words) are
+// staged in LDS; thread (slot a, k) takes the k-th sixteenth of them, counts
+// its slot's partners of both parts there, and after a scan over the 16
+// threads writes them (partner slot cj * 8 + jl, in ascending entry and jl)
+// at their places in the slot's list, and the padding (the slot itself) after
+// each part.  The slots' lists are assembled in LDS and leave with
+// coalesced stores.  A supercluster with more full entries than the LDS
+// stage holds, or lists longer than the LDS buffer, is read from and written
+// to memory directly.
+constexpr int kPlMakeSlots = 4;
+static_assert(kPlMakeSlots % kPlAtoms == 0, "whole groups per workgroup");
+constexpr int kPlMakeLds = 4096;   // list entries assembled in LDS
+constexpr int kPlMakeWords = 512;  // full entries staged in LDS
+
+// The writing half of k_pl_make, into `out` (the LDS buffer or the list
+// itself; a separate instance for each, so that LDS stores stay LDS stores).
+// word(f) and cj8(f) give the slot's word and the cluster's first slot of the
+// supercluster's f-th full entry.
+template <typename W, typename C>
+__device__ inline void pl_make_write(int *out, int fa, int fb, int pi, int po, int2 cnt, int2 gl, int sbase, int k,
+                                     int slot, W word, C cj8) {
+    for (int f = fa; f < fb; f++) {
+        const uint32_t v = word(f);
+        if ((v & 255u) == 0u) continue;
+        const int c8 = cj8(f);
+#pragma unroll
+        for (int jl = 0; jl < 8; jl++) {
+            if ((v >> jl) & 1u) {
+                const bool fp = (v >> (8 + jl)) & 1u;
+                out[fp ? pi : po] = c8 + jl;
+                pi += fp ? 1 : 0;
+                po += fp ? 0 : 1;
+            }
+        }
+    }
+    // padding of both parts: the slot itself
+    for (int p = cnt.x + k; p < gl.x; p += 16) out[sbase + p] = slot;
+    for (int p = gl.x + cnt.y + k; p < gl.y; p += 16) out[sbase + p] = slot;
+}
+
+__global__ __launch_bounds__(64) void k_pl_make(const int *__restrict__ meta, const int *__restrict__ fsc_start,
+                                                 const int *__restrict__ fent_cj, const uint16_t *__restrict__ fent_w,
+                                                 const int2 *__restrict__ pl_cnt, const int *__restrict__ sc_off,
+                                                 const int2 *__restrict__ pl_glen, const int *__restrict__ gloc,
+                                                 int *__restrict__ pl_goff, int *__restrict__ pl_nbr) {
+    __shared__ int buf[kPlMakeLds];
+    __shared__ uint2 words[kPlMakeWords];  // per full entry, the 4 slots' words
+    __shared__ int cjs[kPlMakeWords];
+    const int sc = blockIdx.x / (64 / kPlMakeSlots), q = blockIdx.x % (64 / kPlMakeSlots);
+    if (meta[1] || meta[kMetaFentOver] || meta[kMetaPlOver] || sc >= meta[0]) return;  // whole block
+    const int th = threadIdx.x, a = th >> 4, k = th & 15;
+    const int slot = sc * 64 + q * kPlMakeSlots + a;
+    const int g0 = (sc * 64 + q * kPlMakeSlots) / kPlAtoms;  // first group of the workgroup
+    const int g = slot / kPlAtoms;
+    // the workgroup's lists start at the supercluster's offset plus its first
+    // group's offset in the supercluster; the groups' offsets (for the force
+    // kernel) follow
+    const int base0 = sc_off[sc] + gloc[g0];
+    int len = 0, gofs = 0;  // (gofs: group g's offset from base0)
+#pragma unroll
+    for (int u = g0; u < g0 + kPlMakeSlots / kPlAtoms; u++) {
+        if (th == 0) pl_goff[u] = base0 + len;
+        if (u < g) gofs += kPlAtoms * pl_glen[u].y;
+        len += kPlAtoms * pl_glen[u].y;
+    }
+    const int2 gl = pl_glen[g];
+    const int2 cnt = pl_cnt[slot];
+    const int sbase = gofs + (slot % kPlAtoms) * gl.y;  // the slot's list, from the workgroup's start
+    const int f0 = fsc_start[sc], nf = fsc_start[sc + 1] - f0;
+    const int fa = (int)((int64_t)nf * k / 16), fb = (int)((int64_t)nf * (k + 1) / 16);
+    const bool staged = nf <= kPlMakeWords;
+    if (staged) {
+        static_assert(kPlMakeSlots == 4, "one quad of slots per workgroup (the words' layout)");
+        const uint2 *qw = reinterpret_cast<const uint2 *>(fent_w + fent_word_at(f0, nf, q * 4, 0));
+        for (int f = th; f < nf; f += 64) {
+            words[f] = qw[f];
+            cjs[f] = fent_cj[f0 + f];
+        }
+        __syncthreads();
+    }
+    // count, scan over the slot's 16 threads, write: with the words and
+    // clusters from LDS (staged) or from memory
+    auto run = [&](auto word, auto cj8) {
+        int ci = 0, co = 0;
+        for (int f = fa; f < fb; f++) {
+            const uint32_t v = word(f);
+            const uint32_t fi = (v >> 8) & 255u;
+            ci += __popc(fi);
+            co += __popc(v & 255u & ~fi);
+        }
+        // exclusive scan of (ci, co) over the 16 threads of the slot (a row of 16 lanes)
+        int pi = ci, po = co;
+        for (int o = 1; o < 16; o <<= 1) {
+            const int ti = __shfl_up(pi, o, 16), to = __shfl_up(po, o, 16);
+            if (k >= o) { pi += ti; po += to; }
+        }
+        pi += sbase - ci;
+        po += sbase + gl.x - co;
+        if (len > kPlMakeLds)  // (the same for the whole block)
+            pl_make_write(pl_nbr + base0, fa, fb, pi, po, cnt, gl, sbase, k, slot, word, cj8);
+        else
+            pl_make_write(buf, fa, fb, pi, po, cnt, gl, sbase, k, slot, word, cj8);
+    };
+    const int sh = 16 * (a & 1), t = slot - sc * 64;
+    if (staged)
+        run([&](int f) -> uint32_t { return ((a < 2 ? words[f].x : words[f].y) >> sh) & 0xFFFFu; },
+            [&](int f) { return cjs[f] * 8; });
+    else
+        run([&](int f) -> uint32_t { return fent_w[fent_word_at(f0, nf, t, f)]; },
+            [&](int f) { return fent_cj[f0 + f] * 8; });
+    if (len > kPlMakeLds) return;
+    __syncthreads();
+    int4 *dst = reinterpret_cast<int4 *>(pl_nbr + base0);  // (offsets are multiples of kPlPad entries)
+    const int4 *src = reinterpret_cast<const int4 *>(buf);
+    for (int i = th; i < len / 4; i += 64) dst[i] = src[i];
+}
+
 // sigma/2 in the force kernel's length unit: fixed-point steps for cubic boxes, else A
 inline float kernel_sigma_scale(const gm_sys *s) {
     return span_is_cubic(s) ? s->nbp.inv_s : 1.0f;
@@ -1135,9 +1500,45 @@ hipError_t launch_pair_fill_and_masks(gm_sys *s) {
                                                          s->sc_start.p, s->ent_sc.p, s->ent_cj.p, s->ent_imask.p);
     const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((s->ent_cap + 3) / 4, kMaskBlocks));
     k_masks<<<(unsigned)blocks, 256, 0, st>>>(s->meta.p, s->ent_sc.p, s->ent_cj.p, s->ent_imask.p, s->ent_meta.p,
-                                              s->ent_dmin.p, s->ent_lb.p, s->slot_atom.p, s->sq.p, s->x.p, s->y.p,
-                                              s->z.p, s->excl_off.p, s->excl_nbr.p, s->sxb.p, s->bp,
+                                              s->ent_dmin.p, s->ent_lb.p, s->slot_atom.p, s->sq.p, s->refx.p,
+                                              s->refy.p, s->refz.p, s->excl_off.p, s->excl_nbr.p, s->sxb.p, s->bp,
                                               s->col_count.p);
+    return hipGetLastError();
+}
+
+hipError_t launch_pl_count(gm_sys *s) {
+    hipStream_t st = s->stream;
+    if (s->max_sc > 0)
+        k_fpair_count<<<(unsigned)s->max_sc, 256, 0, st>>>(s->meta.p, s->bp, s->sc_col.p, s->col_sc.p, s->cl_ref.p,
+                                                           s->cl_half.p, s->sc_ref.p, s->sc_half.p, s->fsc_count.p);
+    k_fsc_scan<<<1, 1024, 0, st>>>(s->meta.p, s->fsc_count.p, s->fsc_start.p,
+                                   (int)std::min<int64_t>(s->fent_cap, 0x7fffffff));
+    return hipGetLastError();
+}
+
+hipError_t launch_pl_fill(gm_sys *s) {
+    hipStream_t st = s->stream;
+    if (s->max_sc > 0)
+        k_fpair_fill<<<(unsigned)s->max_sc, 256, 0, st>>>(s->meta.p, s->bp, s->sc_col.p, s->col_sc.p, s->cl_ref.p,
+                                                          s->cl_half.p, s->sc_ref.p, s->sc_half.p, s->fsc_start.p,
+                                                          s->fent_sc.p, s->fent_cj.p);
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((s->fent_cap + 3) / 4, kMaskBlocks));
+    k_pl_masks<<<(unsigned)blocks, 256, 0, st>>>(s->meta.p, s->fent_sc.p, s->fent_cj.p, s->fsc_start.p, s->fent_w.p,
+                                                 s->slot_atom.p, s->sq.p, s->refx.p, s->refy.p, s->refz.p,
+                                                 s->excl_off.p, s->excl_nbr.p, s->sxb.p, s->bp, s->col_count.p);
+    if (s->max_sc > 0)
+        k_pl_count<<<(unsigned)s->max_sc, 1024, 0, st>>>(s->meta.p, s->fsc_start.p, s->fent_w.p, s->pl_cnt.p,
+                                                         s->pl_glen.p, s->pl_gloc.p, s->fsc_count.p);
+    // (no list buffer yet: capacity -1, so that even empty lists overflow and
+    // the host allocates it, with the slack the force kernel reads)
+    // (sc_len and sc_off reuse the full-entry count and scan buffers: the
+    // full-entry offsets are in fsc_start, and the counts were only scanned)
+    k_pl_scan<<<1, 1024, 0, st>>>(s->meta.p, s->fsc_count.p, s->pl_scoff.p,
+                                  s->pl_nbr.p ? (int)std::min<int64_t>(s->pl_cap, 0x7fffffff) : -1, s->pl_nbr.p);
+    if (s->max_sc > 0)
+        k_pl_make<<<(unsigned)(s->max_sc * (64 / kPlMakeSlots)), 64, 0, st>>>(
+            s->meta.p, s->fsc_start.p, s->fent_cj.p, s->fent_w.p, s->pl_cnt.p, s->pl_scoff.p, s->pl_glen.p,
+            s->pl_gloc.p, s->pl_goff.p, s->pl_nbr.p);
     return hipGetLastError();
 }
 
@@ -1156,8 +1557,8 @@ hipError_t launch_list_stats(gm_sys *s) {
     if (e != hipSuccess) return e;
     unsigned long long *counters = (unsigned long long *)(s->meta.p + 8);
     k_list_counts<<<(unsigned)blocks, 256, 0, s->stream>>>(s->meta.p, s->ent_sc.p, s->ent_meta.p, s->ent_lb.p,
-                                                            s->slot_atom.p, s->sq.p, s->x.p, s->y.p, s->z.p, s->bp,
-                                                            s->mask_parts.p);
+                                                            s->slot_atom.p, s->sq.p, s->refx.p, s->refy.p, s->refz.p,
+                                                            s->bp, s->mask_parts.p);
     k_sum_parts<<<1, 1024, 0, s->stream>>>(s->mask_parts.p, blocks * 4, counters);
     return hipGetLastError();
 }

@@ -46,6 +46,28 @@ constexpr int kLdsColumns = 8192;
 constexpr int kMetaNent = 16, kMetaEntOver = 17, kMetaChunk = 18, kMetaNwi = 19, kMetaNblk = 20;
 // work-item costs (half tiles) lie in [8, 8 + kMaxRun * 17]: one sort bin each
 constexpr int kCostBins = 8 + kMaxRun * 17 + 1;
+// Partner lists (the default nonbonded path): every slot's listed partners,
+// both directions, from the full pair search (entries (sc, cj) for every
+// cluster cj near the supercluster).  build words: full entries, their
+// capacity overflow, partner-list length (padding included), its overflow
+constexpr int kMetaNfent = 21, kMetaFentOver = 22, kMetaNpl = 23, kMetaPlOver = 24;
+constexpr int kMetaWords = 32;
+// A partner-list wavefront evaluates kPlAtoms consecutive slots (a group),
+// kPlLanes lanes each; lane l of a slot takes the list's entries l, l + 16, ...
+constexpr int kPlAtoms = 4, kPlLanes = 16;
+// Each slot's list holds first the partners within r_cutoff + kPlInner (A)
+// at the build, then the others; both parts are padded with the slot itself
+// (r2 = 0: skipped) to the group's longest part rounded up to kPlPad (whole
+// pairs of rounds: the force kernel takes two rounds per iteration).
+constexpr int kPlPad = 2 * kPlLanes;
+// The force kernel reads list codes up to three rounds past a part: the list
+// buffer holds this many valid entries (slot 0) after the lists.
+constexpr int kPlSlack = 4 * kPlLanes;
+// The partner-list path reads LJ parameters from a table of the system's
+// distinct (sigma/2, 2 sqrt eps) pairs in LDS, at most this many (else the
+// system evaluates its tile list).
+constexpr int kMaxLjTypes = 256;
+constexpr double kPlInner = 0.1;
 // Fixed-point force unit: 2^-32 kcal/mol/A per integer step.
 constexpr double kFixedScale = 4294967296.0;
 constexpr double kFixedInv = 1.0 / 4294967296.0;
@@ -85,6 +107,8 @@ struct BuildParams {
     int kx, ky;         // column reach: columns more than k apart hold no pair within r_search
     float prune_rc;     // r_cutoff, A
     float prune_inv_step;  // 1 / width of one tile-distance byte step, 1/A
+    float r_in2f;       // (r_cutoff + kPlInner)^2: a partner's list part
+    float band_in;      // estimate at or below r_in2f + band_in: the first part
     int ncx, ncy, ncol;
     int periodic;
     int n_atoms;
@@ -303,6 +327,29 @@ struct gm_sys {
     DevBuf<int> cost_bins;                // sort bins: counts, then offsets (heaviest first)
     int64_t ent_cap = 0;                  // entries the entry buffers hold (0: size them at the next build)
     int64_t item_cap = 0;                 // work items the work-item buffers hold
+    bool tiles_built = false;             // the tile list and work items of the last build exist
+    // (a build with partner lists makes the tile list only when something
+    // asks for it: pair-list export, statistics, shell-mode evaluations)
+    // partner lists (built with the list when use_pl): the full pair search's
+    // entries, per (full entry, lane) the listed and first-part masks over the
+    // 8 j slots, per slot its two parts' counts, per group its offset and
+    // padded part lengths
+    bool use_pl = true;                   // nonbonded evaluation over partner lists (false: tiles)
+    int n_types = 0;                      // distinct (sigma, eps) pairs, 0 if more than kMaxLjTypes
+    DevBuf<int> atom_wtype;               // per atom: its type's byte offset in the table (8 B per type)
+    DevBuf<float2> type_par;              // per type: sigma/2 (A), 2 sqrt(eps), f32
+    bool pl_built = false;                // the last build made partner lists
+    DevBuf<int> fsc_count, fsc_start;     // full entries per supercluster, their offsets (nsc + 1)
+    DevBuf<int> fent_sc, fent_cj;
+    DevBuf<uint16_t> fent_w;              // listed mask | first-part mask << 8
+    DevBuf<int2> pl_cnt;                  // per slot: partners in the first part, in the second
+    DevBuf<int> pl_goff;                  // per group: list offset
+    DevBuf<int> pl_scoff;                 // per supercluster: list offset (nsc + 1)
+    DevBuf<int> pl_gloc;                  // per group: list offset in its supercluster
+    DevBuf<int2> pl_glen;                 // per group: padded first-part length, whole padded length
+    DevBuf<int> pl_nbr;                   // partner slots
+    int64_t fent_cap = 0, pl_cap = 0;     // capacities (0: size at the next build)
+    int64_t nfent = 0, npl = 0;
     bool stats_pending = false;           // list statistics not yet summed (gm_get_stats sums them)
     bool cols_clean = false;              // column counters left at zero by the last build
     // host arrays in and out: a pinned staging buffer (3 n doubles each way) and
@@ -375,6 +422,10 @@ void set_quant_span(gm_sys *s, const double L[3], int ncx, int ncy);
 void refresh_shell(gm_sys *s);
 // cubic fixed-point span (the fast tile variant works in step units)
 inline bool span_is_cubic(const gm_sys *s) { return s->qbox[0] == s->qbox[1] && s->qbox[1] == s->qbox[2]; }
+// plain evaluations go over the partner lists (else the tile list)
+inline bool pl_active(const gm_sys *s) { return s->use_pl && s->pl_built && s->n_types > 0; }
+// LJ energy records of a plain evaluation: one per partner-list group, or one per fixed-point block
+inline int64_t lj_records(const gm_sys *s) { return pl_active(s) ? s->nsc * (64 / kPlAtoms) : s->n_blk; }
 // the first atom whose force was not finite in the last evaluation (INT_MAX if none)
 inline int *bad_first_word(gm_sys *s) { return reinterpret_cast<int *>(s->e_out.p + 6); }
 }  // namespace gm
@@ -393,6 +444,11 @@ hipError_t launch_pair_count(gm_sys *s, int64_t ent_cap, int chunk_req);
 // entries per item, 0 automatic), work-item and block offsets: into meta
 hipError_t launch_sc_scan(gm_sys *s, int64_t ent_cap, int chunk_req);
 hipError_t launch_pair_fill_and_masks(gm_sys *s);
+// partner lists: full pair search counts and their scan (full-entry count and
+// overflow into meta), then the masks, per-slot counts, group offsets
+// (length and overflow into meta), the lists and their padding
+hipError_t launch_pl_count(gm_sys *s);
+hipError_t launch_pl_fill(gm_sys *s);
 // heaviest-first work items from the scanned list (after launch_sc_scan)
 hipError_t launch_work_items(gm_sys *s);
 // list statistics (pairs, tiles, within r_cutoff) into meta[8..15], counted on demand
@@ -405,7 +461,10 @@ hipError_t launch_needs_rebuild(gm_sys *s, const double *x, const double *y, con
 // gm_nonbonded.hip: tiles + bonded terms in one launch
 // (shell: tiles only, with nbp_shell)
 // (after launch_prep: slot-order fixed-point positions and the displacement bound)
-hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double *z, hipEvent_t ev_start = nullptr);
+// types: the slot word w is the atom's type-table offset (partner-list
+// path) instead of sigma/2 (tile path)
+hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double *z, hipEvent_t ev_start = nullptr,
+                       bool types = false);
 // ev_start / ev_stop (optional): stamped by the dispatch at the kernel's start and end
 hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z, bool shell = false,
                          hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);

@@ -51,8 +51,9 @@ extern "C" gm_status gm_compute_forces_device_noalloc(gm_sys *s, const double *d
     const size_t b_bad = align256(sizeof(unsigned) * n_pad);
     const size_t b_disp = align256(sizeof(unsigned) * kDispSlots * kDispStride);
     const size_t b_items = align256(sizeof(double) * (size_t)(s->nb + s->na + s->nd + s->n14));
-    const size_t b_blk = align256(sizeof(double) * (size_t)s->n_blk);
-    const size_t b_rec = s->fi_records ? align256(sizeof(float) * 192 * (size_t)s->n_blk) : 0;
+    const bool recs = s->fi_records && !pl_active(s);  // i-force records (tile path, small work items)
+    const size_t b_blk = align256(sizeof(double) * (size_t)lj_records(s));
+    const size_t b_rec = recs ? align256(sizeof(float) * 192 * (size_t)s->n_blk) : 0;
     const size_t need = b_F + b_bad + b_disp + b_items + b_blk + b_rec;
     if (need > s->ctx->scratch_bytes)
         return set_error(GM_ERR_CAPACITY, "gm_compute_forces_device_noalloc: this list needs " +
@@ -74,7 +75,7 @@ extern "C" gm_status gm_compute_forces_device_noalloc(gm_sys *s, const double *d
     o += b_items;
     b.e_blk = reinterpret_cast<double *>(base + o);
     o += b_blk;
-    b.fi_blk = s->fi_records ? reinterpret_cast<float *>(base + o) : nullptr;
+    b.fi_blk = recs ? reinterpret_cast<float *>(base + o) : nullptr;
     GM_HIP(launch_fill_u64(b.F, zeroed / sizeof(unsigned long long), s->stream));
     // the system's own accumulator is untouched (still zero for its next evaluation)
     return compute_on_device(s, dx, dy, dz, dfx, dfy, dfz, false, &b);

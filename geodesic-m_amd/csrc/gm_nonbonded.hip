@@ -140,7 +140,8 @@ __device__ inline float prep_atom(const PrepArgs &P, int a) {
     const int qy = quantize_fixed(y, P.iq[1]);
     const int qz = quantize_fixed(z, P.iq[2]);
     const bool fin = isfinite(x) && isfinite(y) && isfinite(z);
-    P.qpos[s] = make_int4(qx, qy, qz, fin ? __float_as_int(sh) : (int)0x80000000);
+    const int w = P.wtype ? (P.wtype[a] | (fin ? 0 : (int)0x80000000)) : (fin ? __float_as_int(sh) : (int)0x80000000);
+    P.qpos[s] = make_int4(qx, qy, qz, w);
     if (!fin) {
         P.bad[s] = 1u;
         return INFINITY;
@@ -512,6 +513,226 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
     tile_work<CUBIC, SHELL>((int)blockIdx.x - n_bonded_groups, lane, A);
 }
 
+// ---- Partner-list path (the default) -----------------------------------------
+// One wavefront per group of kPlAtoms consecutive slots, kPlLanes lanes per
+// slot: lane l of a slot takes its list's entries l, l + 16, ... and keeps the
+// slot's force in registers, so no force leaves a lane until the end, and
+// every lane of a round evaluates a listed pair (the lists are dense; their
+// padding is the slot itself, r2 = 0, skipped).  Each pair is evaluated from
+// both of its slots with the same operands, so both slots see the same f32
+// force (third law) and energy; the energy sums are halved.  A lane's f32 sum
+// runs over fixed list positions and the 16 lanes combine in a fixed order,
+// so results depend only on the lists, not on scheduling.
+//
+// Pruning: while no atom has moved farther than (kPlInner - 1e-4 r_cutoff) / 2
+// since the build, every partner of a list's second part (farther than
+// r_cutoff + kPlInner at the build) is now beyond r_cutoff, where the pair's
+// terms are exactly zero: only the first part is evaluated.
+struct PlArgs {
+    const int4 *qpos;    // per slot: fixed-point x, y, z; w = the atom's type-table offset
+    const float2 *type_par;  // per type: sigma/2 (A), 2 sqrt(eps)
+    int n_types;
+    float sig_scale;     // sigma/2 in the kernel's length unit: A * sig_scale
+    const int *goff;     // per group: list offset
+    const int2 *glen;    // per group: padded first-part length, whole padded length
+    const int *nbr;      // partner slots
+    const unsigned *disp;  // displacement words (null: no pruning)
+    float dmax_first;      // evaluate the first part only while the displacement is at most this
+    unsigned long long *F;
+    unsigned *bad;
+    double *e_grp;       // per group: half its LJ energy sum
+    int n_pad;
+    int n_grp;
+    NbParams p;
+};
+
+// Group evaluated by tile workgroup k (0-based) of a launch whose first W0
+// workgroups do other work: workgroup w runs on XCD w % 8, and XCD x takes a
+// contiguous range of groups (neighbouring slots share one L2).
+__device__ inline int xcd_group(int k, int W0, int N) {
+    const int w = W0 + k, x = w & 7;
+    int start = 0;
+    for (int x2 = 0; x2 < x; x2++) {
+        const int f2 = W0 + ((x2 - W0) & 7);
+        start += f2 < W0 + N ? (W0 + N - f2 + 7) / 8 : 0;
+    }
+    return start + (w - (W0 + ((x - W0) & 7))) / 8;
+}
+
+// Sum over a slot's kPlLanes lanes, in every one of them (the same bits in
+// each): the 8-lane sum, then the other 8 of the row of 16, then (32 lanes)
+// the other row of the pair.
+__device__ inline float slot_sum(float v) {
+    v = dpp_sum8(v);
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xF, 0xF, false));
+    if (kPlLanes == 32) {
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+        v = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+    return v;
+}
+
+template <bool CUBIC>
+__device__ __attribute__((always_inline)) inline void pl_work(const int g, const int lane, const PlArgs &A) {
+    // Every fused multiply-add below is written out: the pair arithmetic is
+    // inlined more than once (loop body and tail), and contraction left to the
+    // compiler could fuse a different product in each copy, which would make a
+    // pair's bits depend on the round it falls in.
+#pragma clang fp contract(off)
+    const NbParams &p = A.p;
+    static_assert(kPlAtoms * kPlLanes == 64 && (kPlLanes == 16 || kPlLanes == 32), "one wavefront per group");
+    const int al = lane / kPlLanes, ls = lane % kPlLanes;
+    const int s = g * kPlAtoms + al;
+    // LJ parameters by type, in LDS: sigma/2 in the kernel's length unit, 2 sqrt(eps)
+    __shared__ float2 s_tp[kMaxLjTypes];
+    for (int k = lane; k < A.n_types; k += 64) {
+        const float2 t = A.type_par[k];
+        s_tp[k] = make_float2(t.x * A.sig_scale, t.y);
+    }
+    bool first_only = false;
+    if (A.disp) {
+        float d = lane < kDispSlots ? __uint_as_float(A.disp[lane * kDispStride]) : 0.f;
+        d = row_max16(d);  // lane 0 holds the largest of the kDispSlots (16) words
+        const float dmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(d)));
+        first_only = dmax <= A.dmax_first;  // (false for a NaN bound)
+    }
+    const int2 gl = A.glen[g];
+    const int rounds = (first_only ? gl.x : gl.y) / kPlLanes;
+    // the group's list base (wave-uniform) and the lane's byte offset in it;
+    // loads address base + 32-bit offset, so that no 64-bit address is formed per load
+    const char *__restrict__ nbase = reinterpret_cast<const char *>(A.nbr + A.goff[g]);
+    const unsigned lofs = (unsigned)(al * gl.y + ls) * 4u;
+    const char *__restrict__ qpb = reinterpret_cast<const char *>(A.qpos);
+    auto qpos_of = [&](int c) { return *reinterpret_cast<const int4 *>(qpb + ((unsigned)c << 4)); };
+    // a slot word's type parameters (its low 11 bits are the table offset)
+    auto par_of = [&](int w) {
+        return *reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(s_tp) + (w & 0x7f8));
+    };
+    const int4 qi = qpos_of(s);
+    __syncthreads();  // s_tp
+    const float2 ti = par_of(qi.w);
+    const float shi = ti.x, ei = ti.y;
+    const float rc2 = CUBIC ? p.rc2_u : p.rc2;
+    const unsigned rc2_bits = (unsigned)__float_as_int(rc2);
+    const float inv_w = CUBIC ? p.inv_w_u : p.inv_w, nrw = in_vgpr(p.nrw);
+    const float ds_coef = CUBIC ? p.ds_coef_u : p.ds_coef;
+    const float fscale = CUBIC ? p.fscale_u : 4294967296.0f;
+    float fx = 0.f, fy = 0.f, fz = 0.f, ee = 0.f;
+    int wmin = qi.w;  // smallest slot word seen: negative iff a position is not finite
+    // One pair of the list: slot i against partner slot data qj.
+    auto pair = [&](const int4 qj) {
+        float dx = fixdiff(qj.x, qi.x);
+        float dy = fixdiff(qj.y, qi.y);
+        float dz = fixdiff(qj.z, qi.z);
+        if (!CUBIC) { dx *= p.sx; dy *= p.sy; dz *= p.sz; }
+        const float r2p = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        // 0 < r2 <= rc2 as one unsigned compare of the bit patterns; a pair not
+        // evaluated continues at r2 = 1e36, where its terms are exactly +0
+        const bool ok = ((unsigned)__float_as_int(r2p) - 1u) < rc2_bits;
+        const float r2 = ok ? r2p : 1.0e36f;
+        const float2 tj = par_of(qj.w);
+        const float inv_r = __builtin_amdgcn_rsqf(r2);
+        float inv_r2 = inv_r * inv_r;
+        inv_r2 = fmaf(inv_r2, fmaf(-r2, inv_r2, 1.0f), inv_r2);
+        const float sig = tj.x + shi;
+        const float s2 = sig * sig * inv_r2;
+        const float s6 = s2 * s2 * s2;
+        const float lj_a = fmaf(s6, s6, -s6);            // s12 - s6
+        const float lj_b = s6 * fmaf(s6, 12.0f, -6.0f);  // 6 (2 s12 - s6)
+        const float e4 = ei * tj.y;
+        const float rr = r2 * inv_r;
+        const float u = __builtin_amdgcn_fmed3f(fmaf(rr, inv_w, nrw), 0.0f, 1.0f);
+        const float u2 = u * u;
+        const float pu = fmaf(u, fmaf(u, -6.0f, 15.0f), -10.0f);
+        const float qu = fmaf(-u, u, u);
+        const float sw = fmaf(u2 * u, pu, 1.0f);
+        const float dsdr = ds_coef * qu * qu;
+        const float inner = fmaf(-(lj_a * dsdr), rr, lj_b * sw);
+        const float fr = e4 * inv_r2 * inner;
+        fx = fmaf(-fr, dx, fx);
+        fy = fmaf(-fr, dy, fy);
+        fz = fmaf(-fr, dz, fz);
+        ee = fmaf(e4 * lj_a, sw, ee);
+        wmin = min(wmin, qj.w);
+    };
+    // Four rounds per iteration, in two halves of two (the parts hold whole
+    // pairs of rounds), with two register sets: while one half's partner data
+    // is used, the other's is on its way, requested a whole half earlier, and
+    // list codes arrive a half before their partner data is requested.  (The
+    // empty asm statements are compiler barriers: they keep the loads issued
+    // ahead of the arithmetic instead of next to their first use.)  Codes up
+    // to three rounds past a part are read ahead and not used: they are later
+    // list entries or the list buffer's slack, all valid slots.
+    if (rounds > 0) {
+        auto code = [&](int r) { return *reinterpret_cast<const int *>(nbase + lofs + r * (kPlLanes * 4)); };
+        const int a0 = code(0), a1 = code(1);
+        int b0 = code(2), b1 = code(3);
+        int4 qa0 = qpos_of(a0), qa1 = qpos_of(a1);
+        int r = 0;
+        for (; r + 4 <= rounds; r += 4) {
+            const int n0 = code(r + 4), n1 = code(r + 5);
+            const int4 qb0 = qpos_of(b0), qb1 = qpos_of(b1);
+            asm volatile("" ::: "memory");
+            pair(qa0);
+            pair(qa1);
+            b0 = code(r + 6);
+            b1 = code(r + 7);
+            qa0 = qpos_of(n0);
+            qa1 = qpos_of(n1);
+            asm volatile("" ::: "memory");
+            pair(qb0);
+            pair(qb1);
+        }
+        if (r < rounds) {  // two rounds left
+            pair(qa0);
+            pair(qa1);
+        }
+    }
+    fx = slot_sum(fx);
+    fy = slot_sum(fy);
+    fz = slot_sum(fz);
+    // A slot whose position, or a listed partner's, is not finite (rare): its
+    // force is reported as NaN, and so is the energy when such a pair is listed.
+    bool hit = false;
+    if (__builtin_expect(__ballot(wmin < 0) != 0ull, 0)) {
+        for (int r = 0; r < rounds; r++) {
+            const int c = *reinterpret_cast<const int *>(nbase + lofs + r * (kPlLanes * 4));
+            hit = hit || (c != s && (qi.w < 0 || A.qpos[c].w < 0));
+        }
+        const bool row_hit = (__ballot(hit) >> (kPlLanes * al)) & ((1ull << kPlLanes) - 1ull);
+        if (row_hit && ls == 0) A.bad[s] = 1u;
+    }
+    if (ls == 0) {
+        const bool okf = fabsf(fx * fscale) < kFixedLimitF && fabsf(fy * fscale) < kFixedLimitF &&
+                         fabsf(fz * fscale) < kFixedLimitF;
+        if (!okf) {
+            A.bad[s] = 1u;
+        } else {
+            __hip_atomic_fetch_add(&A.F[s], (unsigned long long)to_fixed_f(fx, fscale), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&A.F[(size_t)A.n_pad + s], (unsigned long long)to_fixed_f(fy, fscale),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&A.F[2 * (size_t)A.n_pad + s], (unsigned long long)to_fixed_f(fz, fscale),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    const double eg = wave_sum_f64((double)ee);
+    if (lane == 0) A.e_grp[g] = __ballot(hit) != 0ull ? __builtin_nan("") : 0.5 * eg;
+}
+
+template <bool CUBIC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_forces_pl(PlArgs A, TermArgs ta) {
+    const int lane = threadIdx.x;
+    const int64_t n_terms = ta.nb + ta.na + ta.nd + ta.n14;
+    const int n_bonded_groups = (int)((n_terms + 63) / 64);
+    if ((int)blockIdx.x < n_bonded_groups) {
+        const int64_t k = (int64_t)blockIdx.x * 64 + lane;
+        if (k < n_terms) term_dispatch(ta, k);
+        return;
+    }
+    pl_work<CUBIC>(xcd_group((int)blockIdx.x - n_bonded_groups, n_bonded_groups, A.n_grp), lane, A);
+}
+
 // Persistent batched tile kernel: a fixed grid of wavefronts takes tile work
 // items (replica, work item) from eight counters, one per XCD.  Workgroups
 // are dealt to the 8 XCDs in turn, so wavefront w runs on XCD w % 8 and takes
@@ -625,20 +846,64 @@ PrepArgs prep_args(gm_sys *s, const double *x, const double *y, const double *z)
 
 // Slot-order fixed-point positions and the displacement bound; precedes every
 // launch_forces on the same positions.
-hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double *z, hipEvent_t ev_start) {
+hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double *z, hipEvent_t ev_start,
+                       bool types) {
     if (s->n == 0) return ev_start ? hipEventRecord(ev_start, s->stream) : hipSuccess;
     const unsigned grid = (unsigned)((s->n + 255) / 256);
+    PrepArgs P = prep_args(s, x, y, z);
+    if (types) P.wtype = s->atom_wtype.p;
     if (ev_start)  // stamped by the dispatch at the kernel's start
-        hipExtLaunchKernelGGL(k_prep, dim3(grid), dim3(256), 0, s->stream, ev_start, nullptr, 0, prep_args(s, x, y, z));
+        hipExtLaunchKernelGGL(k_prep, dim3(grid), dim3(256), 0, s->stream, ev_start, nullptr, 0, P);
     else
-        k_prep<<<grid, 256, 0, s->stream>>>(prep_args(s, x, y, z));
+        k_prep<<<grid, 256, 0, s->stream>>>(P);
     return hipGetLastError();
 }
 
-// One launch: ceil(n_terms / 64) bonded workgroups, then the tile work items.
+PlArgs pl_args(gm_sys *s) {
+    PlArgs A{};
+    A.qpos = s->qpos.p;
+    A.type_par = s->type_par.p;
+    A.n_types = s->n_types;
+    A.sig_scale = span_is_cubic(s) ? s->nbp.inv_s : 1.0f;  // as the prep kernel's sigma/2
+    A.goff = s->pl_goff.p;
+    A.glen = s->pl_glen.p;
+    A.nbr = s->pl_nbr.p;
+    A.disp = (s->prune && s->prune_geom) ? s->ev.disp : nullptr;
+    // 2 d + 1e-4 r_cutoff <= kPlInner: the second parts lie beyond r_cutoff
+    A.dmax_first = (float)((kPlInner - 1e-4 * s->rc) * 0.5);
+    A.F = s->ev.F;
+    A.bad = s->ev.bad;
+    A.e_grp = s->ev.e_blk;
+    A.n_pad = (int)(s->nsc * 64);
+    A.n_grp = (int)(s->nsc * (64 / kPlAtoms));
+    A.p = s->nbp;
+    return A;
+}
+
+// One launch: ceil(n_terms / 64) bonded workgroups, then the partner-list
+// groups (or the tile work items when the system has no partner lists).
 // Shell mode: tile work items only.
 hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z, bool shell,
                          hipEvent_t ev_start, hipEvent_t ev_stop) {
+    if (!shell && pl_active(s)) {
+        const TermArgs ta = term_args(s, x, y, z);
+        const PlArgs A = pl_args(s);
+        const int64_t grid = (int64_t)A.n_grp + (ta.nb + ta.na + ta.nd + ta.n14 + 63) / 64;
+        if (grid > 0) {
+            auto kern = span_is_cubic(s) ? k_forces_pl<true> : k_forces_pl<false>;
+            if (ev_start || ev_stop)
+                hipExtLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64), 0, s->stream, ev_start, ev_stop, 0, A,
+                                      ta);
+            else
+                kern<<<(unsigned)grid, 64, 0, s->stream>>>(A, ta);
+        } else {
+            hipError_t e = hipSuccess;
+            if (ev_start) e = hipEventRecord(ev_start, s->stream);
+            if (e == hipSuccess && ev_stop) e = hipEventRecord(ev_stop, s->stream);
+            if (e != hipSuccess) return e;
+        }
+        return hipGetLastError();
+    }
     TermArgs ta = term_args(s, x, y, z);
     TileArgs A = tile_args(s);
     if (shell) {

@@ -355,6 +355,7 @@ gm_status gm_surrogate_create(gm_ctx *ctx, const gm_system_desc *exact, const gm
     // shell mode of the exact system: weight S(r) - S_in(r), rebuilt with its
     // step-unit constants whenever the system's fixed-point span changes
     s->ex->has_shell = true;
+    s->ex->use_pl = false;  // (shell-mode evaluations walk the tile list)
     s->ex->shell_rsw = d->r_inner_switch;
     s->ex->shell_w = d->r_inner_cutoff - d->r_inner_switch;
     refresh_shell(s->ex);

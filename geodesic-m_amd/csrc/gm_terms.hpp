@@ -234,6 +234,8 @@ struct PrepArgs {
     int4 *qpos;                        // x, y, z and w = sigma/2 (NaN for a non-finite position)
     const float2 *atom_par;            // per atom (sigma/2, sqrt eps), A
     float sig_scale;                   // sigma/2 in the force kernel's length unit: A * sig_scale
+    const int *wtype;                  // non-null: w = the atom's type-table offset (bit 31 for a
+                                       // non-finite position) instead of sigma/2
     unsigned *disp;
     unsigned *bad;                     // per slot: set for an atom whose position is not finite
     int *bad_first;                    // reset to INT_MAX at the start of every evaluation

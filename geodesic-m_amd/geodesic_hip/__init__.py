@@ -94,7 +94,8 @@ class Stats(C.Structure):
                 ("n_entries", _I64), ("n_tiles", _I64), ("n_tiles_cutoff", _I64), ("n_list_pairs", _I64),
                 ("n_pairs_cutoff", _I64),
                 ("n_work_items", _I64), ("ms_nonbonded", C.c_float), ("ms_eval", C.c_float),
-                ("ms_build", C.c_float)]
+                ("ms_build", C.c_float), ("n_partner_entries", _I64), ("pair_layout", C.c_int32),
+                ("n_lj_types", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -170,6 +171,7 @@ def lib():
             "gm_profile_end": ([_P, C.POINTER(Profile)], C.c_int),
             "gm_set_work_chunk": ([_P, C.c_int32], C.c_int),
             "gm_set_tile_pruning": ([_P, C.c_int32], C.c_int),
+            "gm_set_pair_layout": ([_P, C.c_int32], C.c_int),
             "gm_set_positions": ([_P, _P, _P, _P], C.c_int),
             "gm_compute_forces_resident": ([_P], C.c_int),
             "gm_get_forces": ([_P, _P, _P, _P], C.c_int),
@@ -547,6 +549,14 @@ class System:
         """Skip tiles provably beyond r_cutoff (default on); results are
         bit-identical either way."""
         _check(lib().gm_set_tile_pruning(self.h, 1 if enable else 0))
+
+    LAYOUTS = {"tiles": 0, "partners": 1}
+
+    def set_pair_layout(self, layout):
+        """Nonbonded list layout: "partners" (per-atom partner lists, the
+        default) or "tiles" (8x8 cluster tiles).  Systems with more than 256
+        distinct (sigma, epsilon) pairs always use tiles."""
+        _check(lib().gm_set_pair_layout(self.h, self.LAYOUTS[layout]))
 
     def stats(self):
         s = Stats()

@@ -183,6 +183,9 @@ typedef struct {
     float ms_nonbonded;        /* nonbonded kernel of the last profiled evaluation (hipEvent) */
     float ms_eval;             /* last profiled evaluation, whole (device time) */
     float ms_build;            /* last list build (device time) */
+    int64_t n_partner_entries; /* partner-list layout: list entries, padding included (0 for tiles) */
+    int32_t pair_layout;       /* GM_LAYOUT_TILES or GM_LAYOUT_PARTNERS: the layout evaluations use */
+    int32_t n_lj_types;        /* distinct (sigma, epsilon) pairs, 0 when more than 256 */
 } gm_stats;
 gm_status gm_get_stats(gm_sys *sys, gm_stats *out);
 
@@ -224,6 +227,30 @@ gm_status gm_set_work_chunk(gm_sys *sys, int32_t entries_per_item);
  * tests every listed pair, neighbor.rs / nonbonded.rs:38-41).  A batch reads
  * its replicas' setting when its lists are built. */
 gm_status gm_set_tile_pruning(gm_sys *sys, int32_t enable);
+
+/* Tuning: the layout of the nonbonded list that evaluations walk.
+ *   GM_LAYOUT_PARTNERS (default): per atom, its partners within r_skin,
+ *     those within r_cutoff + 0.1 A at the build first; four atoms share a
+ *     wavefront (16 lanes each), LJ parameters come from a per-type table in
+ *     LDS, every pair is computed once from each side with the same
+ *     arithmetic (so the two forces are exact negatives) and its energy
+ *     halved.  With pruning on (gm_set_tile_pruning) and every atom less
+ *     than about 0.05 A from its build position, the second part of each
+ *     list is skipped (it lies beyond r_cutoff; results bit-identical).
+ *     Needs at most 256 distinct (sigma, epsilon) pairs: a system with more
+ *     uses tiles whatever is set.
+ *   GM_LAYOUT_TILES: 8x8 tiles of clusters of 8 (one wavefront per tile row
+ *     of 8 i clusters), each pair once.
+ * Both evaluate the same listed pairs in f32 with fixed-point force sums;
+ * results agree to f32 rounding, not bit for bit (the order of operations
+ * inside a pair differs).  The pair list, its
+ * statistics and shell-mode evaluations always read the tile list: with
+ * partner lists a build leaves it out and it is made when first asked for.
+ * Takes effect at once on a built list.  A batch's replicas always use tiles
+ * (GM_ERR_INVALID_ARG for a replica).  No reference counterpart (the
+ * reference walks a flat pair list, neighbor.rs / nonbonded.rs:38-41). */
+enum { GM_LAYOUT_TILES = 0, GM_LAYOUT_PARTNERS = 1 };
+gm_status gm_set_pair_layout(gm_sys *sys, int32_t layout);
 
 /* ---- Device-resident dynamics ----------------------------------------------
  * integrator.rs:14-71 (half_kick, drift_and_constrain), constraint.rs:62-259

@@ -48,6 +48,7 @@ int main(void) {
   F(gm_system_desc, n_angles) F(gm_system_desc, n_dihedrals) F(gm_system_desc, dihed_delta)
   F(gm_system_desc, n_excl) F(gm_system_desc, n_pairs14) F(gm_system_desc, pair14_j)
   F(gm_stats, n_tiles_cutoff) F(gm_stats, n_pairs_cutoff) F(gm_stats, ms_nonbonded) F(gm_stats, ms_build)
+  F(gm_stats, n_partner_entries) F(gm_stats, pair_layout) F(gm_stats, n_lj_types)
   printf("gm_surrogate_desc %zu\n", sizeof(gm_surrogate_desc));
   printf("gm_train_desc %zu\n", sizeof(gm_train_desc));
   printf("gm_train_report %zu\n", sizeof(gm_train_report));

@@ -69,11 +69,13 @@ def test_each_replica_matches_its_oracle(batch_case):
 
 
 def test_batch_equals_each_replica_alone(batch_case):
-    """Same work items, same arithmetic: forces and energies byte-identical."""
+    """A plain system on the tile layout (the one a batch evaluates): same work
+    items, same arithmetic, forces and energies byte-identical."""
     reps, b, _, e, (fx, fy, fz) = batch_case
     for k, r in enumerate(reps):
         say(f"[gpu] plain replica {k}")
         g = gh.System(r.sigma, r.epsilon, r.topo, r.params)
+        g.set_pair_layout("tiles")
         x, y, z = r.x.copy(), r.y.copy(), r.z.copy()
         g.build_neighbor_list(x, y, z)
         ep, px, py, pz = g.compute_forces(x, y, z)
@@ -119,6 +121,7 @@ def test_work_shape_does_not_change_results(batch_case, entries, waves):
     fx, fy, fz = f
     for k, r in enumerate(reps[:2]):
         g = gh.System(r.sigma, r.epsilon, r.topo, r.params)
+        g.set_pair_layout("tiles")  # the layout a batch evaluates
         px, py, pz = r.x.copy(), r.y.copy(), r.z.copy()
         g.build_neighbor_list(px, py, pz)
         g.set_work_chunk(entries)

@@ -95,6 +95,7 @@ def test_512_replicas_against_the_oracle_and_alone():
     for k in sample:
         r = reps[k]
         g = gh.System(r.sigma, r.epsilon, r.topo, r.params)
+        g.set_pair_layout("tiles")  # the layout a batch evaluates
         px, py, pz = r.x.copy(), r.y.copy(), r.z.copy()
         g.build_neighbor_list(px, py, pz)
         ep, gx, gy, gz = g.compute_forces(px, py, pz)  # its own automatic work shape

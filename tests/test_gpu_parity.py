@@ -74,8 +74,10 @@ def check_forces(f, g, tag, synthetic=False):
     return rms
 
 
-def run_gpu(x, y, z, sigma, epsilon, topo, p):
+def run_gpu(x, y, z, sigma, epsilon, topo, p, layout=None):
     s = gh.System(sigma, epsilon, topo, p)
+    if layout is not None:
+        s.set_pair_layout(layout)
     x, y, z = x.copy(), y.copy(), z.copy()
     s.build_neighbor_list(x, y, z)
     e, fx, fy, fz = s.compute_forces(x, y, z)
@@ -173,8 +175,8 @@ def test_24k_work_chunking_does_not_change_results(sys24k):
     blocks, so every chunk (1 and 7 round up to 8; 16 and more switch the i
     forces from per-block records to atomics) gives the same bytes."""
     s, _ = sys24k
-    say("[gpu] 24k chunk sizes")
-    g, (x, y, z), e0, f0 = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params)
+    say("[gpu] 24k chunk sizes (tile layout)")
+    g, (x, y, z), e0, f0 = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params, layout="tiles")
     for chunk in (1, 7, 8, 16, 24, 32, 64, 100000, 0):
         g.set_work_chunk(chunk)
         e, *f = g.compute_forces(x, y, z)
@@ -192,16 +194,19 @@ def displaced(x, y, z, amp, seed):
     return x + d[0], y + d[1], z + d[2]
 
 
-def test_tile_pruning_is_exact(sys24k):
+@pytest.mark.parametrize("layout", ["partners", "tiles"])
+def test_tile_pruning_is_exact(sys24k, layout):
     """Tiles whose closest listed pair at build time is provably beyond r_c
     after the atoms' largest displacement are skipped: results are byte-
     identical with pruning off, at the build positions, after displacements
     below half the skin (0.75 A; the bound shrinks what is pruned) and past it
     (nothing can be pruned).  At 0.7 A the forces from the stale list also match
-    the oracle evaluating the same list (CpuBackend between rebuilds)."""
+    the oracle evaluating the same list (CpuBackend between rebuilds).  With
+    partner lists, pruning skips the second part of every list while the
+    largest displacement is below about 0.05 A (amplitude 0.02 A)."""
     s, ((wx, wy, wz), (pi, pj), _, _) = sys24k
-    say("[gpu] 24k tile pruning")
-    g, (x, y, z), _, _ = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params)
+    say(f"[gpu] 24k pruning, {layout}")
+    g, (x, y, z), _, _ = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params, layout=layout)
     p = s.params
     def forces(xs, ys, zs):
         # past half the skin, random 1.2 A moves bring some atoms of this dense
@@ -213,7 +218,7 @@ def test_tile_pruning_is_exact(sys24k):
             assert err.status == gh.GM_ERR_NUMERICAL and amp == 1.2
             return err.result
 
-    for amp in (0.0, 0.2, 0.7, 1.2):
+    for amp in (0.0, 0.02, 0.2, 0.7, 1.2):
         xs, ys, zs = (x, y, z) if amp == 0.0 else displaced(x, y, z, amp, seed=int(amp * 10))
         g.set_tile_pruning(True)
         e1, *f1 = forces(xs, ys, zs)
@@ -231,6 +236,77 @@ def test_tile_pruning_is_exact(sys24k):
                                              bonded_min_image=p.bonded_min_image, periodic=p.periodic)
             check_terms(e1, t, "24k displaced 0.7 A")
             check_forces(f1, fo, "24k displaced 0.7 A", synthetic=True)
+
+
+def test_24k_layouts_agree_and_switch_in_place(sys24k):
+    """Partner lists (the default) and cluster tiles evaluate the same list:
+    both within the bars against the oracle and against each other; a layout
+    chosen after the build gives the bytes of a build made with it; the pair
+    list and the statistics are the same under either."""
+    s, (_, (pi, pj), t, f) = sys24k
+    say("[gpu] 24k layouts")
+    gp, (x, y, z), ep, fp = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params)
+    gt, _, et, ft = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params, layout="tiles")
+    sp, st = gp.stats(), gt.stats()
+    assert sp["pair_layout"] == 1 and st["pair_layout"] == 0 and sp["n_lj_types"] == 16
+    assert sp["n_partner_entries"] >= 2 * len(pi) and st["n_partner_entries"] == 0
+    for k in ("n_list_pairs", "n_pairs_cutoff", "n_tiles", "n_entries"):
+        assert sp[k] == st[k], k
+    check_terms(ep, t, "24k partners")
+    check_forces(fp, f, "24k partners", synthetic=True)
+    check_terms(et, t, "24k tiles")
+    check_forces(ft, f, "24k tiles", synthetic=True)
+    check_terms(ep, et, "partners vs tiles")
+    check_forces(fp, ft, "partners vs tiles", synthetic=True)
+    gp.set_pair_layout("tiles")
+    e1, *f1 = gp.compute_forces(x, y, z)
+    gt.set_pair_layout("partners")
+    e2, *f2 = gt.compute_forces(x, y, z)
+    for a, b, g, h in ((e1, f1, et, ft), (e2, f2, ep, fp)):
+        assert a.total == g.total and a.lj == g.lj
+        assert all(u.tobytes() == v.tobytes() for u, v in zip(b, h))
+    for g in (gp, gt):
+        qi, qj = g.pair_list()
+        assert np.array_equal(qi, pi) and np.array_equal(qj, pj)
+
+
+def test_more_than_256_lj_types_use_tiles():
+    """300 distinct (sigma, epsilon) pairs: past the partner kernel's LDS type
+    table, so the system evaluates tiles whichever layout is asked for, and
+    matches the oracle."""
+    n = 3000
+    s = systems.protein_solvent(n, (n / 0.1) ** (1 / 3), 300, seed=41, n_types=300)
+    assert len(set(zip(s.sigma.tolist(), s.epsilon.tolist()))) > 256
+    say("[gpu] 300 LJ types")
+    (wx, wy, wz), (pi, pj), t, f = oracle_reference(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params,
+                                                    method="brute")
+    g, _, e, fg = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params, layout="partners")
+    st = g.stats()
+    assert st["pair_layout"] == 0 and st["n_lj_types"] == 0
+    check_terms(e, t, "300 types")
+    check_forces(fg, f, "300 types", synthetic=True)
+
+
+@pytest.mark.parametrize("layout", ["partners", "tiles"])
+def test_rectangular_periodic_box(layout):
+    """A periodic box with three different edges (the fixed-point positions
+    then use per-axis steps): list, energies and forces against the oracle."""
+    n = 6000
+    s = systems.protein_solvent(n, (n / 0.1) ** (1 / 3), 600, seed=43)
+    L = s.params.box[0]
+    box = (L, 1.25 * L, 1.5 * L)
+    p = gh.Params(box=box, periodic=True, r_cutoff=s.params.r_cutoff, r_switch=s.params.r_switch,
+                  r_skin=s.params.r_skin, scale_14=s.params.scale_14, bonded_min_image=s.params.bonded_min_image)
+    x, y, z = s.x.copy(), s.y * 1.25, s.z * 1.5
+    say(f"[gpu] rectangular box {box}, {layout}")
+    (wx, wy, wz), (pi, pj), t, f = oracle_reference(x, y, z, s.sigma, s.epsilon, s.topo, p, method="cells")
+    g, (gx, gy, gz), e, fg = run_gpu(x, y, z, s.sigma, s.epsilon, s.topo, p, layout=layout)
+    assert g.stats()["pair_layout"] == (1 if layout == "partners" else 0)
+    assert gx.tobytes() == wx.tobytes() and gy.tobytes() == wy.tobytes() and gz.tobytes() == wz.tobytes()
+    qi, qj = g.pair_list()
+    assert np.array_equal(qi, pi) and np.array_equal(qj, pj)
+    check_terms(e, t, f"rectangular {layout}")
+    check_forces(fg, f, f"rectangular {layout}", synthetic=True)
 
 
 def test_reference_bench_lattice_with_exact_ties():

@@ -19,8 +19,10 @@ def say(msg):
     sys.stdout.flush()
 
 
-def built(s):
+def built(s, layout=None):
     g = gh.System(s.sigma, s.epsilon, s.topo, s.params)
+    if layout is not None:
+        g.set_pair_layout(layout)
     x, y, z = s.x.copy(), s.y.copy(), s.z.copy()
     g.build_neighbor_list(x, y, z)
     return g, [gh.DeviceArray.from_host(a) for a in (x, y, z)]
@@ -38,11 +40,12 @@ def both_ways(g, d, n):
     return out
 
 
-def test_noalloc_equals_the_allocating_entry_point_twice():
+@pytest.mark.parametrize("layout", ["partners", "tiles"])
+def test_noalloc_equals_the_allocating_entry_point_twice(layout):
     cases = [systems.system_24k(), systems.protein_solvent(3000, (3000 / 0.1) ** (1 / 3), 375, seed=7)]
     for s in cases:
-        say(f"[gpu] noalloc: {s.n} atoms")
-        g, d = built(s)
+        say(f"[gpu] noalloc: {s.n} atoms, {layout}")
+        g, d = built(s, layout)
         (fa, ea), (fb, eb) = both_ways(g, d, s.n)
         for u, v in zip(fa, fb):
             assert u.tobytes() == v.tobytes()
@@ -53,11 +56,12 @@ def test_noalloc_equals_the_allocating_entry_point_twice():
 
 
 def test_noalloc_records_mode_and_refusal():
-    """Small work items keep per-block i-force records in the array too; a
-    250k list with such records needs more than 32 MiB and is refused before
-    any launch, and still evaluates with atomics (work items of 32)."""
+    """On the tile layout, small work items keep per-block i-force records in
+    the array too; a 250k list with such records needs more than 32 MiB and is
+    refused before any launch, and still evaluates with atomics (work items of
+    32)."""
     s = systems.system_24k()
-    g, d = built(s)
+    g, d = built(s, "tiles")
     for chunk in (8, 32):
         g.set_work_chunk(chunk)
         (fa, ea), (fb, eb) = both_ways(g, d, s.n)
@@ -65,7 +69,7 @@ def test_noalloc_records_mode_and_refusal():
             assert u.tobytes() == v.tobytes()
         assert ea.total == eb.total
     big = systems.system_250k()
-    g2, d2 = built(big)
+    g2, d2 = built(big, "tiles")
     g2.set_work_chunk(8)
     f = [gh.DeviceArray(big.n) for _ in range(3)]
     with pytest.raises(gh.GmError) as ei:
