@@ -2,9 +2,10 @@
 
 Workload: the 24,000-atom periodic system (62 A box, 16 LJ types, 3,000-atom
 chain + 7,000 rigid triplets; r_c 9, r_sw 7.5, list 10.5 A).  One step = one
-force evaluation (nonbonded tiles + bonds/angles/dihedrals/scaled 1-4 +
-fixed-point reduction + f64 output) at positions already resident in HBM.
-The Verlet list is built once before timing (its device time is reported).
+force evaluation (nonbonded partner lists, or cluster tiles with --layout
+tiles, + bonds/angles/dihedrals/scaled 1-4 + fixed-point reduction + f64
+output) at positions already resident in HBM.  The Verlet list is built once
+before timing (its device time is reported).
 
 Metric: pair interactions / s, where a pair interaction is an unordered,
 non-excluded pair with r <= r_cutoff, counted once.
@@ -30,6 +31,9 @@ FLOP_PER_PAIR = 40          # canonical count per pair interaction (FMA = 2)
 PEAK_FP32_TFLOPS = 157.3    # MI355X published FP32 peak
 PEAK_HBM_GBS = 8000.0
 PROFILE_EVERY = 20          # timed region: HIP events on every 20th evaluation
+FORCE_KERNEL = {"partners": "k_forces_pl<cubic> (partner lists + bonded workgroups)",
+                "tiles": "k_forces<cubic> (tiles + bonded workgroups)"}
+TRAFFIC_FILE = {"partners": "nonbonded_pl_24k_traffic.json", "tiles": "nonbonded_24k_traffic.json"}
 
 
 def traffic_from_profiles(name):
@@ -192,7 +196,7 @@ def bench_reference_workloads(cpu=True):
     return out
 
 
-def bench_build(repeats=20):
+def bench_build(repeats=20, layout="partners"):
     """The Verlet list build alone, at steady state (buffers sized, the build
     replayed as one graph), on the 24,000- and 250,000-atom systems: device
     time between the events around the build's kernels, and wall time of
@@ -204,6 +208,7 @@ def bench_build(repeats=20):
     for name, mk in (("24k", systems.system_24k), ("250k", systems.system_250k)):
         s = mk()
         g = gh.System(s.sigma, s.epsilon, s.topo, s.params)
+        g.set_pair_layout(layout)
         d = [gh.DeviceArray.from_host(a) for a in (s.x, s.y, s.z)]
         for _ in range(3):
             g.build_neighbor_list_device(*(a.ptr for a in d))
@@ -222,13 +227,14 @@ def bench_build(repeats=20):
     return out
 
 
-def bench_250k(steps=100, warmup=10):
+def bench_250k(steps=100, warmup=10, layout="partners"):
     """250,000 atoms, 136 A box: force evaluation at resident positions and the
     list build, amortised over the 20-step rebuild interval."""
     import geodesic_hip as gh
     from geodesic_hip import systems
     s = systems.system_250k()
     g = gh.System(s.sigma, s.epsilon, s.topo, s.params)
+    g.set_pair_layout(layout)
     x, y, z = s.x.copy(), s.y.copy(), s.z.copy()
     g.build_neighbor_list(x, y, z)
     builds = []
@@ -242,7 +248,7 @@ def bench_250k(steps=100, warmup=10):
     n_cut = st["n_pairs_cutoff"]
     roof = roofline_pass(g, n_cut, 50)
     return {"workload": "force evaluation, 250,000 atoms, 136 A periodic box, rebuild every 20 steps",
-            "n_pairs_cutoff": n_cut, "ms_per_eval": ms, "ms_per_eval_events_every_eval": ms_all_events,
+            "kernel": FORCE_KERNEL[layout], "n_pairs_cutoff": n_cut, "ms_per_eval": ms, "ms_per_eval_events_every_eval": ms_all_events,
             "kernel_ms_avg": roof["kernel_ms_avg"], "roofline": roof,
             "list_build_ms": build_ms, "list_build_ms_wall": wall_ms,
             "ms_per_step_amortised": ms + max(build_ms, wall_ms) / 20.0,
@@ -519,7 +525,9 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--chunk", type=int, default=-1, help="entries per work item (tuning)")
+    ap.add_argument("--chunk", type=int, default=-1, help="entries per work item (tuning, tile layout)")
+    ap.add_argument("--layout", choices=("partners", "tiles"), default="partners",
+                    help="nonbonded list layout (gm_set_pair_layout)")
     ap.add_argument("--no-250k", action="store_true")
     ap.add_argument("--no-reference-workloads", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -536,7 +544,7 @@ def main():
         sys.exit(2)
 
     if args.only == "250k":
-        print(json.dumps(bench_250k()))
+        print(json.dumps(bench_250k(layout=args.layout)))
         return
     if args.only == "batched":
         print(json.dumps(bench_batched()))
@@ -551,7 +559,7 @@ def main():
         print(json.dumps(bench_reference_workloads(cpu=not args.no_cpu_baseline)))
         return
     if args.only == "build":
-        print(json.dumps(bench_build()))
+        print(json.dumps(bench_build(layout=args.layout)))
         return
 
     import geodesic_hip as gh
@@ -559,6 +567,7 @@ def main():
 
     s = systems.system_24k()
     g = gh.System(s.sigma, s.epsilon, s.topo, s.params)
+    g.set_pair_layout(args.layout)
     x, y, z = s.x.copy(), s.y.copy(), s.z.copy()
     g.build_neighbor_list(x, y, z)
     first_build_ms = g.stats()["ms_build"]
@@ -606,8 +615,8 @@ def main():
         "roofline": {"bound": "valu", "compute": "fp32 VALU (no matrix work in this kernel)",
                      "achieved": roof["achieved"], "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": roof["frac"],
-                     "traffic": traffic_from_profiles("nonbonded_24k_traffic.json"),
-                     "kernel": "k_forces<cubic> (tiles + bonded workgroups)", "kernel_ms_avg": roof["kernel_ms_avg"],
+                     "traffic": traffic_from_profiles(TRAFFIC_FILE[args.layout]),
+                     "kernel": FORCE_KERNEL[args.layout], "kernel_ms_avg": roof["kernel_ms_avg"],
                      "kernel_ms_min": roof["kernel_ms_min"], "flop_per_pair": FLOP_PER_PAIR,
                      "timed_launches": roof["timed_launches"],
                      "timing": f"dispatch-stamped events on each of {ROOF_LAUNCHES} launches after the timed region"},
@@ -620,13 +629,13 @@ def main():
         "list_build_ms_first": first_build_ms,
         "list_build_ms_wall": build_wall,
         "list": {k: st[k] for k in ("n_superclusters", "n_entries", "n_tiles", "n_tiles_cutoff",
-                                    "n_work_items")},
+                                    "n_work_items", "n_partner_entries", "pair_layout", "n_lj_types")},
         "energy_total": e.total,
     }
     if not args.no_reference_workloads:
         out["reference_benchmarks"] = bench_reference_workloads(cpu=not args.no_cpu_baseline)
     if not args.no_250k:
-        out["system_250k"] = bench_250k()
+        out["system_250k"] = bench_250k(layout=args.layout)
     if not args.no_batched:
         bb = bench_batched()
         out["batched"] = bb

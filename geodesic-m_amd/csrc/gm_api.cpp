@@ -586,6 +586,7 @@ EvalBufs own_bufs(gm_sys *s) {
     b.e_items = s->e_items.p;
     b.e_blk = s->e_blk.p;
     b.fi_blk = (s->fi_records && !pl_active(s)) ? s->fi_blk.p : nullptr;
+    b.brec = pl_active(s) && s->n_contrib > 0 ? s->brec.p : nullptr;
     return b;
 }
 
@@ -893,9 +894,58 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     TRY(upload(s->dl, d->dihed_l, (size_t)s->nd, st));
     TRY(upload(s->dkphi, d->dihed_kphi, (size_t)s->nd, st));
     TRY(upload(s->dn, d->dihed_n, (size_t)s->nd, st));
-    TRY(upload(s->ddelta, d->dihed_delta, (size_t)s->nd, st));
+    {
+        std::vector<double2> ph((size_t)s->nd);
+        for (int64_t k = 0; k < s->nd; k++) ph[(size_t)k] = make_double2(std::cos(d->dihed_delta[k]),
+                                                                          std::sin(d->dihed_delta[k]));
+        TRY(upload(s->dphase, ph.data(), ph.size(), st));
+        TRY(hipStreamSynchronize(st));  // (ph ends with this scope)
+    }
     TRY(upload(s->p14i, d->pair14_i, (size_t)s->n14, st));
     TRY(upload(s->p14j, d->pair14_j, (size_t)s->n14, st));
+
+    // Bonded force records (a partner-list evaluation's bonded terms store each
+    // contribution as its own record; the finish kernel adds an atom's records,
+    // which lie together, atoms in index order).  Contributions are numbered as
+    // the force kernel numbers them: 2 per bond, 3 per angle (i, k, j), 4 per
+    // dihedral, 2 per scaled 1-4 pair (only when scale_14 is not 0), in term
+    // order; bc_pos maps each to its record.
+    {
+        const int64_t n14a = (s->scale14 != 0.0) ? s->n14 : 0;
+        const int64_t nc = 2 * s->nb + 3 * s->na + 4 * s->nd + 2 * n14a;
+        if (nc >= ((int64_t)1 << 31)) return fail(set_error(GM_ERR_INVALID_ARG, "too many bonded terms"));
+        std::vector<uint32_t> atom_of;
+        atom_of.reserve((size_t)nc);
+        for (int64_t k = 0; k < s->nb; k++) {
+            atom_of.push_back(d->bond_i[k]);
+            atom_of.push_back(d->bond_j[k]);
+        }
+        for (int64_t k = 0; k < s->na; k++) {
+            atom_of.push_back(d->angle_i[k]);
+            atom_of.push_back(d->angle_k[k]);
+            atom_of.push_back(d->angle_j[k]);
+        }
+        for (int64_t k = 0; k < s->nd; k++) {
+            atom_of.push_back(d->dihed_i[k]);
+            atom_of.push_back(d->dihed_j[k]);
+            atom_of.push_back(d->dihed_k[k]);
+            atom_of.push_back(d->dihed_l[k]);
+        }
+        for (int64_t k = 0; k < n14a; k++) {
+            atom_of.push_back(d->pair14_i[k]);
+            atom_of.push_back(d->pair14_j[k]);
+        }
+        std::vector<int> boff((size_t)n + 1, 0), bpos((size_t)nc);
+        for (uint32_t a : atom_of) boff[(size_t)a + 1]++;
+        for (int64_t i = 0; i < n; i++) boff[(size_t)i + 1] += boff[(size_t)i];
+        std::vector<int> fill(boff.begin(), boff.end() - 1);
+        for (int64_t c = 0; c < nc; c++) bpos[(size_t)c] = fill[atom_of[(size_t)c]]++;
+        TRY(upload(s->bc_off, boff.data(), boff.size(), st));
+        TRY(upload(s->bc_pos, bpos.data(), bpos.size(), st));
+        if (nc > 0) TRY(s->brec.ensure((size_t)(3 * nc)));
+        s->n_contrib = nc;
+        TRY(hipStreamSynchronize(st));  // (the host arrays end with this scope)
+    }
 
     // Exclusions as a canonical set (neighbor.rs:18-23): partners > i, sorted.
     std::vector<uint64_t> keys;
@@ -967,6 +1017,7 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
                            &s->sc_blk_off, &s->fsc_count, &s->fsc_start, &s->pl_scoff})
         TRY(b->ensure((size_t)s->max_sc + 1));
     TRY(s->pl_cnt.ensure(slots));
+    TRY(s->srec.ensure(slots));
     TRY(s->pl_glen.ensure(slots / kPlAtoms));
     TRY(s->pl_goff.ensure(slots / kPlAtoms + 1));
     TRY(s->pl_gloc.ensure(slots / kPlAtoms));
@@ -1011,9 +1062,10 @@ gm_status gm_system_destroy(gm_sys *s) {
         v->n = 0;
     }
     DevBuf<double> *dbl[] = {&s->sigma, &s->eps, &s->bk, &s->br0, &s->akth, &s->ath0, &s->dkphi,
-                             &s->ddelta, &s->pos3, &s->frc3, &s->refx, &s->refy, &s->refz,
+                             &s->pos3, &s->frc3, &s->refx, &s->refy, &s->refz,
                              &s->sx, &s->sy, &s->sz, &s->e_items, &s->e_blk, &s->e_out, &s->ext};
     for (auto *b : dbl) b->release();
+    s->dphase.release();
     if (s->pinned_e) (void)hipHostFree(s->pinned_e);
     s->pinned_e = nullptr;
     DevBuf<uint32_t> *u32[] = {&s->bi, &s->bj, &s->ai, &s->aj, &s->ak, &s->di, &s->dj, &s->dk,
@@ -1025,6 +1077,9 @@ gm_status gm_system_destroy(gm_sys *s) {
                           &s->atom_slot, &s->sc_count, &s->sc_start, &s->ent_sc, &s->ent_cj};
     for (auto *b : i32) b->release();
     s->excl_off.release();
+    s->bc_off.release();
+    s->bc_pos.release();
+    s->brec.release();
     s->excl_rng.release();
     s->excl_bits.release();
     s->mask_parts.release();
@@ -1049,6 +1104,7 @@ gm_status gm_system_destroy(gm_sys *s) {
         b->release();
     s->fent_w.release();
     s->pl_cnt.release();
+    s->srec.release();
     s->pl_glen.release();
     s->type_par.release();
     drop_build_graph(s);

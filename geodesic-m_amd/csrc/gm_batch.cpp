@@ -77,6 +77,7 @@ gm_status make_plan(gm_batch *b) {
         d.tile = tile_args(s);
         d.prep = prep_args(s, s->x.p, s->y.p, s->z.p);
         d.terms = term_args(s, s->x.p, s->y.p, s->z.p);
+        d.terms.rec = nullptr;  // (tile evaluations: bonded forces go to the accumulator)
         d.n_bonded_groups = (int)((d.terms.nb + d.terms.na + d.terms.nd + d.terms.n14 + 63) / 64);
         d.n_wi = (int)s->n_wi;
         d.n_blk = (int)s->n_blk;

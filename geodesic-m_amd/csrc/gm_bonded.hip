@@ -63,9 +63,10 @@ __device__ inline long long to_fixed(float v, float scale) { return (long long)_
 // 1024-thread workgroup (wavefronts G g .. G g + G - 1 for the workgroup's
 // supercluster g): wavefront q adds the i-force records of the
 // supercluster's blocks q, q + G, ... in fixed point; the first then adds the
-// others' sums and the accumulator (j forces, bonded terms), clears the
-// accumulator and writes each slot's f64 force to its atom (atom index +
-// atom_base).  Integer sums: the order does not change the result.  A slot
+// others' sums, the accumulator (j forces, bonded terms; or the partner-list
+// forces) and, when brec is given, the slot's bonded force records
+// [srec[s].x, srec[s].y), clears the accumulator and writes each slot's f64
+// force to its atom (atom index + atom_base).  Integer sums: the order does not change the result.  A slot
 // marked in bad (a contribution that was not finite or out of the fixed-point
 // range, or a position that was not finite), or with a record that is, gets
 // NaN forces, and its atom is raised into *bad_first (atomic minimum).  Every
@@ -74,7 +75,8 @@ __device__ inline long long to_fixed(float v, float scale) { return (long long)_
 __device__ inline void sc_finish(bool valid, int sc, int G, unsigned long long *F, size_t n_pad,
                                  const float *fi_blk, const int *sc_blk_off, float fi_scale,
                                  const int *slot_atom, int64_t atom_base, double *fx, double *fy, double *fz,
-                                 unsigned *bad, int *bad_first) {
+                                 unsigned *bad, int *bad_first, const long long *brec = nullptr,
+                                 int64_t n_rec = 0, const int2 *srec = nullptr) {
     __shared__ long long red[16][3][64];
     __shared__ unsigned redb[16][64];
     const int t = threadIdx.x, wv = t >> 6, l = t & 63, q = wv & (G - 1);
@@ -83,11 +85,13 @@ __device__ inline void sc_finish(bool valid, int sc, int G, unsigned long long *
     unsigned long long f[3] = {0ull, 0ull, 0ull};
     int a = -1;
     unsigned bd = 0u;
+    int2 rr = make_int2(0, 0);
     if (valid && q == 0) {
 #pragma unroll
         for (int c = 0; c < 3; c++) f[c] = F[(size_t)c * n_pad + s];
         a = slot_atom[s];
         bd = bad[s];
+        if (brec) rr = srec[s];
     }
     long long ax = 0, ay = 0, az = 0;
     unsigned rb = 0u;
@@ -120,6 +124,12 @@ __device__ inline void sc_finish(bool valid, int sc, int G, unsigned long long *
     for (int o = 0; o < G; o++) bd |= redb[wv + o][l];
     if (bd) bad[s] = 0u;
     if (a < 0) return;
+#pragma unroll 4
+    for (int k = rr.x; k < rr.y; k++) {
+        f[0] += (unsigned long long)brec[k];
+        f[1] += (unsigned long long)brec[n_rec + k];
+        f[2] += (unsigned long long)brec[2 * n_rec + k];
+    }
     if (bd) {
         fx[atom_base + a] = __builtin_nan("");
         fy[atom_base + a] = __builtin_nan("");
@@ -140,7 +150,8 @@ __global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, const in
                                                  const float *fi_blk, const int *sc_blk_off,
                                                  float fi_scale, int G, double *fx,
                                                  double *fy, double *fz, unsigned *disp, unsigned *bad,
-                                                 int *bad_first) {
+                                                 int *bad_first, const long long *brec, int64_t n_rec,
+                                                 const int2 *srec) {
     __shared__ double sh[16];
     const int b = blockIdx.x, t = threadIdx.x;
     if (b == 0 && t < kDispSlots) disp[t * kDispStride] = 0u;
@@ -151,7 +162,7 @@ __global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, const in
     }
     const int sc = (b - sg.nseg) * (16 / G) + (t >> 6) / G;
     sc_finish((size_t)sc * 64 < n_pad, sc, G, F, n_pad, fi_blk, sc_blk_off, fi_scale, slot_atom, 0,
-              fx, fy, fz, bad, bad_first);
+              fx, fy, fz, bad, bad_first, brec, n_rec, srec);
 }
 
 // Batched end of evaluation: blocks [0, R) sum replica b's five segments
@@ -223,13 +234,18 @@ hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz, bool lj_
     const unsigned grid = 5u + (unsigned)((s->nsc + 16 / G - 1) / (16 / G));
     const float fi_scale = span_is_cubic(s) ? s->nbp.fscale_u : 4294967296.0f;
     int *bf = bad_first_word(s);
+    // bonded force records (stored by a partner-list evaluation's force kernel)
+    const long long *brec = lj_only ? nullptr : s->ev.brec;
+    const int2 *srec = s->srec.p;
+    const int64_t nrec = s->n_contrib;
     if (ev_stop)  // stamped by the dispatch at the kernel's end
         hipExtLaunchKernelGGL(k_finish, dim3(grid), dim3(1024), 0, s->stream, nullptr, ev_stop, 0, sg, s->e_out.p,
                               (const int *)s->slot_atom.p, s->ev.F, n_pad, fi, (const int *)s->sc_blk_off.p,
-                              fi_scale, G, fx, fy, fz, s->ev.disp, s->ev.bad, bf);
+                              fi_scale, G, fx, fy, fz, s->ev.disp, s->ev.bad, bf, brec, nrec, srec);
     else
         k_finish<<<grid, 1024, 0, s->stream>>>(sg, s->e_out.p, s->slot_atom.p, s->ev.F, n_pad, fi, s->sc_blk_off.p,
-                                                fi_scale, G, fx, fy, fz, s->ev.disp, s->ev.bad, bf);
+                                                fi_scale, G, fx, fy, fz, s->ev.disp, s->ev.bad, bf, brec, nrec,
+                                                srec);
     return hipGetLastError();
 }
 

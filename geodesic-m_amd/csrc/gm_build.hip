@@ -1395,6 +1395,16 @@ __global__ __launch_bounds__(64) void k_pl_make(const int *__restrict__ meta, co
     for (int i = th; i < len / 4; i += 64) dst[i] = src[i];
 }
 
+// Per slot, the range of its atom's bonded force records (empty for an empty
+// slot), for the finish kernel of a partner-list evaluation.
+__global__ __launch_bounds__(256) void k_slot_rec(const int *__restrict__ meta, const int *__restrict__ slot_atom,
+                                                  const int *__restrict__ bc_off, int2 *__restrict__ srec) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (meta[1] || s >= meta[0] * 64) return;
+    const int a = slot_atom[s];
+    srec[s] = a >= 0 ? make_int2(bc_off[a], bc_off[a + 1]) : make_int2(0, 0);
+}
+
 // sigma/2 in the force kernel's length unit: fixed-point steps for cubic boxes, else A
 inline float kernel_sigma_scale(const gm_sys *s) {
     return span_is_cubic(s) ? s->nbp.inv_s : 1.0f;
@@ -1539,6 +1549,9 @@ hipError_t launch_pl_fill(gm_sys *s) {
         k_pl_make<<<(unsigned)(s->max_sc * (64 / kPlMakeSlots)), 64, 0, st>>>(
             s->meta.p, s->fsc_start.p, s->fent_cj.p, s->fent_w.p, s->pl_cnt.p, s->pl_scoff.p, s->pl_glen.p,
             s->pl_gloc.p, s->pl_goff.p, s->pl_nbr.p);
+    if (s->max_sc > 0 && s->n_contrib > 0)
+        k_slot_rec<<<(unsigned)((s->max_sc * 64 + 255) / 256), 256, 0, st>>>(s->meta.p, s->slot_atom.p, s->bc_off.p,
+                                                                           s->srec.p);
     return hipGetLastError();
 }
 

@@ -184,6 +184,8 @@ struct EvalBufs {
     double *e_items;
     double *e_blk;
     float *fi_blk;  // null when the i forces go to F with atomics
+    long long *brec;  // bonded force records, 3 x n_contrib (partner-list evaluations); null: bonded
+                      // forces go to F with atomics
 };
 // Work counters of the batched tile kernel (one per XCD), this many words apart.
 constexpr int kCtrStride = 32;
@@ -242,7 +244,8 @@ struct gm_sys {
     DevBuf<double> sigma, eps;
     DevBuf<float2> atom_par;  // (sigma/2, sqrt(eps)) in f32
     DevBuf<uint32_t> bi, bj, ai, aj, ak, di, dj, dk, dl, dn, p14i, p14j;
-    DevBuf<double> bk, br0, akth, ath0, dkphi, ddelta;
+    DevBuf<double> bk, br0, akth, ath0, dkphi;
+    DevBuf<double2> dphase;  // per dihedral: cos(delta), sin(delta)
     DevBuf<int64_t> excl_off;
     DevBuf<uint32_t> excl_nbr;
     DevBuf<uint2> excl_rng;  // per atom (smallest, largest) exclusion partner above it; (1, 0) if none
@@ -327,6 +330,14 @@ struct gm_sys {
     DevBuf<int> cost_bins;                // sort bins: counts, then offsets (heaviest first)
     int64_t ent_cap = 0;                  // entries the entry buffers hold (0: size them at the next build)
     int64_t item_cap = 0;                 // work items the work-item buffers hold
+    // bonded force records, stored in atom order: per atom, the range of its
+    // records (bc_off, n + 1); per term contribution (2 per bond, 3 per angle,
+    // 4 per dihedral, 2 per scaled 1-4 pair, in term order) its record's
+    // place (bc_pos); the records (3 x n_contrib, fixed point)
+    DevBuf<int> bc_off, bc_pos;
+    DevBuf<long long> brec;
+    DevBuf<int2> srec;  // per slot: its atom's records [x, y) (made with the partner lists)
+    int64_t n_contrib = 0;
     bool tiles_built = false;             // the tile list and work items of the last build exist
     // (a build with partner lists makes the tile list only when something
     // asks for it: pair-list export, statistics, shell-mode evaluations)
@@ -458,18 +469,22 @@ hipError_t launch_list_stats(gm_sys *s);
 // (INT_MAX if none), flag[2] the first atom whose force was not finite in the
 // last evaluation (INT_MAX if none)
 hipError_t launch_needs_rebuild(gm_sys *s, const double *x, const double *y, const double *z, int *flag = nullptr);
-// gm_nonbonded.hip: tiles + bonded terms in one launch
-// (shell: tiles only, with nbp_shell)
-// (after launch_prep: slot-order fixed-point positions and the displacement bound)
-// types: the slot word w is the atom's type-table offset (partner-list
-// path) instead of sigma/2 (tile path)
-hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double *z, hipEvent_t ev_start = nullptr,
-                       bool types = false);
+// gm_nonbonded.hip.  An evaluation is three launches:
+//   prep: slot-order fixed-point positions and the displacement bound (pl, a
+//     partner-list evaluation: the slot word w is the atom's type-table
+//     offset instead of sigma/2);
+//   forces: bonded terms + partner lists (each slot's LJ force stored to the
+//     accumulator; bonded forces as records), or bonded terms + tiles into the
+//     accumulator with atomics (shell: tiles only, with nbp_shell);
+//   finish (gm_bonded.hip): energy reduction, f64 force output (accumulator
+//     plus, after partner lists, the slot's bonded records), accumulator clear
+//     (lj_only: the bonded energy segments are empty, as after a shell-mode
+//     launch).
 // ev_start / ev_stop (optional): stamped by the dispatch at the kernel's start and end
+hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double *z, hipEvent_t ev_start = nullptr,
+                       bool pl = false);
 hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z, bool shell = false,
                          hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
-// gm_bonded.hip: energy reduction, f64 output, accumulator clear (lj_only:
-// the bonded energy segments are empty, as after a shell-mode launch)
 hipError_t launch_finish(gm_sys *s, double *fx, double *fy, double *fz, bool lj_only = false,
                          hipEvent_t ev_stop = nullptr);
 }  // namespace gm

@@ -3,7 +3,8 @@
 // For callers that cannot call hipMalloc (code under HIP graph capture, tight
 // serving loops, harnesses that forbid it).  It is gm_compute_forces_device
 // with the evaluation's scratch (fixed-point accumulator, non-finite flags,
-// displacement words, bonded and per-block energies, i-force records) taken
+// displacement words, bonded and per-block energies, i-force records, bonded
+// force records) taken
 // from `scratch`, a static device array of kScratchBytes the library holds on
 // every GPU it runs on until the process ends, instead of the system's own
 // buffers.  The system (list, slot positions, energies, outputs) is the
@@ -45,7 +46,7 @@ extern "C" gm_status gm_compute_forces_device_noalloc(gm_sys *s, const double *d
     if (!(s->n == 0 || (dx && dy && dz && dfx && dfy && dfz))) return set_error(GM_ERR_INVALID_ARG, "NULL array");
     if (!s->built) return set_error(GM_ERR_NOT_BUILT, "compute_forces before build_neighbor_list");
     if (!s->ctx->scratch) return set_error(GM_ERR_KERNEL, "static scratch array has no device address");
-    // layout: [F 3 x slots][bad slots][disp]  (zeroed by every call)  [e_items][e_blk][fi_blk]
+    // layout: [F 3 x slots][bad slots][disp]  (zeroed by every call)  [e_items][e_blk][fi_blk][brec]
     const size_t n_pad = (size_t)s->nsc * 64;
     const size_t b_F = align256(sizeof(unsigned long long) * 3 * n_pad);
     const size_t b_bad = align256(sizeof(unsigned) * n_pad);
@@ -54,7 +55,9 @@ extern "C" gm_status gm_compute_forces_device_noalloc(gm_sys *s, const double *d
     const bool recs = s->fi_records && !pl_active(s);  // i-force records (tile path, small work items)
     const size_t b_blk = align256(sizeof(double) * (size_t)lj_records(s));
     const size_t b_rec = recs ? align256(sizeof(float) * 192 * (size_t)s->n_blk) : 0;
-    const size_t need = b_F + b_bad + b_disp + b_items + b_blk + b_rec;
+    const bool brecs = pl_active(s) && s->n_contrib > 0;  // bonded force records (partner lists)
+    const size_t b_brec = brecs ? align256(sizeof(long long) * 3 * (size_t)s->n_contrib) : 0;
+    const size_t need = b_F + b_bad + b_disp + b_items + b_blk + b_rec + b_brec;
     if (need > s->ctx->scratch_bytes)
         return set_error(GM_ERR_CAPACITY, "gm_compute_forces_device_noalloc: this list needs " +
                                               std::to_string(need) + " bytes of scratch, more than the " +
@@ -76,6 +79,8 @@ extern "C" gm_status gm_compute_forces_device_noalloc(gm_sys *s, const double *d
     b.e_blk = reinterpret_cast<double *>(base + o);
     o += b_blk;
     b.fi_blk = recs ? reinterpret_cast<float *>(base + o) : nullptr;
+    o += b_rec;
+    b.brec = brecs ? reinterpret_cast<long long *>(base + o) : nullptr;  // (every record written each call)
     GM_HIP(launch_fill_u64(b.F, zeroed / sizeof(unsigned long long), s->stream));
     // the system's own accumulator is untouched (still zero for its next evaluation)
     return compute_on_device(s, dx, dy, dz, dfx, dfy, dfz, false, &b);

@@ -538,7 +538,7 @@ struct PlArgs {
     const int *nbr;      // partner slots
     const unsigned *disp;  // displacement words (null: no pruning)
     float dmax_first;      // evaluate the first part only while the displacement is at most this
-    unsigned long long *F;
+    unsigned long long *F;  // per slot: its LJ force in fixed point (plain stores)
     unsigned *bad;
     double *e_grp;       // per group: half its LJ energy sum
     int n_pad;
@@ -546,9 +546,9 @@ struct PlArgs {
     NbParams p;
 };
 
-// Group evaluated by tile workgroup k (0-based) of a launch whose first W0
+// Item (of N) evaluated by workgroup W0 + k of a launch whose first W0
 // workgroups do other work: workgroup w runs on XCD w % 8, and XCD x takes a
-// contiguous range of groups (neighbouring slots share one L2).
+// contiguous range of items (neighbouring slots share one L2).
 __device__ inline int xcd_group(int k, int W0, int N) {
     const int w = W0 + k, x = w & 7;
     int start = 0;
@@ -572,6 +572,9 @@ __device__ inline float slot_sum(float v) {
     return v;
 }
 
+constexpr int kPlWaves = 4;  // wavefronts (groups) per workgroup of the partner-list kernel
+static_assert((64 / kPlAtoms) % kPlWaves == 0, "whole workgroups per supercluster");
+
 template <bool CUBIC>
 __device__ __attribute__((always_inline)) inline void pl_work(const int g, const int lane, const PlArgs &A) {
     // Every fused multiply-add below is written out: the pair arithmetic is
@@ -585,7 +588,7 @@ __device__ __attribute__((always_inline)) inline void pl_work(const int g, const
     const int s = g * kPlAtoms + al;
     // LJ parameters by type, in LDS: sigma/2 in the kernel's length unit, 2 sqrt(eps)
     __shared__ float2 s_tp[kMaxLjTypes];
-    for (int k = lane; k < A.n_types; k += 64) {
+    for (int k = (int)threadIdx.x; k < A.n_types; k += (int)blockDim.x) {
         const float2 t = A.type_par[k];
         s_tp[k] = make_float2(t.x * A.sig_scale, t.y);
     }
@@ -655,14 +658,14 @@ __device__ __attribute__((always_inline)) inline void pl_work(const int g, const
         ee = fmaf(e4 * lj_a, sw, ee);
         wmin = min(wmin, qj.w);
     };
-    // Four rounds per iteration, in two halves of two (the parts hold whole
-    // pairs of rounds), with two register sets: while one half's partner data
-    // is used, the other's is on its way, requested a whole half earlier, and
-    // list codes arrive a half before their partner data is requested.  (The
-    // empty asm statements are compiler barriers: they keep the loads issued
-    // ahead of the arithmetic instead of next to their first use.)  Codes up
-    // to three rounds past a part are read ahead and not used: they are later
-    // list entries or the list buffer's slack, all valid slots.
+    // Four rounds per iteration, in two halves of two, with two register
+    // sets: while one half's partner data is used, the other's is on its way,
+    // requested a whole half earlier, and list codes arrive a half before
+    // their partner data is requested.  (The empty asm statements are
+    // compiler barriers: they keep the loads issued ahead of the arithmetic
+    // instead of next to their first use.)  Codes up to three rounds past a
+    // part are read ahead and not all used: they are later list entries or
+    // the list buffer's slack, all valid slots.
     if (rounds > 0) {
         auto code = [&](int r) { return *reinterpret_cast<const int *>(nbase + lofs + r * (kPlLanes * 4)); };
         const int a0 = code(0), a1 = code(1);
@@ -683,10 +686,10 @@ __device__ __attribute__((always_inline)) inline void pl_work(const int g, const
             pair(qb0);
             pair(qb1);
         }
-        if (r < rounds) {  // two rounds left
-            pair(qa0);
-            pair(qa1);
-        }
+        const int left = rounds - r;  // 0 to 3 rounds after the last whole iteration
+        if (left >= 1) pair(qa0);
+        if (left >= 2) pair(qa1);
+        if (left >= 3) pair(qpos_of(b0));
     }
     fx = slot_sum(fx);
     fy = slot_sum(fy);
@@ -702,35 +705,37 @@ __device__ __attribute__((always_inline)) inline void pl_work(const int g, const
         const bool row_hit = (__ballot(hit) >> (kPlLanes * al)) & ((1ull << kPlLanes) - 1ull);
         if (row_hit && ls == 0) A.bad[s] = 1u;
     }
+    // The slot's whole LJ force is this wavefront's: a plain store into the
+    // accumulator (nothing else adds to it in a partner-list evaluation; the
+    // bonded terms write their own records, which the finish kernel adds).
     if (ls == 0) {
         const bool okf = fabsf(fx * fscale) < kFixedLimitF && fabsf(fy * fscale) < kFixedLimitF &&
                          fabsf(fz * fscale) < kFixedLimitF;
-        if (!okf) {
-            A.bad[s] = 1u;
-        } else {
-            __hip_atomic_fetch_add(&A.F[s], (unsigned long long)to_fixed_f(fx, fscale), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&A.F[(size_t)A.n_pad + s], (unsigned long long)to_fixed_f(fy, fscale),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&A.F[2 * (size_t)A.n_pad + s], (unsigned long long)to_fixed_f(fz, fscale),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        if (!okf) A.bad[s] = 1u;
+        A.F[s] = okf ? (unsigned long long)to_fixed_f(fx, fscale) : 0ull;
+        A.F[(size_t)A.n_pad + s] = okf ? (unsigned long long)to_fixed_f(fy, fscale) : 0ull;
+        A.F[2 * (size_t)A.n_pad + s] = okf ? (unsigned long long)to_fixed_f(fz, fscale) : 0ull;
     }
     const double eg = wave_sum_f64((double)ee);
     if (lane == 0) A.e_grp[g] = __ballot(hit) != 0ull ? __builtin_nan("") : 0.5 * eg;
 }
 
+// Workgroups of kPlWaves wavefronts: the bonded workgroups first (a term per
+// thread; they store force records), then one group per wavefront (kPlWaves
+// consecutive groups per workgroup; the group count, 64 / kPlAtoms per
+// supercluster, is a multiple).
 template <bool CUBIC>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_forces_pl(PlArgs A, TermArgs ta) {
-    const int lane = threadIdx.x;
+__global__ __launch_bounds__(64 * kPlWaves) __attribute__((amdgpu_waves_per_eu(8))) void k_forces_pl(PlArgs A,
+                                                                                                    TermArgs ta) {
     const int64_t n_terms = ta.nb + ta.na + ta.nd + ta.n14;
-    const int n_bonded_groups = (int)((n_terms + 63) / 64);
-    if ((int)blockIdx.x < n_bonded_groups) {
-        const int64_t k = (int64_t)blockIdx.x * 64 + lane;
+    const int n_bonded_wg = (int)((n_terms + 64 * kPlWaves - 1) / (64 * kPlWaves));
+    if ((int)blockIdx.x < n_bonded_wg) {
+        const int64_t k = (int64_t)blockIdx.x * (64 * kPlWaves) + threadIdx.x;
         if (k < n_terms) term_dispatch(ta, k);
         return;
     }
-    pl_work<CUBIC>(xcd_group((int)blockIdx.x - n_bonded_groups, n_bonded_groups, A.n_grp), lane, A);
+    const int wg = xcd_group((int)blockIdx.x - n_bonded_wg, n_bonded_wg, A.n_grp / kPlWaves);
+    pl_work<CUBIC>(wg * kPlWaves + (int)(threadIdx.x >> 6), (int)(threadIdx.x & 63), A);
 }
 
 // Persistent batched tile kernel: a fixed grid of wavefronts takes tile work
@@ -791,13 +796,16 @@ TermArgs term_args(gm_sys *s, const double *x, const double *y, const double *z)
     t.bi = s->bi.p; t.bj = s->bj.p; t.bk = s->bk.p; t.br0 = s->br0.p;
     t.ai = s->ai.p; t.aj = s->aj.p; t.ak = s->ak.p; t.akth = s->akth.p; t.ath0 = s->ath0.p;
     t.di = s->di.p; t.dj = s->dj.p; t.dk = s->dk.p; t.dl = s->dl.p; t.dn = s->dn.p;
-    t.dkphi = s->dkphi.p; t.ddelta = s->ddelta.p;
+    t.dkphi = s->dkphi.p; t.dphase = s->dphase.p;
     t.p14i = s->p14i.p; t.p14j = s->p14j.p;
     t.sigma = s->sigma.p; t.epsilon = s->eps.p;
     t.scale14 = s->scale14;
     t.nb = s->nb; t.na = s->na; t.nd = s->nd;
     t.n14 = (s->scale14 != 0.0) ? s->n14 : 0;
     t.e_items = s->ev.e_items;
+    t.rec = s->ev.brec;
+    t.rec_pos = s->bc_pos.p;
+    t.n_rec = s->n_contrib;
     return t;
 }
 
@@ -847,11 +855,11 @@ PrepArgs prep_args(gm_sys *s, const double *x, const double *y, const double *z)
 // Slot-order fixed-point positions and the displacement bound; precedes every
 // launch_forces on the same positions.
 hipError_t launch_prep(gm_sys *s, const double *x, const double *y, const double *z, hipEvent_t ev_start,
-                       bool types) {
+                       bool pl) {
     if (s->n == 0) return ev_start ? hipEventRecord(ev_start, s->stream) : hipSuccess;
     const unsigned grid = (unsigned)((s->n + 255) / 256);
     PrepArgs P = prep_args(s, x, y, z);
-    if (types) P.wtype = s->atom_wtype.p;
+    if (pl) P.wtype = s->atom_wtype.p;  // partner lists: type-table offsets in the slot words
     if (ev_start)  // stamped by the dispatch at the kernel's start
         hipExtLaunchKernelGGL(k_prep, dim3(grid), dim3(256), 0, s->stream, ev_start, nullptr, 0, P);
     else
@@ -880,22 +888,23 @@ PlArgs pl_args(gm_sys *s) {
     return A;
 }
 
-// One launch: ceil(n_terms / 64) bonded workgroups, then the partner-list
-// groups (or the tile work items when the system has no partner lists).
+// One launch: the bonded workgroups, then the partner-list groups (or the
+// tile work items when the system has no partner lists).
 // Shell mode: tile work items only.
 hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const double *z, bool shell,
                          hipEvent_t ev_start, hipEvent_t ev_stop) {
     if (!shell && pl_active(s)) {
         const TermArgs ta = term_args(s, x, y, z);
         const PlArgs A = pl_args(s);
-        const int64_t grid = (int64_t)A.n_grp + (ta.nb + ta.na + ta.nd + ta.n14 + 63) / 64;
+        const int64_t grid = (int64_t)A.n_grp / kPlWaves + (ta.nb + ta.na + ta.nd + ta.n14 + 64 * kPlWaves - 1) /
+                                                            (64 * kPlWaves);
         if (grid > 0) {
             auto kern = span_is_cubic(s) ? k_forces_pl<true> : k_forces_pl<false>;
             if (ev_start || ev_stop)
-                hipExtLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64), 0, s->stream, ev_start, ev_stop, 0, A,
-                                      ta);
+                hipExtLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * kPlWaves), 0, s->stream, ev_start,
+                                      ev_stop, 0, A, ta);
             else
-                kern<<<(unsigned)grid, 64, 0, s->stream>>>(A, ta);
+                kern<<<(unsigned)grid, 64 * kPlWaves, 0, s->stream>>>(A, ta);
         } else {
             hipError_t e = hipSuccess;
             if (ev_start) e = hipEventRecord(ev_start, s->stream);

@@ -2,8 +2,10 @@
 // 1-4 LJ.  f64 restatement of force/bonded.rs (paths relative to the reference
 // repository root) in the reference's operation order; every function body
 // disables FMA contraction so the arithmetic is the CPU path's whatever flags
-// the including file uses.  Forces go to the int64 fixed-point accumulator
-// (2^-32 kcal/mol/A) in slot order with integer atomics.
+// the including file uses.  Forces are rounded to int64 fixed point (2^-32
+// kcal/mol/A) per contribution and either added to the accumulator (slot
+// order) with integer atomics, or stored as the term's own records, which the
+// finish kernel adds per atom (the same integers, summed without atomics).
 #pragma once
 
 #include "gm_internal.hpp"
@@ -35,6 +37,18 @@ __device__ inline void fixed_add_d(unsigned long long *p, double v, unsigned *ba
     __hip_atomic_fetch_add(p, (unsigned long long)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The same rounding into a record: 0 (and the slot marked) for a value that
+// is not finite or out of range.
+__device__ inline void fixed_put_d(long long *p, double v, unsigned *bad) {
+    const double t = v * 4294967296.0;
+    if (!(fabs(t) < kFixedLimit)) {
+        *bad = 1u;
+        *p = 0;
+        return;
+    }
+    *p = __double2ll_rn(t);
+}
+
 // Everything a bonded term reads: positions (atom order, f64), topology,
 // the slot map and the accumulator.
 struct TermArgs {
@@ -50,19 +64,48 @@ struct TermArgs {
     const uint32_t *ai, *aj, *ak;
     const double *akth, *ath0;
     const uint32_t *di, *dj, *dk, *dl, *dn;
-    const double *dkphi, *ddelta;
+    const double *dkphi;
+    const double2 *dphase;  // per dihedral: cos(delta), sin(delta)
     const uint32_t *p14i, *p14j;
     const double *sigma, *epsilon;
     double scale14;
     int64_t nb, na, nd, n14;  // n14 = 0 when scale14 == 0
     double *e_items;           // nb + na + nd + n14 energies
+    long long *rec;            // non-null: force records, [x, y, z][n_rec], not F: contribution c
+                               // (the term's first plus the atom's place in the term) goes to
+                               // record rec_pos[c]
+    const int *rec_pos;
+    int64_t n_rec;
 };
 
-__device__ inline void term_add(const TermArgs &t, uint32_t atom, double fx, double fy, double fz) {
+// Record numbers: bonds 2 each, then angles 3, dihedrals 4, 1-4 pairs 2, in
+// term order; within a term, its atoms in the order the term adds them.
+__device__ inline int64_t rec_angle0(const TermArgs &t) { return 2 * t.nb; }
+__device__ inline int64_t rec_dihedral0(const TermArgs &t) { return 2 * t.nb + 3 * t.na; }
+__device__ inline int64_t rec_lj14_0(const TermArgs &t) { return 2 * t.nb + 3 * t.na + 4 * t.nd; }
+
+// Force (fx, fy, fz) on atom from the term's record c.
+__device__ inline void term_add(const TermArgs &t, int64_t c, uint32_t atom, double fx, double fy, double fz) {
     const size_t s = (size_t)t.atom_slot[atom];
+    if (t.rec) {
+        const int64_t r = t.rec_pos[c];
+        fixed_put_d(&t.rec[r], fx, &t.bad[s]);
+        fixed_put_d(&t.rec[t.n_rec + r], fy, &t.bad[s]);
+        fixed_put_d(&t.rec[2 * t.n_rec + r], fz, &t.bad[s]);
+        return;
+    }
     fixed_add_d(&t.F[s], fx, &t.bad[s]);
     fixed_add_d(&t.F[t.n_pad + s], fy, &t.bad[s]);
     fixed_add_d(&t.F[2 * t.n_pad + s], fz, &t.bad[s]);
+}
+
+// A term that adds no force (degenerate geometry): its records hold zeros.
+__device__ inline void term_none(const TermArgs &t, int64_t c, int k) {
+    if (!t.rec) return;
+    for (int a = 0; a < k; a++) {
+        const int64_t r = t.rec_pos[c + a];
+        t.rec[r] = t.rec[t.n_rec + r] = t.rec[2 * t.n_rec + r] = 0;
+    }
 }
 
 // force/bonded.rs:5-40
@@ -74,12 +117,16 @@ __device__ inline double term_bond(const TermArgs &t, int64_t n) {
     const double dy = term_diff(t.y[j], t.y[i], t.mi, t.box[1]);
     const double dz = term_diff(t.z[j], t.z[i], t.mi, t.box[2]);
     const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    if (r == 0.0) return 0.0;
+    const int64_t c = 2 * n;
+    if (r == 0.0) {
+        term_none(t, c, 2);
+        return 0.0;
+    }
     const double dv_dr = 2.0 * k * (r - r0);
     const double e = k * (r - r0) * (r - r0);
     const double ux = dx / r, uy = dy / r, uz = dz / r;
-    term_add(t, i, dv_dr * ux, dv_dr * uy, dv_dr * uz);
-    term_add(t, j, -(dv_dr * ux), -(dv_dr * uy), -(dv_dr * uz));
+    term_add(t, c, i, dv_dr * ux, dv_dr * uy, dv_dr * uz);
+    term_add(t, c + 1, j, -(dv_dr * ux), -(dv_dr * uy), -(dv_dr * uz));
     return e;
 }
 
@@ -94,7 +141,11 @@ __device__ inline double term_angle(const TermArgs &t, int64_t n) {
                           term_diff(t.z[kk], t.z[j], t.mi, t.box[2])};
     const double r1 = sqrt(b1[0] * b1[0] + b1[1] * b1[1] + b1[2] * b1[2]);
     const double r2 = sqrt(b2[0] * b2[0] + b2[1] * b2[1] + b2[2] * b2[2]);
-    if (r1 == 0.0 || r2 == 0.0) return 0.0;
+    const int64_t c = rec_angle0(t) + 3 * n;
+    if (r1 == 0.0 || r2 == 0.0) {
+        term_none(t, c, 3);
+        return 0.0;
+    }
     const double dot = b1[0] * b2[0] + b1[1] * b2[1] + b1[2] * b2[2];
     double cos_theta = dot / (r1 * r2);
     cos_theta = cos_theta < -1.0 ? -1.0 : (cos_theta > 1.0 ? 1.0 : cos_theta);
@@ -109,9 +160,9 @@ __device__ inline double term_angle(const TermArgs &t, int64_t n) {
         f_i[c] = a * (b2[c] / (r1 * r2) - cos_theta * b1[c] / (r1 * r1));
         f_k[c] = a * (b1[c] / (r1 * r2) - cos_theta * b2[c] / (r2 * r2));
     }
-    term_add(t, i, f_i[0], f_i[1], f_i[2]);
-    term_add(t, kk, f_k[0], f_k[1], f_k[2]);
-    term_add(t, j, -(f_i[0] + f_k[0]), -(f_i[1] + f_k[1]), -(f_i[2] + f_k[2]));
+    term_add(t, c, i, f_i[0], f_i[1], f_i[2]);
+    term_add(t, c + 1, kk, f_k[0], f_k[1], f_k[2]);
+    term_add(t, c + 2, j, -(f_i[0] + f_k[0]), -(f_i[1] + f_k[1]), -(f_i[2] + f_k[2]));
     return e;
 }
 
@@ -126,11 +177,20 @@ __device__ inline double term_dot(const double *a, const double *b) {
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
-// force/bonded.rs:112-164 (Blondel-Karplus)
+// force/bonded.rs:112-164 (Blondel-Karplus).  The reference takes
+// phi = atan2(y, x) and then cos / sin of n phi - delta; here cos phi and
+// sin phi are (x, y) / |(x, y)|, cos n phi + i sin n phi their n-th complex
+// power (binary powering), and the phase enters by the angle-difference
+// formulas with cos delta and sin delta computed once per term at system
+// creation: the same values to f64 rounding, without the three
+// transcendental calls that made this term the longest dependent chain of
+// an evaluation.
 __device__ inline double term_dihedral(const TermArgs &t, int64_t n) {
 #pragma clang fp contract(off)
     const uint32_t i = t.di[n], j = t.dj[n], k = t.dk[n], l = t.dl[n];
-    const double k_phi = t.dkphi[n], mult = (double)t.dn[n], delta = t.ddelta[n];
+    const uint32_t nmult = t.dn[n];
+    const double k_phi = t.dkphi[n], mult = (double)nmult;
+    const double2 phase = t.dphase[n];
     const double b1[3] = {term_diff(t.x[j], t.x[i], t.mi, t.box[0]), term_diff(t.y[j], t.y[i], t.mi, t.box[1]),
                           term_diff(t.z[j], t.z[i], t.mi, t.box[2])};
     const double b2[3] = {term_diff(t.x[k], t.x[j], t.mi, t.box[0]), term_diff(t.y[k], t.y[j], t.mi, t.box[1]),
@@ -143,13 +203,31 @@ __device__ inline double term_dihedral(const TermArgs &t, int64_t n) {
     const double m_len2 = term_dot(m, m);
     const double n_len2 = term_dot(nv, nv);
     const double b2_len = sqrt(term_dot(b2, b2));
-    if (m_len2 == 0.0 || n_len2 == 0.0 || b2_len == 0.0) return 0.0;
+    const int64_t c = rec_dihedral0(t) + 4 * n;
+    if (m_len2 == 0.0 || n_len2 == 0.0 || b2_len == 0.0) {
+        term_none(t, c, 4);
+        return 0.0;
+    }
     const double xx = term_dot(m, nv);
     term_cross(m, nv, mn);
     const double yy = term_dot(mn, b2) / b2_len;
-    const double phi = atan2(yy, xx);
-    const double e = k_phi * (1.0 + cos(mult * phi - delta));
-    const double dv_dphi = k_phi * mult * sin(mult * phi - delta);
+    const double rinv = 1.0 / sqrt(xx * xx + yy * yy);  // (x, y) != 0: m and n are not 0
+    double pc = xx * rinv, ps = yy * rinv;               // cos phi, sin phi
+    double cn = 1.0, sn = 0.0;                           // cos n phi, sin n phi
+    for (uint32_t b = nmult; b != 0u; b >>= 1) {
+        if (b & 1u) {
+            const double c = cn * pc - sn * ps;
+            sn = cn * ps + sn * pc;
+            cn = c;
+        }
+        const double c = pc * pc - ps * ps;
+        ps = 2.0 * pc * ps;
+        pc = c;
+    }
+    const double cos_v = cn * phase.x + sn * phase.y;  // cos(n phi - delta)
+    const double sin_v = sn * phase.x - cn * phase.y;  // sin(n phi - delta)
+    const double e = k_phi * (1.0 + cos_v);
+    const double dv_dphi = k_phi * mult * sin_v;
     const double si = -dv_dphi * b2_len / m_len2;
     const double sl = dv_dphi * b2_len / n_len2;
     const double f_i[3] = {m[0] * si, m[1] * si, m[2] * si};
@@ -162,10 +240,10 @@ __device__ inline double term_dihedral(const TermArgs &t, int64_t n) {
         f_j[c] = f_i[c] * cj + f_l[c] * q;
         f_k[c] = f_i[c] * p - f_l[c] * ck;
     }
-    term_add(t, i, f_i[0], f_i[1], f_i[2]);
-    term_add(t, j, f_j[0], f_j[1], f_j[2]);
-    term_add(t, k, f_k[0], f_k[1], f_k[2]);
-    term_add(t, l, f_l[0], f_l[1], f_l[2]);
+    term_add(t, c, i, f_i[0], f_i[1], f_i[2]);
+    term_add(t, c + 1, j, f_j[0], f_j[1], f_j[2]);
+    term_add(t, c + 2, k, f_k[0], f_k[1], f_k[2]);
+    term_add(t, c + 3, l, f_l[0], f_l[1], f_l[2]);
     return e;
 }
 
@@ -181,7 +259,11 @@ __device__ inline double term_lj14(const TermArgs &t, int64_t n) {
     const double r2 = dx * dx + dy * dy + dz * dz;
     const double sig = 0.5 * (t.sigma[i] + t.sigma[j]);
     const double eps = sqrt(t.epsilon[i] * t.epsilon[j]);
-    if (r2 == 0.0 || eps == 0.0) return 0.0;
+    const int64_t c = rec_lj14_0(t) + 2 * n;
+    if (r2 == 0.0 || eps == 0.0) {
+        term_none(t, c, 2);
+        return 0.0;
+    }
     const double r = sqrt(r2);
     const double s = sig / r;
     const double s2 = s * s;
@@ -190,8 +272,8 @@ __device__ inline double term_lj14(const TermArgs &t, int64_t n) {
     const double e = t.scale14 * (4.0 * eps * (sr12 - sr6));
     const double f_radial = t.scale14 * (24.0 * eps / r * (2.0 * sr12 - sr6));
     const double fx = f_radial * dx / r, fy = f_radial * dy / r, fz = f_radial * dz / r;
-    term_add(t, i, -fx, -fy, -fz);
-    term_add(t, j, fx, fy, fz);
+    term_add(t, c, i, -fx, -fy, -fz);
+    term_add(t, c + 1, j, fx, fy, fz);
     return e;
 }
 
