@@ -418,6 +418,7 @@ gm_status fit_partner_lists(gm_sys *s, int *m) {
     GM_HIP(s->e_blk.ensure((size_t)std::max<int64_t>(s->max_sc * (64 / kPlAtoms), 1)));  // per group
     s->nfent = m[kMetaNfent];
     s->npl = m[kMetaNpl];
+    s->pl_wide = m[kMetaPlWide] != 0;
     s->pl_built = true;
     return GM_OK;
 }
@@ -1029,7 +1030,9 @@ gm_status gm::system_create_impl(gm_ctx *ctx, const gm_system_desc *d, hipStream
     TRY(s->sq.ensure(slots));
     TRY(s->sxb.ensure(slots));
     TRY(s->sp.ensure(slots));
-    TRY(s->qpos.ensure(slots));
+    TRY(s->qpos_mem.ensure(slots + 2 * (size_t)kQposGuard));
+    s->qpos.p = s->qpos_mem.p + kQposGuard;  // (a view: see kQposGuard)
+    s->qpos.n = slots;
     TRY(s->qeps.ensure(slots));
     TRY(s->disp.ensure((size_t)kDispSlots * kDispStride));
     TRY(launch_fill_i32(reinterpret_cast<int *>(s->disp.p), (size_t)kDispSlots * kDispStride, 0, st));
@@ -1118,7 +1121,9 @@ gm_status gm_system_destroy(gm_sys *s) {
     s->wi_tmp.release();
     s->wi_cost.release();
     s->cost_bins.release();
-    s->qpos.release();
+    s->qpos.p = nullptr;  // (a view into qpos_mem)
+    s->qpos.n = 0;
+    s->qpos_mem.release();
     s->qeps.release();
     s->disp.release();
     s->ent_lb.release();

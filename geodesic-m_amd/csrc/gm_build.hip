@@ -1060,6 +1060,7 @@ __global__ __launch_bounds__(1024) void k_fsc_scan(int *meta, const int *fsc_cou
     if (threadIdx.x == 0) {
         meta[kMetaNfent] = tot;
         meta[kMetaFentOver] = tot > fent_cap ? 1 : 0;
+        meta[kMetaPlWide] = 0;  // (raised by the mask kernel)
     }
 }
 
@@ -1105,7 +1106,7 @@ __device__ inline size_t fent_word_at(int f0, int nf, int t, int f) {
 // most r_in2f + band_in (the list's first part; a pair in the second part is
 // farther than r_cutoff + kPlInner at the build).  The estimate of a pair does
 // not depend on which slot is i, so both directions get the same decision.
-__global__ __launch_bounds__(256) void k_pl_masks(const int *__restrict__ meta, const int *__restrict__ fent_sc,
+__global__ __launch_bounds__(256) void k_pl_masks(int *meta, const int *__restrict__ fent_sc,
                                                   const int *__restrict__ fent_cj, const int *__restrict__ fsc_start,
                                                   uint16_t *__restrict__ fent_w,
                                                   const int *__restrict__ slot_atom, const int4 *__restrict__ sq,
@@ -1123,6 +1124,11 @@ __global__ __launch_bounds__(256) void k_pl_masks(const int *__restrict__ meta, 
         const int e = it.e;
         const int sc = __builtin_amdgcn_readfirstlane(fent_sc[e]);
         const int cj = __builtin_amdgcn_readfirstlane(fent_cj[e]);
+        // a partner farther than the compact codes reach from its group's first
+        // slot (any slot of cluster cj against groups sc * 64 .. sc * 64 + 60):
+        // the lists hold 32-bit slot numbers
+        if (t == 0 && (cj * 8 - (sc * 64 + 60) < -kQposGuard || cj * 8 + 7 - sc * 64 > kQposGuard - 1))
+            atomicOr(&meta[kMetaPlWide], 1);
         const int f0 = fsc_start[sc], nf = fsc_start[sc + 1] - f0;
         const size_t si = (size_t)sc * 64 + t;
         const int4 qa = sq[si];
@@ -1277,7 +1283,8 @@ __global__ __launch_bounds__(1024) void k_pl_scan(int *meta, const int *sc_len, 
     if (ok && !over && threadIdx.x < kPlSlack) pl_nbr[tot + threadIdx.x] = 0;
 }
 
-// One workgroup of 64 threads per 4 slots (16 threads each): the
+// One workgroup of 64 threads per 4 slots (16 threads each) (compact codes
+// or slot numbers, as the build word kMetaPlWide says): the
 // supercluster's full entries (j cluster and the 4 slots' words) are
 // staged in LDS; thread (slot a, k) takes the k-th sixteenth of them, counts
 // its slot's partners of both parts there, and after a scan over the 16
@@ -1292,30 +1299,40 @@ static_assert(kPlMakeSlots % kPlAtoms == 0, "whole groups per workgroup");
 constexpr int kPlMakeLds = 4096;   // list entries assembled in LDS
 constexpr int kPlMakeWords = 512;  // full entries staged in LDS
 
+// Compact lists pair the rounds: entries p and p + 16 of each block of 32
+// (rounds 2b and 2b + 1 of one lane) are the two halves of one 32-bit word,
+// so that a lane reads two rounds' codes with one load.  Blocks start at
+// multiples of 32 entries (the parts' padding keeps every list there).
+__device__ inline int pl_pair_pos(int x) { return (x & ~31) | ((x & 15) << 1) | ((x >> 4) & 1); }
+static_assert(kPlLanes == 16 && kPlPad % 32 == 0, "paired rounds: 16 lanes, parts in blocks of 32");
+
 // The writing half of k_pl_make, into `out` (the LDS buffer or the list
 // itself; a separate instance for each, so that LDS stores stay LDS stores).
 // word(f) and cj8(f) give the slot's word and the cluster's first slot of the
-// supercluster's f-th full entry.
-template <typename W, typename C>
-__device__ inline void pl_make_write(int *out, int fa, int fb, int pi, int po, int2 cnt, int2 gl, int sbase, int k,
+// supercluster's f-th full entry.  C16: compact codes (slot minus the group's
+// first slot plus kQposGuard) at paired positions; else slot numbers.
+template <bool C16, typename T, typename W, typename C>
+__device__ inline void pl_make_write(T *out, int fa, int fb, int pi, int po, int2 cnt, int2 gl, int sbase, int k,
                                      int slot, W word, C cj8) {
+    const int gb = C16 ? (slot / kPlAtoms) * kPlAtoms - kQposGuard : 0;  // code = slot - gb
+    auto at = [](int x) { return C16 ? pl_pair_pos(x) : x; };
     for (int f = fa; f < fb; f++) {
         const uint32_t v = word(f);
         if ((v & 255u) == 0u) continue;
-        const int c8 = cj8(f);
+        const int c8 = cj8(f) - gb;
 #pragma unroll
         for (int jl = 0; jl < 8; jl++) {
             if ((v >> jl) & 1u) {
                 const bool fp = (v >> (8 + jl)) & 1u;
-                out[fp ? pi : po] = c8 + jl;
+                out[at(fp ? pi : po)] = (T)(c8 + jl);
                 pi += fp ? 1 : 0;
                 po += fp ? 0 : 1;
             }
         }
     }
     // padding of both parts: the slot itself
-    for (int p = cnt.x + k; p < gl.x; p += 16) out[sbase + p] = slot;
-    for (int p = gl.x + cnt.y + k; p < gl.y; p += 16) out[sbase + p] = slot;
+    for (int p = cnt.x + k; p < gl.x; p += 16) out[at(sbase + p)] = (T)(slot - gb);
+    for (int p = gl.x + cnt.y + k; p < gl.y; p += 16) out[at(sbase + p)] = (T)(slot - gb);
 }
 
 __global__ __launch_bounds__(64) void k_pl_make(const int *__restrict__ meta, const int *__restrict__ fsc_start,
@@ -1328,6 +1345,7 @@ __global__ __launch_bounds__(64) void k_pl_make(const int *__restrict__ meta, co
     __shared__ int cjs[kPlMakeWords];
     const int sc = blockIdx.x / (64 / kPlMakeSlots), q = blockIdx.x % (64 / kPlMakeSlots);
     if (meta[1] || meta[kMetaFentOver] || meta[kMetaPlOver] || sc >= meta[0]) return;  // whole block
+    const bool wide = meta[kMetaPlWide] != 0;
     const int th = threadIdx.x, a = th >> 4, k = th & 15;
     const int slot = sc * 64 + q * kPlMakeSlots + a;
     const int g0 = (sc * 64 + q * kPlMakeSlots) / kPlAtoms;  // first group of the workgroup
@@ -1376,10 +1394,19 @@ __global__ __launch_bounds__(64) void k_pl_make(const int *__restrict__ meta, co
         }
         pi += sbase - ci;
         po += sbase + gl.x - co;
-        if (len > kPlMakeLds)  // (the same for the whole block)
-            pl_make_write(pl_nbr + base0, fa, fb, pi, po, cnt, gl, sbase, k, slot, word, cj8);
-        else
-            pl_make_write(buf, fa, fb, pi, po, cnt, gl, sbase, k, slot, word, cj8);
+        if (len > kPlMakeLds) {  // (the same for the whole block)
+            if (wide)
+                pl_make_write<false>(pl_nbr + base0, fa, fb, pi, po, cnt, gl, sbase, k, slot, word, cj8);
+            else
+                pl_make_write<true>(reinterpret_cast<uint16_t *>(pl_nbr) + base0, fa, fb, pi, po, cnt, gl, sbase,
+                                    k, slot, word, cj8);
+        } else {
+            if (wide)
+                pl_make_write<false>(buf, fa, fb, pi, po, cnt, gl, sbase, k, slot, word, cj8);
+            else
+                pl_make_write<true>(reinterpret_cast<uint16_t *>(buf), fa, fb, pi, po, cnt, gl, sbase, k, slot,
+                                    word, cj8);
+        }
     };
     const int sh = 16 * (a & 1), t = slot - sc * 64;
     if (staged)
@@ -1390,9 +1417,12 @@ __global__ __launch_bounds__(64) void k_pl_make(const int *__restrict__ meta, co
             [&](int f) { return fent_cj[f0 + f] * 8; });
     if (len > kPlMakeLds) return;
     __syncthreads();
-    int4 *dst = reinterpret_cast<int4 *>(pl_nbr + base0);  // (offsets are multiples of kPlPad entries)
+    // (offsets are multiples of kPlPad entries: 16-byte aligned either way)
+    int4 *dst = wide ? reinterpret_cast<int4 *>(pl_nbr + base0)
+                     : reinterpret_cast<int4 *>(reinterpret_cast<uint16_t *>(pl_nbr) + base0);
     const int4 *src = reinterpret_cast<const int4 *>(buf);
-    for (int i = th; i < len / 4; i += 64) dst[i] = src[i];
+    const int n4 = wide ? len / 4 : len / 8;
+    for (int i = th; i < n4; i += 64) dst[i] = src[i];
 }
 
 // Per slot, the range of its atom's bonded force records (empty for an empty

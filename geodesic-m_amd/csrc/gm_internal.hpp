@@ -51,6 +51,15 @@ constexpr int kCostBins = 8 + kMaxRun * 17 + 1;
 // cluster cj near the supercluster).  build words: full entries, their
 // capacity overflow, partner-list length (padding included), its overflow
 constexpr int kMetaNfent = 21, kMetaFentOver = 22, kMetaNpl = 23, kMetaPlOver = 24;
+// build word: some full entry's partners lie farther than 32767 slots from its
+// group's first slot, so the lists hold 32-bit slot numbers (else 16-bit codes)
+constexpr int kMetaPlWide = 25;
+// Compact lists: a partner's code is its slot minus the group's first slot,
+// plus kQposGuard (0 .. 65535).  Slot positions (qpos) sit inside an
+// allocation with kQposGuard slots of padding on each side, so that any code
+// decodes to an address inside it (codes read ahead past a list belong to
+// other groups; their data is loaded and never used).
+constexpr int kQposGuard = 32768;
 constexpr int kMetaWords = 32;
 // A partner-list wavefront evaluates kPlAtoms consecutive slots (a group),
 // kPlLanes lanes each; lane l of a slot takes the list's entries l, l + 16, ...
@@ -279,6 +288,8 @@ struct gm_sys {
     DevBuf<uint4> sxb;       // per slot, excl_bits of its atom; 0 for padding
     DevBuf<float2> sp;       // (sigma/2, sqrt eps) per slot
     DevBuf<int4> qpos;       // per evaluation: fixed-point coords per slot + bits of sigma/2
+                             // (a view into qpos_mem, after its kQposGuard guard slots)
+    DevBuf<int4> qpos_mem;
     DevBuf<float> qeps;      // 2 sqrt(eps) per slot
     DevBuf<unsigned> disp;   // largest displacement since the build (f32 bits, A), in
                              // kDispSlots words; raised by the prep kernel, cleared by the finish kernel
@@ -361,6 +372,7 @@ struct gm_sys {
     DevBuf<int> pl_nbr;                   // partner slots
     int64_t fent_cap = 0, pl_cap = 0;     // capacities (0: size at the next build)
     int64_t nfent = 0, npl = 0;
+    bool pl_wide = false;                 // the lists hold 32-bit slot numbers (else 16-bit codes)
     bool stats_pending = false;           // list statistics not yet summed (gm_get_stats sums them)
     bool cols_clean = false;              // column counters left at zero by the last build
     // host arrays in and out: a pinned staging buffer (3 n doubles each way) and

@@ -575,7 +575,9 @@ __device__ inline float slot_sum(float v) {
 constexpr int kPlWaves = 4;  // wavefronts (groups) per workgroup of the partner-list kernel
 static_assert((64 / kPlAtoms) % kPlWaves == 0, "whole workgroups per supercluster");
 
-template <bool CUBIC>
+// C16: the lists hold compact codes (slot minus the group's first slot plus
+// kQposGuard, two rounds per 32-bit word); else 32-bit slot numbers.
+template <bool CUBIC, bool C16>
 __device__ __attribute__((always_inline)) inline void pl_work(const int g, const int lane, const PlArgs &A) {
     // Every fused multiply-add below is written out: the pair arithmetic is
     // inlined more than once (loop body and tail), and contraction left to the
@@ -603,10 +605,30 @@ __device__ __attribute__((always_inline)) inline void pl_work(const int g, const
     const int rounds = (first_only ? gl.x : gl.y) / kPlLanes;
     // the group's list base (wave-uniform) and the lane's byte offset in it;
     // loads address base + 32-bit offset, so that no 64-bit address is formed per load
-    const char *__restrict__ nbase = reinterpret_cast<const char *>(A.nbr + A.goff[g]);
-    const unsigned lofs = (unsigned)(al * gl.y + ls) * 4u;
+    const char *__restrict__ nbase =
+        C16 ? reinterpret_cast<const char *>(reinterpret_cast<const uint16_t *>(A.nbr) + A.goff[g])
+            : reinterpret_cast<const char *>(A.nbr + A.goff[g]);
+    const unsigned lofs = C16 ? (unsigned)(al * gl.y) * 2u + (unsigned)ls * 4u : (unsigned)(al * gl.y + ls) * 4u;
     const char *__restrict__ qpb = reinterpret_cast<const char *>(A.qpos);
     auto qpos_of = [&](int c) { return *reinterpret_cast<const int4 *>(qpb + ((unsigned)c << 4)); };
+    // a partner's slot data from its list entry: the slot number, or the
+    // compact code (from the guard start, offset by the group's first slot)
+    const char *__restrict__ qpg = reinterpret_cast<const char *>(A.qpos - kQposGuard);
+    const unsigned gslot = (unsigned)(g * kPlAtoms);
+    auto partner = [&](int c) {
+        return C16 ? *reinterpret_cast<const int4 *>(qpg + ((gslot + (unsigned)c) << 4)) : qpos_of(c);
+    };
+    // the entries of rounds r and r + 1 (r even)
+    auto codes2 = [&](int r, int &c0, int &c1) {
+        if (C16) {
+            const unsigned v = *reinterpret_cast<const unsigned *>(nbase + lofs + (unsigned)(r >> 1) * 64u);
+            c0 = (int)(v & 0xffffu);
+            c1 = (int)(v >> 16);
+        } else {
+            c0 = *reinterpret_cast<const int *>(nbase + lofs + r * (kPlLanes * 4));
+            c1 = *reinterpret_cast<const int *>(nbase + lofs + (r + 1) * (kPlLanes * 4));
+        }
+    };
     // a slot word's type parameters (its low 11 bits are the table offset)
     auto par_of = [&](int w) {
         return *reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(s_tp) + (w & 0x7f8));
@@ -667,21 +689,21 @@ __device__ __attribute__((always_inline)) inline void pl_work(const int g, const
     // part are read ahead and not all used: they are later list entries or
     // the list buffer's slack, all valid slots.
     if (rounds > 0) {
-        auto code = [&](int r) { return *reinterpret_cast<const int *>(nbase + lofs + r * (kPlLanes * 4)); };
-        const int a0 = code(0), a1 = code(1);
-        int b0 = code(2), b1 = code(3);
-        int4 qa0 = qpos_of(a0), qa1 = qpos_of(a1);
+        int a0, a1, b0, b1;
+        codes2(0, a0, a1);
+        codes2(2, b0, b1);
+        int4 qa0 = partner(a0), qa1 = partner(a1);
         int r = 0;
         for (; r + 4 <= rounds; r += 4) {
-            const int n0 = code(r + 4), n1 = code(r + 5);
-            const int4 qb0 = qpos_of(b0), qb1 = qpos_of(b1);
+            int n0, n1;
+            codes2(r + 4, n0, n1);
+            const int4 qb0 = partner(b0), qb1 = partner(b1);
             asm volatile("" ::: "memory");
             pair(qa0);
             pair(qa1);
-            b0 = code(r + 6);
-            b1 = code(r + 7);
-            qa0 = qpos_of(n0);
-            qa1 = qpos_of(n1);
+            codes2(r + 6, b0, b1);
+            qa0 = partner(n0);
+            qa1 = partner(n1);
             asm volatile("" ::: "memory");
             pair(qb0);
             pair(qb1);
@@ -689,7 +711,7 @@ __device__ __attribute__((always_inline)) inline void pl_work(const int g, const
         const int left = rounds - r;  // 0 to 3 rounds after the last whole iteration
         if (left >= 1) pair(qa0);
         if (left >= 2) pair(qa1);
-        if (left >= 3) pair(qpos_of(b0));
+        if (left >= 3) pair(partner(b0));
     }
     fx = slot_sum(fx);
     fy = slot_sum(fy);
@@ -698,9 +720,13 @@ __device__ __attribute__((always_inline)) inline void pl_work(const int g, const
     // force is reported as NaN, and so is the energy when such a pair is listed.
     bool hit = false;
     if (__builtin_expect(__ballot(wmin < 0) != 0ull, 0)) {
-        for (int r = 0; r < rounds; r++) {
-            const int c = *reinterpret_cast<const int *>(nbase + lofs + r * (kPlLanes * 4));
-            hit = hit || (c != s && (qi.w < 0 || A.qpos[c].w < 0));
+        for (int r = 0; r < rounds; r += 2) {
+            int c[2];
+            codes2(r, c[0], c[1]);
+            for (int h = 0; h < 2 && r + h < rounds; h++) {
+                const int sj = C16 ? (int)gslot + c[h] - kQposGuard : c[h];  // a listed slot (or s)
+                hit = hit || (sj != s && (qi.w < 0 || A.qpos[sj].w < 0));
+            }
         }
         const bool row_hit = (__ballot(hit) >> (kPlLanes * al)) & ((1ull << kPlLanes) - 1ull);
         if (row_hit && ls == 0) A.bad[s] = 1u;
@@ -724,7 +750,7 @@ __device__ __attribute__((always_inline)) inline void pl_work(const int g, const
 // thread; they store force records), then one group per wavefront (kPlWaves
 // consecutive groups per workgroup; the group count, 64 / kPlAtoms per
 // supercluster, is a multiple).
-template <bool CUBIC>
+template <bool CUBIC, bool C16>
 __global__ __launch_bounds__(64 * kPlWaves) __attribute__((amdgpu_waves_per_eu(8))) void k_forces_pl(PlArgs A,
                                                                                                     TermArgs ta) {
     const int64_t n_terms = ta.nb + ta.na + ta.nd + ta.n14;
@@ -735,7 +761,7 @@ __global__ __launch_bounds__(64 * kPlWaves) __attribute__((amdgpu_waves_per_eu(8
         return;
     }
     const int wg = xcd_group((int)blockIdx.x - n_bonded_wg, n_bonded_wg, A.n_grp / kPlWaves);
-    pl_work<CUBIC>(wg * kPlWaves + (int)(threadIdx.x >> 6), (int)(threadIdx.x & 63), A);
+    pl_work<CUBIC, C16>(wg * kPlWaves + (int)(threadIdx.x >> 6), (int)(threadIdx.x & 63), A);
 }
 
 // Persistent batched tile kernel: a fixed grid of wavefronts takes tile work
@@ -899,7 +925,8 @@ hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const doub
         const int64_t grid = (int64_t)A.n_grp / kPlWaves + (ta.nb + ta.na + ta.nd + ta.n14 + 64 * kPlWaves - 1) /
                                                             (64 * kPlWaves);
         if (grid > 0) {
-            auto kern = span_is_cubic(s) ? k_forces_pl<true> : k_forces_pl<false>;
+            auto kern = span_is_cubic(s) ? (s->pl_wide ? k_forces_pl<true, false> : k_forces_pl<true, true>)
+                                         : (s->pl_wide ? k_forces_pl<false, false> : k_forces_pl<false, true>);
             if (ev_start || ev_stop)
                 hipExtLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * kPlWaves), 0, s->stream, ev_start,
                                       ev_stop, 0, A, ta);
