@@ -1120,15 +1120,16 @@ __global__ __launch_bounds__(256) void k_pl_masks(int *meta, const int *__restri
     for (int c = blockIdx.x * 256 + threadIdx.x; c < bp.ncol; c += gridDim.x * 256) col_count[c] = 0;
     const int t = threadIdx.x & 63;
     const int n = (meta[1] || meta[kMetaFentOver]) ? 0 : meta[kMetaNfent];
+    bool wide = false;  // (the same in every lane)
     for (EntryWalk it = entry_walk(n); it.e < it.e_hi; it.e += it.step) {
         const int e = it.e;
         const int sc = __builtin_amdgcn_readfirstlane(fent_sc[e]);
         const int cj = __builtin_amdgcn_readfirstlane(fent_cj[e]);
         // a partner farther than the compact codes reach from its group's first
         // slot (any slot of cluster cj against groups sc * 64 .. sc * 64 + 60):
-        // the lists hold 32-bit slot numbers
-        if (t == 0 && (cj * 8 - (sc * 64 + 60) < -kQposGuard || cj * 8 + 7 - sc * 64 > kQposGuard - 1))
-            atomicOr(&meta[kMetaPlWide], 1);
+        // the lists hold 32-bit slot numbers (one flag per wavefront, raised
+        // after its walk)
+        wide = wide || cj * 8 - (sc * 64 + 60) < -kQposGuard || cj * 8 + 7 - sc * 64 > kQposGuard - 1;
         const int f0 = fsc_start[sc], nf = fsc_start[sc + 1] - f0;
         const size_t si = (size_t)sc * 64 + t;
         const int4 qa = sq[si];
@@ -1189,6 +1190,7 @@ __global__ __launch_bounds__(256) void k_pl_masks(int *meta, const int *__restri
         }
         fent_w[fent_word_at(f0, nf, t, e - f0)] = (uint16_t)(inb | ((inb & first) << 8));
     }
+    if (wide && t == 0) atomicOr(&meta[kMetaPlWide], 1);
 }
 
 // One workgroup of 16 wavefronts per supercluster, lane t = slot sc * 64 + t:
@@ -1283,146 +1285,100 @@ __global__ __launch_bounds__(1024) void k_pl_scan(int *meta, const int *sc_len, 
     if (ok && !over && threadIdx.x < kPlSlack) pl_nbr[tot + threadIdx.x] = 0;
 }
 
-// One workgroup of 64 threads per 4 slots (16 threads each) (compact codes
-// or slot numbers, as the build word kMetaPlWide says): the
-// supercluster's full entries (j cluster and the 4 slots' words) are
-// staged in LDS; thread (slot a, k) takes the k-th sixteenth of them, counts
-// its slot's partners of both parts there, and after a scan over the 16
-// threads writes them (partner slot cj * 8 + jl, in ascending entry and jl)
-// at their places in the slot's list, and the padding (the slot itself) after
-// each part.  The slots' lists are assembled in LDS and leave with
-// coalesced stores.  A supercluster with more full entries than the LDS
-// stage holds, or lists longer than the LDS buffer, is read from and written
-// to memory directly.
-constexpr int kPlMakeSlots = 4;
-static_assert(kPlMakeSlots % kPlAtoms == 0, "whole groups per workgroup");
-constexpr int kPlMakeLds = 4096;   // list entries assembled in LDS
-constexpr int kPlMakeWords = 512;  // full entries staged in LDS
-
 // Compact lists pair the rounds: entries p and p + 16 of each block of 32
 // (rounds 2b and 2b + 1 of one lane) are the two halves of one 32-bit word,
 // so that a lane reads two rounds' codes with one load.  Blocks start at
 // multiples of 32 entries (the parts' padding keeps every list there).
 __device__ inline int pl_pair_pos(int x) { return (x & ~31) | ((x & 15) << 1) | ((x >> 4) & 1); }
 static_assert(kPlLanes == 16 && kPlPad % 32 == 0, "paired rounds: 16 lanes, parts in blocks of 32");
+static_assert(kPlAtoms == 4, "a group is one quad of slots (the mask words' layout)");
 
-// The writing half of k_pl_make, into `out` (the LDS buffer or the list
-// itself; a separate instance for each, so that LDS stores stay LDS stores).
-// word(f) and cj8(f) give the slot's word and the cluster's first slot of the
-// supercluster's f-th full entry.  C16: compact codes (slot minus the group's
-// first slot plus kQposGuard) at paired positions; else slot numbers.
-template <bool C16, typename T, typename W, typename C>
-__device__ inline void pl_make_write(T *out, int fa, int fb, int pi, int po, int2 cnt, int2 gl, int sbase, int k,
-                                     int slot, W word, C cj8) {
-    const int gb = C16 ? (slot / kPlAtoms) * kPlAtoms - kQposGuard : 0;  // code = slot - gb
-    auto at = [](int x) { return C16 ? pl_pair_pos(x) : x; };
-    for (int f = fa; f < fb; f++) {
-        const uint32_t v = word(f);
-        if ((v & 255u) == 0u) continue;
-        const int c8 = cj8(f) - gb;
-#pragma unroll
-        for (int jl = 0; jl < 8; jl++) {
-            if ((v >> jl) & 1u) {
-                const bool fp = (v >> (8 + jl)) & 1u;
-                out[at(fp ? pi : po)] = (T)(c8 + jl);
-                pi += fp ? 1 : 0;
-                po += fp ? 0 : 1;
-            }
-        }
-    }
-    // padding of both parts: the slot itself
-    for (int p = cnt.x + k; p < gl.x; p += 16) out[at(sbase + p)] = (T)(slot - gb);
-    for (int p = gl.x + cnt.y + k; p < gl.y; p += 16) out[at(sbase + p)] = (T)(slot - gb);
+// Inclusive sum over the wavefront: within rows of 16 lanes by DPP row
+// shifts (lanes shifted in from outside the row read 0), then the lower rows'
+// totals.
+__device__ inline int wave_incl_sum(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);  // row_shr:8
+    const int r0 = __builtin_amdgcn_readlane(x, 15), r1 = __builtin_amdgcn_readlane(x, 31),
+              r2 = __builtin_amdgcn_readlane(x, 47);
+    const int row = (int)(threadIdx.x & 63) >> 4;
+    return x + (row >= 1 ? r0 : 0) + (row >= 2 ? r1 : 0) + (row >= 3 ? r2 : 0);
 }
 
-__global__ __launch_bounds__(64) void k_pl_make(const int *__restrict__ meta, const int *__restrict__ fsc_start,
-                                                 const int *__restrict__ fent_cj, const uint16_t *__restrict__ fent_w,
-                                                 const int2 *__restrict__ pl_cnt, const int *__restrict__ sc_off,
-                                                 const int2 *__restrict__ pl_glen, const int *__restrict__ gloc,
-                                                 int *__restrict__ pl_goff, int *__restrict__ pl_nbr) {
-    __shared__ int buf[kPlMakeLds];
-    __shared__ uint2 words[kPlMakeWords];  // per full entry, the 4 slots' words
-    __shared__ int cjs[kPlMakeWords];
-    const int sc = blockIdx.x / (64 / kPlMakeSlots), q = blockIdx.x % (64 / kPlMakeSlots);
-    if (meta[1] || meta[kMetaFentOver] || meta[kMetaPlOver] || sc >= meta[0]) return;  // whole block
-    const bool wide = meta[kMetaPlWide] != 0;
-    const int th = threadIdx.x, a = th >> 4, k = th & 15;
-    const int slot = sc * 64 + q * kPlMakeSlots + a;
-    const int g0 = (sc * 64 + q * kPlMakeSlots) / kPlAtoms;  // first group of the workgroup
-    const int g = slot / kPlAtoms;
-    // the workgroup's lists start at the supercluster's offset plus its first
-    // group's offset in the supercluster; the groups' offsets (for the force
-    // kernel) follow
-    const int base0 = sc_off[sc] + gloc[g0];
-    int len = 0, gofs = 0;  // (gofs: group g's offset from base0)
-#pragma unroll
-    for (int u = g0; u < g0 + kPlMakeSlots / kPlAtoms; u++) {
-        if (th == 0) pl_goff[u] = base0 + len;
-        if (u < g) gofs += kPlAtoms * pl_glen[u].y;
-        len += kPlAtoms * pl_glen[u].y;
-    }
+// One wavefront per group (four per workgroup), one slot of the group after
+// the other: the supercluster's full entries 64 at a time, lane l taking
+// entry f = base + l; its word's partners of each part are counted, summed
+// over the lanes in entry order (a wavefront scan of both counts at once), and
+// written (partner slot cj * 8 + jl, in ascending entry and jl) at their
+// places in the slot's list; then the padding of both parts (the slot
+// itself).  Compact codes (slot minus the group's first slot plus kQposGuard,
+// paired rounds) or slot numbers, as the build word kMetaPlWide says.  Lane 0
+// writes the group's list offset for the force kernel.
+template <bool C16>
+__device__ inline void pl_make_group(int g, int sc, const int *fsc_start, const int *fent_cj,
+                                     const uint16_t *fent_w, const int *sc_off, const int2 *pl_glen, const int *gloc,
+                                     int *pl_goff, int *pl_nbr) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int base = sc_off[sc] + gloc[g];
+    if (lane == 0) pl_goff[g] = base;
     const int2 gl = pl_glen[g];
-    const int2 cnt = pl_cnt[slot];
-    const int sbase = gofs + (slot % kPlAtoms) * gl.y;  // the slot's list, from the workgroup's start
     const int f0 = fsc_start[sc], nf = fsc_start[sc + 1] - f0;
-    const int fa = (int)((int64_t)nf * k / 16), fb = (int)((int64_t)nf * (k + 1) / 16);
-    const bool staged = nf <= kPlMakeWords;
-    if (staged) {
-        static_assert(kPlMakeSlots == 4, "one quad of slots per workgroup (the words' layout)");
-        const uint2 *qw = reinterpret_cast<const uint2 *>(fent_w + fent_word_at(f0, nf, q * 4, 0));
-        for (int f = th; f < nf; f += 64) {
-            words[f] = qw[f];
-            cjs[f] = fent_cj[f0 + f];
+    const int q = g % (64 / kPlAtoms);  // the group's quad in the supercluster
+    const uint16_t *qw = fent_w + fent_word_at(f0, nf, q * 4, 0);  // slot a, entry f: qw[f * 4 + a]
+    const int gb = C16 ? g * kPlAtoms - kQposGuard : 0;  // code = slot - gb
+    for (int a = 0; a < kPlAtoms; a++) {
+        const int slot = g * kPlAtoms + a;
+        const size_t lb = (size_t)base + (size_t)a * gl.y;  // the slot's list
+        auto put = [&](int p, int v) {
+            if (C16)
+                reinterpret_cast<uint16_t *>(pl_nbr)[lb + pl_pair_pos(p)] = (uint16_t)v;
+            else
+                pl_nbr[lb + p] = v;
+        };
+        int pi = 0, po = gl.x;  // next place in each part (the same in every lane)
+        for (int fb = 0; fb < nf; fb += 64) {
+            const int f = fb + lane;
+            uint32_t v = 0;
+            int c8 = 0;
+            if (f < nf) {
+                v = qw[(size_t)f * 4 + a];
+                c8 = fent_cj[f0 + f] * 8 - gb;
+            }
+            const uint32_t fi = v & (v >> 8) & 255u, so = v & ~(v >> 8) & 255u;
+            const int cnt = __popc(fi) | (__popc(so) << 16);  // (at most 512 per wavefront: no carry)
+            const int inc = wave_incl_sum(cnt);
+            const int tot = __builtin_amdgcn_readlane(inc, 63);
+            int xi = pi + ((inc - cnt) & 0xffff), xo = po + ((inc - cnt) >> 16);
+#pragma unroll
+            for (int jl = 0; jl < 8; jl++) {
+                const bool in_f = (fi >> jl) & 1u, in_s = (so >> jl) & 1u;
+                if (in_f || in_s) put(in_f ? xi : xo, c8 + jl);
+                xi += in_f ? 1 : 0;
+                xo += in_s ? 1 : 0;
+            }
+            pi += tot & 0xffff;
+            po += tot >> 16;
         }
-        __syncthreads();
+        for (int p = pi + lane; p < gl.x; p += 64) put(p, slot - gb);
+        for (int p = po + lane; p < gl.y; p += 64) put(p, slot - gb);
     }
-    // count, scan over the slot's 16 threads, write: with the words and
-    // clusters from LDS (staged) or from memory
-    auto run = [&](auto word, auto cj8) {
-        int ci = 0, co = 0;
-        for (int f = fa; f < fb; f++) {
-            const uint32_t v = word(f);
-            const uint32_t fi = (v >> 8) & 255u;
-            ci += __popc(fi);
-            co += __popc(v & 255u & ~fi);
-        }
-        // exclusive scan of (ci, co) over the 16 threads of the slot (a row of 16 lanes)
-        int pi = ci, po = co;
-        for (int o = 1; o < 16; o <<= 1) {
-            const int ti = __shfl_up(pi, o, 16), to = __shfl_up(po, o, 16);
-            if (k >= o) { pi += ti; po += to; }
-        }
-        pi += sbase - ci;
-        po += sbase + gl.x - co;
-        if (len > kPlMakeLds) {  // (the same for the whole block)
-            if (wide)
-                pl_make_write<false>(pl_nbr + base0, fa, fb, pi, po, cnt, gl, sbase, k, slot, word, cj8);
-            else
-                pl_make_write<true>(reinterpret_cast<uint16_t *>(pl_nbr) + base0, fa, fb, pi, po, cnt, gl, sbase,
-                                    k, slot, word, cj8);
-        } else {
-            if (wide)
-                pl_make_write<false>(buf, fa, fb, pi, po, cnt, gl, sbase, k, slot, word, cj8);
-            else
-                pl_make_write<true>(reinterpret_cast<uint16_t *>(buf), fa, fb, pi, po, cnt, gl, sbase, k, slot,
-                                    word, cj8);
-        }
-    };
-    const int sh = 16 * (a & 1), t = slot - sc * 64;
-    if (staged)
-        run([&](int f) -> uint32_t { return ((a < 2 ? words[f].x : words[f].y) >> sh) & 0xFFFFu; },
-            [&](int f) { return cjs[f] * 8; });
+}
+
+__global__ __launch_bounds__(256) void k_pl_make(const int *__restrict__ meta, const int *__restrict__ fsc_start,
+                                                  const int *__restrict__ fent_cj, const uint16_t *__restrict__ fent_w,
+                                                  const int *__restrict__ sc_off, const int2 *__restrict__ pl_glen,
+                                                  const int *__restrict__ gloc, int *__restrict__ pl_goff,
+                                                  int *__restrict__ pl_nbr) {
+    if (meta[1] || meta[kMetaFentOver] || meta[kMetaPlOver]) return;  // the same for the whole grid
+    const int g = __builtin_amdgcn_readfirstlane((int)blockIdx.x * 4 + (int)(threadIdx.x >> 6));
+    const int sc = g / (64 / kPlAtoms);
+    if (sc >= meta[0]) return;  // (the same for the whole wavefront)
+    if (meta[kMetaPlWide])
+        pl_make_group<false>(g, sc, fsc_start, fent_cj, fent_w, sc_off, pl_glen, gloc, pl_goff, pl_nbr);
     else
-        run([&](int f) -> uint32_t { return fent_w[fent_word_at(f0, nf, t, f)]; },
-            [&](int f) { return fent_cj[f0 + f] * 8; });
-    if (len > kPlMakeLds) return;
-    __syncthreads();
-    // (offsets are multiples of kPlPad entries: 16-byte aligned either way)
-    int4 *dst = wide ? reinterpret_cast<int4 *>(pl_nbr + base0)
-                     : reinterpret_cast<int4 *>(reinterpret_cast<uint16_t *>(pl_nbr) + base0);
-    const int4 *src = reinterpret_cast<const int4 *>(buf);
-    const int n4 = wide ? len / 4 : len / 8;
-    for (int i = th; i < n4; i += 64) dst[i] = src[i];
+        pl_make_group<true>(g, sc, fsc_start, fent_cj, fent_w, sc_off, pl_glen, gloc, pl_goff, pl_nbr);
 }
 
 // Per slot, the range of its atom's bonded force records (empty for an empty
@@ -1576,9 +1532,9 @@ hipError_t launch_pl_fill(gm_sys *s) {
     k_pl_scan<<<1, 1024, 0, st>>>(s->meta.p, s->fsc_count.p, s->pl_scoff.p,
                                   s->pl_nbr.p ? (int)std::min<int64_t>(s->pl_cap, 0x7fffffff) : -1, s->pl_nbr.p);
     if (s->max_sc > 0)
-        k_pl_make<<<(unsigned)(s->max_sc * (64 / kPlMakeSlots)), 64, 0, st>>>(
-            s->meta.p, s->fsc_start.p, s->fent_cj.p, s->fent_w.p, s->pl_cnt.p, s->pl_scoff.p, s->pl_glen.p,
-            s->pl_gloc.p, s->pl_goff.p, s->pl_nbr.p);
+        k_pl_make<<<(unsigned)(s->max_sc * (64 / kPlAtoms) / 4), 256, 0, st>>>(
+            s->meta.p, s->fsc_start.p, s->fent_cj.p, s->fent_w.p, s->pl_scoff.p, s->pl_glen.p, s->pl_gloc.p,
+            s->pl_goff.p, s->pl_nbr.p);
     if (s->max_sc > 0 && s->n_contrib > 0)
         k_slot_rec<<<(unsigned)((s->max_sc * 64 + 255) / 256), 256, 0, st>>>(s->meta.p, s->slot_atom.p, s->bc_off.p,
                                                                            s->srec.p);

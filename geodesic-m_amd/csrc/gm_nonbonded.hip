@@ -761,7 +761,10 @@ __global__ __launch_bounds__(64 * kPlWaves) __attribute__((amdgpu_waves_per_eu(8
         return;
     }
     const int wg = xcd_group((int)blockIdx.x - n_bonded_wg, n_bonded_wg, A.n_grp / kPlWaves);
-    pl_work<CUBIC, C16>(wg * kPlWaves + (int)(threadIdx.x >> 6), (int)(threadIdx.x & 63), A);
+    // (the group is the same in the whole wavefront: said so, everything
+    // derived from it lives in scalar registers)
+    const int g = __builtin_amdgcn_readfirstlane(wg * kPlWaves + (int)(threadIdx.x >> 6));
+    pl_work<CUBIC, C16>(g, (int)(threadIdx.x & 63), A);
 }
 
 // Persistent batched tile kernel: a fixed grid of wavefronts takes tile work
