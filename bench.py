@@ -36,6 +36,11 @@ FORCE_KERNEL = {"partners": "k_forces_pl<cubic> (partner lists + bonded workgrou
 TRAFFIC_FILE = {"partners": "nonbonded_pl_24k_traffic.json", "tiles": "nonbonded_24k_traffic.json"}
 
 
+def layout_name(stats):
+    """The layout a system's evaluations use (gm_stats.pair_layout)."""
+    return "partners" if stats["pair_layout"] == 1 else "tiles"
+
+
 def traffic_from_profiles(name):
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
@@ -196,7 +201,7 @@ def bench_reference_workloads(cpu=True):
     return out
 
 
-def bench_build(repeats=20, layout="partners"):
+def bench_build(repeats=20, layout="auto"):
     """The Verlet list build alone, at steady state (buffers sized, the build
     replayed as one graph), on the 24,000- and 250,000-atom systems: device
     time between the events around the build's kernels, and wall time of
@@ -227,9 +232,10 @@ def bench_build(repeats=20, layout="partners"):
     return out
 
 
-def bench_250k(steps=100, warmup=10, layout="partners"):
+def bench_250k(steps=100, warmup=10, layout="auto"):
     """250,000 atoms, 136 A box: force evaluation at resident positions and the
-    list build, amortised over the 20-step rebuild interval."""
+    list build, amortised over the 20-step rebuild interval; and the workload's
+    motion (protocol_250k), which evaluates displaced positions."""
     import geodesic_hip as gh
     from geodesic_hip import systems
     s = systems.system_250k()
@@ -247,13 +253,56 @@ def bench_250k(steps=100, warmup=10, layout="partners"):
     build_ms = statistics.median(builds)
     n_cut = st["n_pairs_cutoff"]
     roof = roofline_pass(g, n_cut, 50)
+    proto = protocol_250k(g, s)
     return {"workload": "force evaluation, 250,000 atoms, 136 A periodic box, rebuild every 20 steps",
-            "kernel": FORCE_KERNEL[layout], "n_pairs_cutoff": n_cut, "ms_per_eval": ms, "ms_per_eval_events_every_eval": ms_all_events,
+            "kernel": FORCE_KERNEL[layout_name(st)], "n_pairs_cutoff": n_cut, "ms_per_eval": ms, "ms_per_eval_events_every_eval": ms_all_events,
+            "protocol": proto,
             "kernel_ms_avg": roof["kernel_ms_avg"], "roofline": roof,
             "list_build_ms": build_ms, "list_build_ms_wall": wall_ms,
             "ms_per_step_amortised": ms + max(build_ms, wall_ms) / 20.0,
             "pair_interactions_per_s": n_cut / (ms * 1e-3),
             "pair_interactions_per_s_amortised": n_cut / ((ms + build_ms / 20.0) * 1e-3)}
+
+
+def protocol_250k(g, s, interval=20, sigma=0.02, cycles=3, seed=7):
+    """The 250k workload's motion: Gaussian displacements of sigma A per
+    component per step, the list rebuilt every `interval` steps.  One cycle =
+    a build from the first frame, then an evaluation at each of the interval's
+    frames (precomputed, resident in HBM); timed over whole cycles (wall),
+    and the force kernel over one cycle with dispatch-stamped events."""
+    import geodesic_hip as gh
+    rng = np.random.default_rng(seed)
+    pos = np.stack([s.x, s.y, s.z]).astype(np.float64)
+    frames = []
+    for _ in range(interval):
+        frames.append([gh.DeviceArray.from_host(np.ascontiguousarray(v)) for v in pos])
+        pos = pos + rng.normal(0.0, sigma, pos.shape)
+    out = [gh.DeviceArray(s.n) for _ in range(3)]
+
+    def cycle():
+        for k, fr in enumerate(frames):
+            if k == 0:
+                g.build_neighbor_list_device(*(a.ptr for a in fr))
+            g.compute_forces_device(*(a.ptr for a in fr), *(o.ptr for o in out))
+        g.sync()
+
+    cycle()  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(cycles):
+        cycle()
+    ms = (time.perf_counter() - t0) * 1e3 / (cycles * interval)
+    g.profile_begin(1)
+    cycle()
+    prof = g.profile_end()
+    kern = prof["ms_nonbonded_total"] / max(prof["n_evals"], 1)
+    d = np.sqrt(((pos - np.stack([s.x, s.y, s.z])) ** 2).sum(axis=0)).max()
+    for fr in frames:
+        for a in fr:
+            a.free()
+    for o in out:
+        o.free()
+    return {"interval": interval, "sigma_A_per_step": sigma, "ms_per_step_build_included": ms,
+            "kernel_ms_avg": kern, "max_displacement_at_interval_end_A": float(d)}
 
 
 def bench_batched(n_replicas=512, steps=20):
@@ -526,7 +575,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--chunk", type=int, default=-1, help="entries per work item (tuning, tile layout)")
-    ap.add_argument("--layout", choices=("partners", "tiles"), default="partners",
+    ap.add_argument("--layout", choices=("auto", "partners", "tiles"), default="auto",
                     help="nonbonded list layout (gm_set_pair_layout)")
     ap.add_argument("--no-250k", action="store_true")
     ap.add_argument("--no-reference-workloads", action="store_true")
@@ -615,8 +664,8 @@ def main():
         "roofline": {"bound": "valu", "compute": "fp32 VALU (no matrix work in this kernel)",
                      "achieved": roof["achieved"], "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": roof["frac"],
-                     "traffic": traffic_from_profiles(TRAFFIC_FILE[args.layout]),
-                     "kernel": FORCE_KERNEL[args.layout], "kernel_ms_avg": roof["kernel_ms_avg"],
+                     "traffic": traffic_from_profiles(TRAFFIC_FILE[layout_name(st)]),
+                     "kernel": FORCE_KERNEL[layout_name(st)], "kernel_ms_avg": roof["kernel_ms_avg"],
                      "kernel_ms_min": roof["kernel_ms_min"], "flop_per_pair": FLOP_PER_PAIR,
                      "timed_launches": roof["timed_launches"],
                      "timing": f"dispatch-stamped events on each of {ROOF_LAUNCHES} launches after the timed region"},

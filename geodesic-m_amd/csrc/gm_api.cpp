@@ -541,6 +541,16 @@ gm_status build_on_device(gm_sys *s, bool wrap, double *px, double *py, double *
         s->nent = 0;
         s->stats.n_entries = 0;
         s->stats.n_work_items = s->nsc * (64 / kPlAtoms);
+        if (s->layout == GM_LAYOUT_AUTO && s->pl_wide) {
+            // automatic layout: lists that need 32-bit slot numbers cost 4 bytes
+            // per entry, and the tiles' gradual pruning does better as soon as
+            // the atoms move; this system evaluates tiles from now on
+            s->auto_wide = true;
+            s->use_pl = false;
+            s->pl_built = false;
+            drop_build_graph(s);
+            if ((rc = ensure_tiles(s)) != GM_OK) { s->built = false; return rc; }
+        }
     } else {
         s->nent = m[kMetaNent];
         s->stats.n_entries = s->nent;
@@ -1344,9 +1354,11 @@ gm_status gm_set_work_chunk(gm_sys *s, int32_t entries_per_item) {
 }
 
 gm_status gm_set_pair_layout(gm_sys *s, int32_t layout) {
-    GM_REQUIRE(s && (layout == GM_LAYOUT_TILES || layout == GM_LAYOUT_PARTNERS), "bad argument");
+    GM_REQUIRE(s && (layout == GM_LAYOUT_TILES || layout == GM_LAYOUT_PARTNERS || layout == GM_LAYOUT_AUTO),
+               "bad argument");
     GM_REQUIRE(!s->batch_owned, "gm_set_pair_layout: a batch evaluates its replicas' tile lists");
-    const bool pl = layout == GM_LAYOUT_PARTNERS && s->n_types > 0;
+    s->layout = layout;
+    const bool pl = s->n_types > 0 && (layout == GM_LAYOUT_PARTNERS || (layout == GM_LAYOUT_AUTO && !s->auto_wide));
     if (pl == s->use_pl) return GM_OK;
     s->use_pl = pl;
     drop_build_graph(s);

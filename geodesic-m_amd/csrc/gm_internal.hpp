@@ -357,6 +357,9 @@ struct gm_sys {
     // 8 j slots, per slot its two parts' counts, per group its offset and
     // padded part lengths
     bool use_pl = true;                   // nonbonded evaluation over partner lists (false: tiles)
+    int layout = GM_LAYOUT_AUTO;          // the caller's choice (gm_set_pair_layout)
+    bool auto_wide = false;               // automatic layout: this system's partner lists need 32-bit
+                                          // slot numbers, so it evaluates tiles
     int n_types = 0;                      // distinct (sigma, eps) pairs, 0 if more than kMaxLjTypes
     DevBuf<int> atom_wtype;               // per atom: its type's byte offset in the table (8 B per type)
     DevBuf<float2> type_par;              // per type: sigma/2 (A), 2 sqrt(eps), f32

@@ -550,11 +550,12 @@ class System:
         bit-identical either way."""
         _check(lib().gm_set_tile_pruning(self.h, 1 if enable else 0))
 
-    LAYOUTS = {"tiles": 0, "partners": 1}
+    LAYOUTS = {"tiles": 0, "partners": 1, "auto": 2}
 
     def set_pair_layout(self, layout):
-        """Nonbonded list layout: "partners" (per-atom partner lists, the
-        default) or "tiles" (8x8 cluster tiles).  Systems with more than 256
+        """Nonbonded list layout: "auto" (the default: partner lists unless they
+        need 32-bit slot numbers, then tiles), "partners" (per-atom partner
+        lists) or "tiles" (8x8 cluster tiles).  Systems with more than 256
         distinct (sigma, epsilon) pairs always use tiles."""
         _check(lib().gm_set_pair_layout(self.h, self.LAYOUTS[layout]))
 

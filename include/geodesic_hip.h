@@ -229,27 +229,35 @@ gm_status gm_set_work_chunk(gm_sys *sys, int32_t entries_per_item);
 gm_status gm_set_tile_pruning(gm_sys *sys, int32_t enable);
 
 /* Tuning: the layout of the nonbonded list that evaluations walk.
- *   GM_LAYOUT_PARTNERS (default): per atom, its partners within r_skin,
- *     those within r_cutoff + 0.1 A at the build first; four atoms share a
- *     wavefront (16 lanes each), LJ parameters come from a per-type table in
- *     LDS, every pair is computed once from each side with the same
- *     arithmetic (so the two forces are exact negatives) and its energy
- *     halved.  With pruning on (gm_set_tile_pruning) and every atom less
- *     than about 0.05 A from its build position, the second part of each
- *     list is skipped (it lies beyond r_cutoff; results bit-identical).
- *     Needs at most 256 distinct (sigma, epsilon) pairs: a system with more
- *     uses tiles whatever is set.
+ *   GM_LAYOUT_PARTNERS: per atom, its partners within r_skin, those within
+ *     r_cutoff + 0.1 A at the build first; four atoms share a wavefront (16
+ *     lanes each), LJ parameters come from a per-type table in LDS, every
+ *     pair is computed once from each side with the same arithmetic (so the
+ *     two forces are exact negatives) and its energy halved; no atomics.
+ *     With pruning on (gm_set_tile_pruning) and every atom less than about
+ *     0.05 A from its build position, the second part of each list is
+ *     skipped (it lies beyond r_cutoff; results bit-identical).  Entries are
+ *     16-bit codes relative to the atoms' group when the system's geometry
+ *     allows (neighbouring cells within 32,767 slots), else 32-bit slot
+ *     numbers.  Needs at most 256 distinct (sigma, epsilon) pairs: a system
+ *     with more uses tiles whatever is set.
  *   GM_LAYOUT_TILES: 8x8 tiles of clusters of 8 (one wavefront per tile row
- *     of 8 i clusters), each pair once.
+ *     of 8 i clusters), each pair once; tiles whose closest pair has moved
+ *     beyond r_cutoff are skipped, gradually as the atoms move.
+ *   GM_LAYOUT_AUTO (the default): partner lists, unless the first build finds
+ *     that they need 32-bit slot numbers; such a system (about 250,000 atoms
+ *     and more at liquid density) evaluates tiles from then on, which are
+ *     faster there once the atoms have moved.
  * Both evaluate the same listed pairs in f32 with fixed-point force sums;
  * results agree to f32 rounding, not bit for bit (the order of operations
- * inside a pair differs).  The pair list, its
- * statistics and shell-mode evaluations always read the tile list: with
- * partner lists a build leaves it out and it is made when first asked for.
- * Takes effect at once on a built list.  A batch's replicas always use tiles
- * (GM_ERR_INVALID_ARG for a replica).  No reference counterpart (the
- * reference walks a flat pair list, neighbor.rs / nonbonded.rs:38-41). */
-enum { GM_LAYOUT_TILES = 0, GM_LAYOUT_PARTNERS = 1 };
+ * inside a pair differs).  The pair list, its statistics and shell-mode
+ * evaluations always read the tile list: with partner lists a build leaves
+ * it out and it is made when first asked for.  Takes effect at once on a
+ * built list.  A batch's replicas always use tiles (GM_ERR_INVALID_ARG for a
+ * replica).  gm_stats.pair_layout says which layout evaluations use.  No
+ * reference counterpart (the reference walks a flat pair list, neighbor.rs /
+ * nonbonded.rs:38-41). */
+enum { GM_LAYOUT_TILES = 0, GM_LAYOUT_PARTNERS = 1, GM_LAYOUT_AUTO = 2 };
 gm_status gm_set_pair_layout(gm_sys *sys, int32_t layout);
 
 /* ---- Device-resident dynamics ----------------------------------------------

@@ -598,19 +598,28 @@ def test_rebuild_past_the_entry_capacity():
     assert e2.total == e.total and all(a.tobytes() == b.tobytes() for a, b in zip(fg, fg2))
 
 
-def test_250k_full_size_properties():
-    """At the full 250k size: the list equals the oracle's, energies and forces
-    within the bars, and reruns byte-identical."""
+@pytest.fixture(scope="module")
+def sys250k():
     s = systems.system_250k()
     say("[gpu] 250k oracle reference (cells)")
-    (wx, wy, wz), (pi, pj), t, f = oracle_reference(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params,
-                                                    method="cells", threads=16)
-    say("[gpu] 250k build + compute")
-    g, (x, y, z), e, fg = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params)
+    ref = oracle_reference(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params, method="cells", threads=16)
+    return s, ref
+
+
+@pytest.mark.parametrize("layout", ["auto", "partners"])
+def test_250k_full_size_properties(sys250k, layout):
+    """At the full 250k size: the list equals the oracle's, energies and forces
+    within the bars, and reruns byte-identical.  Its neighbouring cells lie
+    more than 32,767 slots apart: the automatic layout evaluates tiles, and
+    partner lists (asked for) hold 32-bit slot numbers."""
+    s, ((wx, wy, wz), (pi, pj), t, f) = sys250k
+    say(f"[gpu] 250k build + compute, {layout}")
+    g, (x, y, z), e, fg = run_gpu(s.x, s.y, s.z, s.sigma, s.epsilon, s.topo, s.params, layout=layout)
+    assert g.stats()["pair_layout"] == (0 if layout == "auto" else 1)
     qi, qj = g.pair_list()
     assert np.array_equal(qi, pi) and np.array_equal(qj, pj)
-    check_terms(e, t, "250k")
-    check_forces(fg, f, "250k", synthetic=True)
+    check_terms(e, t, f"250k {layout}")
+    check_forces(fg, f, f"250k {layout}", synthetic=True)
     e2, *f2 = g.compute_forces(x, y, z)
     for u, v in zip(fg, f2):
         assert u.tobytes() == v.tobytes()
