@@ -305,15 +305,19 @@ def protocol_250k(g, s, interval=20, sigma=0.02, cycles=3, seed=7):
             "kernel_ms_avg": kern, "max_displacement_at_interval_end_A": float(d)}
 
 
-def bench_batched(n_replicas=512, steps=20):
+def bench_batched(n_replicas=512, steps=20, layout="auto"):
     """512 independent replicas (log-uniform 500..20000 atoms): one graph launch
-    per batched step vs the plain path evaluating the replicas one by one."""
+    per batched step vs the plain path evaluating the replicas one by one.  The
+    batch's default layout (partner lists of every replica) is the headline;
+    the same batch switched in place to tiles, with the autotuner's shape, is
+    reported beside it."""
     import geodesic_hip as gh
     from geodesic_hip import systems
     t0 = time.perf_counter()
     reps = systems.replica_set(n_replicas)
     t_gen = time.perf_counter() - t0
     b = gh.Batch([(r.sigma, r.epsilon, r.topo, r.params) for r in reps])
+    b.set_pair_layout(layout)
     x = np.concatenate([r.x for r in reps])
     y = np.concatenate([r.y for r in reps])
     z = np.concatenate([r.z for r in reps])
@@ -321,18 +325,31 @@ def bench_batched(n_replicas=512, steps=20):
     b.build_neighbor_lists(x, y, z)
     t_build = time.perf_counter() - t0
     n_cut = sum(b.replica(k).stats()["n_pairs_cutoff"] for k in range(len(reps)))
-    chunk, waves, tune_ms = b.autotune()
-    for _ in range(3):
-        b.compute()
-    b.sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        b.compute()
-    _, last_ms = b.sync()
-    batched_ms = (time.perf_counter() - t0) * 1e3 / steps
-    # plain path: every replica as its own system (own stream, automatic work
-    # shape) with its own launches (prep, forces, finish): all 512 evaluations
-    # launched, then all waited for
+
+    def measure():
+        chunk, waves, tune_ms = b.autotune()
+        for _ in range(3):
+            b.compute()
+        b.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.compute()
+        _, last_ms = b.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        return ms, last_ms, {"entries_per_item": chunk, "waves_per_cu": waves, "ms": tune_ms}
+
+    main_layout = b.pair_layout()
+    batched_ms, last_ms, shape = measure()
+    tiles = None
+    if main_layout != "tiles":
+        b.set_pair_layout("tiles")
+        t_ms, t_last, t_shape = measure()
+        tiles = {"ms_per_batched_step": t_ms, "device_ms_last_step": t_last,
+                 "pair_interactions_per_s": n_cut / (t_ms * 1e-3), "autotuned_shape": t_shape}
+    b.close()
+    # plain path: every replica as its own system (own stream, automatic
+    # layout and work shape) with its own launches (prep, forces, finish): all
+    # 512 evaluations launched, then all waited for
     plain = []
     for r in reps:
         g = gh.System(r.sigma, r.epsilon, r.topo, r.params)
@@ -353,18 +370,23 @@ def bench_batched(n_replicas=512, steps=20):
     plain_ms = (time.perf_counter() - t0) * 1e3 / sweeps
     for g in plain:
         g.close()
+    if tiles is not None:
+        tiles["speedup_vs_plain"] = plain_ms / tiles["ms_per_batched_step"]
     step_tflops = n_cut * FLOP_PER_PAIR / (last_ms * 1e-3) / 1e12
-    return {"workload": f"{len(reps)} replicas, {b.n} atoms, log-uniform 500..20000 atoms, "
-                        "one persistent launch per step from a captured hipGraph",
+    kern = "partner-list groups of every replica" if main_layout == "partners" else "persistent tiles"
+    return {"workload": f"{len(reps)} replicas, {sum(r.n for r in reps)} atoms, log-uniform 500..20000 atoms, "
+                        f"one launch per step from a captured hipGraph ({kern})",
+            "pair_layout": main_layout,
             "roofline": {"bound": "valu", "achieved": step_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                          "frac": step_tflops / PEAK_FP32_TFLOPS,
-                         "timing": "device time of the whole graph step (prep, bonded, tiles, finish)"},
-            "n_atoms": b.n, "n_pairs_cutoff": n_cut, "ms_per_batched_step": batched_ms,
+                         "timing": "device time of the whole graph step (prep, bonded, nonbonded, finish)"},
+            "n_atoms": sum(r.n for r in reps), "n_pairs_cutoff": n_cut, "ms_per_batched_step": batched_ms,
             "device_ms_last_step": last_ms, "ms_plain_path_all_replicas": plain_ms,
             "pair_interactions_per_s": n_cut / (batched_ms * 1e-3),
             "plain_pair_interactions_per_s": n_cut / (plain_ms * 1e-3),
             "speedup_vs_plain": plain_ms / batched_ms,
-            "autotuned_shape": {"entries_per_item": chunk, "waves_per_cu": waves, "ms": tune_ms},
+            "autotuned_shape": shape,
+            "tile_layout": tiles,
             "setup_s": {"generate": t_gen, "build_lists": t_build}}
 
 
@@ -596,7 +618,7 @@ def main():
         print(json.dumps(bench_250k(layout=args.layout)))
         return
     if args.only == "batched":
-        print(json.dumps(bench_batched()))
+        print(json.dumps(bench_batched(layout=args.layout)))
         return
     if args.only == "surrogate":
         print(json.dumps(bench_surrogate()))
@@ -686,7 +708,7 @@ def main():
     if not args.no_250k:
         out["system_250k"] = bench_250k(layout=args.layout)
     if not args.no_batched:
-        bb = bench_batched()
+        bb = bench_batched(layout=args.layout)
         out["batched"] = bb
     if not args.no_surrogate:
         out["surrogate"] = bench_surrogate()

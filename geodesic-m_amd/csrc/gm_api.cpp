@@ -92,8 +92,10 @@ gm_status finish_work_items(gm_sys *s, const int *m) {
     s->n_wi = m[kMetaNwi];
     s->n_blk = m[kMetaNblk];
     s->fi_records = m[kMetaChunk] < 16;
+    const void *old_fi = s->fi_blk.p, *old_e = s->e_blk.p;
     if (s->fi_records) GM_HIP(s->fi_blk.ensure((size_t)std::max<int64_t>(s->n_blk, 1) * 192));
     GM_HIP(s->e_blk.ensure((size_t)std::max<int64_t>(s->n_blk, 1)));  // LJ energy records, per block
+    if (s->fi_blk.p != old_fi || s->e_blk.p != old_e) s->buf_gen++;
     if (!s->use_pl) s->stats.n_work_items = s->n_wi;
     return GM_OK;
 }
@@ -415,7 +417,9 @@ gm_status fit_partner_lists(gm_sys *s, int *m) {
         if ((rc = read_meta(s, m)) != GM_OK) return rc;
     }
     if (m[kMetaFentOver] || m[kMetaPlOver]) return set_error(GM_ERR_KERNEL, "list build: partner lists overflow");
+    const void *old_e = s->e_blk.p;
     GM_HIP(s->e_blk.ensure((size_t)std::max<int64_t>(s->max_sc * (64 / kPlAtoms), 1)));  // per group
+    if (s->e_blk.p != old_e) s->buf_gen++;
     s->nfent = m[kMetaNfent];
     s->npl = m[kMetaNpl];
     s->pl_wide = m[kMetaPlWide] != 0;
@@ -470,6 +474,23 @@ gm_status ensure_tiles(gm_sys *s) {
     s->stats.n_entries = s->nent;
     if ((rc = finish_work_items(s, m)) != GM_OK) return rc;
     s->tiles_built = true;
+    return GM_OK;
+}
+
+// The pair layout of a system (gm_set_pair_layout, also for a batch's
+// replicas): lists of the other layout are made from the current build.
+gm_status set_pair_layout(gm_sys *s, int32_t layout) {
+    s->layout = layout;
+    const bool pl = s->n_types > 0 && (layout == GM_LAYOUT_PARTNERS || (layout == GM_LAYOUT_AUTO && !s->auto_wide));
+    if (pl == s->use_pl) return GM_OK;
+    s->use_pl = pl;
+    drop_build_graph(s);
+    if (!s->built) return GM_OK;
+    s->stats.pair_layout = pl ? GM_LAYOUT_PARTNERS : GM_LAYOUT_TILES;
+    if (!pl) return ensure_tiles(s);
+    gm_status rc = make_partner_lists(s);
+    if (rc != GM_OK) return rc;
+    s->stats.n_partner_entries = s->npl;
     return GM_OK;
 }
 
@@ -1356,19 +1377,8 @@ gm_status gm_set_work_chunk(gm_sys *s, int32_t entries_per_item) {
 gm_status gm_set_pair_layout(gm_sys *s, int32_t layout) {
     GM_REQUIRE(s && (layout == GM_LAYOUT_TILES || layout == GM_LAYOUT_PARTNERS || layout == GM_LAYOUT_AUTO),
                "bad argument");
-    GM_REQUIRE(!s->batch_owned, "gm_set_pair_layout: a batch evaluates its replicas' tile lists");
-    s->layout = layout;
-    const bool pl = s->n_types > 0 && (layout == GM_LAYOUT_PARTNERS || (layout == GM_LAYOUT_AUTO && !s->auto_wide));
-    if (pl == s->use_pl) return GM_OK;
-    s->use_pl = pl;
-    drop_build_graph(s);
-    if (!s->built) return GM_OK;
-    s->stats.pair_layout = pl ? GM_LAYOUT_PARTNERS : GM_LAYOUT_TILES;
-    if (!pl) return ensure_tiles(s);
-    gm_status rc = make_partner_lists(s);
-    if (rc != GM_OK) return rc;
-    s->stats.n_partner_entries = s->npl;
-    return GM_OK;
+    GM_REQUIRE(!s->batch_owned, "gm_set_pair_layout: a batch sets its replicas' layout (gm_batch_set_pair_layout)");
+    return set_pair_layout(s, layout);
 }
 
 

@@ -5,12 +5,16 @@
 // Each replica is an ordinary gm_sys (own box, topology, Verlet list) sharing
 // the batch's stream.  After the lists are built, a device table of replica
 // descriptors and two item lists (replica, bonded group) and (replica, tile
-// work item) are written.  k_batch_prep writes every replica's slot-order
-// fixed-point positions and displacement bound; then k_batch_bonded runs the
-// bonded terms on a side branch while the persistent k_batch_tiles walks the
-// tile items with a fixed grid of wavefronts (per-XCD work counters); k_batch_finish
-// then reduces per-replica f64 energies and writes the concatenated f64
-// forces.  The three kernels are captured once per list build into a
+// work item or partner-list workgroup) are written.  k_batch_prep writes every
+// replica's slot-order fixed-point positions and displacement bound; then
+// k_batch_bonded runs the bonded terms on a side branch while the nonbonded
+// kernel runs: k_batch_pl over every replica's partner-list groups (the
+// default, when every replica's lists hold 16-bit codes), or the persistent
+// k_batch_tiles walking the tile items with a fixed grid of wavefronts
+// (per-XCD work counters); k_batch_finish then reduces per-replica f64
+// energies and writes the concatenated f64 forces.  The layout is decided by
+// the lists alone, never by timing, so a batch's bytes do not depend on it
+// being tuned.  The three kernels are captured once per list build into a
 // hipGraph, so a batched evaluation is one graph launch (the four kernels).  Every item runs the
 // single-system arithmetic, so each replica's forces and energies are
 // bit-identical to evaluating it alone with the same work shape.
@@ -44,8 +48,11 @@ struct gm_batch {
     int n_items = 0;
     int chunk = 0;         // entries per tile work item (0 = automatic)
     int waves_per_cu = 16; // persistent wavefronts per compute unit
+    int layout = GM_LAYOUT_AUTO;  // the caller's choice (gm_batch_set_pair_layout)
+    bool pl = false;              // the current plan evaluates partner lists
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> plan_gen;  // each replica's buffer generation when the plan was made
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool built = false;
 };
@@ -70,20 +77,33 @@ gm_status make_plan(gm_batch *b) {
     const int R = (int)b->reps.size();
     std::vector<ReplicaDesc> desc((size_t)R);
     std::vector<int2> bonded_items, tile_items;
+    b->plan_gen.assign((size_t)R, 0);
     for (int r = 0; r < R; r++) {
         gm_sys *s = b->reps[(size_t)r];
         ReplicaDesc &d = desc[(size_t)r];
         s->ev = own_bufs(s);
+        b->plan_gen[(size_t)r] = s->buf_gen;
         d.tile = tile_args(s);
         d.prep = prep_args(s, s->x.p, s->y.p, s->z.p);
         d.terms = term_args(s, s->x.p, s->y.p, s->z.p);
-        d.terms.rec = nullptr;  // (tile evaluations: bonded forces go to the accumulator)
         d.n_bonded_groups = (int)((d.terms.nb + d.terms.na + d.terms.nd + d.terms.n14 + 63) / 64);
-        d.n_wi = (int)s->n_wi;
-        d.n_blk = (int)s->n_blk;
         d.tile.bad_first = d.prep.bad_first = b->bad_first.p + r;
         for (int g = 0; g < d.n_bonded_groups; g++) bonded_items.push_back(make_int2(r, g));
-        for (int w = 0; w < d.n_wi; w++) tile_items.push_back(make_int2(r, w));
+        if (b->pl) {
+            // partner lists: type-table offsets in the slot words, bonded forces
+            // as records (term_args), one item per workgroup of kPlWaves groups
+            d.pl = pl_args(s);
+            d.srec = s->srec.p;
+            d.prep.wtype = s->atom_wtype.p;
+            d.n_wi = 0;
+            d.n_blk = (int)lj_records(s);
+            for (int g = 0; g < d.pl.n_grp; g += kPlWaves) tile_items.push_back(make_int2(r, g));
+        } else {
+            d.terms.rec = nullptr;  // (tile evaluations: bonded forces go to the accumulator)
+            d.n_wi = (int)s->n_wi;
+            d.n_blk = (int)s->n_blk;
+            for (int w = 0; w < d.n_wi; w++) tile_items.push_back(make_int2(r, w));
+        }
     }
     std::vector<int2> items(bonded_items);
     items.insert(items.end(), tile_items.begin(), tile_items.end());
@@ -92,12 +112,12 @@ gm_status make_plan(gm_batch *b) {
     std::vector<int2> scl;
     for (int pass = 0; pass < 2; pass++)
         for (int r = 0; r < R; r++)
-            if (b->reps[(size_t)r]->fi_records == (pass == 0))
+            if ((b->reps[(size_t)r]->fi_records && !b->pl) == (pass == 0))
                 for (int64_t k = 0; k < b->reps[(size_t)r]->nsc; k++) scl.push_back(make_int2(r, (int)k));
     b->n_scl = (int64_t)scl.size();
     b->n_scl_rec = 0;
     for (gm_sys *s : b->reps)
-        if (s->fi_records) b->n_scl_rec += s->nsc;
+        if (s->fi_records && !b->pl) b->n_scl_rec += s->nsc;
     GM_HIP(b->scl.ensure(std::max<size_t>(scl.size(), 1)));
     if (!scl.empty())
         GM_HIP(hipMemcpyAsync(b->scl.p, scl.data(), sizeof(int2) * scl.size(), hipMemcpyHostToDevice, st));
@@ -112,7 +132,8 @@ gm_status make_plan(gm_batch *b) {
 
     drop_graph(b);
     const int grid = std::max(1, b->waves_per_cu * (b->ctx->n_cu > 0 ? b->ctx->n_cu : 256));
-    // graph: prep -> fork -> {bonded terms on the side branch | persistent tiles} -> join -> finish
+    // graph: prep -> fork -> {bonded terms on the side branch | partner lists or persistent tiles}
+    // -> join -> finish
     // (the prep runs alone: sharing the machine with the bonded terms would
     // stretch it, and the tiles wait for it)
     GM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -122,7 +143,9 @@ gm_status make_plan(gm_batch *b) {
     if (e1 == hipSuccess) e1 = launch_batch_bonded(b->d_reps.p, b->d_items.p, b->n_bonded_items, b->side);
     if (e1 == hipSuccess) e1 = hipEventRecord(b->join, b->side);
     if (e1 == hipSuccess)
-        e1 = launch_batch_tiles(b->d_reps.p, b->d_items.p + b->n_bonded_items, b->n_items, grid, b->ctr.p, st);
+        e1 = b->pl ? launch_batch_pl(b->d_reps.p, b->d_items.p + b->n_bonded_items, b->n_items, st)
+                   : launch_batch_tiles(b->d_reps.p, b->d_items.p + b->n_bonded_items, b->n_items, grid, b->ctr.p,
+                                        st);
     if (e1 == hipSuccess) e1 = hipStreamWaitEvent(st, b->join, 0);
     if (e1 == hipSuccess)
         e1 = launch_batch_finish(b->d_reps.p, R, b->scl.p, b->n_scl_rec, b->n_scl - b->n_scl_rec, b->atom_off.p,
@@ -136,10 +159,43 @@ gm_status make_plan(gm_batch *b) {
     return GM_OK;
 }
 
+// The replicas' layout after a build: partner lists when the batch's layout
+// allows them and every replica's lists hold 16-bit codes (the batched kernel's
+// form), else tiles for every replica.  Decided by the lists alone.
+gm_status choose_layout(gm_batch *b) {
+    bool all = b->layout != GM_LAYOUT_TILES;
+    for (size_t r = 0; r < b->reps.size() && all; r++) {
+        const gm_sys *s = b->reps[r];
+        if (!pl_active(s) || s->pl_wide) {
+            if (b->layout == GM_LAYOUT_PARTNERS)
+                return set_error(GM_ERR_INVALID_ARG, "batched partner lists need 16-bit codes and LJ types; replica " +
+                                                         std::to_string(r) + " has neither");
+            all = false;
+        }
+    }
+    if (!all)
+        for (gm_sys *s : b->reps) {
+            gm_status rc = set_pair_layout(s, GM_LAYOUT_TILES);
+            if (rc != GM_OK) return rc;
+        }
+    b->pl = all;
+    return GM_OK;
+}
+
+// The plan again when a replica's evaluation buffers were reallocated since
+// it was made (a replica handle's pair-list export or statistics make a
+// partner-list replica's tile list on demand, which can grow the LJ energy
+// records the partner-list groups write).
+gm_status plan_current(gm_batch *b) {
+    for (size_t r = 0; r < b->reps.size(); r++)
+        if (r >= b->plan_gen.size() || b->reps[r]->buf_gen != b->plan_gen[r]) return make_plan(b);
+    return GM_OK;
+}
+
 gm_status apply_chunk(gm_batch *b) {
     for (gm_sys *s : b->reps) {
         s->chunk = b->chunk;
-        if (s->built) {
+        if (s->built && !pl_active(s)) {
             gm_status rc = make_work_items(s);
             if (rc != GM_OK) return rc;
         }
@@ -174,8 +230,7 @@ gm_status gm_batch_create(gm_ctx *ctx, const gm_system_desc *descs, int32_t n_re
         gm_sys *s = nullptr;
         gm_status rc = system_create_impl(ctx, &descs[r], b->stream, &s);
         if (rc != GM_OK) return fail(rc);
-        s->batch_owned = true;
-        s->use_pl = false;  // the batch evaluates its replicas' tile lists
+        s->batch_owned = true;  // (the batch sets its layout at every build)
         b->reps.push_back(s);
         b->off.push_back(b->off.back() + descs[r].n_atoms);
     }
@@ -240,6 +295,11 @@ gm_status gm_batch_build_neighbor_lists(gm_batch *b, double *x, double *y, doubl
     for (size_t r = 0; r < b->reps.size(); r++) {
         gm_sys *s = b->reps[r];
         s->chunk = b->chunk;
+        // every build starts from the batch's layout (a replica that needs
+        // 32-bit codes turns to tiles in its own build, and choose_layout below
+        // then turns the others)
+        gm_status rl = set_pair_layout(s, b->layout);
+        if (rl != GM_OK) return rl;
         const int64_t o = b->off[r];
         const size_t bytes = sizeof(double) * (size_t)s->n;
         if (s->n > 0) {
@@ -256,7 +316,9 @@ gm_status gm_batch_build_neighbor_lists(gm_batch *b, double *x, double *y, doubl
         }
     }
     GM_HIP(hipStreamSynchronize(st));
-    gm_status rc = make_plan(b);
+    gm_status rc = choose_layout(b);
+    if (rc != GM_OK) return rc;
+    rc = make_plan(b);
     if (rc != GM_OK) return rc;
     b->built = true;
     return GM_OK;
@@ -281,6 +343,8 @@ gm_status gm_batch_set_positions(gm_batch *b, const double *x, const double *y, 
 gm_status gm_batch_compute(gm_batch *b) {
     GM_REQUIRE(b, "NULL batch");
     if (!b->built) return set_error(GM_ERR_NOT_BUILT, "gm_batch_compute before gm_batch_build_neighbor_lists");
+    gm_status rc = plan_current(b);
+    if (rc != GM_OK) return rc;
     GM_HIP(hipEventRecord(b->ev0, b->stream));
     GM_HIP(hipGraphLaunch(b->exec, b->stream));
     GM_HIP(hipEventRecord(b->ev1, b->stream));
@@ -348,9 +412,53 @@ gm_status gm_batch_set_shape(gm_batch *b, int32_t entries_per_item, int32_t wave
     return GM_OK;
 }
 
+gm_status gm_batch_set_pair_layout(gm_batch *b, int32_t layout) {
+    GM_REQUIRE(b && (layout == GM_LAYOUT_TILES || layout == GM_LAYOUT_PARTNERS || layout == GM_LAYOUT_AUTO),
+               "gm_batch_set_pair_layout: bad argument");
+    b->layout = layout;
+    if (!b->built) return GM_OK;
+    b->built = false;  // (until the new plan is made)
+    for (gm_sys *s : b->reps) {
+        s->chunk = b->chunk;
+        gm_status rc = set_pair_layout(s, layout);
+        if (rc != GM_OK) return rc;
+    }
+    gm_status rc = choose_layout(b);
+    if (rc != GM_OK) return rc;
+    rc = make_plan(b);
+    if (rc != GM_OK) return rc;
+    b->built = true;
+    return GM_OK;
+}
+
+gm_status gm_batch_get_pair_layout(gm_batch *b, int32_t *layout) {
+    GM_REQUIRE(b && layout, "NULL argument");
+    *layout = b->pl ? GM_LAYOUT_PARTNERS : GM_LAYOUT_TILES;
+    return GM_OK;
+}
+
 gm_status gm_batch_autotune(gm_batch *b, int32_t *entries_per_item, int32_t *waves_per_cu, float *best_ms) {
     GM_REQUIRE(b, "NULL batch");
     if (!b->built) return set_error(GM_ERR_NOT_BUILT, "gm_batch_autotune before gm_batch_build_neighbor_lists");
+    if (b->pl) {
+        // partner lists have no tile shape: time the step, report shape 0 x 0
+        gm_status rc = plan_current(b);
+        if (rc != GM_OK) return rc;
+        float total = 0.f;
+        for (int it = 0; it < 8; it++) {
+            GM_HIP(hipEventRecord(b->ev0, b->stream));
+            GM_HIP(hipGraphLaunch(b->exec, b->stream));
+            GM_HIP(hipEventRecord(b->ev1, b->stream));
+            GM_HIP(hipEventSynchronize(b->ev1));
+            float t = 0.f;
+            GM_HIP(hipEventElapsedTime(&t, b->ev0, b->ev1));
+            if (it >= 2) total += t;
+        }
+        if (entries_per_item) *entries_per_item = 0;
+        if (waves_per_cu) *waves_per_cu = 0;
+        if (best_ms) *best_ms = total / 6.0f;
+        return GM_OK;
+    }
     static const int chunks[] = {8, 16, 24, 32};  // whole fixed-point blocks
     static const int waves[] = {8, 12, 16, 20};  // at most 5 per SIMD are resident
     float best = 1e30f;

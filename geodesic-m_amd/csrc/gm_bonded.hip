@@ -170,7 +170,8 @@ __global__ __launch_bounds__(1024) void k_finish(Segs sg, double *eout, const in
 // blocks finish (replica, supercluster) pairs of scl exactly like k_finish,
 // writing forces at the replica's atom offset: the first n_rec pairs (of
 // replicas that keep i-force records) four per block with four wavefronts
-// each, the next n_atm sixteen per block with one wavefront each.
+// each, the next n_atm sixteen per block with one wavefront each (which also
+// adds the bonded force records of a partner-list replica).
 __global__ __launch_bounds__(1024) void k_batch_finish(const ReplicaDesc *__restrict__ reps, int n_reps,
                                                        const int2 *__restrict__ scl, int64_t n_rec, int64_t n_atm,
                                                        const int64_t *__restrict__ atom_off, double *eout,
@@ -198,8 +199,10 @@ __global__ __launch_bounds__(1024) void k_batch_finish(const ReplicaDesc *__rest
     const bool valid = bb < b_rec ? pidx < n_rec : pidx < n_rec + n_atm;
     const int2 rs = valid ? scl[pidx] : make_int2(0, 0);
     const ReplicaDesc &R = reps[rs.x];
+    // (a partner-list replica's bonded forces are records: R.terms.rec; else null)
     sc_finish(valid, rs.y, G, R.terms.F, R.terms.n_pad, R.tile.fi_blk, R.tile.sc_blk_off,
-              R.tile.fi_scale, R.tile.slot_atom, atom_off[rs.x], fx, fy, fz, R.tile.bad, R.tile.bad_first);
+              R.tile.fi_scale, R.tile.slot_atom, atom_off[rs.x], fx, fy, fz, R.tile.bad, R.tile.bad_first,
+              R.terms.rec, R.terms.n_rec, R.srec);
 }
 
 }  // namespace

@@ -64,6 +64,7 @@ constexpr int kMetaWords = 32;
 // A partner-list wavefront evaluates kPlAtoms consecutive slots (a group),
 // kPlLanes lanes each; lane l of a slot takes the list's entries l, l + 16, ...
 constexpr int kPlAtoms = 4, kPlLanes = 16;
+constexpr int kPlWaves = 4;  // wavefronts (groups) per workgroup of the partner-list kernels
 // Each slot's list holds first the partners within r_cutoff + kPlInner (A)
 // at the build, then the others; both parts are padded with the slot itself
 // (r2 = 0: skipped) to the group's longest part rounded up to kPlPad (whole
@@ -226,6 +227,10 @@ struct gm_ctx {
 
 struct gm_sys {
     gm_ctx *ctx = nullptr;
+    // bumped whenever an evaluation buffer that a batch's captured plan points
+    // at (LJ energy records, i-force records) is reallocated: the batch then
+    // makes its plan again before it launches
+    uint64_t buf_gen = 0;
     hipStream_t stream = nullptr;
     bool owns_stream = true;
     hipEvent_t ev_b0 = nullptr, ev_b1 = nullptr;
@@ -441,6 +446,12 @@ gm_status collect(gm_sys *s, gm_energy_terms *e);
 // collected evaluation, or GM_OK.
 gm_status nonfinite_status(const gm_sys *s);
 gm_status make_work_items(gm_sys *s);
+// The tile list and work items of a system built with partner lists (made
+// from the build's binning when first asked for).
+gm_status ensure_tiles(gm_sys *s);
+// The system's pair layout (GM_LAYOUT_*); lists of the other layout are made
+// from the current build.
+gm_status set_pair_layout(gm_sys *s, int32_t layout);
 // Fixed-point span L (per axis) and xy column counts: every constant that
 // depends on the length of one fixed-point step.
 void set_quant_span(gm_sys *s, const double L[3], int ncx, int ncy);
@@ -517,6 +528,10 @@ hipError_t launch_batch_prep(const ReplicaDesc *reps, const int *atom_rep, const
 hipError_t launch_batch_tiles(const ReplicaDesc *reps, const int2 *items, int n_items, int grid, unsigned *ctr,
                               hipStream_t st);
 hipError_t launch_batch_bonded(const ReplicaDesc *reps, const int2 *items, int n_items, hipStream_t st);
+// items: (replica, first group) per workgroup of kPlWaves partner-list groups
+hipError_t launch_batch_pl(const ReplicaDesc *reps, const int2 *items, int n_items, hipStream_t st);
+struct PlArgs;
+PlArgs pl_args(gm_sys *s);
 hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int2 *scl, int64_t n_rec, int64_t n_atm,
                                const int64_t *atom_off, double *eout, double *fx, double *fy, double *fz,
                                hipStream_t st);

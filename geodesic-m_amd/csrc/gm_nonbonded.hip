@@ -528,23 +528,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
 // since the build, every partner of a list's second part (farther than
 // r_cutoff + kPlInner at the build) is now beyond r_cutoff, where the pair's
 // terms are exactly zero: only the first part is evaluated.
-struct PlArgs {
-    const int4 *qpos;    // per slot: fixed-point x, y, z; w = the atom's type-table offset
-    const float2 *type_par;  // per type: sigma/2 (A), 2 sqrt(eps)
-    int n_types;
-    float sig_scale;     // sigma/2 in the kernel's length unit: A * sig_scale
-    const int *goff;     // per group: list offset
-    const int2 *glen;    // per group: padded first-part length, whole padded length
-    const int *nbr;      // partner slots
-    const unsigned *disp;  // displacement words (null: no pruning)
-    float dmax_first;      // evaluate the first part only while the displacement is at most this
-    unsigned long long *F;  // per slot: its LJ force in fixed point (plain stores)
-    unsigned *bad;
-    double *e_grp;       // per group: half its LJ energy sum
-    int n_pad;
-    int n_grp;
-    NbParams p;
-};
+// The kernel's arguments, PlArgs, are declared in gm_terms.hpp.
+
+// Loads from device memory at byte address p, typed as global: pointers that
+// come from a descriptor in memory (a batch's replica table) rather than from
+// kernel arguments would otherwise be loaded as generic (flat) addresses with
+// 64-bit arithmetic per load; through these the loads take a scalar base and
+// a 32-bit offset, and a wave-uniform one becomes a scalar load.
+typedef int gm_v4i __attribute__((ext_vector_type(4)));
+typedef int gm_v2i __attribute__((ext_vector_type(2)));
+__device__ inline int4 gload_i4(const char *p) {
+    const gm_v4i v = *(const __attribute__((address_space(1))) gm_v4i *)(const __attribute__((address_space(1))) char *)p;
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+__device__ inline int2 gload_i2(const char *p) {
+    const gm_v2i v = *(const __attribute__((address_space(1))) gm_v2i *)(const __attribute__((address_space(1))) char *)p;
+    return make_int2(v.x, v.y);
+}
+__device__ inline int gload_i1(const char *p) {
+    return *(const __attribute__((address_space(1))) int *)(const __attribute__((address_space(1))) char *)p;
+}
 
 // Item (of N) evaluated by workgroup W0 + k of a launch whose first W0
 // workgroups do other work: workgroup w runs on XCD w % 8, and XCD x takes a
@@ -572,7 +575,6 @@ __device__ inline float slot_sum(float v) {
     return v;
 }
 
-constexpr int kPlWaves = 4;  // wavefronts (groups) per workgroup of the partner-list kernel
 static_assert((64 / kPlAtoms) % kPlWaves == 0, "whole workgroups per supercluster");
 
 // C16: the lists hold compact codes (slot minus the group's first slot plus
@@ -596,37 +598,39 @@ __device__ __attribute__((always_inline)) inline void pl_work(const int g, const
     }
     bool first_only = false;
     if (A.disp) {
-        float d = lane < kDispSlots ? __uint_as_float(A.disp[lane * kDispStride]) : 0.f;
+        float d = lane < kDispSlots
+                      ? __int_as_float(gload_i1(reinterpret_cast<const char *>(A.disp) + lane * kDispStride * 4))
+                      : 0.f;
         d = row_max16(d);  // lane 0 holds the largest of the kDispSlots (16) words
         const float dmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(d)));
         first_only = dmax <= A.dmax_first;  // (false for a NaN bound)
     }
-    const int2 gl = A.glen[g];
+    const int2 gl = gload_i2(reinterpret_cast<const char *>(A.glen) + (size_t)g * 8);
     const int rounds = (first_only ? gl.x : gl.y) / kPlLanes;
     // the group's list base (wave-uniform) and the lane's byte offset in it;
     // loads address base + 32-bit offset, so that no 64-bit address is formed per load
     const char *__restrict__ nbase =
-        C16 ? reinterpret_cast<const char *>(reinterpret_cast<const uint16_t *>(A.nbr) + A.goff[g])
-            : reinterpret_cast<const char *>(A.nbr + A.goff[g]);
+        reinterpret_cast<const char *>(A.nbr) +
+        (size_t)gload_i1(reinterpret_cast<const char *>(A.goff) + (size_t)g * 4) * (C16 ? 2 : 4);
     const unsigned lofs = C16 ? (unsigned)(al * gl.y) * 2u + (unsigned)ls * 4u : (unsigned)(al * gl.y + ls) * 4u;
     const char *__restrict__ qpb = reinterpret_cast<const char *>(A.qpos);
-    auto qpos_of = [&](int c) { return *reinterpret_cast<const int4 *>(qpb + ((unsigned)c << 4)); };
+    auto qpos_of = [&](int c) { return gload_i4(qpb + ((unsigned)c << 4)); };
     // a partner's slot data from its list entry: the slot number, or the
     // compact code (from the guard start, offset by the group's first slot)
     const char *__restrict__ qpg = reinterpret_cast<const char *>(A.qpos - kQposGuard);
     const unsigned gslot = (unsigned)(g * kPlAtoms);
     auto partner = [&](int c) {
-        return C16 ? *reinterpret_cast<const int4 *>(qpg + ((gslot + (unsigned)c) << 4)) : qpos_of(c);
+        return C16 ? gload_i4(qpg + ((gslot + (unsigned)c) << 4)) : qpos_of(c);
     };
     // the entries of rounds r and r + 1 (r even)
     auto codes2 = [&](int r, int &c0, int &c1) {
         if (C16) {
-            const unsigned v = *reinterpret_cast<const unsigned *>(nbase + lofs + (unsigned)(r >> 1) * 64u);
+            const unsigned v = (unsigned)gload_i1(nbase + lofs + (unsigned)(r >> 1) * 64u);
             c0 = (int)(v & 0xffffu);
             c1 = (int)(v >> 16);
         } else {
-            c0 = *reinterpret_cast<const int *>(nbase + lofs + r * (kPlLanes * 4));
-            c1 = *reinterpret_cast<const int *>(nbase + lofs + (r + 1) * (kPlLanes * 4));
+            c0 = gload_i1(nbase + lofs + r * (kPlLanes * 4));
+            c1 = gload_i1(nbase + lofs + (r + 1) * (kPlLanes * 4));
         }
     };
     // a slot word's type parameters (its low 11 bits are the table offset)
@@ -809,6 +813,22 @@ __global__ __launch_bounds__(64) void k_batch_bonded(const ReplicaDesc *__restri
     if (k < t.nb + t.na + t.nd + t.n14) term_dispatch(t, k);
 }
 
+// Partner-list groups of every replica: one workgroup per item (replica, first
+// of kPlWaves consecutive groups; a replica's group count is a multiple of
+// kPlWaves, so a workgroup never spans two replicas and its LDS type table is
+// one replica's).  Each XCD takes a contiguous range of items, so neighbouring
+// groups of one replica share its L2.  Every group runs the single-system
+// code: a replica's forces and energies are those of evaluating it alone.
+// Replicas are cubic and their lists hold 16-bit codes (the batch checks both).
+__global__ __launch_bounds__(64 * kPlWaves) __attribute__((amdgpu_waves_per_eu(8))) void k_batch_pl(
+    const ReplicaDesc *__restrict__ reps, const int2 *__restrict__ items, int n_items) {
+    const int2 it = items[xcd_group((int)blockIdx.x, 0, n_items)];
+    // (replica and group are the same in the whole wavefront: scalar registers)
+    const int r = __builtin_amdgcn_readfirstlane(it.x);
+    const int g = __builtin_amdgcn_readfirstlane(it.y + (int)(threadIdx.x >> 6));
+    pl_work<true, true>(g, (int)(threadIdx.x & 63), reps[r].pl);
+}
+
 }  // namespace
 
 namespace gm {
@@ -978,6 +998,11 @@ hipError_t launch_batch_prep(const ReplicaDesc *reps, const int *atom_rep, const
 hipError_t launch_batch_tiles(const ReplicaDesc *reps, const int2 *items, int n_items, int grid, unsigned *ctr,
                               hipStream_t st) {
     if (n_items > 0 && grid > 0) k_batch_tiles<<<(unsigned)grid, 64, 0, st>>>(reps, items, n_items, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_pl(const ReplicaDesc *reps, const int2 *items, int n_items, hipStream_t st) {
+    if (n_items > 0) k_batch_pl<<<(unsigned)n_items, 64 * kPlWaves, 0, st>>>(reps, items, n_items);
     return hipGetLastError();
 }
 

@@ -326,14 +326,36 @@ struct PrepArgs {
     int n;
 };
 
-// One replica of a batch: its tile and bonded arguments and item split.
+// Everything one partner-list group reads (one system or one replica).
+struct PlArgs {
+    const int4 *qpos;    // per slot: fixed-point x, y, z; w = the atom's type-table offset
+    const float2 *type_par;  // per type: sigma/2 (A), 2 sqrt(eps)
+    int n_types;
+    float sig_scale;     // sigma/2 in the kernel's length unit: A * sig_scale
+    const int *goff;     // per group: list offset
+    const int2 *glen;    // per group: padded first-part length, whole padded length
+    const int *nbr;      // partner slots
+    const unsigned *disp;  // displacement words (null: no pruning)
+    float dmax_first;      // evaluate the first part only while the displacement is at most this
+    unsigned long long *F;  // per slot: its LJ force in fixed point (plain stores)
+    unsigned *bad;
+    double *e_grp;       // per group: half its LJ energy sum
+    int n_pad;
+    int n_grp;
+    NbParams p;
+};
+
+// One replica of a batch: its arguments for either layout (the batch
+// evaluates every replica in one), the bonded arguments and the item split.
 struct ReplicaDesc {
     TileArgs tile;
+    PlArgs pl;                         // partner-list batches
+    const int2 *srec;                  // partner-list batches: per slot, its atom's bonded records
     TermArgs terms;
     PrepArgs prep;
     int n_bonded_groups;               // items [0, n) are bonded groups, then work items
     int n_wi;
-    int n_blk;                         // fixed-point blocks (LJ energy segment length)
+    int n_blk;                         // LJ energy records: fixed-point blocks, or partner-list groups
 };
 
 // Term k of the flat range [bonds | angles | dihedrals | 1-4]; writes its energy.

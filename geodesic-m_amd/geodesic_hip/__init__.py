@@ -188,6 +188,8 @@ def lib():
             "gm_batch_sync": ([_P, _P, C.POINTER(C.c_float)], C.c_int),
             "gm_batch_get_forces": ([_P, _P, _P, _P], C.c_int),
             "gm_batch_set_shape": ([_P, C.c_int32, C.c_int32], C.c_int),
+            "gm_batch_set_pair_layout": ([_P, C.c_int32], C.c_int),
+            "gm_batch_get_pair_layout": ([_P, C.POINTER(C.c_int32)], C.c_int),
             "gm_batch_autotune": ([_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_float)],
                                   C.c_int),
             "gm_surrogate_create": ([_P, C.POINTER(SystemDesc), C.POINTER(SurrogateDesc), C.POINTER(_P)],
@@ -566,9 +568,10 @@ class System:
 
 
 class Batch:
-    """gm_batch: independent replicas evaluated by one persistent launch from a
-    captured hipGraph.  `members` is a list of (sigma, epsilon, topo, params);
-    positions and forces are concatenated in member order."""
+    """gm_batch: independent replicas evaluated by one nonbonded launch from a
+    captured hipGraph (partner lists of every replica, or the persistent tile
+    kernel).  `members` is a list of (sigma, epsilon, topo, params); positions
+    and forces are concatenated in member order."""
 
     def __init__(self, members, ctx: Context | None = None):
         ctx = ctx or default_context()
@@ -629,10 +632,25 @@ class Batch:
         _check(lib().gm_batch_get_forces(self.h, _p(fx), _p(fy), _p(fz)))
         return fx, fy, fz
 
+    def set_pair_layout(self, layout):
+        """"auto" (the default: partner lists when every replica's lists hold
+        16-bit codes, else tiles for all), "partners" or "tiles"; applied at
+        once when built, else at the build."""
+        _check(lib().gm_batch_set_pair_layout(self.h, System.LAYOUTS[layout]))
+
+    def pair_layout(self):
+        """The layout the current plan evaluates: "partners" or "tiles"."""
+        v = C.c_int32(-1)
+        _check(lib().gm_batch_get_pair_layout(self.h, C.byref(v)))
+        return {0: "tiles", 1: "partners"}[v.value]
+
     def set_shape(self, entries_per_item, waves_per_cu):
+        """Tile layout work shape (speed only: every shape gives the same bytes)."""
         _check(lib().gm_batch_set_shape(self.h, int(entries_per_item), int(waves_per_cu)))
 
     def autotune(self):
+        """Tile layout: the fastest (entries per item, waves per CU) and its ms,
+        kept.  Partner lists have no shape: (0, 0, ms of the step)."""
         c, w, ms = C.c_int32(0), C.c_int32(0), C.c_float(0.0)
         _check(lib().gm_batch_autotune(self.h, C.byref(c), C.byref(w), C.byref(ms)))
         return c.value, w.value, ms.value

@@ -337,11 +337,11 @@ gm_status gm_jacobi_factor(gm_sys *sys, double total_energy, double *out);
 
 /* ---- Batched replicas [ext: no reference counterpart] ---------------------
  * Many independent systems (own atom counts, boxes, topologies, parameters)
- * evaluated by one persistent launch replayed from a captured hipGraph, for
+ * evaluated by one nonbonded launch replayed from a captured hipGraph, for
  * landscape mapping.  Replica boxes must be cubic.  Positions/forces are
  * concatenated in replica order.
  * Each replica's forces and energies are bit-identical to evaluating it alone
- * with the same work shape. */
+ * in the same pair layout (no work shape changes a bit). */
 typedef struct gm_batch gm_batch;
 gm_status gm_batch_create(gm_ctx *ctx, const gm_system_desc *descs, int32_t n_replicas,
                           gm_batch **out);
@@ -358,12 +358,22 @@ gm_status gm_batch_compute(gm_batch *b);
  * NULL); ms the device time of the last gm_batch_compute (may be NULL). */
 gm_status gm_batch_sync(gm_batch *b, gm_energy_terms *e, float *ms);
 gm_status gm_batch_get_forces(gm_batch *b, double *fx, double *fy, double *fz);
-/* Work shape: list entries per tile work item (0 = automatic, at most 32)
- * and persistent wavefronts per compute unit.  The shape fixes how pair
- * forces are grouped before fixed-point conversion, so pin it (or the
- * autotuner's choice) for byte-identical reruns. */
+/* Pair layout of every replica (GM_LAYOUT_*; applied at once when built,
+ * else at the build).  GM_LAYOUT_AUTO (the default): partner lists (one
+ * launch over every replica's groups) when every replica's lists hold 16-bit
+ * codes, else tiles (the persistent tile kernel) for all; decided by the lists
+ * alone.  GM_LAYOUT_PARTNERS: GM_ERR_INVALID_ARG at the build when a replica's
+ * lists would need 32-bit codes. */
+gm_status gm_batch_set_pair_layout(gm_batch *b, int32_t layout);
+/* The layout the current plan evaluates: GM_LAYOUT_PARTNERS or GM_LAYOUT_TILES. */
+gm_status gm_batch_get_pair_layout(gm_batch *b, int32_t *layout);
+/* Tile layout work shape: list entries per tile work item (0 = automatic, at
+ * most 32) and persistent wavefronts per compute unit.  Speed only: i forces
+ * are rounded per fixed-point block of 8 entries, so every shape gives the
+ * same bytes. */
 gm_status gm_batch_set_shape(gm_batch *b, int32_t entries_per_item, int32_t waves_per_cu);
-/* Times a grid of shapes on the current lists and keeps the fastest. */
+/* Tile layout: times a grid of shapes on the current lists and keeps the
+ * fastest.  Partner lists have no shape: the step is timed and 0, 0 returned. */
 gm_status gm_batch_autotune(gm_batch *b, int32_t *entries_per_item, int32_t *waves_per_cu,
                             float *best_ms);
 

@@ -1,6 +1,7 @@
 """Batched replicas on the GPU (no reference counterpart): every replica of a
 batch against its own f64 oracle, against the same replica evaluated alone
-(byte-identical), across graph replays, and across work shapes."""
+(byte-identical), across graph replays, and across work shapes; in both
+batch layouts (the default, partner lists of every replica, and tiles)."""
 import sys
 
 import numpy as np
@@ -34,13 +35,19 @@ def concat(reps):
             np.concatenate([r.z for r in reps]))
 
 
-@pytest.fixture(scope="module")
-def batch_case():
+# the batch's layout setting -> the layout its replicas then evaluate
+LAYOUT = {"auto": "partners", "tiles": "tiles"}
+
+
+@pytest.fixture(scope="module", params=["auto", "tiles"])
+def batch_case(request):
     reps = make_replicas(SIZES)
-    say("[gpu] batch build")
+    say(f"[gpu] batch build, layout {request.param}")
     b = gh.Batch([(r.sigma, r.epsilon, r.topo, r.params) for r in reps])
+    b.set_pair_layout(request.param)
     x, y, z = concat(reps)
     b.build_neighbor_lists(x, y, z)
+    assert b.pair_layout() == LAYOUT[request.param]
     b.compute()
     e, ms = b.sync()
     f = b.get_forces()
@@ -69,13 +76,13 @@ def test_each_replica_matches_its_oracle(batch_case):
 
 
 def test_batch_equals_each_replica_alone(batch_case):
-    """A plain system on the tile layout (the one a batch evaluates): same work
-    items, same arithmetic, forces and energies byte-identical."""
+    """A plain system in the layout the batch evaluates: same lists, same
+    arithmetic, forces and energies byte-identical."""
     reps, b, _, e, (fx, fy, fz) = batch_case
     for k, r in enumerate(reps):
-        say(f"[gpu] plain replica {k}")
+        say(f"[gpu] plain replica {k}, {b.pair_layout()}")
         g = gh.System(r.sigma, r.epsilon, r.topo, r.params)
-        g.set_pair_layout("tiles")
+        g.set_pair_layout(b.pair_layout())
         x, y, z = r.x.copy(), r.y.copy(), r.z.copy()
         g.build_neighbor_list(x, y, z)
         ep, px, py, pz = g.compute_forces(x, y, z)
@@ -103,11 +110,12 @@ def test_graph_replays_are_byte_identical(batch_case):
 
 @pytest.mark.parametrize("entries,waves", [(6, 8), (8, 20), (16, 12), (32, 16)])
 def test_work_shape_does_not_change_results(batch_case, entries, waves):
-    """Work items of 8 entries (6 rounds up) leave i forces as per-block
+    """Tiles: work items of 8 entries (6 rounds up) leave i forces as per-block
     records, 16 and 32 as atomics (and the finish kernel then takes one
     wavefront per supercluster); the persistent grid varies too.  Forces are
     rounded per block of 8 entries either way: every replica's bytes equal
-    the automatic shape's, and the replica evaluated alone."""
+    the automatic shape's, and the replica evaluated alone.  Partner lists
+    have no shape: the same calls leave every byte as it was."""
     reps, b, (x, y, z), e0, f0 = batch_case
     say(f"[gpu] batch shape {entries} entries per item, {waves} waves per CU")
     b.set_shape(entries, waves)
@@ -121,7 +129,7 @@ def test_work_shape_does_not_change_results(batch_case, entries, waves):
     fx, fy, fz = f
     for k, r in enumerate(reps[:2]):
         g = gh.System(r.sigma, r.epsilon, r.topo, r.params)
-        g.set_pair_layout("tiles")  # the layout a batch evaluates
+        g.set_pair_layout(b.pair_layout())
         px, py, pz = r.x.copy(), r.y.copy(), r.z.copy()
         g.build_neighbor_list(px, py, pz)
         g.set_work_chunk(entries)
@@ -136,7 +144,10 @@ def test_autotuned_shape_is_pinned_and_reproducible(batch_case):
     reps, b, (x, y, z), e0, f0 = batch_case
     say("[gpu] autotune")
     c, w, ms = b.autotune()
-    assert c in (8, 16, 24, 32) and 1 <= w <= 64 and ms > 0.0
+    if b.pair_layout() == "tiles":
+        assert c in (8, 16, 24, 32) and 1 <= w <= 64 and ms > 0.0
+    else:  # partner lists: no shape to tune, the step timed
+        assert c == 0 and w == 0 and ms > 0.0
     b.compute()
     e1, _ = b.sync()
     f1 = b.get_forces()
@@ -165,6 +176,84 @@ def test_replica_handles_refuse_list_and_shape_changes(batch_case):
     e, _ = b.sync()
     for u, v in zip(b.get_forces(), f0):
         assert u.tobytes() == v.tobytes()
+
+
+def test_layout_switch_in_place(batch_case):
+    """A built batch switched to the other layout and back: the other layout's
+    bytes equal that layout evaluated alone (first replica), and switching
+    back restores every byte."""
+    reps, b, (x, y, z), e0, f0 = batch_case
+    here = b.pair_layout()
+    other = "tiles" if here == "partners" else "partners"
+    say(f"[gpu] batch layout {here} -> {other} -> {here}")
+    b.set_pair_layout(other)
+    assert b.pair_layout() == other
+    b.compute()
+    e1, _ = b.sync()
+    fx, fy, fz = b.get_forces()
+    r = reps[0]
+    g = gh.System(r.sigma, r.epsilon, r.topo, r.params)
+    g.set_pair_layout(other)
+    px, py, pz = r.x.copy(), r.y.copy(), r.z.copy()
+    g.build_neighbor_list(px, py, pz)
+    ep, gx, gy, gz = g.compute_forces(px, py, pz)
+    o1 = b.offsets[1]
+    assert gx.tobytes() == fx[:o1].tobytes() and gy.tobytes() == fy[:o1].tobytes()
+    assert gz.tobytes() == fz[:o1].tobytes() and ep.total == e1[0].total
+    b.set_pair_layout("auto" if here == "partners" else "tiles")
+    assert b.pair_layout() == here
+    b.compute()
+    e2, _ = b.sync()
+    for u, v in zip(b.get_forces(), f0):
+        assert u.tobytes() == v.tobytes()
+    assert [t.total for t in e2] == [t.total for t in e0]
+
+
+def test_replica_exports_then_batched_evaluations():
+    """Statistics and pair-list exports through replica handles make a
+    partner-list replica's tile list on demand, which can reallocate the LJ
+    energy records its groups write; the batch then makes its plan again
+    before launching.  Partner-list evaluations after the exports equal those
+    of a batch never exported from, and the same batch switched to tiles with
+    a new work shape equals each replica evaluated alone in tiles."""
+    reps = make_replicas([500, 1200, 2400, 3100, 4000, 5200, 800, 1800], seed=77)
+    members = [(r.sigma, r.epsilon, r.topo, r.params) for r in reps]
+    x, y, z = concat(reps)
+    say("[gpu] batch without exports")
+    b0 = gh.Batch(members)
+    b0.build_neighbor_lists(x.copy(), y.copy(), z.copy())
+    b0.compute()
+    e0, _ = b0.sync()
+    f0 = b0.get_forces()
+    b0.close()
+    say("[gpu] batch with statistics and exports before evaluating")
+    b = gh.Batch(members)
+    b.build_neighbor_lists(x, y, z)
+    assert b.pair_layout() == "partners"
+    n_cut = sum(b.replica(k).stats()["n_pairs_cutoff"] for k in range(len(reps)))
+    assert n_cut > 0
+    for it in range(3):
+        b.compute()
+        e, _ = b.sync()
+        for u, v in zip(b.get_forces(), f0):
+            assert u.tobytes() == v.tobytes(), it
+        assert [t.total for t in e] == [t.total for t in e0] and [t.lj for t in e] == [t.lj for t in e0]
+    say("[gpu] switched to tiles, 8 entries per item")
+    b.set_pair_layout("tiles")
+    b.set_shape(8, 8)
+    b.compute()
+    e, _ = b.sync()
+    fx, fy, fz = b.get_forces()
+    for k, r in enumerate(reps):
+        g = gh.System(r.sigma, r.epsilon, r.topo, r.params)
+        g.set_pair_layout("tiles")
+        px, py, pz = r.x.copy(), r.y.copy(), r.z.copy()
+        g.build_neighbor_list(px, py, pz)
+        ep, gx, gy, gz = g.compute_forces(px, py, pz)
+        o0, o1 = b.offsets[k], b.offsets[k + 1]
+        assert gx.tobytes() == fx[o0:o1].tobytes() and gy.tobytes() == fy[o0:o1].tobytes(), k
+        assert gz.tobytes() == fz[o0:o1].tobytes() and ep.total == e[k].total, k
+    b.close()
 
 
 def test_batch_errors():

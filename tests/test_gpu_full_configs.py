@@ -64,45 +64,59 @@ def oracle_eval(x, y, z, s, p, pair_list=None):
 # ---- 512 replicas -----------------------------------------------------------
 
 def test_512_replicas_against_the_oracle_and_alone():
+    """The 512-replica batch in its default layout (partner lists), then
+    switched in place to tiles with the autotuner's shape: every list
+    bit-exact, every replica within the bars of its f64 oracle, and a spread
+    sample byte-identical to the replica evaluated alone in the same layout."""
     reps = systems.replica_set(512)
     n_tot = sum(r.n for r in reps)
-    say(f"[gpu] 512 replicas, {n_tot} atoms: batch build + autotune + compute")
+    say(f"[gpu] 512 replicas, {n_tot} atoms: batch build + compute")
     b = gh.Batch([(r.sigma, r.epsilon, r.topo, r.params) for r in reps])
     x = np.concatenate([r.x for r in reps])
     y = np.concatenate([r.y for r in reps])
     z = np.concatenate([r.z for r in reps])
     b.build_neighbor_lists(x, y, z)
-    chunk, waves, _ = b.autotune()
-    b.compute()
-    e, _ = b.sync()
-    fx, fy, fz = b.get_forces()
-    say(f"[gpu] autotuned shape: {chunk} entries per item, {waves} waves per CU")
-    worst_rms, worst_e = 0.0, 0.0
-    for k, r in enumerate(reps):
-        o0, o1 = b.offsets[k], b.offsets[k + 1]
-        (wx, wy, wz), (pi, pj), t, (gx, gy, gz) = oracle_eval(r.x, r.y, r.z, r, r.params)
-        assert x[o0:o1].tobytes() == wx.tobytes(), k  # wrapped in place as neighbor.rs:13-15
-        qi, qj = b.replica(k).pair_list()
-        assert np.array_equal(qi, pi) and np.array_equal(qj, pj), f"replica {k} (n={r.n}): list"
-        check_terms(e[k], t, f"replica {k}")
-        rms = force_rms_rel(fx[o0:o1], fy[o0:o1], fz[o0:o1], gx, gy, gz)
-        assert rms <= F_RMS, f"replica {k}: force rms {rms:.3e}"
-        worst_rms = max(worst_rms, rms)
-        worst_e = max(worst_e, abs(e[k].total - t.total) / abs(t.total))
-        if k % 64 == 0:
-            say(f"[gpu] replica {k}/512 (n={r.n}) ok; worst so far: force rms {worst_rms:.2e}, energy {worst_e:.2e}")
+    assert b.pair_layout() == "partners"
+    ref = []
     sample = sorted({0, 1, 137, 255, 384, 511, int(np.argmax([r.n for r in reps])), int(np.argmin([r.n for r in reps]))})
-    for k in sample:
-        r = reps[k]
-        g = gh.System(r.sigma, r.epsilon, r.topo, r.params)
-        g.set_pair_layout("tiles")  # the layout a batch evaluates
-        px, py, pz = r.x.copy(), r.y.copy(), r.z.copy()
-        g.build_neighbor_list(px, py, pz)
-        ep, gx, gy, gz = g.compute_forces(px, py, pz)  # its own automatic work shape
-        o0, o1 = b.offsets[k], b.offsets[k + 1]
-        assert gx.tobytes() == fx[o0:o1].tobytes() and gy.tobytes() == fy[o0:o1].tobytes(), k
-        assert gz.tobytes() == fz[o0:o1].tobytes() and ep.total == e[k].total and ep.lj == e[k].lj, k
-    say(f"[gpu] batch vs alone byte-identical on replicas {sample}")
+    for layout in ("partners", "tiles"):
+        if layout == "tiles":
+            b.set_pair_layout("tiles")
+            chunk, waves, _ = b.autotune()
+            say(f"[gpu] tiles, autotuned shape: {chunk} entries per item, {waves} waves per CU")
+        assert b.pair_layout() == layout
+        b.compute()
+        e, _ = b.sync()
+        fx, fy, fz = b.get_forces()
+        worst_rms, worst_e = 0.0, 0.0
+        for k, r in enumerate(reps):
+            o0, o1 = b.offsets[k], b.offsets[k + 1]
+            if layout == "partners":
+                (wx, wy, wz), (pi, pj), t, g3 = oracle_eval(r.x, r.y, r.z, r, r.params)
+                assert x[o0:o1].tobytes() == wx.tobytes(), k  # wrapped in place as neighbor.rs:13-15
+                qi, qj = b.replica(k).pair_list()
+                assert np.array_equal(qi, pi) and np.array_equal(qj, pj), f"replica {k} (n={r.n}): list"
+                ref.append((t, g3))
+            t, (gx, gy, gz) = ref[k]
+            check_terms(e[k], t, f"{layout}, replica {k}")
+            rms = force_rms_rel(fx[o0:o1], fy[o0:o1], fz[o0:o1], gx, gy, gz)
+            assert rms <= F_RMS, f"{layout}, replica {k}: force rms {rms:.3e}"
+            worst_rms = max(worst_rms, rms)
+            worst_e = max(worst_e, abs(e[k].total - t.total) / abs(t.total))
+            if k % 128 == 0:
+                say(f"[gpu] {layout}: replica {k}/512 (n={r.n}) ok; worst so far: force rms {worst_rms:.2e}, "
+                    f"energy {worst_e:.2e}")
+        for k in sample:
+            r = reps[k]
+            g = gh.System(r.sigma, r.epsilon, r.topo, r.params)
+            g.set_pair_layout(layout)
+            px, py, pz = r.x.copy(), r.y.copy(), r.z.copy()
+            g.build_neighbor_list(px, py, pz)
+            ep, gx, gy, gz = g.compute_forces(px, py, pz)  # (tiles: its own automatic work shape)
+            o0, o1 = b.offsets[k], b.offsets[k + 1]
+            assert gx.tobytes() == fx[o0:o1].tobytes() and gy.tobytes() == fy[o0:o1].tobytes(), k
+            assert gz.tobytes() == fz[o0:o1].tobytes() and ep.total == e[k].total and ep.lj == e[k].lj, k
+        say(f"[gpu] {layout}: batch vs alone byte-identical on replicas {sample}; worst force rms {worst_rms:.2e}")
 
 
 # ---- 250k rebuild protocol ----------------------------------------------------
