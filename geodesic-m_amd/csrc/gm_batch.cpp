@@ -50,6 +50,7 @@ struct gm_batch {
     int waves_per_cu = 16; // persistent wavefronts per compute unit
     int layout = GM_LAYOUT_AUTO;  // the caller's choice (gm_batch_set_pair_layout)
     bool pl = false;              // the current plan evaluates partner lists
+    bool pair_table = false;      // ... with type-pair tables (every replica has at most kPlPairTypes types)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<uint64_t> plan_gen;  // each replica's buffer generation when the plan was made
@@ -78,6 +79,7 @@ gm_status make_plan(gm_batch *b) {
     std::vector<ReplicaDesc> desc((size_t)R);
     std::vector<int2> bonded_items, tile_items;
     b->plan_gen.assign((size_t)R, 0);
+    bool all_pt = true;
     for (int r = 0; r < R; r++) {
         gm_sys *s = b->reps[(size_t)r];
         ReplicaDesc &d = desc[(size_t)r];
@@ -93,6 +95,7 @@ gm_status make_plan(gm_batch *b) {
             // partner lists: type-table offsets in the slot words, bonded forces
             // as records (term_args), one item per workgroup of kPlWaves groups
             d.pl = pl_args(s);
+            all_pt = all_pt && d.pl.pt_shift >= 0;
             d.srec = s->srec.p;
             d.prep.wtype = s->atom_wtype.p;
             d.n_wi = 0;
@@ -121,6 +124,9 @@ gm_status make_plan(gm_batch *b) {
     GM_HIP(b->scl.ensure(std::max<size_t>(scl.size(), 1)));
     if (!scl.empty())
         GM_HIP(hipMemcpyAsync(b->scl.p, scl.data(), sizeof(int2) * scl.size(), hipMemcpyHostToDevice, st));
+    b->pair_table = b->pl && all_pt;
+    if (!b->pair_table)  // one kernel form for the whole launch: by type in every replica
+        for (ReplicaDesc &d : desc) d.pl.pt_shift = -1;
     b->n_bonded_items = (int)bonded_items.size();
     b->n_items = (int)tile_items.size();
     GM_HIP(b->d_reps.ensure((size_t)R));
@@ -143,7 +149,7 @@ gm_status make_plan(gm_batch *b) {
     if (e1 == hipSuccess) e1 = launch_batch_bonded(b->d_reps.p, b->d_items.p, b->n_bonded_items, b->side);
     if (e1 == hipSuccess) e1 = hipEventRecord(b->join, b->side);
     if (e1 == hipSuccess)
-        e1 = b->pl ? launch_batch_pl(b->d_reps.p, b->d_items.p + b->n_bonded_items, b->n_items, st)
+        e1 = b->pl ? launch_batch_pl(b->d_reps.p, b->d_items.p + b->n_bonded_items, b->n_items, b->pair_table, st)
                    : launch_batch_tiles(b->d_reps.p, b->d_items.p + b->n_bonded_items, b->n_items, grid, b->ctr.p,
                                         st);
     if (e1 == hipSuccess) e1 = hipStreamWaitEvent(st, b->join, 0);

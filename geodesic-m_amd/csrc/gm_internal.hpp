@@ -65,6 +65,7 @@ constexpr int kMetaWords = 32;
 // kPlLanes lanes each; lane l of a slot takes the list's entries l, l + 16, ...
 constexpr int kPlAtoms = 4, kPlLanes = 16;
 constexpr int kPlWaves = 4;  // wavefronts (groups) per workgroup of the partner-list kernels
+constexpr int kPlPairTypes = 32;  // partner-list kernels: a type-pair table for up to this many types
 // Each slot's list holds first the partners within r_cutoff + kPlInner (A)
 // at the build, then the others; both parts are padded with the slot itself
 // (r2 = 0: skipped) to the group's longest part rounded up to kPlPad (whole
@@ -529,7 +530,8 @@ hipError_t launch_batch_tiles(const ReplicaDesc *reps, const int2 *items, int n_
                               hipStream_t st);
 hipError_t launch_batch_bonded(const ReplicaDesc *reps, const int2 *items, int n_items, hipStream_t st);
 // items: (replica, first group) per workgroup of kPlWaves partner-list groups
-hipError_t launch_batch_pl(const ReplicaDesc *reps, const int2 *items, int n_items, hipStream_t st);
+hipError_t launch_batch_pl(const ReplicaDesc *reps, const int2 *items, int n_items, bool pair_table,
+                           hipStream_t st);
 struct PlArgs;
 PlArgs pl_args(gm_sys *s);
 hipError_t launch_batch_finish(const ReplicaDesc *reps, int n_reps, const int2 *scl, int64_t n_rec, int64_t n_atm,

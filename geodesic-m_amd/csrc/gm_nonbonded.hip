@@ -579,7 +579,13 @@ static_assert((64 / kPlAtoms) % kPlWaves == 0, "whole workgroups per supercluste
 
 // C16: the lists hold compact codes (slot minus the group's first slot plus
 // kQposGuard, two rounds per 32-bit word); else 32-bit slot numbers.
-template <bool CUBIC, bool C16>
+// PT: the LJ parameters come from a table by type pair (sigma_ij^2 in the
+// kernel's length unit, E4 = 4 eps_ij), T x T entries for T = 2^pt_shift <= 32
+// types, filled with exactly the per-pair arithmetic it replaces (the sum of
+// the two sigma/2, its square, the product of the two 2 sqrt eps): the same
+// bits, three VALU instructions fewer per pair.  Else by type, as (sigma/2,
+// 2 sqrt eps).
+template <bool CUBIC, bool C16, bool PT>
 __device__ __attribute__((always_inline)) inline void pl_work(const int g, const int lane, const PlArgs &A) {
     // Every fused multiply-add below is written out: the pair arithmetic is
     // inlined more than once (loop body and tail), and contraction left to the
@@ -592,9 +598,25 @@ __device__ __attribute__((always_inline)) inline void pl_work(const int g, const
     const int s = g * kPlAtoms + al;
     // LJ parameters by type, in LDS: sigma/2 in the kernel's length unit, 2 sqrt(eps)
     __shared__ float2 s_tp[kMaxLjTypes];
-    for (int k = (int)threadIdx.x; k < A.n_types; k += (int)blockDim.x) {
-        const float2 t = A.type_par[k];
-        s_tp[k] = make_float2(t.x * A.sig_scale, t.y);
+    __shared__ float2 s_pt[kPlPairTypes * kPlPairTypes];
+    const int tsh = PT ? A.pt_shift : 0;
+    if (PT) {
+        const int T = 1 << tsh;
+        for (int k = (int)threadIdx.x; k < T * T; k += (int)blockDim.x) {
+            const int a = k >> tsh, b = k & (T - 1);
+            float2 v = make_float2(0.f, 0.f);
+            if (a < A.n_types && b < A.n_types) {
+                const float2 ta = A.type_par[a], tb = A.type_par[b];
+                const float sig = tb.x * A.sig_scale + ta.x * A.sig_scale;
+                v = make_float2(sig * sig, ta.y * tb.y);
+            }
+            s_pt[k] = v;
+        }
+    } else {
+        for (int k = (int)threadIdx.x; k < A.n_types; k += (int)blockDim.x) {
+            const float2 t = A.type_par[k];
+            s_tp[k] = make_float2(t.x * A.sig_scale, t.y);
+        }
     }
     bool first_only = false;
     if (A.disp) {
@@ -638,9 +660,14 @@ __device__ __attribute__((always_inline)) inline void pl_work(const int g, const
         return *reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(s_tp) + (w & 0x7f8));
     };
     const int4 qi = qpos_of(s);
-    __syncthreads();  // s_tp
-    const float2 ti = par_of(qi.w);
+    __syncthreads();  // s_tp / s_pt
+    const float2 ti = PT ? make_float2(0.f, 0.f) : par_of(qi.w);
     const float shi = ti.x, ei = ti.y;
+    // PT: the row of slot i's type (a type offset past the table, which only a
+    // slot without an atom can hold, is folded into it: its pairs are never
+    // evaluated, and every entry is finite)
+    const unsigned tmask = ((8u << tsh) - 1u) & ~7u;
+    const unsigned ti_row = ((unsigned)qi.w & tmask) << tsh;
     const float rc2 = CUBIC ? p.rc2_u : p.rc2;
     const unsigned rc2_bits = (unsigned)__float_as_int(rc2);
     const float inv_w = CUBIC ? p.inv_w_u : p.inv_w, nrw = in_vgpr(p.nrw);
@@ -659,16 +686,25 @@ __device__ __attribute__((always_inline)) inline void pl_work(const int g, const
         // evaluated continues at r2 = 1e36, where its terms are exactly +0
         const bool ok = ((unsigned)__float_as_int(r2p) - 1u) < rc2_bits;
         const float r2 = ok ? r2p : 1.0e36f;
-        const float2 tj = par_of(qj.w);
+        float sig2, e4;
+        if (PT) {
+            const float2 q = *reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(s_pt) +
+                                                               (((unsigned)qj.w & tmask) | ti_row));
+            sig2 = q.x;
+            e4 = q.y;
+        } else {
+            const float2 tj = par_of(qj.w);
+            const float sig = tj.x + shi;
+            sig2 = sig * sig;
+            e4 = ei * tj.y;
+        }
         const float inv_r = __builtin_amdgcn_rsqf(r2);
         float inv_r2 = inv_r * inv_r;
         inv_r2 = fmaf(inv_r2, fmaf(-r2, inv_r2, 1.0f), inv_r2);
-        const float sig = tj.x + shi;
-        const float s2 = sig * sig * inv_r2;
+        const float s2 = sig2 * inv_r2;
         const float s6 = s2 * s2 * s2;
         const float lj_a = fmaf(s6, s6, -s6);            // s12 - s6
         const float lj_b = s6 * fmaf(s6, 12.0f, -6.0f);  // 6 (2 s12 - s6)
-        const float e4 = ei * tj.y;
         const float rr = r2 * inv_r;
         const float u = __builtin_amdgcn_fmed3f(fmaf(rr, inv_w, nrw), 0.0f, 1.0f);
         const float u2 = u * u;
@@ -754,7 +790,7 @@ __device__ __attribute__((always_inline)) inline void pl_work(const int g, const
 // thread; they store force records), then one group per wavefront (kPlWaves
 // consecutive groups per workgroup; the group count, 64 / kPlAtoms per
 // supercluster, is a multiple).
-template <bool CUBIC, bool C16>
+template <bool CUBIC, bool C16, bool PT>
 __global__ __launch_bounds__(64 * kPlWaves) __attribute__((amdgpu_waves_per_eu(8))) void k_forces_pl(PlArgs A,
                                                                                                     TermArgs ta) {
     const int64_t n_terms = ta.nb + ta.na + ta.nd + ta.n14;
@@ -768,7 +804,7 @@ __global__ __launch_bounds__(64 * kPlWaves) __attribute__((amdgpu_waves_per_eu(8
     // (the group is the same in the whole wavefront: said so, everything
     // derived from it lives in scalar registers)
     const int g = __builtin_amdgcn_readfirstlane(wg * kPlWaves + (int)(threadIdx.x >> 6));
-    pl_work<CUBIC, C16>(g, (int)(threadIdx.x & 63), A);
+    pl_work<CUBIC, C16, PT>(g, (int)(threadIdx.x & 63), A);
 }
 
 // Persistent batched tile kernel: a fixed grid of wavefronts takes tile work
@@ -820,13 +856,14 @@ __global__ __launch_bounds__(64) void k_batch_bonded(const ReplicaDesc *__restri
 // groups of one replica share its L2.  Every group runs the single-system
 // code: a replica's forces and energies are those of evaluating it alone.
 // Replicas are cubic and their lists hold 16-bit codes (the batch checks both).
+template <bool PT>
 __global__ __launch_bounds__(64 * kPlWaves) __attribute__((amdgpu_waves_per_eu(8))) void k_batch_pl(
     const ReplicaDesc *__restrict__ reps, const int2 *__restrict__ items, int n_items) {
     const int2 it = items[xcd_group((int)blockIdx.x, 0, n_items)];
     // (replica and group are the same in the whole wavefront: scalar registers)
     const int r = __builtin_amdgcn_readfirstlane(it.x);
     const int g = __builtin_amdgcn_readfirstlane(it.y + (int)(threadIdx.x >> 6));
-    pl_work<true, true>(g, (int)(threadIdx.x & 63), reps[r].pl);
+    pl_work<true, true, PT>(g, (int)(threadIdx.x & 63), reps[r].pl);
 }
 
 }  // namespace
@@ -934,6 +971,10 @@ PlArgs pl_args(gm_sys *s) {
     A.n_pad = (int)(s->nsc * 64);
     A.n_grp = (int)(s->nsc * (64 / kPlAtoms));
     A.p = s->nbp;
+    // the pair table's size: the types rounded up to a power of two, at most kPlPairTypes
+    int sh = 0;
+    while ((1 << sh) < s->n_types) sh++;
+    A.pt_shift = (1 << sh) <= kPlPairTypes ? sh : -1;
     return A;
 }
 
@@ -948,8 +989,12 @@ hipError_t launch_forces(gm_sys *s, const double *x, const double *y, const doub
         const int64_t grid = (int64_t)A.n_grp / kPlWaves + (ta.nb + ta.na + ta.nd + ta.n14 + 64 * kPlWaves - 1) /
                                                             (64 * kPlWaves);
         if (grid > 0) {
-            auto kern = span_is_cubic(s) ? (s->pl_wide ? k_forces_pl<true, false> : k_forces_pl<true, true>)
-                                         : (s->pl_wide ? k_forces_pl<false, false> : k_forces_pl<false, true>);
+            const bool pt = A.pt_shift >= 0;
+            auto kern = span_is_cubic(s)
+                            ? (s->pl_wide ? (pt ? k_forces_pl<true, false, true> : k_forces_pl<true, false, false>)
+                                          : (pt ? k_forces_pl<true, true, true> : k_forces_pl<true, true, false>))
+                            : (s->pl_wide ? (pt ? k_forces_pl<false, false, true> : k_forces_pl<false, false, false>)
+                                          : (pt ? k_forces_pl<false, true, true> : k_forces_pl<false, true, false>));
             if (ev_start || ev_stop)
                 hipExtLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * kPlWaves), 0, s->stream, ev_start,
                                       ev_stop, 0, A, ta);
@@ -1001,8 +1046,12 @@ hipError_t launch_batch_tiles(const ReplicaDesc *reps, const int2 *items, int n_
     return hipGetLastError();
 }
 
-hipError_t launch_batch_pl(const ReplicaDesc *reps, const int2 *items, int n_items, hipStream_t st) {
-    if (n_items > 0) k_batch_pl<<<(unsigned)n_items, 64 * kPlWaves, 0, st>>>(reps, items, n_items);
+hipError_t launch_batch_pl(const ReplicaDesc *reps, const int2 *items, int n_items, bool pair_table,
+                           hipStream_t st) {
+    if (n_items > 0 && pair_table)
+        k_batch_pl<true><<<(unsigned)n_items, 64 * kPlWaves, 0, st>>>(reps, items, n_items);
+    else if (n_items > 0)
+        k_batch_pl<false><<<(unsigned)n_items, 64 * kPlWaves, 0, st>>>(reps, items, n_items);
     return hipGetLastError();
 }
 

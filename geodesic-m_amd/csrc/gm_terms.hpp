@@ -342,6 +342,7 @@ struct PlArgs {
     double *e_grp;       // per group: half its LJ energy sum
     int n_pad;
     int n_grp;
+    int pt_shift;        // LJ parameters by type pair, 2^pt_shift types a side (-1: by type)
     NbParams p;
 };
 
